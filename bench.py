@@ -1,0 +1,261 @@
+"""Benchmark of the fused WENO-JS5 / Rusanov / SSP-RK3 hot path on one GPU.
+
+Workload (the large single-instance configuration): periodic inviscid
+Burgers on [0, 1], N = 2^24 cells, FP64, WENO-JS5 (eps = 1e-6, p = 2) with
+the Rusanov flux, SSP-RK3 at Courant 0.5 (theta = 1), u0 = exact cell
+averages of a seeded sum of 8 sinusoids (wavenumbers 1..32) plus a step. A
+bench step is one SSP-RK3 time step over all N cells (three fused stage
+kernels with the device-side CFL time step); the workload's 100 steps are
+the default ``--steps``. ``value`` is cell-updates/s = N * steps / wall
+time with the state resident in HBM.
+
+Also reported: the roofline of the dominant stage kernel (algorithmic bytes
+per launch over its event-timed average duration, against the 8 TB/s HBM
+peak) and the CPU restatement of the same step loop on the host cores, on a
+bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+METRIC = "WENO5-RK3 cell-updates/s + adjoint grads/s, learned vs plain at equal L1 err"
+HBM_PEAK_GBS = 8000.0
+
+# words moved per cell by each stage kernel: stage 1 reads u^n and writes
+# u^(1); stages 2 and 3 read u^n and u^(k) and write the next state
+STAGE_WORDS = {1: 2, 2: 3, 3: 3}
+STEP_WORDS = 8
+
+
+def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--n", type=int, default=1 << 24)
+    p.add_argument("--dtype", choices=["f64", "f32"], default="f64")
+    p.add_argument("--scheme", choices=["rusanov", "godunov", "advection"],
+                   default="rusanov")
+    p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--block", type=int, default=0)
+    p.add_argument("--cells-per-thread", type=int, default=0)
+    p.add_argument("--event-steps", type=int, default=20,
+                   help="steps of the per-kernel event-timed pass")
+    p.add_argument("--cpu-seconds", type=float, default=12.0,
+                   help="target duration of the CPU baseline sample (0: skip)")
+    return p.parse_args(argv)
+
+
+def make_problem(args: argparse.Namespace):
+    import torch
+
+    from pyshocks_amd import funcs, make_uniform_cell_grid
+    from pyshocks_amd.advection import Godunov as AdvGodunov
+    from pyshocks_amd.burgers import Godunov, Rusanov
+    from pyshocks_amd.reconstruction import WENOJS53
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import fused_plan, make_ssprk33
+
+    dtype = torch.float64 if args.dtype == "f64" else torch.float32
+    grid = make_uniform_cell_grid(0.0, 1.0, args.n, nghosts=3, dtype=dtype,
+                                  device="cuda")
+    rec = WENOJS53(eps=1.0e-6)
+    if args.scheme == "rusanov":
+        scheme, theta = Rusanov(rec=rec), 1.0
+    elif args.scheme == "godunov":
+        scheme, theta = Godunov(rec=rec), 1.0
+    else:
+        vel = torch.ones(grid.n, dtype=dtype, device="cuda")
+        scheme, theta = AdvGodunov(rec=rec, velocity=vel), 0.5
+    stepper = make_ssprk33(scheme, grid, PeriodicBoundary(), theta=theta)
+    plan = fused_plan(stepper)
+    assert plan is not None
+    plan.desc.block = args.block
+    plan.desc.cells_per_thread = args.cells_per_thread
+    u0 = funcs.cell_averages(funcs.random_sine_step(args.seed), grid)
+    return grid, plan, u0, theta
+
+
+def gpu_leg(args: argparse.Namespace) -> dict:
+    import numpy as np
+    import torch
+
+    from pyshocks_amd import _lib
+
+    _lib.load()
+    grid, plan, u0, theta = make_problem(args)
+    n = args.n
+    itemsize = 8 if args.dtype == "f64" else 4
+
+    u = u0.clone()
+    k1 = torch.empty_like(u)
+    k2 = torch.empty_like(u)
+    ctrl = plan.new_control(u, 0.0, float(np.finfo(plan.np_dtype).max), None)
+
+    if args.warmup > 0:
+        plan.steps(u, u, k1, k2, ctrl, args.warmup)
+    torch.cuda.synchronize()
+
+    t0 = time.perf_counter()
+    plan.steps(u, u, k1, k2, ctrl, args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    c = plan.read_control(ctrl)
+    if c.status != _lib.PSA_OK or c.m != args.warmup + args.steps:
+        raise RuntimeError(f"step control ended early: status {c.status} m {c.m}")
+    if not bool(torch.isfinite(u).all()):
+        raise RuntimeError("non-finite state after the timed steps")
+
+    # per-kernel event timing on the launch stream
+    nev = max(1, args.event_steps)
+    events = {s: [] for s in (1, 2, 3)}
+    bufs = {1: (None, u, k1), 2: (u, k1, k2), 3: (u, k2, u)}
+    for _ in range(nev):
+        for s in (1, 2, 3):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            un, uk, out = bufs[s]
+            e0.record()
+            plan.stage(s, un, uk, out, ctrl=ctrl)
+            e1.record()
+            events[s].append((e0, e1))
+    torch.cuda.synchronize()
+    stage_ms = {s: float(np.mean([a.elapsed_time(b) for a, b in events[s]]))
+                for s in events}
+    dominant = max(stage_ms, key=lambda s: stage_ms[s])
+    algo_bytes = STAGE_WORDS[dominant] * n * itemsize
+    achieved = algo_bytes / (stage_ms[dominant] * 1.0e-3) / 1.0e9
+
+    ms_per_step = elapsed / args.steps * 1.0e3
+    step_gbs = STEP_WORDS * n * itemsize / (ms_per_step * 1.0e-3) / 1.0e9
+    return {
+        "value": n * args.steps / elapsed,
+        "ms_per_step": ms_per_step,
+        "stage_ms": stage_ms,
+        "dominant": dominant,
+        "algo_bytes": algo_bytes,
+        "achieved_gbs": achieved,
+        "step_gbs": step_gbs,
+        "t_final": float(c.t),
+        "theta": theta,
+    }
+
+
+def cpu_leg(args: argparse.Namespace, gpu: dict) -> dict | None:
+    """The C restatement of the reference step loop on the host cores, on a
+    bounded sample of the same workload."""
+    if args.cpu_seconds <= 0:
+        return None
+    import numpy as np
+
+    from oracle import build as oracle_build
+    from oracle import exact as E
+    from oracle import ref_c
+
+    if not ref_c.LIB.exists():
+        oracle_build.build(verbose=False)
+    threads = ref_c.num_threads()
+    n = args.n
+    u0 = E.exact_cell_averages(E.large_initial_condition(seed=args.seed), n)
+    h = 1.0 / n
+    kind = args.scheme
+    cfl = 0.5 * h if kind != "advection" else h
+    vel = np.ones(n + 6) if kind == "advection" else None
+
+    # one step to size the sample, then enough steps for ~cpu_seconds
+    t0 = time.perf_counter()
+    u, _, _, _ = ref_c.solve(kind, u0, n=n, g=3, dx=h, eps=1e-6, theta=gpu["theta"],
+                             cfl_num=cfl, maxit=1, velocity=vel)
+    one = time.perf_counter() - t0
+    nsteps = int(max(1, min(200, args.cpu_seconds / max(one, 1e-9))))
+    t0 = time.perf_counter()
+    ref_c.solve(kind, u, n=n, g=3, dx=h, eps=1e-6, theta=gpu["theta"], cfl_num=cfl,
+                maxit=nsteps, velocity=vel)
+    elapsed = time.perf_counter() - t0
+    return {
+        "value": n * nsteps / elapsed,
+        "unit": "cell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"C/OpenMP restatement of the reference step loop, same workload "
+                  f"(N={n}, {args.dtype if args.dtype == 'f64' else 'f64'}), "
+                  f"{nsteps} SSP-RK3 steps after 1 sizing step, {elapsed:.1f} s",
+    }
+
+
+def main(argv: list[str] | None = None) -> int:
+    args = parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the workload is a single-GPU instance, see DESIGN.md for how "
+              "it would shard)", file=sys.stderr)
+        return 2
+
+    gpu = gpu_leg(args)
+    cpu = cpu_leg(args, gpu)
+
+    traffic = None
+    prof = ROOT / "profiles" / "traffic.json"
+    if prof.exists():
+        try:
+            data = json.loads(prof.read_text())
+            key = f"{args.scheme}_{args.dtype}_{args.n}_stage{gpu['dominant']}"
+            traffic = data.get(key)
+        except (OSError, ValueError):
+            traffic = None
+
+    dtype_name = "f64" if args.dtype == "f64" else "f32"
+    line = {
+        "metric": METRIC,
+        "value": gpu["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": gpu["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype_name,
+        "data": "synthetic",
+        "config": {
+            "workload": f"periodic {'burgers' if args.scheme != 'advection' else 'advection'} "
+                        f"{args.scheme} WENO-JS5 SSP-RK3, N={args.n}, {dtype_name}, "
+                        f"Courant 0.5, sine-sum+step u0 seed {args.seed}",
+            "n_cells": args.n,
+            "scheme": args.scheme,
+            "step": "one SSP-RK3 time step (3 fused stage kernels, device CFL dt)",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": gpu["achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": gpu["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": f"k_stage stage {gpu['dominant']}",
+            "algorithmic_bytes_per_launch": gpu["algo_bytes"],
+            "stage_ms": gpu["stage_ms"],
+            "step_achieved_gbs": gpu["step_gbs"],
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

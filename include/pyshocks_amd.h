@@ -1,0 +1,202 @@
+/*
+ * pyshocks_amd: C ABI of the MI355X (gfx950) WENO-JS5 / Rusanov / Godunov /
+ * SSP-RK3 library.
+ *
+ * Conventions
+ * -----------
+ * - Every function returns an int status: PSA_OK (0) or a negative PSA_ERR_*
+ *   code; psa_get_last_error() returns the thread-local message of the last
+ *   failure.
+ * - Arrays are contiguous device pointers of the given dtype (PSA_FLOAT64 or
+ *   PSA_FLOAT32). A grid of n interior cells and g = 3 ghost cells per side
+ *   has arrays of m = n + 2 g cells; face arrays have m + 1 entries.
+ * - The caller owns every buffer. No function keeps a pointer past its return
+ *   or allocates device memory; all work is enqueued asynchronously on the
+ *   given hipStream_t (passed as void*; NULL is the default stream).
+ * - Reference citations are relative to the pyshocks 2025.10 source tree
+ *   (src/pyshocks/...), which this library replaces for the hot path.
+ */
+#ifndef PYSHOCKS_AMD_H_
+#define PYSHOCKS_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes ----------------------------------------------------------- */
+#define PSA_OK 0
+#define PSA_ERR_INVALID_ARGUMENT -1 /* bad shape / pointer / parameter */
+#define PSA_ERR_NOT_IMPLEMENTED -2  /* unsupported dtype / flux / mode */
+#define PSA_ERR_HIP -3              /* HIP runtime error (launch, device) */
+#define PSA_ERR_NONFINITE_DT -4     /* timestepping.py:144-145 */
+#define PSA_ERR_WORKSPACE -5        /* workspace too small */
+
+/* dtypes ------------------------------------------------------------------ */
+#define PSA_FLOAT64 0
+#define PSA_FLOAT32 1
+
+/* numerical fluxes ---------------------------------------------------------
+ * PSA_FLUX_RUSANOV          burgers.Rusanov  -> scalar_flux_rusanov
+ *                           (burgers/schemes.py:99-118, scalar.py:192-249)
+ * PSA_FLUX_BURGERS_UPWIND   burgers.Godunov  -> scalar_flux_upwind
+ *                           (burgers/schemes.py:75-89, scalar.py:91-132)
+ * PSA_FLUX_ADVECTION_UPWIND advection.Godunov -> upwind_flux
+ *                           (advection/schemes.py:100-129)               */
+#define PSA_FLUX_RUSANOV 0
+#define PSA_FLUX_BURGERS_UPWIND 1
+#define PSA_FLUX_ADVECTION_UPWIND 2
+
+/* time step modes of the device-side step control */
+#define PSA_DT_CFL 0   /* dt = theta * cfl_num / max|u_int| (or max|a_int|) */
+#define PSA_DT_FIXED 1 /* dt = fixed_dt every step                           */
+
+/* Library / error queries. */
+int psa_version(void);
+int psa_control_bytes(void); /* sizeof(psa_control) */
+int psa_get_last_error(char *buf, size_t n);
+int psa_device_count(int *count);
+
+/* weno.py:114-140 weno_smoothness(weno_js_53_coefficients(), u):
+ * beta (3, m), row-major, zero padding outside [0, m) as jnp.convolve('same'). */
+int psa_weno_smoothness(int dtype, const void *u, void *beta, int64_t m,
+                        void *stream);
+
+/* weno.py:143-157 weno_interp: uhat (3, m). */
+int psa_weno_interp(int dtype, const void *u, void *uhat, int64_t m,
+                    void *stream);
+
+/* weno.py:247-256 weno_js_weights (p = 2): omega (3, m). */
+int psa_weno_js_weights(int dtype, const void *u, void *omega, int64_t m,
+                        double eps, void *stream);
+
+/* reconstruction.py:358-377 reconstruct(WENOJS53(eps), ...): (ul, ur), each
+ * of m cells; ul is the value at x_{i-1/2}, ur at x_{i+1/2}. */
+int psa_weno_reconstruct(int dtype, const void *f, void *ul, void *ur,
+                         int64_t m, double eps, void *stream);
+
+/* scalar.py:529-540 apply_boundary(PeriodicBoundary): out = u with the
+ * g ghost cells per side filled with their periodic images (out may be u). */
+int psa_apply_boundary_periodic(int dtype, const void *u, void *out,
+                                int64_t m, int g, void *stream);
+
+/* schemes.py:322-336 numerical_flux on u as given (no boundary applied):
+ * fnum (m + 1) with fnum[0] = fnum[m] = 0. velocity (m) is required for
+ * PSA_FLUX_ADVECTION_UPWIND and ignored otherwise. nu (m - 1) is the Rusanov
+ * artificial-viscosity factor grid.df ** (alpha - 1), or NULL for alpha == 1. */
+int psa_numerical_flux(int dtype, int flux, const void *u,
+                       const void *velocity, const void *nu, void *fnum,
+                       int64_t m, double eps, void *stream);
+
+/* schemes.py:339-346 (conservation law) and advection/schemes.py:62-73:
+ * out (m) = -(F[1:] - F[:-1]) / dx (times -velocity for advection), with F
+ * the numerical flux of the periodic ghost-filled u; every entry, ghosts
+ * included, as the reference computes it. */
+int psa_apply_operator(int dtype, int flux, const void *u,
+                       const void *velocity, const void *nu, void *out,
+                       int64_t m, int g, double dx, double eps, void *stream);
+
+/* max |u[g:m-g]| (burgers/schemes.py:42-49,121-127 and
+ * advection/schemes.py:51-59 smax / amax) into the device scalar out. */
+int psa_max_abs_interior(int dtype, const void *u, void *out, int64_t m,
+                         int g, void *stream);
+
+/* grid.py:445-506 _norm / norm over the interior slice [lo, hi) of u:
+ * PSA_NORM_L1 sum |u| * weight, PSA_NORM_L2 sqrt(sum u^2 * weight),
+ * PSA_NORM_INF max |u| (NaN propagates), PSA_NORM_TVD sum ||u[i+1]| - |u[i]||.
+ * out is a device scalar of the dtype; the workspace holds
+ * psa_norm_workspace_bytes() bytes. Deterministic (fixed reduction order). */
+#define PSA_NORM_L1 1
+#define PSA_NORM_L2 2
+#define PSA_NORM_INF 3
+#define PSA_NORM_TVD 4
+int psa_norm_workspace_bytes(void);
+int psa_interior_norm(int dtype, int p, const void *u, int64_t lo, int64_t hi,
+                      double weight, void *out, void *workspace,
+                      size_t workspace_bytes, void *stream);
+
+/* Fused SSP-RK3 -------------------------------------------------------------
+ * One HIP kernel per Runge-Kutta stage: WENO-JS5 smoothness, weights and
+ * both reconstructions, the face flux and the stage combination of
+ * timestepping.py:312-320 in one pass over an LDS tile with a 3-cell halo.
+ * The periodic ghost fill is read by wrapped index; outputs are the interior
+ * cells with their ghost images rewritten.
+ *
+ * The step control of timestepping.py:89-152 runs on the device in a
+ * psa_control block: stage 1 decides whether the step is taken
+ * (t < tfinal and m < maxit), predicts dt (PSA_DT_CFL: theta * cfl_num /
+ * max|u_int| of the current state, clipped to tfinal - t, plus 1e-15),
+ * stage 2 advances t and m, stage 3 reduces max|u_int| of the new state.
+ * A non-finite dt stops the run with status PSA_ERR_NONFINITE_DT. */
+typedef struct psa_control {
+  double t;                     /* time of the current state               */
+  double dt;                    /* dt of the last step taken               */
+  double tfinal;                /* final time (finfo(dtype).max if none)   */
+  double theta;                 /* Courant factor                          */
+  double cfl_num;               /* 0.5 dx^(2-alpha), 0.5 dx or dx          */
+  double fixed_dt;              /* dt for PSA_DT_FIXED                     */
+  long long m;                  /* steps taken                             */
+  long long maxit;              /* step limit, < 0 for none                */
+  unsigned long long umax_bits; /* max|u_int| (bit pattern of the dtype)   */
+  int active;                   /* 1 while the step in flight is taken     */
+  int status;                   /* PSA_OK or PSA_ERR_NONFINITE_DT          */
+  int dt_mode;                  /* PSA_DT_CFL or PSA_DT_FIXED              */
+  int speed_from_state;         /* 1: max|u| of each state (Burgers)       */
+  int pad_[2];
+} psa_control;
+
+typedef struct psa_solver_desc {
+  int dtype;                    /* PSA_FLOAT64 / PSA_FLOAT32               */
+  int flux;                     /* PSA_FLUX_*                              */
+  int64_t n;                    /* interior cells                          */
+  int g;                        /* ghost cells per side (>= 3)             */
+  int velocity_is_const;        /* advection: velocity == velocity_const;
+                                   velocity may then be NULL               */
+  double dx;                    /* cell size h                             */
+  double eps;                   /* WENO-JS epsilon                         */
+  double velocity_const;        /* advection constant velocity             */
+  const void *velocity;         /* advection velocity (m) if not const     */
+  const void *nu;               /* Rusanov df^(alpha-1) (m - 1) or NULL    */
+  int block;                    /* threads per workgroup (0 = tuned)       */
+  int cells_per_thread;         /* cells per thread (0 = tuned)            */
+} psa_solver_desc;
+
+/* Initialise the control block (device memory, sizeof(psa_control)) for a
+ * run starting from state u at time tstart: stores t, tfinal, maxit, the dt
+ * mode and, for PSA_DT_CFL, max|u_int| (Burgers) or max|velocity_int|
+ * (advection). maxit < 0 means no limit. */
+int psa_control_init(const psa_solver_desc *desc, const void *u,
+                     void *control, double tstart, double tfinal,
+                     long long maxit, int dt_mode, double theta,
+                     double cfl_num, double fixed_dt, void *stream);
+
+/* Enqueue nsteps steps of timestepping.step with SSP-RK3: u_in -> u_out
+ * (u_out may equal u_in for an in-place update), stage buffers k1, k2 of m
+ * cells. Steps after the control block says stop are no-ops, so a fixed
+ * number of steps can be enqueued (or captured in a graph) and the control
+ * block read back afterwards. */
+int psa_ssprk33_steps(const psa_solver_desc *desc, const void *u_in,
+                      void *u_out, void *k1, void *k2, void *control,
+                      long long nsteps, void *stream);
+
+/* One stage (1, 2 or 3) of timestepping.py:312-320: stage 1 out = uk + dt L(uk),
+ * stage 2 out = 3/4 un + 1/4 (uk + dt L(uk)), stage 3 out = 1/3 un +
+ * 2/3 (uk + dt L(uk)), with the step control of the control block, or, if
+ * control is NULL, the device scalar dt. out may equal un but not uk. */
+int psa_ssprk33_stage(const psa_solver_desc *desc, int stage, const void *un,
+                      const void *uk, void *out, void *control,
+                      const void *dt, void *stream);
+
+/* timestepping.py:312-320 advance(SSPRK33, dt, t, u) for the bound operator
+ * with the dt of a device scalar of the dtype: out = advance(u). */
+int psa_ssprk33_advance(const psa_solver_desc *desc, const void *u,
+                        void *out, void *k1, void *k2, const void *dt,
+                        void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYSHOCKS_AMD_H_ */

@@ -1,0 +1,500 @@
+"""CPU oracle: a NumPy FP64 restatement of pyshocks' WENO-JS5 / Rusanov /
+Godunov / SSP-RK3 hot path.
+
+TEST INFRASTRUCTURE ONLY. This module is the checker for the HIP library in
+``pyshocks_amd``; it is imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` and by nothing in the product
+path. Every function follows the cited reference file:line (paths relative to
+the pyshocks source tree, version 2025.10) with the same expression order, so
+that it reproduces the reference's arithmetic up to the summation order inside
+``jnp.convolve``.
+
+Pins (each checked in ``tests/test_oracle_*.py``):
+
+* coefficients: row sums, the 5th-order interpolant identity, beta on
+  constants / linear data;
+* convolution direction: explicit loops vs the vectorised form at 2e-15
+  (the pattern of ``tests/test_weno.py:34-108``);
+* polynomial exactness of the candidate and combined interpolants;
+* weight behaviour on smooth / step data (``tests/test_weno.py:117-166``);
+* EOC >= 4.5 of the reconstruction (``tests/test_weno.py:288-338``);
+* conservation and constant-state invariants of the operator;
+* SSP-RK3 order on ``u' = exp(-t)`` (``tests/test_timestepping.py:26-99``);
+* end-to-end: periodic advection EOC >= 4.5 against the exact shift
+  (``tests/test_convergence.py:395-460``) and Burgers against the exact
+  Hopf-Lax cell averages (``oracle/exact.py``) through a shock.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Any, Callable, Iterator
+
+import numpy as np
+
+# {{{ coefficients
+
+
+@dataclass(frozen=True)
+class Stencil:
+    """WENO stencil coefficients (``weno.py:55-98``)."""
+
+    a: np.ndarray
+    b: np.ndarray
+    c: np.ndarray
+    d: np.ndarray
+
+
+def weno_js_53_coefficients() -> Stencil:
+    """Fifth-order WENO-JS coefficients (``weno.py:206-244``).
+
+    Taps are ordered ``(i + 2, i + 1, i, i - 1, i - 2)`` so that they can be
+    used with a 'same' convolution, exactly as in the reference.
+    """
+    a = np.array([13.0 / 12.0, 1.0 / 4.0], dtype=np.float64)
+    b = np.array(
+        [
+            [[0.0, 0.0, 1.0, -2.0, 1.0], [0.0, 0.0, 3.0, -4.0, 1.0]],
+            [[0.0, 1.0, -2.0, 1.0, 0.0], [0.0, 1.0, 0.0, -1.0, 0.0]],
+            [[1.0, -2.0, 1.0, 0.0, 0.0], [1.0, -4.0, 3.0, 0.0, 0.0]],
+        ],
+        dtype=np.float64,
+    )
+    c = np.array(
+        [
+            [0.0, 0.0, 11.0 / 6.0, -7.0 / 6.0, 2.0 / 6.0],
+            [0.0, 2.0 / 6.0, 5.0 / 6.0, -1.0 / 6.0, 0.0],
+            [-1.0 / 6.0, 5.0 / 6.0, 2.0 / 6.0, 0.0, 0.0],
+        ],
+        dtype=np.float64,
+    )
+    d = np.array([[1.0 / 10.0, 6.0 / 10.0, 3.0 / 10.0]], dtype=np.float64).T
+
+    return Stencil(a=a, b=b, c=c, d=d)
+
+
+WENOJS53 = weno_js_53_coefficients()
+
+# }}}
+
+
+# {{{ weno
+
+
+def convolve1d(u: np.ndarray, w: np.ndarray) -> np.ndarray:
+    """'same' convolution with implicit zero padding (``convolve.py:99-114``,
+    the ``ConvolutionType.Same`` branch): ``out[m] = sum_k w[k] u[m + 2 - k]``.
+    """
+    return np.convolve(u, w, mode="same")
+
+
+def weno_smoothness(s: Stencil, u: np.ndarray) -> np.ndarray:
+    """Smoothness indicators ``beta`` of shape ``(3, M)`` (``weno.py:114-140``)."""
+    return np.stack([
+        sum(s.a[j] * convolve1d(u, s.b[i, j, :]) ** 2 for j in range(s.a.size))
+        for i in range(s.b.shape[0])
+    ])
+
+
+def weno_interp(s: Stencil, u: np.ndarray) -> np.ndarray:
+    """Candidate interpolants ``uhat`` of shape ``(3, M)`` (``weno.py:143-157``)."""
+    return np.stack([convolve1d(u, s.c[i, :]) for i in range(s.c.shape[0])])
+
+
+def weno_js_weights(s: Stencil, u: np.ndarray, *, eps: float) -> np.ndarray:
+    """Nonlinear weights ``omega`` with p = 2 (``weno.py:247-256``)."""
+    beta = weno_smoothness(s, u)
+    alpha = s.d / (eps + beta) ** 2
+
+    return alpha / np.sum(alpha, axis=0, keepdims=True)
+
+
+def reconstruct_side(f: np.ndarray, eps: float, s: Stencil = WENOJS53) -> np.ndarray:
+    """Left-biased value at ``x_{m + 1/2}`` (``reconstruction.py:351-355``)."""
+    omega = weno_js_weights(s, f, eps=eps)
+    uhat = weno_interp(s, f)
+
+    return np.sum(omega * uhat, axis=0)
+
+
+def reconstruct_wenojs(
+    f: np.ndarray, eps: float, s: Stencil = WENOJS53
+) -> tuple[np.ndarray, np.ndarray]:
+    """``(ul, ur)`` of a WENO-JS reconstruction (``reconstruction.py:358-377``)."""
+    ur = reconstruct_side(f, eps, s)
+    ul = reconstruct_side(f[::-1], eps, s)[::-1]
+
+    return ul, ur
+
+
+# }}}
+
+
+# {{{ grid
+
+
+@dataclass(frozen=True)
+class Grid:
+    """Uniform cell grid (``grid.py:48-121``)."""
+
+    a: float
+    b: float
+    nghosts: int
+    x: np.ndarray
+    dx: np.ndarray
+    f: np.ndarray
+    df: np.ndarray
+    dx_min: np.floating[Any]
+    dx_max: np.floating[Any]
+
+    @property
+    def n(self) -> int:
+        return int(self.x.size)
+
+    @property
+    def i_(self) -> slice:
+        return np.s_[self.nghosts : self.x.size - self.nghosts]
+
+    @property
+    def f_(self) -> slice:
+        return np.s_[self.nghosts : self.f.size - self.nghosts]
+
+
+def make_uniform_cell_grid(
+    a: float, b: float, n: int, *, nghosts: int = 1, dtype: Any = np.float64
+) -> Grid:
+    """``grid.py:131-179``."""
+    if b < a:
+        raise ValueError(f"Incorrect interval a > b: {a!r} > {b!r}.")
+    if n <= 0:
+        raise ValueError(f"Number of cells should be > 0: {n!r} <= 0.")
+
+    h = (b - a) / n
+    f = np.linspace(a - nghosts * h, b + nghosts * h, n + 2 * nghosts + 1, dtype=dtype)
+    x = (f[1:] + f[:-1]) / 2
+    df = np.diff(x)
+    dx = np.full_like(x, h)
+
+    return Grid(
+        a=a, b=b, nghosts=nghosts, x=x, dx=dx, f=f, df=df,
+        dx_min=np.array(h, dtype=x.dtype), dx_max=np.array(h, dtype=x.dtype),
+    )
+
+
+def leggauss_cell_average(grid: Grid, fn: Callable[[np.ndarray], np.ndarray],
+                          order: int) -> np.ndarray:
+    """Gauss-Legendre cell averages over every cell of ``grid.f``
+    (``grid.py:393-435``)."""
+    from numpy.polynomial.legendre import leggauss
+
+    xi, wi = leggauss(order)
+    xi = xi.reshape(-1, 1)
+    wi = wi.reshape(-1, 1)
+
+    x = grid.f
+    dx = x[1:] - x[:-1]
+    dxm = 0.5 * dx.reshape(1, -1)
+    xm = 0.5 * (x[1:] + x[:-1]).reshape(1, -1)
+
+    xim = xm + dxm * xi
+    wim = dxm * wi
+
+    return np.sum(fn(xim) * wim, axis=0) / dx
+
+
+def norm(grid: Grid, u: np.ndarray, *, p: Any = 1, weighted: bool = False) -> float:
+    """Interior norms (``grid.py:445-506``)."""
+    if u.shape[:1] == grid.x.shape:
+        dx = grid.dx[grid.i_] if weighted else 1.0
+        u = np.abs(u[grid.i_])
+    elif u.shape[:1] == grid.f.shape:
+        dx = grid.df[grid.f_] if weighted else 1.0
+        u = np.abs(u[grid.f_])
+    else:
+        raise ValueError(f"Array has unexpected shape: {u.shape[:1]}")
+
+    if p == 1:
+        return float(np.sum(u * dx))
+    if p == 2:
+        return float(np.sqrt(np.sum(u**2 * dx)))
+    if p in {np.inf, "inf"}:
+        return float(np.max(u))
+    if p == "tvd":
+        return float(np.sum(np.abs(np.diff(u))))
+
+    raise ValueError(f"Unrecognized norm order 'p': {p!r}.")
+
+
+def rnorm(grid: Grid, u: np.ndarray, v: np.ndarray, *, p: Any = 1,
+          weighted: bool = False, atol: float = 1.0e-14) -> float:
+    """Relative interior error ``grid.py:509-531``."""
+    vnorm = norm(grid, v, p=p, weighted=weighted)
+    if vnorm < atol:
+        vnorm = 1.0
+
+    return norm(grid, u - v, p=p, weighted=weighted) / vnorm
+
+
+# }}}
+
+
+# {{{ boundary, fluxes, operators
+
+
+def apply_boundary_periodic(grid: Grid, u: np.ndarray) -> np.ndarray:
+    """Periodic ghost fill (``scalar.py:529-540``)."""
+    g = grid.nghosts
+    m = u.size
+    u = u.copy()
+    u[m - g:] = u[g: 2 * g]
+    u[:g] = u[m - 2 * g: m - g]
+
+    return u
+
+
+def burgers_flux(u: np.ndarray) -> np.ndarray:
+    """``burgers/schemes.py:37-39``."""
+    return u**2 / 2
+
+
+def scalar_flux_rusanov(grid: Grid, u: np.ndarray, a: np.ndarray, *, eps: float,
+                        alpha: float = 1.0,
+                        flux: Callable[[np.ndarray], np.ndarray] = burgers_flux,
+                        ) -> np.ndarray:
+    """Rusanov flux with the speed from the cell values (``scalar.py:192-249``)."""
+    if abs(alpha - 1.0) > 1.0e-8:
+        nu: Any = grid.df ** (alpha - 1)
+    else:
+        nu = np.array(1.0, dtype=u.dtype)
+
+    ul, ur = reconstruct_wenojs(u, eps)
+    fl = flux(ul)
+    fr = flux(ur)
+
+    a = np.abs(a)
+    a = np.maximum(a[1:], a[:-1])
+
+    fnum = 0.5 * (fl[1:] + fr[:-1]) - 0.5 * a * nu * (ul[1:] - ur[:-1])
+    return np.pad(fnum, 1)
+
+
+def scalar_flux_upwind(grid: Grid, u: np.ndarray, a: np.ndarray, *, eps: float,
+                       flux: Callable[[np.ndarray], np.ndarray] = burgers_flux,
+                       ) -> np.ndarray:
+    """Upwind flux switched on the averaged reconstructed speed
+    (``scalar.py:91-132``)."""
+    ul, ur = reconstruct_wenojs(u, eps)
+    al, ar = reconstruct_wenojs(a, eps)
+
+    fl = flux(ul)
+    fr = flux(ur)
+
+    aavg = (ar[:-1] + al[1:]) / 2
+    fnum = np.where(aavg > 0, fr[:-1], fl[1:])
+
+    return np.pad(fnum, 1)
+
+
+def advection_upwind_flux(grid: Grid, u: np.ndarray, velocity: np.ndarray, *,
+                          eps: float) -> np.ndarray:
+    """``advection/schemes.py:100-114``."""
+    ul, ur = reconstruct_wenojs(u, eps)
+    al, ar = reconstruct_wenojs(velocity, eps)
+
+    aavg = (ar[:-1] + al[1:]) / 2
+    fnum = np.where(aavg > 0, ur[:-1], ul[1:])
+
+    return np.pad(fnum, 1)
+
+
+@dataclass(frozen=True)
+class Scheme:
+    """A bound scheme of the hot path: ``kind`` is ``"rusanov"`` or
+    ``"godunov"`` (Burgers) or ``"advection"`` (linear advection, Godunov)."""
+
+    kind: str
+    eps: float
+    alpha: float = 1.0
+    velocity: np.ndarray | None = None
+
+
+def numerical_flux(scheme: Scheme, grid: Grid, u: np.ndarray) -> np.ndarray:
+    """``burgers/schemes.py:83-118`` and ``advection/schemes.py:125-129``."""
+    if scheme.kind == "rusanov":
+        return scalar_flux_rusanov(grid, u, u, eps=scheme.eps, alpha=scheme.alpha)
+    if scheme.kind == "godunov":
+        return scalar_flux_upwind(grid, u, u, eps=scheme.eps)
+    if scheme.kind == "advection":
+        assert scheme.velocity is not None
+        return advection_upwind_flux(grid, u, scheme.velocity, eps=scheme.eps)
+
+    raise NotImplementedError(scheme.kind)
+
+
+def apply_operator(scheme: Scheme, grid: Grid, t: float, u: np.ndarray) -> np.ndarray:
+    """Conservation law ``schemes.py:339-346``; advection
+    ``advection/schemes.py:62-73``."""
+    u = apply_boundary_periodic(grid, u)
+    f = numerical_flux(scheme, grid, u)
+
+    if scheme.kind == "advection":
+        return -scheme.velocity * (f[1:] - f[:-1]) / grid.dx
+
+    return -(f[1:] - f[:-1]) / grid.dx
+
+
+def predict_timestep(scheme: Scheme, grid: Grid, t: float, u: np.ndarray) -> np.floating[Any]:
+    """``burgers/schemes.py:42-49`` (Godunov), ``:121-127`` (Rusanov) and
+    ``advection/schemes.py:51-59``."""
+    if scheme.kind == "rusanov":
+        smax = np.max(np.abs(u[grid.i_]))
+        return 0.5 * grid.dx_min ** (2 - scheme.alpha) / smax
+    if scheme.kind == "godunov":
+        smax = np.max(np.abs(u[grid.i_]))
+        return 0.5 * grid.dx_min / smax
+    if scheme.kind == "advection":
+        amax = np.max(np.abs(scheme.velocity[grid.i_]))
+        return grid.dx_min / amax
+
+    raise NotImplementedError(scheme.kind)
+
+
+# }}}
+
+
+# {{{ time stepping
+
+
+def ssprk33_advance(fn: Callable[[Any, np.ndarray], np.ndarray], dt: Any, t: Any,
+                    u: np.ndarray) -> np.ndarray:
+    """``timestepping.py:312-320``."""
+    k1 = u + dt * fn(t, u)
+    k2 = 3.0 / 4.0 * u + 1.0 / 4.0 * (k1 + dt * fn(t + dt, k1))
+    return 1.0 / 3.0 * u + 2.0 / 3.0 * (k2 + dt * fn(t + 0.5 * dt, k2))
+
+
+@dataclass(frozen=True)
+class StepCompleted:
+    """``timestepping.py:47-65``."""
+
+    t: Any
+    tfinal: Any
+    dt: Any
+    iteration: int
+    u: np.ndarray
+
+
+@dataclass
+class InMemoryCheckpoint:
+    """``checkpointing.py:101-140``."""
+
+    basename: str
+    storage: dict = field(default_factory=dict)
+
+    def save(self, idx: int, values: dict) -> None:
+        key = (self.basename, idx)
+        if key in self.storage:
+            raise KeyError(f"Cannot set existing checkpoint at {idx!r}.")
+        self.storage[key] = values
+
+    def load(self, idx: int) -> dict:
+        key = (self.basename, idx)
+        if key not in self.storage:
+            raise KeyError(f"Cannot find checkpoint at index {idx!r}.")
+        return self.storage[key]
+
+
+def step(predict: Callable[[Any, np.ndarray], Any],
+         source: Callable[[Any, np.ndarray], np.ndarray],
+         u0: np.ndarray, *, maxit: int | None = None, tstart: float = 0.0,
+         tfinal: float | None = None,
+         checkpoint: InMemoryCheckpoint | None = None) -> Iterator[StepCompleted]:
+    """The host loop of ``timestepping.py:89-152`` with SSP-RK3 as ``advance``."""
+    dtype = u0.dtype.type
+    if tfinal is None:
+        tfinal = float(np.finfo(u0.dtype).max)
+
+    m = 0
+    t = dtype(tstart)
+    tfinal = dtype(tfinal)
+    u = u0
+
+    yield StepCompleted(t=t, tfinal=tfinal, dt=dtype(0.0), iteration=m, u=u)
+
+    while True:
+        if checkpoint is not None:
+            checkpoint.save(m, {"m": m, "t": t, "u": u})
+
+        if tfinal is not None and t >= tfinal:
+            break
+
+        if maxit is not None and m >= maxit:
+            break
+
+        dt = dtype(predict(t, u))
+        if tfinal != np.inf:
+            dt_min = tfinal - t
+            dt = dtype((dt if dt < dt_min else dt_min) + dtype(1.0e-15))
+
+        if not np.isfinite(dt):
+            raise ValueError(f"Time step is not finite: {dt!r}.")
+
+        u = ssprk33_advance(source, dt, t, u)
+
+        m += 1
+        t = dtype(t + dt)
+
+        yield StepCompleted(t=t, tfinal=tfinal, dt=dt, iteration=m, u=u)
+
+
+def solve(scheme: Scheme, grid: Grid, u0: np.ndarray, *, theta: float,
+          tfinal: float | None = None, maxit: int | None = None,
+          fixed_dt: float | None = None) -> tuple[np.ndarray, list[float], float]:
+    """Forward solve as the reference harness does (``examples/burgers.py:161-179``):
+    ``predict = theta * predict_timestep`` (or a fixed ``dt``), ``source =
+    apply_operator``. Returns ``(u, dts, t)``."""
+
+    if fixed_dt is None:
+        def predict(t: Any, u: np.ndarray) -> Any:
+            return theta * predict_timestep(scheme, grid, t, u)
+    else:
+        def predict(t: Any, u: np.ndarray) -> Any:
+            return fixed_dt
+
+    def source(t: Any, u: np.ndarray) -> np.ndarray:
+        return apply_operator(scheme, grid, t, u)
+
+    dts = []
+    event = None
+    for event in step(predict, source, u0, maxit=maxit, tfinal=tfinal):
+        if event.iteration > 0:
+            dts.append(float(event.dt))
+
+    assert event is not None
+    return event.u, dts, float(event.t)
+
+
+def predict_maxit_from_timestep(tfinal: float, dt: float) -> tuple[int, float]:
+    """``timestepping.py:251-261``."""
+    maxit = int(tfinal / dt)
+    dt = tfinal / maxit + 1.0e-15
+
+    return maxit, dt
+
+
+# }}}
+
+
+# {{{ eoc
+
+
+def estimate_order_of_convergence(h: np.ndarray, error: np.ndarray) -> float:
+    """Least-squares slope of ``log error`` vs ``log h`` (``tools.py:143-159``)."""
+    h = np.asarray(h, dtype=np.float64)
+    error = np.asarray(error, dtype=np.float64)
+    eps = np.finfo(h.dtype).eps
+    c = np.polyfit(np.log10(h + eps), np.log10(error + eps), 1)
+    return float(c[-2])
+
+
+# }}}

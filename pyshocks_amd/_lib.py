@@ -1,0 +1,203 @@
+"""ctypes binding of ``libpyshocks_amd.so`` (declared in
+``include/pyshocks_amd.h``).
+
+The library binds to the HIP runtime that torch has already loaded (both use
+the SONAME ``libamdhip64.so.7``), so torch's streams and device pointers are
+valid arguments. There is no CPU fallback: every entry point raises if the
+library is missing or a tensor is not on the GPU.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+from typing import Any
+
+import torch
+
+LIBPATH = pathlib.Path(__file__).resolve().parent / "libpyshocks_amd.so"
+
+# status codes
+PSA_OK = 0
+PSA_ERR_INVALID_ARGUMENT = -1
+PSA_ERR_NOT_IMPLEMENTED = -2
+PSA_ERR_HIP = -3
+PSA_ERR_NONFINITE_DT = -4
+PSA_ERR_WORKSPACE = -5
+
+PSA_FLOAT64 = 0
+PSA_FLOAT32 = 1
+
+PSA_FLUX_RUSANOV = 0
+PSA_FLUX_BURGERS_UPWIND = 1
+PSA_FLUX_ADVECTION_UPWIND = 2
+
+PSA_DT_CFL = 0
+PSA_DT_FIXED = 1
+
+
+class NonFiniteTimestepError(ValueError):
+    """Raised for the ``Time step is not finite`` condition of
+    ``timestepping.py:144-145``."""
+
+
+class PsaControl(ctypes.Structure):
+    _fields_ = [
+        ("t", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("tfinal", ctypes.c_double),
+        ("theta", ctypes.c_double),
+        ("cfl_num", ctypes.c_double),
+        ("fixed_dt", ctypes.c_double),
+        ("m", ctypes.c_longlong),
+        ("maxit", ctypes.c_longlong),
+        ("umax_bits", ctypes.c_ulonglong),
+        ("active", ctypes.c_int),
+        ("status", ctypes.c_int),
+        ("dt_mode", ctypes.c_int),
+        ("speed_from_state", ctypes.c_int),
+        ("pad_", ctypes.c_int * 2),
+    ]
+
+
+class PsaSolverDesc(ctypes.Structure):
+    _fields_ = [
+        ("dtype", ctypes.c_int),
+        ("flux", ctypes.c_int),
+        ("n", ctypes.c_int64),
+        ("g", ctypes.c_int),
+        ("velocity_is_const", ctypes.c_int),
+        ("dx", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("velocity_const", ctypes.c_double),
+        ("velocity", ctypes.c_void_p),
+        ("nu", ctypes.c_void_p),
+        ("block", ctypes.c_int),
+        ("cells_per_thread", ctypes.c_int),
+    ]
+
+
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_LL = ctypes.c_longlong
+_D = ctypes.c_double
+
+# name -> argtypes (restype is always int)
+SIGNATURES: dict[str, list[Any]] = {
+    "psa_version": [],
+    "psa_control_bytes": [],
+    "psa_get_last_error": [ctypes.c_char_p, ctypes.c_size_t],
+    "psa_device_count": [ctypes.POINTER(ctypes.c_int)],
+    "psa_weno_smoothness": [_I, _VP, _VP, _I64, _VP],
+    "psa_weno_interp": [_I, _VP, _VP, _I64, _VP],
+    "psa_weno_js_weights": [_I, _VP, _VP, _I64, _D, _VP],
+    "psa_weno_reconstruct": [_I, _VP, _VP, _VP, _I64, _D, _VP],
+    "psa_apply_boundary_periodic": [_I, _VP, _VP, _I64, _I, _VP],
+    "psa_numerical_flux": [_I, _I, _VP, _VP, _VP, _VP, _I64, _D, _VP],
+    "psa_apply_operator": [_I, _I, _VP, _VP, _VP, _VP, _I64, _I, _D, _D, _VP],
+    "psa_max_abs_interior": [_I, _VP, _VP, _I64, _I, _VP],
+    "psa_norm_workspace_bytes": [],
+    "psa_interior_norm": [_I, _I, _VP, _I64, _I64, _D, _VP, _VP, ctypes.c_size_t,
+                          _VP],
+    "psa_control_init": [ctypes.POINTER(PsaSolverDesc), _VP, _VP, _D, _D, _LL,
+                         _I, _D, _D, _D, _VP],
+    "psa_ssprk33_steps": [ctypes.POINTER(PsaSolverDesc), _VP, _VP, _VP, _VP,
+                          _VP, _LL, _VP],
+    "psa_ssprk33_advance": [ctypes.POINTER(PsaSolverDesc), _VP, _VP, _VP, _VP,
+                            _VP, _VP],
+    "psa_ssprk33_stage": [ctypes.POINTER(PsaSolverDesc), _I, _VP, _VP, _VP, _VP,
+                          _VP, _VP],
+}
+
+_LIB: ctypes.CDLL | None = None
+
+
+def load(path: str | os.PathLike[str] | None = None) -> ctypes.CDLL:
+    """Load the library (once) and declare its signatures. Raises
+    :class:`RuntimeError` if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+
+    libpath = pathlib.Path(path) if path is not None else LIBPATH
+    if not libpath.exists():
+        raise RuntimeError(
+            f"{libpath.name} is not built: run 'python -m pyshocks_amd.build' "
+            "(the HIP library has no CPU fallback)")
+
+    lib = ctypes.CDLL(str(libpath))
+    for name, argtypes in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = ctypes.c_int
+
+    if lib.psa_control_bytes() != ctypes.sizeof(PsaControl):
+        raise RuntimeError("psa_control layout mismatch between header and binding")
+
+    _LIB = lib
+    return lib
+
+
+def last_error() -> str:
+    lib = load()
+    buf = ctypes.create_string_buffer(1024)
+    lib.psa_get_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int, what: str = "") -> None:
+    """Map a status code onto the reference's exception classes."""
+    if status == PSA_OK:
+        return
+    msg = f"{what}: {last_error()}" if what else last_error()
+    if status == PSA_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if status == PSA_ERR_NOT_IMPLEMENTED:
+        raise NotImplementedError(msg)
+    if status == PSA_ERR_NONFINITE_DT:
+        raise NonFiniteTimestepError(msg)
+    raise RuntimeError(msg)
+
+
+def call(name: str, *args: Any) -> None:
+    lib = load()
+    check(getattr(lib, name)(*args), name)
+
+
+# {{{ tensor helpers
+
+
+def dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float64:
+        return PSA_FLOAT64
+    if dtype == torch.float32:
+        return PSA_FLOAT32
+    raise NotImplementedError(f"unsupported dtype: {dtype}")
+
+
+def require_device(*tensors: torch.Tensor | None) -> None:
+    """Every array handed to the library must be a contiguous GPU tensor."""
+    for t in tensors:
+        if t is None:
+            continue
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"expected a torch.Tensor, got {type(t).__name__}")
+        if not t.is_cuda:
+            raise ValueError(
+                "pyshocks_amd computes on the GPU only; got a tensor on "
+                f"{t.device} (move it to 'cuda')")
+        if not t.is_contiguous():
+            raise ValueError("arrays must be contiguous")
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def stream_of(t: torch.Tensor) -> int:
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+# }}}

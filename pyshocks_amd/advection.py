@@ -1,0 +1,140 @@
+"""Linear advection ``u_t + a u_x = 0`` with the Godunov (upwind) scheme.
+
+Mirrors ``advection/schemes.py`` of pyshocks (``AdvectionScheme``
+``:41-48``, ``predict_timestep`` ``:51-59``, ``apply_operator`` ``:62-73``,
+``upwind_flux`` ``:100-114``, ``Godunov`` ``:118-129``) and the registry of
+``advection/__init__.py:59-96``.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, fields
+from typing import TYPE_CHECKING, Any
+
+import torch
+
+from pyshocks_amd import _lib
+from pyshocks_amd.reductions import max_abs_interior
+from pyshocks_amd.schemes import (
+    Boundary,
+    FiniteVolumeSchemeBase,
+    SchemeBase,
+    apply_operator,
+    numerical_flux,
+    predict_timestep,
+)
+
+if TYPE_CHECKING:
+    from pyshocks_amd.grid import Grid
+
+
+@dataclass(frozen=True)
+class AdvectionScheme(SchemeBase):
+    """Base class for numerical schemes for linear advection."""
+
+    velocity: torch.Tensor | None
+    """Advection velocity at the cell centres (shape of ``grid.x``)."""
+
+
+@predict_timestep.register(AdvectionScheme)
+def _predict_timestep_advection(scheme: AdvectionScheme, grid: Grid, bc: Boundary,
+                                t: Any, u: torch.Tensor) -> torch.Tensor:
+    assert scheme.velocity is not None
+    amax = max_abs_interior(scheme.velocity, grid.nghosts)
+    return grid.dx_min / amax
+
+
+@dataclass(frozen=True)
+class FiniteVolumeScheme(AdvectionScheme, FiniteVolumeSchemeBase):
+    """Finite volume schemes for linear advection."""
+
+
+def _eps_and_check(scheme: AdvectionScheme) -> float:
+    from pyshocks_amd.reconstruction import WENOJS53
+
+    if not isinstance(scheme.rec, WENOJS53):
+        raise NotImplementedError(
+            f"reconstruction {type(scheme.rec).__name__} is not on the GPU path")
+    return float(scheme.rec.eps)
+
+
+def upwind_flux(scheme: AdvectionScheme, grid: Grid, bc: Boundary,
+                u: torch.Tensor) -> torch.Tensor:
+    """``F = ur if (ar + al) / 2 > 0 else ul`` with ``u`` and the velocity
+    both reconstructed by WENO-JS5."""
+    assert scheme.velocity is not None
+    assert scheme.rec is not None
+    assert u.shape[0] == grid.x.numel()
+
+    from pyshocks_amd.scalar import _numerical_flux
+
+    return _numerical_flux(_lib.PSA_FLUX_ADVECTION_UPWIND, u, scheme.velocity,
+                           None, _eps_and_check(scheme))
+
+
+@dataclass(frozen=True)
+class Godunov(FiniteVolumeScheme):
+    """A Godunov (upwind) scheme for the advection equation."""
+
+
+@numerical_flux.register(Godunov)
+def _numerical_flux_advection_godunov(scheme: Godunov, grid: Grid, bc: Boundary,
+                                      t: Any, u: torch.Tensor) -> torch.Tensor:
+    return upwind_flux(scheme, grid, bc, u)
+
+
+@apply_operator.register(Godunov)
+def _apply_operator_advection(scheme: Godunov, grid: Grid, bc: Boundary, t: Any,
+                              u: torch.Tensor) -> torch.Tensor:
+    """``-velocity * (F[1:] - F[:-1]) / dx`` with the periodic ghost fill, in
+    one kernel."""
+    from pyshocks_amd.scalar import PeriodicBoundary
+
+    assert scheme.velocity is not None
+    if not isinstance(bc, PeriodicBoundary):
+        raise NotImplementedError(
+            f"boundary {type(bc).__name__} is not on the GPU path")
+    _lib.require_device(u, scheme.velocity)
+    out = torch.empty_like(u)
+    _lib.call("psa_apply_operator", _lib.dtype_code(u.dtype),
+              _lib.PSA_FLUX_ADVECTION_UPWIND, _lib.ptr(u), _lib.ptr(scheme.velocity),
+              None, _lib.ptr(out), u.numel(), grid.nghosts, grid.h,
+              _eps_and_check(scheme), _lib.stream_of(u))
+    return out
+
+
+Upwind = Godunov
+
+# {{{ registry
+
+_SCHEMES: dict[str, type[AdvectionScheme]] = {
+    "default": Godunov,
+    "godunov": Godunov,
+    "upwind": Godunov,
+}
+
+_NOT_ON_GPU = ("esweno32", "sbp", "splitgodunov")
+
+
+def scheme_ids() -> tuple[str, ...]:
+    return tuple(_SCHEMES.keys())
+
+
+def make_scheme_from_name(name: str, **kwargs: Any) -> AdvectionScheme:
+    """``advection/__init__.py:76-96``: ``velocity`` defaults to ``None``."""
+    cls = _SCHEMES.get(name)
+    if cls is None:
+        if name in _NOT_ON_GPU:
+            raise NotImplementedError(
+                f"Scheme {name!r} is not part of the GPU hot path; available: "
+                f"{', '.join(scheme_ids())}.")
+        raise ValueError(f"Scheme {name!r} not found. Try one of "
+                         f"{', '.join(scheme_ids())}.")
+
+    if "velocity" not in kwargs:
+        kwargs["velocity"] = None
+
+    return cls(**{f.name: kwargs[f.name] for f in fields(cls) if f.name in kwargs})
+
+
+# }}}

@@ -1,0 +1,1107 @@
+// Forward WENO-JS5 / Rusanov / Godunov / SSP-RK3 kernels for gfx950 and the
+// C ABI declared in include/pyshocks_amd.h.
+//
+// Layout: every state is a contiguous array of m = n + 2 g cells (the
+// reference's grid.x layout, grid.py:131-179). The fused stage kernel tiles
+// the cell index space from 0 in tiles of BLOCK * CPT cells; each workgroup
+// stages its tile plus a 4-cell halo per side in LDS from coalesced loads,
+// every thread reconstructs CPT consecutive cells (both sides from one set
+// of smoothness indicators), neighbouring threads exchange one right and one
+// left state through LDS, each thread evaluates its CPT + 1 faces, applies
+// the conservative difference and the SSP-RK3 stage combination, and the
+// results leave through LDS as coalesced stores.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "../../include/pyshocks_amd.h"
+#include "psa_internal.hpp"
+#include "weno5_device.hpp"
+
+namespace psa {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int check_launch(const char *what) {
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    return set_error(PSA_ERR_HIP, std::string(what) + ": " +
+                                      hipGetErrorString(err));
+  }
+  return PSA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// loads
+
+// Value of cell gm of the periodic ghost-filled array (scalar.py:529-540):
+// ghost cells read their interior image; outside [0, m) the 'same'
+// convolution pads with zeros (convolve.py:113-114).
+template <typename T>
+__device__ __forceinline__ T load_filled(const T *__restrict__ u, int64_t gm,
+                                         int64_t m, int64_t n, int g) {
+  if (gm < 0 || gm >= m) return T(0);
+  int64_t idx = gm;
+  if (idx < g) {
+    idx += n;
+  } else if (idx >= m - g) {
+    idx -= n;
+  }
+  return u[idx];
+}
+
+// Value of cell gm of an array used as given, zero outside [0, m).
+template <typename T>
+__device__ __forceinline__ T load_raw(const T *__restrict__ u, int64_t gm,
+                                      int64_t m) {
+  return (gm >= 0 && gm < m) ? u[gm] : T(0);
+}
+
+// Non-negative value -> order-preserving bit pattern (NaN sorts above inf).
+__device__ __forceinline__ unsigned long long abs_bits(double v) {
+  return static_cast<unsigned long long>(__double_as_longlong(fabs(v)));
+}
+__device__ __forceinline__ unsigned long long abs_bits(float v) {
+  return static_cast<unsigned long long>(
+      static_cast<unsigned int>(__float_as_uint(fabsf(v))));
+}
+template <typename T>
+__device__ __forceinline__ T from_bits(unsigned long long b);
+template <>
+__device__ __forceinline__ double from_bits<double>(unsigned long long b) {
+  return __longlong_as_double(static_cast<long long>(b));
+}
+template <>
+__device__ __forceinline__ float from_bits<float>(unsigned long long b) {
+  return __uint_as_float(static_cast<unsigned int>(b));
+}
+
+template <int BLOCK>
+__device__ __forceinline__ unsigned long long block_max_bits(
+    unsigned long long v, unsigned long long *s_red) {
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (lane == 0) s_red[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BLOCK / 64; ++w) v = s_red[w] > v ? s_red[w] : v;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// standalone weno.py kernels (reference expression order)
+
+template <typename T>
+__global__ void k_weno_fields(const T *__restrict__ u, int64_t m, T eps,
+                              T *__restrict__ beta, T *__restrict__ uhat,
+                              T *__restrict__ omega) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const T um2 = load_raw(u, i - 2, m), um1 = load_raw(u, i - 1, m);
+  const T u0 = u[i];
+  const T up1 = load_raw(u, i + 1, m), up2 = load_raw(u, i + 2, m);
+
+  T b0, b1, b2;
+  weno_beta(um2, um1, u0, up1, up2, b0, b1, b2);
+  if (beta != nullptr) {
+    beta[i] = b0;
+    beta[m + i] = b1;
+    beta[2 * m + i] = b2;
+  }
+  if (uhat != nullptr) {
+    T h0, h1, h2;
+    weno_uhat(um2, um1, u0, up1, up2, h0, h1, h2);
+    uhat[i] = h0;
+    uhat[m + i] = h1;
+    uhat[2 * m + i] = h2;
+  }
+  if (omega != nullptr) {
+    using C = Weno5Coeffs<T>;
+    const T e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+    const T a0 = C::d0 / (e0 * e0);
+    const T a1 = C::d1 / (e1 * e1);
+    const T a2 = C::d2 / (e2 * e2);
+    const T s = (a0 + a1) + a2;
+    omega[i] = a0 / s;
+    omega[m + i] = a1 / s;
+    omega[2 * m + i] = a2 / s;
+  }
+}
+
+template <typename T>
+__global__ void k_weno_reconstruct(const T *__restrict__ f, int64_t m, T eps,
+                                   T *__restrict__ ul, T *__restrict__ ur) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const T um2 = load_raw(f, i - 2, m), um1 = load_raw(f, i - 1, m);
+  const T u0 = f[i];
+  const T up1 = load_raw(f, i + 1, m), up2 = load_raw(f, i + 2, m);
+  ur[i] = weno_side_ref(um2, um1, u0, up1, up2, eps);
+  ul[i] = weno_side_ref(up2, up1, u0, um1, um2, eps);
+}
+
+template <typename T>
+__global__ void k_apply_boundary(const T *__restrict__ u, T *__restrict__ out,
+                                 int64_t m, int64_t n, int g) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  // in place is safe: ghosts read interior cells, interior cells read
+  // themselves
+  if (i < g || i >= m - g) {
+    out[i] = load_filled(u, i, m, n, g);
+  } else if (out != u) {
+    out[i] = u[i];
+  }
+}
+
+// numerical_flux on u as given: one thread per face j in [0, m], face j lies
+// between cells j - 1 and j; fnum[0] = fnum[m] = 0 (jnp.pad(fnum, 1)).
+template <typename T, int FLUX>
+__global__ void k_numerical_flux(const T *__restrict__ u,
+                                 const T *__restrict__ vel,
+                                 const T *__restrict__ nu, int64_t m, T eps,
+                                 T *__restrict__ fnum) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j > m) return;
+  if (j == 0 || j == m) {
+    fnum[j] = T(0);
+    return;
+  }
+  // cells j - 3 .. j + 2
+  T v[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = load_raw(u, j - 3 + k, m);
+  // ur of cell j - 1 and ul of cell j
+  const T urL = weno_side_ref(v[0], v[1], v[2], v[3], v[4], eps);
+  const T ulR = weno_side_ref(v[5], v[4], v[3], v[2], v[1], eps);
+  T F;
+  if constexpr (FLUX == PSA_FLUX_RUSANOV) {
+    const T nuj = nu != nullptr ? nu[j - 1] : T(1);
+    F = flux_rusanov(urL, ulR, v[2], v[3], nuj);
+  } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
+    F = flux_burgers_upwind(urL, ulR);
+  } else {
+    T a[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a[k] = load_raw(vel, j - 3 + k, m);
+    const T arL = weno_side_ref(a[0], a[1], a[2], a[3], a[4], eps);
+    const T alR = weno_side_ref(a[5], a[4], a[3], a[2], a[1], eps);
+    F = flux_advection_upwind(urL, ulR, arL, alR);
+  }
+  fnum[j] = F;
+}
+
+template <typename T>
+__global__ void k_fill_zero_bits(unsigned long long *bits) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *bits = 0ull;
+}
+
+template <typename T>
+__global__ void k_store_zero(T *out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = T(0);
+}
+
+// max |u[g:m-g]| by atomicMax on the bit pattern of the (non-negative)
+// result; out points to a scalar of the dtype (zeroed beforehand).
+template <typename T, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_max_abs(const T *__restrict__ u,
+                                                   int64_t m, int g,
+                                                   T *__restrict__ out,
+                                                   unsigned long long *bits) {
+  __shared__ unsigned long long s_red[BLOCK / 64];
+  unsigned long long v = 0;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * BLOCK;
+  for (int64_t i = g + static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+       i < m - g; i += stride) {
+    const unsigned long long b = abs_bits(u[i]);
+    v = b > v ? b : v;
+  }
+  v = block_max_bits<BLOCK>(v, s_red);
+  if (threadIdx.x == 0) {
+    if (bits != nullptr) {
+      atomicMax(bits, v);
+    } else if constexpr (sizeof(T) == 8) {
+      atomicMax(reinterpret_cast<unsigned long long *>(out), v);
+    } else {
+      atomicMax(reinterpret_cast<unsigned int *>(out),
+                static_cast<unsigned int>(v));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// device-side step control (timestepping.py:89-152)
+
+template <typename T>
+__global__ void k_control_init(psa_control *ctrl, double t, double tfinal,
+                               double theta, double cfl_num, double fixed_dt,
+                               long long maxit, int dt_mode,
+                               int speed_from_state, double speed_const) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  ctrl->t = t;
+  ctrl->dt = 0.0;
+  ctrl->tfinal = tfinal;
+  ctrl->theta = theta;
+  ctrl->cfl_num = cfl_num;
+  ctrl->fixed_dt = fixed_dt;
+  ctrl->m = 0;
+  ctrl->maxit = maxit;
+  ctrl->umax_bits = abs_bits(static_cast<T>(speed_const));
+  ctrl->active = 0;
+  ctrl->status = PSA_OK;
+  ctrl->dt_mode = dt_mode;
+  ctrl->speed_from_state = speed_from_state;
+}
+
+// Decision of stage 1, evaluated identically by every workgroup.
+template <typename T>
+__device__ __forceinline__ bool control_decide(const psa_control *ctrl, T &dt,
+                                               int &status) {
+  status = ctrl->status;
+  if (status != PSA_OK) return false;
+  const T t = static_cast<T>(ctrl->t);
+  const T tfinal = static_cast<T>(ctrl->tfinal);
+  if (t >= tfinal) return false;
+  if (ctrl->maxit >= 0 && ctrl->m >= ctrl->maxit) return false;
+
+  if (ctrl->dt_mode == PSA_DT_FIXED) {
+    dt = static_cast<T>(ctrl->fixed_dt);
+  } else {
+    const T smax = from_bits<T>(ctrl->umax_bits);
+    dt = static_cast<T>(ctrl->theta) * (static_cast<T>(ctrl->cfl_num) / smax);
+  }
+  if (!isinf(tfinal)) {
+    const T dt_min = tfinal - t;
+    dt = (dt < dt_min ? dt : dt_min) + static_cast<T>(1.0e-15);
+  }
+  if (!isfinite(dt)) {
+    status = PSA_ERR_NONFINITE_DT;
+    return false;
+  }
+  return true;
+}
+
+template <typename T>
+__global__ void k_copy_if_inactive(const T *__restrict__ src,
+                                   T *__restrict__ dst, int64_t m,
+                                   const psa_control *ctrl) {
+  if (ctrl->active != 0) return;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < m) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// fused tiled stage kernel
+
+template <typename T>
+struct StageArgs {
+  const T *un;      // u^n (stages 2 and 3)
+  const T *uk;      // stage input: the stencil is evaluated on it
+  T *out;           // stage output
+  const T *vel;     // advection velocity (VEL == 2)
+  const T *nu;      // Rusanov df^(alpha-1) per face j - 1, or null
+  psa_control *ctrl;
+  const T *dt_ptr;  // used when ctrl is null
+  int64_t m;
+  int64_t n;
+  int g;
+  T dx;
+  T inv_dx;
+  T eps;
+  T vconst;
+};
+
+template <typename T, int VEL>
+__device__ __forceinline__ T velocity_at(const T *s_v, int j, T vconst) {
+  if constexpr (VEL == 2) {
+    return s_v[j];
+  } else {
+    return vconst;
+  }
+}
+
+// VEL: 0 Burgers (no velocity), 1 constant velocity, 2 velocity array.
+// STAGE: 0 operator (all m cells, faces 0 and m padded with zero flux),
+//        1..3 SSP-RK3 stages (interior cells, ghost images rewritten).
+template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT>
+__global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
+  constexpr int TILE = BLOCK * CPT;
+  constexpr int HALO = 4;
+  constexpr int NV = VEL == 2 ? TILE + 2 * HALO : 1;
+  constexpr int NVX = VEL == 2 ? BLOCK + 1 : 1;
+  __shared__ T s_u[TILE + 2 * HALO];
+  __shared__ T s_v[NV];
+  __shared__ T s_io[TILE];
+  __shared__ T s_urL[BLOCK + 1];
+  __shared__ T s_ulF[BLOCK + 1];
+  __shared__ T s_arL[NVX];
+  __shared__ T s_alF[NVX];
+  __shared__ unsigned long long s_red[BLOCK / 64];
+
+  const int tid = threadIdx.x;
+  const int64_t S = static_cast<int64_t>(blockIdx.x) * TILE;
+  const int64_t m = p.m;
+  const int64_t n = p.n;
+  const int g = p.g;
+
+  // ---- step control
+  T dt = T(0);
+  if constexpr (STAGE > 0) {
+    if (p.ctrl != nullptr) {
+      if constexpr (STAGE == 1) {
+        int status;
+        const bool active = control_decide<T>(p.ctrl, dt, status);
+        if (blockIdx.x == 0 && tid == 0) {
+          p.ctrl->active = active ? 1 : 0;
+          p.ctrl->status = status;
+          if (active) p.ctrl->dt = static_cast<double>(dt);
+        }
+        if (!active) return;
+      } else {
+        if (p.ctrl->active == 0) return;
+        dt = static_cast<T>(p.ctrl->dt);
+        if constexpr (STAGE == 2) {
+          if (blockIdx.x == 0 && tid == 0) {
+            p.ctrl->t = static_cast<double>(static_cast<T>(p.ctrl->t) + dt);
+            p.ctrl->m = p.ctrl->m + 1;
+            if (p.ctrl->speed_from_state) p.ctrl->umax_bits = 0ull;
+          }
+        }
+      }
+    } else {
+      dt = *p.dt_ptr;
+    }
+  }
+
+  // ---- stage the tile and its halo
+  for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {
+    s_u[j] = load_filled(p.uk, S - HALO + j, m, n, g);
+  }
+  if constexpr (VEL == 2) {
+    for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {
+      s_v[j] = load_raw(p.vel, S - HALO + j, m);
+    }
+  }
+  if constexpr (STAGE >= 2) {
+    for (int j = tid; j < TILE; j += BLOCK) {
+      const int64_t gm = S + j;
+      s_io[j] = gm < m ? p.un[gm] : T(0);
+    }
+  }
+  __syncthreads();
+
+  // ---- reconstruct CPT consecutive cells
+  using C = Weno5Coeffs<T>;
+  const int l0 = tid * CPT;
+  T w[CPT + 4];  // cells l0 - 2 .. l0 + CPT + 1
+#pragma unroll
+  for (int k = 0; k < CPT + 4; ++k) w[k] = s_u[l0 + 2 + k];
+
+  // a0 * (second difference)^2 at cells l0 - 1 .. l0 + CPT, shared by the
+  // three indicators of neighbouring cells
+  T E[CPT + 2];
+#pragma unroll
+  for (int k = 0; k < CPT + 2; ++k) {
+    const T d2 = (w[k + 2] - T(2) * w[k + 1]) + w[k];
+    E[k] = C::a0 * (d2 * d2);
+  }
+
+  T ul[CPT], ur[CPT];
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) {
+    const T um2 = w[r], um1 = w[r + 1], u0 = w[r + 2], up1 = w[r + 3],
+            up2 = w[r + 4];
+    const T t01 = (T(3) * u0 - T(4) * um1) + um2;
+    const T t11 = up1 - um1;
+    const T t21 = (up2 - T(4) * up1) + T(3) * u0;
+    const T b0 = E[r] + C::a1 * (t01 * t01);
+    const T b1 = E[r + 1] + C::a1 * (t11 * t11);
+    const T b2 = E[r + 2] + C::a1 * (t21 * t21);
+    weno_pair_from_beta(um2, um1, u0, up1, up2, b0, b1, b2, p.eps, ul[r],
+                        ur[r]);
+  }
+
+  T al[VEL == 2 ? CPT : 1], ar[VEL == 2 ? CPT : 1];
+  if constexpr (VEL == 2) {
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+      const int b = l0 + HALO + r;
+      weno_pair(s_v[b - 2], s_v[b - 1], s_v[b], s_v[b + 1], s_v[b + 2], p.eps,
+                al[r], ar[r]);
+    }
+  }
+
+  // ---- exchange edge states with the neighbouring threads
+  s_urL[tid + 1] = ur[CPT - 1];
+  s_ulF[tid] = ul[0];
+  if constexpr (VEL == 2) {
+    s_arL[tid + 1] = ar[CPT - 1];
+    s_alF[tid] = al[0];
+  }
+  if (tid == 0) {
+    // right state of local cell -1
+    s_urL[0] = weno_right(s_u[1], s_u[2], s_u[3], s_u[4], s_u[5], p.eps);
+    if constexpr (VEL == 2) {
+      s_arL[0] = weno_right(s_v[1], s_v[2], s_v[3], s_v[4], s_v[5], p.eps);
+    }
+  }
+  if (tid == BLOCK - 1) {
+    // left state of local cell TILE (mirrored stencil)
+    constexpr int c = TILE + HALO;
+    s_ulF[BLOCK] = weno_right(s_u[c + 2], s_u[c + 1], s_u[c], s_u[c - 1],
+                              s_u[c - 2], p.eps);
+    if constexpr (VEL == 2) {
+      s_alF[BLOCK] = weno_right(s_v[c + 2], s_v[c + 1], s_v[c], s_v[c - 1],
+                                s_v[c - 2], p.eps);
+    }
+  }
+  __syncthreads();
+
+  // ---- faces l0 - 1/2 .. l0 + CPT - 1/2
+  T F[CPT + 1];
+#pragma unroll
+  for (int k = 0; k <= CPT; ++k) {
+    const T urL = k == 0 ? s_urL[tid] : ur[k - 1];
+    const T ulR = k == CPT ? s_ulF[tid + 1] : ul[k];
+    const int64_t gm = S + l0 + k;  // face between cells gm - 1 and gm
+    T Fk;
+    if constexpr (FLUX == PSA_FLUX_RUSANOV) {
+      T nuk = T(1);
+      if (p.nu != nullptr && gm >= 1 && gm <= m - 1) nuk = p.nu[gm - 1];
+      Fk = flux_rusanov(urL, ulR, w[k + 1], w[k + 2], nuk);
+    } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
+      Fk = flux_burgers_upwind(urL, ulR);
+    } else {
+      T arL, alR;
+      if constexpr (VEL == 2) {
+        arL = k == 0 ? s_arL[tid] : ar[k - 1];
+        alR = k == CPT ? s_alF[tid + 1] : al[k];
+      } else {
+        // a constant velocity reconstructs to itself up to rounding, so the
+        // upwind switch only sees its sign
+        arL = p.vconst;
+        alR = p.vconst;
+      }
+      Fk = flux_advection_upwind(urL, ulR, arL, alR);
+    }
+    if constexpr (STAGE == 0) {
+      if (gm <= 0 || gm >= m) Fk = T(0);
+    }
+    F[k] = Fk;
+  }
+
+  // ---- conservative difference and stage combination
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) {
+    const T dF = F[r + 1] - F[r];
+    T L;
+    if constexpr (STAGE == 0) {
+      if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
+        L = (-velocity_at<T, VEL>(s_v, l0 + HALO + r, p.vconst) * dF) / p.dx;
+      } else {
+        L = -dF / p.dx;
+      }
+    } else {
+      if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
+        L = (-velocity_at<T, VEL>(s_v, l0 + HALO + r, p.vconst) * dF) *
+            p.inv_dx;
+      } else {
+        L = -dF * p.inv_dx;
+      }
+    }
+    const T uk = w[r + 2];
+    T val;
+    if constexpr (STAGE == 0) {
+      val = L;
+    } else if constexpr (STAGE == 1) {
+      val = uk + dt * L;
+    } else if constexpr (STAGE == 2) {
+      val = T(3.0 / 4.0) * s_io[l0 + r] + T(1.0 / 4.0) * (uk + dt * L);
+    } else {
+      val = T(1.0 / 3.0) * s_io[l0 + r] + T(2.0 / 3.0) * (uk + dt * L);
+    }
+    s_io[l0 + r] = val;
+  }
+  __syncthreads();
+
+  // ---- coalesced stores (+ ghost images, + max|u| of the new state)
+  unsigned long long vmax = 0;
+  for (int j = tid; j < TILE; j += BLOCK) {
+    const int64_t gm = S + j;
+    if constexpr (STAGE == 0) {
+      if (gm < m) p.out[gm] = s_io[j];
+    } else {
+      if (gm >= g && gm < m - g) {
+        const T v = s_io[j];
+        p.out[gm] = v;
+        if (gm < 2 * g) p.out[gm + n] = v;
+        if (gm >= m - 2 * g) p.out[gm - n] = v;
+        if constexpr (STAGE == 3) {
+          const unsigned long long b = abs_bits(v);
+          vmax = b > vmax ? b : vmax;
+        }
+      }
+    }
+  }
+  if constexpr (STAGE == 3) {
+    if (p.ctrl != nullptr && p.ctrl->speed_from_state) {
+      vmax = block_max_bits<BLOCK>(vmax, s_red);
+      if (tid == 0) atomicMax(&p.ctrl->umax_bits, vmax);
+    }
+  }
+}
+
+}  // namespace psa
+
+// ===========================================================================
+// host side
+
+using namespace psa;
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline unsigned int blocks_for(int64_t work, int per_block) {
+  return static_cast<unsigned int>((work + per_block - 1) / per_block);
+}
+
+int check_dtype(int dtype) {
+  if (dtype != PSA_FLOAT64 && dtype != PSA_FLOAT32) {
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "unsupported dtype " + std::to_string(dtype));
+  }
+  return PSA_OK;
+}
+
+int check_grid(int64_t m, int g) {
+  if (g < 3) {
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "WENO-JS5 needs at least 3 ghost cells (nghosts >= "
+                     "stencil_width), got " +
+                         std::to_string(g));
+  }
+  if (m - 2 * static_cast<int64_t>(g) < g) {
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "periodic ghost fill needs at least nghosts interior "
+                     "cells, got m = " +
+                         std::to_string(m));
+  }
+  return PSA_OK;
+}
+
+template <typename T>
+int launch_max_abs(const T *u, int64_t m, int g, T *out,
+                   unsigned long long *bits, hipStream_t stream) {
+  constexpr int B = 256;
+  const int64_t work = m - 2 * static_cast<int64_t>(g);
+  unsigned int blocks = blocks_for(work, B * 8);
+  if (blocks < 1) blocks = 1;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL((k_max_abs<T, B>), dim3(blocks), dim3(B), 0, stream, u,
+                     m, g, out, bits);
+  return check_launch("max_abs");
+}
+
+}  // namespace
+
+// --- stage dispatch ---------------------------------------------------------
+
+namespace psa {
+
+struct TileConfig {
+  int block;
+  int cpt;
+};
+
+template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT>
+int launch_stage_cfg(const StageArgs<T> &a, hipStream_t stream) {
+  constexpr int TILE = BLOCK * CPT;
+  const unsigned int blocks = blocks_for(a.m, TILE);
+  hipLaunchKernelGGL((k_stage<T, FLUX, VEL, STAGE, BLOCK, CPT>), dim3(blocks),
+                     dim3(BLOCK), 0, stream, a);
+  return check_launch("ssprk33 stage");
+}
+
+template <typename T, int FLUX, int VEL, int STAGE>
+int launch_stage_tiles(const StageArgs<T> &a, TileConfig cfg,
+                       hipStream_t stream) {
+  if (cfg.block == 256 && cfg.cpt == 1)
+    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 1>(a, stream);
+  if (cfg.block == 256 && cfg.cpt == 2)
+    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 2>(a, stream);
+  if (cfg.block == 256 && cfg.cpt == 4)
+    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 4>(a, stream);
+  return set_error(PSA_ERR_INVALID_ARGUMENT,
+                   "unsupported tile configuration block=" +
+                       std::to_string(cfg.block) +
+                       " cells_per_thread=" + std::to_string(cfg.cpt));
+}
+
+template <typename T, int STAGE>
+int launch_stage(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
+                 hipStream_t stream) {
+  if (flux == PSA_FLUX_RUSANOV)
+    return launch_stage_tiles<T, PSA_FLUX_RUSANOV, 0, STAGE>(a, cfg, stream);
+  if (flux == PSA_FLUX_BURGERS_UPWIND)
+    return launch_stage_tiles<T, PSA_FLUX_BURGERS_UPWIND, 0, STAGE>(a, cfg,
+                                                                    stream);
+  if (flux == PSA_FLUX_ADVECTION_UPWIND) {
+    if (vel == 1)
+      return launch_stage_tiles<T, PSA_FLUX_ADVECTION_UPWIND, 1, STAGE>(
+          a, cfg, stream);
+    return launch_stage_tiles<T, PSA_FLUX_ADVECTION_UPWIND, 2, STAGE>(a, cfg,
+                                                                      stream);
+  }
+  return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                   "unsupported flux " + std::to_string(flux));
+}
+
+TileConfig default_tiles(int dtype) {
+  (void)dtype;
+  return TileConfig{256, 4};
+}
+
+int check_desc(const psa_solver_desc *d) {
+  if (d == nullptr) return set_error(PSA_ERR_INVALID_ARGUMENT, "null desc");
+  int st = check_dtype(d->dtype);
+  if (st != PSA_OK) return st;
+  if (d->n <= 0) return set_error(PSA_ERR_INVALID_ARGUMENT, "n must be > 0");
+  st = check_grid(d->n + 2 * static_cast<int64_t>(d->g), d->g);
+  if (st != PSA_OK) return st;
+  if (!(d->dx > 0.0) || !std::isfinite(d->dx))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "dx must be positive");
+  if (!(d->eps >= 0.0))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "eps must be >= 0");
+  if (d->flux == PSA_FLUX_ADVECTION_UPWIND && !d->velocity_is_const &&
+      d->velocity == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "advection needs a velocity array or a constant");
+  if (d->flux < 0 || d->flux > 2)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "unsupported flux " + std::to_string(d->flux));
+  return PSA_OK;
+}
+
+TileConfig tiles_of(const psa_solver_desc *d) {
+  TileConfig c = default_tiles(d->dtype);
+  if (d->block > 0) c.block = d->block;
+  if (d->cells_per_thread > 0) c.cpt = d->cells_per_thread;
+  return c;
+}
+
+template <typename T>
+StageArgs<T> make_args(const psa_solver_desc *d) {
+  StageArgs<T> a{};
+  a.m = d->n + 2 * static_cast<int64_t>(d->g);
+  a.n = d->n;
+  a.g = d->g;
+  a.dx = static_cast<T>(d->dx);
+  a.inv_dx = static_cast<T>(1.0) / static_cast<T>(d->dx);
+  a.eps = static_cast<T>(d->eps);
+  a.vconst = static_cast<T>(d->velocity_const);
+  a.vel = static_cast<const T *>(d->velocity);
+  a.nu = static_cast<const T *>(d->nu);
+  return a;
+}
+
+inline int vel_mode(const psa_solver_desc *d) {
+  if (d->flux != PSA_FLUX_ADVECTION_UPWIND) return 0;
+  return d->velocity_is_const ? 1 : 2;
+}
+
+template <typename T>
+int run_steps(const psa_solver_desc *d, const void *u_in, void *u_out,
+              void *k1, void *k2, void *control, const void *dt_dev,
+              long long nsteps, hipStream_t stream) {
+  StageArgs<T> a = make_args<T>(d);
+  a.ctrl = static_cast<psa_control *>(control);
+  a.dt_ptr = static_cast<const T *>(dt_dev);
+  const TileConfig cfg = tiles_of(d);
+  const int vel = vel_mode(d);
+  const T *un = static_cast<const T *>(u_in);
+  T *uo = static_cast<T *>(u_out);
+  T *b1 = static_cast<T *>(k1);
+  T *b2 = static_cast<T *>(k2);
+  int st = PSA_OK;
+  for (long long s = 0; s < nsteps; ++s) {
+    // after the first step the state lives in u_out
+    const T *cur = s == 0 ? un : uo;
+    a.un = cur;
+    a.uk = cur;
+    a.out = b1;
+    st = launch_stage<T, 1>(d->flux, vel, a, cfg, stream);
+    if (st != PSA_OK) return st;
+    a.uk = b1;
+    a.out = b2;
+    st = launch_stage<T, 2>(d->flux, vel, a, cfg, stream);
+    if (st != PSA_OK) return st;
+    a.uk = b2;
+    a.out = uo;
+    st = launch_stage<T, 3>(d->flux, vel, a, cfg, stream);
+    if (st != PSA_OK) return st;
+    if (s == 0 && un != uo && a.ctrl != nullptr) {
+      // the first step reads u_in and writes u_out; if it was not taken,
+      // u_out must still receive the state
+      const unsigned int blocks = blocks_for(a.m, 256);
+      hipLaunchKernelGGL(k_copy_if_inactive<T>, dim3(blocks), dim3(256), 0,
+                         stream, un, uo, a.m, a.ctrl);
+      st = check_launch("copy_if_inactive");
+      if (st != PSA_OK) return st;
+    }
+  }
+  return PSA_OK;
+}
+
+}  // namespace psa
+
+namespace {
+
+int weno_fields(int dtype, const void *u, void *beta, void *uhat,
+                       void *omega, int64_t m, double eps, void *stream) {
+  int st = check_dtype(dtype);
+  if (st != PSA_OK) return st;
+  if (u == nullptr || m <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null input or m <= 0");
+  auto s = static_cast<hipStream_t>(stream);
+  const unsigned int blocks = blocks_for(m, kThreads);
+  if (dtype == PSA_FLOAT64) {
+    hipLaunchKernelGGL(k_weno_fields<double>, dim3(blocks), dim3(kThreads), 0,
+                       s, static_cast<const double *>(u), m, eps,
+                       static_cast<double *>(beta), static_cast<double *>(uhat),
+                       static_cast<double *>(omega));
+  } else {
+    hipLaunchKernelGGL(k_weno_fields<float>, dim3(blocks), dim3(kThreads), 0,
+                       s, static_cast<const float *>(u), m,
+                       static_cast<float>(eps), static_cast<float *>(beta),
+                       static_cast<float *>(uhat), static_cast<float *>(omega));
+  }
+  return check_launch("weno_fields");
+}
+
+template <typename T>
+int numerical_flux_t(int flux, const void *u, const void *velocity,
+                            const void *nu, void *fnum, int64_t m, double eps,
+                            hipStream_t s) {
+  const unsigned int blocks = blocks_for(m + 1, kThreads);
+  const T *uu = static_cast<const T *>(u);
+  const T *vv = static_cast<const T *>(velocity);
+  const T *nn = static_cast<const T *>(nu);
+  T *ff = static_cast<T *>(fnum);
+  const T e = static_cast<T>(eps);
+  if (flux == PSA_FLUX_RUSANOV) {
+    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_RUSANOV>), dim3(blocks),
+                       dim3(kThreads), 0, s, uu, vv, nn, m, e, ff);
+  } else if (flux == PSA_FLUX_BURGERS_UPWIND) {
+    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_BURGERS_UPWIND>),
+                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e,
+                       ff);
+  } else {
+    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_ADVECTION_UPWIND>),
+                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e,
+                       ff);
+  }
+  return check_launch("numerical_flux");
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+int psa_version(void) { return 1; }
+
+int psa_control_bytes(void) { return static_cast<int>(sizeof(psa_control)); }
+
+int psa_get_last_error(char *buf, size_t n) {
+  if (buf == nullptr || n == 0) return PSA_ERR_INVALID_ARGUMENT;
+  const std::string &e = psa::g_last_error;
+  const size_t k = e.size() < n - 1 ? e.size() : n - 1;
+  std::memcpy(buf, e.data(), k);
+  buf[k] = '\0';
+  return PSA_OK;
+}
+
+int psa_device_count(int *count) {
+  if (count == nullptr) return set_error(PSA_ERR_INVALID_ARGUMENT, "null");
+  hipError_t err = hipGetDeviceCount(count);
+  if (err != hipSuccess) {
+    *count = 0;
+    return set_error(PSA_ERR_HIP, hipGetErrorString(err));
+  }
+  return PSA_OK;
+}
+
+int psa_weno_smoothness(int dtype, const void *u, void *beta, int64_t m,
+                        void *stream) {
+  if (beta == nullptr) return set_error(PSA_ERR_INVALID_ARGUMENT, "null beta");
+  return weno_fields(dtype, u, beta, nullptr, nullptr, m, 0.0, stream);
+}
+
+int psa_weno_interp(int dtype, const void *u, void *uhat, int64_t m,
+                    void *stream) {
+  if (uhat == nullptr) return set_error(PSA_ERR_INVALID_ARGUMENT, "null uhat");
+  return weno_fields(dtype, u, nullptr, uhat, nullptr, m, 0.0, stream);
+}
+
+int psa_weno_js_weights(int dtype, const void *u, void *omega, int64_t m,
+                        double eps, void *stream) {
+  if (omega == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null omega");
+  return weno_fields(dtype, u, nullptr, nullptr, omega, m, eps, stream);
+}
+
+int psa_weno_reconstruct(int dtype, const void *f, void *ul, void *ur,
+                         int64_t m, double eps, void *stream) {
+  int st = check_dtype(dtype);
+  if (st != PSA_OK) return st;
+  if (f == nullptr || ul == nullptr || ur == nullptr || m <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer or m <= 0");
+  auto s = static_cast<hipStream_t>(stream);
+  const unsigned int blocks = blocks_for(m, kThreads);
+  if (dtype == PSA_FLOAT64) {
+    hipLaunchKernelGGL(k_weno_reconstruct<double>, dim3(blocks),
+                       dim3(kThreads), 0, s, static_cast<const double *>(f), m,
+                       eps, static_cast<double *>(ul),
+                       static_cast<double *>(ur));
+  } else {
+    hipLaunchKernelGGL(k_weno_reconstruct<float>, dim3(blocks), dim3(kThreads),
+                       0, s, static_cast<const float *>(f), m,
+                       static_cast<float>(eps), static_cast<float *>(ul),
+                       static_cast<float *>(ur));
+  }
+  return check_launch("weno_reconstruct");
+}
+
+int psa_apply_boundary_periodic(int dtype, const void *u, void *out,
+                                int64_t m, int g, void *stream) {
+  int st = check_dtype(dtype);
+  if (st != PSA_OK) return st;
+  if (g < 0 || m - 2 * static_cast<int64_t>(g) < g || m <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "bad grid size");
+  if (u == nullptr || out == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  auto s = static_cast<hipStream_t>(stream);
+  const int64_t n = m - 2 * static_cast<int64_t>(g);
+  const unsigned int blocks = blocks_for(m, kThreads);
+  if (dtype == PSA_FLOAT64) {
+    hipLaunchKernelGGL(k_apply_boundary<double>, dim3(blocks), dim3(kThreads),
+                       0, s, static_cast<const double *>(u),
+                       static_cast<double *>(out), m, n, g);
+  } else {
+    hipLaunchKernelGGL(k_apply_boundary<float>, dim3(blocks), dim3(kThreads),
+                       0, s, static_cast<const float *>(u),
+                       static_cast<float *>(out), m, n, g);
+  }
+  return check_launch("apply_boundary");
+}
+
+int psa_numerical_flux(int dtype, int flux, const void *u,
+                       const void *velocity, const void *nu, void *fnum,
+                       int64_t m, double eps, void *stream) {
+  int st = check_dtype(dtype);
+  if (st != PSA_OK) return st;
+  if (flux < 0 || flux > 2)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "unsupported flux " + std::to_string(flux));
+  if (u == nullptr || fnum == nullptr || m < 2)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer or m < 2");
+  if (flux == PSA_FLUX_ADVECTION_UPWIND && velocity == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "advection upwind flux needs a velocity array");
+  auto s = static_cast<hipStream_t>(stream);
+  if (dtype == PSA_FLOAT64)
+    return numerical_flux_t<double>(flux, u, velocity, nu, fnum, m, eps, s);
+  return numerical_flux_t<float>(flux, u, velocity, nu, fnum, m, eps, s);
+}
+
+int psa_apply_operator(int dtype, int flux, const void *u,
+                       const void *velocity, const void *nu, void *out,
+                       int64_t m, int g, double dx, double eps, void *stream) {
+  psa_solver_desc d{};
+  d.dtype = dtype;
+  d.flux = flux;
+  d.g = g;
+  d.n = m - 2 * static_cast<int64_t>(g);
+  d.dx = dx;
+  d.eps = eps;
+  d.velocity = velocity;
+  d.velocity_is_const = 0;
+  d.nu = nu;
+  int st = check_desc(&d);
+  if (st != PSA_OK) return st;
+  if (u == nullptr || out == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  if (u == out)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "apply_operator cannot write into its input");
+  auto s = static_cast<hipStream_t>(stream);
+  const TileConfig cfg = default_tiles(dtype);
+  const int vel = vel_mode(&d);
+  if (dtype == PSA_FLOAT64) {
+    StageArgs<double> a = make_args<double>(&d);
+    a.uk = static_cast<const double *>(u);
+    a.out = static_cast<double *>(out);
+    return launch_stage<double, 0>(flux, vel, a, cfg, s);
+  }
+  StageArgs<float> a = make_args<float>(&d);
+  a.uk = static_cast<const float *>(u);
+  a.out = static_cast<float *>(out);
+  return launch_stage<float, 0>(flux, vel, a, cfg, s);
+}
+
+int psa_max_abs_interior(int dtype, const void *u, void *out, int64_t m,
+                         int g, void *stream) {
+  int st = check_dtype(dtype);
+  if (st != PSA_OK) return st;
+  if (u == nullptr || out == nullptr || g < 0 || m - 2 * (int64_t)g <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "bad arguments");
+  auto s = static_cast<hipStream_t>(stream);
+  if (dtype == PSA_FLOAT64) {
+    hipLaunchKernelGGL(k_store_zero<double>, dim3(1), dim3(64), 0, s,
+                       static_cast<double *>(out));
+    st = check_launch("max_abs init");
+    if (st != PSA_OK) return st;
+    return launch_max_abs<double>(static_cast<const double *>(u), m, g,
+                                  static_cast<double *>(out), nullptr, s);
+  }
+  hipLaunchKernelGGL(k_store_zero<float>, dim3(1), dim3(64), 0, s,
+                     static_cast<float *>(out));
+  st = check_launch("max_abs init");
+  if (st != PSA_OK) return st;
+  return launch_max_abs<float>(static_cast<const float *>(u), m, g,
+                               static_cast<float *>(out), nullptr, s);
+}
+
+int psa_control_init(const psa_solver_desc *desc, const void *u,
+                     void *control, double tstart, double tfinal,
+                     long long maxit, int dt_mode, double theta,
+                     double cfl_num, double fixed_dt, void *stream) {
+  int st = check_desc(desc);
+  if (st != PSA_OK) return st;
+  if (control == nullptr || u == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  if (dt_mode != PSA_DT_CFL && dt_mode != PSA_DT_FIXED)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "bad dt mode");
+  auto s = static_cast<hipStream_t>(stream);
+  const bool advection = desc->flux == PSA_FLUX_ADVECTION_UPWIND;
+  const int speed_from_state = (dt_mode == PSA_DT_CFL && !advection) ? 1 : 0;
+  auto *ctrl = static_cast<psa_control *>(control);
+  // a constant advection velocity without an array: max|a| = |a|
+  const bool const_speed = advection && desc->velocity == nullptr;
+  const double speed = const_speed ? desc->velocity_const : 0.0;
+  if (desc->dtype == PSA_FLOAT64) {
+    hipLaunchKernelGGL(k_control_init<double>, dim3(1), dim3(64), 0, s, ctrl,
+                       tstart, tfinal, theta, cfl_num, fixed_dt, maxit,
+                       dt_mode, speed_from_state, speed);
+  } else {
+    hipLaunchKernelGGL(k_control_init<float>, dim3(1), dim3(64), 0, s, ctrl,
+                       tstart, tfinal, theta, cfl_num, fixed_dt, maxit,
+                       dt_mode, speed_from_state, speed);
+  }
+  st = check_launch("control_init");
+  if (st != PSA_OK) return st;
+  if (dt_mode != PSA_DT_CFL || const_speed) return PSA_OK;
+
+  const int64_t m = desc->n + 2 * static_cast<int64_t>(desc->g);
+  const void *src = advection ? desc->velocity : u;
+  if (desc->dtype == PSA_FLOAT64)
+    return launch_max_abs<double>(static_cast<const double *>(src), m,
+                                  desc->g, nullptr, &ctrl->umax_bits, s);
+  return launch_max_abs<float>(static_cast<const float *>(src), m, desc->g,
+                               nullptr, &ctrl->umax_bits, s);
+}
+
+int psa_ssprk33_steps(const psa_solver_desc *desc, const void *u_in,
+                      void *u_out, void *k1, void *k2, void *control,
+                      long long nsteps, void *stream) {
+  int st = check_desc(desc);
+  if (st != PSA_OK) return st;
+  if (u_in == nullptr || u_out == nullptr || k1 == nullptr || k2 == nullptr ||
+      control == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  if (k1 == k2 || k1 == u_in || k2 == u_in || k1 == u_out || k2 == u_out)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "stage buffers must be distinct from the state");
+  if (nsteps < 0) return set_error(PSA_ERR_INVALID_ARGUMENT, "nsteps < 0");
+  if (nsteps == 0) return PSA_OK;
+  auto s = static_cast<hipStream_t>(stream);
+  if (desc->dtype == PSA_FLOAT64)
+    return run_steps<double>(desc, u_in, u_out, k1, k2, control, nullptr,
+                             nsteps, s);
+  return run_steps<float>(desc, u_in, u_out, k1, k2, control, nullptr, nsteps,
+                          s);
+}
+
+int psa_ssprk33_stage(const psa_solver_desc *desc, int stage, const void *un,
+                      const void *uk, void *out, void *control,
+                      const void *dt, void *stream) {
+  int st = check_desc(desc);
+  if (st != PSA_OK) return st;
+  if (stage < 1 || stage > 3)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "stage must be 1, 2 or 3");
+  if (uk == nullptr || out == nullptr || (stage > 1 && un == nullptr))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null state pointer");
+  if (control == nullptr && dt == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "need a control block or dt");
+  if (out == uk)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "a stage cannot write into its stencil input");
+  auto s = static_cast<hipStream_t>(stream);
+  const TileConfig cfg = tiles_of(desc);
+  const int vel = vel_mode(desc);
+  if (desc->dtype == PSA_FLOAT64) {
+    StageArgs<double> a = make_args<double>(desc);
+    a.un = static_cast<const double *>(stage == 1 ? uk : un);
+    a.uk = static_cast<const double *>(uk);
+    a.out = static_cast<double *>(out);
+    a.ctrl = static_cast<psa_control *>(control);
+    a.dt_ptr = static_cast<const double *>(dt);
+    if (stage == 1) return launch_stage<double, 1>(desc->flux, vel, a, cfg, s);
+    if (stage == 2) return launch_stage<double, 2>(desc->flux, vel, a, cfg, s);
+    return launch_stage<double, 3>(desc->flux, vel, a, cfg, s);
+  }
+  StageArgs<float> a = make_args<float>(desc);
+  a.un = static_cast<const float *>(stage == 1 ? uk : un);
+  a.uk = static_cast<const float *>(uk);
+  a.out = static_cast<float *>(out);
+  a.ctrl = static_cast<psa_control *>(control);
+  a.dt_ptr = static_cast<const float *>(dt);
+  if (stage == 1) return launch_stage<float, 1>(desc->flux, vel, a, cfg, s);
+  if (stage == 2) return launch_stage<float, 2>(desc->flux, vel, a, cfg, s);
+  return launch_stage<float, 3>(desc->flux, vel, a, cfg, s);
+}
+
+int psa_ssprk33_advance(const psa_solver_desc *desc, const void *u,
+                        void *out, void *k1, void *k2, const void *dt,
+                        void *stream) {
+  int st = check_desc(desc);
+  if (st != PSA_OK) return st;
+  if (u == nullptr || out == nullptr || k1 == nullptr || k2 == nullptr ||
+      dt == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  if (k1 == k2 || k1 == u || k2 == u || k1 == out || k2 == out)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "stage buffers must be distinct from the state");
+  auto s = static_cast<hipStream_t>(stream);
+  if (desc->dtype == PSA_FLOAT64)
+    return run_steps<double>(desc, u, out, k1, k2, nullptr, dt, 1, s);
+  return run_steps<float>(desc, u, out, k1, k2, nullptr, dt, 1, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,217 @@
+// Device-side arithmetic of the WENO-JS5 / Rusanov / upwind hot path.
+//
+// Two forms of the reconstruction are provided:
+//
+// * ``weno_side_ref``: the reference's expression order for one side
+//   (weno.py:114-157,247-256 and reconstruction.py:351-355): beta from the
+//   (a, b) coefficients, alpha = d / (eps + beta)^2, omega = alpha / sum(alpha),
+//   sum(omega * uhat). Used by the standalone weno_* / reconstruct kernels.
+//
+// * ``weno_pair``: both sides of one cell at once for the fused kernels. The
+//   mirrored left reconstruction (reconstruction.py:375 reverses the array)
+//   uses the same three smoothness indicators as the right one with beta0 and
+//   beta2 swapped, so they are computed once. In FP64 the six weight
+//   divisions and the two normalisations are replaced by one division: with
+//   q_k = (eps + beta_k)^2, omega_k = (d_k prod_{j != k} q_j) / sum(...), and
+//   both sides share the pairwise products q_i q_j. A guard falls back to the
+//   reference form when the product leaves the normal range.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace psa {
+
+template <typename T>
+struct Weno5Coeffs {
+  // smoothness (weno.py:218-228): a = [13/12, 1/4]
+  static constexpr T a0 = T(13.0 / 12.0);
+  static constexpr T a1 = T(1.0 / 4.0);
+  // interpolation (weno.py:231-239)
+  static constexpr T c11 = T(11.0 / 6.0);
+  static constexpr T c7 = T(7.0 / 6.0);
+  static constexpr T c2 = T(2.0 / 6.0);
+  static constexpr T c5 = T(5.0 / 6.0);
+  static constexpr T c1 = T(1.0 / 6.0);
+  // ideal weights (weno.py:242)
+  static constexpr T d0 = T(1.0 / 10.0);
+  static constexpr T d1 = T(6.0 / 10.0);
+  static constexpr T d2 = T(3.0 / 10.0);
+};
+
+template <typename T>
+struct Limits;
+template <>
+struct Limits<double> {
+  static constexpr double min_normal = DBL_MIN;
+  static constexpr double max_finite = DBL_MAX;
+};
+template <>
+struct Limits<float> {
+  static constexpr float min_normal = FLT_MIN;
+  static constexpr float max_finite = FLT_MAX;
+};
+
+// Smoothness indicators of the right (left-biased) reconstruction at cell i
+// from v = (u[i-2], u[i-1], u[i], u[i+1], u[i+2]) (weno.py:114-140 with the
+// taps of weno.py:219-226; each row convolves as sum_k b[k] u[i+2-k]).
+template <typename T>
+__device__ __forceinline__ void weno_beta(T um2, T um1, T u0, T up1, T up2,
+                                          T &b0, T &b1, T &b2) {
+  using C = Weno5Coeffs<T>;
+  const T t00 = (u0 - T(2) * um1) + um2;
+  const T t01 = (T(3) * u0 - T(4) * um1) + um2;
+  const T t10 = (up1 - T(2) * u0) + um1;
+  const T t11 = up1 - um1;
+  const T t20 = (up2 - T(2) * up1) + u0;
+  const T t21 = (up2 - T(4) * up1) + T(3) * u0;
+  b0 = C::a0 * (t00 * t00) + C::a1 * (t01 * t01);
+  b1 = C::a0 * (t10 * t10) + C::a1 * (t11 * t11);
+  b2 = C::a0 * (t20 * t20) + C::a1 * (t21 * t21);
+}
+
+// Candidate interpolants of the right reconstruction (weno.py:143-157).
+template <typename T>
+__device__ __forceinline__ void weno_uhat(T um2, T um1, T u0, T up1, T up2,
+                                          T &h0, T &h1, T &h2) {
+  using C = Weno5Coeffs<T>;
+  h0 = (C::c11 * u0 - C::c7 * um1) + C::c2 * um2;
+  h1 = (C::c2 * up1 + C::c5 * u0) - C::c1 * um1;
+  h2 = (-C::c1 * up2 + C::c5 * up1) + C::c2 * u0;
+}
+
+// One side in the reference's expression order (weno.py:247-256,
+// reconstruction.py:351-355). For the right value at x_{i+1/2} pass
+// (u[i-2], ..., u[i+2]); for the left value at x_{i-1/2} pass the mirrored
+// stencil (u[i+2], ..., u[i-2]).
+template <typename T>
+__device__ __forceinline__ T weno_side_ref(T um2, T um1, T u0, T up1, T up2,
+                                           T eps) {
+  using C = Weno5Coeffs<T>;
+  T b0, b1, b2, h0, h1, h2;
+  weno_beta(um2, um1, u0, up1, up2, b0, b1, b2);
+  weno_uhat(um2, um1, u0, up1, up2, h0, h1, h2);
+  const T e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+  const T a0 = C::d0 / (e0 * e0);
+  const T a1 = C::d1 / (e1 * e1);
+  const T a2 = C::d2 / (e2 * e2);
+  const T s = (a0 + a1) + a2;
+  return ((a0 / s) * h0 + (a1 / s) * h1) + (a2 / s) * h2;
+}
+
+// Right and left reconstructions of one cell from shared indicators.
+// b0, b1, b2 are the right-side indicators of the cell.
+template <typename T>
+__device__ __forceinline__ void weno_pair_from_beta(T um2, T um1, T u0, T up1,
+                                                    T up2, T b0, T b1, T b2,
+                                                    T eps, T &ul, T &ur);
+
+template <>
+__device__ __forceinline__ void weno_pair_from_beta<double>(
+    double um2, double um1, double u0, double up1, double up2, double b0,
+    double b1, double b2, double eps, double &ul, double &ur) {
+  using C = Weno5Coeffs<double>;
+  double hr0, hr1, hr2, hl0, hl1, hl2;
+  weno_uhat(um2, um1, u0, up1, up2, hr0, hr1, hr2);
+  weno_uhat(up2, up1, u0, um1, um2, hl0, hl1, hl2);
+
+  const double e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+  const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
+  const double p01 = q0 * q1, p02 = q0 * q2, p12 = q1 * q2;
+
+  // right: alpha_k ~ d_k / q_k; left: d_0 / q_2, d_1 / q_1, d_2 / q_0
+  const double wr0 = C::d0 * p12, wr1 = C::d1 * p02, wr2 = C::d2 * p01;
+  const double wl0 = C::d0 * p01, wl2 = C::d2 * p12;
+  const double sr = (wr0 + wr1) + wr2;
+  const double sl = (wl0 + wr1) + wl2;
+  const double nr = (wr0 * hr0 + wr1 * hr1) + wr2 * hr2;
+  const double nl = (wl0 * hl0 + wr1 * hl1) + wl2 * hl2;
+
+  const double den = sr * sl;
+  if (den >= Limits<double>::min_normal && den <= Limits<double>::max_finite) {
+    const double inv = 1.0 / den;
+    ur = (nr * sl) * inv;
+    ul = (nl * sr) * inv;
+  } else {
+    // reference form: safe over the whole exponent range
+    const double a0 = C::d0 / q0, a1 = C::d1 / q1, a2 = C::d2 / q2;
+    const double s = (a0 + a1) + a2;
+    ur = ((a0 / s) * hr0 + (a1 / s) * hr1) + (a2 / s) * hr2;
+    const double l0 = C::d0 / q2, l2 = C::d2 / q0;
+    const double sll = (l0 + a1) + l2;
+    ul = ((l0 / sll) * hl0 + (a1 / sll) * hl1) + (l2 / sll) * hl2;
+  }
+}
+
+template <>
+__device__ __forceinline__ void weno_pair_from_beta<float>(
+    float um2, float um1, float u0, float up1, float up2, float b0, float b1,
+    float b2, float eps, float &ul, float &ur) {
+  using C = Weno5Coeffs<float>;
+  float hr0, hr1, hr2, hl0, hl1, hl2;
+  weno_uhat(um2, um1, u0, up1, up2, hr0, hr1, hr2);
+  weno_uhat(up2, up1, u0, um1, um2, hl0, hl1, hl2);
+
+  const float e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+  const float r0 = __builtin_amdgcn_rcpf(e0 * e0);
+  const float r1 = __builtin_amdgcn_rcpf(e1 * e1);
+  const float r2 = __builtin_amdgcn_rcpf(e2 * e2);
+  const float ar0 = C::d0 * r0, a1 = C::d1 * r1, ar2 = C::d2 * r2;
+  const float al0 = C::d0 * r2, al2 = C::d2 * r0;
+  const float sr = (ar0 + a1) + ar2;
+  const float sl = (al0 + a1) + al2;
+  ur = ((ar0 * hr0 + a1 * hr1) + ar2 * hr2) * __builtin_amdgcn_rcpf(sr);
+  ul = ((al0 * hl0 + a1 * hl1) + al2 * hl2) * __builtin_amdgcn_rcpf(sl);
+}
+
+template <typename T>
+__device__ __forceinline__ void weno_pair(T um2, T um1, T u0, T up1, T up2,
+                                          T eps, T &ul, T &ur) {
+  T b0, b1, b2;
+  weno_beta(um2, um1, u0, up1, up2, b0, b1, b2);
+  weno_pair_from_beta(um2, um1, u0, up1, up2, b0, b1, b2, eps, ul, ur);
+}
+
+// One-sided fast reconstruction (right value at x_{i+1/2}); the left value
+// is obtained by passing the mirrored stencil.
+template <typename T>
+__device__ __forceinline__ T weno_right(T um2, T um1, T u0, T up1, T up2,
+                                        T eps) {
+  T ul, ur;
+  weno_pair(um2, um1, u0, up1, up2, eps, ul, ur);
+  return ur;
+}
+
+// ---------------------------------------------------------------------------
+// Face fluxes. "l" is the cell left of the face (its right reconstruction
+// urL), "r" the cell right of it (its left reconstruction ulR).
+
+// scalar.py:238-248 with flux = u^2 / 2 (burgers/schemes.py:37-39):
+// F = 0.5 (fl + fr) - 0.5 a nu (ul - ur), a = max(|u_l|, |u_r|).
+template <typename T>
+__device__ __forceinline__ T flux_rusanov(T urL, T ulR, T uL, T uR, T nu) {
+  const T fl = (ulR * ulR) / T(2);
+  const T fr = (urL * urL) / T(2);
+  const T a = fmax(fabs(uR), fabs(uL));
+  return T(0.5) * (fl + fr) - ((T(0.5) * a) * nu) * (ulR - urL);
+}
+
+// scalar.py:125-130: aavg = (ar + al) / 2 with a = u reconstructed the same
+// way as u, F = fr if aavg > 0 else fl.
+template <typename T>
+__device__ __forceinline__ T flux_burgers_upwind(T urL, T ulR) {
+  const T aavg = (urL + ulR) / T(2);
+  return aavg > T(0) ? (urL * urL) / T(2) : (ulR * ulR) / T(2);
+}
+
+// advection/schemes.py:110-112: F = ur if (ar + al) / 2 > 0 else ul.
+template <typename T>
+__device__ __forceinline__ T flux_advection_upwind(T urL, T ulR, T arL,
+                                                   T alR) {
+  const T aavg = (arL + alR) / T(2);
+  return aavg > T(0) ? urL : ulR;
+}
+
+}  // namespace psa

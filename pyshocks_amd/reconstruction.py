@@ -1,0 +1,125 @@
+"""Reconstruction interface and WENO-JS5.
+
+Mirrors ``reconstruction.py`` of pyshocks: ``Reconstruction`` (``:49-76``),
+``reconstruct`` (``:80-118``), ``WENOJS`` (``:306-316``), ``WENOJS53``
+(``:336-348``), the WENO-JS registration (``:358-377``) and
+``make_reconstruction_from_name`` (``:528-564``).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, fields
+from functools import singledispatch
+from typing import TYPE_CHECKING, Any, ClassVar
+
+import torch
+
+from pyshocks_amd import _lib, weno
+
+if TYPE_CHECKING:
+    from pyshocks_amd.grid import Grid
+    from pyshocks_amd.schemes import BoundaryType
+
+
+@dataclass(frozen=True)
+class Reconstruction:
+    """A reconstruction algorithm."""
+
+    @property
+    def name(self) -> str:
+        return type(self).__name__.lower()
+
+    @property
+    def order(self) -> int:
+        raise NotImplementedError(type(self).__name__)
+
+    @property
+    def stencil_width(self) -> int:
+        raise NotImplementedError(type(self).__name__)
+
+
+@singledispatch
+def reconstruct(rec: Reconstruction, grid: Grid, bc: BoundaryType, f: torch.Tensor,
+                u: torch.Tensor, wavespeed: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """``(fl, fr)``: ``fr[i]`` at ``x_{i+1/2}``, ``fl[i]`` at ``x_{i-1/2}``."""
+    raise NotImplementedError(type(rec).__name__)
+
+
+@dataclass(frozen=True)
+class WENOJS(Reconstruction):
+    """A WENO-JS reconstruction."""
+
+    eps: float
+    s: ClassVar[weno.Stencil]
+
+    @property
+    def stencil_width(self) -> int:
+        return self.order
+
+
+@dataclass(frozen=True)
+class WENOJS53(WENOJS):
+    """Fifth-order WENO-JS (third order at discontinuities)."""
+
+    eps: float = 1.0e-12
+
+    def __post_init__(self) -> None:
+        object.__setattr__(self, "s", weno.weno_js_53_coefficients())
+
+    @property
+    def order(self) -> int:
+        return 3
+
+
+@reconstruct.register(WENOJS53)
+def _reconstruct_wenojs53(rec: WENOJS53, grid: Grid, bc: BoundaryType,
+                          f: torch.Tensor, u: torch.Tensor,
+                          wavespeed: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    from pyshocks_amd.grid import UniformGrid
+
+    assert grid.nghosts >= rec.stencil_width
+
+    if not isinstance(grid, UniformGrid):
+        raise NotImplementedError("WENO-JS is only implemented for uniform grids.")
+
+    _lib.require_device(f)
+    ul = torch.empty_like(f)
+    ur = torch.empty_like(f)
+    _lib.call("psa_weno_reconstruct", _lib.dtype_code(f.dtype), _lib.ptr(f),
+              _lib.ptr(ul), _lib.ptr(ur), f.numel(), float(rec.eps),
+              _lib.stream_of(f))
+    return ul, ur
+
+
+# {{{ registry
+
+_RECONSTRUCTION: dict[str, type[Reconstruction]] = {
+    "default": WENOJS53,
+    "wenojs53": WENOJS53,
+}
+
+# names the reference registers that are outside the GPU hot path
+_NOT_ON_GPU = ("constant", "muscl", "muscls", "wenojs32", "esweno32", "ssweno242")
+
+
+def reconstruction_ids() -> tuple[str, ...]:
+    return tuple(_RECONSTRUCTION.keys())
+
+
+def make_reconstruction_from_name(name: str, **kwargs: Any) -> Reconstruction:
+    """Unknown names raise ``ValueError`` (``reconstruction.py:553-560``);
+    fields not in the class are ignored (``:562-564``)."""
+    cls = _RECONSTRUCTION.get(name)
+    if cls is None:
+        if name in _NOT_ON_GPU:
+            raise NotImplementedError(
+                f"Reconstruction {name!r} is not part of the GPU hot path; "
+                f"available: {', '.join(reconstruction_ids())}.")
+        raise ValueError(
+            f"Flux limiter {name!r} not found. Try one of "
+            f"{', '.join(reconstruction_ids())}.")
+
+    return cls(**{f.name: kwargs[f.name] for f in fields(cls) if f.name in kwargs})
+
+
+# }}}

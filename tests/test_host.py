@@ -1,0 +1,228 @@
+"""CPU tests: the C ABI library loads and exports every declared symbol, the
+host-side mirror of the reference API behaves as the reference does, and the
+committed fixtures are reproduced by the oracle."""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import re
+
+import numpy as np
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+HEADERS = sorted((ROOT / "include").glob("*.h"))
+LIB = ROOT / "pyshocks_amd" / "libpyshocks_amd.so"
+
+
+def declared_functions() -> list[str]:
+    names: list[str] = []
+    for h in HEADERS:
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*int\s+(psa_\w+)\s*\(", text, flags=re.M)
+    return names
+
+
+def test_header_declares_entry_points() -> None:
+    names = declared_functions()
+    assert "psa_ssprk33_steps" in names
+    assert "psa_weno_reconstruct" in names
+    assert len(names) == len(set(names))
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def test_library_exports_every_declared_symbol() -> None:
+    lib = ctypes.CDLL(str(LIB))
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def test_binding_covers_header() -> None:
+    from pyshocks_amd import _lib
+
+    assert set(declared_functions()) <= set(_lib.SIGNATURES), \
+        set(declared_functions()) - set(_lib.SIGNATURES)
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def test_struct_layouts_and_errors() -> None:
+    from pyshocks_amd import _lib
+
+    lib = _lib.load()
+    assert lib.psa_control_bytes() == ctypes.sizeof(_lib.PsaControl)
+    # argument validation runs before any device work
+    assert lib.psa_weno_reconstruct(7, None, None, None, 10, 1e-6, None) == \
+        _lib.PSA_ERR_NOT_IMPLEMENTED
+    assert "dtype" in _lib.last_error()
+    with pytest.raises(ValueError):
+        _lib.check(lib.psa_apply_boundary_periodic(0, None, None, 4, 3, None))
+    desc = _lib.PsaSolverDesc()
+    desc.dtype = 0
+    desc.flux = 0
+    desc.n = 16
+    desc.g = 2
+    desc.dx = 1 / 16
+    with pytest.raises(ValueError, match="ghost"):
+        _lib.check(lib.psa_ssprk33_steps(ctypes.byref(desc), None, None, None, None,
+                                         None, 1, None))
+
+
+def test_product_refuses_cpu_tensors() -> None:
+    import torch
+
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.burgers import Rusanov
+    from pyshocks_amd.reconstruction import WENOJS53
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.schemes import apply_operator
+
+    grid = make_uniform_cell_grid(0.0, 1.0, 16, nghosts=3, device="cpu")
+    u = torch.zeros(22, dtype=torch.float64)
+    with pytest.raises((ValueError, RuntimeError)):
+        apply_operator(Rusanov(rec=WENOJS53(eps=1e-6)), grid, PeriodicBoundary(), 0.0, u)
+
+
+def test_registries_and_errors() -> None:
+    from pyshocks_amd import advection, burgers
+    from pyshocks_amd.reconstruction import WENOJS53, make_reconstruction_from_name
+
+    rec = make_reconstruction_from_name("wenojs53", eps=1e-6, unused=3)
+    assert isinstance(rec, WENOJS53) and rec.eps == 1e-6
+    assert WENOJS53().eps == 1e-12
+    assert rec.order == 3 and rec.stencil_width == 3
+    with pytest.raises(ValueError):
+        make_reconstruction_from_name("nope")
+    with pytest.raises(NotImplementedError):
+        make_reconstruction_from_name("muscl")
+
+    s = burgers.make_scheme_from_name("rusanov", rec=rec, alpha=0.9, other=1)
+    assert isinstance(s, burgers.Rusanov) and s.alpha == 0.9
+    assert s.name == "rusanov_wenojs53" and s.order == 3 and s.stencil_width == 3
+    with pytest.raises(ValueError):
+        burgers.make_scheme_from_name("nope")
+    a = advection.make_scheme_from_name("godunov", rec=rec)
+    assert a.velocity is None
+
+
+def test_checkpoint_semantics() -> None:
+    from pyshocks_amd.checkpointing import InMemoryCheckpoint, load, save
+
+    chk = InMemoryCheckpoint(basename="Iteration")
+    for i in range(5):
+        save(chk, i, {"i": i})
+    assert 3 in chk and 7 not in chk
+    assert load(chk, 3)["i"] == 3
+    assert load(chk, 3, include=("j",)) == {}
+    with pytest.raises(KeyError):
+        save(chk, 3, {"i": 3})
+    with pytest.raises(KeyError):
+        load(chk, 9)
+
+
+def test_fused_plan_selection() -> None:
+    from pyshocks_amd import apply_operator, make_uniform_cell_grid
+    from pyshocks_amd.burgers import Godunov, Rusanov
+    from pyshocks_amd.reconstruction import WENOJS53
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import (
+        SSPRK33,
+        FixedTimestep,
+        fused_plan,
+        make_ssprk33,
+    )
+
+    grid = make_uniform_cell_grid(0.0, 1.0, 64, nghosts=3, device="cpu")
+    bc = PeriodicBoundary()
+    st = make_ssprk33(Rusanov(rec=WENOJS53(eps=1e-6)), grid, bc, theta=1.0)
+    plan = fused_plan(st)
+    assert plan is not None
+    assert plan.cfl_num == 0.5 / 64 and plan.theta == 1.0
+    st = make_ssprk33(Godunov(rec=WENOJS53(eps=1e-6)), grid, bc, dt=1e-3)
+    plan = fused_plan(st)
+    assert plan is not None and plan.fixed_dt == 1e-3
+
+    opaque = SSPRK33(predict_timestep=FixedTimestep(1e-3),
+                     source=lambda t, u: apply_operator(st.source.scheme, grid, bc, t, u),
+                     checkpoint=None)
+    assert fused_plan(opaque) is None
+
+
+def test_generic_ssprk33_order_on_host() -> None:
+    """The generic path of ``step``/``advance`` (any callables) follows the
+    reference loop: SSP-RK3 order on u' = exp(-t)
+    (``tests/test_timestepping.py:26-99``)."""
+    import torch
+
+    from pyshocks_amd.timestepping import SSPRK33, predict_maxit_from_timestep, step
+
+    def source(t, u):
+        return torch.tensor([np.exp(-float(t))], dtype=torch.float64)
+
+    tfinal = 4.0
+    u0 = torch.tensor([-1.0], dtype=torch.float64)
+    hs, errs = [], []
+    for n in range(2, 7):
+        maxit, dt = predict_maxit_from_timestep(tfinal, 1.0 / 2.0**n)
+        st = SSPRK33(predict_timestep=lambda t, u, dt=dt: dt, source=source,
+                     checkpoint=None)
+        ev = None
+        for ev in step(st, u0, maxit=maxit):
+            pass
+        errs.append(abs(float(ev.u[0]) + np.exp(-tfinal)) / np.exp(-tfinal))
+        hs.append(dt)
+    from pyshocks_amd.tools import estimate_order_of_convergence
+
+    assert estimate_order_of_convergence(np.array(hs), np.array(errs))[1] >= 2.9
+
+
+def test_golden_fixtures_reproduced_by_oracle() -> None:
+    """Regression of the committed fixtures against the oracle."""
+    import subprocess
+    import sys
+    import tempfile
+
+    golden = ROOT / "tests" / "golden"
+    with tempfile.TemporaryDirectory() as tmp:
+        script = (golden / "make_golden.py").read_text()
+        (pathlib.Path(tmp) / "make_golden.py").write_text(
+            script.replace("HERE = pathlib.Path(__file__).resolve().parent",
+                           f"HERE = pathlib.Path({tmp!r})").replace(
+                "sys.path.insert(0, str(HERE.parents[1]))",
+                f"sys.path.insert(0, {str(ROOT)!r})"))
+        subprocess.run([sys.executable, str(pathlib.Path(tmp) / "make_golden.py")],
+                       check=True, capture_output=True)
+        for ref in golden.glob("*.npz"):
+            new = np.load(pathlib.Path(tmp) / ref.name)
+            old = np.load(ref)
+            assert set(new.files) == set(old.files)
+            for k in old.files:
+                np.testing.assert_allclose(new[k], old[k], rtol=1e-14, atol=1e-300)
+
+
+def test_c_oracle_matches_numpy_oracle() -> None:
+    from oracle import build as oracle_build
+    from oracle import exact as E
+    from oracle import ref_c
+    from oracle import weno_np as W
+
+    if not ref_c.LIB.exists():
+        oracle_build.build(verbose=False)
+    for kind in ("rusanov", "godunov", "advection"):
+        n = 200
+        grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+        u0 = E.exact_cell_averages(E.large_initial_condition(seed=2), n)
+        vel = np.ones(n + 6)
+        theta = 0.5 if kind == "advection" else 1.0
+        u, dts, t = W.solve(W.Scheme(kind, eps=1e-6, velocity=vel), grid, u0,
+                            theta=theta, tfinal=0.05)
+        cfl = 1.0 / n if kind == "advection" else 0.5 / n
+        uc, tc, it, dtc = ref_c.solve(kind, u0, n=n, g=3, dx=1.0 / n, eps=1e-6,
+                                      theta=theta, cfl_num=cfl, tfinal=0.05,
+                                      velocity=vel)
+        assert it == len(dts)
+        np.testing.assert_allclose(dtc, dts, rtol=1e-14, atol=1e-16)
+        i = grid.i_
+        assert np.linalg.norm(uc[i] - u[i]) / np.linalg.norm(u[i]) < 1e-13

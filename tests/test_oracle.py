@@ -1,0 +1,375 @@
+"""Pins of the CPU oracle (``oracle/weno_np.py``) that do not depend on it.
+
+Each test checks the restatement against something a plausible mistake
+(a dropped term, a flipped tap order, a sign, a wrong index) would violate:
+closed-form identities of the coefficients, explicit loops written as in the
+reference's own tests, exactness on polynomials, convergence orders, exact
+solutions (a shift for advection, Hopf-Lax for Burgers) and invariants.
+"""
+
+from __future__ import annotations
+
+from itertools import product
+
+import numpy as np
+import pytest
+
+from oracle import exact as E
+from oracle import weno_np as W
+
+S = W.WENOJS53
+
+
+# {{{ coefficients
+
+
+def test_coefficient_identities() -> None:
+    # candidate interpolants reproduce constants
+    np.testing.assert_allclose(S.c.sum(axis=1), 1.0, rtol=0, atol=1e-15)
+    # ideal weights are a partition of unity
+    assert abs(S.d.sum() - 1.0) < 1e-15
+    # sum_k d_k c_k is the 5th-order upwind-biased interpolant at x_{i+1/2},
+    # taps (i-2 .. i+2) = [2, -13, 47, 27, -3] / 60, i.e. reversed in the
+    # (i+2 .. i-2) ordering of the table
+    combined = (S.d * S.c).sum(axis=0)
+    expected = np.array([-3.0, 27.0, 47.0, -13.0, 2.0]) / 60.0
+    np.testing.assert_allclose(combined, expected, rtol=0, atol=1e-15)
+
+    # smoothness indicators vanish on constants and equal s^2 on u_j = s j
+    u = np.full(16, 3.25)
+    beta = W.weno_smoothness(S, u)
+    np.testing.assert_allclose(beta[:, 2:-2], 0.0, atol=1e-14)
+    s = 0.7
+    u = s * np.arange(16, dtype=np.float64)
+    beta = W.weno_smoothness(S, u)
+    np.testing.assert_allclose(beta[:, 2:-2], s**2, rtol=1e-13)
+
+
+def test_flipped_taps_fail_identity() -> None:
+    """A transposed tap order must not satisfy the identity above."""
+    flipped = (S.d * S.c[:, ::-1]).sum(axis=0)
+    expected = np.array([-3.0, 27.0, 47.0, -13.0, 2.0]) / 60.0
+    assert np.max(np.abs(flipped - expected)) > 0.1
+
+
+# }}}
+
+
+# {{{ loops vs vectorised (tests/test_weno.py:34-108)
+
+
+def test_smoothness_and_interp_loops() -> None:
+    rtol = 2.0e-15
+    n = 64
+    a, b, c = S.a, S.b, S.c
+    nghosts = b.shape[-1] // 2
+    stencil = np.arange(-nghosts, nghosts + 1)
+    n = n + 2 * nghosts
+    m = np.s_[nghosts:-nghosts]
+
+    u = np.sin(np.linspace(0.0, 2.0 * np.pi, n))
+
+    beta0 = np.zeros((3, n))
+    for j in range(*m.indices(n)):
+        for i, k in product(range(3), range(a.size)):
+            beta0[i, j] += a[k] * np.sum(u[j + stencil] * b[i, k, ::-1]) ** 2
+    beta1 = W.weno_smoothness(S, u)
+
+    uhat0 = np.zeros((3, n))
+    for j in range(*m.indices(n)):
+        for i in range(3):
+            uhat0[i, j] = np.sum(u[j + stencil] * c[i, ::-1])
+    uhat1 = W.weno_interp(S, u)
+
+    err = np.linalg.norm(beta0[:, m] - beta1[:, m]) / np.linalg.norm(beta0[:, m])
+    assert err < rtol
+    err = np.linalg.norm(uhat0[:, m] - uhat1[:, m]) / np.linalg.norm(uhat0[:, m])
+    assert err < rtol
+
+
+# }}}
+
+
+# {{{ polynomial exactness
+
+
+def _poly_cell_averages(coeffs: np.ndarray, f: np.ndarray) -> np.ndarray:
+    P = np.polynomial.polynomial.Polynomial(coeffs).integ()
+    return (P(f[1:]) - P(f[:-1])) / (f[1:] - f[:-1])
+
+
+@pytest.mark.parametrize("degree", [0, 1, 2])
+def test_candidates_exact_on_quadratics(degree: int) -> None:
+    rng = np.random.default_rng(degree)
+    coeffs = rng.uniform(-1, 1, size=degree + 1)
+    f = np.linspace(0.0, 2.0, 21)
+    u = _poly_cell_averages(coeffs, f)
+    p = np.polynomial.polynomial.Polynomial(coeffs)
+
+    uhat = W.weno_interp(S, u)
+    for k in range(3):
+        np.testing.assert_allclose(uhat[k, 2:-2], p(f[1:])[2:-2], rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("degree", [3, 4])
+def test_combined_exact_on_quartics(degree: int) -> None:
+    rng = np.random.default_rng(10 + degree)
+    coeffs = rng.uniform(-1, 1, size=degree + 1)
+    f = np.linspace(0.0, 2.0, 21)
+    u = _poly_cell_averages(coeffs, f)
+    p = np.polynomial.polynomial.Polynomial(coeffs)
+
+    uhat = W.weno_interp(S, u)
+    right = np.sum(S.d * uhat, axis=0)
+    np.testing.assert_allclose(right[2:-2], p(f[1:])[2:-2], rtol=1e-12, atol=1e-13)
+
+    # the left states are the mirrored reconstruction at x_{i-1/2}
+    uhat_l = W.weno_interp(S, u[::-1])
+    left = np.sum(S.d * uhat_l, axis=0)[::-1]
+    np.testing.assert_allclose(left[2:-2], p(f[:-1])[2:-2], rtol=1e-12, atol=1e-13)
+
+    # a single candidate is only third order: it must not be exact here
+    assert np.max(np.abs(uhat[0, 2:-2] - p(f[1:])[2:-2])) > 1e-8
+
+
+# }}}
+
+
+# {{{ weights (tests/test_weno.py:117-166)
+
+
+@pytest.mark.parametrize("is_smooth", [True, False])
+def test_weights_smooth_and_step(is_smooth: bool) -> None:
+    n = 128 + 6
+    m = np.s_[3:-3]
+    theta = np.linspace(0.0, 2.0 * np.pi, n)
+    u = np.sin(theta) if is_smooth else (theta < np.pi).astype(np.float64)
+    omega = W.weno_js_weights(S, u, eps=1.0e-12)
+    error = np.max(np.abs(omega[:, m] - S.d))
+    if is_smooth:
+        assert error < 0.1
+    else:
+        assert error > 0.1
+
+
+# }}}
+
+
+# {{{ reconstruction order (tests/test_weno.py:288-338)
+
+
+def test_reconstruction_order() -> None:
+    hs, el, er = [], [], []
+    for n in range(32, 256 + 1, 32):
+        grid = W.make_uniform_cell_grid(-1.0, 1.0, n, nghosts=3)
+        u0 = W.leggauss_cell_average(grid, lambda x: np.sin(2 * np.pi * x), order=6)
+        ul, ur = W.reconstruct_wenojs(u0, 1.0e-12)
+        fx = np.sin(2 * np.pi * grid.f)
+        el.append(W.rnorm(grid, ul, fx[:-1], p=np.inf))
+        er.append(W.rnorm(grid, ur, fx[1:], p=np.inf))
+        hs.append(grid.dx_max)
+
+    assert W.estimate_order_of_convergence(np.array(hs), np.array(el)) >= 4.5
+    assert W.estimate_order_of_convergence(np.array(hs), np.array(er)) >= 4.5
+
+
+# }}}
+
+
+# {{{ operator invariants
+
+
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "advection"])
+def test_operator_conservation_and_constants(kind: str) -> None:
+    n = 64
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    rng = np.random.default_rng(1)
+    vel = np.ones(grid.n)
+    sch = W.Scheme(kind, eps=1e-6, velocity=vel)
+
+    u = rng.uniform(-1, 1, size=grid.n)
+    L = W.apply_operator(sch, grid, 0.0, u)
+    # periodic telescoping: sum_int L dx = 0
+    assert abs(np.sum(L[grid.i_]) / n) < 1e-14
+
+    u = np.full(grid.n, 0.75)
+    L = W.apply_operator(sch, grid, 0.0, u)
+    np.testing.assert_allclose(L[grid.i_], 0.0, atol=1e-12)
+
+
+def test_rusanov_closed_form_small() -> None:
+    """Brute-force flux on a tiny grid with hand-evaluated formulas."""
+    n = 8
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    rng = np.random.default_rng(7)
+    u = W.apply_boundary_periodic(grid, rng.uniform(-1, 1, size=grid.n))
+    eps = 1e-6
+
+    def side(v: np.ndarray) -> float:
+        # v = (u_{i-2}, ..., u_{i+2}), Jiang-Shu (2.17), (2.11)-(2.13), (2.15)
+        b0 = 13 / 12 * (v[0] - 2 * v[1] + v[2]) ** 2 + 1 / 4 * (v[0] - 4 * v[1] + 3 * v[2]) ** 2
+        b1 = 13 / 12 * (v[1] - 2 * v[2] + v[3]) ** 2 + 1 / 4 * (v[1] - v[3]) ** 2
+        b2 = 13 / 12 * (v[2] - 2 * v[3] + v[4]) ** 2 + 1 / 4 * (3 * v[2] - 4 * v[3] + v[4]) ** 2
+        q0 = (2 * v[0] - 7 * v[1] + 11 * v[2]) / 6
+        q1 = (-v[1] + 5 * v[2] + 2 * v[3]) / 6
+        q2 = (2 * v[2] + 5 * v[3] - v[4]) / 6
+        a = np.array([0.1 / (eps + b0) ** 2, 0.6 / (eps + b1) ** 2, 0.3 / (eps + b2) ** 2])
+        w = a / a.sum()
+        return float(w @ np.array([q0, q1, q2]))
+
+    F = W.scalar_flux_rusanov(grid, u, u, eps=eps)
+    for j in range(3, grid.n - 2):  # faces with full stencils
+        ur = side(u[j - 3: j + 2])
+        ul = side(u[j - 2: j + 3][::-1])
+        s = max(abs(u[j - 1]), abs(u[j]))
+        ref = 0.5 * (ul**2 / 2 + ur**2 / 2) - 0.5 * s * (ul - ur)
+        assert abs(F[j] - ref) < 1e-13
+
+
+# }}}
+
+
+# {{{ time stepping
+
+
+def test_ssprk33_order() -> None:
+    """``tests/test_timestepping.py:26-99`` for SSPRK33: u' = exp(-t)."""
+
+    def source(t, u):
+        return np.array([np.exp(-t)])
+
+    tfinal = 4.0
+    u0 = np.array([-1.0])
+    u_ex = -np.exp(-tfinal)
+    hs, errs = [], []
+    for n in range(2, 7):
+        maxit, dt = W.predict_maxit_from_timestep(tfinal, 1.0 / 2.0**n)
+        event = None
+        for event in W.step(lambda t, u, dt=dt: dt, source, u0, maxit=maxit):
+            pass
+        errs.append(abs(event.u[0] - u_ex) / abs(u_ex))
+        hs.append(dt)
+    assert W.estimate_order_of_convergence(np.array(hs), np.array(errs)) >= 2.9
+
+
+def test_step_dt_rule() -> None:
+    """Every step adds 1e-15 after clipping to tfinal - t."""
+    n = 64
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    u0 = E.exact_cell_averages(E.c1_initial_condition(), n)
+    sch = W.Scheme("rusanov", eps=1e-6)
+    u, dts, t = W.solve(sch, grid, u0, theta=1.0, tfinal=0.1)
+    assert t >= 0.1
+    assert abs(t - 0.1) < 1e-14
+    # all but the last step are CFL steps of 0.5 dx / max|u|, plus 1e-15
+    assert all(d > 1e-15 for d in dts)
+    assert dts[-1] <= dts[0] + 1e-15
+
+
+# }}}
+
+
+# {{{ end-to-end against exact solutions
+
+
+def test_advection_order_exact_shift() -> None:
+    """``tests/test_convergence.py:395-460`` (godunov + wenojs53), periodic on
+    [-1, 1] to t = 1 with dt = 8 (2/n)^(5/3)."""
+    a, b, tfinal = -1.0, 1.0, 1.0
+
+    def ic(grid: W.Grid, x: np.ndarray) -> np.ndarray:
+        xm = (grid.b + grid.a) / 2
+        xx = (x - xm) / (grid.b - grid.a)
+        return np.sin(2 * np.pi * xx + np.sin(2 * np.pi * xx) / np.pi)
+
+    hs, errs = [], []
+    for n in range(32, 256 + 1, 32):
+        grid = W.make_uniform_cell_grid(a, b, n, nghosts=3)
+        vel = np.ones(grid.n)
+        sch = W.Scheme("advection", eps=1.0e-12, velocity=vel)
+        u0 = W.leggauss_cell_average(grid, lambda x: ic(grid, x), order=5)
+        uhat = W.leggauss_cell_average(grid, lambda x: ic(grid, x - tfinal), order=5)
+        maxit, dt = W.predict_maxit_from_timestep(tfinal, 8.0 * ((b - a) / n) ** (5.0 / 3.0))
+        u, _, _ = W.solve(sch, grid, u0, theta=1.0, maxit=maxit, fixed_dt=dt)
+        errs.append(W.norm(grid, u - uhat, p=1, weighted=True)
+                    / W.norm(grid, uhat, p=1, weighted=True))
+        hs.append(float(np.max(np.diff(grid.f))))
+    assert W.estimate_order_of_convergence(np.array(hs), np.array(errs)) >= 4.5
+
+
+def test_hopf_lax_exact_is_exact_before_shock() -> None:
+    """The Hopf-Lax primitive reproduces the characteristic solution for a
+    smooth state (checks the exact-solution helper itself)."""
+    ic = E.c1_initial_condition()
+    t = 0.1  # before the shock time 1 / (2 pi)
+    x = np.linspace(0.0, 1.0, 33)
+    w = E.hopf_lax_primitive(ic, x, t)
+    # characteristic foot y: x = y + t u0(y); w = U0(y) + t u0(y)^2 / 2
+    y = x.copy()
+    for _ in range(60):
+        r = y + t * ic(y) - x
+        y = y - r / (1 + t * 2 * np.pi * np.cos(2 * np.pi * y))
+    wc = ic.antiderivative(y) + t * ic(y) ** 2 / 2
+    np.testing.assert_allclose(w, wc, rtol=0, atol=1e-13)
+
+
+def test_burgers_smooth_order_hopf_lax() -> None:
+    ic = E.c1_initial_condition()
+    hs, errs = [], []
+    for n in [64, 128, 256]:
+        grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+        u0 = E.exact_cell_averages(ic, n)
+        sch = W.Scheme("rusanov", eps=1e-6)
+        u, _, t = W.solve(sch, grid, u0, theta=1.0, tfinal=0.1)
+        ue = E.burgers_exact(ic, n, t)
+        errs.append(W.norm(grid, u - ue, p=1, weighted=True))
+        hs.append(1.0 / n)
+    # time error O(dt^3) with dt ~ h dominates
+    assert W.estimate_order_of_convergence(np.array(hs), np.array(errs)) >= 2.8
+
+
+@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+def test_burgers_shock_hopf_lax(kind: str) -> None:
+    """Past the shock (t = 0.5), against exact Hopf-Lax cell averages."""
+    ic = E.c1_initial_condition()
+    errs = []
+    for n in [64, 128, 256]:
+        grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+        u0 = E.exact_cell_averages(ic, n)
+        sch = W.Scheme(kind, eps=1e-6)
+        u, _, t = W.solve(sch, grid, u0, theta=1.0, tfinal=0.5)
+        ue = E.burgers_exact(ic, n, t)
+        errs.append(W.rnorm(grid, u, ue, p=1, weighted=True))
+    assert errs[-1] < 5.0e-3
+    assert errs[0] / errs[-1] > 3.0  # first order through the shock
+
+
+def test_burgers_flux_sign_error_is_caught() -> None:
+    """A planted sign error in the dissipation term must fail the Hopf-Lax
+    shock check above (before the shock the term is O(h^5) and nearly
+    invisible, so the plant is checked at t = 0.5)."""
+    ic = E.c1_initial_condition()
+    n = 128
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    u0 = E.exact_cell_averages(ic, n)
+
+    def bad_flux(grid, u, a, *, eps, alpha=1.0, flux=W.burgers_flux):
+        ul, ur = W.reconstruct_wenojs(u, eps)
+        s = np.maximum(np.abs(a[1:]), np.abs(a[:-1]))
+        f = 0.5 * (flux(ul)[1:] + flux(ur)[:-1]) + 0.5 * s * (ul[1:] - ur[:-1])
+        return np.pad(f, 1)
+
+    orig = W.scalar_flux_rusanov
+    try:
+        W.scalar_flux_rusanov = bad_flux
+        with np.errstate(all="ignore"):
+            u, _, t = W.solve(W.Scheme("rusanov", eps=1e-6), grid, u0, theta=1.0,
+                              tfinal=0.5)
+    finally:
+        W.scalar_flux_rusanov = orig
+    ue = E.burgers_exact(ic, n, t)
+    err = W.rnorm(grid, u, ue, p=1, weighted=True)
+    assert not (err < 5.0e-3)
+
+
+# }}}
