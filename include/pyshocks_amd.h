@@ -130,6 +130,9 @@ int psa_interior_norm(int dtype, int p, const void *u, int64_t lo, int64_t hi,
  * max|u_int| of the current state, clipped to tfinal - t, plus 1e-15),
  * stage 2 advances t and m, stage 3 reduces max|u_int| of the new state.
  * A non-finite dt stops the run with status PSA_ERR_NONFINITE_DT. */
+#define PSA_UMAX_SLOTS 64      /* max|u| partials, one 128-byte line each */
+#define PSA_UMAX_SLOT_STRIDE 16 /* in unsigned long long                    */
+
 typedef struct psa_control {
   double t;                     /* time of the current state               */
   double dt;                    /* dt of the last step taken               */
@@ -139,12 +142,17 @@ typedef struct psa_control {
   double fixed_dt;              /* dt for PSA_DT_FIXED                     */
   long long m;                  /* steps taken                             */
   long long maxit;              /* step limit, < 0 for none                */
-  unsigned long long umax_bits; /* max|u_int| (bit pattern of the dtype)   */
+  unsigned long long umax_bits; /* max|u_int| of the state the last step
+                                   started from (bit pattern of the dtype) */
   int active;                   /* 1 while the step in flight is taken     */
   int status;                   /* PSA_OK or PSA_ERR_NONFINITE_DT          */
   int dt_mode;                  /* PSA_DT_CFL or PSA_DT_FIXED              */
   int speed_from_state;         /* 1: max|u| of each state (Burgers)       */
   int pad_[2];
+  /* stage 3 reduces max|u_int| of the new state into these slots (a
+   * workgroup uses slot blockIdx % PSA_UMAX_SLOTS); stage 1 of the next
+   * step combines them and stage 2 clears them. */
+  unsigned long long umax_slot[PSA_UMAX_SLOTS][PSA_UMAX_SLOT_STRIDE];
 } psa_control;
 
 typedef struct psa_solver_desc {
@@ -160,7 +168,9 @@ typedef struct psa_solver_desc {
   const void *velocity;         /* advection velocity (m) if not const     */
   const void *nu;               /* Rusanov df^(alpha-1) (m - 1) or NULL    */
   int block;                    /* threads per workgroup (0 = tuned)       */
-  int cells_per_thread;         /* cells per thread (0 = tuned)            */
+  int cells_per_thread;         /* cells per thread (0 = tuned); negative:
+                                   persistent workgroups that prefetch
+                                   their next tile                         */
 } psa_solver_desc;
 
 /* Initialise the control block (device memory, sizeof(psa_control)) for a

@@ -58,6 +58,7 @@ class PsaControl(ctypes.Structure):
         ("dt_mode", ctypes.c_int),
         ("speed_from_state", ctypes.c_int),
         ("pad_", ctypes.c_int * 2),
+        ("umax_slot", (ctypes.c_ulonglong * 16) * 64),
     ]
 
 

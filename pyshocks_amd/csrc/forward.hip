@@ -260,6 +260,8 @@ __global__ void k_control_init(psa_control *ctrl, double t, double tfinal,
   ctrl->m = 0;
   ctrl->maxit = maxit;
   ctrl->umax_bits = abs_bits(static_cast<T>(speed_const));
+  for (int k = 0; k < PSA_UMAX_SLOTS; ++k) ctrl->umax_slot[k][0] = 0ull;
+  ctrl->umax_slot[0][0] = ctrl->umax_bits;
   ctrl->active = 0;
   ctrl->status = PSA_OK;
   ctrl->dt_mode = dt_mode;
@@ -267,9 +269,24 @@ __global__ void k_control_init(psa_control *ctrl, double t, double tfinal,
 }
 
 // Decision of stage 1, evaluated identically by every workgroup.
+// max of the PSA_UMAX_SLOTS partials: lane k of every wave reads slot k and
+// the wave reduces them, so each wave holds the result without a barrier.
+__device__ __forceinline__ unsigned long long gather_umax(
+    const psa_control *ctrl) {
+  static_assert(PSA_UMAX_SLOTS == 64, "one slot per lane");
+  const int lane = threadIdx.x & 63;
+  unsigned long long v = ctrl->umax_slot[lane][0];
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
 template <typename T>
-__device__ __forceinline__ bool control_decide(const psa_control *ctrl, T &dt,
-                                               int &status) {
+__device__ __forceinline__ bool control_decide(const psa_control *ctrl,
+                                               unsigned long long umax_bits,
+                                               T &dt, int &status) {
   status = ctrl->status;
   if (status != PSA_OK) return false;
   const T t = static_cast<T>(ctrl->t);
@@ -280,7 +297,7 @@ __device__ __forceinline__ bool control_decide(const psa_control *ctrl, T &dt,
   if (ctrl->dt_mode == PSA_DT_FIXED) {
     dt = static_cast<T>(ctrl->fixed_dt);
   } else {
-    const T smax = from_bits<T>(ctrl->umax_bits);
+    const T smax = from_bits<T>(umax_bits);
     dt = static_cast<T>(ctrl->theta) * (static_cast<T>(ctrl->cfl_num) / smax);
   }
   if (!isinf(tfinal)) {
@@ -318,6 +335,7 @@ struct StageArgs {
   int64_t m;
   int64_t n;
   int g;
+  int aligned;      // uk, un, out, vel are 16-byte aligned
   T dx;
   T inv_dx;
   T eps;
@@ -333,18 +351,88 @@ __device__ __forceinline__ T velocity_at(const T *s_v, int j, T vconst) {
   }
 }
 
+// 16-byte vectors of the dtype
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int n = 2;
+};
+template <>
+struct Vec16<float> {
+  using type = float4;
+  static constexpr int n = 4;
+};
+
+template <typename T>
+__device__ __forceinline__ typename Vec16<T>::type load16(const T *p) {
+  return *reinterpret_cast<const typename Vec16<T>::type *>(p);
+}
+template <typename T>
+__device__ __forceinline__ void store16(T *p, typename Vec16<T>::type v) {
+  *reinterpret_cast<typename Vec16<T>::type *>(p) = v;
+}
+__device__ __forceinline__ unsigned long long vmax_bits(double2 v,
+                                                        unsigned long long m) {
+  const unsigned long long a = abs_bits(v.x), b = abs_bits(v.y);
+  m = a > m ? a : m;
+  return b > m ? b : m;
+}
+__device__ __forceinline__ unsigned long long vmax_bits(float4 v,
+                                                        unsigned long long m) {
+  const unsigned long long a = abs_bits(v.x), b = abs_bits(v.y);
+  const unsigned long long c = abs_bits(v.z), d = abs_bits(v.w);
+  m = a > m ? a : m;
+  m = b > m ? b : m;
+  m = c > m ? c : m;
+  return d > m ? d : m;
+}
+
+// COUNT 16-byte chunks moved between global memory and LDS by BLOCK
+// threads: every load of a thread is issued before its first LDS write, so a
+// wave keeps all of its loads in flight at once.
+template <typename T, int BLOCK, int COUNT>
+struct Chunks {
+  using V = typename Vec16<T>::type;
+  static constexpr int VN = Vec16<T>::n;
+  static constexpr int CH = (COUNT + BLOCK - 1) / BLOCK;
+  V r[CH];
+
+  __device__ __forceinline__ void load(const T *src, int tid) {
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int c = tid + k * BLOCK;
+      if (k < CH - 1 || COUNT % BLOCK == 0 || c < COUNT) r[k] = load16(src + c * VN);
+    }
+  }
+  __device__ __forceinline__ void store(T *dst, int tid) const {
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int c = tid + k * BLOCK;
+      if (k < CH - 1 || COUNT % BLOCK == 0 || c < COUNT) store16(dst + c * VN, r[k]);
+    }
+  }
+};
+
 // VEL: 0 Burgers (no velocity), 1 constant velocity, 2 velocity array.
 // STAGE: 0 operator (all m cells, faces 0 and m padded with zero flux),
 //        1..3 SSP-RK3 stages (interior cells, ghost images rewritten).
-template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT>
+//
+// The grid may be smaller than the number of tiles: each workgroup then
+// walks the tiles blockIdx.x, blockIdx.x + gridDim.x, ... and loads the
+// chunks of its next interior tile into registers before it computes the
+// current one, so the next tile's HBM reads overlap the WENO arithmetic.
+template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT, bool PF>
 __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   constexpr int TILE = BLOCK * CPT;
   constexpr int HALO = 4;
+  constexpr int VN = Vec16<T>::n;
   constexpr int NV = VEL == 2 ? TILE + 2 * HALO : 1;
   constexpr int NVX = VEL == 2 ? BLOCK + 1 : 1;
-  __shared__ T s_u[TILE + 2 * HALO];
-  __shared__ T s_v[NV];
-  __shared__ T s_io[TILE];
+  __shared__ __align__(16) T s_u[TILE + 2 * HALO];
+  __shared__ __align__(16) T s_v[NV];
+  __shared__ __align__(16) T s_io[TILE];
   __shared__ T s_urL[BLOCK + 1];
   __shared__ T s_ulF[BLOCK + 1];
   __shared__ T s_arL[NVX];
@@ -352,21 +440,23 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   __shared__ unsigned long long s_red[BLOCK / 64];
 
   const int tid = threadIdx.x;
-  const int64_t S = static_cast<int64_t>(blockIdx.x) * TILE;
   const int64_t m = p.m;
   const int64_t n = p.n;
   const int g = p.g;
+  const int64_t ntiles = (m + TILE - 1) / TILE;
 
-  // ---- step control
+  // ---- step control (once per workgroup)
   T dt = T(0);
   if constexpr (STAGE > 0) {
     if (p.ctrl != nullptr) {
       if constexpr (STAGE == 1) {
         int status;
-        const bool active = control_decide<T>(p.ctrl, dt, status);
+        const unsigned long long umax = gather_umax(p.ctrl);
+        const bool active = control_decide<T>(p.ctrl, umax, dt, status);
         if (blockIdx.x == 0 && tid == 0) {
           p.ctrl->active = active ? 1 : 0;
           p.ctrl->status = status;
+          p.ctrl->umax_bits = umax;
           if (active) p.ctrl->dt = static_cast<double>(dt);
         }
         if (!active) return;
@@ -377,7 +467,11 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
           if (blockIdx.x == 0 && tid == 0) {
             p.ctrl->t = static_cast<double>(static_cast<T>(p.ctrl->t) + dt);
             p.ctrl->m = p.ctrl->m + 1;
-            if (p.ctrl->speed_from_state) p.ctrl->umax_bits = 0ull;
+          }
+          // stage 1 has read the slots; clear them for stage 3
+          if (p.ctrl->speed_from_state && blockIdx.x == 0 &&
+              tid < PSA_UMAX_SLOTS) {
+            p.ctrl->umax_slot[tid][0] = 0ull;
           }
         }
       }
@@ -386,180 +480,249 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
     }
   }
 
-  // ---- stage the tile and its halo
-  for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {
-    s_u[j] = load_filled(p.uk, S - HALO + j, m, n, g);
-  }
-  if constexpr (VEL == 2) {
-    for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {
-      s_v[j] = load_raw(p.vel, S - HALO + j, m);
-    }
-  }
-  if constexpr (STAGE >= 2) {
-    for (int j = tid; j < TILE; j += BLOCK) {
-      const int64_t gm = S + j;
-      s_io[j] = gm < m ? p.un[gm] : T(0);
-    }
-  }
-  __syncthreads();
+  // A tile whose cells and halo are interior needs neither the ghost
+  // mapping nor bounds checks, and (STAGE > 0) writes no ghost images.
+  auto is_fast = [&](int64_t tile) {
+    const int64_t S = tile * TILE;
+    return p.aligned != 0 && STAGE > 0 && S - HALO >= g &&
+           S + TILE + HALO <= m - g;
+  };
 
-  // ---- reconstruct CPT consecutive cells
+  Chunks<T, BLOCK, (TILE + 2 * HALO) / VN> cu;
+  Chunks<T, BLOCK, VEL == 2 ? (TILE + 2 * HALO) / VN : 1> cv;
+  Chunks<T, BLOCK, STAGE >= 2 ? TILE / VN : 1> cn;
+  auto prefetch = [&](int64_t tile) {
+    const int64_t S = tile * TILE;
+    cu.load(p.uk + S - HALO, tid);
+    if constexpr (VEL == 2) cv.load(p.vel + S - HALO, tid);
+    if constexpr (STAGE >= 2) cn.load(p.un + S, tid);
+  };
+
+  int64_t tile = blockIdx.x;
+  if constexpr (PF) {
+    if (tile < ntiles && is_fast(tile)) prefetch(tile);
+  }
+
+  unsigned long long vmax = 0;
   using C = Weno5Coeffs<T>;
-  const int l0 = tid * CPT;
-  T w[CPT + 4];  // cells l0 - 2 .. l0 + CPT + 1
-#pragma unroll
-  for (int k = 0; k < CPT + 4; ++k) w[k] = s_u[l0 + 2 + k];
+  for (; tile < ntiles; tile += gridDim.x) {
+    const int64_t S = tile * TILE;
+    const bool fast = is_fast(tile);
 
-  // a0 * (second difference)^2 at cells l0 - 1 .. l0 + CPT, shared by the
-  // three indicators of neighbouring cells
-  T E[CPT + 2];
-#pragma unroll
-  for (int k = 0; k < CPT + 2; ++k) {
-    const T d2 = (w[k + 2] - T(2) * w[k + 1]) + w[k];
-    E[k] = C::a0 * (d2 * d2);
-  }
+    // ---- stage the tile and its halo (the previous tile's LDS reads are
+    // complete after this barrier)
+    __syncthreads();
+    if (fast) {
+      if constexpr (!PF) prefetch(tile);
+      cu.store(s_u, tid);
+      if constexpr (VEL == 2) cv.store(s_v, tid);
+      if constexpr (STAGE >= 2) cn.store(s_io, tid);
+    } else {
+      for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {
+        s_u[j] = load_filled(p.uk, S - HALO + j, m, n, g);
+      }
+      if constexpr (VEL == 2) {
+        for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {
+          s_v[j] = load_raw(p.vel, S - HALO + j, m);
+        }
+      }
+      if constexpr (STAGE >= 2) {
+        for (int j = tid; j < TILE; j += BLOCK) {
+          const int64_t gm = S + j;
+          s_io[j] = gm < m ? p.un[gm] : T(0);
+        }
+      }
+    }
+    __syncthreads();
 
-  T ul[CPT], ur[CPT];
-#pragma unroll
-  for (int r = 0; r < CPT; ++r) {
-    const T um2 = w[r], um1 = w[r + 1], u0 = w[r + 2], up1 = w[r + 3],
-            up2 = w[r + 4];
-    const T t01 = (T(3) * u0 - T(4) * um1) + um2;
-    const T t11 = up1 - um1;
-    const T t21 = (up2 - T(4) * up1) + T(3) * u0;
-    const T b0 = E[r] + C::a1 * (t01 * t01);
-    const T b1 = E[r + 1] + C::a1 * (t11 * t11);
-    const T b2 = E[r + 2] + C::a1 * (t21 * t21);
-    weno_pair_from_beta(um2, um1, u0, up1, up2, b0, b1, b2, p.eps, ul[r],
-                        ur[r]);
-  }
+    // ---- the next tile's reads overlap this tile's arithmetic
+    if constexpr (PF) {
+      const int64_t next = tile + gridDim.x;
+      if (next < ntiles && is_fast(next)) prefetch(next);
+    }
 
-  T al[VEL == 2 ? CPT : 1], ar[VEL == 2 ? CPT : 1];
-  if constexpr (VEL == 2) {
+    // ---- reconstruct CPT consecutive cells
+    const int l0 = tid * CPT;
+    T w[CPT + 4];  // cells l0 - 2 .. l0 + CPT + 1
+#pragma unroll
+    for (int k = 0; k < CPT + 4; ++k) w[k] = s_u[l0 + 2 + k];
+
+    // a0 * (second difference)^2 at cells l0 - 1 .. l0 + CPT, shared by the
+    // three indicators of neighbouring cells
+    T E[CPT + 2];
+#pragma unroll
+    for (int k = 0; k < CPT + 2; ++k) {
+      const T d2 = (w[k + 2] - T(2) * w[k + 1]) + w[k];
+      E[k] = C::a0 * (d2 * d2);
+    }
+
+    T ul[CPT], ur[CPT];
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
-      const int b = l0 + HALO + r;
-      weno_pair(s_v[b - 2], s_v[b - 1], s_v[b], s_v[b + 1], s_v[b + 2], p.eps,
-                al[r], ar[r]);
+      const T um2 = w[r], um1 = w[r + 1], u0 = w[r + 2], up1 = w[r + 3],
+              up2 = w[r + 4];
+      const T t01 = (T(3) * u0 - T(4) * um1) + um2;
+      const T t11 = up1 - um1;
+      const T t21 = (up2 - T(4) * up1) + T(3) * u0;
+      const T b0 = E[r] + C::a1 * (t01 * t01);
+      const T b1 = E[r + 1] + C::a1 * (t11 * t11);
+      const T b2 = E[r + 2] + C::a1 * (t21 * t21);
+      weno_pair_from_beta(um2, um1, u0, up1, up2, b0, b1, b2, p.eps, ul[r],
+                          ur[r]);
     }
-  }
 
-  // ---- exchange edge states with the neighbouring threads
-  s_urL[tid + 1] = ur[CPT - 1];
-  s_ulF[tid] = ul[0];
-  if constexpr (VEL == 2) {
-    s_arL[tid + 1] = ar[CPT - 1];
-    s_alF[tid] = al[0];
-  }
-  if (tid == 0) {
-    // right state of local cell -1
-    s_urL[0] = weno_right(s_u[1], s_u[2], s_u[3], s_u[4], s_u[5], p.eps);
+    T al[VEL == 2 ? CPT : 1], ar[VEL == 2 ? CPT : 1];
     if constexpr (VEL == 2) {
-      s_arL[0] = weno_right(s_v[1], s_v[2], s_v[3], s_v[4], s_v[5], p.eps);
-    }
-  }
-  if (tid == BLOCK - 1) {
-    // left state of local cell TILE (mirrored stencil)
-    constexpr int c = TILE + HALO;
-    s_ulF[BLOCK] = weno_right(s_u[c + 2], s_u[c + 1], s_u[c], s_u[c - 1],
-                              s_u[c - 2], p.eps);
-    if constexpr (VEL == 2) {
-      s_alF[BLOCK] = weno_right(s_v[c + 2], s_v[c + 1], s_v[c], s_v[c - 1],
-                                s_v[c - 2], p.eps);
-    }
-  }
-  __syncthreads();
-
-  // ---- faces l0 - 1/2 .. l0 + CPT - 1/2
-  T F[CPT + 1];
 #pragma unroll
-  for (int k = 0; k <= CPT; ++k) {
-    const T urL = k == 0 ? s_urL[tid] : ur[k - 1];
-    const T ulR = k == CPT ? s_ulF[tid + 1] : ul[k];
-    const int64_t gm = S + l0 + k;  // face between cells gm - 1 and gm
-    T Fk;
-    if constexpr (FLUX == PSA_FLUX_RUSANOV) {
-      T nuk = T(1);
-      if (p.nu != nullptr && gm >= 1 && gm <= m - 1) nuk = p.nu[gm - 1];
-      Fk = flux_rusanov(urL, ulR, w[k + 1], w[k + 2], nuk);
-    } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
-      Fk = flux_burgers_upwind(urL, ulR);
-    } else {
-      T arL, alR;
+      for (int r = 0; r < CPT; ++r) {
+        const int b = l0 + HALO + r;
+        weno_pair(s_v[b - 2], s_v[b - 1], s_v[b], s_v[b + 1], s_v[b + 2],
+                  p.eps, al[r], ar[r]);
+      }
+    }
+
+    // ---- exchange edge states with the neighbouring threads
+    s_urL[tid + 1] = ur[CPT - 1];
+    s_ulF[tid] = ul[0];
+    if constexpr (VEL == 2) {
+      s_arL[tid + 1] = ar[CPT - 1];
+      s_alF[tid] = al[0];
+    }
+    if (tid == 0) {
+      // right state of local cell -1
+      s_urL[0] = weno_right(s_u[1], s_u[2], s_u[3], s_u[4], s_u[5], p.eps);
       if constexpr (VEL == 2) {
-        arL = k == 0 ? s_arL[tid] : ar[k - 1];
-        alR = k == CPT ? s_alF[tid + 1] : al[k];
-      } else {
-        // a constant velocity reconstructs to itself up to rounding, so the
-        // upwind switch only sees its sign
-        arL = p.vconst;
-        alR = p.vconst;
+        s_arL[0] = weno_right(s_v[1], s_v[2], s_v[3], s_v[4], s_v[5], p.eps);
       }
-      Fk = flux_advection_upwind(urL, ulR, arL, alR);
     }
-    if constexpr (STAGE == 0) {
-      if (gm <= 0 || gm >= m) Fk = T(0);
+    if (tid == BLOCK - 1) {
+      // left state of local cell TILE (mirrored stencil)
+      constexpr int c = TILE + HALO;
+      s_ulF[BLOCK] = weno_right(s_u[c + 2], s_u[c + 1], s_u[c], s_u[c - 1],
+                                s_u[c - 2], p.eps);
+      if constexpr (VEL == 2) {
+        s_alF[BLOCK] = weno_right(s_v[c + 2], s_v[c + 1], s_v[c], s_v[c - 1],
+                                  s_v[c - 2], p.eps);
+      }
     }
-    F[k] = Fk;
-  }
+    __syncthreads();
 
-  // ---- conservative difference and stage combination
+    // ---- faces l0 - 1/2 .. l0 + CPT - 1/2
+    T F[CPT + 1];
+    const bool has_nu = FLUX == PSA_FLUX_RUSANOV && p.nu != nullptr;
 #pragma unroll
-  for (int r = 0; r < CPT; ++r) {
-    const T dF = F[r + 1] - F[r];
-    T L;
-    if constexpr (STAGE == 0) {
-      if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
-        L = (-velocity_at<T, VEL>(s_v, l0 + HALO + r, p.vconst) * dF) / p.dx;
+    for (int k = 0; k <= CPT; ++k) {
+      const T urL = k == 0 ? s_urL[tid] : ur[k - 1];
+      const T ulR = k == CPT ? s_ulF[tid + 1] : ul[k];
+      T Fk;
+      if constexpr (FLUX == PSA_FLUX_RUSANOV) {
+        if (has_nu) {
+          const int64_t gm = S + l0 + k;  // face between cells gm - 1 and gm
+          const T nuk = (gm >= 1 && gm <= m - 1) ? p.nu[gm - 1] : T(1);
+          Fk = flux_rusanov(urL, ulR, w[k + 1], w[k + 2], nuk);
+        } else {
+          Fk = flux_rusanov_nu1(urL, ulR, w[k + 1], w[k + 2]);
+        }
+      } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
+        Fk = flux_burgers_upwind(urL, ulR);
       } else {
-        L = -dF / p.dx;
+        T arL, alR;
+        if constexpr (VEL == 2) {
+          arL = k == 0 ? s_arL[tid] : ar[k - 1];
+          alR = k == CPT ? s_alF[tid + 1] : al[k];
+        } else {
+          // a constant velocity reconstructs to itself up to rounding, so
+          // the upwind switch only sees its sign
+          arL = p.vconst;
+          alR = p.vconst;
+        }
+        Fk = flux_advection_upwind(urL, ulR, arL, alR);
       }
-    } else {
-      if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
-        L = (-velocity_at<T, VEL>(s_v, l0 + HALO + r, p.vconst) * dF) *
-            p.inv_dx;
-      } else {
-        L = -dF * p.inv_dx;
+      if constexpr (STAGE == 0) {
+        const int64_t gm = S + l0 + k;
+        if (gm <= 0 || gm >= m) Fk = T(0);
       }
+      F[k] = Fk;
     }
-    const T uk = w[r + 2];
-    T val;
-    if constexpr (STAGE == 0) {
-      val = L;
-    } else if constexpr (STAGE == 1) {
-      val = uk + dt * L;
-    } else if constexpr (STAGE == 2) {
-      val = T(3.0 / 4.0) * s_io[l0 + r] + T(1.0 / 4.0) * (uk + dt * L);
-    } else {
-      val = T(1.0 / 3.0) * s_io[l0 + r] + T(2.0 / 3.0) * (uk + dt * L);
-    }
-    s_io[l0 + r] = val;
-  }
-  __syncthreads();
 
-  // ---- coalesced stores (+ ghost images, + max|u| of the new state)
-  unsigned long long vmax = 0;
-  for (int j = tid; j < TILE; j += BLOCK) {
-    const int64_t gm = S + j;
-    if constexpr (STAGE == 0) {
-      if (gm < m) p.out[gm] = s_io[j];
+    // ---- conservative difference and stage combination
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+      const T dF = F[r + 1] - F[r];
+      T L;
+      if constexpr (STAGE == 0) {
+        if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
+          L = (-velocity_at<T, VEL>(s_v, l0 + HALO + r, p.vconst) * dF) / p.dx;
+        } else {
+          L = -dF / p.dx;
+        }
+      } else {
+        if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
+          L = (-velocity_at<T, VEL>(s_v, l0 + HALO + r, p.vconst) * dF) *
+              p.inv_dx;
+        } else {
+          L = -dF * p.inv_dx;
+        }
+      }
+      const T uk = w[r + 2];
+      T val;
+      if constexpr (STAGE == 0) {
+        val = L;
+      } else if constexpr (STAGE == 1) {
+        val = uk + dt * L;
+      } else if constexpr (STAGE == 2) {
+        val = T(3.0 / 4.0) * s_io[l0 + r] + T(1.0 / 4.0) * (uk + dt * L);
+      } else {
+        val = T(1.0 / 3.0) * s_io[l0 + r] + T(2.0 / 3.0) * (uk + dt * L);
+      }
+      s_io[l0 + r] = val;
+    }
+    __syncthreads();
+
+    // ---- coalesced stores (+ ghost images, + max|u| of the new state)
+    if (fast) {
+      Chunks<T, BLOCK, TILE / VN> co;
+#pragma unroll
+      for (int k = 0; k < co.CH; ++k) {
+        const int c = tid + k * BLOCK;
+        if (k < co.CH - 1 || (TILE / VN) % BLOCK == 0 || c < TILE / VN)
+          co.r[k] = load16(s_io + c * VN);
+      }
+      co.store(p.out + S, tid);
+      if constexpr (STAGE == 3) {
+#pragma unroll
+        for (int k = 0; k < co.CH; ++k) {
+          const int c = tid + k * BLOCK;
+          if (k < co.CH - 1 || (TILE / VN) % BLOCK == 0 || c < TILE / VN)
+            vmax = vmax_bits(co.r[k], vmax);
+        }
+      }
     } else {
-      if (gm >= g && gm < m - g) {
-        const T v = s_io[j];
-        p.out[gm] = v;
-        if (gm < 2 * g) p.out[gm + n] = v;
-        if (gm >= m - 2 * g) p.out[gm - n] = v;
-        if constexpr (STAGE == 3) {
-          const unsigned long long b = abs_bits(v);
-          vmax = b > vmax ? b : vmax;
+      for (int j = tid; j < TILE; j += BLOCK) {
+        const int64_t gm = S + j;
+        if constexpr (STAGE == 0) {
+          if (gm < m) p.out[gm] = s_io[j];
+        } else {
+          if (gm >= g && gm < m - g) {
+            const T v = s_io[j];
+            p.out[gm] = v;
+            if (gm < 2 * g) p.out[gm + n] = v;
+            if (gm >= m - 2 * g) p.out[gm - n] = v;
+            if constexpr (STAGE == 3) {
+              const unsigned long long b = abs_bits(v);
+              vmax = b > vmax ? b : vmax;
+            }
+          }
         }
       }
     }
   }
+
   if constexpr (STAGE == 3) {
     if (p.ctrl != nullptr && p.ctrl->speed_from_state) {
       vmax = block_max_bits<BLOCK>(vmax, s_red);
-      if (tid == 0) atomicMax(&p.ctrl->umax_bits, vmax);
+      if (tid == 0)
+        atomicMax(&p.ctrl->umax_slot[blockIdx.x % PSA_UMAX_SLOTS][0], vmax);
     }
   }
 }
@@ -627,12 +790,51 @@ struct TileConfig {
   int cpt;
 };
 
-template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT>
-int launch_stage_cfg(const StageArgs<T> &a, hipStream_t stream) {
+inline bool aligned16(const void *p) {
+  return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+}
+
+// Workgroups the device holds at once for a kernel (CUs x occupancy),
+// queried once per kernel instantiation and device.
+template <typename K>
+unsigned int resident_blocks(K kernel, int block) {
+  constexpr int kMaxDevices = 64;
+  static unsigned int cache[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return 0;
+  if (cache[dev] == 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(kernel), block, 0) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess ||
+        per_cu <= 0 || cus <= 0) {
+      (void)hipGetLastError();
+      return 0;
+    }
+    cache[dev] = static_cast<unsigned int>(per_cu) * static_cast<unsigned int>(cus);
+  }
+  return cache[dev];
+}
+
+// PF: persistent workgroups (grid = resident capacity) that prefetch their
+// next tile; otherwise one workgroup per tile.
+template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT, bool PF>
+int launch_stage_cfg(const StageArgs<T> &args, hipStream_t stream) {
   constexpr int TILE = BLOCK * CPT;
-  const unsigned int blocks = blocks_for(a.m, TILE);
-  hipLaunchKernelGGL((k_stage<T, FLUX, VEL, STAGE, BLOCK, CPT>), dim3(blocks),
-                     dim3(BLOCK), 0, stream, a);
+  StageArgs<T> a = args;
+  a.aligned = aligned16(a.uk) && aligned16(a.out) &&
+              (a.un == nullptr || aligned16(a.un)) &&
+              (a.vel == nullptr || aligned16(a.vel));
+  unsigned int blocks = blocks_for(a.m, TILE);
+  if constexpr (PF) {
+    const unsigned int resident =
+        resident_blocks(k_stage<T, FLUX, VEL, STAGE, BLOCK, CPT, PF>, BLOCK);
+    if (resident > 0 && blocks > resident) blocks = resident;
+  }
+  hipLaunchKernelGGL((k_stage<T, FLUX, VEL, STAGE, BLOCK, CPT, PF>),
+                     dim3(blocks), dim3(BLOCK), 0, stream, a);
   return check_launch("ssprk33 stage");
 }
 
@@ -640,11 +842,13 @@ template <typename T, int FLUX, int VEL, int STAGE>
 int launch_stage_tiles(const StageArgs<T> &a, TileConfig cfg,
                        hipStream_t stream) {
   if (cfg.block == 256 && cfg.cpt == 1)
-    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 1>(a, stream);
+    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 1, false>(a, stream);
   if (cfg.block == 256 && cfg.cpt == 2)
-    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 2>(a, stream);
+    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 2, false>(a, stream);
   if (cfg.block == 256 && cfg.cpt == 4)
-    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 4>(a, stream);
+    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 4, false>(a, stream);
+  if (cfg.block == 256 && cfg.cpt == -4)
+    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 4, true>(a, stream);
   return set_error(PSA_ERR_INVALID_ARGUMENT,
                    "unsupported tile configuration block=" +
                        std::to_string(cfg.block) +
@@ -715,6 +919,7 @@ StageArgs<T> make_args(const psa_solver_desc *d) {
   a.vconst = static_cast<T>(d->velocity_const);
   a.vel = static_cast<const T *>(d->velocity);
   a.nu = static_cast<const T *>(d->nu);
+  a.aligned = 1;
   return a;
 }
 
@@ -1022,9 +1227,9 @@ int psa_control_init(const psa_solver_desc *desc, const void *u,
   const void *src = advection ? desc->velocity : u;
   if (desc->dtype == PSA_FLOAT64)
     return launch_max_abs<double>(static_cast<const double *>(src), m,
-                                  desc->g, nullptr, &ctrl->umax_bits, s);
+                                  desc->g, nullptr, &ctrl->umax_slot[0][0], s);
   return launch_max_abs<float>(static_cast<const float *>(src), m, desc->g,
-                               nullptr, &ctrl->umax_bits, s);
+                               nullptr, &ctrl->umax_slot[0][0], s);
 }
 
 int psa_ssprk33_steps(const psa_solver_desc *desc, const void *u_in,

@@ -131,7 +131,13 @@ __device__ __forceinline__ void weno_pair_from_beta<double>(
 
   const double den = sr * sl;
   if (den >= Limits<double>::min_normal && den <= Limits<double>::max_finite) {
-    const double inv = 1.0 / den;
+    // hardware reciprocal refined by two Newton steps (each squares the
+    // relative error): within an ulp or two of 1 / den
+    double inv = __builtin_amdgcn_rcp(den);
+    double e = fma(-den, inv, 1.0);
+    inv = fma(inv, e, inv);
+    e = fma(-den, inv, 1.0);
+    inv = fma(inv, e, inv);
     ur = (nr * sl) * inv;
     ul = (nl * sr) * inv;
   } else {
@@ -196,6 +202,15 @@ __device__ __forceinline__ T flux_rusanov(T urL, T ulR, T uL, T uR, T nu) {
   const T fr = (urL * urL) / T(2);
   const T a = fmax(fabs(uR), fabs(uL));
   return T(0.5) * (fl + fr) - ((T(0.5) * a) * nu) * (ulR - urL);
+}
+
+// The same flux with nu = 1 and the two flux halves combined,
+// 0.5 (ul^2 / 2 + ur^2 / 2) = 0.25 (ul^2 + ur^2); it differs from the
+// reference's expression order by rounding only.
+template <typename T>
+__device__ __forceinline__ T flux_rusanov_nu1(T urL, T ulR, T uL, T uR) {
+  const T a = fmax(fabs(uR), fabs(uL));
+  return T(0.25) * fma(ulR, ulR, urL * urL) - (T(0.5) * a) * (ulR - urL);
 }
 
 // scalar.py:125-130: aavg = (ar + al) / 2 with a = u reconstructed the same

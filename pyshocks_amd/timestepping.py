@@ -247,14 +247,20 @@ def _scheme_plan(source: Any) -> tuple[_lib.PsaSolverDesc, tuple[Any, ...]] | No
     return desc, keep
 
 
-def fused_plan(stepper: Stepper) -> FusedPlan | None:
-    """The fused plan of ``stepper``, or ``None`` for the generic path."""
+def fused_plan(stepper: Stepper, *,
+               tiles: tuple[int, int] | None = None) -> FusedPlan | None:
+    """The fused plan of ``stepper``, or ``None`` for the generic path.
+
+    ``tiles = (threads per workgroup, cells per thread)`` overrides the
+    library's tile choice for the stage kernels."""
     if not isinstance(stepper, SSPRK33):
         return None
     sp = _scheme_plan(stepper.source)
     if sp is None:
         return None
     desc, keep = sp
+    if tiles is not None:
+        desc.block, desc.cells_per_thread = int(tiles[0]), int(tiles[1])
     source: BoundOperator = stepper.source  # type: ignore[assignment]
     grid = source.grid
     npt = _np_dtype(grid.dtype)
@@ -390,7 +396,7 @@ def _step_generic(stepper: Stepper, u0: torch.Tensor, *, maxit: int | None,
 
 def evolve(stepper: Stepper, u0: torch.Tensor, *, maxit: int | None = None,
            tstart: float = 0.0, tfinal: float | None = None,
-           chunk: int = 64) -> StepCompleted:
+           chunk: int = 64, tiles: tuple[int, int] | None = None) -> StepCompleted:
     """Run :func:`step` to completion without per-step host synchronisation
     and return the final :class:`StepCompleted`.
 

@@ -204,7 +204,10 @@ def test_fused_c1_burgers_rusanov(torch_mod) -> None:
             dts.append(float(event.dt))
     assert event is not None
     assert len(dts) == len(g["dts"])
-    np.testing.assert_allclose(dts, g["dts"], rtol=RTOL64, atol=0)
+    np.testing.assert_allclose(dts[:-1], g["dts"][:-1], rtol=RTOL64, atol=0)
+    # the last step is clipped to tfinal - t + 1e-15, where t is a sum of
+    # 368 steps that may differ from the oracle's in the last few ulps
+    assert abs(dts[-1] - g["dts"][-1]) < 1e-15
     assert abs(float(event.t) - float(g["tfinal"])) < 1e-15
     i = np.s_[3:-3]
     for k in (1, 10):
@@ -245,14 +248,15 @@ def test_fused_advection(torch_mod) -> None:
     assert _rel(_host(event.u)[3:-3], g["ufinal"][3:-3]) < RTOL64
 
 
-def test_fused_advection_velocity_array(torch_mod) -> None:
-    """A non-constant velocity array takes the reconstructed-velocity path."""
+@pytest.mark.parametrize("n", [200, 20000])
+def test_fused_advection_velocity_array(torch_mod, n: int) -> None:
+    """A non-constant velocity array takes the reconstructed-velocity path
+    (n = 20000 also takes the interior-tile fast path)."""
     torch = torch_mod
     from oracle import weno_np as W
     from pyshocks_amd import make_uniform_cell_grid
     from pyshocks_amd.timestepping import evolve
 
-    n = 200
     rng = np.random.default_rng(11)
     x = (np.arange(n + 6) - 2.5) / n
     vel = 0.5 + 0.25 * np.sin(2 * np.pi * x)
@@ -267,9 +271,13 @@ def test_fused_advection_velocity_array(torch_mod) -> None:
     assert _rel(_host(ev.u)[3:-3], u[3:-3]) < RTOL64
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
-def test_fused_large_sine_step(torch_mod, kind: str) -> None:
-    """N = 2^16, sum of 8 sinusoids plus a step, 100 steps at Courant 0.5."""
+@pytest.mark.parametrize(("kind", "tiles"), [
+    ("rusanov", (256, 1)), ("rusanov", (256, 2)), ("rusanov", (256, 4)),
+    ("godunov", (256, 4)),
+])
+def test_fused_large_sine_step(torch_mod, kind: str, tiles) -> None:
+    """N = 2^16, sum of 8 sinusoids plus a step, 100 steps at Courant 0.5,
+    for each tile configuration of the stage kernels."""
     torch = torch_mod
     from oracle import exact as E
     from oracle import weno_np as W
@@ -280,7 +288,7 @@ def test_fused_large_sine_step(torch_mod, kind: str) -> None:
     u0 = E.exact_cell_averages(E.large_initial_condition(seed=0), n)
     grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
     ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
-    ev = evolve(_stepper(torch, kind, grid), _dev(torch, u0), maxit=100)
+    ev = evolve(_stepper(torch, kind, grid), _dev(torch, u0), maxit=100, tiles=tiles)
     u, dts, t = W.solve(W.Scheme(kind, eps=1e-6), ogrid, u0, theta=1.0, maxit=100)
     assert ev.iteration == 100
     assert abs(float(ev.t) - t) <= 1e-12 * t
