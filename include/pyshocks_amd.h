@@ -205,6 +205,50 @@ int psa_ssprk33_advance(const psa_solver_desc *desc, const void *u,
                         void *out, void *k1, void *k2, const void *dt,
                         void *stream);
 
+/* Discrete adjoint --------------------------------------------------------
+ * Replaces the dense jax.jacfwd Jacobian of timestepping.py:155-215
+ * (adjoint_step) with matrix-free vector-Jacobian products of the fused
+ * step. Cotangents are arrays of m cells whose ghost entries are ignored on
+ * input and zero on output. Supported: Rusanov (alpha = 1), Burgers upwind
+ * and advection with a constant velocity; dt is a device scalar held
+ * constant. Non-smooth points follow jax's conventions (max ties split
+ * 0.5 / 0.5, sign(0) = 0, jnp.where differentiates the selected branch). */
+
+/* Reverse of one stage. stage 0: out = J_L(state)^T mu. stage 3:
+ * out = 2/3 (mu + dt J_L(state)^T mu) with state = k2 and mu the step
+ * cotangent; stage 2: out = 1/4 (mu + dt J_L(state)^T mu) with state = k1;
+ * stage 1: out = 1/3 lam + 3/4 lam_k2 + mu + dt J_L(state)^T mu with
+ * state = u. */
+int psa_ssprk33_stage_vjp(const psa_solver_desc *desc, int stage,
+                          const void *state, const void *mu, const void *lam,
+                          const void *lam_k2, void *out, const void *dt,
+                          void *stream);
+
+/* lam_out = J(u)^T lam for one SSP-RK3 step from u (the per-step product of
+ * adjoint_step); work holds 4 arrays of m cells (each rounded up to 256
+ * bytes). */
+int psa_ssprk33_step_vjp(const psa_solver_desc *desc, const void *u,
+                         const void *lam, void *lam_out, void *work,
+                         size_t work_bytes, const void *dt, void *stream);
+
+/* Checkpointed forward solve and its reverse sweep: nsteps fixed-dt steps
+ * from u0 storing the state every `every` steps in the workspace, then the
+ * gradient grad_u0 = (d u_final / d u0)^T lam_final by recomputing each
+ * window of `every` steps from its checkpoint (keeping every stage state)
+ * and sweeping it backwards. Both calls use the same workspace of
+ * psa_adjoint_workspace_bytes() bytes. */
+int psa_adjoint_workspace_bytes(const psa_solver_desc *desc, long long nsteps,
+                                int every, size_t *bytes);
+int psa_ssprk33_forward_checkpointed(const psa_solver_desc *desc,
+                                     const void *u0, void *u_final,
+                                     long long nsteps, const void *dt,
+                                     int every, void *workspace,
+                                     size_t workspace_bytes, void *stream);
+int psa_ssprk33_adjoint(const psa_solver_desc *desc, long long nsteps,
+                        const void *dt, int every, const void *lam_final,
+                        void *grad_u0, void *workspace, size_t workspace_bytes,
+                        void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,6 +110,16 @@ SIGNATURES: dict[str, list[Any]] = {
                             _VP, _VP],
     "psa_ssprk33_stage": [ctypes.POINTER(PsaSolverDesc), _I, _VP, _VP, _VP, _VP,
                           _VP, _VP],
+    "psa_ssprk33_stage_vjp": [ctypes.POINTER(PsaSolverDesc), _I, _VP, _VP, _VP,
+                              _VP, _VP, _VP, _VP],
+    "psa_ssprk33_step_vjp": [ctypes.POINTER(PsaSolverDesc), _VP, _VP, _VP, _VP,
+                             ctypes.c_size_t, _VP, _VP],
+    "psa_adjoint_workspace_bytes": [ctypes.POINTER(PsaSolverDesc), _LL, _I,
+                                    ctypes.POINTER(ctypes.c_size_t)],
+    "psa_ssprk33_forward_checkpointed": [ctypes.POINTER(PsaSolverDesc), _VP, _VP,
+                                         _LL, _VP, _I, _VP, ctypes.c_size_t, _VP],
+    "psa_ssprk33_adjoint": [ctypes.POINTER(PsaSolverDesc), _LL, _VP, _I, _VP, _VP,
+                            _VP, ctypes.c_size_t, _VP],
 }
 
 _LIB: ctypes.CDLL | None = None

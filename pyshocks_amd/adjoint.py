@@ -1,12 +1,131 @@
-"""Discrete adjoint of the fused SSP-RK3 step (reverse stage kernels)."""
+"""Discrete adjoint of the fused SSP-RK3 step.
+
+The reference computes adjoints with a dense ``jax.jacfwd`` Jacobian of one
+step (``timestepping.py:155-215``, feasible only for small grids, as noted at
+``:167-172``). Here the transposed Jacobian is applied matrix-free by the
+library's reverse stage kernels:
+
+* :func:`ssprk33_step_vjp` -- ``J(u)^T p`` for one step (used by
+  :func:`~pyshocks_amd.timestepping.adjoint_step`);
+* :func:`checkpointed_solve` / :class:`SSPRK33Solve` -- a fixed-dt solve of
+  ``nsteps`` steps whose backward pass recomputes windows of ``every`` steps
+  from checkpoints and sweeps them in reverse, exposed to torch autograd.
+
+Cotangents use the grid layout (ghost entries are ignored on input and are
+zero on output).
+"""
 
 from __future__ import annotations
 
+import ctypes
 from typing import Any
 
 import torch
 
+from pyshocks_amd import _lib
 
-def ssprk33_step_vjp(source: Any, u: torch.Tensor, p: torch.Tensor, dt: Any) -> torch.Tensor:
-    """``J(u)^T p`` for one SSP-RK3 step of the bound operator."""
-    raise NotImplementedError("the reverse stage kernels are not built yet")
+
+def _plan_of(source: Any) -> _lib.PsaSolverDesc:
+    from pyshocks_amd.timestepping import _scheme_plan
+
+    sp = _scheme_plan(source)
+    if sp is None:
+        raise NotImplementedError(
+            "the adjoint needs a BoundOperator of a hot-path scheme")
+    return sp[0]
+
+
+def _dt_tensor(dt: Any, like: torch.Tensor) -> torch.Tensor:
+    return torch.as_tensor(dt, dtype=like.dtype, device=like.device).reshape(()).contiguous()
+
+
+def _array_bytes(m: int, dtype: torch.dtype) -> int:
+    item = 8 if dtype == torch.float64 else 4
+    return ((m * item + 255) // 256) * 256
+
+
+def operator_vjp(source: Any, u: torch.Tensor, mu: torch.Tensor) -> torch.Tensor:
+    """``J_L(u)^T mu`` for the operator ``L = apply_operator(scheme, grid,
+    bc, t, .)`` of a bound source."""
+    desc = _plan_of(source)
+    _lib.require_device(u, mu)
+    out = torch.empty_like(u)
+    _lib.call("psa_ssprk33_stage_vjp", ctypes.byref(desc), 0, _lib.ptr(u),
+              _lib.ptr(mu), None, None, _lib.ptr(out), None, _lib.stream_of(u))
+    return out
+
+
+def ssprk33_step_vjp(source: Any, u: torch.Tensor, p: torch.Tensor,
+                     dt: Any) -> torch.Tensor:
+    """``J(u)^T p`` for one SSP-RK3 step of size ``dt`` from ``u``."""
+    desc = _plan_of(source)
+    _lib.require_device(u, p)
+    dt_dev = _dt_tensor(dt, u)
+    ab = _array_bytes(u.numel(), u.dtype)
+    work = torch.empty(4 * ab, dtype=torch.uint8, device=u.device)
+    out = torch.empty_like(u)
+    _lib.call("psa_ssprk33_step_vjp", ctypes.byref(desc), _lib.ptr(u), _lib.ptr(p),
+              _lib.ptr(out), _lib.ptr(work), work.numel(), _lib.ptr(dt_dev),
+              _lib.stream_of(u))
+    return out
+
+
+class CheckpointedSolve:
+    """A fixed-dt forward solve that keeps checkpoints for its adjoint."""
+
+    def __init__(self, source: Any, u0: torch.Tensor, nsteps: int, dt: Any,
+                 every: int = 50) -> None:
+        self.desc = _plan_of(source)
+        _lib.require_device(u0)
+        self.nsteps = int(nsteps)
+        self.every = int(every)
+        self.dt = _dt_tensor(dt, u0)
+        nbytes = ctypes.c_size_t(0)
+        _lib.call("psa_adjoint_workspace_bytes", ctypes.byref(self.desc),
+                  self.nsteps, self.every, ctypes.byref(nbytes))
+        self.workspace = torch.empty(nbytes.value, dtype=torch.uint8, device=u0.device)
+        self.u_final = torch.empty_like(u0)
+        _lib.call("psa_ssprk33_forward_checkpointed", ctypes.byref(self.desc),
+                  _lib.ptr(u0), _lib.ptr(self.u_final), self.nsteps,
+                  _lib.ptr(self.dt), self.every, _lib.ptr(self.workspace),
+                  self.workspace.numel(), _lib.stream_of(u0))
+
+    def gradient(self, lam_final: torch.Tensor) -> torch.Tensor:
+        """``(d u_final / d u0)^T lam_final``."""
+        lam_final = lam_final.contiguous()
+        _lib.require_device(lam_final)
+        grad = torch.empty_like(lam_final)
+        _lib.call("psa_ssprk33_adjoint", ctypes.byref(self.desc), self.nsteps,
+                  _lib.ptr(self.dt), self.every, _lib.ptr(lam_final),
+                  _lib.ptr(grad), _lib.ptr(self.workspace), self.workspace.numel(),
+                  _lib.stream_of(lam_final))
+        return grad
+
+
+def checkpointed_solve(source: Any, u0: torch.Tensor, nsteps: int, dt: Any,
+                       every: int = 50) -> CheckpointedSolve:
+    return CheckpointedSolve(source, u0, nsteps, dt, every)
+
+
+class SSPRK33Solve(torch.autograd.Function):
+    """``u_final = S^nsteps(u0)`` for the fused fixed-dt SSP-RK3 step, with a
+    backward pass by the reverse kernels and checkpoint/recompute every
+    ``every`` steps."""
+
+    @staticmethod
+    def forward(ctx: Any, u0: torch.Tensor, source: Any, nsteps: int, dt: Any,
+                every: int = 50) -> torch.Tensor:
+        solve = CheckpointedSolve(source, u0.detach().contiguous(), nsteps, dt, every)
+        ctx.solve = solve
+        return solve.u_final.clone()
+
+    @staticmethod
+    def backward(ctx: Any, grad_out: torch.Tensor) -> tuple[Any, ...]:
+        grad = ctx.solve.gradient(grad_out.contiguous())
+        return grad, None, None, None, None
+
+
+def ssprk33_solve(u0: torch.Tensor, source: Any, nsteps: int, dt: Any,
+                  every: int = 50) -> torch.Tensor:
+    """Differentiable fixed-dt solve (see :class:`SSPRK33Solve`)."""
+    return SSPRK33Solve.apply(u0, source, nsteps, dt, every)

@@ -1,0 +1,527 @@
+// Discrete adjoint (vector-Jacobian products) of the fused SSP-RK3 step.
+//
+// The reference differentiates one step with a dense forward-mode Jacobian
+// (timestepping.py:155-215, jax.jacfwd of advance). Here the transposed
+// Jacobian is applied matrix-free, one reverse kernel per Runge-Kutta stage:
+//
+//   forward  k1 = x + dt L(x)
+//            k2 = 3/4 x + 1/4 (k1 + dt L(k1))
+//            x+ = 1/3 x + 2/3 (k2 + dt L(k2))
+//   reverse  lk2 = 2/3 (l + dt JL(k2)^T l)
+//            lk1 = 1/4 (lk2 + dt JL(k1)^T lk2)
+//            lx  = 1/3 l + 3/4 lk2 + lk1 + dt JL(x)^T lk1
+//
+// JL^T mu is formed per cell from the closed-form gradient of the two WENO-JS5
+// reconstructions of the cell and the partial derivatives of the face flux.
+// Derivative conventions at the non-smooth points follow jax: jnp.maximum
+// splits a tie 0.5 / 0.5, d|u|/du = sign(u) with sign(0) = 0, and the upwind
+// selection (jnp.where) passes the derivative of the selected branch only.
+// dt is a constant (adjoint_step recovers it as t_{m+1} - t_m).
+//
+// The state and the cotangents are periodic: every index is mapped to its
+// interior image, which also covers the 5-cell halo the transpose needs.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "../../include/pyshocks_amd.h"
+#include "psa_internal.hpp"
+#include "weno5_device.hpp"
+
+namespace psa {
+namespace {
+
+constexpr int kVjpBlock = 256;
+constexpr int kVjpOut = kVjpBlock - 6;  // output cells per workgroup
+
+// cell gm of a periodic array with n interior cells and g ghosts
+template <typename T>
+__device__ __forceinline__ T load_periodic(const T *__restrict__ u, int64_t gm,
+                                           int64_t n, int g) {
+  int64_t k = (gm - g) % n;
+  if (k < 0) k += n;
+  return u[g + k];
+}
+
+// Right reconstruction (value at x_{i+1/2}) of a cell from
+// (u[i-2], ..., u[i+2]) and its gradient with respect to those five values.
+template <typename T>
+__device__ __forceinline__ T weno_side_grad(T um2, T um1, T u0, T up1, T up2,
+                                            T eps, T G[5]) {
+  using C = Weno5Coeffs<T>;
+  const T t00 = (u0 - T(2) * um1) + um2;
+  const T t01 = (T(3) * u0 - T(4) * um1) + um2;
+  const T t10 = (up1 - T(2) * u0) + um1;
+  const T t11 = up1 - um1;
+  const T t20 = (up2 - T(2) * up1) + u0;
+  const T t21 = (up2 - T(4) * up1) + T(3) * u0;
+  const T b0 = C::a0 * (t00 * t00) + C::a1 * (t01 * t01);
+  const T b1 = C::a0 * (t10 * t10) + C::a1 * (t11 * t11);
+  const T b2 = C::a0 * (t20 * t20) + C::a1 * (t21 * t21);
+  T h0, h1, h2;
+  weno_uhat(um2, um1, u0, up1, up2, h0, h1, h2);
+
+  const T e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+  const T a0 = C::d0 / (e0 * e0);
+  const T a1 = C::d1 / (e1 * e1);
+  const T a2 = C::d2 / (e2 * e2);
+  const T s = (a0 + a1) + a2;
+  const T w0 = a0 / s, w1 = a1 / s, w2 = a2 / s;
+  const T r = (w0 * h0 + w1 * h1) + w2 * h2;
+
+  // dr/dbeta_k = -2 omega_k (uhat_k - r) / (eps + beta_k)
+  const T q0 = T(-2) * w0 * (h0 - r) / e0;
+  const T q1 = T(-2) * w1 * (h1 - r) / e1;
+  const T q2 = T(-2) * w2 * (h2 - r) / e2;
+  // dr/dt_kj = dr/dbeta_k * 2 a_j t_kj
+  const T p00 = q0 * (T(2) * C::a0) * t00, p01 = q0 * (T(2) * C::a1) * t01;
+  const T p10 = q1 * (T(2) * C::a0) * t10, p11 = q1 * (T(2) * C::a1) * t11;
+  const T p20 = q2 * (T(2) * C::a0) * t20, p21 = q2 * (T(2) * C::a1) * t21;
+
+  G[0] = (p00 + p01) + w0 * C::c2;
+  G[1] = (T(-2) * p00 - T(4) * p01 + p10 - p11) - w0 * C::c7 - w1 * C::c1;
+  G[2] = (p00 + T(3) * p01 - T(2) * p10 + p20 + T(3) * p21) + w0 * C::c11 +
+         w1 * C::c5 + w2 * C::c2;
+  G[3] = (p10 + p11 - T(2) * p20 - T(4) * p21) + w1 * C::c2 + w2 * C::c5;
+  G[4] = (p20 + p21) - w2 * C::c1;
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T sgn(T x) {
+  return x > T(0) ? T(1) : (x < T(0) ? T(-1) : T(0));
+}
+
+template <typename T>
+struct VjpArgs {
+  const T *state;   // stencil input of the forward stage
+  const T *mu;      // cotangent of the stage operator output
+  const T *lam;     // RSTAGE 1: cotangent of the step output
+  const T *lamk2;   // RSTAGE 1: cotangent of k2
+  T *out;
+  const T *dt_ptr;
+  int64_t m;
+  int64_t n;
+  int g;
+  T inv_dx;
+  T eps;
+  T vconst;
+};
+
+// RSTAGE 0: out = JL(state)^T mu; 3, 2, 1: the reverse stages above, with
+// mu = l, lk2, lk1 respectively.
+template <typename T, int FLUX, int RSTAGE>
+__global__ __launch_bounds__(kVjpBlock) void k_stage_vjp(VjpArgs<T> p) {
+  constexpr int B = kVjpBlock;
+  __shared__ T s_u[B + 4];   // cells S - 5 ..
+  __shared__ T s_mu[B + 1];  // cells S - 3 ..
+  __shared__ T s_ur[B], s_ul[B];
+  __shared__ T s_cR[B], s_cL[B], s_sL[B], s_sR[B];
+  __shared__ T s_P[5][B];
+
+  const int tid = threadIdx.x;
+  const int64_t n = p.n;
+  const int g = p.g;
+  const int64_t S = g + static_cast<int64_t>(blockIdx.x) * kVjpOut;
+
+  if (blockIdx.x == 0 && tid < g) {
+    // ghost entries of a cotangent are zero
+    p.out[tid] = T(0);
+    p.out[p.m - g + tid] = T(0);
+  }
+
+  for (int j = tid; j < B + 4; j += B) s_u[j] = load_periodic(p.state, S - 5 + j, n, g);
+  for (int j = tid; j < B + 1; j += B) s_mu[j] = load_periodic(p.mu, S - 3 + j, n, g);
+  __syncthreads();
+
+  // ---- reconstructions of computing cell c = S - 3 + tid and gradients
+  T GR[5], GLm[5];
+  const T v0 = s_u[tid], v1 = s_u[tid + 1], v2 = s_u[tid + 2],
+          v3 = s_u[tid + 3], v4 = s_u[tid + 4];
+  s_ur[tid] = weno_side_grad(v0, v1, v2, v3, v4, p.eps, GR);
+  s_ul[tid] = weno_side_grad(v4, v3, v2, v1, v0, p.eps, GLm);
+  if (tid == 0) {
+    s_cL[0] = T(0);
+    s_sR[0] = T(0);
+  }
+  __syncthreads();
+
+  // ---- face c + 1/2: cotangent of the flux and its partial derivatives
+  if (tid < B - 1) {
+    const T urL = s_ur[tid], ulR = s_ul[tid + 1];
+    const T uL = s_u[tid + 2], uR = s_u[tid + 3];
+    T gF = (s_mu[tid + 1] - s_mu[tid]) * p.inv_dx;
+    T dur = T(0), dul = T(0), duL = T(0), duR = T(0);
+    if constexpr (FLUX == PSA_FLUX_RUSANOV) {
+      // F = 0.5 (ul^2/2 + ur^2/2) - 0.5 a (ul - ur), a = max(|uR|, |uL|)
+      const T aR = fabs(uR), aL = fabs(uL);
+      const T a = aR > aL ? aR : aL;
+      dur = T(0.5) * urL + T(0.5) * a;
+      dul = T(0.5) * ulR - T(0.5) * a;
+      const T dFda = T(-0.5) * (ulR - urL);
+      const T wR = aR > aL ? T(1) : (aR < aL ? T(0) : T(0.5));
+      duR = dFda * wR * sgn(uR);
+      duL = dFda * (T(1) - wR) * sgn(uL);
+    } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
+      const T aavg = (urL + ulR) / T(2);
+      if (aavg > T(0)) {
+        dur = urL;
+      } else {
+        dul = ulR;
+      }
+    } else {
+      // constant velocity a: L = -a (F[1:] - F[:-1]) / dx, F = ur if a > 0
+      gF *= p.vconst;
+      if (p.vconst > T(0)) {
+        dur = T(1);
+      } else {
+        dul = T(1);
+      }
+    }
+    s_cR[tid] = gF * dur;
+    s_cL[tid + 1] = gF * dul;
+    s_sL[tid] = gF * duL;
+    s_sR[tid + 1] = gF * duR;
+  }
+  __syncthreads();
+
+  // ---- scatter: contribution of cell c's reconstructions to c - 2 .. c + 2
+  if (tid >= 1 && tid <= B - 2) {
+    const T cR = s_cR[tid], cL = s_cL[tid];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) s_P[r][tid] = cR * GR[r] + cL * GLm[4 - r];
+  }
+  __syncthreads();
+
+  // ---- gather for output cell j = S + tid (local computing index q)
+  if (tid < kVjpOut) {
+    const int64_t j = S + tid;
+    if (j < p.m - g) {
+      const int q = tid + 3;
+      T gsum = (s_sL[q] + s_sR[q]);
+#pragma unroll
+      for (int r = 0; r < 5; ++r) gsum += s_P[r][q + 2 - r];
+      const T mu = s_mu[q];
+      T val;
+      if constexpr (RSTAGE == 0) {
+        val = gsum;
+      } else {
+        const T dt = *p.dt_ptr;
+        if constexpr (RSTAGE == 3) {
+          val = T(2.0 / 3.0) * (mu + dt * gsum);
+        } else if constexpr (RSTAGE == 2) {
+          val = T(1.0 / 4.0) * (mu + dt * gsum);
+        } else {
+          val = (T(1.0 / 3.0) * p.lam[j] + T(3.0 / 4.0) * p.lamk2[j]) +
+                (mu + dt * gsum);
+        }
+      }
+      p.out[j] = val;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_zero_ghosts(T *u, int64_t m, int g) {
+  const int t = threadIdx.x;
+  if (blockIdx.x == 0 && t < g) {
+    u[t] = T(0);
+    u[m - g + t] = T(0);
+  }
+}
+
+template <typename T, int FLUX>
+int launch_vjp_flux(int rstage, const VjpArgs<T> &a, hipStream_t s) {
+  const unsigned int blocks =
+      static_cast<unsigned int>((a.n + kVjpOut - 1) / kVjpOut);
+  switch (rstage) {
+    case 0:
+      hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 0>), dim3(blocks),
+                         dim3(kVjpBlock), 0, s, a);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 1>), dim3(blocks),
+                         dim3(kVjpBlock), 0, s, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 2>), dim3(blocks),
+                         dim3(kVjpBlock), 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 3>), dim3(blocks),
+                         dim3(kVjpBlock), 0, s, a);
+      break;
+  }
+  return check_launch("stage vjp");
+}
+
+template <typename T>
+int launch_vjp(const psa_solver_desc *d, int rstage, const void *state,
+               const void *mu, const void *lam, const void *lamk2, void *out,
+               const void *dt, hipStream_t s) {
+  VjpArgs<T> a{};
+  a.state = static_cast<const T *>(state);
+  a.mu = static_cast<const T *>(mu);
+  a.lam = static_cast<const T *>(lam);
+  a.lamk2 = static_cast<const T *>(lamk2);
+  a.out = static_cast<T *>(out);
+  a.dt_ptr = static_cast<const T *>(dt);
+  a.n = d->n;
+  a.g = d->g;
+  a.m = d->n + 2 * static_cast<int64_t>(d->g);
+  a.inv_dx = T(1) / static_cast<T>(d->dx);
+  a.eps = static_cast<T>(d->eps);
+  a.vconst = static_cast<T>(d->velocity_const);
+  if (d->flux == PSA_FLUX_RUSANOV)
+    return launch_vjp_flux<T, PSA_FLUX_RUSANOV>(rstage, a, s);
+  if (d->flux == PSA_FLUX_BURGERS_UPWIND)
+    return launch_vjp_flux<T, PSA_FLUX_BURGERS_UPWIND>(rstage, a, s);
+  return launch_vjp_flux<T, PSA_FLUX_ADVECTION_UPWIND>(rstage, a, s);
+}
+
+int check_vjp_desc(const psa_solver_desc *d) {
+  if (d == nullptr) return set_error(PSA_ERR_INVALID_ARGUMENT, "null desc");
+  if (d->dtype != PSA_FLOAT64 && d->dtype != PSA_FLOAT32)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported dtype");
+  if (d->g < 3 || d->n < d->g)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "bad grid size");
+  if (d->flux < 0 || d->flux > 2)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported flux");
+  if (d->flux == PSA_FLUX_RUSANOV && d->nu != nullptr)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the adjoint supports the Rusanov flux with alpha = 1");
+  if (d->flux == PSA_FLUX_ADVECTION_UPWIND && !d->velocity_is_const)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the adjoint supports advection with a constant velocity");
+  if (!(d->dx > 0.0))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "dx must be positive");
+  return PSA_OK;
+}
+
+size_t elem_size(const psa_solver_desc *d) {
+  return d->dtype == PSA_FLOAT64 ? sizeof(double) : sizeof(float);
+}
+
+int copy_async(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  hipError_t err = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
+  if (err != hipSuccess) return set_error(PSA_ERR_HIP, hipGetErrorString(err));
+  return PSA_OK;
+}
+
+struct AdjointLayout {
+  long long ncp;     // checkpoints at steps 0, every, 2 every, ...
+  long long nwin;    // arrays of the recompute window
+  size_t array_bytes;
+  size_t bytes;
+};
+
+AdjointLayout adjoint_layout(const psa_solver_desc *d, long long nsteps,
+                             int every) {
+  AdjointLayout L{};
+  const size_t m = static_cast<size_t>(d->n + 2 * static_cast<int64_t>(d->g));
+  L.array_bytes = ((m * elem_size(d) + 255) / 256) * 256;
+  L.ncp = nsteps / every + 1;
+  L.nwin = 3LL * every;
+  // checkpoints, window (x, k1, k2 per step), running state, 4 cotangents
+  L.bytes = static_cast<size_t>(L.ncp + L.nwin + 1 + 4) * L.array_bytes;
+  return L;
+}
+
+}  // namespace
+}  // namespace psa
+
+// defined in forward.hip
+extern "C" int psa_ssprk33_stage(const psa_solver_desc *desc, int stage,
+                                 const void *un, const void *uk, void *out,
+                                 void *control, const void *dt, void *stream);
+
+using namespace psa;
+
+extern "C" {
+
+int psa_ssprk33_stage_vjp(const psa_solver_desc *desc, int stage,
+                          const void *state, const void *mu, const void *lam,
+                          const void *lam_k2, void *out, const void *dt,
+                          void *stream) {
+  int st = check_vjp_desc(desc);
+  if (st != PSA_OK) return st;
+  if (stage < 0 || stage > 3)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "stage must be 0..3");
+  if (state == nullptr || mu == nullptr || out == nullptr ||
+      (stage > 0 && dt == nullptr) ||
+      (stage == 1 && (lam == nullptr || lam_k2 == nullptr)))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  if (out == mu || out == state || (stage == 1 && (out == lam || out == lam_k2)))
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "the output must not alias an input");
+  auto s = static_cast<hipStream_t>(stream);
+  if (desc->dtype == PSA_FLOAT64)
+    return launch_vjp<double>(desc, stage, state, mu, lam, lam_k2, out, dt, s);
+  return launch_vjp<float>(desc, stage, state, mu, lam, lam_k2, out, dt, s);
+}
+
+int psa_ssprk33_step_vjp(const psa_solver_desc *desc, const void *u,
+                         const void *lam, void *lam_out, void *work,
+                         size_t work_bytes, const void *dt, void *stream) {
+  int st = check_vjp_desc(desc);
+  if (st != PSA_OK) return st;
+  if (u == nullptr || lam == nullptr || lam_out == nullptr || work == nullptr ||
+      dt == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  const size_t m = static_cast<size_t>(desc->n + 2 * static_cast<int64_t>(desc->g));
+  const size_t ab = ((m * elem_size(desc) + 255) / 256) * 256;
+  if (work_bytes < 4 * ab)
+    return set_error(PSA_ERR_WORKSPACE, "step vjp needs 4 arrays of workspace");
+  if (lam_out == lam || lam_out == u)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "lam_out must not alias");
+  char *w = static_cast<char *>(work);
+  void *k1 = w, *k2 = w + ab, *lk2 = w + 2 * ab, *lk1 = w + 3 * ab;
+  // recompute the stages of the forward step
+  st = psa_ssprk33_stage(desc, 1, u, u, k1, nullptr, dt, stream);
+  if (st != PSA_OK) return st;
+  st = psa_ssprk33_stage(desc, 2, u, k1, k2, nullptr, dt, stream);
+  if (st != PSA_OK) return st;
+  st = psa_ssprk33_stage_vjp(desc, 3, k2, lam, nullptr, nullptr, lk2, dt, stream);
+  if (st != PSA_OK) return st;
+  st = psa_ssprk33_stage_vjp(desc, 2, k1, lk2, nullptr, nullptr, lk1, dt, stream);
+  if (st != PSA_OK) return st;
+  return psa_ssprk33_stage_vjp(desc, 1, u, lk1, lam, lk2, lam_out, dt, stream);
+}
+
+int psa_adjoint_workspace_bytes(const psa_solver_desc *desc, long long nsteps,
+                                int every, size_t *bytes) {
+  int st = check_vjp_desc(desc);
+  if (st != PSA_OK) return st;
+  if (bytes == nullptr || nsteps < 0 || every <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "bad arguments");
+  *bytes = adjoint_layout(desc, nsteps, every).bytes;
+  return PSA_OK;
+}
+
+int psa_ssprk33_forward_checkpointed(const psa_solver_desc *desc,
+                                     const void *u0, void *u_final,
+                                     long long nsteps, const void *dt,
+                                     int every, void *workspace,
+                                     size_t workspace_bytes, void *stream) {
+  int st = check_vjp_desc(desc);
+  if (st != PSA_OK) return st;
+  if (u0 == nullptr || u_final == nullptr || dt == nullptr ||
+      workspace == nullptr || nsteps < 0 || every <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "bad arguments");
+  const AdjointLayout L = adjoint_layout(desc, nsteps, every);
+  if (workspace_bytes < L.bytes)
+    return set_error(PSA_ERR_WORKSPACE, "adjoint workspace too small");
+  auto s = static_cast<hipStream_t>(stream);
+  const size_t m = static_cast<size_t>(desc->n + 2 * static_cast<int64_t>(desc->g));
+  const size_t bytes = m * elem_size(desc);
+  char *base = static_cast<char *>(workspace);
+  auto cp = [&](long long k) { return base + k * L.array_bytes; };
+  char *win = cp(L.ncp);
+  char *cur = win + L.nwin * L.array_bytes;
+  void *k1 = win, *k2 = win + L.array_bytes;
+
+  st = copy_async(cp(0), u0, bytes, s);
+  if (st != PSA_OK) return st;
+  st = copy_async(cur, u0, bytes, s);
+  if (st != PSA_OK) return st;
+  for (long long step = 0; step < nsteps; ++step) {
+    st = psa_ssprk33_stage(desc, 1, cur, cur, k1, nullptr, dt, stream);
+    if (st != PSA_OK) return st;
+    st = psa_ssprk33_stage(desc, 2, cur, k1, k2, nullptr, dt, stream);
+    if (st != PSA_OK) return st;
+    st = psa_ssprk33_stage(desc, 3, cur, k2, cur, nullptr, dt, stream);
+    if (st != PSA_OK) return st;
+    if ((step + 1) % every == 0) {
+      st = copy_async(cp((step + 1) / every), cur, bytes, s);
+      if (st != PSA_OK) return st;
+    }
+  }
+  return copy_async(u_final, cur, bytes, s);
+}
+
+int psa_ssprk33_adjoint(const psa_solver_desc *desc, long long nsteps,
+                        const void *dt, int every, const void *lam_final,
+                        void *grad_u0, void *workspace, size_t workspace_bytes,
+                        void *stream) {
+  int st = check_vjp_desc(desc);
+  if (st != PSA_OK) return st;
+  if (lam_final == nullptr || grad_u0 == nullptr || dt == nullptr ||
+      workspace == nullptr || nsteps < 0 || every <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "bad arguments");
+  const AdjointLayout L = adjoint_layout(desc, nsteps, every);
+  if (workspace_bytes < L.bytes)
+    return set_error(PSA_ERR_WORKSPACE, "adjoint workspace too small");
+  auto s = static_cast<hipStream_t>(stream);
+  const size_t m = static_cast<size_t>(desc->n + 2 * static_cast<int64_t>(desc->g));
+  const size_t bytes = m * elem_size(desc);
+  char *base = static_cast<char *>(workspace);
+  auto cp = [&](long long k) { return base + k * L.array_bytes; };
+  char *win = cp(L.ncp);
+  auto X = [&](long long i) { return win + (3 * i) * L.array_bytes; };
+  auto K1 = [&](long long i) { return win + (3 * i + 1) * L.array_bytes; };
+  auto K2 = [&](long long i) { return win + (3 * i + 2) * L.array_bytes; };
+  char *lam_a = win + L.nwin * L.array_bytes + L.array_bytes;
+  char *lam_b = lam_a + L.array_bytes;
+  char *lk2 = lam_b + L.array_bytes;
+  char *lk1 = lk2 + L.array_bytes;
+
+  st = copy_async(lam_a, lam_final, bytes, s);
+  if (st != PSA_OK) return st;
+  char *lam = lam_a, *lam_next = lam_b;
+  const long long nwin = (nsteps + every - 1) / every;
+  for (long long w = nwin - 1; w >= 0; --w) {
+    const long long s0 = w * every;
+    const long long s1 = (s0 + every < nsteps) ? s0 + every : nsteps;
+    // recompute the window from its checkpoint, keeping every stage state
+    st = copy_async(X(0), cp(w), bytes, s);
+    if (st != PSA_OK) return st;
+    for (long long step = s0; step < s1; ++step) {
+      const long long i = step - s0;
+      st = psa_ssprk33_stage(desc, 1, X(i), X(i), K1(i), nullptr, dt, stream);
+      if (st != PSA_OK) return st;
+      st = psa_ssprk33_stage(desc, 2, X(i), K1(i), K2(i), nullptr, dt, stream);
+      if (st != PSA_OK) return st;
+      if (step + 1 < s1) {
+        st = psa_ssprk33_stage(desc, 3, X(i), K2(i), X(i + 1), nullptr, dt,
+                               stream);
+        if (st != PSA_OK) return st;
+      }
+    }
+    // reverse sweep through the window
+    for (long long step = s1 - 1; step >= s0; --step) {
+      const long long i = step - s0;
+      st = psa_ssprk33_stage_vjp(desc, 3, K2(i), lam, nullptr, nullptr, lk2, dt,
+                                 stream);
+      if (st != PSA_OK) return st;
+      st = psa_ssprk33_stage_vjp(desc, 2, K1(i), lk2, nullptr, nullptr, lk1, dt,
+                                 stream);
+      if (st != PSA_OK) return st;
+      st = psa_ssprk33_stage_vjp(desc, 1, X(i), lk1, lam, lk2, lam_next, dt,
+                                 stream);
+      if (st != PSA_OK) return st;
+      char *tmp = lam;
+      lam = lam_next;
+      lam_next = tmp;
+    }
+  }
+  st = copy_async(grad_u0, lam, bytes, s);
+  if (st != PSA_OK) return st;
+  if (nsteps == 0) {
+    // no step: the gradient is the incoming cotangent with zero ghosts
+    const int64_t mm = static_cast<int64_t>(m);
+    if (desc->dtype == PSA_FLOAT64) {
+      hipLaunchKernelGGL(k_zero_ghosts<double>, dim3(1), dim3(64), 0, s,
+                         static_cast<double *>(grad_u0), mm, desc->g);
+    } else {
+      hipLaunchKernelGGL(k_zero_ghosts<float>, dim3(1), dim3(64), 0, s,
+                         static_cast<float *>(grad_u0), mm, desc->g);
+    }
+    return check_launch("zero ghosts");
+  }
+  return PSA_OK;
+}
+
+}  // extern "C"
