@@ -55,7 +55,25 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
                    help="steps of the per-kernel event-timed pass")
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="target duration of the CPU baseline sample (0: skip)")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run a plain device copy of N FP64 values (N * 8 "
+                        "bytes read and written per launch), the reference "
+                        "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     return p.parse_args(argv)
+
+
+def copy_calibration(args: argparse.Namespace) -> int:
+    import torch
+
+    a = torch.randn(args.n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(max(1, args.steps)):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": True, "n": args.n,
+                      "bytes_read_per_launch": 8 * args.n,
+                      "bytes_written_per_launch": 8 * args.n}), flush=True)
+    return 0
 
 
 def make_problem(args: argparse.Namespace):
@@ -198,6 +216,8 @@ def cpu_leg(args: argparse.Namespace, gpu: dict) -> dict | None:
 
 def main(argv: list[str] | None = None) -> int:
     args = parse_args(argv)
+    if args.copy_calibration:
+        return copy_calibration(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the workload is a single-GPU instance, see DESIGN.md for how "
