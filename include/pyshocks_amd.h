@@ -168,9 +168,7 @@ typedef struct psa_solver_desc {
   const void *velocity;         /* advection velocity (m) if not const     */
   const void *nu;               /* Rusanov df^(alpha-1) (m - 1) or NULL    */
   int block;                    /* threads per workgroup (0 = tuned)       */
-  int cells_per_thread;         /* cells per thread (0 = tuned); negative:
-                                   persistent workgroups that prefetch
-                                   their next tile                         */
+  int cells_per_thread;         /* cells per thread (0 = tuned)            */
 } psa_solver_desc;
 
 /* Initialise the control block (device memory, sizeof(psa_control)) for a
@@ -204,6 +202,40 @@ int psa_ssprk33_stage(const psa_solver_desc *desc, int stage, const void *un,
 int psa_ssprk33_advance(const psa_solver_desc *desc, const void *u,
                         void *out, void *k1, void *k2, const void *dt,
                         void *stream);
+
+/* Batched solver ------------------------------------------------------------
+ * ninst independent periodic Burgers instances (Rusanov with alpha = 1, or
+ * Godunov) of uneven size, each advanced from t = 0 to its own tfinal with
+ * its own CFL time step (timestepping.py:89-152) in one persistent launch:
+ * every workgroup takes the next instance from `order` (an atomic counter)
+ * and runs its whole time loop; the launch is graph-capturable (the counter
+ * is reset by a kernel). Instance i occupies cells [offsets[i],
+ * offsets[i] + ncells[i] + 2 g) of the packed arrays u, k1, k2, and its
+ * offset must be a multiple of 16 bytes. All arrays in the descriptor are
+ * device pointers. Outputs per instance: final time, steps taken, status
+ * (PSA_OK; PSA_ERR_NONFINITE_DT for a non-finite dt or one that no longer
+ * advances t; PSA_ERR_INVALID_ARGUMENT when max_steps is reached). */
+typedef struct psa_batch_desc {
+  int dtype;                    /* PSA_FLOAT64 / PSA_FLOAT32               */
+  int flux;                     /* PSA_FLUX_RUSANOV / _BURGERS_UPWIND      */
+  int g;                        /* ghost cells per side (>= 3)             */
+  int grid;                     /* workgroups (0 = resident capacity)      */
+  long long ninst;              /* instances                               */
+  const long long *offsets;     /* [ninst] first cell of each instance     */
+  const int *ncells;            /* [ninst] interior cells                  */
+  const double *dx;             /* [ninst] cell size                       */
+  const double *tfinal;         /* [ninst] final time                      */
+  const int *order;             /* [ninst] processing order                */
+  double eps;                   /* WENO-JS epsilon                         */
+  double theta;                 /* Courant factor                          */
+  long long max_steps;          /* step limit per instance                 */
+  int block;                    /* threads per workgroup (0 = default)     */
+  int cells_per_thread;         /* cells per thread (0 = default)          */
+} psa_batch_desc;
+
+int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1, void *k2,
+                    unsigned int *counter, double *t_out, long long *steps_out,
+                    int *status_out, void *stream);
 
 /* Discrete adjoint --------------------------------------------------------
  * Replaces the dense jax.jacfwd Jacobian of timestepping.py:155-215

@@ -79,6 +79,26 @@ class PsaSolverDesc(ctypes.Structure):
     ]
 
 
+class PsaBatchDesc(ctypes.Structure):
+    _fields_ = [
+        ("dtype", ctypes.c_int),
+        ("flux", ctypes.c_int),
+        ("g", ctypes.c_int),
+        ("grid", ctypes.c_int),
+        ("ninst", ctypes.c_longlong),
+        ("offsets", ctypes.c_void_p),
+        ("ncells", ctypes.c_void_p),
+        ("dx", ctypes.c_void_p),
+        ("tfinal", ctypes.c_void_p),
+        ("order", ctypes.c_void_p),
+        ("eps", ctypes.c_double),
+        ("theta", ctypes.c_double),
+        ("max_steps", ctypes.c_longlong),
+        ("block", ctypes.c_int),
+        ("cells_per_thread", ctypes.c_int),
+    ]
+
+
 _VP = ctypes.c_void_p
 _I = ctypes.c_int
 _I64 = ctypes.c_int64
@@ -110,6 +130,8 @@ SIGNATURES: dict[str, list[Any]] = {
                             _VP, _VP],
     "psa_ssprk33_stage": [ctypes.POINTER(PsaSolverDesc), _I, _VP, _VP, _VP, _VP,
                           _VP, _VP],
+    "psa_batch_solve": [ctypes.POINTER(PsaBatchDesc), _VP, _VP, _VP, _VP, _VP, _VP,
+                        _VP, _VP],
     "psa_ssprk33_stage_vjp": [ctypes.POINTER(PsaSolverDesc), _I, _VP, _VP, _VP,
                               _VP, _VP, _VP, _VP],
     "psa_ssprk33_step_vjp": [ctypes.POINTER(PsaSolverDesc), _VP, _VP, _VP, _VP,

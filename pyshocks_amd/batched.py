@@ -1,0 +1,210 @@
+"""Batched request front end: many independent periodic Burgers instances of
+uneven size advanced in one persistent launch.
+
+Each instance ``i`` is a uniform periodic grid of ``ncells[i]`` cells on
+``[a, b]`` (3 ghost cells per side) with its own initial state and final time,
+advanced by the fused WENO-JS5 / Rusanov (or Godunov) / SSP-RK3 step with its
+own CFL time step -- the loop of ``timestepping.py:89-152`` per instance, as
+the reference harness would run it for each request separately. The instances
+are packed into three device arrays (state and two stage buffers) at 16-byte
+aligned offsets; the library runs them longest-first from an atomic queue,
+one workgroup per instance at a time (``psa_batch_solve``). The launch can be
+captured in a HIP graph and replayed (:meth:`BatchedSolver.capture`).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from pyshocks_amd import _lib
+
+ALIGN = 64  # elements; keeps every instance 16-byte aligned in FP32 and FP64
+
+
+@dataclass
+class BatchResult:
+    t: np.ndarray
+    """Final time of each instance."""
+    steps: np.ndarray
+    """Steps taken by each instance."""
+    status: np.ndarray
+    """Per-instance status (0 = reached its final time)."""
+
+
+class BatchedSolver:
+    """Packed storage and launch parameters of a batch of instances."""
+
+    def __init__(self, ncells: Sequence[int], *, dtype: torch.dtype = torch.float32,
+                 scheme: str = "rusanov", eps: float = 1.0e-6, theta: float = 1.0,
+                 a: float = 0.0, b: float = 1.0, nghosts: int = 3,
+                 device: torch.device | str = "cuda",
+                 tiles: tuple[int, int] | None = None, grid: int = 0) -> None:
+        if scheme not in ("rusanov", "godunov"):
+            raise NotImplementedError(
+                f"the batched solver takes Burgers 'rusanov' or 'godunov', not {scheme!r}")
+        if nghosts < 3:
+            raise ValueError("WENO-JS5 needs nghosts >= 3")
+        ncells = np.asarray(ncells, dtype=np.int64)
+        if ncells.ndim != 1 or ncells.size == 0 or np.any(ncells < nghosts):
+            raise ValueError("every instance needs at least nghosts interior cells")
+
+        self.dtype = dtype
+        self.device = torch.device(device)
+        self.g = nghosts
+        self.ncells = ncells
+        self.m = ncells + 2 * nghosts
+        self.h = (b - a) / ncells.astype(np.float64)
+        self.a = a
+        self.b = b
+        sizes = ((self.m + ALIGN - 1) // ALIGN) * ALIGN
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        self.total = int(np.sum(sizes))
+
+        self.u = torch.zeros(self.total, dtype=dtype, device=self.device)
+        self.k1 = torch.zeros_like(self.u)
+        self.k2 = torch.zeros_like(self.u)
+
+        dev = self.device
+        self._offsets = torch.tensor(self.offsets, dtype=torch.int64, device=dev)
+        self._ncells = torch.tensor(ncells.astype(np.int32), dtype=torch.int32, device=dev)
+        self._dx = torch.tensor(self.h, dtype=torch.float64, device=dev)
+        # longest first: the step count of an instance grows like its cell count
+        order = np.argsort(-ncells, kind="stable").astype(np.int32)
+        self._order = torch.tensor(order, dtype=torch.int32, device=dev)
+        self._tfinal = torch.zeros(ncells.size, dtype=torch.float64, device=dev)
+        self._counter = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.t_out = torch.zeros(ncells.size, dtype=torch.float64, device=dev)
+        self.steps_out = torch.zeros(ncells.size, dtype=torch.int64, device=dev)
+        self.status_out = torch.zeros(ncells.size, dtype=torch.int32, device=dev)
+
+        self.desc = _lib.PsaBatchDesc()
+        self.desc.dtype = _lib.dtype_code(dtype)
+        self.desc.flux = _lib.PSA_FLUX_RUSANOV if scheme == "rusanov" \
+            else _lib.PSA_FLUX_BURGERS_UPWIND
+        self.desc.g = nghosts
+        self.desc.grid = int(grid)
+        self.desc.ninst = int(ncells.size)
+        self.desc.offsets = self._offsets.data_ptr()
+        self.desc.ncells = self._ncells.data_ptr()
+        self.desc.dx = self._dx.data_ptr()
+        self.desc.tfinal = self._tfinal.data_ptr()
+        self.desc.order = self._order.data_ptr()
+        self.desc.eps = float(eps)
+        self.desc.theta = float(theta)
+        self.desc.max_steps = 4_000_000
+        if tiles is None:
+            from pyshocks_amd.autotune import lookup_batch
+
+            tiles = lookup_batch(self.desc.dtype, self.desc.flux)
+        if tiles is not None:
+            self.desc.block, self.desc.cells_per_thread = int(tiles[0]), int(tiles[1])
+
+    @property
+    def ninst(self) -> int:
+        return int(self.ncells.size)
+
+    def state(self, i: int) -> torch.Tensor:
+        """View of instance ``i`` (``ncells[i] + 2 g`` cells)."""
+        off = int(self.offsets[i])
+        return self.u[off: off + int(self.m[i])]
+
+    def set_states(self, states: Sequence[np.ndarray | torch.Tensor]) -> None:
+        """Copy initial states (each of ``ncells[i] + 2 g`` cells) into the
+        packed array with one host-to-device transfer."""
+        np_dtype = np.float64 if self.dtype == torch.float64 else np.float32
+        host = np.zeros(self.total, dtype=np_dtype)
+        for i, s in enumerate(states):
+            s = s.detach().cpu().numpy() if isinstance(s, torch.Tensor) else np.asarray(s)
+            if s.size != self.m[i]:
+                raise ValueError(f"instance {i}: expected {self.m[i]} cells, got {s.size}")
+            off = int(self.offsets[i])
+            host[off: off + s.size] = s
+        self.u.copy_(torch.from_numpy(host))
+
+    def set_tfinal(self, tfinal: float | Sequence[float]) -> None:
+        tf = np.broadcast_to(np.asarray(tfinal, dtype=np.float64), (self.ninst,))
+        self._tfinal.copy_(torch.from_numpy(np.ascontiguousarray(tf)))
+
+    def launch(self) -> None:
+        """Enqueue the batched solve on the current stream (no sync)."""
+        _lib.call("psa_batch_solve", ctypes.byref(self.desc), self.u.data_ptr(),
+                  self.k1.data_ptr(), self.k2.data_ptr(), self._counter.data_ptr(),
+                  self.t_out.data_ptr(), self.steps_out.data_ptr(),
+                  self.status_out.data_ptr(), _lib.stream_of(self.u))
+
+    def solve(self, tfinal: float | Sequence[float]) -> BatchResult:
+        """Advance every instance from t = 0 to ``tfinal`` (per instance)."""
+        self.set_tfinal(tfinal)
+        self.launch()
+        return self.result()
+
+    def result(self) -> BatchResult:
+        return BatchResult(t=self.t_out.cpu().numpy(), steps=self.steps_out.cpu().numpy(),
+                           status=self.status_out.cpu().numpy())
+
+    def capture(self, u0: torch.Tensor, tfinal: float | Sequence[float]) -> torch.cuda.CUDAGraph:
+        """A graph that restores the packed initial state ``u0`` (a device
+        copy, no memset) and runs the batched solve; replay it with
+        ``graph.replay()``."""
+        self.set_tfinal(tfinal)
+        if u0.shape != self.u.shape or u0.dtype != self.dtype:
+            raise ValueError("u0 must be a packed state of this batch")
+        graph = torch.cuda.CUDAGraph()
+        stream = torch.cuda.Stream(device=self.device)
+        stream.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(stream):
+            # warm up outside capture so lazy initialisation is not recorded
+            self.u.copy_(u0)
+            self.launch()
+        torch.cuda.current_stream(self.device).wait_stream(stream)
+        torch.cuda.synchronize(self.device)
+        with torch.cuda.graph(graph):
+            self.u.copy_(u0)
+            self.launch()
+        return graph
+
+    def cell_updates(self) -> float:
+        """Sum over instances of interior cells x steps of the last solve."""
+        return float(np.sum(self.ncells * self.steps_out.cpu().numpy()))
+
+
+# {{{ the uneven-size workload
+
+
+def random_instances(ninst: int, *, nmin: int = 128, nmax: int = 65536,
+                     seed: int = 0) -> tuple[np.ndarray, list]:
+    """``ninst`` sizes drawn log-uniformly from ``[nmin, nmax]`` and initial
+    conditions ``c + sum_{k=1..8} A_k sin(2 pi k x + phi_k)`` with
+    ``c ~ U[0, 1]``, ``A_k ~ U[-1, 1] / k`` and ``phi_k ~ U[0, 2 pi)``."""
+    from pyshocks_amd.funcs import SineStep
+
+    rng = np.random.default_rng(seed)
+    ns = np.round(np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=ninst))).astype(np.int64)
+    ics = []
+    for _ in range(ninst):
+        c = rng.uniform(0.0, 1.0)
+        amps = rng.uniform(-1.0, 1.0, size=8) / np.arange(1, 9)
+        phis = rng.uniform(0.0, 2.0 * np.pi, size=8)
+        ics.append(SineStep(c=float(c), amplitudes=tuple(float(x) for x in amps),
+                            wavenumbers=tuple(range(1, 9)),
+                            phases=tuple(float(x) for x in phis)))
+    return ns, ics
+
+
+def exact_averages_np(ic, n: int, nghosts: int = 3) -> np.ndarray:
+    """Exact cell averages of a sine-sum IC on a uniform periodic grid of
+    ``n`` cells on ``[0, 1]`` (ghosts included), in FP64 on the host."""
+    h = 1.0 / n
+    f = h * np.arange(-nghosts, n + nghosts + 1, dtype=np.float64)
+    w = ic.c * f
+    for a, k, p in zip(ic.amplitudes, ic.wavenumbers, ic.phases):
+        w = w + a * (np.cos(p) - np.cos(2.0 * np.pi * k * f + p)) / (2.0 * np.pi * k)
+    return (w[1:] - w[:-1]) / h
+
+
+# }}}

@@ -1,0 +1,430 @@
+// Device building blocks of the fused stage computation, shared by the
+// single-instance stage kernels (forward.hip) and the batched solver
+// (batched.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/pyshocks_amd.h"
+#include "weno5_device.hpp"
+
+namespace psa {
+
+// ---------------------------------------------------------------------------
+// loads
+
+// Value of cell gm of the periodic ghost-filled array (scalar.py:529-540):
+// ghost cells read their interior image; outside [0, m) the 'same'
+// convolution pads with zeros (convolve.py:113-114).
+template <typename T>
+__device__ __forceinline__ T load_filled(const T *__restrict__ u, int64_t gm,
+                                         int64_t m, int64_t n, int g) {
+  if (gm < 0 || gm >= m) return T(0);
+  int64_t idx = gm;
+  if (idx < g) {
+    idx += n;
+  } else if (idx >= m - g) {
+    idx -= n;
+  }
+  return u[idx];
+}
+
+// Value of cell gm of an array used as given, zero outside [0, m).
+template <typename T>
+__device__ __forceinline__ T load_raw(const T *__restrict__ u, int64_t gm,
+                                      int64_t m) {
+  return (gm >= 0 && gm < m) ? u[gm] : T(0);
+}
+
+// Non-negative value -> order-preserving bit pattern (NaN sorts above inf).
+__device__ __forceinline__ unsigned long long abs_bits(double v) {
+  return static_cast<unsigned long long>(__double_as_longlong(fabs(v)));
+}
+__device__ __forceinline__ unsigned long long abs_bits(float v) {
+  return static_cast<unsigned long long>(
+      static_cast<unsigned int>(__float_as_uint(fabsf(v))));
+}
+template <typename T>
+__device__ __forceinline__ T from_bits(unsigned long long b);
+template <>
+__device__ __forceinline__ double from_bits<double>(unsigned long long b) {
+  return __longlong_as_double(static_cast<long long>(b));
+}
+template <>
+__device__ __forceinline__ float from_bits<float>(unsigned long long b) {
+  return __uint_as_float(static_cast<unsigned int>(b));
+}
+
+template <int BLOCK>
+__device__ __forceinline__ unsigned long long block_max_bits(
+    unsigned long long v, unsigned long long *s_red) {
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (lane == 0) s_red[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BLOCK / 64; ++w) v = s_red[w] > v ? s_red[w] : v;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// fused tiled stage kernel
+
+template <typename T>
+struct StageArgs {
+  const T *un;      // u^n (stages 2 and 3)
+  const T *uk;      // stage input: the stencil is evaluated on it
+  T *out;           // stage output
+  const T *vel;     // advection velocity (VEL == 2)
+  const T *nu;      // Rusanov df^(alpha-1) per face j - 1, or null
+  psa_control *ctrl;
+  const T *dt_ptr;  // used when ctrl is null
+  int64_t m;
+  int64_t n;
+  int g;
+  int aligned;      // uk, un, out, vel are 16-byte aligned
+  T dx;
+  T inv_dx;
+  T eps;
+  T vconst;
+};
+
+template <typename T, int VEL>
+__device__ __forceinline__ T velocity_at(const T *s_v, int j, T vconst) {
+  if constexpr (VEL == 2) {
+    return s_v[j];
+  } else {
+    return vconst;
+  }
+}
+
+// 16-byte vectors of the dtype
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int n = 2;
+};
+template <>
+struct Vec16<float> {
+  using type = float4;
+  static constexpr int n = 4;
+};
+
+template <typename T>
+__device__ __forceinline__ typename Vec16<T>::type load16(const T *p) {
+  return *reinterpret_cast<const typename Vec16<T>::type *>(p);
+}
+template <typename T>
+__device__ __forceinline__ void store16(T *p, typename Vec16<T>::type v) {
+  *reinterpret_cast<typename Vec16<T>::type *>(p) = v;
+}
+__device__ __forceinline__ unsigned long long vmax_bits(double2 v,
+                                                        unsigned long long m) {
+  const unsigned long long a = abs_bits(v.x), b = abs_bits(v.y);
+  m = a > m ? a : m;
+  return b > m ? b : m;
+}
+__device__ __forceinline__ unsigned long long vmax_bits(float4 v,
+                                                        unsigned long long m) {
+  const unsigned long long a = abs_bits(v.x), b = abs_bits(v.y);
+  const unsigned long long c = abs_bits(v.z), d = abs_bits(v.w);
+  m = a > m ? a : m;
+  m = b > m ? b : m;
+  m = c > m ? c : m;
+  return d > m ? d : m;
+}
+
+// COUNT 16-byte chunks moved between global memory and LDS by BLOCK
+// threads: every load of a thread is issued before its first LDS write, so a
+// wave keeps all of its loads in flight at once.
+template <typename T, int BLOCK, int COUNT>
+struct Chunks {
+  using V = typename Vec16<T>::type;
+  static constexpr int VN = Vec16<T>::n;
+  static constexpr int CH = (COUNT + BLOCK - 1) / BLOCK;
+  V r[CH];
+
+  __device__ __forceinline__ void load(const T *src, int tid) {
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int c = tid + k * BLOCK;
+      if (k < CH - 1 || COUNT % BLOCK == 0 || c < COUNT) r[k] = load16(src + c * VN);
+    }
+  }
+  __device__ __forceinline__ void store(T *dst, int tid) const {
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int c = tid + k * BLOCK;
+      if (k < CH - 1 || COUNT % BLOCK == 0 || c < COUNT) store16(dst + c * VN, r[k]);
+    }
+  }
+};
+
+
+// LDS of one stage tile.
+template <typename T, int VEL, int BLOCK, int CPT>
+struct StageSmem {
+  static constexpr int TILE = BLOCK * CPT;
+  static constexpr int HALO = 4;
+  static constexpr int NV = VEL == 2 ? TILE + 2 * HALO : 1;
+  static constexpr int NVX = VEL == 2 ? BLOCK + 1 : 1;
+  __align__(16) T u[TILE + 2 * HALO];
+  __align__(16) T v[NV];
+  __align__(16) T io[TILE];
+  T urL[BLOCK + 1];
+  T ulF[BLOCK + 1];
+  T arL[NVX];
+  T alF[NVX];
+  unsigned long long red[BLOCK / 64];
+};
+
+// One tile of one stage: cells [S, S + BLOCK * CPT) of the arrays of p.
+//
+// VEL: 0 Burgers (no velocity), 1 constant velocity, 2 velocity array.
+// STAGE: 0 operator (all m cells, faces 0 and m padded with zero flux),
+//        1..3 SSP-RK3 stages (interior cells, ghost images rewritten).
+//
+// The workgroup stages the tile plus a 4-cell halo in LDS, every thread
+// reconstructs CPT consecutive cells (both sides from one set of smoothness
+// indicators), neighbouring threads exchange one right and one left state
+// through LDS, each thread evaluates its CPT + 1 faces, applies the
+// conservative difference and the stage combination, and the results leave
+// through LDS as coalesced stores. For STAGE 3, vmax accumulates the bit
+// pattern of max |u| of the cells written. Ends with the LDS free for reuse
+// only after the caller's next barrier.
+template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT>
+__device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
+                                           T dt,
+                                           StageSmem<T, VEL, BLOCK, CPT> &sm,
+                                           unsigned long long &vmax) {
+  constexpr int TILE = BLOCK * CPT;
+  constexpr int HALO = 4;
+  constexpr int VN = Vec16<T>::n;
+  const int tid = threadIdx.x;
+  const int64_t m = p.m;
+  const int64_t n = p.n;
+  const int g = p.g;
+
+  // A tile whose cells and halo are interior needs neither the ghost
+  // mapping nor bounds checks, and (STAGE > 0) writes no ghost images.
+  const bool fast = p.aligned != 0 && STAGE > 0 && S - HALO >= g &&
+                    S + TILE + HALO <= m - g;
+
+  // ---- stage the tile and its halo
+  if (fast) {
+    // all loads of the tile, the velocity and u^n first, then the LDS writes
+    Chunks<T, BLOCK, (TILE + 2 * HALO) / VN> cu;
+    Chunks<T, BLOCK, VEL == 2 ? (TILE + 2 * HALO) / VN : 1> cv;
+    Chunks<T, BLOCK, STAGE >= 2 ? TILE / VN : 1> cn;
+    cu.load(p.uk + S - HALO, tid);
+    if constexpr (VEL == 2) cv.load(p.vel + S - HALO, tid);
+    if constexpr (STAGE >= 2) cn.load(p.un + S, tid);
+    cu.store(sm.u, tid);
+    if constexpr (VEL == 2) cv.store(sm.v, tid);
+    if constexpr (STAGE >= 2) cn.store(sm.io, tid);
+  } else {
+    for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {
+      sm.u[j] = load_filled(p.uk, S - HALO + j, m, n, g);
+    }
+    if constexpr (VEL == 2) {
+      for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {
+        sm.v[j] = load_raw(p.vel, S - HALO + j, m);
+      }
+    }
+    if constexpr (STAGE >= 2) {
+      for (int j = tid; j < TILE; j += BLOCK) {
+        const int64_t gm = S + j;
+        sm.io[j] = gm < m ? p.un[gm] : T(0);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- reconstruct CPT consecutive cells
+  using C = Weno5Coeffs<T>;
+  const int l0 = tid * CPT;
+  T w[CPT + 4];  // cells l0 - 2 .. l0 + CPT + 1
+#pragma unroll
+  for (int k = 0; k < CPT + 4; ++k) w[k] = sm.u[l0 + 2 + k];
+
+  // a0 * (second difference)^2 at cells l0 - 1 .. l0 + CPT, shared by the
+  // three indicators of neighbouring cells
+  T E[CPT + 2];
+#pragma unroll
+  for (int k = 0; k < CPT + 2; ++k) {
+    const T d2 = (w[k + 2] - T(2) * w[k + 1]) + w[k];
+    E[k] = C::a0 * (d2 * d2);
+  }
+
+  T ul[CPT], ur[CPT];
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) {
+    const T um2 = w[r], um1 = w[r + 1], u0 = w[r + 2], up1 = w[r + 3],
+            up2 = w[r + 4];
+    const T t01 = (T(3) * u0 - T(4) * um1) + um2;
+    const T t11 = up1 - um1;
+    const T t21 = (up2 - T(4) * up1) + T(3) * u0;
+    const T b0 = E[r] + C::a1 * (t01 * t01);
+    const T b1 = E[r + 1] + C::a1 * (t11 * t11);
+    const T b2 = E[r + 2] + C::a1 * (t21 * t21);
+    weno_pair_from_beta(um2, um1, u0, up1, up2, b0, b1, b2, p.eps, ul[r],
+                        ur[r]);
+  }
+
+  T al[VEL == 2 ? CPT : 1], ar[VEL == 2 ? CPT : 1];
+  if constexpr (VEL == 2) {
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+      const int b = l0 + HALO + r;
+      weno_pair(sm.v[b - 2], sm.v[b - 1], sm.v[b], sm.v[b + 1], sm.v[b + 2],
+                p.eps, al[r], ar[r]);
+    }
+  }
+
+  // ---- exchange edge states with the neighbouring threads
+  sm.urL[tid + 1] = ur[CPT - 1];
+  sm.ulF[tid] = ul[0];
+  if constexpr (VEL == 2) {
+    sm.arL[tid + 1] = ar[CPT - 1];
+    sm.alF[tid] = al[0];
+  }
+  if (tid == 0) {
+    // right state of local cell -1
+    sm.urL[0] = weno_right(sm.u[1], sm.u[2], sm.u[3], sm.u[4], sm.u[5], p.eps);
+    if constexpr (VEL == 2) {
+      sm.arL[0] = weno_right(sm.v[1], sm.v[2], sm.v[3], sm.v[4], sm.v[5], p.eps);
+    }
+  }
+  if (tid == BLOCK - 1) {
+    // left state of local cell TILE (mirrored stencil)
+    constexpr int c = TILE + HALO;
+    sm.ulF[BLOCK] = weno_right(sm.u[c + 2], sm.u[c + 1], sm.u[c], sm.u[c - 1],
+                               sm.u[c - 2], p.eps);
+    if constexpr (VEL == 2) {
+      sm.alF[BLOCK] = weno_right(sm.v[c + 2], sm.v[c + 1], sm.v[c],
+                                 sm.v[c - 1], sm.v[c - 2], p.eps);
+    }
+  }
+  __syncthreads();
+
+  // ---- faces l0 - 1/2 .. l0 + CPT - 1/2
+  T F[CPT + 1];
+  const bool has_nu = FLUX == PSA_FLUX_RUSANOV && p.nu != nullptr;
+#pragma unroll
+  for (int k = 0; k <= CPT; ++k) {
+    const T urL = k == 0 ? sm.urL[tid] : ur[k - 1];
+    const T ulR = k == CPT ? sm.ulF[tid + 1] : ul[k];
+    T Fk;
+    if constexpr (FLUX == PSA_FLUX_RUSANOV) {
+      if (has_nu) {
+        const int64_t gm = S + l0 + k;  // face between cells gm - 1 and gm
+        const T nuk = (gm >= 1 && gm <= m - 1) ? p.nu[gm - 1] : T(1);
+        Fk = flux_rusanov(urL, ulR, w[k + 1], w[k + 2], nuk);
+      } else {
+        Fk = flux_rusanov_nu1(urL, ulR, w[k + 1], w[k + 2]);
+      }
+    } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
+      Fk = flux_burgers_upwind(urL, ulR);
+    } else {
+      T arL, alR;
+      if constexpr (VEL == 2) {
+        arL = k == 0 ? sm.arL[tid] : ar[k - 1];
+        alR = k == CPT ? sm.alF[tid + 1] : al[k];
+      } else {
+        // a constant velocity reconstructs to itself up to rounding, so the
+        // upwind switch only sees its sign
+        arL = p.vconst;
+        alR = p.vconst;
+      }
+      Fk = flux_advection_upwind(urL, ulR, arL, alR);
+    }
+    if constexpr (STAGE == 0) {
+      const int64_t gm = S + l0 + k;
+      if (gm <= 0 || gm >= m) Fk = T(0);
+    }
+    F[k] = Fk;
+  }
+
+  // ---- conservative difference and stage combination
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) {
+    const T dF = F[r + 1] - F[r];
+    T L;
+    if constexpr (STAGE == 0) {
+      if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
+        L = (-velocity_at<T, VEL>(sm.v, l0 + HALO + r, p.vconst) * dF) / p.dx;
+      } else {
+        L = -dF / p.dx;
+      }
+    } else {
+      if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
+        L = (-velocity_at<T, VEL>(sm.v, l0 + HALO + r, p.vconst) * dF) *
+            p.inv_dx;
+      } else {
+        L = -dF * p.inv_dx;
+      }
+    }
+    const T uk = w[r + 2];
+    T val;
+    if constexpr (STAGE == 0) {
+      val = L;
+    } else if constexpr (STAGE == 1) {
+      val = uk + dt * L;
+    } else if constexpr (STAGE == 2) {
+      val = T(3.0 / 4.0) * sm.io[l0 + r] + T(1.0 / 4.0) * (uk + dt * L);
+    } else {
+      val = T(1.0 / 3.0) * sm.io[l0 + r] + T(2.0 / 3.0) * (uk + dt * L);
+    }
+    sm.io[l0 + r] = val;
+  }
+  __syncthreads();
+
+  // ---- coalesced stores (+ ghost images, + max|u| of the new state)
+  if (fast) {
+    Chunks<T, BLOCK, TILE / VN> co;
+#pragma unroll
+    for (int k = 0; k < co.CH; ++k) {
+      const int c = tid + k * BLOCK;
+      if (k < co.CH - 1 || (TILE / VN) % BLOCK == 0 || c < TILE / VN)
+        co.r[k] = load16(sm.io + c * VN);
+    }
+    co.store(p.out + S, tid);
+    if constexpr (STAGE == 3) {
+#pragma unroll
+      for (int k = 0; k < co.CH; ++k) {
+        const int c = tid + k * BLOCK;
+        if (k < co.CH - 1 || (TILE / VN) % BLOCK == 0 || c < TILE / VN)
+          vmax = vmax_bits(co.r[k], vmax);
+      }
+    }
+  } else {
+    for (int j = tid; j < TILE; j += BLOCK) {
+      const int64_t gm = S + j;
+      if constexpr (STAGE == 0) {
+        if (gm < m) p.out[gm] = sm.io[j];
+      } else {
+        if (gm >= g && gm < m - g) {
+          const T v = sm.io[j];
+          p.out[gm] = v;
+          if (gm < 2 * g) p.out[gm + n] = v;
+          if (gm >= m - 2 * g) p.out[gm - n] = v;
+          if constexpr (STAGE == 3) {
+            const unsigned long long b = abs_bits(v);
+            vmax = b > vmax ? b : vmax;
+          }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace psa

@@ -259,6 +259,10 @@ def fused_plan(stepper: Stepper, *,
     if sp is None:
         return None
     desc, keep = sp
+    if tiles is None:
+        from pyshocks_amd.autotune import lookup_stage
+
+        tiles = lookup_stage(desc)
     if tiles is not None:
         desc.block, desc.cells_per_thread = int(tiles[0]), int(tiles[1])
     source: BoundOperator = stepper.source  # type: ignore[assignment]

@@ -1,0 +1,140 @@
+"""GPU tests of the batched persistent solver: every instance must take the
+same steps as the single-instance fused solve and agree with it to 1e-12
+(FP64; the tile arithmetic is shared, but each kernel's floating-point
+contraction may differ by an ulp) or 1e-5 (FP32), match the oracle to 1e-12
+(FP64), graph replays must reproduce the eager result, and the queue /
+step-limit logic must terminate correctly."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+
+    from pyshocks_amd import _lib
+
+    _lib.load()
+    return torch
+
+
+def _single(torch, n: int, u0: np.ndarray, dtype, tfinal: float, scheme: str = "rusanov"):
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.burgers import Godunov, Rusanov
+    from pyshocks_amd.reconstruction import WENOJS53
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import evolve, make_ssprk33
+
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, dtype=dtype)
+    sch = (Rusanov if scheme == "rusanov" else Godunov)(rec=WENOJS53(eps=1e-6))
+    st = make_ssprk33(sch, grid, PeriodicBoundary(), theta=1.0)
+    u = torch.tensor(u0, device="cuda").to(dtype)
+    return evolve(st, u, tfinal=tfinal, tiles=(256, 4))
+
+
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+@pytest.mark.parametrize("scheme", ["rusanov", "godunov"])
+def test_batched_equals_single_instance(torch_mod, dtype_name: str, scheme: str) -> None:
+    torch = torch_mod
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    dtype = getattr(torch, dtype_name)
+    _, ics = random_instances(6, seed=3)
+    ns = np.array([3, 37, 128, 1000, 3000, 4100])
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    solver = BatchedSolver(ns, dtype=dtype, scheme=scheme, tiles=(256, 4), grid=3)
+    solver.set_states(states)
+    res = solver.solve(0.05)
+    assert np.all(res.status == 0)
+    for i, n in enumerate(ns):
+        ev = _single(torch, int(n), states[i], dtype, 0.05, scheme)
+        assert res.steps[i] == ev.iteration
+        assert res.t[i] == float(ev.t)
+        got = solver.state(i).cpu().double().numpy()[3:-3]
+        ref = ev.u.cpu().double().numpy()[3:-3]
+        tol = 1e-12 if dtype == torch.float64 else 1e-5
+        assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref)
+
+
+def test_batched_vs_oracle(torch_mod) -> None:
+    torch = torch_mod
+    from oracle import weno_np as W
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    _, ics = random_instances(3, seed=4)
+    ns = np.array([64, 200, 513])
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    solver = BatchedSolver(ns, dtype=torch.float64)
+    solver.set_states(states)
+    res = solver.solve(0.3)
+    for i, n in enumerate(ns):
+        grid = W.make_uniform_cell_grid(0.0, 1.0, int(n), nghosts=3)
+        u, dts, t = W.solve(W.Scheme("rusanov", eps=1e-6), grid, states[i], theta=1.0,
+                            tfinal=0.3)
+        assert res.steps[i] == len(dts)
+        got = solver.state(i).cpu().numpy()[3:-3]
+        assert np.linalg.norm(got - u[3:-3]) / np.linalg.norm(u[3:-3]) < 1e-12
+
+
+def test_batched_graph_replay(torch_mod) -> None:
+    torch = torch_mod
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    ns, ics = random_instances(40, nmin=16, nmax=2048, seed=5)
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    solver = BatchedSolver(ns, dtype=torch.float32)
+    solver.set_states(states)
+    u0 = solver.u.clone()
+    eager = solver.solve(0.02)
+    ref = solver.u.clone()
+    graph = solver.capture(u0, 0.02)
+    for _ in range(3):
+        solver.u.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(solver.u, ref)
+        res = solver.result()
+        np.testing.assert_array_equal(res.steps, eager.steps)
+
+
+def test_batched_step_limit_terminates(torch_mod) -> None:
+    torch = torch_mod
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    ns, ics = random_instances(5, nmin=32, nmax=256, seed=6)
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    solver = BatchedSolver(ns, dtype=torch.float64, grid=2)
+    solver.desc.max_steps = 5
+    solver.set_states(states)
+    res = solver.solve(10.0)
+    np.testing.assert_array_equal(res.steps, 5)
+    np.testing.assert_array_equal(res.status, -1)
+
+
+def test_autotune_picks_a_launchable_config(torch_mod) -> None:
+    torch = torch_mod
+    from pyshocks_amd import autotune, funcs, make_uniform_cell_grid
+    from pyshocks_amd.batched import exact_averages_np, random_instances
+    from pyshocks_amd.burgers import Rusanov
+    from pyshocks_amd.reconstruction import WENOJS53
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import fused_plan, make_ssprk33
+
+    grid = make_uniform_cell_grid(0.0, 1.0, 1 << 16, nghosts=3)
+    st = make_ssprk33(Rusanov(rec=WENOJS53(eps=1e-6)), grid, PeriodicBoundary())
+    plan = fused_plan(st)
+    u0 = funcs.cell_averages(funcs.random_sine_step(0), grid)
+    best, times = autotune.tune_stage(plan, u0, steps=2)
+    assert best in times and len(times) >= 3
+    assert fused_plan(st).desc.block == best[0]
+
+    ns, ics = random_instances(16, nmin=64, nmax=4096, seed=7)
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    bbest, btimes = autotune.tune_batch(ns, states, tfinal=0.005)
+    assert bbest in btimes
