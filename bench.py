@@ -55,6 +55,14 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
                    help="steps of the per-kernel event-timed pass")
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="target duration of the CPU baseline sample (0: skip)")
+    p.add_argument("--workload", choices=["single", "batched", "adjoint"],
+                   default="single",
+                   help="single: the large single instance (default); batched: "
+                        "4096 uneven Burgers instances in one persistent graph "
+                        "launch; adjoint: dJ/du0 at N = 16384, 2000 steps")
+    p.add_argument("--no-adjoint", action="store_true",
+                   help="skip the adjoint gradients/s figure of the default run")
+    p.add_argument("--ninst", type=int, default=4096)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain device copy of N FP64 values (N * 8 "
                         "bytes read and written per launch), the reference "
@@ -214,10 +222,143 @@ def cpu_leg(args: argparse.Namespace, gpu: dict) -> dict | None:
     }
 
 
+def adjoint_leg(args: argparse.Namespace, repeats: int = 3) -> dict:
+    """Gradients/s of J = 0.5 ||u(T) - u_target||^2 dx with respect to u0 for
+    periodic Burgers Rusanov, N = 16384, FP64, 2000 SSP-RK3 steps of a fixed
+    dt (Courant 0.5 on the unperturbed u0 = 0.5 + sin(2 pi x)), checkpoints
+    every 50 steps; u_target is the solve from u0 + a seeded perturbation."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from pyshocks_amd import funcs, make_uniform_cell_grid
+    from pyshocks_amd.adjoint import CheckpointedSolve
+    from pyshocks_amd.burgers import Rusanov
+    from pyshocks_amd.reconstruction import WENOJS53
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import BoundOperator
+
+    n, nsteps, every = 16384, 2000, 50
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, device="cuda")
+    src = BoundOperator(Rusanov(rec=WENOJS53(eps=1e-6)), grid, PeriodicBoundary())
+    u0 = funcs.cell_averages(funcs.sine_offset(0.5), grid)
+    dt = 0.5 * grid.h / float(u0[3:-3].abs().max())
+    rng = np.random.default_rng(args.seed)
+    pert = funcs.SineStep(c=0.0, amplitudes=tuple(0.01 * rng.uniform(-1, 1, 4)),
+                          wavenumbers=(1, 2, 3, 4),
+                          phases=tuple(rng.uniform(0, 2 * math.pi, 4)))
+    target = CheckpointedSolve(src, u0 + funcs.cell_averages(pert, grid), nsteps, dt,
+                               every).u_final
+    torch.cuda.synchronize()
+
+    def gradient() -> torch.Tensor:
+        solve = CheckpointedSolve(src, u0, nsteps, dt, every)
+        lam = (solve.u_final - target) * grid.h
+        return solve.gradient(lam)
+
+    gradient()  # warm up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        g = gradient()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if not bool(torch.isfinite(g).all()):
+        raise RuntimeError("non-finite gradient")
+    return {"gradients_per_s": repeats / elapsed,
+            "ms_per_gradient": elapsed / repeats * 1e3,
+            "config": f"burgers rusanov N={n} f64, {nsteps} steps fixed dt, "
+                      f"checkpoints every {every}, J = 0.5||u(T)-u_target||^2 dx"}
+
+
+def batched_leg(args: argparse.Namespace) -> dict:
+    """cell-updates/s of the uneven-size batch: --ninst Burgers instances with
+    N log-uniform in [128, 65536] (seeded), FP32, Rusanov, each to t = 0.3,
+    one persistent launch replayed from a HIP graph, tiles autotuned."""
+    import numpy as np
+    import torch
+
+    from pyshocks_amd import _lib, autotune
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    _lib.load()
+    ns, ics = random_instances(args.ninst, seed=args.seed)
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    sample = np.random.default_rng(args.seed + 1).choice(len(ns), size=min(32, len(ns)),
+                                                        replace=False)
+    best, _ = autotune.tune_batch(ns[sample], [states[i] for i in sample],
+                                  dtype=torch.float32, tfinal=0.01)
+    solver = BatchedSolver(ns, dtype=torch.float32)
+    solver.set_states(states)
+    u0 = solver.u.clone()
+    graph = solver.capture(u0, 0.3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    graph.replay()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    res = solver.result()
+    if np.any(res.status != 0):
+        raise RuntimeError(f"{int(np.sum(res.status != 0))} instances did not finish")
+    cu = solver.cell_updates()
+    return {"value": cu / elapsed, "seconds": elapsed, "cell_updates": cu,
+            "tiles": list(best), "ninst": len(ns), "total_cells": int(np.sum(ns)),
+            "max_steps": int(res.steps.max()), "ns": ns, "states": states}
+
+
+def batched_cpu_leg(args: argparse.Namespace, ns, states) -> dict | None:
+    """The C restatement on a seeded subset of the batch's instances (FP64,
+    to t = 0.3), within about --cpu-seconds."""
+    if args.cpu_seconds <= 0:
+        return None
+    import numpy as np
+
+    from oracle import build as oracle_build
+    from oracle import ref_c
+
+    if not ref_c.LIB.exists():
+        oracle_build.build(verbose=False)
+    rng = np.random.default_rng(args.seed + 2)
+    order = rng.permutation(len(ns))
+    done, cells, t0 = 0, 0.0, time.perf_counter()
+    for i in order:
+        n = int(ns[i])
+        _, _, steps, _ = ref_c.solve("rusanov", np.asarray(states[i], dtype=np.float64),
+                                     n=n, g=3, dx=1.0 / n, eps=1e-6, theta=1.0,
+                                     cfl_num=0.5 / n, tfinal=0.3)
+        cells += n * steps
+        done += 1
+        if time.perf_counter() - t0 > args.cpu_seconds:
+            break
+    elapsed = time.perf_counter() - t0
+    return {"value": cells / elapsed, "unit": "cell-updates/s", "cores": ref_c.num_threads(),
+            "kind": "port",
+            "sample": f"C/OpenMP restatement, {done} seeded instances of the batch "
+                      f"(FP64) each to t = 0.3, {elapsed:.1f} s"}
+
+
 def main(argv: list[str] | None = None) -> int:
     args = parse_args(argv)
     if args.copy_calibration:
         return copy_calibration(args)
+    if args.workload == "batched":
+        b = batched_leg(args)
+        cpu = batched_cpu_leg(args, b.pop("ns"), b.pop("states"))
+        print(json.dumps({"metric": METRIC, "value": b["value"], "unit": "cell-updates/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+                          "data": "synthetic",
+                          "config": {"workload": f"{b['ninst']} periodic Burgers Rusanov "
+                                                 "instances, N log-uniform [128, 65536], "
+                                                 "f32, each to t = 0.3, one graph launch"},
+                          "batched": b, "cpu_baseline": cpu}), flush=True)
+        return 0
+    if args.workload == "adjoint":
+        a = adjoint_leg(args)
+        print(json.dumps({"metric": METRIC, "value": a["gradients_per_s"],
+                          "unit": "gradients/s", "n_gpus": 1, "higher_is_better": True,
+                          "dtype": "f64", "data": "synthetic", "adjoint": a}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the workload is a single-GPU instance, see DESIGN.md for how "
@@ -225,6 +366,7 @@ def main(argv: list[str] | None = None) -> int:
         return 2
 
     gpu = gpu_leg(args)
+    adjoint = None if args.no_adjoint else adjoint_leg(args)
     cpu = cpu_leg(args, gpu)
 
     traffic = None
@@ -272,6 +414,7 @@ def main(argv: list[str] | None = None) -> int:
             "step_achieved_gbs": gpu["step_gbs"],
         },
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
     return 0

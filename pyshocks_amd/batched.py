@@ -151,18 +151,20 @@ class BatchedSolver:
         """A graph that restores the packed initial state ``u0`` (a device
         copy, no memset) and runs the batched solve; replay it with
         ``graph.replay()``."""
-        self.set_tfinal(tfinal)
         if u0.shape != self.u.shape or u0.dtype != self.dtype:
             raise ValueError("u0 must be a packed state of this batch")
         graph = torch.cuda.CUDAGraph()
         stream = torch.cuda.Stream(device=self.device)
         stream.wait_stream(torch.cuda.current_stream(self.device))
+        # warm up outside capture (a zero-length solve) so that lazy
+        # initialisation is not recorded
+        self.set_tfinal(0.0)
         with torch.cuda.stream(stream):
-            # warm up outside capture so lazy initialisation is not recorded
             self.u.copy_(u0)
             self.launch()
         torch.cuda.current_stream(self.device).wait_stream(stream)
         torch.cuda.synchronize(self.device)
+        self.set_tfinal(tfinal)
         with torch.cuda.graph(graph):
             self.u.copy_(u0)
             self.launch()
