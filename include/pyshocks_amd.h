@@ -169,6 +169,9 @@ typedef struct psa_solver_desc {
   const void *nu;               /* Rusanov df^(alpha-1) (m - 1) or NULL    */
   int block;                    /* threads per workgroup (0 = tuned)       */
   int cells_per_thread;         /* cells per thread (0 = tuned)            */
+  const float *mlp;             /* learned weights (PSA_MLP_NPARAM FP32) or
+                                   NULL for WENO-JS                        */
+  double mlp_tau;               /* shock threshold of the learned weights  */
 } psa_solver_desc;
 
 /* Initialise the control block (device memory, sizeof(psa_control)) for a
@@ -192,7 +195,10 @@ int psa_ssprk33_steps(const psa_solver_desc *desc, const void *u_in,
 /* One stage (1, 2 or 3) of timestepping.py:312-320: stage 1 out = uk + dt L(uk),
  * stage 2 out = 3/4 un + 1/4 (uk + dt L(uk)), stage 3 out = 1/3 un +
  * 2/3 (uk + dt L(uk)), with the step control of the control block, or, if
- * control is NULL, the device scalar dt. out may equal un but not uk. */
+ * control is NULL, the device scalar dt. out may equal un but not uk.
+ * Stage 0 is the operator itself, out = L(uk) on all m cells
+ * (schemes.py:466-473, as psa_apply_operator), for any descriptor,
+ * including one with learned weights; un, control and dt are unused. */
 int psa_ssprk33_stage(const psa_solver_desc *desc, int stage, const void *un,
                       const void *uk, void *out, void *control,
                       const void *dt, void *stream);
@@ -231,6 +237,9 @@ typedef struct psa_batch_desc {
   long long max_steps;          /* step limit per instance                 */
   int block;                    /* threads per workgroup (0 = default)     */
   int cells_per_thread;         /* cells per thread (0 = default)          */
+  const float *mlp;             /* learned weights or NULL (see
+                                   psa_solver_desc)                        */
+  double mlp_tau;               /* shock threshold of the learned weights  */
 } psa_batch_desc;
 
 int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1, void *k2,
@@ -280,6 +289,42 @@ int psa_ssprk33_adjoint(const psa_solver_desc *desc, long long nsteps,
                         const void *dt, int every, const void *lam_final,
                         void *grad_u0, void *workspace, size_t workspace_bytes,
                         void *stream);
+
+/* Learned WENO weights ----------------------------------------------------
+ * No counterpart in the reference, which has only the fixed WENO-JS weights
+ * (weno.py:247-256). A 5 -> 32 -> 32 -> 3 perceptron (ReLU, softmax over
+ * logits + ln d) maps the normalised 5-cell stencil to the three weights;
+ * BF16 matrix-core products with FP32 accumulation; parameters are
+ * PSA_MLP_NPARAM FP32 values packed as W1[32][5], b1[32], W2[32][32],
+ * b2[32], W3[3][32], b3[3]. Cells whose smoothness indicators satisfy
+ * max beta > tau (min beta + eps) keep the WENO-JS weights. The solvers take
+ * the weights through psa_solver_desc.mlp / psa_batch_desc.mlp. */
+#define PSA_MLP_NPARAM 1347
+#define PSA_MLP_RECORD 12 /* floats per training sample: x[5], e[3], wjs[3], c */
+
+/* omega (n x 3) and optionally the logits (n x 3) of n input rows (n x 5). */
+int psa_mlp_weights(const float *params, const float *features, long long n,
+                    float *omega, float *logits, void *stream);
+
+/* Mean training loss over n sample records and its gradient: grad receives
+ * PSA_MLP_NPARAM gradient entries followed by the loss (PSA_MLP_NPARAM + 1
+ * floats). Per sample, with r = sum_k omega_k e_k:
+ *   c (r^2 / (mean_k e_k^2 + delta) + lambda_js sum_k (omega_k - wjs_k)^2). */
+int psa_mlp_grad_workspace_bytes(long long n, size_t *bytes);
+int psa_mlp_grad(const float *params, const float *samples, long long n,
+                 double lambda_js, double delta, float *grad, void *work,
+                 size_t work_bytes, void *stream);
+
+/* One Adam update (step counts from 1) of the PSA_MLP_NPARAM parameters. */
+int psa_mlp_adam(float *params, const float *grad, float *m, float *v,
+                 long long step, double lr, double beta1, double beta2,
+                 double eps, void *stream);
+
+/* Learned counterpart of psa_weno_reconstruct (same zero padding outside
+ * [0, m)): ul, ur with the learned weights, WENO-JS in flagged cells. */
+int psa_learned_reconstruct(int dtype, const float *params, double tau,
+                            const void *u, void *ul, void *ur, int64_t m,
+                            double eps, void *stream);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,8 @@ class PsaSolverDesc(ctypes.Structure):
         ("nu", ctypes.c_void_p),
         ("block", ctypes.c_int),
         ("cells_per_thread", ctypes.c_int),
+        ("mlp", ctypes.c_void_p),
+        ("mlp_tau", ctypes.c_double),
     ]
 
 
@@ -96,6 +98,8 @@ class PsaBatchDesc(ctypes.Structure):
         ("max_steps", ctypes.c_longlong),
         ("block", ctypes.c_int),
         ("cells_per_thread", ctypes.c_int),
+        ("mlp", ctypes.c_void_p),
+        ("mlp_tau", ctypes.c_double),
     ]
 
 
@@ -142,7 +146,15 @@ SIGNATURES: dict[str, list[Any]] = {
                                          _LL, _VP, _I, _VP, ctypes.c_size_t, _VP],
     "psa_ssprk33_adjoint": [ctypes.POINTER(PsaSolverDesc), _LL, _VP, _I, _VP, _VP,
                             _VP, ctypes.c_size_t, _VP],
+    "psa_mlp_weights": [_VP, _VP, _LL, _VP, _VP, _VP],
+    "psa_mlp_grad_workspace_bytes": [_LL, ctypes.POINTER(ctypes.c_size_t)],
+    "psa_mlp_grad": [_VP, _VP, _LL, _D, _D, _VP, _VP, ctypes.c_size_t, _VP],
+    "psa_mlp_adam": [_VP, _VP, _VP, _VP, _LL, _D, _D, _D, _D, _VP],
+    "psa_learned_reconstruct": [_I, _VP, _D, _VP, _VP, _VP, _I64, _D, _VP],
 }
+
+MLP_NPARAM = 1347
+MLP_RECORD = 12
 
 _LIB: ctypes.CDLL | None = None
 

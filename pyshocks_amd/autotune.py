@@ -25,7 +25,8 @@ _BATCH_CACHE: dict[tuple[Any, ...], tuple[int, int]] = {}
 
 def _stage_key(desc: Any) -> tuple[Any, ...]:
     vel = 0 if desc.flux != 2 else (1 if desc.velocity_is_const else 2)
-    return (int(desc.dtype), int(desc.flux), vel, max(0, math.ceil(math.log2(max(desc.n, 1)))))
+    return (int(desc.dtype), int(desc.flux), vel, bool(desc.mlp),
+            max(0, math.ceil(math.log2(max(desc.n, 1)))))
 
 
 def lookup_stage(desc: Any) -> tuple[int, int] | None:
@@ -67,14 +68,15 @@ def tune_stage(plan: Any, u0: torch.Tensor, *, steps: int = 4,
     return best, times
 
 
-def lookup_batch(dtype: int, flux: int) -> tuple[int, int] | None:
-    return _BATCH_CACHE.get((int(dtype), int(flux)))
+def lookup_batch(dtype: int, flux: int, learned: bool = False) -> tuple[int, int] | None:
+    return _BATCH_CACHE.get((int(dtype), int(flux), bool(learned)))
 
 
 def tune_batch(ncells: Sequence[int], states: Sequence[np.ndarray], *,
                dtype: torch.dtype = torch.float32, scheme: str = "rusanov",
                tfinal: float = 0.01, eps: float = 1.0e-6, theta: float = 1.0,
                candidates: Sequence[tuple[int, int]] = BATCH_CANDIDATES,
+               mlp: Any = None,
                ) -> tuple[tuple[int, int], dict[tuple[int, int], float]]:
     """Time the batched solve of a sample of instances (to a short final
     time) for each candidate and cache the fastest."""
@@ -85,7 +87,7 @@ def tune_batch(ncells: Sequence[int], states: Sequence[np.ndarray], *,
     for cand in candidates:
         try:
             solver = BatchedSolver(ncells, dtype=dtype, scheme=scheme, eps=eps,
-                                   theta=theta, tiles=cand)
+                                   theta=theta, tiles=cand, mlp=mlp)
             solver.set_states(states)
             u0 = solver.u.clone()
             solver.solve(tfinal)  # warm up
@@ -103,5 +105,5 @@ def tune_batch(ncells: Sequence[int], states: Sequence[np.ndarray], *,
         raise RuntimeError("no tile candidate could be launched")
     best = min(times, key=lambda c: times[c])
     flux = _lib.PSA_FLUX_RUSANOV if scheme == "rusanov" else _lib.PSA_FLUX_BURGERS_UPWIND
-    _BATCH_CACHE[(_lib.dtype_code(dtype), flux)] = best
+    _BATCH_CACHE[(_lib.dtype_code(dtype), flux, mlp is not None)] = best
     return best, times

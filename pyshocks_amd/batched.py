@@ -3,8 +3,8 @@ uneven size advanced in one persistent launch.
 
 Each instance ``i`` is a uniform periodic grid of ``ncells[i]`` cells on
 ``[a, b]`` (3 ghost cells per side) with its own initial state and final time,
-advanced by the fused WENO-JS5 / Rusanov (or Godunov) / SSP-RK3 step with its
-own CFL time step -- the loop of ``timestepping.py:89-152`` per instance, as
+advanced by the fused WENO-JS5 / Rusanov (or Godunov) / SSP-RK3 step (or its
+learned-weight form, ``mlp``) with its own CFL time step -- the loop of ``timestepping.py:89-152`` per instance, as
 the reference harness would run it for each request separately. The instances
 are packed into three device arrays (state and two stage buffers) at 16-byte
 aligned offsets; the library runs them longest-first from an atomic queue,
@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass
-from typing import Sequence
+from typing import Any, Sequence
 
 import numpy as np
 import torch
@@ -43,7 +43,8 @@ class BatchedSolver:
                  scheme: str = "rusanov", eps: float = 1.0e-6, theta: float = 1.0,
                  a: float = 0.0, b: float = 1.0, nghosts: int = 3,
                  device: torch.device | str = "cuda",
-                 tiles: tuple[int, int] | None = None, grid: int = 0) -> None:
+                 tiles: tuple[int, int] | None = None, grid: int = 0,
+                 mlp: Any = None) -> None:
         if scheme not in ("rusanov", "godunov"):
             raise NotImplementedError(
                 f"the batched solver takes Burgers 'rusanov' or 'godunov', not {scheme!r}")
@@ -97,10 +98,15 @@ class BatchedSolver:
         self.desc.eps = float(eps)
         self.desc.theta = float(theta)
         self.desc.max_steps = 4_000_000
+        self.mlp = mlp
+        if mlp is not None:
+            # learned weights (pyshocks_amd.learned.MLPWeights)
+            self.desc.mlp = mlp.ptr
+            self.desc.mlp_tau = float(mlp.tau)
         if tiles is None:
             from pyshocks_amd.autotune import lookup_batch
 
-            tiles = lookup_batch(self.desc.dtype, self.desc.flux)
+            tiles = lookup_batch(self.desc.dtype, self.desc.flux, mlp is not None)
         if tiles is not None:
             self.desc.block, self.desc.cells_per_thread = int(tiles[0]), int(tiles[1])
 

@@ -296,6 +296,9 @@ int check_vjp_desc(const psa_solver_desc *d) {
   if (d->flux == PSA_FLUX_ADVECTION_UPWIND && !d->velocity_is_const)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "the adjoint supports advection with a constant velocity");
+  if (d->mlp != nullptr)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the adjoint supports the WENO-JS weights only");
   if (!(d->dx > 0.0))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "dx must be positive");
   return PSA_OK;

@@ -45,6 +45,8 @@ struct BatchArgs {
   T eps;
   T theta;
   int g;
+  const float *mlp;
+  T tau;
 };
 
 template <int BLOCK>
@@ -57,13 +59,15 @@ __device__ __forceinline__ unsigned long long block_max_bcast(
   return *s_out;
 }
 
-template <typename T, int FLUX, int BLOCK, int CPT>
+template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED>
 __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
   constexpr int TILE = BLOCK * CPT;
   __shared__ StageSmem<T, 0, BLOCK, CPT> sm;
   __shared__ unsigned long long s_bcast;
   __shared__ long long s_next;
   const int tid = threadIdx.x;
+  MlpWave W;
+  if constexpr (LEARNED) mlp_load(W, a.mlp);
 
   for (;;) {
     __syncthreads();  // previous instance done with the LDS and s_next
@@ -80,6 +84,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
     p.dx = static_cast<T>(a.dx[i]);
     p.inv_dx = T(1) / p.dx;
     p.eps = a.eps;
+    p.mlp = a.mlp;
+    p.tau = a.tau;
     p.aligned = 1;  // instance offsets are 16-byte aligned
     T *u = a.u + a.offsets[i];
     T *k1 = a.k1 + a.offsets[i];
@@ -122,14 +128,16 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
       p.out = k1;
       for (int64_t tile = 0; tile < ntiles; ++tile) {
         __syncthreads();
-        stage_tile<T, FLUX, 0, 1, BLOCK, CPT>(p, tile * TILE, dt, sm, vdummy);
+        stage_tile<T, FLUX, 0, 1, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
+                                                          vdummy, W);
       }
       __syncthreads();  // k1 complete
       p.uk = k1;
       p.out = k2;
       for (int64_t tile = 0; tile < ntiles; ++tile) {
         __syncthreads();
-        stage_tile<T, FLUX, 0, 2, BLOCK, CPT>(p, tile * TILE, dt, sm, vdummy);
+        stage_tile<T, FLUX, 0, 2, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
+                                                          vdummy, W);
       }
       __syncthreads();  // k2 complete
       p.uk = k2;
@@ -137,7 +145,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
       unsigned long long vmax = 0;
       for (int64_t tile = 0; tile < ntiles; ++tile) {
         __syncthreads();
-        stage_tile<T, FLUX, 0, 3, BLOCK, CPT>(p, tile * TILE, dt, sm, vmax);
+        stage_tile<T, FLUX, 0, 3, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
+                                                       vmax, W);
       }
       __syncthreads();  // u complete
       umax = block_max_bcast<BLOCK>(vmax, sm.red, &s_bcast);
@@ -173,25 +182,37 @@ unsigned int resident(K kernel, int block) {
   return static_cast<unsigned int>(per_cu) * static_cast<unsigned int>(cus);
 }
 
-template <typename T, int FLUX, int BLOCK, int CPT>
+template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED = false>
 int launch_batch_cfg(const BatchArgs<T> &a, int grid, hipStream_t s) {
   unsigned int blocks =
       grid > 0 ? static_cast<unsigned int>(grid)
-               : resident(k_batch<T, FLUX, BLOCK, CPT>, BLOCK);
+               : resident(k_batch<T, FLUX, BLOCK, CPT, LEARNED>, BLOCK);
   if (static_cast<long long>(blocks) > a.ninst)
     blocks = static_cast<unsigned int>(a.ninst);
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.counter);
   int st = check_launch("batch reset");
   if (st != PSA_OK) return st;
-  hipLaunchKernelGGL((k_batch<T, FLUX, BLOCK, CPT>), dim3(blocks), dim3(BLOCK),
-                     0, s, a);
+  hipLaunchKernelGGL((k_batch<T, FLUX, BLOCK, CPT, LEARNED>), dim3(blocks),
+                     dim3(BLOCK), 0, s, a);
   return check_launch("batch solve");
 }
 
 template <typename T, int FLUX>
 int launch_batch_tiles(const BatchArgs<T> &a, int block, int cpt, int grid,
                        hipStream_t s) {
+  if (a.mlp != nullptr) {
+    if (block == 256 && cpt == 4)
+      return launch_batch_cfg<T, FLUX, 256, 4, true>(a, grid, s);
+    if (block == 512 && cpt == 2)
+      return launch_batch_cfg<T, FLUX, 512, 2, true>(a, grid, s);
+    if (block == 1024 && cpt == 2)
+      return launch_batch_cfg<T, FLUX, 1024, 2, true>(a, grid, s);
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "unsupported batch tile configuration for learned "
+                     "weights block=" + std::to_string(block) +
+                         " cells_per_thread=" + std::to_string(cpt));
+  }
   if (block == 256 && cpt == 4) return launch_batch_cfg<T, FLUX, 256, 4>(a, grid, s);
   if (block == 256 && cpt == 2) return launch_batch_cfg<T, FLUX, 256, 2>(a, grid, s);
   if (block == 512 && cpt == 2) return launch_batch_cfg<T, FLUX, 512, 2>(a, grid, s);
@@ -225,6 +246,8 @@ int launch_batch(const psa_batch_desc *d, void *u, void *k1, void *k2,
   a.eps = static_cast<T>(d->eps);
   a.theta = static_cast<T>(d->theta);
   a.g = d->g;
+  a.mlp = d->mlp;
+  a.tau = static_cast<T>(d->mlp_tau);
   const int block = d->block > 0 ? d->block : 256;
   const int cpt = d->cells_per_thread > 0 ? d->cells_per_thread : 4;
   if (d->flux == PSA_FLUX_RUSANOV)
@@ -252,6 +275,8 @@ extern "C" int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1,
     return set_error(PSA_ERR_INVALID_ARGUMENT, "needs at least 3 ghost cells");
   if (desc->ninst < 0 || desc->max_steps < 0 || !(desc->eps >= 0.0))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "bad batch parameters");
+  if (desc->mlp != nullptr && !(desc->mlp_tau >= 0.0))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "mlp_tau must be >= 0");
   if (u == nullptr || k1 == nullptr || k2 == nullptr || counter == nullptr ||
       t_out == nullptr || steps_out == nullptr || status_out == nullptr ||
       desc->offsets == nullptr || desc->ncells == nullptr ||

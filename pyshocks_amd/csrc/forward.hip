@@ -262,11 +262,14 @@ __global__ void k_copy_if_inactive(const T *__restrict__ src,
 // ---------------------------------------------------------------------------
 // fused stage kernel: one workgroup per tile
 
-template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT>
+template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT,
+          bool LEARNED>
 __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   constexpr int TILE = BLOCK * CPT;
   __shared__ StageSmem<T, VEL, BLOCK, CPT> sm;
   const int tid = threadIdx.x;
+  MlpWave W;
+  if constexpr (LEARNED) mlp_load(W, p.mlp);
   const int64_t ntiles = (p.m + TILE - 1) / TILE;
 
   // ---- step control
@@ -307,7 +310,8 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   unsigned long long vmax = 0;
   const int64_t tile = blockIdx.x;  // the grid has one workgroup per tile
   if (tile < ntiles) {
-    stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT>(p, tile * TILE, dt, sm, vmax);
+    stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT, LEARNED>(p, tile * TILE, dt,
+                                                         sm, vmax, W);
   }
 
   if constexpr (STAGE == 3) {
@@ -410,7 +414,8 @@ unsigned int resident_blocks(K kernel, int block) {
   return cache[dev];
 }
 
-template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT>
+template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT,
+          bool LEARNED = false>
 int launch_stage_cfg(const StageArgs<T> &args, hipStream_t stream) {
   constexpr int TILE = BLOCK * CPT;
   StageArgs<T> a = args;
@@ -418,8 +423,8 @@ int launch_stage_cfg(const StageArgs<T> &args, hipStream_t stream) {
               (a.un == nullptr || aligned16(a.un)) &&
               (a.vel == nullptr || aligned16(a.vel));
   const unsigned int blocks = blocks_for(a.m, TILE);
-  hipLaunchKernelGGL((k_stage<T, FLUX, VEL, STAGE, BLOCK, CPT>), dim3(blocks),
-                     dim3(BLOCK), 0, stream, a);
+  hipLaunchKernelGGL((k_stage<T, FLUX, VEL, STAGE, BLOCK, CPT, LEARNED>),
+                     dim3(blocks), dim3(BLOCK), 0, stream, a);
   return check_launch("ssprk33 stage");
 }
 
@@ -442,9 +447,32 @@ int launch_stage_tiles(const StageArgs<T> &a, TileConfig cfg,
                        " cells_per_thread=" + std::to_string(cfg.cpt));
 }
 
+// Learned weights: Burgers fluxes and two tile shapes.
+template <typename T, int FLUX, int STAGE>
+int launch_stage_learned(const StageArgs<T> &a, TileConfig cfg,
+                         hipStream_t stream) {
+  if (cfg.block == 256 && cfg.cpt == 4)
+    return launch_stage_cfg<T, FLUX, 0, STAGE, 256, 4, true>(a, stream);
+  if (cfg.block == 256 && cfg.cpt == 2)
+    return launch_stage_cfg<T, FLUX, 0, STAGE, 256, 2, true>(a, stream);
+  return set_error(PSA_ERR_INVALID_ARGUMENT,
+                   "unsupported tile configuration for learned weights block=" +
+                       std::to_string(cfg.block) +
+                       " cells_per_thread=" + std::to_string(cfg.cpt));
+}
+
 template <typename T, int STAGE>
 int launch_stage(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
                  hipStream_t stream) {
+  if (a.mlp != nullptr) {
+    if (flux == PSA_FLUX_RUSANOV)
+      return launch_stage_learned<T, PSA_FLUX_RUSANOV, STAGE>(a, cfg, stream);
+    if (flux == PSA_FLUX_BURGERS_UPWIND)
+      return launch_stage_learned<T, PSA_FLUX_BURGERS_UPWIND, STAGE>(a, cfg,
+                                                                    stream);
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "learned weights are implemented for Burgers only");
+  }
   if (flux == PSA_FLUX_RUSANOV)
     return launch_stage_tiles<T, PSA_FLUX_RUSANOV, 0, STAGE>(a, cfg, stream);
   if (flux == PSA_FLUX_BURGERS_UPWIND)
@@ -484,11 +512,18 @@ int check_desc(const psa_solver_desc *d) {
   if (d->flux < 0 || d->flux > 2)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "unsupported flux " + std::to_string(d->flux));
+  if (d->mlp != nullptr && d->flux == PSA_FLUX_ADVECTION_UPWIND)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "learned weights are implemented for Burgers only");
+  if (d->mlp != nullptr && !(d->mlp_tau >= 0.0))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "mlp_tau must be >= 0");
   return PSA_OK;
 }
 
 TileConfig tiles_of(const psa_solver_desc *d) {
   TileConfig c = default_tiles(d->dtype);
+  // the learned kernels keep fewer cells per thread to hold occupancy
+  if (d->mlp != nullptr) c.cpt = 2;
   if (d->block > 0) c.block = d->block;
   if (d->cells_per_thread > 0) c.cpt = d->cells_per_thread;
   return c;
@@ -507,6 +542,8 @@ StageArgs<T> make_args(const psa_solver_desc *d) {
   a.vel = static_cast<const T *>(d->velocity);
   a.nu = static_cast<const T *>(d->nu);
   a.aligned = 1;
+  a.mlp = d->mlp;
+  a.tau = static_cast<T>(d->mlp_tau);
   return a;
 }
 
@@ -845,11 +882,12 @@ int psa_ssprk33_stage(const psa_solver_desc *desc, int stage, const void *un,
                       const void *dt, void *stream) {
   int st = check_desc(desc);
   if (st != PSA_OK) return st;
-  if (stage < 1 || stage > 3)
-    return set_error(PSA_ERR_INVALID_ARGUMENT, "stage must be 1, 2 or 3");
+  if (stage < 0 || stage > 3)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "stage must be 0 (operator), 1, 2 or 3");
   if (uk == nullptr || out == nullptr || (stage > 1 && un == nullptr))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "null state pointer");
-  if (control == nullptr && dt == nullptr)
+  if (stage > 0 && control == nullptr && dt == nullptr)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "need a control block or dt");
   if (out == uk)
     return set_error(PSA_ERR_INVALID_ARGUMENT,
@@ -864,6 +902,10 @@ int psa_ssprk33_stage(const psa_solver_desc *desc, int stage, const void *un,
     a.out = static_cast<double *>(out);
     a.ctrl = static_cast<psa_control *>(control);
     a.dt_ptr = static_cast<const double *>(dt);
+    if (stage == 0) {
+      a.ctrl = nullptr;
+      return launch_stage<double, 0>(desc->flux, vel, a, cfg, s);
+    }
     if (stage == 1) return launch_stage<double, 1>(desc->flux, vel, a, cfg, s);
     if (stage == 2) return launch_stage<double, 2>(desc->flux, vel, a, cfg, s);
     return launch_stage<double, 3>(desc->flux, vel, a, cfg, s);
@@ -874,6 +916,10 @@ int psa_ssprk33_stage(const psa_solver_desc *desc, int stage, const void *un,
   a.out = static_cast<float *>(out);
   a.ctrl = static_cast<psa_control *>(control);
   a.dt_ptr = static_cast<const float *>(dt);
+  if (stage == 0) {
+    a.ctrl = nullptr;
+    return launch_stage<float, 0>(desc->flux, vel, a, cfg, s);
+  }
   if (stage == 1) return launch_stage<float, 1>(desc->flux, vel, a, cfg, s);
   if (stage == 2) return launch_stage<float, 2>(desc->flux, vel, a, cfg, s);
   return launch_stage<float, 3>(desc->flux, vel, a, cfg, s);

@@ -8,6 +8,7 @@
 #include <cstdint>
 
 #include "../../include/pyshocks_amd.h"
+#include "mlp_device.hpp"
 #include "weno5_device.hpp"
 
 namespace psa {
@@ -94,6 +95,8 @@ struct StageArgs {
   T inv_dx;
   T eps;
   T vconst;
+  const float *mlp;  // learned weights (LEARNED kernels)
+  T tau;             // their shock threshold
 };
 
 template <typename T, int VEL>
@@ -190,6 +193,8 @@ struct StageSmem {
 // One tile of one stage: cells [S, S + BLOCK * CPT) of the arrays of p.
 //
 // VEL: 0 Burgers (no velocity), 1 constant velocity, 2 velocity array.
+// LEARNED: the reconstruction takes the learned weights of W (Burgers only,
+//        every wave of the workgroup must run the tile).
 // STAGE: 0 operator (all m cells, faces 0 and m padded with zero flux),
 //        1..3 SSP-RK3 stages (interior cells, ghost images rewritten).
 //
@@ -201,11 +206,14 @@ struct StageSmem {
 // through LDS as coalesced stores. For STAGE 3, vmax accumulates the bit
 // pattern of max |u| of the cells written. Ends with the LDS free for reuse
 // only after the caller's next barrier.
-template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT>
+template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT,
+          bool LEARNED = false>
 __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
                                            T dt,
                                            StageSmem<T, VEL, BLOCK, CPT> &sm,
-                                           unsigned long long &vmax) {
+                                           unsigned long long &vmax,
+                                           const MlpWave &W) {
+  static_assert(!LEARNED || VEL == 0, "learned weights are for Burgers");
   constexpr int TILE = BLOCK * CPT;
   constexpr int HALO = 4;
   constexpr int VN = Vec16<T>::n;
@@ -278,6 +286,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     const T b2 = E[r + 2] + C::a1 * (t21 * t21);
     weno_pair_from_beta(um2, um1, u0, up1, up2, b0, b1, b2, p.eps, ul[r],
                         ur[r]);
+    if constexpr (LEARNED) {
+      learned_pair(W, um2, um1, u0, up1, up2, b0, b1, b2, p.eps, p.tau, ul[r],
+                   ur[r], ul[r], ur[r]);
+    }
   }
 
   T al[VEL == 2 ? CPT : 1], ar[VEL == 2 ? CPT : 1];
@@ -297,14 +309,29 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     sm.arL[tid + 1] = ar[CPT - 1];
     sm.alF[tid] = al[0];
   }
-  if (tid == 0) {
+  if constexpr (LEARNED) {
+    // wave 0 (a wave-uniform branch): lanes 0..31 reconstruct local cell -1,
+    // lanes 32..63 local cell TILE from its mirrored stencil (as the WENO-JS
+    // path does, so flagged cells match it bit for bit); lane 0 keeps the
+    // right state of the first, lane 32 the left state of the second
+    if ((tid >> 6) == 0) {
+      const bool hi = (tid & 63) >= 32;
+      const int c = hi ? TILE + HALO : HALO - 1;
+      const int d = hi ? -1 : 1;
+      T el, er;
+      learned_cell(W, sm.u[c - 2 * d], sm.u[c - d], sm.u[c], sm.u[c + d],
+                   sm.u[c + 2 * d], p.eps, p.tau, el, er);
+      if (tid == 0) sm.urL[0] = er;
+      if (tid == 32) sm.ulF[BLOCK] = er;
+    }
+  } else if (tid == 0) {
     // right state of local cell -1
     sm.urL[0] = weno_right(sm.u[1], sm.u[2], sm.u[3], sm.u[4], sm.u[5], p.eps);
     if constexpr (VEL == 2) {
       sm.arL[0] = weno_right(sm.v[1], sm.v[2], sm.v[3], sm.v[4], sm.v[5], p.eps);
     }
   }
-  if (tid == BLOCK - 1) {
+  if (!LEARNED && tid == BLOCK - 1) {
     // left state of local cell TILE (mirrored stencil)
     constexpr int c = TILE + HALO;
     sm.ulF[BLOCK] = weno_right(sm.u[c + 2], sm.u[c + 1], sm.u[c], sm.u[c - 1],

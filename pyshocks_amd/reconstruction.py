@@ -8,7 +8,7 @@ Mirrors ``reconstruction.py`` of pyshocks: ``Reconstruction`` (``:49-76``),
 
 from __future__ import annotations
 
-from dataclasses import dataclass, fields
+from dataclasses import dataclass, field, fields
 from functools import singledispatch
 from typing import TYPE_CHECKING, Any, ClassVar
 
@@ -91,11 +91,43 @@ def _reconstruct_wenojs53(rec: WENOJS53, grid: Grid, bc: BoundaryType,
     return ul, ur
 
 
+@dataclass(frozen=True)
+class LearnedWENOJS53(WENOJS53):
+    """WENO-JS5 candidates with learned nonlinear weights (a small perceptron
+    of the normalised stencil, :mod:`pyshocks_amd.learned`); cells flagged by
+    the smoothness indicators keep the WENO-JS weights. No counterpart in the
+    reference. ``weights`` defaults to the trained network shipped with the
+    package."""
+
+    weights: Any = field(default=None, compare=False, hash=False)
+
+    def __post_init__(self) -> None:
+        super().__post_init__()
+        if self.weights is None:
+            from pyshocks_amd.learned import MLPWeights
+
+            object.__setattr__(self, "weights", MLPWeights.load())
+
+
+@reconstruct.register(LearnedWENOJS53)
+def _reconstruct_learned(rec: LearnedWENOJS53, grid: Grid, bc: BoundaryType,
+                         f: torch.Tensor, u: torch.Tensor,
+                         wavespeed: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    from pyshocks_amd.grid import UniformGrid
+    from pyshocks_amd.learned import learned_reconstruct
+
+    assert grid.nghosts >= rec.stencil_width
+    if not isinstance(grid, UniformGrid):
+        raise NotImplementedError("WENO-JS is only implemented for uniform grids.")
+    return learned_reconstruct(rec.weights, f, rec.eps)
+
+
 # {{{ registry
 
 _RECONSTRUCTION: dict[str, type[Reconstruction]] = {
     "default": WENOJS53,
     "wenojs53": WENOJS53,
+    "wenojs53-learned": LearnedWENOJS53,
 }
 
 # names the reference registers that are outside the GPU hot path

@@ -53,6 +53,29 @@ def rusanov_nu(grid: Grid, alpha: float) -> torch.Tensor | None:
     return None
 
 
+def _learned(scheme: ConservationLawScheme) -> Any:
+    from pyshocks_amd.reconstruction import LearnedWENOJS53
+
+    return scheme.rec if isinstance(scheme.rec, LearnedWENOJS53) else None
+
+
+def _learned_flux(rec: Any, kind: int, u: torch.Tensor,
+                  nu: torch.Tensor | None) -> torch.Tensor:
+    """Face fluxes of the learned reconstruction (``scalar.py:125-130`` and
+    ``:238-248`` on the learned states)."""
+    from pyshocks_amd.learned import learned_reconstruct
+
+    ul, ur = learned_reconstruct(rec.weights, u, rec.eps)
+    fl, fr = ul * ul / 2, ur * ur / 2
+    if kind == _lib.PSA_FLUX_RUSANOV:
+        a = torch.maximum(u[1:].abs(), u[:-1].abs())
+        nu_ = 1.0 if nu is None else nu
+        f = 0.5 * (fl[1:] + fr[:-1]) - 0.5 * a * nu_ * (ul[1:] - ur[:-1])
+    else:
+        f = torch.where((ur[:-1] + ul[1:]) / 2 > 0, fr[:-1], fl[1:])
+    return torch.nn.functional.pad(f, (1, 1))
+
+
 def scalar_flux_upwind(scheme: ConservationLawScheme, grid: Grid, bc: BoundaryType,
                        t: Any, a: torch.Tensor, u: torch.Tensor) -> torch.Tensor:
     """Upwind flux on the averaged reconstructed speed; the speed must be the
@@ -62,6 +85,9 @@ def scalar_flux_upwind(scheme: ConservationLawScheme, grid: Grid, bc: BoundaryTy
     _check_burgers(scheme)
     if a is not u and not torch.equal(a, u):
         raise NotImplementedError("the GPU upwind flux takes the speed a = u")
+    rec = _learned(scheme)
+    if rec is not None:
+        return _learned_flux(rec, _lib.PSA_FLUX_BURGERS_UPWIND, u, None)
     return _numerical_flux(_lib.PSA_FLUX_BURGERS_UPWIND, u, None, None,
                            _eps_of(scheme))
 
@@ -76,6 +102,9 @@ def scalar_flux_rusanov(scheme: ConservationLawScheme, grid: Grid, bc: BoundaryT
     _check_burgers(scheme)
     if a is not u and not torch.equal(a, u):
         raise NotImplementedError("the GPU Rusanov flux takes the speed a = u")
+    rec = _learned(scheme)
+    if rec is not None:
+        return _learned_flux(rec, _lib.PSA_FLUX_RUSANOV, u, rusanov_nu(grid, alpha))
     return _numerical_flux(_lib.PSA_FLUX_RUSANOV, u, None, rusanov_nu(grid, alpha),
                            _eps_of(scheme))
 

@@ -223,12 +223,20 @@ def _scheme_plan(source: Any) -> tuple[_lib.PsaSolverDesc, tuple[Any, ...]] | No
     desc.dx = grid.h
     desc.eps = float(scheme.rec.eps)
     keep: tuple[Any, ...] = ()
+    from pyshocks_amd.reconstruction import LearnedWENOJS53
+
+    if isinstance(scheme.rec, LearnedWENOJS53):
+        if not isinstance(scheme, (burgers.Rusanov, burgers.Godunov)):
+            return None
+        desc.mlp = scheme.rec.weights.ptr
+        desc.mlp_tau = float(scheme.rec.weights.tau)
+        keep = (scheme.rec.weights,)
     if isinstance(scheme, burgers.Rusanov):
         desc.flux = _lib.PSA_FLUX_RUSANOV
         nu = rusanov_nu(grid, scheme.alpha)
         if nu is not None:
             desc.nu = nu.data_ptr()
-            keep = (nu,)
+            keep = keep + (nu,)
     elif isinstance(scheme, burgers.Godunov):
         desc.flux = _lib.PSA_FLUX_BURGERS_UPWIND
     elif isinstance(scheme, advection.Godunov):
@@ -406,9 +414,10 @@ def evolve(stepper: Stepper, u0: torch.Tensor, *, maxit: int | None = None,
 
     Steps are enqueued ``chunk`` at a time (one chunk of exactly ``maxit``
     steps when only ``maxit`` bounds the run) and the device-side control
-    block is read back once per chunk. ``u0`` is not modified.
+    block is read back once per chunk. ``u0`` is not modified. ``tiles``
+    as in :func:`fused_plan`.
     """
-    plan = fused_plan(stepper)
+    plan = fused_plan(stepper, tiles=tiles)
     if plan is None:
         event = None
         for event in step(stepper, u0, maxit=maxit, tstart=tstart, tfinal=tfinal):

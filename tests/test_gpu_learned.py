@@ -1,0 +1,188 @@
+"""The learned-weight solvers on the GPU: the fused operator against the CPU
+oracle, the WENO-JS fallback bit for bit, fused against generic stepping,
+batched against single-instance. The weights are FP32 (softmax in FP32) and
+the hidden layers BF16, so the oracle comparison bounds the bulk of the
+cells at the FP32-weight level and allows rare BF16 rounding-boundary flips
+(see ``test_gpu_mlp.py``)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import mlp_np as M
+from oracle import weno_np as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+
+    from pyshocks_amd import _lib
+
+    _lib.load()
+    return torch
+
+
+def _weights(p, tau):
+    from pyshocks_amd.learned import MLPWeights
+
+    return MLPWeights.from_numpy(p, tau)
+
+
+def _params(seed=12, scale=0.3):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(M.NPARAM) * scale).astype(np.float32)
+
+
+def _u0(n, dtype=np.float64):
+    x = (np.arange(n + 6) - 3 + 0.5) / n
+    u = 0.5 + np.sin(2 * np.pi * x) + 0.3 * np.sin(10 * np.pi * x + 1.0) + 0.5 * (x > 0.4)
+    u[:3] = u[n:n + 3]
+    u[-3:] = u[3:6]
+    return u.astype(dtype)
+
+
+def _stepper(torch, kind, grid, rec):
+    from pyshocks_amd.burgers import Godunov, Rusanov
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import make_ssprk33
+
+    sch = (Rusanov if kind == "rusanov" else Godunov)(rec=rec)
+    return make_ssprk33(sch, grid, PeriodicBoundary(), theta=1.0)
+
+
+@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+def test_learned_operator_vs_oracle(torch_mod, kind, dtype_name):
+    torch = torch_mod
+    from pyshocks_amd import apply_operator, make_uniform_cell_grid
+    from pyshocks_amd.burgers import Godunov, Rusanov
+    from pyshocks_amd.reconstruction import LearnedWENOJS53
+    from pyshocks_amd.scalar import PeriodicBoundary
+
+    dtype = getattr(torch, dtype_name)
+    n = 3000
+    p = _params()
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, dtype=dtype)
+    rec = LearnedWENOJS53(eps=1e-6, weights=_weights(p, 20.0))
+    sch = (Rusanov if kind == "rusanov" else Godunov)(rec=rec)
+    u = _u0(n, np.float64 if dtype_name == "float64" else np.float32)
+    L = apply_operator(sch, grid, PeriodicBoundary(), 0.0,
+                       torch.tensor(u, device="cuda")).cpu().double().numpy()[3:-3]
+    og = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    ref = M.learned_apply_operator(p, 20.0, kind, og, u, 1e-6)[3:-3]
+    scale = np.max(np.abs(ref))
+    d = np.abs(L - ref) / scale
+    tol = 1e-5 if dtype_name == "float64" else 2e-4
+    assert np.mean(d < tol) > 0.99, np.sort(d)[-10:]
+    assert np.max(d) < 5e-2
+    # conservation: the two tiles sharing a face compute the same flux
+    assert abs(np.sum(L)) * grid.h < (1e-12 if dtype_name == "float64" else 1e-4) * scale
+
+
+@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+def test_fallback_is_wenojs_bit_for_bit(torch_mod, kind, dtype_name):
+    """tau = 0 flags every cell with any variation: the learned solver must
+    then reproduce the WENO-JS solver exactly."""
+    torch = torch_mod
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.reconstruction import LearnedWENOJS53, WENOJS53
+    from pyshocks_amd.timestepping import evolve
+
+    dtype = getattr(torch, dtype_name)
+    n = 5000
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, dtype=dtype)
+    u0 = torch.tensor(_u0(n), device="cuda").to(dtype)
+    plain = evolve(_stepper(torch, kind, grid, WENOJS53(eps=1e-6)), u0, maxit=30,
+                   tiles=(256, 2))
+    learned = evolve(_stepper(torch, kind, grid,
+                              LearnedWENOJS53(eps=1e-6, weights=_weights(_params(), 0.0))),
+                     u0, maxit=30, tiles=(256, 2))
+    assert learned.iteration == plain.iteration == 30
+    assert torch.equal(learned.u[3:-3], plain.u[3:-3])
+    assert float(learned.t) == float(plain.t)
+
+
+@pytest.mark.parametrize("tiles", [(256, 2), (256, 4)])
+def test_learned_fused_vs_generic(torch_mod, tiles):
+    torch = torch_mod
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.reconstruction import LearnedWENOJS53
+    from pyshocks_amd.timestepping import SSPRK33, evolve, step
+
+    n = 2048
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    rec = LearnedWENOJS53(eps=1e-6, weights=_weights(_params(), 20.0))
+    st = _stepper(torch, "rusanov", grid, rec)
+    u0 = torch.tensor(_u0(n), device="cuda")
+    fused = evolve(st, u0, maxit=10, tiles=tiles)
+    generic = SSPRK33(predict_timestep=lambda t, u: st.predict_timestep(t, u),
+                      source=lambda t, u: st.source(t, u), checkpoint=None)
+    ev = None
+    for ev in step(generic, u0, maxit=10):
+        pass
+    assert ev is not None and ev.iteration == 10
+    a, b = fused.u.cpu().numpy()[3:-3], ev.u.cpu().numpy()[3:-3]
+    assert np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+def test_batched_learned_equals_single_instance(torch_mod, dtype_name):
+    torch = torch_mod
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+    from pyshocks_amd.reconstruction import LearnedWENOJS53
+    from pyshocks_amd.timestepping import evolve
+
+    dtype = getattr(torch, dtype_name)
+    w = _weights(_params(), 20.0)
+    _, ics = random_instances(4, seed=8)
+    ns = np.array([5, 300, 1500, 4100])
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    solver = BatchedSolver(ns, dtype=dtype, tiles=(256, 4), grid=2, mlp=w)
+    solver.set_states(states)
+    res = solver.solve(0.04)
+    assert np.all(res.status == 0)
+    for i, n in enumerate(ns):
+        grid = make_uniform_cell_grid(0.0, 1.0, int(n), nghosts=3, dtype=dtype)
+        st = _stepper(torch, "rusanov", grid, LearnedWENOJS53(eps=1e-6, weights=w))
+        ev = evolve(st, torch.tensor(states[i], device="cuda").to(dtype), tfinal=0.04,
+                    tiles=(256, 4))
+        assert res.steps[i] == ev.iteration
+        got = solver.state(i).cpu().double().numpy()[3:-3]
+        ref = ev.u.cpu().double().numpy()[3:-3]
+        # FP32: the two kernels round the state differently in the last ulp
+        # (contraction), which moves a few BF16 inputs across a rounding
+        # boundary over the ~100 steps
+        tol = 1e-10 if dtype == torch.float64 else 5e-4
+        assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref)
+
+
+def test_ideal_network_is_fifth_order_on_smooth_data(torch_mod):
+    """All-zero parameters and no fallback give the linear 5th-order
+    scheme: on smooth advection-free Burgers data before the shock its error
+    is below WENO-JS's with eps = 1e-6."""
+    torch = torch_mod
+    from oracle import exact as E
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.reconstruction import LearnedWENOJS53, WENOJS53
+    from pyshocks_amd.timestepping import evolve
+
+    ic = E.SineStepIC(c=0.5, amplitudes=(0.3,), wavenumbers=(1,), phases=(0.0,))
+    n, t = 128, 0.3
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    u0 = torch.tensor(E.exact_cell_averages(ic, n), device="cuda")
+    h = 1.0 / n
+    f = h * np.arange(0, n + 1)
+    exact = np.diff(E.hopf_lax_primitive(ic, f, t)) / h
+    errs = {}
+    for name, rec in [("js", WENOJS53(eps=1e-6)),
+                      ("ideal", LearnedWENOJS53(eps=1e-6,
+                                                weights=_weights(M.ideal_params(), 1e30)))]:
+        ev = evolve(_stepper(torch, "rusanov", grid, rec), u0, tfinal=t)
+        errs[name] = np.sum(np.abs(ev.u.cpu().numpy()[3:-3] - exact)) * h
+    assert errs["ideal"] < errs["js"], errs
