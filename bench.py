@@ -55,13 +55,15 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
                    help="steps of the per-kernel event-timed pass")
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="target duration of the CPU baseline sample (0: skip)")
-    p.add_argument("--workload", choices=["single", "batched", "adjoint"],
+    p.add_argument("--workload", choices=["single", "batched", "adjoint", "learned"],
                    default="single",
                    help="single: the large single instance (default); batched: "
                         "4096 uneven Burgers instances in one persistent graph "
                         "launch; adjoint: dJ/du0 at N = 16384, 2000 steps")
     p.add_argument("--no-adjoint", action="store_true",
                    help="skip the adjoint gradients/s figure of the default run")
+    p.add_argument("--no-learned", action="store_true",
+                   help="skip the learned-vs-plain equal-error figure of the default run")
     p.add_argument("--ninst", type=int, default=4096)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain device copy of N FP64 values (N * 8 "
@@ -338,6 +340,64 @@ def batched_cpu_leg(args: argparse.Namespace, ns, states) -> dict | None:
                       f"(FP64) each to t = 0.3, {elapsed:.1f} s"}
 
 
+def learned_leg(args: argparse.Namespace, *, ninst: int = 128, nmax: int = 8192,
+                full: bool = False, plain_single_value: float | None = None) -> dict:
+    """Learned-weight batched solver (FP32, shipped network) against the
+    plain WENO-JS5 FP64 path at equal L1 error, on seeded sine-plus-step
+    Burgers instances with N log-uniform in [1024, nmax], t = 0.5.
+
+    L1 errors are against exact Hopf-Lax cell averages. The plain path's
+    error-vs-N exponent p comes from the same instances at 2 N (the subset
+    N <= 8192 when ``full``); matching the learned error needs
+    N' = N r^(1/p) plain cells, r = mean E_plain / mean E_learned, i.e.
+    r^(2/p) times the cell-updates (cells and CFL steps both grow with N).
+    Effective throughput = the plain cell-updates needed / learned wall time."""
+    import numpy as np
+    import torch
+
+    from pyshocks_amd import autotune
+    from pyshocks_amd.learned import MLPWeights, l1_errors, training_ics
+
+    w = MLPWeights.load()
+    rng = np.random.default_rng(5000 + args.seed)
+    ns = np.round(np.exp(rng.uniform(np.log(1024), np.log(nmax), ninst))).astype(np.int64)
+    ics = training_ics(ninst, seed=5000 + args.seed)
+    from pyshocks_amd import exact
+
+    u0s = exact.burgers_cell_averages(ics[:16], ns[:16], 0.0)
+    best, _ = autotune.tune_batch(ns[:16], [u.cpu().numpy() for u in u0s],
+                                  dtype=torch.float32, tfinal=0.01, mlp=w,
+                                  candidates=((256, 4), (512, 2), (1024, 2)))
+    e_l, info_l = l1_errors(ics, ns, 0.5, mlp=w, dtype=torch.float32, tiles=best)
+    e_p, info_p = l1_errors(ics, ns, 0.5, dtype=torch.float64)
+    sub = ns <= 8192
+    e_p2, _ = l1_errors([ic for ic, k in zip(ics, sub) if k], 2 * ns[sub], 0.5,
+                        dtype=torch.float64)
+    p = float(np.log2(e_p[sub].mean() / e_p2.mean()))
+    r = float(e_p.mean() / e_l.mean())
+    scale = r ** (2.0 / p) if p > 0 else float("nan")
+    cu_plain_equal = info_p["cell_updates"] * scale
+    plain_batched = info_p["cell_updates"] / info_p["seconds"]
+    learned_tp = info_l["cell_updates"] / info_l["seconds"]
+    out = {
+        "config": f"{ninst} seeded sine+step Burgers instances, N log-uniform "
+                  f"[1024, {nmax}], t = 0.5, learned FP32 batched (tiles {list(best)}) vs "
+                  f"plain WENO-JS5 FP64",
+        "l1_plain_f64": float(e_p.mean()),
+        "l1_learned_f32": float(e_l.mean()),
+        "error_ratio": r,
+        "plain_order": p,
+        "learned_cell_updates_per_s": learned_tp,
+        "learned_seconds": info_l["seconds"],
+        "plain_f64_batched_cell_updates_per_s": plain_batched,
+        "effective_cell_updates_per_s": cu_plain_equal / info_l["seconds"],
+        "speedup_vs_plain_f64_batched": (cu_plain_equal / plain_batched) / info_l["seconds"],
+    }
+    if plain_single_value:
+        out["speedup_vs_plain_f64_single"] = (cu_plain_equal / plain_single_value) / info_l["seconds"]
+    return out
+
+
 def main(argv: list[str] | None = None) -> int:
     args = parse_args(argv)
     if args.copy_calibration:
@@ -353,6 +413,14 @@ def main(argv: list[str] | None = None) -> int:
                                                  "f32, each to t = 0.3, one graph launch"},
                           "batched": b, "cpu_baseline": cpu}), flush=True)
         return 0
+    if args.workload == "learned":
+        lr = learned_leg(args, ninst=1024, nmax=32768, full=True)
+        print(json.dumps({"metric": METRIC, "value": lr["effective_cell_updates_per_s"],
+                          "unit": "cell-updates/s (plain FP64 cell-updates at equal L1 "
+                                  "error per learned second)",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+                          "data": "synthetic", "learned": lr}), flush=True)
+        return 0
     if args.workload == "adjoint":
         a = adjoint_leg(args)
         print(json.dumps({"metric": METRIC, "value": a["gradients_per_s"],
@@ -367,6 +435,8 @@ def main(argv: list[str] | None = None) -> int:
 
     gpu = gpu_leg(args)
     adjoint = None if args.no_adjoint else adjoint_leg(args)
+    learned = None if args.no_learned else learned_leg(
+        args, plain_single_value=gpu["value"] if args.dtype == "f64" else None)
     cpu = cpu_leg(args, gpu)
 
     traffic = None
@@ -415,6 +485,7 @@ def main(argv: list[str] | None = None) -> int:
         },
         "cpu_baseline": cpu,
         "adjoint": adjoint,
+        "learned": learned,
     }
     print(json.dumps(line), flush=True)
     return 0

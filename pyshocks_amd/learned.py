@@ -206,3 +206,197 @@ class Trainer:
 
 
 # }}}
+
+
+# {{{ training data
+
+
+def training_ics(count: int, seed: int = 0) -> list:
+    """Seeded sine-plus-step initial conditions ``c + sum_{k=1..8} A_k
+    sin(2 pi k x + phi_k) + H 1[x >= xs]`` with ``c ~ U[0, 1]``,
+    ``A_k ~ U[-1, 1] / k``, ``H ~ U[-1, 1]``, ``xs ~ U(0.05, 0.95)``."""
+    from pyshocks_amd.funcs import SineStep
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        c = rng.uniform(0.0, 1.0)
+        amps = rng.uniform(-1.0, 1.0, 8) / np.arange(1, 9)
+        phis = rng.uniform(0.0, 2.0 * np.pi, 8)
+        out.append(SineStep(c=float(c), amplitudes=tuple(float(a) for a in amps),
+                            wavenumbers=tuple(range(1, 9)),
+                            phases=tuple(float(p) for p in phis),
+                            step_height=float(rng.uniform(-1.0, 1.0)),
+                            step_location=float(rng.uniform(0.05, 0.95))))
+    return out
+
+
+def _js_weights(v: torch.Tensor, eps: float) -> tuple[torch.Tensor, torch.Tensor]:
+    """WENO-JS weights (``weno.py:247-256``) and indicators of right-side
+    stencils ``v`` (..., 5), FP64."""
+    um2, um1, u0, up1, up2 = v.unbind(-1)
+    b0 = 13 / 12 * (um2 - 2 * um1 + u0) ** 2 + 1 / 4 * (um2 - 4 * um1 + 3 * u0) ** 2
+    b1 = 13 / 12 * (um1 - 2 * u0 + up1) ** 2 + 1 / 4 * (um1 - up1) ** 2
+    b2 = 13 / 12 * (u0 - 2 * up1 + up2) ** 2 + 1 / 4 * (3 * u0 - 4 * up1 + up2) ** 2
+    beta = torch.stack([b0, b1, b2], dim=-1)
+    d = torch.tensor(IDEAL, dtype=v.dtype, device=v.device)
+    alpha = d / (eps + beta) ** 2
+    return alpha / alpha.sum(dim=-1, keepdim=True), beta
+
+
+def _candidates(v: torch.Tensor) -> torch.Tensor:
+    um2, um1, u0, up1, up2 = v.unbind(-1)
+    return torch.stack([11 / 6 * u0 - 7 / 6 * um1 + 2 / 6 * um2,
+                        2 / 6 * up1 + 5 / 6 * u0 - 1 / 6 * um1,
+                        -1 / 6 * up2 + 5 / 6 * up1 + 2 / 6 * u0], dim=-1)
+
+
+def stencil_records(u: torch.Tensor, uface: torch.Tensor, *, eps: float, tau: float,
+                    smin: float = 1.0e-9) -> torch.Tensor:
+    """Training records of every interior cell of the periodic states ``u``
+    (B, n + 6) against the exact face values ``uface`` (B, n + 1): the right
+    reconstruction of cell i with face i + 1/2 and the left one (mirrored
+    stencil) with face i - 1/2. Cells flagged as shocks (they keep WENO-JS)
+    and near-constant stencils get weight 0."""
+    n = u.shape[1] - 6
+    idx = torch.arange(3, n + 3, device=u.device)
+    v = torch.stack([u[:, idx + k] for k in (-2, -1, 0, 1, 2)], dim=-1)  # (B, n, 5)
+    recs = []
+    for side in ("right", "left"):
+        vs = v if side == "right" else v.flip(-1)
+        ustar = uface[:, 1:] if side == "right" else uface[:, :-1]
+        mean = (((vs[..., 0] + vs[..., 4]) + (vs[..., 1] + vs[..., 3])) + vs[..., 2]) * 0.2
+        s = (vs - mean[..., None]).abs().amax(dim=-1)
+        x = features(vs)
+        wjs, beta = _js_weights(vs, eps)
+        e = (_candidates(vs) - ustar[..., None]) / torch.where(s > 0, s, torch.ones_like(s))[..., None]
+        shock = beta.amax(dim=-1) > tau * (beta.amin(dim=-1) + eps)
+        c = ((s > smin) & ~shock).to(torch.float64)
+        rec = torch.cat([x.double(), e, wjs, c[..., None]], dim=-1)
+        recs.append(rec.reshape(-1, RECORD))
+    return torch.cat(recs).to(torch.float32)
+
+
+def make_training_samples(ics: list, *, n: int = 4096, times: tuple[float, ...] = (
+        0.01, 0.02, 0.04, 0.06, 0.08, 0.1, 0.13, 0.16, 0.2, 0.25, 0.3, 0.35, 0.4, 0.45, 0.5),
+        eps: float = 1.0e-6, tau: float = 20.0, device: str = "cuda") -> torch.Tensor:
+    """Records from plain WENO-JS5 / Rusanov / SSP-RK3 runs (FP64, the
+    batched solver) of ``ics`` on ``n`` cells, snapshotted at ``times``,
+    against the exact Hopf-Lax face values at the snapshot times."""
+    from pyshocks_amd import exact
+    from pyshocks_amd.batched import BatchedSolver
+
+    B = len(ics)
+    u0 = exact.burgers_cell_averages(ics, [n] * B, 0.0, device=device)
+    solver = BatchedSolver([n] * B, dtype=torch.float64, eps=eps, device=device)
+    out = []
+    for t in times:
+        solver.set_states(u0)
+        res = solver.solve(t)
+        if np.any(res.status != 0):
+            raise RuntimeError("a training run did not finish")
+        u = torch.stack([solver.state(i) for i in range(B)])
+        uf = exact.burgers_face_values(ics, n, res.t, device=device)
+        out.append(stencil_records(u, uf, eps=eps, tau=tau))
+    return torch.cat(out)
+
+
+# }}}
+
+
+# {{{ evaluation
+
+
+def l1_errors(ics: list, ns: np.ndarray, t: float, *, mlp: MLPWeights | None = None,
+              dtype: torch.dtype = torch.float64, eps: float = 1.0e-6,
+              device: str = "cuda", tiles: tuple[int, int] | None = None,
+              ) -> tuple[np.ndarray, dict[str, Any]]:
+    """Interior L1 errors ``sum_i |u_i - u_exact_i| h`` at time ``t`` of the
+    batched solver (WENO-JS, or the learned weights ``mlp``) on instances
+    ``ics`` with ``ns`` cells, against exact Hopf-Lax cell averages."""
+    from pyshocks_amd import exact
+    from pyshocks_amd.batched import BatchedSolver
+
+    ns = np.asarray(ns, dtype=np.int64)
+    u0 = exact.burgers_cell_averages(ics, ns, 0.0, device=device)
+    ue = exact.burgers_cell_averages(ics, ns, t, device=device)
+    solver = BatchedSolver(ns, dtype=dtype, eps=eps, device=device, mlp=mlp, tiles=tiles)
+    solver.set_states(u0)
+    torch.cuda.synchronize()
+    import time
+
+    t0 = time.perf_counter()
+    res = solver.solve(t)
+    elapsed = time.perf_counter() - t0
+    if np.any(res.status != 0):
+        raise RuntimeError("an evaluation run did not finish")
+    err = np.array([float((solver.state(i)[3:-3].double() - ue[i][3:-3]).abs().sum()) / n
+                    for i, n in enumerate(ns)])
+    return err, {"seconds": elapsed, "cell_updates": float(np.sum(ns * res.steps)),
+                 "steps": res.steps}
+
+
+# }}}
+
+
+def main(argv: list[str] | None = None) -> int:
+    """Train the network on the device and save it:
+    ``python -m pyshocks_amd.learned --out wenonet.npz``."""
+    import argparse
+    import time
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--out", default=str(TRAINED_PATH))
+    ap.add_argument("--ics", type=int, default=84)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--batch", type=int, default=32768)
+    ap.add_argument("--lr", type=float, default=1e-2)
+    ap.add_argument("--lambda-js", type=float, default=3e-3)
+    ap.add_argument("--delta", type=float, default=1e-12)
+    ap.add_argument("--tau", type=float, default=20.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--tau-sweep", type=float, nargs="*", default=[])
+    args = ap.parse_args(argv)
+
+    _lib.load()
+    t0 = time.perf_counter()
+    samples = make_training_samples(training_ics(args.ics, seed=1000 + args.seed), n=args.n,
+                                    tau=args.tau)
+    total = samples.shape[0]
+    samples = samples[samples[:, 11] > 0].contiguous()
+    print(f"[train] {total} records ({samples.shape[0]} used) in "
+          f"{time.perf_counter() - t0:.1f} s", flush=True)
+    w = MLPWeights.from_numpy(init_params(args.seed), args.tau)
+    tr = Trainer(w, lr=args.lr, lambda_js=args.lambda_js, delta=args.delta)
+    t0 = time.perf_counter()
+    tr.fit(samples, epochs=args.epochs, batch=args.batch, seed=args.seed)
+    torch.cuda.synchronize()
+    print(f"[train] {args.epochs} epochs in {time.perf_counter() - t0:.1f} s; loss per "
+          f"epoch {' '.join(f'{x:.4e}' for x in tr.history)}", flush=True)
+    w.save(args.out, loss=np.asarray(tr.history), ics=args.ics, n=args.n,
+           lambda_js=args.lambda_js, delta=args.delta, seed=args.seed)
+
+    # validation on unseen instances, FP64: plain WENO-JS, the untrained
+    # network (ideal weights d with the same fallback) and the trained one
+    ics = training_ics(32, seed=2000 + args.seed)
+    ns = np.round(np.exp(np.random.default_rng(args.seed).uniform(
+        np.log(1024), np.log(8192), 32))).astype(np.int64)
+    ideal = MLPWeights.from_numpy(np.zeros(NPARAM, np.float32), args.tau)
+    for tv in (0.05, 0.2, 0.5):
+        e0, _ = l1_errors(ics, ns, tv)
+        ei, _ = l1_errors(ics, ns, tv, mlp=ideal)
+        e1, _ = l1_errors(ics, ns, tv, mlp=w)
+        print(f"[train] validation t={tv}: mean L1 plain {e0.mean():.4e} ideal+fallback "
+              f"{ei.mean():.4e} learned {e1.mean():.4e} (learned/plain {e1.mean() / e0.mean():.4f},"
+              f" learned/ideal {e1.mean() / ei.mean():.4f})", flush=True)
+    for tau in args.tau_sweep:
+        wt = MLPWeights(w.params, tau)
+        e0, _ = l1_errors(ics, ns, 0.5)
+        e1, _ = l1_errors(ics, ns, 0.5, mlp=wt)
+        print(f"[train] tau {tau}: t=0.5 learned/plain {e1.mean() / e0.mean():.4f}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

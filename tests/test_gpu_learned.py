@@ -186,3 +186,45 @@ def test_ideal_network_is_fifth_order_on_smooth_data(torch_mod):
         ev = evolve(_stepper(torch, "rusanov", grid, rec), u0, tfinal=t)
         errs[name] = np.sum(np.abs(ev.u.cpu().numpy()[3:-3] - exact)) * h
     assert errs["ideal"] < errs["js"], errs
+
+
+def test_training_records(torch_mod):
+    torch = torch_mod
+    from pyshocks_amd.learned import make_training_samples, training_ics
+
+    ics = training_ics(2, seed=77)
+    rec = make_training_samples(ics, n=256, times=(0.02, 0.3), tau=20.0)
+    assert rec.shape == (2 * 2 * 2 * 256, 12)
+    r = rec.cpu().numpy()
+    assert np.all(np.isfinite(r))
+    assert np.all(np.abs(r[:, :5]) <= 1.0)
+    np.testing.assert_allclose(r[:, 8:11].sum(axis=1), 1.0, atol=1e-6)
+    used = r[:, 11] > 0
+    assert 0.5 < used.mean() < 1.0  # shock cells are left out
+    # on used (smooth) records the ideal combination beats the worst candidate
+    ideal = r[used, 5:8] @ np.array([0.1, 0.6, 0.3])
+    worst = np.max(np.abs(r[used, 5:8]), axis=1)
+    assert np.median(np.abs(ideal) / worst) < 0.2
+
+
+def test_trained_network_gain(torch_mod):
+    """The shipped network against plain WENO-JS and against the untrained
+    network (ideal weights with the same shock fallback) on unseen seeded
+    instances, FP64, before and after shock formation."""
+    torch = torch_mod
+    from pyshocks_amd.learned import MLPWeights, l1_errors, training_ics
+
+    w = MLPWeights.load()
+    ideal = MLPWeights.from_numpy(np.zeros(M.NPARAM, np.float32), w.tau)
+    ics = training_ics(16, seed=3000)
+    ns = np.round(np.exp(np.random.default_rng(3000).uniform(np.log(1024), np.log(4096),
+                                                             16))).astype(np.int64)
+    for t in (0.1, 0.5):
+        e_plain, _ = l1_errors(ics, ns, t)
+        e_ideal, _ = l1_errors(ics, ns, t, mlp=ideal)
+        e_learned, _ = l1_errors(ics, ns, t, mlp=w)
+        assert e_learned.mean() < 0.85 * e_plain.mean(), (t, e_learned.mean(), e_plain.mean())
+        assert e_learned.mean() < 0.95 * e_ideal.mean(), (t, e_learned.mean(), e_ideal.mean())
+        # FP32 state keeps the gain
+        e32, _ = l1_errors(ics, ns, t, mlp=w, dtype=torch.float32)
+        assert e32.mean() < 0.9 * e_plain.mean()
