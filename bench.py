@@ -65,6 +65,14 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
     p.add_argument("--no-learned", action="store_true",
                    help="skip the learned-vs-plain equal-error figure of the default run")
     p.add_argument("--ninst", type=int, default=4096)
+    p.add_argument("--schedule", choices=["default", "tile", "pipeline"], default="default",
+                   help="stage-kernel schedule (one workgroup per tile, or the persistent "
+                        "pipelined grid)")
+    p.add_argument("--learned", action="store_true",
+                   help="single workload with the shipped learned weights")
+    p.add_argument("--team-cells", type=int, default=0,
+                   help="batched workload: cells per team member (0 = no teams)")
+    p.add_argument("--max-team", type=int, default=32)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain device copy of N FP64 values (N * 8 "
                         "bytes read and written per launch), the reference "
@@ -100,6 +108,10 @@ def make_problem(args: argparse.Namespace):
     grid = make_uniform_cell_grid(0.0, 1.0, args.n, nghosts=3, dtype=dtype,
                                   device="cuda")
     rec = WENOJS53(eps=1.0e-6)
+    if args.learned:
+        from pyshocks_amd.reconstruction import LearnedWENOJS53
+
+        rec = LearnedWENOJS53(eps=1.0e-6)
     if args.scheme == "rusanov":
         scheme, theta = Rusanov(rec=rec), 1.0
     elif args.scheme == "godunov":
@@ -112,6 +124,7 @@ def make_problem(args: argparse.Namespace):
     assert plan is not None
     plan.desc.block = args.block
     plan.desc.cells_per_thread = args.cells_per_thread
+    plan.desc.schedule = {"default": 0, "tile": 1, "pipeline": 2}[args.schedule]
     u0 = funcs.cell_averages(funcs.random_sine_step(args.seed), grid)
     return grid, plan, u0, theta
 
@@ -291,7 +304,8 @@ def batched_leg(args: argparse.Namespace) -> dict:
                                                         replace=False)
     best, _ = autotune.tune_batch(ns[sample], [states[i] for i in sample],
                                   dtype=torch.float32, tfinal=0.01)
-    solver = BatchedSolver(ns, dtype=torch.float32)
+    solver = BatchedSolver(ns, dtype=torch.float32, team_cells=args.team_cells or None,
+                           max_team=args.max_team)
     solver.set_states(states)
     u0 = solver.u.clone()
     graph = solver.capture(u0, 0.3)

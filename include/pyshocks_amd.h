@@ -172,7 +172,16 @@ typedef struct psa_solver_desc {
   const float *mlp;             /* learned weights (PSA_MLP_NPARAM FP32) or
                                    NULL for WENO-JS                        */
   double mlp_tau;               /* shock threshold of the learned weights  */
+  int schedule;                 /* stages 1-3: PSA_SCHEDULE_*              */
+  int pad_;
 } psa_solver_desc;
+
+/* Stage-kernel schedules: one workgroup per tile, or a persistent grid that
+ * copies its next tile into LDS while computing the current one (WENO-JS
+ * weights, no velocity array); both compute the same values. */
+#define PSA_SCHEDULE_DEFAULT 0
+#define PSA_SCHEDULE_TILE 1
+#define PSA_SCHEDULE_PIPELINE 2
 
 /* Initialise the control block (device memory, sizeof(psa_control)) for a
  * run starting from state u at time tstart: stores t, tfinal, maxit, the dt
@@ -240,7 +249,28 @@ typedef struct psa_batch_desc {
   const float *mlp;             /* learned weights or NULL (see
                                    psa_solver_desc)                        */
   double mlp_tau;               /* shock threshold of the learned weights  */
+  int time_f64;                 /* 0: t and dt in the state dtype, as
+                                   timestepping.py:136-152; 1: in FP64
+                                   (FP32 states then end exactly at tfinal) */
+  int max_team;                 /* largest team size (teams only)          */
+  /* Teams (all NULL/0: one workgroup per instance, in `order`). Instance i
+   * is advanced by team[i] workgroups that split its tiles and meet at a
+   * device-wide barrier after every stage; the queue is then the nslots
+   * slots (slot_inst[s], slot_rank[s]) listing each instance's members
+   * consecutively, ranks 0 .. team[i] - 1. team_state is caller-owned device
+   * scratch of ninst * PSA_TEAM_STATE_BYTES bytes, cleared by the call.
+   * Every team must fit the grid, which must fit the device at once; a
+   * member waiting longer than team_wait_s seconds (0 = 60) at a barrier
+   * stops its instance with status PSA_ERR_HIP. */
+  const int *team;              /* [ninst] team sizes                      */
+  const int *slot_inst;         /* [nslots] instance of each slot          */
+  const int *slot_rank;         /* [nslots] member rank of each slot       */
+  long long nslots;
+  void *team_state;
+  double team_wait_s;
 } psa_batch_desc;
+
+#define PSA_TEAM_STATE_BYTES 64
 
 int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1, void *k2,
                     unsigned int *counter, double *t_out, long long *steps_out,

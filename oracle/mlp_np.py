@@ -9,10 +9,13 @@ round-to-nearest-even, everything else in FP64 where the kernels use FP32
 accumulators):
 
 * inputs of a stencil ``v``: ``mean = (((v0 + v4) + (v1 + v3)) + v2) * 0.2``,
-  ``s = max |v - mean|``, ``x = (v - mean) / s`` (0 if ``s == 0``), in the
+  ``s = max |v - mean|``, ``x = (v - mean) * (1 / s)`` (0 if ``s == 0``), in the
   state dtype, then FP32;
 * ``h1 = relu(W1 x + b1)``, ``h2 = relu(W2 h1 + b2)``,
   ``z = W3 h2 + b3 + ln d``, ``omega = softmax(z)``;
+* the value is ``h1 + omega0 (h0 - h1) + omega2 (h2 - h1)`` of the three
+  candidates ``h`` (the weighted sum, insensitive to the FP32 rounding of
+  ``sum omega``);
 * a cell is flagged (keeps WENO-JS) when ``max beta > tau (min beta + eps)``.
 
 Pins (``tests/test_oracle_mlp.py``): the unrounded gradient against central
@@ -74,8 +77,8 @@ def features(v: np.ndarray) -> np.ndarray:
     mean = (((v[..., 0] + v[..., 4]) + (v[..., 1] + v[..., 3])) + v[..., 2]) * dt(0.2)
     d = v - mean[..., None]
     s = np.max(np.abs(d), axis=-1)
-    with np.errstate(invalid="ignore", divide="ignore"):
-        x = np.where(s[..., None] > 0, d / np.where(s > 0, s, 1)[..., None], 0)
+    inv = dt(1) / np.where(s > 0, s, dt(1))
+    x = np.where(s[..., None] > 0, d * inv[..., None], dt(0))
     return x.astype(np.float32)
 
 
@@ -168,6 +171,12 @@ def candidates(v: np.ndarray) -> np.ndarray:
                      -1 / 6 * up2 + 5 / 6 * up1 + 2 / 6 * u0], axis=1)
 
 
+def _combine(w: np.ndarray, h: np.ndarray) -> np.ndarray:
+    """``sum_k w_k h_k`` as ``h1 + w0 (h0 - h1) + w2 (h2 - h1)``: equal when
+    the weights sum to 1, and insensitive to their rounding otherwise."""
+    return (h[:, 1] + w[:, 0] * (h[:, 0] - h[:, 1])) + w[:, 2] * (h[:, 2] - h[:, 1])
+
+
 def learned_reconstruct(p: np.ndarray, tau: float, u: np.ndarray, eps: float, *,
                         emulate: bool = True) -> tuple[np.ndarray, np.ndarray]:
     """``(ul, ur)`` of every cell: learned weights, WENO-JS where flagged."""
@@ -177,8 +186,8 @@ def learned_reconstruct(p: np.ndarray, tau: float, u: np.ndarray, eps: float, *,
     x = features(v)
     wr = forward(p, x, emulate=emulate)["omega"]
     wl = forward(p, x[:, ::-1], emulate=emulate)["omega"]
-    ur = np.sum(wr * candidates(v.astype(np.float64)), axis=1)
-    ul = np.sum(wl * candidates(vm.astype(np.float64)), axis=1)
+    ur = _combine(wr, candidates(v.astype(np.float64)))
+    ul = _combine(wl, candidates(vm.astype(np.float64)))
     flag = shock_flags(v, eps, tau)
     jl, jr = weno_np.reconstruct_wenojs(u.astype(np.float64), eps)
     return np.where(flag, jl, ul), np.where(flag, jr, ur)

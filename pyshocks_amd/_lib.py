@@ -78,6 +78,8 @@ class PsaSolverDesc(ctypes.Structure):
         ("cells_per_thread", ctypes.c_int),
         ("mlp", ctypes.c_void_p),
         ("mlp_tau", ctypes.c_double),
+        ("schedule", ctypes.c_int),
+        ("pad_", ctypes.c_int),
     ]
 
 
@@ -100,6 +102,14 @@ class PsaBatchDesc(ctypes.Structure):
         ("cells_per_thread", ctypes.c_int),
         ("mlp", ctypes.c_void_p),
         ("mlp_tau", ctypes.c_double),
+        ("time_f64", ctypes.c_int),
+        ("max_team", ctypes.c_int),
+        ("team", ctypes.c_void_p),
+        ("slot_inst", ctypes.c_void_p),
+        ("slot_rank", ctypes.c_void_p),
+        ("nslots", ctypes.c_longlong),
+        ("team_state", ctypes.c_void_p),
+        ("team_wait_s", ctypes.c_double),
     ]
 
 
@@ -153,6 +163,8 @@ SIGNATURES: dict[str, list[Any]] = {
     "psa_learned_reconstruct": [_I, _VP, _D, _VP, _VP, _VP, _I64, _D, _VP],
 }
 
+TEAM_STATE_BYTES = 64
+PSA_SCHEDULE_DEFAULT, PSA_SCHEDULE_TILE, PSA_SCHEDULE_PIPELINE = 0, 1, 2
 MLP_NPARAM = 1347
 MLP_RECORD = 12
 

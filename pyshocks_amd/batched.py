@@ -44,7 +44,9 @@ class BatchedSolver:
                  a: float = 0.0, b: float = 1.0, nghosts: int = 3,
                  device: torch.device | str = "cuda",
                  tiles: tuple[int, int] | None = None, grid: int = 0,
-                 mlp: Any = None) -> None:
+                 mlp: Any = None, time_f64: bool = True,
+                 team_cells: int | None = None, max_team: int = 32,
+                 team_wait_s: float = 60.0) -> None:
         if scheme not in ("rusanov", "godunov"):
             raise NotImplementedError(
                 f"the batched solver takes Burgers 'rusanov' or 'godunov', not {scheme!r}")
@@ -98,6 +100,9 @@ class BatchedSolver:
         self.desc.eps = float(eps)
         self.desc.theta = float(theta)
         self.desc.max_steps = 4_000_000
+        # FP64 time keeps FP32 instances on schedule over ~10^5 steps; False
+        # accumulates t in the state dtype as the reference's step loop does
+        self.desc.time_f64 = 1 if time_f64 else 0
         self.mlp = mlp
         if mlp is not None:
             # learned weights (pyshocks_amd.learned.MLPWeights)
@@ -109,6 +114,39 @@ class BatchedSolver:
             tiles = lookup_batch(self.desc.dtype, self.desc.flux, mlp is not None)
         if tiles is not None:
             self.desc.block, self.desc.cells_per_thread = int(tiles[0]), int(tiles[1])
+        self._set_teams(order, team_cells, max_team, team_wait_s)
+
+    def _set_teams(self, order: np.ndarray, team_cells: int | None, max_team: int,
+                   wait_s: float) -> None:
+        """Instances above ``team_cells`` cells are split among a team of
+        ``ceil(m / team_cells)`` workgroups (at most ``max_team``, at most one
+        per tile) that meet at a device-wide barrier after every stage."""
+        tile = (self.desc.block or 256) * (self.desc.cells_per_thread or 4)
+        ntiles = (self.m + tile - 1) // tile
+        if team_cells:
+            team = np.clip((self.m + team_cells - 1) // team_cells, 1,
+                           np.minimum(max_team, ntiles)).astype(np.int32)
+        else:
+            team = np.ones(self.ninst, dtype=np.int32)
+        self.team = team
+        if np.all(team == 1):
+            self.desc.team = None
+            return
+        slot_inst = np.repeat(order, team[order]).astype(np.int32)
+        slot_rank = np.concatenate([np.arange(team[i]) for i in order]).astype(np.int32)
+        dev = self.device
+        self._team = torch.tensor(team, dtype=torch.int32, device=dev)
+        self._slot_inst = torch.tensor(slot_inst, dtype=torch.int32, device=dev)
+        self._slot_rank = torch.tensor(slot_rank, dtype=torch.int32, device=dev)
+        self._team_state = torch.zeros(self.ninst * _lib.TEAM_STATE_BYTES, dtype=torch.uint8,
+                                       device=dev)
+        self.desc.team = self._team.data_ptr()
+        self.desc.slot_inst = self._slot_inst.data_ptr()
+        self.desc.slot_rank = self._slot_rank.data_ptr()
+        self.desc.nslots = int(slot_inst.size)
+        self.desc.max_team = int(team.max())
+        self.desc.team_state = self._team_state.data_ptr()
+        self.desc.team_wait_s = float(wait_s)
 
     @property
     def ninst(self) -> int:

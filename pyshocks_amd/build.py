@@ -30,6 +30,8 @@ HIPCC_FLAGS = [
     # no packed FP32 instructions: hipcc would otherwise form them from
     # scalar float code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core results in VGPRs (no copies out of accumulation registers)
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-command-line-argument",

@@ -1,18 +1,27 @@
 // Batched solver: many independent periodic Burgers instances of uneven size
 // advanced to their final times in one persistent launch.
 //
-// Each workgroup takes the next instance from a queue (an atomic counter over
-// a caller-provided order, longest first) and runs its whole time loop: the
+// Work is a queue of slots (an atomic counter over a caller-provided list,
+// longest instances first). A slot is one member of an instance's team: an
+// instance of team size 1 is run by one workgroup alone, a large instance by
+// a team of workgroups that split its tiles and meet at a global barrier
+// after every stage. Every member runs the instance's whole time loop: the
 // CFL time step of timestepping.py:89-152 with predict_timestep of
 // burgers/schemes.py:42-49,121-127 (Rusanov with alpha = 1 or Godunov), and
-// the three SSP-RK3 stages of timestepping.py:312-320, each a sweep over the
-// instance's tiles with the same tile computation as the single-instance
-// stage kernels, separated by workgroup barriers. max|u| of each new state is
-// reduced inside the workgroup, so no step needs a grid-wide
-// synchronisation. A workgroup exits when the queue is empty; an instance
-// stops at its final time, at max_steps (status PSA_ERR_INVALID_ARGUMENT) or
-// at a non-finite time step or one too small to advance t
-// (PSA_ERR_NONFINITE_DT), so every workgroup terminates.
+// the three SSP-RK3 stages of timestepping.py:312-320, each a sweep over its
+// tiles with the same tile computation as the single-instance stage kernels.
+// max|u| of each new state is reduced in the workgroup and, for a team,
+// through an atomic in the team's state block; all members take the same dt
+// from the same reduced value, so they stop at the same step.
+//
+// Termination: the grid never exceeds the workgroups the device holds at
+// once, and teams never exceed the grid, so every slot is eventually taken
+// (slots are taken in order, so only the most recent team can be missing
+// members, and the workgroups of all other teams finish and take them). A
+// member that waits at a barrier longer than the wait limit marks the team
+// aborted (status PSA_ERR_HIP) and every member leaves. An instance stops at
+// its final time, at max_steps (status PSA_ERR_INVALID_ARGUMENT) or at a
+// non-finite time step or one too small to advance t (PSA_ERR_NONFINITE_DT).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -26,6 +35,16 @@
 namespace psa {
 namespace {
 
+// Synchronisation state of one team (PSA_TEAM_STATE_BYTES per instance).
+struct TeamState {
+  unsigned int arrive;  // barrier arrivals, monotone
+  int abort;
+  unsigned int pad0[2];
+  unsigned long long umax[2];  // max|u| bits, alternating between steps
+  unsigned long long pad1[4];
+};
+static_assert(sizeof(TeamState) == PSA_TEAM_STATE_BYTES, "team state size");
+
 template <typename T>
 struct BatchArgs {
   T *u;
@@ -35,16 +54,22 @@ struct BatchArgs {
   const int *ncells;
   const double *dx;
   const double *tfinal;
-  const int *order;
-  long long ninst;
+  const int *order;       // slot -> instance (nslots entries)
+  const int *slot_rank;   // slot -> member rank, or null (all ranks 0)
+  const int *team;        // instance -> team size, or null (all 1)
+  TeamState *teams;       // instance -> team state (needed if team != null)
+  long long nslots;
   unsigned int *counter;
   double *t_out;
   long long *steps_out;
   int *status_out;
   long long max_steps;
+  long long wait_ticks;   // barrier wait limit in wall-clock ticks
   T eps;
   T theta;
+  double theta64;
   int g;
+  int time_f64;
   const float *mlp;
   T tau;
 };
@@ -59,23 +84,66 @@ __device__ __forceinline__ unsigned long long block_max_bcast(
   return *s_out;
 }
 
+// Barrier of the `size` members of a team: arrivals count up monotonically,
+// so the k-th barrier (k = 1, 2, ...) waits for size * k arrivals. Writes of
+// the workgroup before the barrier are released at device scope and the
+// other members' writes acquired after it. Returns false if the team was
+// aborted (by a member that waited too long).
+__device__ __forceinline__ bool team_barrier(TeamState *ts, int size,
+                                             unsigned int k, long long wait_ticks,
+                                             int *s_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __atomic_thread_fence(__ATOMIC_RELEASE);  // device (agent) scope
+    __hip_atomic_fetch_add(&ts->arrive, 1u, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int target = static_cast<unsigned int>(size) * k;
+    const long long start = wall_clock64();
+    int ok = 1;
+    for (;;) {
+      const unsigned int got = __hip_atomic_load(&ts->arrive, __ATOMIC_ACQUIRE,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+      if (got >= target) break;
+      if (__hip_atomic_load(&ts->abort, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        ok = 0;
+        break;
+      }
+      if (wall_clock64() - start > wait_ticks) {
+        __hip_atomic_store(&ts->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    *s_flag = ok;
+  }
+  __syncthreads();
+  return *s_flag != 0;
+}
+
 template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED>
 __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
   constexpr int TILE = BLOCK * CPT;
   __shared__ StageSmem<T, 0, BLOCK, CPT> sm;
   __shared__ unsigned long long s_bcast;
   __shared__ long long s_next;
+  __shared__ int s_flag;
   const int tid = threadIdx.x;
   MlpWave W;
   if constexpr (LEARNED) mlp_load(W, a.mlp);
 
   for (;;) {
-    __syncthreads();  // previous instance done with the LDS and s_next
+    __syncthreads();  // previous slot done with the LDS and s_next
     if (tid == 0) s_next = static_cast<long long>(atomicAdd(a.counter, 1u));
     __syncthreads();
     const long long k = s_next;
-    if (k >= a.ninst) break;
+    if (k >= a.nslots) break;
     const int i = a.order[k];
+    const int rank = a.slot_rank != nullptr ? a.slot_rank[k] : 0;
+    const int size = a.team != nullptr ? a.team[i] : 1;
+    TeamState *ts = size > 1 ? a.teams + i : nullptr;
 
     StageArgs<T> p{};
     p.n = a.ncells[i];
@@ -91,78 +159,151 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
     T *k1 = a.k1 + a.offsets[i];
     T *k2 = a.k2 + a.offsets[i];
     const int64_t ntiles = (p.m + TILE - 1) / TILE;
+    // this member's tiles
+    const int64_t tlo = ntiles * rank / size;
+    const int64_t thi = ntiles * (rank + 1) / size;
+    unsigned int nbar = 0;
+    bool alive = true;
 
-    // max |u_int| of the initial state
+    // team reduction of max|u| bits into buffer b (a single workgroup keeps
+    // the block maximum)
+    auto team_max = [&](unsigned long long v, int b) -> unsigned long long {
+      v = block_max_bcast<BLOCK>(v, sm.red, &s_bcast);
+      if (ts == nullptr) return v;
+      if (tid == 0 && v != 0)
+        __hip_atomic_fetch_max(&ts->umax[b], v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      alive = alive && team_barrier(ts, size, ++nbar, a.wait_ticks, &s_flag);
+      if (tid == 0)
+        s_bcast = __hip_atomic_load(&ts->umax[b], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      return s_bcast;
+    };
+    auto barrier = [&]() {
+      if (ts == nullptr) {
+        __syncthreads();
+      } else {
+        alive = alive && team_barrier(ts, size, ++nbar, a.wait_ticks, &s_flag);
+      }
+    };
+
+    // max |u_int| of the initial state over this member's cells
     unsigned long long vb = 0;
-    for (int64_t j = a.g + tid; j < p.m - a.g; j += BLOCK) {
-      const unsigned long long b = abs_bits(u[j]);
-      vb = b > vb ? b : vb;
+    {
+      int64_t lo = tlo * TILE, hi = thi * TILE;
+      lo = lo > a.g ? lo : a.g;
+      hi = hi < p.m - a.g ? hi : p.m - a.g;
+      for (int64_t j = lo + tid; j < hi; j += BLOCK) {
+        const unsigned long long b = abs_bits(u[j]);
+        vb = b > vb ? b : vb;
+      }
     }
-    unsigned long long umax = block_max_bcast<BLOCK>(vb, sm.red, &s_bcast);
+    unsigned long long umax = team_max(vb, 1);
 
-    const T tf = static_cast<T>(a.tfinal[i]);
-    const T cfl = T(0.5) * p.dx;  // 0.5 dx^(2 - alpha), alpha = 1
+    // time control: in the state dtype (timestepping.py:136-152), or with
+    // time_f64 in FP64 (t and the unclipped dt; the stages use dt rounded to
+    // the state dtype and t advances by exactly that value)
+    const double tf64 = a.tfinal[i];
+    const double cfl64 = 0.5 * a.dx[i];  // 0.5 dx^(2 - alpha), alpha = 1
+    const T tf = static_cast<T>(tf64);
+    const T cfl = T(0.5) * p.dx;
     T t = T(0);
+    double t64 = 0.0;
     long long steps = 0;
-    int status = PSA_OK;
-    for (;;) {
-      if (t >= tf) break;
+    int status = alive ? PSA_OK : PSA_ERR_HIP;
+    while (alive) {
+      T dt;
+      if (a.time_f64) {
+        if (t64 >= tf64) break;
+      } else {
+        if (t >= tf) break;
+      }
       if (steps >= a.max_steps) {
         status = PSA_ERR_INVALID_ARGUMENT;
         break;
       }
-      T dt = a.theta * (cfl / from_bits<T>(umax));
-      if (!isinf(tf)) {
-        const T dt_min = tf - t;
-        dt = (dt < dt_min ? dt : dt_min) + static_cast<T>(1.0e-15);
-      }
-      // a step that does not advance t (dt absorbed by t) would never end
-      if (!isfinite(dt) || !(t + dt > t)) {
-        status = PSA_ERR_NONFINITE_DT;
-        break;
+      if (a.time_f64) {
+        double dt64 = a.theta64 * (cfl64 / static_cast<double>(from_bits<T>(umax)));
+        if (!isinf(tf64)) {
+          const double dt_min = tf64 - t64;
+          dt64 = (dt64 < dt_min ? dt64 : dt_min) + 1.0e-15;
+        }
+        dt = static_cast<T>(dt64);
+        if (!isfinite(dt) || !(dt > T(0)) || !(t64 + static_cast<double>(dt) > t64)) {
+          status = PSA_ERR_NONFINITE_DT;
+          break;
+        }
+      } else {
+        dt = a.theta * (cfl / from_bits<T>(umax));
+        if (!isinf(tf)) {
+          const T dt_min = tf - t;
+          dt = (dt < dt_min ? dt : dt_min) + static_cast<T>(1.0e-15);
+        }
+        // a step that does not advance t (dt absorbed by t) would never end
+        if (!isfinite(dt) || !(t + dt > t)) {
+          status = PSA_ERR_NONFINITE_DT;
+          break;
+        }
       }
 
       unsigned long long vdummy = 0;
       p.un = u;
       p.uk = u;
       p.out = k1;
-      for (int64_t tile = 0; tile < ntiles; ++tile) {
+      for (int64_t tile = tlo; tile < thi; ++tile) {
         __syncthreads();
         stage_tile<T, FLUX, 0, 1, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
                                                           vdummy, W);
       }
-      __syncthreads();  // k1 complete
+      barrier();  // k1 complete
+      // every member has read the max|u| reduced two steps ago: clear that
+      // buffer for the next step
+      if (ts != nullptr && rank == 0 && tid == 0)
+        __hip_atomic_store(&ts->umax[(steps + 1) & 1], 0ull, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
       p.uk = k1;
       p.out = k2;
-      for (int64_t tile = 0; tile < ntiles; ++tile) {
+      for (int64_t tile = tlo; tile < thi; ++tile) {
         __syncthreads();
         stage_tile<T, FLUX, 0, 2, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
                                                           vdummy, W);
       }
-      __syncthreads();  // k2 complete
+      barrier();  // k2 complete
       p.uk = k2;
       p.out = u;
       unsigned long long vmax = 0;
-      for (int64_t tile = 0; tile < ntiles; ++tile) {
+      for (int64_t tile = tlo; tile < thi; ++tile) {
         __syncthreads();
         stage_tile<T, FLUX, 0, 3, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
                                                        vmax, W);
       }
-      __syncthreads();  // u complete
-      umax = block_max_bcast<BLOCK>(vmax, sm.red, &s_bcast);
+      umax = team_max(vmax, static_cast<int>(steps & 1));  // u complete
       t = t + dt;
+      t64 = t64 + static_cast<double>(dt);
       steps += 1;
     }
-    if (tid == 0) {
-      a.t_out[i] = static_cast<double>(t);
+    if (!alive) status = PSA_ERR_HIP;
+    if (tid == 0 && rank == 0) {
+      a.t_out[i] = a.time_f64 ? t64 : static_cast<double>(t);
       a.steps_out[i] = steps;
       a.status_out[i] = status;
     }
   }
 }
 
-__global__ void k_batch_reset(unsigned int *counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+// Clears the slot counter and the team states (a kernel, so that the launch
+// can be captured in a graph).
+__global__ void k_batch_reset(unsigned int *counter, TeamState *teams,
+                              long long nteams) {
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx == 0) *counter = 0u;
+  for (long long j = idx; j < nteams; j += static_cast<long long>(gridDim.x) * blockDim.x) {
+    teams[j].arrive = 0u;
+    teams[j].abort = 0;
+    teams[j].umax[0] = 0ull;
+    teams[j].umax[1] = 0ull;
+  }
 }
 
 // Workgroups the device holds at once for a kernel (CUs x occupancy).
@@ -177,20 +318,35 @@ unsigned int resident(K kernel, int block) {
                             dev) != hipSuccess ||
       per_cu <= 0 || cus <= 0) {
     (void)hipGetLastError();
-    return 256;
+    return 0;
   }
   return static_cast<unsigned int>(per_cu) * static_cast<unsigned int>(cus);
 }
 
 template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED = false>
-int launch_batch_cfg(const BatchArgs<T> &a, int grid, hipStream_t s) {
-  unsigned int blocks =
-      grid > 0 ? static_cast<unsigned int>(grid)
-               : resident(k_batch<T, FLUX, BLOCK, CPT, LEARNED>, BLOCK);
-  if (static_cast<long long>(blocks) > a.ninst)
-    blocks = static_cast<unsigned int>(a.ninst);
+int launch_batch_cfg(const BatchArgs<T> &a, int grid, int max_team, long long nteams,
+                     hipStream_t s) {
+  const unsigned int cap = resident(k_batch<T, FLUX, BLOCK, CPT, LEARNED>, BLOCK);
+  if (cap == 0)
+    return set_error(PSA_ERR_HIP, "cannot query the resident workgroup count");
+  unsigned int blocks = grid > 0 ? static_cast<unsigned int>(grid) : cap;
+  if (blocks > cap && max_team > 1)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "teams need a grid the device holds at once (<= " +
+                         std::to_string(cap) + " workgroups)");
+  if (static_cast<long long>(blocks) > a.nslots)
+    blocks = static_cast<unsigned int>(a.nslots);
   if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.counter);
+  if (max_team > static_cast<int>(blocks))
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "a team of " + std::to_string(max_team) +
+                         " members does not fit the grid of " +
+                         std::to_string(blocks) + " workgroups");
+  const long long nt = a.teams != nullptr ? nteams : 0;
+  unsigned int rb = static_cast<unsigned int>((nt + 255) / 256);
+  if (rb < 1) rb = 1;
+  if (rb > 1024) rb = 1024;
+  hipLaunchKernelGGL(k_batch_reset, dim3(rb), dim3(256), 0, s, a.counter, a.teams, nt);
   int st = check_launch("batch reset");
   if (st != PSA_OK) return st;
   hipLaunchKernelGGL((k_batch<T, FLUX, BLOCK, CPT, LEARNED>), dim3(blocks),
@@ -200,24 +356,29 @@ int launch_batch_cfg(const BatchArgs<T> &a, int grid, hipStream_t s) {
 
 template <typename T, int FLUX>
 int launch_batch_tiles(const BatchArgs<T> &a, int block, int cpt, int grid,
-                       hipStream_t s) {
+                       int max_team, long long nteams, hipStream_t s) {
   if (a.mlp != nullptr) {
     if (block == 256 && cpt == 4)
-      return launch_batch_cfg<T, FLUX, 256, 4, true>(a, grid, s);
+      return launch_batch_cfg<T, FLUX, 256, 4, true>(a, grid, max_team, nteams, s);
     if (block == 512 && cpt == 2)
-      return launch_batch_cfg<T, FLUX, 512, 2, true>(a, grid, s);
+      return launch_batch_cfg<T, FLUX, 512, 2, true>(a, grid, max_team, nteams, s);
     if (block == 1024 && cpt == 2)
-      return launch_batch_cfg<T, FLUX, 1024, 2, true>(a, grid, s);
+      return launch_batch_cfg<T, FLUX, 1024, 2, true>(a, grid, max_team, nteams, s);
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "unsupported batch tile configuration for learned "
                      "weights block=" + std::to_string(block) +
                          " cells_per_thread=" + std::to_string(cpt));
   }
-  if (block == 256 && cpt == 4) return launch_batch_cfg<T, FLUX, 256, 4>(a, grid, s);
-  if (block == 256 && cpt == 2) return launch_batch_cfg<T, FLUX, 256, 2>(a, grid, s);
-  if (block == 512 && cpt == 2) return launch_batch_cfg<T, FLUX, 512, 2>(a, grid, s);
-  if (block == 512 && cpt == 4) return launch_batch_cfg<T, FLUX, 512, 4>(a, grid, s);
-  if (block == 1024 && cpt == 2) return launch_batch_cfg<T, FLUX, 1024, 2>(a, grid, s);
+  if (block == 256 && cpt == 4)
+    return launch_batch_cfg<T, FLUX, 256, 4>(a, grid, max_team, nteams, s);
+  if (block == 256 && cpt == 2)
+    return launch_batch_cfg<T, FLUX, 256, 2>(a, grid, max_team, nteams, s);
+  if (block == 512 && cpt == 2)
+    return launch_batch_cfg<T, FLUX, 512, 2>(a, grid, max_team, nteams, s);
+  if (block == 512 && cpt == 4)
+    return launch_batch_cfg<T, FLUX, 512, 4>(a, grid, max_team, nteams, s);
+  if (block == 1024 && cpt == 2)
+    return launch_batch_cfg<T, FLUX, 1024, 2>(a, grid, max_team, nteams, s);
   return set_error(PSA_ERR_INVALID_ARGUMENT,
                    "unsupported batch tile configuration block=" +
                        std::to_string(block) + " cells_per_thread=" +
@@ -236,8 +397,12 @@ int launch_batch(const psa_batch_desc *d, void *u, void *k1, void *k2,
   a.ncells = d->ncells;
   a.dx = d->dx;
   a.tfinal = d->tfinal;
-  a.order = d->order;
-  a.ninst = d->ninst;
+  const bool teams = d->team != nullptr;
+  a.order = teams ? d->slot_inst : d->order;
+  a.slot_rank = teams ? d->slot_rank : nullptr;
+  a.team = teams ? d->team : nullptr;
+  a.teams = teams ? static_cast<TeamState *>(d->team_state) : nullptr;
+  a.nslots = teams ? d->nslots : d->ninst;
   a.counter = counter;
   a.t_out = t_out;
   a.steps_out = steps_out;
@@ -245,15 +410,29 @@ int launch_batch(const psa_batch_desc *d, void *u, void *k1, void *k2,
   a.max_steps = d->max_steps;
   a.eps = static_cast<T>(d->eps);
   a.theta = static_cast<T>(d->theta);
+  a.theta64 = d->theta;
+  a.time_f64 = d->time_f64;
   a.g = d->g;
   a.mlp = d->mlp;
   a.tau = static_cast<T>(d->mlp_tau);
+  int dev = 0, rate_khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) !=
+          hipSuccess ||
+      rate_khz <= 0) {
+    (void)hipGetLastError();
+    rate_khz = 100000;  // 100 MHz
+  }
+  const double wait_s = d->team_wait_s > 0.0 ? d->team_wait_s : 60.0;
+  a.wait_ticks = static_cast<long long>(wait_s * 1000.0 * rate_khz);
   const int block = d->block > 0 ? d->block : 256;
   const int cpt = d->cells_per_thread > 0 ? d->cells_per_thread : 4;
+  const int max_team = teams ? d->max_team : 1;
   if (d->flux == PSA_FLUX_RUSANOV)
-    return launch_batch_tiles<T, PSA_FLUX_RUSANOV>(a, block, cpt, d->grid, s);
+    return launch_batch_tiles<T, PSA_FLUX_RUSANOV>(a, block, cpt, d->grid, max_team,
+                                                   d->ninst, s);
   return launch_batch_tiles<T, PSA_FLUX_BURGERS_UPWIND>(a, block, cpt, d->grid,
-                                                        s);
+                                                        max_team, d->ninst, s);
 }
 
 }  // namespace
@@ -282,6 +461,13 @@ extern "C" int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1,
       desc->offsets == nullptr || desc->ncells == nullptr ||
       desc->dx == nullptr || desc->tfinal == nullptr || desc->order == nullptr)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  if (desc->team != nullptr &&
+      (desc->slot_inst == nullptr || desc->slot_rank == nullptr ||
+       desc->team_state == nullptr || desc->nslots < desc->ninst ||
+       desc->max_team < 1))
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "teams need slot_inst, slot_rank, team_state, nslots and "
+                     "max_team");
   if (desc->ninst == 0) return PSA_OK;
   auto s = static_cast<hipStream_t>(stream);
   if (desc->dtype == PSA_FLOAT64)

@@ -262,18 +262,14 @@ __global__ void k_copy_if_inactive(const T *__restrict__ src,
 // ---------------------------------------------------------------------------
 // fused stage kernel: one workgroup per tile
 
-template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT,
-          bool LEARNED>
-__global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
-  constexpr int TILE = BLOCK * CPT;
-  __shared__ StageSmem<T, VEL, BLOCK, CPT> sm;
+// Step control at the start of a stage launch (timestepping.py:89-152 on
+// the device): stage 1 decides whether the step is taken and publishes dt,
+// stage 2 advances t and m and clears the max|u| slots, stage 3 reads dt.
+// Returns false when the step is not taken (the workgroup then exits).
+template <typename T, int STAGE>
+__device__ __forceinline__ bool stage_control(const StageArgs<T> &p, T &dt) {
   const int tid = threadIdx.x;
-  MlpWave W;
-  if constexpr (LEARNED) mlp_load(W, p.mlp);
-  const int64_t ntiles = (p.m + TILE - 1) / TILE;
-
-  // ---- step control
-  T dt = T(0);
+  dt = T(0);
   if constexpr (STAGE > 0) {
     if (p.ctrl != nullptr) {
       if constexpr (STAGE == 1) {
@@ -286,9 +282,9 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
           p.ctrl->umax_bits = umax;
           if (active) p.ctrl->dt = static_cast<double>(dt);
         }
-        if (!active) return;
+        if (!active) return false;
       } else {
-        if (p.ctrl->active == 0) return;
+        if (p.ctrl->active == 0) return false;
         dt = static_cast<T>(p.ctrl->dt);
         if constexpr (STAGE == 2) {
           if (blockIdx.x == 0 && tid == 0) {
@@ -307,6 +303,22 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
     }
   }
 
+  return true;
+}
+
+template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT,
+          bool LEARNED>
+__global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
+  constexpr int TILE = BLOCK * CPT;
+  __shared__ StageSmem<T, VEL, BLOCK, CPT> sm;
+  const int tid = threadIdx.x;
+  MlpWave W;
+  if constexpr (LEARNED) mlp_load(W, p.mlp);
+  const int64_t ntiles = (p.m + TILE - 1) / TILE;
+
+  T dt;
+  if (!stage_control<T, STAGE>(p, dt)) return;
+
   unsigned long long vmax = 0;
   const int64_t tile = blockIdx.x;  // the grid has one workgroup per tile
   if (tile < ntiles) {
@@ -317,6 +329,74 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   if constexpr (STAGE == 3) {
     if (p.ctrl != nullptr && p.ctrl->speed_from_state) {
       vmax = block_max_bits<BLOCK>(vmax, sm.red);
+      if (tid == 0)
+        atomicMax(&p.ctrl->umax_slot[blockIdx.x % PSA_UMAX_SLOTS][0], vmax);
+    }
+  }
+}
+
+// Pipelined stage kernel: a persistent grid (the workgroups the device holds
+// at once) walks the tiles with a stride of the grid size. While a
+// workgroup computes one tile from one LDS buffer, the next tile (and its
+// u^n) is copied straight from HBM into the other buffer, so every workgroup
+// keeps its next loads in flight through its compute phase. Tiles at the
+// domain ends take the ordinary path (ghost mapping, bounds checks). The
+// per-tile arithmetic is that of k_stage, so results are bit-identical.
+template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT>
+__global__ __launch_bounds__(BLOCK) void k_stage_pipe(StageArgs<T> p) {
+  static_assert(VEL != 2 && STAGE > 0, "pipelined stages 1-3 without a velocity array");
+  constexpr int TILE = BLOCK * CPT;
+  constexpr int HALO = 4;
+  constexpr int VN = Vec16<T>::n;
+  __shared__ StageSmem<T, VEL, BLOCK, CPT> smA;
+  __shared__ StageSmem<T, VEL, BLOCK, CPT> smB;
+  const int tid = threadIdx.x;
+  const MlpWave W{};
+  const int64_t ntiles = (p.m + TILE - 1) / TILE;
+
+  T dt;
+  if (!stage_control<T, STAGE>(p, dt)) return;
+
+  auto fast_tile = [&](int64_t t) {
+    const int64_t S = t * TILE;
+    return p.aligned != 0 && S - HALO >= p.g && S + TILE + HALO <= p.m - p.g;
+  };
+  auto issue = [&](int64_t t, StageSmem<T, VEL, BLOCK, CPT> &sm) {
+    if (t < ntiles && fast_tile(t)) {
+      const int64_t S = t * TILE;
+      dma_chunks<BLOCK, (TILE + 2 * HALO) / VN>(p.uk + S - HALO, sm.u);
+      if constexpr (STAGE >= 2) dma_chunks<BLOCK, TILE / VN>(p.un + S, sm.io);
+    }
+  };
+  unsigned long long vmax = 0;
+  // one tile: wait until every wave is done with nxt (its last reads were
+  // the previous tile's stores), start the copies of the following tile into
+  // it, compute this tile from cur, then wait for the copies
+  auto step = [&](int64_t t, StageSmem<T, VEL, BLOCK, CPT> &cur,
+                  StageSmem<T, VEL, BLOCK, CPT> &nxt) {
+    tile_sync<true>();
+    issue(t + gridDim.x, nxt);
+    stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT, false, true, true>(p, t * TILE, dt,
+                                                                 cur, vmax, W);
+    wait_vm_all();
+    tile_sync<true>();
+  };
+
+  int64_t t = blockIdx.x;
+  issue(t, smA);
+  wait_vm_all();
+  tile_sync<true>();
+  while (t < ntiles) {
+    step(t, smA, smB);
+    t += gridDim.x;
+    if (t >= ntiles) break;
+    step(t, smB, smA);
+    t += gridDim.x;
+  }
+
+  if constexpr (STAGE == 3) {
+    if (p.ctrl != nullptr && p.ctrl->speed_from_state) {
+      vmax = block_max_bits<BLOCK>(vmax, smA.red);
       if (tid == 0)
         atomicMax(&p.ctrl->umax_slot[blockIdx.x % PSA_UMAX_SLOTS][0], vmax);
     }
@@ -384,6 +464,7 @@ namespace psa {
 struct TileConfig {
   int block;
   int cpt;
+  bool pipe;  // stages 1-3: the persistent pipelined kernel
 };
 
 inline bool aligned16(const void *p) {
@@ -428,9 +509,32 @@ int launch_stage_cfg(const StageArgs<T> &args, hipStream_t stream) {
   return check_launch("ssprk33 stage");
 }
 
+template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT>
+int launch_stage_pipe_cfg(const StageArgs<T> &args, hipStream_t stream) {
+  constexpr int TILE = BLOCK * CPT;
+  StageArgs<T> a = args;
+  a.aligned = aligned16(a.uk) && aligned16(a.out) &&
+              (a.un == nullptr || aligned16(a.un));
+  const unsigned int cap = resident_blocks(k_stage_pipe<T, FLUX, VEL, STAGE, BLOCK, CPT>, BLOCK);
+  if (cap == 0) return launch_stage_cfg<T, FLUX, VEL, STAGE, BLOCK, CPT>(args, stream);
+  unsigned int blocks = blocks_for(a.m, TILE);
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL((k_stage_pipe<T, FLUX, VEL, STAGE, BLOCK, CPT>), dim3(blocks),
+                     dim3(BLOCK), 0, stream, a);
+  return check_launch("ssprk33 stage (pipelined)");
+}
+
 template <typename T, int FLUX, int VEL, int STAGE>
 int launch_stage_tiles(const StageArgs<T> &a, TileConfig cfg,
                        hipStream_t stream) {
+  if constexpr (STAGE > 0 && VEL != 2) {
+    if (cfg.pipe) {
+      if (cfg.block == 256 && cfg.cpt == 4)
+        return launch_stage_pipe_cfg<T, FLUX, VEL, STAGE, 256, 4>(a, stream);
+      if (cfg.block == 256 && cfg.cpt == 2)
+        return launch_stage_pipe_cfg<T, FLUX, VEL, STAGE, 256, 2>(a, stream);
+    }
+  }
   if (cfg.block == 256 && cfg.cpt == 1)
     return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 1>(a, stream);
   if (cfg.block == 256 && cfg.cpt == 2)
@@ -491,7 +595,7 @@ int launch_stage(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
 
 TileConfig default_tiles(int dtype) {
   (void)dtype;
-  return TileConfig{256, 4};
+  return TileConfig{256, 4, false};
 }
 
 int check_desc(const psa_solver_desc *d) {
@@ -526,6 +630,8 @@ TileConfig tiles_of(const psa_solver_desc *d) {
   if (d->mlp != nullptr) c.cpt = 2;
   if (d->block > 0) c.block = d->block;
   if (d->cells_per_thread > 0) c.cpt = d->cells_per_thread;
+  if (d->schedule == PSA_SCHEDULE_TILE) c.pipe = false;
+  if (d->schedule == PSA_SCHEDULE_PIPELINE) c.pipe = true;
   return c;
 }
 

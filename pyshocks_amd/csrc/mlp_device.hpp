@@ -9,11 +9,11 @@
 // stencil, as the reference's array reversal does):
 //
 //   mean = (((v0 + v4) + (v1 + v3)) + v2) * 0.2,  s = max_k |v_k - mean|
-//   x_k  = (v_k - mean) / s   (0 when s == 0)
+//   x_k  = (v_k - mean) * (1 / s)   (0 when s == 0)
 //   h1   = relu(W1 bf16(x) + b1),       rounded to BF16
 //   h2   = relu(W2 h1 + b2),            rounded to BF16
 //   z    = W3 h2 + (b3 + ln d),         d = (1/10, 6/10, 3/10)
-//   omega = softmax(z)
+//   omega = softmax(z),  value = h1 + omega0 (h0 - h1) + omega2 (h2 - h1)
 //
 // with W1, W2, W3 rounded to BF16 and every sum accumulated in FP32, so that
 // z = 0 gives the ideal linear weights d. The mean is summed symmetrically so
@@ -30,6 +30,15 @@
 // (q, j) standing for hidden unit pi(q, j) = 4q + j (j < 4) or
 // 16 + 4q + (j - 4) (j >= 4); the A operands of the next layer are built with
 // the same slot -> unit map, so no data moves between layers.
+//
+// A wave evaluates its 64 samples (one per lane) as four chains of 16: chain
+// c takes the samples of lanes 16c .. 16c + 15, which are columns 0..15.
+// W1 is replicated in all four k-slot groups of its A operand, and chain c's
+// B operand carries the inputs in k-slot group c only (lanes 16c .. 16c + 15
+// hold their own inputs, the other lanes zeros), so the product picks W1
+// once. W3 is replicated in the four row groups of its A operand, so every
+// lane group holds the logits of its column: the lanes of group c read
+// chain c's logits from their own registers. No lane exchanges are needed.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -86,7 +95,7 @@ __device__ __forceinline__ void mlp_load(MlpWave &W, const float *__restrict__ P
     const int row = 16 * b + i;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const float w1 = (q == 0 && j < MLP_IN) ? P[MLP_W1 + row * MLP_IN + j] : 0.0f;
+      const float w1 = j < MLP_IN ? P[MLP_W1 + row * MLP_IN + (j < MLP_IN ? j : 0)] : 0.0f;
       W.w1[b][j] = static_cast<__bf16>(w1);
       W.w2[b][j] = static_cast<__bf16>(P[MLP_W2 + row * MLP_HID + mlp_slot_unit(q, j)]);
     }
@@ -96,69 +105,87 @@ __device__ __forceinline__ void mlp_load(MlpWave &W, const float *__restrict__ P
       W.b2[b][v] = P[MLP_B2 + 16 * b + 4 * q + v];
     }
   }
+  // row 4g + r of layer 3 is output r (r < 3) for every row group g
+  const int r3 = i & 3;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    const float w3 = i < MLP_OUT ? P[MLP_W3 + i * MLP_HID + mlp_slot_unit(q, j)] : 0.0f;
+    const float w3 = r3 < MLP_OUT ? P[MLP_W3 + (r3 < MLP_OUT ? r3 : 0) * MLP_HID +
+                                      mlp_slot_unit(q, j)]
+                                  : 0.0f;
     W.w3[j] = static_cast<__bf16>(w3);
   }
   const float lnd[3] = {MLP_LN_D0, MLP_LN_D1, MLP_LN_D2};
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int r = 4 * q + v;
-    W.b3[v] = r < MLP_OUT ? P[MLP_B3 + r] + lnd[r < MLP_OUT ? r : 0] : 0.0f;
-  }
+  for (int v = 0; v < 4; ++v) W.b3[v] = v < MLP_OUT ? P[MLP_B3 + v] + lnd[v] : 0.0f;
+}
+
+typedef short mlp_s16x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 mlp_bf16x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int mlp_u32x4_t __attribute__((ext_vector_type(4)));
+
+typedef float mlp_f32x2_t __attribute__((ext_vector_type(2)));
+
+// Two FP32 values -> packed BF16 (round to nearest even, one conversion).
+__device__ __forceinline__ unsigned int mlp_pack(float a, float b) {
+  const mlp_f32x2_t f = {a, b};
+  return __builtin_bit_cast(unsigned int, __builtin_convertvector(f, mlp_bf16x2_t));
+}
+
+// relu of packed BF16 values: a negative BF16 (sign bit set, -0 included) is
+// a negative 16-bit integer, so an integer max with 0 clears it and keeps
+// every non-negative value.
+__device__ __forceinline__ unsigned int mlp_relu2(unsigned int x) {
+  const mlp_s16x2_t zero = {0, 0};
+  const mlp_s16x2_t r = __builtin_elementwise_max(__builtin_bit_cast(mlp_s16x2_t, x), zero);
+  return __builtin_bit_cast(unsigned int, r);
 }
 
 // B operand of the next layer from the two 16-row blocks of this one:
-// relu, then BF16.
+// BF16, then relu (the same values as relu, then BF16).
 __device__ __forceinline__ bf16x8_t mlp_relu_pack(f32x4_t h0, f32x4_t h1) {
-  bf16x8_t r;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    r[j] = static_cast<__bf16>(fmaxf(h0[j], 0.0f));
-    r[4 + j] = static_cast<__bf16>(fmaxf(h1[j], 0.0f));
-  }
-  return r;
+  const mlp_u32x4_t r = {mlp_relu2(mlp_pack(h0[0], h0[1])), mlp_relu2(mlp_pack(h0[2], h0[3])),
+                         mlp_relu2(mlp_pack(h1[0], h1[1])), mlp_relu2(mlp_pack(h1[2], h1[3]))};
+  return __builtin_bit_cast(bf16x8_t, r);
 }
 
 // Inputs of one stencil (see the header comment). T is the state type.
 template <typename T>
 __device__ __forceinline__ void mlp_features(T v0, T v1, T v2, T v3, T v4,
                                              float x[MLP_IN]) {
+  // every operation rounded as written (no multiply-add contraction), so the
+  // inputs are bit-identical to the same formula evaluated elsewhere
+#pragma clang fp contract(off)
   const T mean = (((v0 + v4) + (v1 + v3)) + v2) * T(0.2);
   const T d0 = v0 - mean, d1 = v1 - mean, d2 = v2 - mean, d3 = v3 - mean,
           d4 = v4 - mean;
   const T s = fmax(fmax(fmax(fabs(d0), fabs(d1)), fmax(fabs(d2), fabs(d3))),
                    fabs(d4));
   if (s > T(0)) {
-    x[0] = static_cast<float>(d0 / s);
-    x[1] = static_cast<float>(d1 / s);
-    x[2] = static_cast<float>(d2 / s);
-    x[3] = static_cast<float>(d3 / s);
-    x[4] = static_cast<float>(d4 / s);
+    const T inv = T(1) / s;
+    x[0] = static_cast<float>(d0 * inv);
+    x[1] = static_cast<float>(d1 * inv);
+    x[2] = static_cast<float>(d2 * inv);
+    x[3] = static_cast<float>(d3 * inv);
+    x[4] = static_cast<float>(d4 * inv);
   } else {
     x[0] = x[1] = x[2] = x[3] = x[4] = 0.0f;
   }
 }
 
 // Logits of one sample per lane (all 64 lanes of the wave must call it).
-// The 64 samples run as 4 chains of 16: chain c takes the samples of lanes
-// 16c .. 16c + 15, whose inputs move to lanes 0..15 of its B operand.
+// With CHAINS = 1 only the lanes 0..15 are evaluated (the others' results
+// are undefined).
+template <int CHAINS = 4>
 __device__ __forceinline__ void mlp_logits(const MlpWave &W, const float x[MLP_IN],
                                            float z[MLP_OUT]) {
-  const int lane = threadIdx.x & 63;
-  const int q = lane >> 4;
-  const int col = lane & 15;
+  const int q = (threadIdx.x & 63) >> 4;
+  const mlp_u32x4_t own = {mlp_pack(x[0], x[1]), mlp_pack(x[2], x[3]),
+                           mlp_pack(x[4], 0.0f), 0u};
+  const mlp_u32x4_t none = {0u, 0u, 0u, 0u};
   z[0] = z[1] = z[2] = 0.0f;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int src = 16 * c + col;
-    bf16x8_t B;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const float xs = j < MLP_IN ? __shfl(x[j < MLP_IN ? j : 0], src, 64) : 0.0f;
-      B[j] = static_cast<__bf16>(q == 0 ? xs : 0.0f);
-    }
+  for (int c = 0; c < CHAINS; ++c) {
+    bf16x8_t B = __builtin_bit_cast(bf16x8_t, q == c ? own : none);
     f32x4_t h0 = mlp_mfma(W.w1[0], B, W.b1[0]);
     f32x4_t h1 = mlp_mfma(W.w1[1], B, W.b1[1]);
     B = mlp_relu_pack(h0, h1);
@@ -166,14 +193,10 @@ __device__ __forceinline__ void mlp_logits(const MlpWave &W, const float x[MLP_I
     h1 = mlp_mfma(W.w2[1], B, W.b2[1]);
     B = mlp_relu_pack(h0, h1);
     const f32x4_t o = mlp_mfma(W.w3, B, W.b3);
-    // lanes 0..15 hold the logits of chain c's samples in rows 0..2
-    const float t0 = __shfl(o[0], col, 64);
-    const float t1 = __shfl(o[1], col, 64);
-    const float t2 = __shfl(o[2], col, 64);
     if (q == c) {
-      z[0] = t0;
-      z[1] = t1;
-      z[2] = t2;
+      z[0] = o[0];
+      z[1] = o[1];
+      z[2] = o[2];
     }
   }
 }
@@ -185,6 +208,14 @@ __device__ __forceinline__ void mlp_softmax(const float z[MLP_OUT], float w[MLP_
   w[0] = e0 * inv;
   w[1] = e1 * inv;
   w[2] = e2 * inv;
+}
+
+// sum_k w_k h_k written as h1 + w0 (h0 - h1) + w2 (h2 - h1): the FP32
+// weights sum to 1 only to rounding, and this form keeps that rounding
+// relative to the spread of the candidates instead of to their size.
+template <typename T>
+__device__ __forceinline__ T mlp_combine(const float w[MLP_OUT], T h0, T h1, T h2) {
+  return (h1 + T(w[0]) * (h0 - h1)) + T(w[2]) * (h2 - h1);
 }
 
 // Shock flag of a cell from its right-side smoothness indicators (the left
@@ -217,11 +248,31 @@ __device__ __forceinline__ void learned_pair(const MlpWave &W, T um2, T um1, T u
   T hr0, hr1, hr2, hl0, hl1, hl2;
   weno_uhat(um2, um1, u0, up1, up2, hr0, hr1, hr2);
   weno_uhat(up2, up1, u0, um1, um2, hl0, hl1, hl2);
-  const T lr = (T(wr[0]) * hr0 + T(wr[1]) * hr1) + T(wr[2]) * hr2;
-  const T ll = (T(wl[0]) * hl0 + T(wl[1]) * hl1) + T(wl[2]) * hl2;
+  const T lr = mlp_combine(wr, hr0, hr1, hr2);
+  const T ll = mlp_combine(wl, hl0, hl1, hl2);
   const bool shock = mlp_shock(b0, b1, b2, eps, tau);
   ur = shock ? urJS : lr;
   ul = shock ? ulJS : ll;
+}
+
+// Right reconstruction of the stencil of lanes 0..15 (wave-collective; the
+// other lanes' results are undefined): indicators, WENO-JS value and the
+// learned value of one 16-sample chain.
+template <typename T>
+__device__ __forceinline__ T learned_right16(const MlpWave &W, T um2, T um1, T u0,
+                                             T up1, T up2, T eps, T tau) {
+  // the WENO-JS value through the same function as the WENO-JS kernels
+  const T urJS = weno_right(um2, um1, u0, up1, up2, eps);
+  T b0, b1, b2;
+  weno_beta(um2, um1, u0, up1, up2, b0, b1, b2);
+  float x[MLP_IN], z[MLP_OUT], w[MLP_OUT];
+  mlp_features(um2, um1, u0, up1, up2, x);
+  mlp_logits<1>(W, x, z);
+  mlp_softmax(z, w);
+  T h0, h1, h2;
+  weno_uhat(um2, um1, u0, up1, up2, h0, h1, h2);
+  const T lr = mlp_combine(w, h0, h1, h2);
+  return mlp_shock(b0, b1, b2, eps, tau) ? urJS : lr;
 }
 
 // Both reconstructions of one cell per lane from its stencil alone

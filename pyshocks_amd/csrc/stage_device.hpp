@@ -173,6 +173,44 @@ struct Chunks {
 };
 
 
+// Workgroup barrier for LDS hand-offs. With ASYNC, only LDS operations are
+// waited for (the workgroup-scope fence of __syncthreads would also wait for
+// every outstanding global load, including direct-to-LDS copies of the next
+// tile that are meant to stay in flight).
+template <bool ASYNC>
+__device__ __forceinline__ void tile_sync() {
+  if constexpr (ASYNC) {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  } else {
+    __syncthreads();
+  }
+}
+
+// Waits until every global memory operation of this wave has completed.
+__device__ __forceinline__ void wait_vm_all() {
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), other counters untouched
+}
+
+// Direct-to-LDS copy of COUNT 16-byte chunks from src to dst (both 16-byte
+// aligned): a wave copies 1 KB per instruction, lane l the l-th chunk of its
+// round, rounds dealt to the waves in turn; the copies complete
+// asynchronously (wait_vm_all, then a barrier, before reading dst).
+template <int BLOCK, int COUNT>
+__device__ __forceinline__ void dma_chunks(const void *src, void *dst) {
+  constexpr int WAVES = BLOCK / 64;
+  constexpr int ROUNDS = (COUNT + 63) / 64;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const char *s = static_cast<const char *>(src);
+  char *d = static_cast<char *>(dst);
+#pragma unroll
+  for (int r = wave; r < ROUNDS; r += WAVES) {
+    const int c = r * 64 + lane;
+    if (COUNT % 64 == 0 || c < COUNT)
+      __builtin_amdgcn_global_load_lds(s + 16 * c, d + 1024 * r, 16, 0, 0);
+  }
+}
+
 // LDS of one stage tile.
 template <typename T, int VEL, int BLOCK, int CPT>
 struct StageSmem {
@@ -195,6 +233,8 @@ struct StageSmem {
 // VEL: 0 Burgers (no velocity), 1 constant velocity, 2 velocity array.
 // LEARNED: the reconstruction takes the learned weights of W (Burgers only,
 //        every wave of the workgroup must run the tile).
+// PRELOADED: a tile on the fast path is already in sm.u (and u^n in sm.io)
+//        and visible to the workgroup; ASYNC: barriers wait for LDS only.
 // STAGE: 0 operator (all m cells, faces 0 and m padded with zero flux),
 //        1..3 SSP-RK3 stages (interior cells, ghost images rewritten).
 //
@@ -207,7 +247,7 @@ struct StageSmem {
 // pattern of max |u| of the cells written. Ends with the LDS free for reuse
 // only after the caller's next barrier.
 template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT,
-          bool LEARNED = false>
+          bool LEARNED = false, bool PRELOADED = false, bool ASYNC = false>
 __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
                                            T dt,
                                            StageSmem<T, VEL, BLOCK, CPT> &sm,
@@ -228,7 +268,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
                     S + TILE + HALO <= m - g;
 
   // ---- stage the tile and its halo
-  if (fast) {
+  if (fast && PRELOADED) {
+    // staged by the caller
+  } else if (fast) {
     // all loads of the tile, the velocity and u^n first, then the LDS writes
     Chunks<T, BLOCK, (TILE + 2 * HALO) / VN> cu;
     Chunks<T, BLOCK, VEL == 2 ? (TILE + 2 * HALO) / VN : 1> cv;
@@ -255,7 +297,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
       }
     }
   }
-  __syncthreads();
+  tile_sync<ASYNC>();
 
   // ---- reconstruct CPT consecutive cells
   using C = Weno5Coeffs<T>;
@@ -310,19 +352,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     sm.alF[tid] = al[0];
   }
   if constexpr (LEARNED) {
-    // wave 0 (a wave-uniform branch): lanes 0..31 reconstruct local cell -1,
-    // lanes 32..63 local cell TILE from its mirrored stencil (as the WENO-JS
-    // path does, so flagged cells match it bit for bit); lane 0 keeps the
-    // right state of the first, lane 32 the left state of the second
+    // wave 0 (a wave-uniform branch) runs one 16-sample chain: lane 0
+    // reconstructs the right state of local cell -1, lane 1 the left state of
+    // local cell TILE from its mirrored stencil (as the WENO-JS path does, so
+    // flagged cells match it bit for bit); lanes 2..15 repeat lane 0
     if ((tid >> 6) == 0) {
-      const bool hi = (tid & 63) >= 32;
+      const bool hi = tid == 1;
       const int c = hi ? TILE + HALO : HALO - 1;
       const int d = hi ? -1 : 1;
-      T el, er;
-      learned_cell(W, sm.u[c - 2 * d], sm.u[c - d], sm.u[c], sm.u[c + d],
-                   sm.u[c + 2 * d], p.eps, p.tau, el, er);
-      if (tid == 0) sm.urL[0] = er;
-      if (tid == 32) sm.ulF[BLOCK] = er;
+      const T r = learned_right16(W, sm.u[c - 2 * d], sm.u[c - d], sm.u[c],
+                                  sm.u[c + d], sm.u[c + 2 * d], p.eps, p.tau);
+      if (tid == 0) sm.urL[0] = r;
+      if (tid == 1) sm.ulF[BLOCK] = r;
     }
   } else if (tid == 0) {
     // right state of local cell -1
@@ -341,7 +382,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
                                  sm.v[c - 1], sm.v[c - 2], p.eps);
     }
   }
-  __syncthreads();
+  tile_sync<ASYNC>();
 
   // ---- faces l0 - 1/2 .. l0 + CPT - 1/2
   T F[CPT + 1];
@@ -413,7 +454,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     }
     sm.io[l0 + r] = val;
   }
-  __syncthreads();
+  tile_sync<ASYNC>();
 
   // ---- coalesced stores (+ ghost images, + max|u| of the new state)
   if (fast) {

@@ -90,8 +90,8 @@ def features(v: torch.Tensor) -> torch.Tensor:
     mean = (((v[..., 0] + v[..., 4]) + (v[..., 1] + v[..., 3])) + v[..., 2]) * 0.2
     d = v - mean[..., None]
     s = d.abs().amax(dim=-1, keepdim=True)
-    x = torch.where(s > 0, d / torch.where(s > 0, s, torch.ones_like(s)),
-                    torch.zeros_like(d))
+    inv = 1.0 / torch.where(s > 0, s, torch.ones_like(s))
+    x = torch.where(s > 0, d * inv, torch.zeros_like(d))
     return x.to(torch.float32)
 
 

@@ -47,7 +47,9 @@ def test_batched_equals_single_instance(torch_mod, dtype_name: str, scheme: str)
     _, ics = random_instances(6, seed=3)
     ns = np.array([3, 37, 128, 1000, 3000, 4100])
     states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
-    solver = BatchedSolver(ns, dtype=dtype, scheme=scheme, tiles=(256, 4), grid=3)
+    # time in the state dtype, as the single-instance path and the reference
+    solver = BatchedSolver(ns, dtype=dtype, scheme=scheme, tiles=(256, 4), grid=3,
+                           time_f64=False)
     solver.set_states(states)
     res = solver.solve(0.05)
     assert np.all(res.status == 0)
@@ -138,3 +140,65 @@ def test_autotune_picks_a_launchable_config(torch_mod) -> None:
     states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
     bbest, btimes = autotune.tune_batch(ns, states, tfinal=0.005)
     assert bbest in btimes
+
+
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+def test_teams_match_single_workgroup_bit_for_bit(torch_mod, dtype_name: str) -> None:
+    """Splitting an instance's tiles among a team changes who computes a
+    tile, not what is computed."""
+    torch = torch_mod
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    dtype = getattr(torch, dtype_name)
+    _, ics = random_instances(5, seed=9)
+    ns = np.array([20000, 50, 3000, 9000, 700])
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    out = []
+    for team_cells in (None, 2048):
+        solver = BatchedSolver(ns, dtype=dtype, tiles=(256, 4), team_cells=team_cells)
+        if team_cells:
+            assert solver.team.max() > 1
+        solver.set_states(states)
+        res = solver.solve(0.02)
+        assert np.all(res.status == 0)
+        out.append((solver.u.clone(), res))
+    assert torch.equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1].steps, out[1][1].steps)
+    np.testing.assert_array_equal(out[0][1].t, out[1][1].t)
+
+
+def test_team_wait_limit_terminates(torch_mod) -> None:
+    """A wait limit of zero makes a member that has to wait at a barrier
+    abort its team: the launch still ends, team instances report
+    PSA_ERR_HIP or finish, single-workgroup instances are unaffected."""
+    torch = torch_mod
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    _, ics = random_instances(4, seed=10)
+    ns = np.array([30000, 100, 200, 12000])
+    solver = BatchedSolver(ns, dtype=torch.float32, tiles=(256, 4), team_cells=2048,
+                           team_wait_s=1e-12)
+    solver.set_states([exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)])
+    res = solver.solve(0.01)
+    teamed = solver.team > 1
+    assert np.all(np.isin(res.status[teamed], [0, -3]))
+    assert np.all(res.status[~teamed] == 0)
+
+
+def test_time_f64_reaches_tfinal(torch_mod) -> None:
+    """FP32 states with FP64 time end at tfinal (to the FP32 rounding of the
+    last step) instead of drifting with FP32 sums of t."""
+    torch = torch_mod
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    _, ics = random_instances(3, seed=11)
+    ns = np.array([4000, 16000, 1000])
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    r = {}
+    for tf64 in (True, False):
+        solver = BatchedSolver(ns, dtype=torch.float32, time_f64=tf64)
+        solver.set_states(states)
+        r[tf64] = solver.solve(0.3)
+        assert np.all(r[tf64].status == 0)
+    assert np.max(np.abs(r[True].t - 0.3)) < 1e-9
+    assert np.max(np.abs(r[False].t - 0.3)) > np.max(np.abs(r[True].t - 0.3))

@@ -143,7 +143,8 @@ def test_batched_learned_equals_single_instance(torch_mod, dtype_name):
     _, ics = random_instances(4, seed=8)
     ns = np.array([5, 300, 1500, 4100])
     states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
-    solver = BatchedSolver(ns, dtype=dtype, tiles=(256, 4), grid=2, mlp=w)
+    solver = BatchedSolver(ns, dtype=dtype, tiles=(256, 4), grid=2, mlp=w, time_f64=False,
+                           team_cells=None)
     solver.set_states(states)
     res = solver.solve(0.04)
     assert np.all(res.status == 0)

@@ -144,10 +144,18 @@ def test_learned_reconstruct_vs_oracle(torch_mod, dtype):
     s = np.ptp(v, axis=1) + 1e-300
     flags = M.shock_flags(M.stencils(u), 1e-6, 20.0)
     assert 4 <= flags.sum() < 40
+    scale = np.max(np.abs(u))
     for got, ref in [(ul, rl), (ur, rr)]:
-        d = np.abs(got.cpu().numpy().astype(np.float64) - ref) / s
-        # the weights are FP32 (softmax in FP32) in both state precisions
-        assert np.mean(d < 2e-5) > 0.99
+        diff = np.abs(got.cpu().numpy().astype(np.float64) - ref)
+        if dtype == "f64":
+            # the weights are FP32 (softmax in FP32): errors relative to the
+            # local spread of the stencil
+            d = diff / s
+            assert np.mean(d < 2e-5) > 0.99
+        else:
+            # FP32 candidates carry rounding relative to |u| itself
+            assert np.mean(diff < 2e-6 * scale) > 0.99
+            d = diff / (s + 1e-5 * scale)
         assert np.max(d) < 2e-2
         if dtype == "f64":
             np.testing.assert_allclose(got.cpu().numpy()[flags], ref[flags], rtol=1e-12, atol=1e-14)
