@@ -172,6 +172,7 @@ typedef struct psa_solver_desc {
   const float *mlp;             /* learned weights (PSA_MLP_NPARAM FP32) or
                                    NULL for WENO-JS                        */
   double mlp_tau;               /* shock threshold of the learned weights  */
+  double mlp_sigma;             /* smooth threshold of the learned weights */
   int schedule;                 /* stages 1-3: PSA_SCHEDULE_*              */
   int pad_;
 } psa_solver_desc;
@@ -249,6 +250,7 @@ typedef struct psa_batch_desc {
   const float *mlp;             /* learned weights or NULL (see
                                    psa_solver_desc)                        */
   double mlp_tau;               /* shock threshold of the learned weights  */
+  double mlp_sigma;             /* smooth threshold of the learned weights */
   int time_f64;                 /* 0: t and dt in the state dtype, as
                                    timestepping.py:136-152; 1: in FP64
                                    (FP32 states then end exactly at tfinal) */
@@ -326,9 +328,11 @@ int psa_ssprk33_adjoint(const psa_solver_desc *desc, long long nsteps,
  * logits + ln d) maps the normalised 5-cell stencil to the three weights;
  * BF16 matrix-core products with FP32 accumulation; parameters are
  * PSA_MLP_NPARAM FP32 values packed as W1[32][5], b1[32], W2[32][32],
- * b2[32], W3[3][32], b3[3]. Cells whose smoothness indicators satisfy
- * max beta > tau (min beta + eps) keep the WENO-JS weights. The solvers take
- * the weights through psa_solver_desc.mlp / psa_batch_desc.mlp. */
+ * b2[32], W3[3][32], b3[3]. Over the cell's smoothness indicators, cells
+ * with max beta > tau (min beta + eps) keep the WENO-JS weights, cells with
+ * max beta <= sigma min beta take the ideal weights d without evaluating the
+ * network, and the others the network's weights. The solvers take the
+ * weights through psa_solver_desc.mlp / psa_batch_desc.mlp. */
 #define PSA_MLP_NPARAM 1347
 #define PSA_MLP_RECORD 12 /* floats per training sample: x[5], e[3], wjs[3], c */
 
@@ -353,8 +357,8 @@ int psa_mlp_adam(float *params, const float *grad, float *m, float *v,
 /* Learned counterpart of psa_weno_reconstruct (same zero padding outside
  * [0, m)): ul, ur with the learned weights, WENO-JS in flagged cells. */
 int psa_learned_reconstruct(int dtype, const float *params, double tau,
-                            const void *u, void *ul, void *ur, int64_t m,
-                            double eps, void *stream);
+                            double sigma, const void *u, void *ul, void *ur,
+                            int64_t m, double eps, void *stream);
 
 #ifdef __cplusplus
 }

@@ -16,7 +16,8 @@ accumulators):
 * the value is ``h1 + omega0 (h0 - h1) + omega2 (h2 - h1)`` of the three
   candidates ``h`` (the weighted sum, insensitive to the FP32 rounding of
   ``sum omega``);
-* a cell is flagged (keeps WENO-JS) when ``max beta > tau (min beta + eps)``.
+* a cell with ``max beta > tau (min beta + eps)`` keeps WENO-JS, one with
+  ``max beta <= sigma min beta`` takes the ideal weights ``d``.
 
 Pins (``tests/test_oracle_mlp.py``): the unrounded gradient against central
 finite differences of the unrounded loss; softmax invariants (weights sum to
@@ -147,6 +148,32 @@ def stencils(u: np.ndarray) -> np.ndarray:
     return np.stack([up[k: k + m] for k in range(5)], axis=1)
 
 
+def regimes(v: np.ndarray, eps: float, tau: float, sigma: float) -> np.ndarray:
+    """0 (shock: WENO-JS) where ``max beta > tau (min beta + eps)``, 1
+    (smooth: ideal weights) where ``max beta <= sigma min beta``, 2 (learned)
+    otherwise, over the right-side indicators."""
+    b = _betas(v)
+    dt = v.dtype.type
+    bmax = np.max(b, axis=1)
+    bmin = np.min(b, axis=1)
+    return np.where(bmax > dt(tau) * (bmin + dt(eps)), 0,
+                    np.where(bmax <= dt(sigma) * bmin, 1, 2))
+
+
+def _betas(v: np.ndarray) -> np.ndarray:
+    dt = v.dtype.type
+    a0, a1 = dt(13.0 / 12.0), dt(1.0 / 4.0)
+    um2, um1, u0, up1, up2 = (v[:, k] for k in range(5))
+    t00 = (u0 - dt(2) * um1) + um2
+    t01 = (dt(3) * u0 - dt(4) * um1) + um2
+    t10 = (up1 - dt(2) * u0) + um1
+    t11 = up1 - um1
+    t20 = (up2 - dt(2) * up1) + u0
+    t21 = (up2 - dt(4) * up1) + dt(3) * u0
+    return np.stack([a0 * t00 * t00 + a1 * t01 * t01, a0 * t10 * t10 + a1 * t11 * t11,
+                     a0 * t20 * t20 + a1 * t21 * t21], axis=1)
+
+
 def shock_flags(v: np.ndarray, eps: float, tau: float) -> np.ndarray:
     """``max beta > tau (min beta + eps)`` from the right-side indicators."""
     dt = v.dtype.type
@@ -178,27 +205,32 @@ def _combine(w: np.ndarray, h: np.ndarray) -> np.ndarray:
 
 
 def learned_reconstruct(p: np.ndarray, tau: float, u: np.ndarray, eps: float, *,
-                        emulate: bool = True) -> tuple[np.ndarray, np.ndarray]:
-    """``(ul, ur)`` of every cell: learned weights, WENO-JS where flagged."""
+                        sigma: float = 0.0, emulate: bool = True
+                        ) -> tuple[np.ndarray, np.ndarray]:
+    """``(ul, ur)`` of every cell: WENO-JS in the shock regime, the ideal
+    weights in the smooth regime, the network's weights otherwise."""
     u = np.asarray(u)
     v = stencils(u)
     vm = v[:, ::-1]
     x = features(v)
     wr = forward(p, x, emulate=emulate)["omega"]
     wl = forward(p, x[:, ::-1], emulate=emulate)["omega"]
-    ur = _combine(wr, candidates(v.astype(np.float64)))
-    ul = _combine(wl, candidates(vm.astype(np.float64)))
-    flag = shock_flags(v, eps, tau)
+    cr, cl = candidates(v.astype(np.float64)), candidates(vm.astype(np.float64))
+    ideal = np.broadcast_to(IDEAL, wr.shape)
+    reg = regimes(v, eps, tau, sigma)
+    ur = np.where(reg == 1, _combine(ideal, cr), _combine(wr, cr))
+    ul = np.where(reg == 1, _combine(ideal, cl), _combine(wl, cl))
     jl, jr = weno_np.reconstruct_wenojs(u.astype(np.float64), eps)
-    return np.where(flag, jl, ul), np.where(flag, jr, ur)
+    return np.where(reg == 0, jl, ul), np.where(reg == 0, jr, ur)
 
 
 def learned_apply_operator(p: np.ndarray, tau: float, kind: str, grid: weno_np.Grid,
-                           u: np.ndarray, eps: float, *, emulate: bool = True) -> np.ndarray:
+                           u: np.ndarray, eps: float, *, sigma: float = 0.0,
+                           emulate: bool = True) -> np.ndarray:
     """The Burgers operator (``schemes.py:466-473``) with the learned
     reconstruction in place of WENO-JS: Rusanov (alpha = 1) or Godunov."""
     u = weno_np.apply_boundary_periodic(grid, np.asarray(u, dtype=np.float64))
-    ul, ur = learned_reconstruct(p, tau, u, eps, emulate=emulate)
+    ul, ur = learned_reconstruct(p, tau, u, eps, sigma=sigma, emulate=emulate)
     fl, fr = ul**2 / 2, ur**2 / 2
     if kind == "rusanov":
         a = np.abs(u)

@@ -78,6 +78,7 @@ class PsaSolverDesc(ctypes.Structure):
         ("cells_per_thread", ctypes.c_int),
         ("mlp", ctypes.c_void_p),
         ("mlp_tau", ctypes.c_double),
+        ("mlp_sigma", ctypes.c_double),
         ("schedule", ctypes.c_int),
         ("pad_", ctypes.c_int),
     ]
@@ -102,6 +103,7 @@ class PsaBatchDesc(ctypes.Structure):
         ("cells_per_thread", ctypes.c_int),
         ("mlp", ctypes.c_void_p),
         ("mlp_tau", ctypes.c_double),
+        ("mlp_sigma", ctypes.c_double),
         ("time_f64", ctypes.c_int),
         ("max_team", ctypes.c_int),
         ("team", ctypes.c_void_p),
@@ -160,7 +162,7 @@ SIGNATURES: dict[str, list[Any]] = {
     "psa_mlp_grad_workspace_bytes": [_LL, ctypes.POINTER(ctypes.c_size_t)],
     "psa_mlp_grad": [_VP, _VP, _LL, _D, _D, _VP, _VP, ctypes.c_size_t, _VP],
     "psa_mlp_adam": [_VP, _VP, _VP, _VP, _LL, _D, _D, _D, _D, _VP],
-    "psa_learned_reconstruct": [_I, _VP, _D, _VP, _VP, _VP, _I64, _D, _VP],
+    "psa_learned_reconstruct": [_I, _VP, _D, _D, _VP, _VP, _VP, _I64, _D, _VP],
 }
 
 TEAM_STATE_BYTES = 64

@@ -108,6 +108,7 @@ class BatchedSolver:
             # learned weights (pyshocks_amd.learned.MLPWeights)
             self.desc.mlp = mlp.ptr
             self.desc.mlp_tau = float(mlp.tau)
+            self.desc.mlp_sigma = float(mlp.sigma)
         if tiles is None:
             from pyshocks_amd.autotune import lookup_batch
 

@@ -72,6 +72,7 @@ struct BatchArgs {
   int time_f64;
   const float *mlp;
   T tau;
+  T sigma;
 };
 
 template <int BLOCK>
@@ -130,6 +131,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
   __shared__ unsigned long long s_bcast;
   __shared__ long long s_next;
   __shared__ int s_flag;
+  __shared__ unsigned int q[LEARNED ? (BLOCK / 64) * learned_queue_words<CPT>() : 1];
   const int tid = threadIdx.x;
   MlpWave W;
   if constexpr (LEARNED) mlp_load(W, a.mlp);
@@ -154,6 +156,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
     p.eps = a.eps;
     p.mlp = a.mlp;
     p.tau = a.tau;
+    p.sigma = a.sigma;
     p.aligned = 1;  // instance offsets are 16-byte aligned
     T *u = a.u + a.offsets[i];
     T *k1 = a.k1 + a.offsets[i];
@@ -254,7 +257,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
       for (int64_t tile = tlo; tile < thi; ++tile) {
         __syncthreads();
         stage_tile<T, FLUX, 0, 1, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
-                                                          vdummy, W);
+                                                          vdummy, W, q);
       }
       barrier();  // k1 complete
       // every member has read the max|u| reduced two steps ago: clear that
@@ -267,7 +270,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
       for (int64_t tile = tlo; tile < thi; ++tile) {
         __syncthreads();
         stage_tile<T, FLUX, 0, 2, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
-                                                          vdummy, W);
+                                                          vdummy, W, q);
       }
       barrier();  // k2 complete
       p.uk = k2;
@@ -276,7 +279,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
       for (int64_t tile = tlo; tile < thi; ++tile) {
         __syncthreads();
         stage_tile<T, FLUX, 0, 3, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
-                                                       vmax, W);
+                                                       vmax, W, q);
       }
       umax = team_max(vmax, static_cast<int>(steps & 1));  // u complete
       t = t + dt;
@@ -415,6 +418,7 @@ int launch_batch(const psa_batch_desc *d, void *u, void *k1, void *k2,
   a.g = d->g;
   a.mlp = d->mlp;
   a.tau = static_cast<T>(d->mlp_tau);
+  a.sigma = static_cast<T>(d->mlp_sigma);
   int dev = 0, rate_khz = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) !=
@@ -454,8 +458,8 @@ extern "C" int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1,
     return set_error(PSA_ERR_INVALID_ARGUMENT, "needs at least 3 ghost cells");
   if (desc->ninst < 0 || desc->max_steps < 0 || !(desc->eps >= 0.0))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "bad batch parameters");
-  if (desc->mlp != nullptr && !(desc->mlp_tau >= 0.0))
-    return set_error(PSA_ERR_INVALID_ARGUMENT, "mlp_tau must be >= 0");
+  if (desc->mlp != nullptr && (!(desc->mlp_tau >= 0.0) || !(desc->mlp_sigma >= 0.0)))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "mlp_tau and mlp_sigma must be >= 0");
   if (u == nullptr || k1 == nullptr || k2 == nullptr || counter == nullptr ||
       t_out == nullptr || steps_out == nullptr || status_out == nullptr ||
       desc->offsets == nullptr || desc->ncells == nullptr ||

@@ -311,6 +311,7 @@ template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT,
 __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   constexpr int TILE = BLOCK * CPT;
   __shared__ StageSmem<T, VEL, BLOCK, CPT> sm;
+  __shared__ unsigned int q[LEARNED ? (BLOCK / 64) * learned_queue_words<CPT>() : 1];
   const int tid = threadIdx.x;
   MlpWave W;
   if constexpr (LEARNED) mlp_load(W, p.mlp);
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   const int64_t tile = blockIdx.x;  // the grid has one workgroup per tile
   if (tile < ntiles) {
     stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT, LEARNED>(p, tile * TILE, dt,
-                                                         sm, vmax, W);
+                                                         sm, vmax, W, q);
   }
 
   if constexpr (STAGE == 3) {
@@ -378,7 +379,13 @@ __global__ __launch_bounds__(BLOCK) void k_stage_pipe(StageArgs<T> p) {
     issue(t + gridDim.x, nxt);
     stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT, false, true, true>(p, t * TILE, dt,
                                                                  cur, vmax, W);
-    wait_vm_all();
+    // the copies into nxt are older than this tile's stores: on the fast path
+    // (a known number of stores per wave) only the stores may stay in flight
+    if (fast_tile(t)) {
+      wait_vm_but<Chunks<T, BLOCK, TILE / VN>::CH>();
+    } else {
+      wait_vm_all();
+    }
     tile_sync<true>();
   };
 
@@ -619,8 +626,8 @@ int check_desc(const psa_solver_desc *d) {
   if (d->mlp != nullptr && d->flux == PSA_FLUX_ADVECTION_UPWIND)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "learned weights are implemented for Burgers only");
-  if (d->mlp != nullptr && !(d->mlp_tau >= 0.0))
-    return set_error(PSA_ERR_INVALID_ARGUMENT, "mlp_tau must be >= 0");
+  if (d->mlp != nullptr && (!(d->mlp_tau >= 0.0) || !(d->mlp_sigma >= 0.0)))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "mlp_tau and mlp_sigma must be >= 0");
   return PSA_OK;
 }
 
@@ -650,6 +657,7 @@ StageArgs<T> make_args(const psa_solver_desc *d) {
   a.aligned = 1;
   a.mlp = d->mlp;
   a.tau = static_cast<T>(d->mlp_tau);
+  a.sigma = static_cast<T>(d->mlp_sigma);
   return a;
 }
 

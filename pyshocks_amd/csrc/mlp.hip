@@ -298,7 +298,7 @@ __global__ void k_mlp_adam(float *__restrict__ p, const float *__restrict__ g,
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_learned_reconstruct(
-    const float *__restrict__ P, T tau, T eps, const T *__restrict__ u,
+    const float *__restrict__ P, T tau, T sigma, T eps, const T *__restrict__ u,
     T *__restrict__ ul, T *__restrict__ ur, int64_t m) {
   MlpWave W;
   mlp_load(W, P);
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void k_learned_reconstruct(
             u0 = load_raw(u, i, m), up1 = load_raw(u, i + 1, m),
             up2 = load_raw(u, i + 2, m);
     T l, r;
-    learned_cell(W, um2, um1, u0, up1, up2, eps, tau, l, r);
+    learned_cell(W, um2, um1, u0, up1, up2, eps, tau, sigma, l, r);
     if (i < m) {
       ul[i] = l;
       ur[i] = r;
@@ -394,23 +394,24 @@ extern "C" int psa_mlp_adam(float *params, const float *grad, float *m, float *v
 }
 
 extern "C" int psa_learned_reconstruct(int dtype, const float *params, double tau,
-                                       const void *u, void *ul, void *ur, int64_t m,
-                                       double eps, void *stream) {
+                                       double sigma, const void *u, void *ul, void *ur,
+                                       int64_t m, double eps, void *stream) {
   if (m < 0) return set_error(PSA_ERR_INVALID_ARGUMENT, "negative size");
   if (m == 0) return PSA_OK;
   if (params == nullptr || u == nullptr || ul == nullptr || ur == nullptr)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
-  if (!(tau >= 0.0) || !(eps >= 0.0))
-    return set_error(PSA_ERR_INVALID_ARGUMENT, "tau and eps must be >= 0");
+  if (!(tau >= 0.0) || !(sigma >= 0.0) || !(eps >= 0.0))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "tau, sigma and eps must be >= 0");
   auto s = static_cast<hipStream_t>(stream);
   const unsigned int nb = grid_for(m, 256, 4096);
   if (dtype == PSA_FLOAT64) {
     hipLaunchKernelGGL(k_learned_reconstruct<double>, dim3(nb), dim3(256), 0, s, params,
-                       tau, eps, static_cast<const double *>(u),
+                       tau, sigma, eps, static_cast<const double *>(u),
                        static_cast<double *>(ul), static_cast<double *>(ur), m);
   } else if (dtype == PSA_FLOAT32) {
     hipLaunchKernelGGL(k_learned_reconstruct<float>, dim3(nb), dim3(256), 0, s, params,
-                       static_cast<float>(tau), static_cast<float>(eps),
+                       static_cast<float>(tau), static_cast<float>(sigma),
+                       static_cast<float>(eps),
                        static_cast<const float *>(u), static_cast<float *>(ul),
                        static_cast<float *>(ur), m);
   } else {

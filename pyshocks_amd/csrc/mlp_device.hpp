@@ -15,6 +15,9 @@
 //   z    = W3 h2 + (b3 + ln d),         d = (1/10, 6/10, 3/10)
 //   omega = softmax(z),  value = h1 + omega0 (h0 - h1) + omega2 (h2 - h1)
 //
+// Cells in the shock regime keep WENO-JS, cells in the smooth regime take
+// the ideal weights d without evaluating the network (mlp_regime).
+//
 // with W1, W2, W3 rounded to BF16 and every sum accumulated in FP32, so that
 // z = 0 gives the ideal linear weights d. The mean is summed symmetrically so
 // that the inputs of the mirrored stencil are exactly the mirrored inputs: a
@@ -201,6 +204,39 @@ __device__ __forceinline__ void mlp_logits(const MlpWave &W, const float x[MLP_I
   }
 }
 
+// mlp_logits for inputs already packed to BF16 (own = {x0 x1, x2 x3, x4 0, 0})
+// and only the first nch chains evaluated (nch wave-uniform).
+__device__ __forceinline__ void mlp_logits_packed(const MlpWave &W, mlp_u32x4_t own,
+                                                  int nch, float z[MLP_OUT]) {
+  const int q = (threadIdx.x & 63) >> 4;
+  const mlp_u32x4_t none = {0u, 0u, 0u, 0u};
+  z[0] = z[1] = z[2] = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (c < nch) {
+      bf16x8_t B = __builtin_bit_cast(bf16x8_t, q == c ? own : none);
+      f32x4_t h0 = mlp_mfma(W.w1[0], B, W.b1[0]);
+      f32x4_t h1 = mlp_mfma(W.w1[1], B, W.b1[1]);
+      B = mlp_relu_pack(h0, h1);
+      h0 = mlp_mfma(W.w2[0], B, W.b2[0]);
+      h1 = mlp_mfma(W.w2[1], B, W.b2[1]);
+      B = mlp_relu_pack(h0, h1);
+      const f32x4_t o = mlp_mfma(W.w3, B, W.b3);
+      if (q == c) {
+        z[0] = o[0];
+        z[1] = o[1];
+        z[2] = o[2];
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ mlp_u32x4_t mlp_pack_inputs(const float x[MLP_IN]) {
+  const mlp_u32x4_t own = {mlp_pack(x[0], x[1]), mlp_pack(x[2], x[3]), mlp_pack(x[4], 0.0f),
+                           0u};
+  return own;
+}
+
 __device__ __forceinline__ void mlp_softmax(const float z[MLP_OUT], float w[MLP_OUT]) {
   const float zm = fmaxf(fmaxf(z[0], z[1]), z[2]);
   const float e0 = __expf(z[0] - zm), e1 = __expf(z[1] - zm), e2 = __expf(z[2] - zm);
@@ -218,6 +254,25 @@ __device__ __forceinline__ T mlp_combine(const float w[MLP_OUT], T h0, T h1, T h
   return (h1 + T(w[0]) * (h0 - h1)) + T(w[2]) * (h2 - h1);
 }
 
+// The ideal weights d = (1/10, 6/10, 3/10) in the same form.
+template <typename T>
+__device__ __forceinline__ T mlp_combine_ideal(T h0, T h1, T h2) {
+  return (h1 + T(0.1) * (h0 - h1)) + T(0.3) * (h2 - h1);
+}
+
+// Regime of a cell from its right-side smoothness indicators (the left
+// side's are the same three):
+//   0 (shock, WENO-JS)     max beta >  tau (min beta + eps)
+//   1 (smooth, ideal d)    max beta <= sigma min beta   (scale-free)
+//   2 (learned)            otherwise.
+template <typename T>
+__device__ __forceinline__ int mlp_regime(T b0, T b1, T b2, T eps, T tau, T sigma) {
+  const T bmax = fmax(fmax(b0, b1), b2);
+  const T bmin = fmin(fmin(b0, b1), b2);
+  if (bmax > tau * (bmin + eps)) return 0;
+  return bmax <= sigma * bmin ? 1 : 2;
+}
+
 // Shock flag of a cell from its right-side smoothness indicators (the left
 // side's are the same three): max beta > tau (min beta + eps).
 template <typename T>
@@ -228,11 +283,12 @@ __device__ __forceinline__ bool mlp_shock(T b0, T b1, T b2, T eps, T tau) {
 }
 
 // Both reconstructions of one cell per lane (wave-collective). ulJS, urJS
-// are the WENO-JS values of the cell, used where the cell is flagged.
+// are the WENO-JS values of the cell, used in the shock regime; the smooth
+// regime takes the ideal weights, the learned one the network's.
 template <typename T>
 __device__ __forceinline__ void learned_pair(const MlpWave &W, T um2, T um1, T u0,
                                              T up1, T up2, T b0, T b1, T b2,
-                                             T eps, T tau, T ulJS, T urJS,
+                                             T eps, T tau, T sigma, T ulJS, T urJS,
                                              T &ul, T &ur) {
   float x[MLP_IN], xm[MLP_IN], zr[MLP_OUT], zl[MLP_OUT], wr[MLP_OUT], wl[MLP_OUT];
   mlp_features(um2, um1, u0, up1, up2, x);
@@ -248,43 +304,50 @@ __device__ __forceinline__ void learned_pair(const MlpWave &W, T um2, T um1, T u
   T hr0, hr1, hr2, hl0, hl1, hl2;
   weno_uhat(um2, um1, u0, up1, up2, hr0, hr1, hr2);
   weno_uhat(up2, up1, u0, um1, um2, hl0, hl1, hl2);
-  const T lr = mlp_combine(wr, hr0, hr1, hr2);
-  const T ll = mlp_combine(wl, hl0, hl1, hl2);
-  const bool shock = mlp_shock(b0, b1, b2, eps, tau);
-  ur = shock ? urJS : lr;
-  ul = shock ? ulJS : ll;
+  const int regime = mlp_regime(b0, b1, b2, eps, tau, sigma);
+  if (regime == 0) {
+    ur = urJS;
+    ul = ulJS;
+  } else if (regime == 1) {
+    ur = mlp_combine_ideal(hr0, hr1, hr2);
+    ul = mlp_combine_ideal(hl0, hl1, hl2);
+  } else {
+    ur = mlp_combine(wr, hr0, hr1, hr2);
+    ul = mlp_combine(wl, hl0, hl1, hl2);
+  }
 }
 
-// Right reconstruction of the stencil of lanes 0..15 (wave-collective; the
-// other lanes' results are undefined): indicators, WENO-JS value and the
-// learned value of one 16-sample chain.
+// Learned right reconstruction of the stencil of lanes 0..15 (wave-
+// collective; the other lanes' results are undefined) with one 16-sample
+// chain; js reports the shock regime (the WENO-JS value is then to be used
+// instead).
 template <typename T>
 __device__ __forceinline__ T learned_right16(const MlpWave &W, T um2, T um1, T u0,
-                                             T up1, T up2, T eps, T tau) {
-  // the WENO-JS value through the same function as the WENO-JS kernels
-  const T urJS = weno_right(um2, um1, u0, up1, up2, eps);
+                                             T up1, T up2, T eps, T tau, T sigma,
+                                             bool &js) {
   T b0, b1, b2;
   weno_beta(um2, um1, u0, up1, up2, b0, b1, b2);
+  const int regime = mlp_regime(b0, b1, b2, eps, tau, sigma);
+  js = regime == 0;
   float x[MLP_IN], z[MLP_OUT], w[MLP_OUT];
   mlp_features(um2, um1, u0, up1, up2, x);
   mlp_logits<1>(W, x, z);
   mlp_softmax(z, w);
   T h0, h1, h2;
   weno_uhat(um2, um1, u0, up1, up2, h0, h1, h2);
-  const T lr = mlp_combine(w, h0, h1, h2);
-  return mlp_shock(b0, b1, b2, eps, tau) ? urJS : lr;
+  return regime == 1 ? mlp_combine_ideal(h0, h1, h2) : mlp_combine(w, h0, h1, h2);
 }
 
 // Both reconstructions of one cell per lane from its stencil alone
 // (wave-collective): indicators, WENO-JS values and the learned values.
 template <typename T>
 __device__ __forceinline__ void learned_cell(const MlpWave &W, T um2, T um1, T u0,
-                                             T up1, T up2, T eps, T tau, T &ul,
-                                             T &ur) {
+                                             T up1, T up2, T eps, T tau, T sigma,
+                                             T &ul, T &ur) {
   T b0, b1, b2, ulJS, urJS;
   weno_beta(um2, um1, u0, up1, up2, b0, b1, b2);
   weno_pair_from_beta(um2, um1, u0, up1, up2, b0, b1, b2, eps, ulJS, urJS);
-  learned_pair(W, um2, um1, u0, up1, up2, b0, b1, b2, eps, tau, ulJS, urJS, ul,
+  learned_pair(W, um2, um1, u0, up1, up2, b0, b1, b2, eps, tau, sigma, ulJS, urJS, ul,
                ur);
 }
 

@@ -97,6 +97,7 @@ struct StageArgs {
   T vconst;
   const float *mlp;  // learned weights (LEARNED kernels)
   T tau;             // their shock threshold
+  T sigma;           // their smooth threshold
 };
 
 template <typename T, int VEL>
@@ -191,6 +192,15 @@ __device__ __forceinline__ void wait_vm_all() {
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), other counters untouched
 }
 
+// Waits until at most N (< 16) global memory operations of this wave are
+// outstanding: with the N most recent ones being stores, every older load
+// (or direct-to-LDS copy) has completed.
+template <int N>
+__device__ __forceinline__ void wait_vm_but() {
+  static_assert(N >= 0 && N < 16, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt(0x0F70 | N);
+}
+
 // Direct-to-LDS copy of COUNT 16-byte chunks from src to dst (both 16-byte
 // aligned): a wave copies 1 KB per instruction, lane l the l-th chunk of its
 // round, rounds dealt to the waves in turn; the copies complete
@@ -211,6 +221,104 @@ __device__ __forceinline__ void dma_chunks(const void *src, void *dst) {
   }
 }
 
+// Entries of the per-wave queue of learned samples (3 words each).
+template <int CPT>
+constexpr int learned_queue_words() {
+  return 64 * 2 * CPT * 3;
+}
+
+// Learned reconstructions of a thread's CPT cells (wave-collective). Cells in
+// the learned regime put the inputs of their two samples (right stencil,
+// mirrored stencil) into the wave's LDS queue q; the wave evaluates the
+// queue 64 samples at a time (and only the chains that hold samples), the
+// weights come back through the same slots. Shock cells keep the WENO-JS
+// values already in ul, ur; smooth cells take the ideal weights.
+template <typename T, int CPT>
+__device__ __forceinline__ void learned_cells(const MlpWave &W, const T *w,
+                                              const int *regime, T *ul, T *ur,
+                                              unsigned int *q) {
+  const int lane = threadIdx.x & 63;
+  unsigned int *qw = q + (threadIdx.x >> 6) * learned_queue_words<CPT>();
+  int nmine = 0;
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) nmine += regime[r] == 2 ? 2 : 0;
+  int incl = nmine;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  const int ntot = __shfl(incl, 63, 64);
+  const int first = incl - nmine;
+  if (ntot > 0) {
+    int slot = first;
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+      if (regime[r] == 2) {
+        float x[MLP_IN], xm[MLP_IN];
+        mlp_features(w[r], w[r + 1], w[r + 2], w[r + 3], w[r + 4], x);
+        xm[0] = x[4];
+        xm[1] = x[3];
+        xm[2] = x[2];
+        xm[3] = x[1];
+        xm[4] = x[0];
+        const mlp_u32x4_t a = mlp_pack_inputs(x), b = mlp_pack_inputs(xm);
+        qw[3 * slot + 0] = a[0];
+        qw[3 * slot + 1] = a[1];
+        qw[3 * slot + 2] = a[2];
+        qw[3 * slot + 3] = b[0];
+        qw[3 * slot + 4] = b[1];
+        qw[3 * slot + 5] = b[2];
+        slot += 2;
+      }
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    for (int r0 = 0; r0 < ntot; r0 += 64) {  // wave-uniform
+      const int e = r0 + lane;
+      mlp_u32x4_t own = {0u, 0u, 0u, 0u};
+      if (e < ntot) {
+        own[0] = qw[3 * e + 0];
+        own[1] = qw[3 * e + 1];
+        own[2] = qw[3 * e + 2];
+      }
+      const int left = ntot - r0;
+      const int nch = left >= 64 ? 4 : (left + 15) >> 4;
+      float z[MLP_OUT], wt[MLP_OUT];
+      mlp_logits_packed(W, own, nch, z);
+      mlp_softmax(z, wt);
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      if (e < ntot) {
+        qw[3 * e + 0] = __float_as_uint(wt[0]);
+        qw[3 * e + 1] = __float_as_uint(wt[1]);
+        qw[3 * e + 2] = __float_as_uint(wt[2]);
+      }
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  }
+  int slot = first;
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) {
+    if (regime[r] == 0) continue;
+    T hr0, hr1, hr2, hl0, hl1, hl2;
+    weno_uhat(w[r], w[r + 1], w[r + 2], w[r + 3], w[r + 4], hr0, hr1, hr2);
+    weno_uhat(w[r + 4], w[r + 3], w[r + 2], w[r + 1], w[r], hl0, hl1, hl2);
+    if (regime[r] == 1) {
+      ur[r] = mlp_combine_ideal(hr0, hr1, hr2);
+      ul[r] = mlp_combine_ideal(hl0, hl1, hl2);
+    } else {
+      const float wr[MLP_OUT] = {__uint_as_float(qw[3 * slot + 0]),
+                                 __uint_as_float(qw[3 * slot + 1]),
+                                 __uint_as_float(qw[3 * slot + 2])};
+      const float wl[MLP_OUT] = {__uint_as_float(qw[3 * slot + 3]),
+                                 __uint_as_float(qw[3 * slot + 4]),
+                                 __uint_as_float(qw[3 * slot + 5])};
+      ur[r] = mlp_combine(wr, hr0, hr1, hr2);
+      ul[r] = mlp_combine(wl, hl0, hl1, hl2);
+      slot += 2;
+    }
+  }
+}
+
 // LDS of one stage tile.
 template <typename T, int VEL, int BLOCK, int CPT>
 struct StageSmem {
@@ -223,6 +331,8 @@ struct StageSmem {
   __align__(16) T io[TILE];
   T urL[BLOCK + 1];
   T ulF[BLOCK + 1];
+  T edge_val[2];   // learned right state of cell -1, left state of cell TILE
+  int edge_js[2];  // 1 where those cells keep WENO-JS
   T arL[NVX];
   T alF[NVX];
   unsigned long long red[BLOCK / 64];
@@ -252,7 +362,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
                                            T dt,
                                            StageSmem<T, VEL, BLOCK, CPT> &sm,
                                            unsigned long long &vmax,
-                                           const MlpWave &W) {
+                                           const MlpWave &W,
+                                           unsigned int *q = nullptr) {
   static_assert(!LEARNED || VEL == 0, "learned weights are for Burgers");
   constexpr int TILE = BLOCK * CPT;
   constexpr int HALO = 4;
@@ -316,6 +427,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   }
 
   T ul[CPT], ur[CPT];
+  int regime[CPT];
 #pragma unroll
   for (int r = 0; r < CPT; ++r) {
     const T um2 = w[r], um1 = w[r + 1], u0 = w[r + 2], up1 = w[r + 3],
@@ -328,11 +440,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     const T b2 = E[r + 2] + C::a1 * (t21 * t21);
     weno_pair_from_beta(um2, um1, u0, up1, up2, b0, b1, b2, p.eps, ul[r],
                         ur[r]);
-    if constexpr (LEARNED) {
-      learned_pair(W, um2, um1, u0, up1, up2, b0, b1, b2, p.eps, p.tau, ul[r],
-                   ur[r], ul[r], ur[r]);
-    }
+    if constexpr (LEARNED) regime[r] = mlp_regime(b0, b1, b2, p.eps, p.tau, p.sigma);
   }
+  if constexpr (LEARNED) learned_cells<T, CPT>(W, w, regime, ul, ur, q);
 
   T al[VEL == 2 ? CPT : 1], ar[VEL == 2 ? CPT : 1];
   if constexpr (VEL == 2) {
@@ -352,27 +462,33 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     sm.alF[tid] = al[0];
   }
   if constexpr (LEARNED) {
-    // wave 0 (a wave-uniform branch) runs one 16-sample chain: lane 0
-    // reconstructs the right state of local cell -1, lane 1 the left state of
-    // local cell TILE from its mirrored stencil (as the WENO-JS path does, so
-    // flagged cells match it bit for bit); lanes 2..15 repeat lane 0
+    // wave 0 (a wave-uniform branch) runs one 16-sample chain: lane 0 the
+    // right state of local cell -1, lane 1 the left state of local cell TILE
+    // from its mirrored stencil (lanes 2..15 repeat lane 0); the WENO-JS
+    // states of those cells come from the same code as in the WENO-JS
+    // kernels below, and the face loop picks one or the other
     if ((tid >> 6) == 0) {
       const bool hi = tid == 1;
       const int c = hi ? TILE + HALO : HALO - 1;
       const int d = hi ? -1 : 1;
+      bool js;
       const T r = learned_right16(W, sm.u[c - 2 * d], sm.u[c - d], sm.u[c],
-                                  sm.u[c + d], sm.u[c + 2 * d], p.eps, p.tau);
-      if (tid == 0) sm.urL[0] = r;
-      if (tid == 1) sm.ulF[BLOCK] = r;
+                                  sm.u[c + d], sm.u[c + 2 * d], p.eps, p.tau,
+                                  p.sigma, js);
+      if (tid < 2) {
+        sm.edge_val[tid] = r;
+        sm.edge_js[tid] = js ? 1 : 0;
+      }
     }
-  } else if (tid == 0) {
+  }
+  if (tid == 0) {
     // right state of local cell -1
     sm.urL[0] = weno_right(sm.u[1], sm.u[2], sm.u[3], sm.u[4], sm.u[5], p.eps);
     if constexpr (VEL == 2) {
       sm.arL[0] = weno_right(sm.v[1], sm.v[2], sm.v[3], sm.v[4], sm.v[5], p.eps);
     }
   }
-  if (!LEARNED && tid == BLOCK - 1) {
+  if (tid == BLOCK - 1) {
     // left state of local cell TILE (mirrored stencil)
     constexpr int c = TILE + HALO;
     sm.ulF[BLOCK] = weno_right(sm.u[c + 2], sm.u[c + 1], sm.u[c], sm.u[c - 1],
@@ -389,8 +505,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   const bool has_nu = FLUX == PSA_FLUX_RUSANOV && p.nu != nullptr;
 #pragma unroll
   for (int k = 0; k <= CPT; ++k) {
-    const T urL = k == 0 ? sm.urL[tid] : ur[k - 1];
-    const T ulR = k == CPT ? sm.ulF[tid + 1] : ul[k];
+    T urL = k == 0 ? sm.urL[tid] : ur[k - 1];
+    T ulR = k == CPT ? sm.ulF[tid + 1] : ul[k];
+    if constexpr (LEARNED) {
+      if (k == 0 && tid == 0 && !sm.edge_js[0]) urL = sm.edge_val[0];
+      if (k == CPT && tid == BLOCK - 1 && !sm.edge_js[1]) ulR = sm.edge_val[1];
+    }
     T Fk;
     if constexpr (FLUX == PSA_FLUX_RUSANOV) {
       if (has_nu) {

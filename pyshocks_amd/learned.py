@@ -52,20 +52,24 @@ class MLPWeights:
 
     params: torch.Tensor
     tau: float = 20.0
+    """Shock regime: ``max beta > tau (min beta + eps)`` keeps WENO-JS."""
+    sigma: float = 0.0
+    """Smooth regime: ``max beta <= sigma min beta`` takes the ideal weights
+    without evaluating the network (0: only constant stencils)."""
 
     def __post_init__(self) -> None:
         if self.params.dtype != torch.float32 or self.params.numel() != NPARAM:
             raise ValueError(f"expected {NPARAM} FP32 parameters")
         if not self.params.is_contiguous():
             raise ValueError("parameters must be contiguous")
-        if not self.tau >= 0.0:
-            raise ValueError("tau must be >= 0")
+        if not self.tau >= 0.0 or not self.sigma >= 0.0:
+            raise ValueError("tau and sigma must be >= 0")
 
     @classmethod
-    def from_numpy(cls, p: np.ndarray, tau: float = 20.0,
+    def from_numpy(cls, p: np.ndarray, tau: float = 20.0, sigma: float = 0.0,
                    device: torch.device | str = "cuda") -> "MLPWeights":
         t = torch.as_tensor(np.asarray(p, dtype=np.float32)).to(device).contiguous()
-        return cls(params=t, tau=float(tau))
+        return cls(params=t, tau=float(tau), sigma=float(sigma))
 
     @classmethod
     def load(cls, path: str | os.PathLike[str] | None = None,
@@ -73,11 +77,12 @@ class MLPWeights:
         """Weights saved by :meth:`save` (default: the trained network shipped
         with the package)."""
         with np.load(TRAINED_PATH if path is None else path, allow_pickle=False) as f:
-            return cls.from_numpy(f["params"], float(f["tau"]), device)
+            sigma = float(f["sigma"]) if "sigma" in f.files else 0.0
+            return cls.from_numpy(f["params"], float(f["tau"]), sigma, device)
 
     def save(self, path: str | os.PathLike[str], **meta: Any) -> None:
         np.savez(path, params=self.params.detach().cpu().numpy(),
-                 tau=np.float64(self.tau), **meta)
+                 tau=np.float64(self.tau), sigma=np.float64(self.sigma), **meta)
 
     @property
     def ptr(self) -> int:
@@ -119,8 +124,8 @@ def learned_reconstruct(w: MLPWeights, u: torch.Tensor, eps: float
     _lib.require_device(u, w.params)
     ul, ur = torch.empty_like(u), torch.empty_like(u)
     _lib.call("psa_learned_reconstruct", _lib.dtype_code(u.dtype), w.ptr, float(w.tau),
-              u.data_ptr(), ul.data_ptr(), ur.data_ptr(), u.numel(), float(eps),
-              _lib.stream_of(u))
+              float(w.sigma), u.data_ptr(), ul.data_ptr(), ur.data_ptr(), u.numel(),
+              float(eps), _lib.stream_of(u))
     return ul, ur
 
 
@@ -252,12 +257,13 @@ def _candidates(v: torch.Tensor) -> torch.Tensor:
 
 
 def stencil_records(u: torch.Tensor, uface: torch.Tensor, *, eps: float, tau: float,
-                    smin: float = 1.0e-9) -> torch.Tensor:
+                    sigma: float = 0.0, smin: float = 1.0e-9) -> torch.Tensor:
     """Training records of every interior cell of the periodic states ``u``
     (B, n + 6) against the exact face values ``uface`` (B, n + 1): the right
     reconstruction of cell i with face i + 1/2 and the left one (mirrored
-    stencil) with face i - 1/2. Cells flagged as shocks (they keep WENO-JS)
-    and near-constant stencils get weight 0."""
+    stencil) with face i - 1/2. Cells in the shock regime (they keep
+    WENO-JS) or the smooth regime (ideal weights, ``sigma``) and
+    near-constant stencils get weight 0."""
     n = u.shape[1] - 6
     idx = torch.arange(3, n + 3, device=u.device)
     v = torch.stack([u[:, idx + k] for k in (-2, -1, 0, 1, 2)], dim=-1)  # (B, n, 5)
@@ -270,8 +276,9 @@ def stencil_records(u: torch.Tensor, uface: torch.Tensor, *, eps: float, tau: fl
         x = features(vs)
         wjs, beta = _js_weights(vs, eps)
         e = (_candidates(vs) - ustar[..., None]) / torch.where(s > 0, s, torch.ones_like(s))[..., None]
-        shock = beta.amax(dim=-1) > tau * (beta.amin(dim=-1) + eps)
-        c = ((s > smin) & ~shock).to(torch.float64)
+        bmax, bmin = beta.amax(dim=-1), beta.amin(dim=-1)
+        learned = (bmax <= tau * (bmin + eps)) & (bmax > sigma * bmin)
+        c = ((s > smin) & learned).to(torch.float64)
         rec = torch.cat([x.double(), e, wjs, c[..., None]], dim=-1)
         recs.append(rec.reshape(-1, RECORD))
     return torch.cat(recs).to(torch.float32)
@@ -279,7 +286,8 @@ def stencil_records(u: torch.Tensor, uface: torch.Tensor, *, eps: float, tau: fl
 
 def make_training_samples(ics: list, *, n: int = 4096, times: tuple[float, ...] = (
         0.01, 0.02, 0.04, 0.06, 0.08, 0.1, 0.13, 0.16, 0.2, 0.25, 0.3, 0.35, 0.4, 0.45, 0.5),
-        eps: float = 1.0e-6, tau: float = 20.0, device: str = "cuda") -> torch.Tensor:
+        eps: float = 1.0e-6, tau: float = 20.0, sigma: float = 0.0,
+        device: str = "cuda") -> torch.Tensor:
     """Records from plain WENO-JS5 / Rusanov / SSP-RK3 runs (FP64, the
     batched solver) of ``ics`` on ``n`` cells, snapshotted at ``times``,
     against the exact Hopf-Lax face values at the snapshot times."""
@@ -297,7 +305,7 @@ def make_training_samples(ics: list, *, n: int = 4096, times: tuple[float, ...] 
             raise RuntimeError("a training run did not finish")
         u = torch.stack([solver.state(i) for i in range(B)])
         uf = exact.burgers_face_values(ics, n, res.t, device=device)
-        out.append(stencil_records(u, uf, eps=eps, tau=tau))
+        out.append(stencil_records(u, uf, eps=eps, tau=tau, sigma=sigma))
     return torch.cat(out)
 
 
@@ -347,6 +355,8 @@ def main(argv: list[str] | None = None) -> int:
 
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--out", default=str(TRAINED_PATH))
+    ap.add_argument("--load", default=None,
+                    help="validate these weights instead of training new ones")
     ap.add_argument("--ics", type=int, default=84)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--epochs", type=int, default=30)
@@ -355,30 +365,39 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--lambda-js", type=float, default=3e-3)
     ap.add_argument("--delta", type=float, default=1e-12)
     ap.add_argument("--tau", type=float, default=20.0)
+    ap.add_argument("--sigma", type=float, default=1.5,
+                    help="smooth-regime threshold stored with the weights (training "
+                         "always uses every record outside the shock regime)")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--tau-sweep", type=float, nargs="*", default=[])
+    ap.add_argument("--sigma-sweep", type=float, nargs="*", default=[])
     args = ap.parse_args(argv)
 
     _lib.load()
-    t0 = time.perf_counter()
-    samples = make_training_samples(training_ics(args.ics, seed=1000 + args.seed), n=args.n,
-                                    tau=args.tau)
-    total = samples.shape[0]
-    samples = samples[samples[:, 11] > 0].contiguous()
-    print(f"[train] {total} records ({samples.shape[0]} used) in "
-          f"{time.perf_counter() - t0:.1f} s", flush=True)
-    w = MLPWeights.from_numpy(init_params(args.seed), args.tau)
-    tr = Trainer(w, lr=args.lr, lambda_js=args.lambda_js, delta=args.delta)
-    t0 = time.perf_counter()
-    tr.fit(samples, epochs=args.epochs, batch=args.batch, seed=args.seed)
-    torch.cuda.synchronize()
-    print(f"[train] {args.epochs} epochs in {time.perf_counter() - t0:.1f} s; loss per "
-          f"epoch {' '.join(f'{x:.4e}' for x in tr.history)}", flush=True)
-    w.save(args.out, loss=np.asarray(tr.history), ics=args.ics, n=args.n,
-           lambda_js=args.lambda_js, delta=args.delta, seed=args.seed)
+    if args.load:
+        w = MLPWeights.load(args.load)
+        w = MLPWeights(w.params, args.tau, args.sigma)
+    else:
+        t0 = time.perf_counter()
+        # every record outside the shock regime: training on the learned
+        # regime alone (a few 1e4 records next to shocks) generalises badly
+        samples = make_training_samples(training_ics(args.ics, seed=1000 + args.seed),
+                                        n=args.n, tau=args.tau, sigma=0.0)
+        total = samples.shape[0]
+        samples = samples[samples[:, 11] > 0].contiguous()
+        print(f"[train] {total} records ({samples.shape[0]} used) in "
+              f"{time.perf_counter() - t0:.1f} s", flush=True)
+        w = MLPWeights.from_numpy(init_params(args.seed), args.tau, args.sigma)
+        tr = Trainer(w, lr=args.lr, lambda_js=args.lambda_js, delta=args.delta)
+        t0 = time.perf_counter()
+        tr.fit(samples, epochs=args.epochs, batch=args.batch, seed=args.seed)
+        torch.cuda.synchronize()
+        print(f"[train] {args.epochs} epochs in {time.perf_counter() - t0:.1f} s; loss per "
+              f"epoch {' '.join(f'{x:.4e}' for x in tr.history)}", flush=True)
+        w.save(args.out, loss=np.asarray(tr.history), ics=args.ics, n=args.n,
+               lambda_js=args.lambda_js, delta=args.delta, seed=args.seed)
 
     # validation on unseen instances, FP64: plain WENO-JS, the untrained
-    # network (ideal weights d with the same fallback) and the trained one
+    # network (ideal weights with the same fallback) and the trained one
     ics = training_ics(32, seed=2000 + args.seed)
     ns = np.round(np.exp(np.random.default_rng(args.seed).uniform(
         np.log(1024), np.log(8192), 32))).astype(np.int64)
@@ -390,11 +409,15 @@ def main(argv: list[str] | None = None) -> int:
         print(f"[train] validation t={tv}: mean L1 plain {e0.mean():.4e} ideal+fallback "
               f"{ei.mean():.4e} learned {e1.mean():.4e} (learned/plain {e1.mean() / e0.mean():.4f},"
               f" learned/ideal {e1.mean() / ei.mean():.4f})", flush=True)
-    for tau in args.tau_sweep:
-        wt = MLPWeights(w.params, tau)
-        e0, _ = l1_errors(ics, ns, 0.5)
-        e1, _ = l1_errors(ics, ns, 0.5, mlp=wt)
-        print(f"[train] tau {tau}: t=0.5 learned/plain {e1.mean() / e0.mean():.4f}", flush=True)
+    for sg in args.sigma_sweep:
+        ws = MLPWeights(w.params, w.tau, sg)
+        ratios = []
+        for tv in (0.05, 0.5):
+            e0, i0 = l1_errors(ics, ns, tv)
+            e1, i1 = l1_errors(ics, ns, tv, mlp=ws)
+            ratios.append(f"t={tv}: learned/plain {e1.mean() / e0.mean():.4f} "
+                          f"time learned/plain {i1['seconds'] / i0['seconds']:.2f}")
+        print(f"[train] sigma {sg}: " + "; ".join(ratios), flush=True)
     return 0
 
 

@@ -230,6 +230,7 @@ def _scheme_plan(source: Any) -> tuple[_lib.PsaSolverDesc, tuple[Any, ...]] | No
             return None
         desc.mlp = scheme.rec.weights.ptr
         desc.mlp_tau = float(scheme.rec.weights.tau)
+        desc.mlp_sigma = float(scheme.rec.weights.sigma)
         keep = (scheme.rec.weights,)
     if isinstance(scheme, burgers.Rusanov):
         desc.flux = _lib.PSA_FLUX_RUSANOV
