@@ -133,8 +133,18 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
   __shared__ int s_flag;
   __shared__ unsigned int q[LEARNED ? (BLOCK / 64) * learned_queue_words<CPT>() : 1];
   const int tid = threadIdx.x;
-  MlpWave W;
-  if constexpr (LEARNED) mlp_load(W, a.mlp);
+  // a persistent workgroup keeps the network's operand registers in LDS
+  __shared__ MlpLdsSlot<LEARNED> wl;
+  const MlpWaveLds *wlp = nullptr;
+  if constexpr (LEARNED) {
+    if (tid < 64) {
+      MlpWave W;
+      mlp_load(W, a.mlp);
+      mlp_store_lds(W, wl.w);
+    }
+    __syncthreads();
+    wlp = &wl.w;
+  }
 
   for (;;) {
     __syncthreads();  // previous slot done with the LDS and s_next
@@ -257,7 +267,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
       for (int64_t tile = tlo; tile < thi; ++tile) {
         __syncthreads();
         stage_tile<T, FLUX, 0, 1, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
-                                                          vdummy, W, q);
+                                                          vdummy, q, wlp);
       }
       barrier();  // k1 complete
       // every member has read the max|u| reduced two steps ago: clear that
@@ -270,7 +280,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
       for (int64_t tile = tlo; tile < thi; ++tile) {
         __syncthreads();
         stage_tile<T, FLUX, 0, 2, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
-                                                          vdummy, W, q);
+                                                          vdummy, q, wlp);
       }
       barrier();  // k2 complete
       p.uk = k2;
@@ -279,7 +289,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
       for (int64_t tile = tlo; tile < thi; ++tile) {
         __syncthreads();
         stage_tile<T, FLUX, 0, 3, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
-                                                       vmax, W, q);
+                                                       vmax, q, wlp);
       }
       umax = team_max(vmax, static_cast<int>(steps & 1));  // u complete
       t = t + dt;

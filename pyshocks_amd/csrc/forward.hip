@@ -313,8 +313,6 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   __shared__ StageSmem<T, VEL, BLOCK, CPT> sm;
   __shared__ unsigned int q[LEARNED ? (BLOCK / 64) * learned_queue_words<CPT>() : 1];
   const int tid = threadIdx.x;
-  MlpWave W;
-  if constexpr (LEARNED) mlp_load(W, p.mlp);
   const int64_t ntiles = (p.m + TILE - 1) / TILE;
 
   T dt;
@@ -324,7 +322,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   const int64_t tile = blockIdx.x;  // the grid has one workgroup per tile
   if (tile < ntiles) {
     stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT, LEARNED>(p, tile * TILE, dt,
-                                                         sm, vmax, W, q);
+                                                         sm, vmax, q);
   }
 
   if constexpr (STAGE == 3) {
@@ -352,7 +350,6 @@ __global__ __launch_bounds__(BLOCK) void k_stage_pipe(StageArgs<T> p) {
   __shared__ StageSmem<T, VEL, BLOCK, CPT> smA;
   __shared__ StageSmem<T, VEL, BLOCK, CPT> smB;
   const int tid = threadIdx.x;
-  const MlpWave W{};
   const int64_t ntiles = (p.m + TILE - 1) / TILE;
 
   T dt;
@@ -378,7 +375,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage_pipe(StageArgs<T> p) {
     tile_sync<true>();
     issue(t + gridDim.x, nxt);
     stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT, false, true, true>(p, t * TILE, dt,
-                                                                 cur, vmax, W);
+                                                                 cur, vmax);
     // the copies into nxt are older than this tile's stores: on the fast path
     // (a known number of stores per wave) only the stores may stay in flight
     if (fast_tile(t)) {

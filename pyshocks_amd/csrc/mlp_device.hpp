@@ -122,6 +122,69 @@ __device__ __forceinline__ void mlp_load(MlpWave &W, const float *__restrict__ P
   for (int v = 0; v < 4; ++v) W.b3[v] = v < MLP_OUT ? P[MLP_B3 + v] + lnd[v] : 0.0f;
 }
 
+// The operand registers of every lane of a wave, kept in LDS by persistent
+// kernels (they depend on the lane only): a lane fetches its 160 bytes when
+// the network runs instead of holding 40 registers throughout.
+struct MlpWaveLds {
+  bf16x8_t w1[2][64];
+  bf16x8_t w2[2][64];
+  bf16x8_t w3[64];
+  f32x4_t b1[2][64];
+  f32x4_t b2[2][64];
+  f32x4_t b3[64];
+};
+
+// LDS room for MlpWaveLds only in the kernels that use it.
+template <bool USED>
+struct MlpLdsSlot {
+  MlpWaveLds w;
+};
+template <>
+struct MlpLdsSlot<false> {
+  int unused;
+};
+
+// One wave stores its operand registers; a barrier must follow.
+__device__ __forceinline__ void mlp_store_lds(const MlpWave &W, MlpWaveLds &L) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    L.w1[b][lane] = W.w1[b];
+    L.w2[b][lane] = W.w2[b];
+    L.b1[b][lane] = W.b1[b];
+    L.b2[b][lane] = W.b2[b];
+  }
+  L.w3[lane] = W.w3;
+  L.b3[lane] = W.b3;
+}
+
+__device__ __forceinline__ void mlp_load_lds(MlpWave &W, const MlpWaveLds &L) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    W.w1[b] = L.w1[b][lane];
+    W.w2[b] = L.w2[b][lane];
+    W.b1[b] = L.b1[b][lane];
+    W.b2[b] = L.b2[b][lane];
+  }
+  W.w3 = L.w3[lane];
+  W.b3 = L.b3[lane];
+}
+
+// Where a kernel finds the network: P in global memory, or L in LDS if set.
+struct MlpSource {
+  const float *P;
+  const MlpWaveLds *L;
+};
+
+__device__ __forceinline__ void mlp_fetch(MlpWave &W, const MlpSource &src) {
+  if (src.L != nullptr) {
+    mlp_load_lds(W, *src.L);
+  } else {
+    mlp_load(W, src.P);
+  }
+}
+
 typedef short mlp_s16x2_t __attribute__((ext_vector_type(2)));
 typedef __bf16 mlp_bf16x2_t __attribute__((ext_vector_type(2)));
 typedef unsigned int mlp_u32x4_t __attribute__((ext_vector_type(4)));
@@ -317,25 +380,33 @@ __device__ __forceinline__ void learned_pair(const MlpWave &W, T um2, T um1, T u
   }
 }
 
-// Learned right reconstruction of the stencil of lanes 0..15 (wave-
-// collective; the other lanes' results are undefined) with one 16-sample
-// chain; js reports the shock regime (the WENO-JS value is then to be used
-// instead).
+// Learned right reconstruction of the stencils of lanes 0 and 1 (wave-
+// collective; the other lanes' results are undefined): the network (fetched
+// from src only then) runs as one 16-sample chain, and only if one of the two
+// cells is in the learned regime; js reports the shock regime (the WENO-JS
+// value is then to be used instead).
 template <typename T>
-__device__ __forceinline__ T learned_right16(const MlpWave &W, T um2, T um1, T u0,
-                                             T up1, T up2, T eps, T tau, T sigma,
+__device__ __forceinline__ T learned_right16(const MlpSource &src, T um2, T um1,
+                                             T u0, T up1, T up2, T eps, T tau, T sigma,
                                              bool &js) {
   T b0, b1, b2;
   weno_beta(um2, um1, u0, up1, up2, b0, b1, b2);
   const int regime = mlp_regime(b0, b1, b2, eps, tau, sigma);
   js = regime == 0;
-  float x[MLP_IN], z[MLP_OUT], w[MLP_OUT];
-  mlp_features(um2, um1, u0, up1, up2, x);
-  mlp_logits<1>(W, x, z);
-  mlp_softmax(z, w);
   T h0, h1, h2;
   weno_uhat(um2, um1, u0, up1, up2, h0, h1, h2);
-  return regime == 1 ? mlp_combine_ideal(h0, h1, h2) : mlp_combine(w, h0, h1, h2);
+  T value = mlp_combine_ideal(h0, h1, h2);
+  const int lane = threadIdx.x & 63;
+  if (__ballot(lane < 2 && regime == 2) != 0) {  // wave-uniform
+    MlpWave W;
+    mlp_fetch(W, src);
+    float x[MLP_IN], z[MLP_OUT], w[MLP_OUT];
+    mlp_features(um2, um1, u0, up1, up2, x);
+    mlp_logits<1>(W, x, z);
+    mlp_softmax(z, w);
+    if (regime == 2) value = mlp_combine(w, h0, h1, h2);
+  }
+  return value;
 }
 
 // Both reconstructions of one cell per lane from its stencil alone

@@ -230,11 +230,12 @@ constexpr int learned_queue_words() {
 // Learned reconstructions of a thread's CPT cells (wave-collective). Cells in
 // the learned regime put the inputs of their two samples (right stencil,
 // mirrored stencil) into the wave's LDS queue q; the wave evaluates the
-// queue 64 samples at a time (and only the chains that hold samples), the
-// weights come back through the same slots. Shock cells keep the WENO-JS
+// queue 64 samples at a time (and only the chains that hold samples; the
+// network is fetched from src only if the queue is not empty), the weights
+// come back through the same slots. Shock cells keep the WENO-JS
 // values already in ul, ur; smooth cells take the ideal weights.
 template <typename T, int CPT>
-__device__ __forceinline__ void learned_cells(const MlpWave &W, const T *w,
+__device__ __forceinline__ void learned_cells(const MlpSource &src, const T *w,
                                               const int *regime, T *ul, T *ur,
                                               unsigned int *q) {
   const int lane = threadIdx.x & 63;
@@ -273,6 +274,9 @@ __device__ __forceinline__ void learned_cells(const MlpWave &W, const T *w,
       }
     }
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    // the network's operand registers live only while the queue is evaluated
+    MlpWave W;
+    mlp_fetch(W, src);
     for (int r0 = 0; r0 < ntot; r0 += 64) {  // wave-uniform
       const int e = r0 + lane;
       mlp_u32x4_t own = {0u, 0u, 0u, 0u};
@@ -345,6 +349,8 @@ struct StageSmem {
 //        every wave of the workgroup must run the tile).
 // PRELOADED: a tile on the fast path is already in sm.u (and u^n in sm.io)
 //        and visible to the workgroup; ASYNC: barriers wait for LDS only.
+// wl: the network's operand registers in LDS (persistent kernels), or null
+//        (then they are loaded from p.mlp where the network runs).
 // STAGE: 0 operator (all m cells, faces 0 and m padded with zero flux),
 //        1..3 SSP-RK3 stages (interior cells, ghost images rewritten).
 //
@@ -362,8 +368,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
                                            T dt,
                                            StageSmem<T, VEL, BLOCK, CPT> &sm,
                                            unsigned long long &vmax,
-                                           const MlpWave &W,
-                                           unsigned int *q = nullptr) {
+                                           unsigned int *q = nullptr,
+                                           const MlpWaveLds *wl = nullptr) {
   static_assert(!LEARNED || VEL == 0, "learned weights are for Burgers");
   constexpr int TILE = BLOCK * CPT;
   constexpr int HALO = 4;
@@ -438,11 +444,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     const T b0 = E[r] + C::a1 * (t01 * t01);
     const T b1 = E[r + 1] + C::a1 * (t11 * t11);
     const T b2 = E[r + 2] + C::a1 * (t21 * t21);
+    // (unconditionally, also in the learned kernels: computed in the same
+    // context, the shock regime reproduces the WENO-JS kernels bit for bit)
     weno_pair_from_beta(um2, um1, u0, up1, up2, b0, b1, b2, p.eps, ul[r],
                         ur[r]);
     if constexpr (LEARNED) regime[r] = mlp_regime(b0, b1, b2, p.eps, p.tau, p.sigma);
   }
-  if constexpr (LEARNED) learned_cells<T, CPT>(W, w, regime, ul, ur, q);
+  const MlpSource src{p.mlp, wl};
+  if constexpr (LEARNED) learned_cells<T, CPT>(src, w, regime, ul, ur, q);
 
   T al[VEL == 2 ? CPT : 1], ar[VEL == 2 ? CPT : 1];
   if constexpr (VEL == 2) {
@@ -472,7 +481,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
       const int c = hi ? TILE + HALO : HALO - 1;
       const int d = hi ? -1 : 1;
       bool js;
-      const T r = learned_right16(W, sm.u[c - 2 * d], sm.u[c - d], sm.u[c],
+      const T r = learned_right16(src, sm.u[c - 2 * d], sm.u[c - d], sm.u[c],
                                   sm.u[c + d], sm.u[c + 2 * d], p.eps, p.tau,
                                   p.sigma, js);
       if (tid < 2) {
