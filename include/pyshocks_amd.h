@@ -148,12 +148,21 @@ typedef struct psa_control {
   int status;                   /* PSA_OK or PSA_ERR_NONFINITE_DT          */
   int dt_mode;                  /* PSA_DT_CFL or PSA_DT_FIXED              */
   int speed_from_state;         /* 1: max|u| of each state (Burgers)       */
-  int pad_[2];
-  /* stage 3 reduces max|u_int| of the new state into these slots (a
-   * workgroup uses slot blockIdx % PSA_UMAX_SLOTS); stage 1 of the next
-   * step combines them and stage 2 clears them. */
+  /* whole-step schedule (PSA_SCHEDULE_STEP): the decision for the next
+   * step, made by the one-wave kernel after the previous step launch (or
+   * before the first one of psa_ssprk33_steps) */
+  double dt_next;
+  int active_next;
+  int pad1_;
+  unsigned long long pad_[3];
+  /* stage 3 reduces max|u_int| of the new state into word 0 of these slots
+   * (a workgroup uses slot blockIdx % PSA_UMAX_SLOTS); stage 1 of the next
+   * step combines them and stage 2 clears them. The whole-step kernel
+   * reduces into word PSA_UMAX_STEP_WORD; the kernel after it leaves the
+   * combined value in word 0 of slot 0 (the other word-0 entries zero). */
   unsigned long long umax_slot[PSA_UMAX_SLOTS][PSA_UMAX_SLOT_STRIDE];
 } psa_control;
+#define PSA_UMAX_STEP_WORD 8
 
 typedef struct psa_solver_desc {
   int dtype;                    /* PSA_FLOAT64 / PSA_FLOAT32               */
@@ -177,12 +186,24 @@ typedef struct psa_solver_desc {
   int pad_;
 } psa_solver_desc;
 
-/* Stage-kernel schedules: one workgroup per tile, or a persistent grid that
- * copies its next tile into LDS while computing the current one (WENO-JS
- * weights, no velocity array); both compute the same values. */
+/* Step schedules of psa_ssprk33_steps:
+ * TILE: one kernel per stage, one workgroup per tile (8 words of HBM traffic
+ *   per cell and step);
+ * PIPELINE: the stage kernels on a persistent grid that copies its next tile
+ *   into LDS while computing the current one (WENO-JS weights, no velocity
+ *   array);
+ * STEP: one kernel per step (and a one-wave step-control kernel after it); a
+ *   workgroup loads its tile of u^n with a 9-cell halo on each side, runs the
+ *   three stages in LDS (recomputing the halo cells of the first two) and
+ *   writes u^(n+1) (2 words per cell and step; WENO-JS or learned weights, no
+ *   velocity array, no Rusanov nu, a control block, tiles 256 x 4 or
+ *   256 x 2). Steps ping-pong between u_out and the stage buffers.
+ * DEFAULT is STEP where it applies and TILE otherwise. All schedules compute
+ * the same values. */
 #define PSA_SCHEDULE_DEFAULT 0
 #define PSA_SCHEDULE_TILE 1
 #define PSA_SCHEDULE_PIPELINE 2
+#define PSA_SCHEDULE_STEP 3
 
 /* Initialise the control block (device memory, sizeof(psa_control)) for a
  * run starting from state u at time tstart: stores t, tfinal, maxit, the dt

@@ -57,7 +57,10 @@ class PsaControl(ctypes.Structure):
         ("status", ctypes.c_int),
         ("dt_mode", ctypes.c_int),
         ("speed_from_state", ctypes.c_int),
-        ("pad_", ctypes.c_int * 2),
+        ("dt_next", ctypes.c_double),
+        ("active_next", ctypes.c_int),
+        ("pad1_", ctypes.c_int),
+        ("pad_", ctypes.c_ulonglong * 3),
         ("umax_slot", (ctypes.c_ulonglong * 16) * 64),
     ]
 
@@ -166,7 +169,7 @@ SIGNATURES: dict[str, list[Any]] = {
 }
 
 TEAM_STATE_BYTES = 64
-PSA_SCHEDULE_DEFAULT, PSA_SCHEDULE_TILE, PSA_SCHEDULE_PIPELINE = 0, 1, 2
+PSA_SCHEDULE_DEFAULT, PSA_SCHEDULE_TILE, PSA_SCHEDULE_PIPELINE, PSA_SCHEDULE_STEP = 0, 1, 2, 3
 MLP_NPARAM = 1347
 MLP_RECORD = 12
 

@@ -32,6 +32,10 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # matrix-core results in VGPRs (no copies out of accumulation registers)
     "-mllvm", "-amdgpu-mfma-vgpr-form=1",
+    # multiply-adds fused within an expression only: a formula rounds the
+    # same way in every kernel it is inlined into (the stage kernels and the
+    # whole-step kernel agree bit for bit)
+    "-ffp-contract=on",
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-command-line-argument",

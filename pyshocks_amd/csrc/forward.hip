@@ -199,8 +199,13 @@ __global__ void k_control_init(psa_control *ctrl, double t, double tfinal,
   ctrl->m = 0;
   ctrl->maxit = maxit;
   ctrl->umax_bits = abs_bits(static_cast<T>(speed_const));
-  for (int k = 0; k < PSA_UMAX_SLOTS; ++k) ctrl->umax_slot[k][0] = 0ull;
+  for (int k = 0; k < PSA_UMAX_SLOTS; ++k) {
+    ctrl->umax_slot[k][0] = 0ull;
+    ctrl->umax_slot[k][PSA_UMAX_STEP_WORD] = 0ull;
+  }
   ctrl->umax_slot[0][0] = ctrl->umax_bits;
+  ctrl->dt_next = 0.0;
+  ctrl->active_next = 0;
   ctrl->active = 0;
   ctrl->status = PSA_OK;
   ctrl->dt_mode = dt_mode;
@@ -407,6 +412,275 @@ __global__ __launch_bounds__(BLOCK) void k_stage_pipe(StageArgs<T> p) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// whole-step kernel: the three SSP-RK3 stages of a tile in LDS
+//
+// A workgroup owns the output cells [A, A + TO) of the array, TO = C - 16
+// with C = BLOCK * CPT, and holds the cells base .. base + C + 8 of u^n,
+// base = A - 12. Thread t computes the cells j0 = 4 + CPT t .. j0 + CPT - 1
+// (relative to base) in every stage, so each stage computes C cells: stage 1
+// is exact on [5, C + 3) (the two outermost faces have no neighbouring
+// thread), stage 2 on [8, C), stage 3 on [11, C - 3), which contains the
+// output cells [12, C - 4). The halo cells are recomputed by the neighbouring
+// tiles (16 of every C cells). Every cell is computed by the arithmetic of
+// the stage kernels, so the results are the same bit for bit.
+
+// The decision for the next step from max|u| of the state it starts from
+// (stage 1 of timestepping.py:89-152, control_decide).
+template <typename T>
+__device__ __forceinline__ void step_decide(psa_control *ctrl,
+                                            unsigned long long umax) {
+  T dt;
+  int status;
+  const bool active = control_decide<T>(ctrl, umax, dt, status);
+  ctrl->status = status;
+  ctrl->umax_bits = umax;
+  ctrl->active_next = active ? 1 : 0;
+  ctrl->dt_next = active ? static_cast<double>(dt) : 0.0;
+}
+
+// Before the first whole-step launch of a sequence (one wave): decides the
+// first step from the word-0 slots (as stage 1 does) and clears the slots
+// the step kernel reduces into.
+template <typename T>
+__global__ void k_step_prologue(psa_control *ctrl) {
+  const int lane = threadIdx.x;
+  const unsigned long long umax = gather_umax(ctrl);
+  ctrl->umax_slot[lane][PSA_UMAX_STEP_WORD] = 0ull;
+  if (lane == 0) step_decide<T>(ctrl, umax);
+}
+
+// After each whole-step launch (one wave): if the step was taken, advances
+// t and m (stage 2), combines the max|u| partials of the new state, leaves
+// them in the word-0 slots for the stage kernels and decides the next step;
+// otherwise records that the step was not taken.
+template <typename T>
+__global__ void k_step_finish(psa_control *ctrl) {
+  const int lane = threadIdx.x;
+  if (ctrl->active_next == 0) {
+    if (lane == 0) ctrl->active = 0;
+    return;
+  }
+  const T dt = static_cast<T>(ctrl->dt_next);
+  unsigned long long umax = ctrl->umax_bits;
+  if (ctrl->speed_from_state) {
+    unsigned long long v = ctrl->umax_slot[lane][PSA_UMAX_STEP_WORD];
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(v, off, 64);
+      v = o > v ? o : v;
+    }
+    umax = v;
+    ctrl->umax_slot[lane][PSA_UMAX_STEP_WORD] = 0ull;
+    ctrl->umax_slot[lane][0] = lane == 0 ? umax : 0ull;
+  }
+  if (lane == 0) {
+    ctrl->t = static_cast<double>(static_cast<T>(ctrl->t) + dt);
+    ctrl->m = ctrl->m + 1;
+    ctrl->dt = static_cast<double>(dt);
+    ctrl->active = 1;
+    step_decide<T>(ctrl, umax);
+  }
+}
+
+template <typename T, int BLOCK, int CPT>
+struct StepSmem {
+  static constexpr int C = BLOCK * CPT;  // cells computed per stage
+  static constexpr int L = C + 8;        // cells held per buffer
+  __align__(16) T x0[L];                 // u^n, then k2
+  __align__(16) T x1[L];                 // k1
+  T urL[BLOCK + 1];
+  T ulF[BLOCK + 1];
+  unsigned long long red[BLOCK / 64];
+};
+
+// One stage of the tile from the LDS buffer X into val (the thread's CPT
+// cells): the reconstruction, the face fluxes and the stage combination of
+// stage_tile (Rusanov with nu = 1, Godunov, or advection with a constant
+// velocity). un holds the thread's cells of u^n.
+template <typename T, int FLUX, int VEL, int BLOCK, int CPT, int STAGE,
+          bool LEARNED>
+__device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
+                                           StepSmem<T, BLOCK, CPT> &sm, T dt,
+                                           const T *un, T *val,
+                                           const MlpSource &src,
+                                           unsigned int *q) {
+  const int tid = threadIdx.x;
+  const int j0 = 4 + CPT * tid;
+  T w[CPT + 4];  // cells j0 - 2 .. j0 + CPT + 1
+#pragma unroll
+  for (int k = 0; k < CPT + 4; ++k) w[k] = X[j0 - 2 + k];
+  T ul[CPT], ur[CPT];
+  reconstruct_cells<T, CPT, LEARNED>(p, w, ul, ur, src, q);
+
+  sm.urL[tid + 1] = ur[CPT - 1];
+  sm.ulF[tid] = ul[0];
+  __syncthreads();
+
+  T F[CPT + 1];
+#pragma unroll
+  for (int k = 0; k <= CPT; ++k) {
+    const T urL = k == 0 ? sm.urL[tid] : ur[k - 1];
+    const T ulR = k == CPT ? sm.ulF[tid + 1] : ul[k];
+    if constexpr (FLUX == PSA_FLUX_RUSANOV) {
+      F[k] = flux_rusanov_nu1(urL, ulR, w[k + 1], w[k + 2]);
+    } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
+      F[k] = flux_burgers_upwind(urL, ulR);
+    } else {
+      F[k] = flux_advection_upwind(urL, ulR, p.vconst, p.vconst);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) {
+    const T dF = F[r + 1] - F[r];
+    T L;
+    if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
+      L = (-p.vconst * dF) * p.inv_dx;
+    } else {
+      L = -dF * p.inv_dx;
+    }
+    const T uk = w[r + 2];
+    if constexpr (STAGE == 1) {
+      val[r] = uk + dt * L;
+    } else if constexpr (STAGE == 2) {
+      val[r] = T(3.0 / 4.0) * un[r] + T(1.0 / 4.0) * (uk + dt * L);
+    } else {
+      val[r] = T(1.0 / 3.0) * un[r] + T(2.0 / 3.0) * (uk + dt * L);
+    }
+  }
+}
+
+// One SSP-RK3 step (timestepping.py:312-320) of u^n = p.uk into
+// u^(n+1) = p.out (distinct arrays) with the step control of the control
+// block: the decision made before the launch says whether the step is taken
+// (if not, the state is copied to p.out) and with which dt; the workgroups
+// reduce max|u_int| of the new state into the step slots for k_step_finish.
+template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED>
+__global__ __launch_bounds__(BLOCK) void k_step(StageArgs<T> p) {
+  static_assert(VEL != 2, "no velocity array in the whole-step kernel");
+  using SM = StepSmem<T, BLOCK, CPT>;
+  constexpr int C = SM::C;
+  constexpr int L = SM::L;
+  constexpr int TO = C - 16;
+  constexpr int VN = Vec16<T>::n;
+  static_assert(C % 16 == 0, "tile geometry");
+  __shared__ SM sm;
+  __shared__ unsigned int q[LEARNED ? (BLOCK / 64) * learned_queue_words<CPT>() : 1];
+  const int tid = threadIdx.x;
+  psa_control *ctrl = p.ctrl;
+  const int64_t m = p.m;
+  const int64_t n = p.n;
+  const int g = p.g;
+  const int64_t A = static_cast<int64_t>(blockIdx.x) * TO;
+  const int64_t base = A - 12;
+  const T *__restrict__ uin = p.uk;
+  T *__restrict__ uout = p.out;
+
+  if (ctrl->active_next == 0) {
+    // the step is not taken: the state moves to the output unchanged
+    for (int j = tid; j < TO; j += BLOCK) {
+      const int64_t a = A + j;
+      if (a < m) uout[a] = uin[a];
+    }
+    return;
+  }
+  const T dt = static_cast<T>(ctrl->dt_next);
+
+  // ---- u^n with a 12-cell margin: interior tiles by 16-byte loads, the
+  // others by periodic index (interior cell (a - g) mod n)
+  const bool fast = p.aligned != 0 && base >= g && base + L <= m - g;
+  if (fast) {
+    Chunks<T, BLOCK, L / VN> cu;
+    cu.load(uin + base, tid);
+    cu.store(sm.x0, tid);
+  } else {
+    for (int j = tid; j < L; j += BLOCK) {
+      int64_t i = (base + j - g) % n;
+      if (i < 0) i += n;
+      sm.x0[j] = uin[g + i];
+    }
+  }
+  // the k1 entries stage 2 reads outside the cells stage 1 writes, and the
+  // exchange entries without a neighbouring thread (their results are
+  // outside the exact ranges; finite values keep them on the fast paths)
+  if (tid < 8) sm.x1[tid < 4 ? tid : C + tid] = T(0);
+  if (tid == 0) {
+    sm.urL[0] = T(0);
+    sm.ulF[BLOCK] = T(0);
+  }
+  __syncthreads();
+
+  const int j0 = 4 + CPT * tid;
+  T un[CPT], val[CPT];
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) un[r] = sm.x0[j0 + r];
+  const MlpSource src{p.mlp, nullptr};
+
+  step_stage<T, FLUX, VEL, BLOCK, CPT, 1, LEARNED>(p, sm.x0, sm, dt, un, val, src, q);
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) sm.x1[j0 + r] = val[r];
+  __syncthreads();
+  // (every read of x0 ended before the exchange barrier of stage 1)
+  step_stage<T, FLUX, VEL, BLOCK, CPT, 2, LEARNED>(p, sm.x1, sm, dt, un, val, src, q);
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) sm.x0[j0 + r] = val[r];
+  __syncthreads();
+  step_stage<T, FLUX, VEL, BLOCK, CPT, 3, LEARNED>(p, sm.x0, sm, dt, un, val, src, q);
+
+  // ---- u^(n+1): the output cells [12, C - 4), with their max|u| if interior
+  unsigned long long vmax = 0;
+  const int64_t a0 = base + j0;
+  if (fast && j0 >= 12 && j0 + CPT <= C - 4) {
+    if constexpr (CPT % VN == 0) {
+#pragma unroll
+      for (int r = 0; r < CPT; r += VN) {
+        typename Vec16<T>::type v;
+        if constexpr (VN == 2) {
+          v = {val[r], val[r + 1]};
+        } else {
+          v = {val[r], val[r + 1], val[r + 2], val[r + 3]};
+        }
+        store16(uout + a0 + r, v);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < CPT; ++r) uout[a0 + r] = val[r];
+    }
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+      const unsigned long long b = abs_bits(val[r]);
+      vmax = b > vmax ? b : vmax;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+      const int j = j0 + r;
+      const int64_t a = a0 + r;
+      if (j >= 12 && j < C - 4 && a >= 0 && a < m) {
+        uout[a] = val[r];
+        if (a >= g && a < m - g) {
+          const unsigned long long b = abs_bits(val[r]);
+          vmax = b > vmax ? b : vmax;
+        }
+      }
+    }
+  }
+
+  // ---- partial max|u| of the new state
+  if (ctrl->speed_from_state != 0) {
+    vmax = block_max_bits<BLOCK>(vmax, sm.red);
+    if (tid == 0)
+      atomicMax(&ctrl->umax_slot[blockIdx.x % PSA_UMAX_SLOTS][PSA_UMAX_STEP_WORD],
+                vmax);
+  }
+}
+
+template <typename T>
+__global__ void k_copy(const T *__restrict__ src, T *__restrict__ dst,
+                       int64_t m) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < m) dst[i] = src[i];
+}
+
 }  // namespace psa
 
 // ===========================================================================
@@ -469,6 +743,7 @@ struct TileConfig {
   int block;
   int cpt;
   bool pipe;  // stages 1-3: the persistent pipelined kernel
+  bool step;  // steps: the whole-step kernel where it applies
 };
 
 inline bool aligned16(const void *p) {
@@ -555,6 +830,56 @@ int launch_stage_tiles(const StageArgs<T> &a, TileConfig cfg,
                        " cells_per_thread=" + std::to_string(cfg.cpt));
 }
 
+// Whole-step kernel: Burgers (WENO-JS or learned weights) and advection
+// with a constant velocity.
+template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED>
+int launch_step_cfg(const StageArgs<T> &args, hipStream_t stream) {
+  constexpr int TO = BLOCK * CPT - 16;
+  StageArgs<T> a = args;
+  a.aligned = aligned16(a.uk) && aligned16(a.out);
+  const unsigned int blocks = blocks_for(a.m, TO);
+  hipLaunchKernelGGL((k_step<T, FLUX, VEL, BLOCK, CPT, LEARNED>), dim3(blocks),
+                     dim3(BLOCK), 0, stream, a);
+  return check_launch("ssprk33 step");
+}
+
+template <typename T, int FLUX, int VEL, bool LEARNED>
+int launch_step_tiles(const StageArgs<T> &a, TileConfig cfg,
+                      hipStream_t stream) {
+  if (cfg.block == 256 && cfg.cpt == 4)
+    return launch_step_cfg<T, FLUX, VEL, 256, 4, LEARNED>(a, stream);
+  if (cfg.block == 256 && cfg.cpt == 2)
+    return launch_step_cfg<T, FLUX, VEL, 256, 2, LEARNED>(a, stream);
+  return set_error(PSA_ERR_INVALID_ARGUMENT,
+                   "unsupported tile configuration for the whole-step kernel block=" +
+                       std::to_string(cfg.block) +
+                       " cells_per_thread=" + std::to_string(cfg.cpt));
+}
+
+// Whether the whole-step kernel runs the steps of this descriptor.
+inline bool step_applies(const psa_solver_desc *d, TileConfig cfg, int vel,
+                         const void *control) {
+  if (!cfg.step || control == nullptr || vel == 2 || d->nu != nullptr) return false;
+  if (d->flux == PSA_FLUX_ADVECTION_UPWIND && d->mlp != nullptr) return false;
+  return cfg.block == 256 && (cfg.cpt == 4 || cfg.cpt == 2);
+}
+
+template <typename T>
+int launch_step(const psa_solver_desc *d, int vel, const StageArgs<T> &a,
+                TileConfig cfg, hipStream_t stream) {
+  if (d->mlp != nullptr) {
+    if (d->flux == PSA_FLUX_RUSANOV)
+      return launch_step_tiles<T, PSA_FLUX_RUSANOV, 0, true>(a, cfg, stream);
+    return launch_step_tiles<T, PSA_FLUX_BURGERS_UPWIND, 0, true>(a, cfg, stream);
+  }
+  if (d->flux == PSA_FLUX_RUSANOV)
+    return launch_step_tiles<T, PSA_FLUX_RUSANOV, 0, false>(a, cfg, stream);
+  if (d->flux == PSA_FLUX_BURGERS_UPWIND)
+    return launch_step_tiles<T, PSA_FLUX_BURGERS_UPWIND, 0, false>(a, cfg, stream);
+  (void)vel;
+  return launch_step_tiles<T, PSA_FLUX_ADVECTION_UPWIND, 1, false>(a, cfg, stream);
+}
+
 // Learned weights: Burgers fluxes and two tile shapes.
 template <typename T, int FLUX, int STAGE>
 int launch_stage_learned(const StageArgs<T> &a, TileConfig cfg,
@@ -599,7 +924,7 @@ int launch_stage(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
 
 TileConfig default_tiles(int dtype) {
   (void)dtype;
-  return TileConfig{256, 4, false};
+  return TileConfig{256, 4, false, true};
 }
 
 int check_desc(const psa_solver_desc *d) {
@@ -634,8 +959,11 @@ TileConfig tiles_of(const psa_solver_desc *d) {
   if (d->mlp != nullptr) c.cpt = 2;
   if (d->block > 0) c.block = d->block;
   if (d->cells_per_thread > 0) c.cpt = d->cells_per_thread;
-  if (d->schedule == PSA_SCHEDULE_TILE) c.pipe = false;
-  if (d->schedule == PSA_SCHEDULE_PIPELINE) c.pipe = true;
+  if (d->schedule == PSA_SCHEDULE_TILE) c.step = false;
+  if (d->schedule == PSA_SCHEDULE_PIPELINE) {
+    c.pipe = true;
+    c.step = false;
+  }
   return c;
 }
 
@@ -677,6 +1005,31 @@ int run_steps(const psa_solver_desc *d, const void *u_in, void *u_out,
   T *b1 = static_cast<T *>(k1);
   T *b2 = static_cast<T *>(k2);
   int st = PSA_OK;
+  if (step_applies(d, cfg, vel, a.ctrl)) {
+    // one launch per step; u^n and u^(n+1) are distinct arrays: the state
+    // goes u_in -> k1 -> k2 -> k1 ... -> u_out (u_in -> k1 -> u_out, by a
+    // copy, for one in-place step)
+    hipLaunchKernelGGL(k_step_prologue<T>, dim3(1), dim3(64), 0, stream, a.ctrl);
+    st = check_launch("step prologue");
+    if (st != PSA_OK) return st;
+    const bool copy_back = nsteps == 1 && un == uo;
+    for (long long s = 0; s < nsteps; ++s) {
+      a.uk = s == 0 ? un : ((s - 1) % 2 == 0 ? b1 : b2);
+      a.out = (s == nsteps - 1 && !copy_back) ? uo : (s % 2 == 0 ? b1 : b2);
+      a.un = a.uk;
+      st = launch_step<T>(d, vel, a, cfg, stream);
+      if (st != PSA_OK) return st;
+      hipLaunchKernelGGL(k_step_finish<T>, dim3(1), dim3(64), 0, stream, a.ctrl);
+      st = check_launch("step finish");
+      if (st != PSA_OK) return st;
+    }
+    if (copy_back) {
+      hipLaunchKernelGGL(k_copy<T>, dim3(blocks_for(a.m, 256)), dim3(256), 0,
+                         stream, b1, uo, a.m);
+      st = check_launch("copy");
+    }
+    return st;
+  }
   for (long long s = 0; s < nsteps; ++s) {
     // after the first step the state lives in u_out
     const T *cur = s == 0 ? un : uo;

@@ -323,6 +323,47 @@ __device__ __forceinline__ void learned_cells(const MlpSource &src, const T *w,
   }
 }
 
+// Left and right states of CPT consecutive cells from w = the cells
+// -2 .. CPT + 1 around them (reconstruction.py:358-377 with WENOJS53; the
+// mirrored left reconstruction uses the same three indicators with beta0 and
+// beta2 swapped, and the second differences are shared between neighbouring
+// cells). LEARNED: the weights of cells outside the shock regime come from
+// the network or the ideal weights (wave-collective, learned_cells).
+template <typename T, int CPT, bool LEARNED>
+__device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
+                                                  const T *w, T *ul, T *ur,
+                                                  const MlpSource &src,
+                                                  unsigned int *q) {
+  using C = Weno5Coeffs<T>;
+  // a0 * (second difference)^2 at cells -1 .. CPT, shared by the three
+  // indicators of neighbouring cells
+  T E[CPT + 2];
+#pragma unroll
+  for (int k = 0; k < CPT + 2; ++k) {
+    const T d2 = (w[k + 2] - T(2) * w[k + 1]) + w[k];
+    E[k] = C::a0 * (d2 * d2);
+  }
+
+  int regime[CPT];
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) {
+    const T um2 = w[r], um1 = w[r + 1], u0 = w[r + 2], up1 = w[r + 3],
+            up2 = w[r + 4];
+    const T t01 = (T(3) * u0 - T(4) * um1) + um2;
+    const T t11 = up1 - um1;
+    const T t21 = (up2 - T(4) * up1) + T(3) * u0;
+    const T b0 = E[r] + C::a1 * (t01 * t01);
+    const T b1 = E[r + 1] + C::a1 * (t11 * t11);
+    const T b2 = E[r + 2] + C::a1 * (t21 * t21);
+    // (unconditionally, also in the learned kernels: computed in the same
+    // context, the shock regime reproduces the WENO-JS kernels bit for bit)
+    weno_pair_from_beta(um2, um1, u0, up1, up2, b0, b1, b2, p.eps, ul[r],
+                        ur[r]);
+    if constexpr (LEARNED) regime[r] = mlp_regime(b0, b1, b2, p.eps, p.tau, p.sigma);
+  }
+  if constexpr (LEARNED) learned_cells<T, CPT>(src, w, regime, ul, ur, q);
+}
+
 // LDS of one stage tile.
 template <typename T, int VEL, int BLOCK, int CPT>
 struct StageSmem {
@@ -417,41 +458,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   tile_sync<ASYNC>();
 
   // ---- reconstruct CPT consecutive cells
-  using C = Weno5Coeffs<T>;
   const int l0 = tid * CPT;
   T w[CPT + 4];  // cells l0 - 2 .. l0 + CPT + 1
 #pragma unroll
   for (int k = 0; k < CPT + 4; ++k) w[k] = sm.u[l0 + 2 + k];
-
-  // a0 * (second difference)^2 at cells l0 - 1 .. l0 + CPT, shared by the
-  // three indicators of neighbouring cells
-  T E[CPT + 2];
-#pragma unroll
-  for (int k = 0; k < CPT + 2; ++k) {
-    const T d2 = (w[k + 2] - T(2) * w[k + 1]) + w[k];
-    E[k] = C::a0 * (d2 * d2);
-  }
-
   T ul[CPT], ur[CPT];
-  int regime[CPT];
-#pragma unroll
-  for (int r = 0; r < CPT; ++r) {
-    const T um2 = w[r], um1 = w[r + 1], u0 = w[r + 2], up1 = w[r + 3],
-            up2 = w[r + 4];
-    const T t01 = (T(3) * u0 - T(4) * um1) + um2;
-    const T t11 = up1 - um1;
-    const T t21 = (up2 - T(4) * up1) + T(3) * u0;
-    const T b0 = E[r] + C::a1 * (t01 * t01);
-    const T b1 = E[r + 1] + C::a1 * (t11 * t11);
-    const T b2 = E[r + 2] + C::a1 * (t21 * t21);
-    // (unconditionally, also in the learned kernels: computed in the same
-    // context, the shock regime reproduces the WENO-JS kernels bit for bit)
-    weno_pair_from_beta(um2, um1, u0, up1, up2, b0, b1, b2, p.eps, ul[r],
-                        ur[r]);
-    if constexpr (LEARNED) regime[r] = mlp_regime(b0, b1, b2, p.eps, p.tau, p.sigma);
-  }
   const MlpSource src{p.mlp, wl};
-  if constexpr (LEARNED) learned_cells<T, CPT>(src, w, regime, ul, ur, q);
+  reconstruct_cells<T, CPT, LEARNED>(p, w, ul, ur, src, q);
 
   T al[VEL == 2 ? CPT : 1], ar[VEL == 2 ? CPT : 1];
   if constexpr (VEL == 2) {
@@ -491,17 +504,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     }
   }
   if (tid == 0) {
-    // right state of local cell -1
-    sm.urL[0] = weno_right(sm.u[1], sm.u[2], sm.u[3], sm.u[4], sm.u[5], p.eps);
+    // right state of local cell -1 (the arithmetic of every other cell)
+    T el, er;
+    reconstruct_cells<T, 1, false>(p, &sm.u[1], &el, &er, src, nullptr);
+    sm.urL[0] = er;
     if constexpr (VEL == 2) {
       sm.arL[0] = weno_right(sm.v[1], sm.v[2], sm.v[3], sm.v[4], sm.v[5], p.eps);
     }
   }
   if (tid == BLOCK - 1) {
-    // left state of local cell TILE (mirrored stencil)
+    // left state of local cell TILE
     constexpr int c = TILE + HALO;
-    sm.ulF[BLOCK] = weno_right(sm.u[c + 2], sm.u[c + 1], sm.u[c], sm.u[c - 1],
-                               sm.u[c - 2], p.eps);
+    T el, er;
+    reconstruct_cells<T, 1, false>(p, &sm.u[c - 2], &el, &er, src, nullptr);
+    sm.ulF[BLOCK] = el;
     if constexpr (VEL == 2) {
       sm.alF[BLOCK] = weno_right(sm.v[c + 2], sm.v[c + 1], sm.v[c],
                                  sm.v[c - 1], sm.v[c - 2], p.eps);

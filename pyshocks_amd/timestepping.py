@@ -199,6 +199,19 @@ class FusedPlan:
                   _lib.ptr(out), _lib.ptr(k1), _lib.ptr(k2), _lib.ptr(dt),
                   _lib.stream_of(u))
 
+    def uses_step_kernel(self) -> bool:
+        """Whether :meth:`steps` runs one whole-step kernel per step (the
+        library's rule: schedule default or step, a constant or no velocity,
+        no Rusanov ``nu``, tiles (256, 4) or (256, 2))."""
+        d = self.desc
+        if d.schedule not in (_lib.PSA_SCHEDULE_DEFAULT, _lib.PSA_SCHEDULE_STEP):
+            return False
+        if d.nu or (d.flux == _lib.PSA_FLUX_ADVECTION_UPWIND and not d.velocity_is_const):
+            return False
+        block = d.block if d.block > 0 else 256
+        cpt = d.cells_per_thread if d.cells_per_thread > 0 else (2 if d.mlp else 4)
+        return block == 256 and cpt in (2, 4)
+
     @staticmethod
     def read_control(ctrl: torch.Tensor) -> _lib.PsaControl:
         host = ctrl.cpu().numpy().tobytes()

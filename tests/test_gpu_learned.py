@@ -145,6 +145,42 @@ def test_learned_fused_vs_generic(torch_mod, tiles):
 
 
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+@pytest.mark.parametrize("tiles", [(256, 2), (256, 4)])
+def test_learned_step_schedule_matches_stage_kernels(torch_mod, dtype_name, tiles):
+    """The whole-step kernel with learned weights (three stages of a tile in
+    LDS) against the three stage kernels, over several tiles and a ragged
+    last one."""
+    torch = torch_mod
+    from pyshocks_amd import _lib, make_uniform_cell_grid
+    from pyshocks_amd.reconstruction import LearnedWENOJS53
+    from pyshocks_amd.timestepping import fused_plan
+
+    dtype = getattr(torch, dtype_name)
+    n = 100_003
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, dtype=dtype)
+    rec = LearnedWENOJS53(eps=1e-6, weights=_weights(_params(), 20.0))
+    st = _stepper(torch, "rusanov", grid, rec)
+    u0 = torch.tensor(_u0(n), device="cuda").to(dtype)
+    out = []
+    for sched in (_lib.PSA_SCHEDULE_TILE, _lib.PSA_SCHEDULE_STEP):
+        plan = fused_plan(st, tiles=tiles)
+        plan.desc.schedule = sched
+        u = u0.clone()
+        k1, k2 = torch.empty_like(u), torch.empty_like(u)
+        ctrl = plan.new_control(u, 0.0, float(np.finfo(plan.np_dtype).max), None)
+        plan.steps(u, u, k1, k2, ctrl, 20)
+        c = plan.read_control(ctrl)
+        assert c.m == 20 and c.status == 0
+        out.append((u, c.t))
+    if dtype_name == "float64":
+        assert out[0][1] == out[1][1]
+        assert torch.equal(out[0][0], out[1][0])
+    else:
+        a, b = out[1][0].double(), out[0][0].double()
+        assert float((a - b).norm()) <= 5e-4 * float(b.norm())
+
+
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
 def test_batched_learned_equals_single_instance(torch_mod, dtype_name):
     torch = torch_mod
     from pyshocks_amd import make_uniform_cell_grid
