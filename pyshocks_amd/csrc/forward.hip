@@ -521,32 +521,12 @@ __device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
   for (int k = 0; k <= CPT; ++k) {
     const T urL = k == 0 ? sm.urL[tid] : ur[k - 1];
     const T ulR = k == CPT ? sm.ulF[tid + 1] : ul[k];
-    if constexpr (FLUX == PSA_FLUX_RUSANOV) {
-      F[k] = flux_rusanov_nu1(urL, ulR, w[k + 1], w[k + 2]);
-    } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
-      F[k] = flux_burgers_upwind(urL, ulR);
-    } else {
-      F[k] = flux_advection_upwind(urL, ulR, p.vconst, p.vconst);
-    }
+    F[k] = stage_flux<T, FLUX>(urL, ulR, w[k + 1], w[k + 2], p.vconst);
   }
+  const T lam = stage_lambda<T, FLUX>(p, dt);
 #pragma unroll
-  for (int r = 0; r < CPT; ++r) {
-    const T dF = F[r + 1] - F[r];
-    T L;
-    if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
-      L = (-p.vconst * dF) * p.inv_dx;
-    } else {
-      L = -dF * p.inv_dx;
-    }
-    const T uk = w[r + 2];
-    if constexpr (STAGE == 1) {
-      val[r] = uk + dt * L;
-    } else if constexpr (STAGE == 2) {
-      val[r] = T(3.0 / 4.0) * un[r] + T(1.0 / 4.0) * (uk + dt * L);
-    } else {
-      val[r] = T(1.0 / 3.0) * un[r] + T(2.0 / 3.0) * (uk + dt * L);
-    }
-  }
+  for (int r = 0; r < CPT; ++r)
+    val[r] = stage_value<T, STAGE>(w[r + 2], un[r], lam, F[r + 1] - F[r]);
 }
 
 // One SSP-RK3 step (timestepping.py:312-320) of u^n = p.uk into

@@ -324,44 +324,96 @@ __device__ __forceinline__ void learned_cells(const MlpSource &src, const T *w,
 }
 
 // Left and right states of CPT consecutive cells from w = the cells
-// -2 .. CPT + 1 around them (reconstruction.py:358-377 with WENOJS53; the
+// -2 .. CPT + 1 around them (reconstruction.py:358-377 with WENOJS53). The
 // mirrored left reconstruction uses the same three indicators with beta0 and
-// beta2 swapped, and the second differences are shared between neighbouring
-// cells). LEARNED: the weights of cells outside the shock regime come from
-// the network or the ideal weights (wave-collective, learned_cells).
+// beta2 swapped. Everything is built from the first differences of w, shared
+// between neighbouring cells: the second differences of the indicators
+// (t00 = D(i) - D(i-1)), t01 = t00 + 2 D(i), t11 = D(i+1) + D(i),
+// t21 = t20 - 2 D(i+1) (weno.py:219-226 rearranged), and the candidates
+// (weno_uhat_diff). LEARNED: the weights of cells outside the shock regime
+// come from the network or the ideal weights (wave-collective, learned_cells).
 template <typename T, int CPT, bool LEARNED>
 __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
                                                   const T *w, T *ul, T *ur,
                                                   const MlpSource &src,
                                                   unsigned int *q) {
   using C = Weno5Coeffs<T>;
-  // a0 * (second difference)^2 at cells -1 .. CPT, shared by the three
-  // indicators of neighbouring cells
-  T E[CPT + 2];
+  // D[k] = w[k + 1] - w[k]: D[k] is u(c) - u(c - 1) for cell c = k - 1
+  T D[CPT + 3];
+#pragma unroll
+  for (int k = 0; k < CPT + 3; ++k) D[k] = w[k + 1] - w[k];
+  // second differences at cells -1 .. CPT and a0 times their squares
+  T d2[CPT + 2], E[CPT + 2];
 #pragma unroll
   for (int k = 0; k < CPT + 2; ++k) {
-    const T d2 = (w[k + 2] - T(2) * w[k + 1]) + w[k];
-    E[k] = C::a0 * (d2 * d2);
+    d2[k] = D[k + 1] - D[k];
+    E[k] = C::a0 * (d2[k] * d2[k]);
   }
 
   int regime[CPT];
 #pragma unroll
   for (int r = 0; r < CPT; ++r) {
-    const T um2 = w[r], um1 = w[r + 1], u0 = w[r + 2], up1 = w[r + 3],
-            up2 = w[r + 4];
-    const T t01 = (T(3) * u0 - T(4) * um1) + um2;
-    const T t11 = up1 - um1;
-    const T t21 = (up2 - T(4) * up1) + T(3) * u0;
+    // cell r: D(r-1) = D[r], D(r) = D[r+1], D(r+1) = D[r+2], D(r+2) = D[r+3]
+    const T t01 = fma(T(2), D[r + 1], d2[r]);
+    const T t11 = D[r + 2] + D[r + 1];
+    const T t21 = fma(T(-2), D[r + 2], d2[r + 2]);
     const T b0 = E[r] + C::a1 * (t01 * t01);
     const T b1 = E[r + 1] + C::a1 * (t11 * t11);
     const T b2 = E[r + 2] + C::a1 * (t21 * t21);
     // (unconditionally, also in the learned kernels: computed in the same
     // context, the shock regime reproduces the WENO-JS kernels bit for bit)
-    weno_pair_from_beta(um2, um1, u0, up1, up2, b0, b1, b2, p.eps, ul[r],
-                        ur[r]);
+    T hr0, hr1, hr2, hl0, hl1, hl2;
+    weno_uhat_diff(w[r + 2], D[r], D[r + 1], D[r + 2], D[r + 3], hr0, hr1, hr2,
+                   hl0, hl1, hl2);
+    weno_combine_pair(hr0, hr1, hr2, hl0, hl1, hl2, b0, b1, b2, p.eps, ul[r],
+                      ur[r]);
     if constexpr (LEARNED) regime[r] = mlp_regime(b0, b1, b2, p.eps, p.tau, p.sigma);
   }
   if constexpr (LEARNED) learned_cells<T, CPT>(src, w, regime, ul, ur, q);
+}
+
+// Face flux of the stage kernels for Rusanov with nu = 1 (doubled, see
+// stage_lambda), Godunov, or advection with a constant velocity.
+template <typename T, int FLUX>
+__device__ __forceinline__ T stage_flux(T urL, T ulR, T uL, T uR, T vconst) {
+  if constexpr (FLUX == PSA_FLUX_RUSANOV) {
+    return flux_rusanov_nu1_x2(urL, ulR, uL, uR);
+  } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
+    return flux_burgers_upwind(urL, ulR);
+  } else {
+    // a constant velocity reconstructs to itself up to rounding, so the
+    // upwind switch only sees its sign
+    return flux_advection_upwind(urL, ulR, vconst, vconst);
+  }
+}
+
+// u + dt L(u) = u - lam (F(i+1/2) - F(i-1/2)) with lam = dt / dx, times the
+// constant velocity for advection (L = -a dF / dx), times 1/2 for the
+// doubled Rusanov flux of stage_flux.
+template <typename T, int FLUX>
+__device__ __forceinline__ T stage_lambda(const StageArgs<T> &p, T dt) {
+  const T lam = dt * p.inv_dx;
+  if constexpr (FLUX == PSA_FLUX_RUSANOV) {
+    return T(0.5) * lam;
+  } else if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
+    return lam * p.vconst;
+  } else {
+    return lam;
+  }
+}
+
+// The SSP-RK3 stage combinations of timestepping.py:312-320 from
+// y = uk + dt L(uk) = uk - lam dF and un = u^n.
+template <typename T, int STAGE>
+__device__ __forceinline__ T stage_value(T uk, T un, T lam, T dF) {
+  const T y = fma(-lam, dF, uk);
+  if constexpr (STAGE == 1) {
+    return y;
+  } else if constexpr (STAGE == 2) {
+    return T(3.0 / 4.0) * un + T(1.0 / 4.0) * y;
+  } else {
+    return T(1.0 / 3.0) * un + T(2.0 / 3.0) * y;
+  }
 }
 
 // LDS of one stage tile.
@@ -528,6 +580,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   // ---- faces l0 - 1/2 .. l0 + CPT - 1/2
   T F[CPT + 1];
   const bool has_nu = FLUX == PSA_FLUX_RUSANOV && p.nu != nullptr;
+  // the stage kernels' flux forms (stage_flux) where they apply
+  const bool plain = STAGE > 0 && VEL != 2 && !has_nu;
 #pragma unroll
   for (int k = 0; k <= CPT; ++k) {
     T urL = k == 0 ? sm.urL[tid] : ur[k - 1];
@@ -537,7 +591,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
       if (k == CPT && tid == BLOCK - 1 && !sm.edge_js[1]) ulR = sm.edge_val[1];
     }
     T Fk;
-    if constexpr (FLUX == PSA_FLUX_RUSANOV) {
+    if (plain) {
+      Fk = stage_flux<T, FLUX>(urL, ulR, w[k + 1], w[k + 2], p.vconst);
+    } else if constexpr (FLUX == PSA_FLUX_RUSANOV) {
       if (has_nu) {
         const int64_t gm = S + l0 + k;  // face between cells gm - 1 and gm
         const T nuk = (gm >= 1 && gm <= m - 1) ? p.nu[gm - 1] : T(1);
@@ -553,8 +609,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
         arL = k == 0 ? sm.arL[tid] : ar[k - 1];
         alR = k == CPT ? sm.alF[tid + 1] : al[k];
       } else {
-        // a constant velocity reconstructs to itself up to rounding, so the
-        // upwind switch only sees its sign
         arL = p.vconst;
         alR = p.vconst;
       }
@@ -568,34 +622,25 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   }
 
   // ---- conservative difference and stage combination
+  T lam = T(0);
+  if constexpr (STAGE > 0) lam = plain ? stage_lambda<T, FLUX>(p, dt) : dt * p.inv_dx;
 #pragma unroll
   for (int r = 0; r < CPT; ++r) {
     const T dF = F[r + 1] - F[r];
-    T L;
-    if constexpr (STAGE == 0) {
-      if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
-        L = (-velocity_at<T, VEL>(sm.v, l0 + HALO + r, p.vconst) * dF) / p.dx;
-      } else {
-        L = -dF / p.dx;
-      }
-    } else {
-      if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
-        L = (-velocity_at<T, VEL>(sm.v, l0 + HALO + r, p.vconst) * dF) *
-            p.inv_dx;
-      } else {
-        L = -dF * p.inv_dx;
-      }
-    }
     const T uk = w[r + 2];
     T val;
     if constexpr (STAGE == 0) {
-      val = L;
-    } else if constexpr (STAGE == 1) {
-      val = uk + dt * L;
-    } else if constexpr (STAGE == 2) {
-      val = T(3.0 / 4.0) * sm.io[l0 + r] + T(1.0 / 4.0) * (uk + dt * L);
+      if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
+        val = (-velocity_at<T, VEL>(sm.v, l0 + HALO + r, p.vconst) * dF) / p.dx;
+      } else {
+        val = -dF / p.dx;
+      }
     } else {
-      val = T(1.0 / 3.0) * sm.io[l0 + r] + T(2.0 / 3.0) * (uk + dt * L);
+      T lr = lam;
+      if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND && VEL == 2)
+        lr = lam * velocity_at<T, VEL>(sm.v, l0 + HALO + r, p.vconst);
+      const T un = STAGE >= 2 ? sm.io[l0 + r] : T(0);
+      val = stage_value<T, STAGE == 0 ? 1 : STAGE>(uk, un, lr, dF);
     }
     sm.io[l0 + r] = val;
   }

@@ -101,22 +101,23 @@ __device__ __forceinline__ T weno_side_ref(T um2, T um1, T u0, T up1, T up2,
   return ((a0 / s) * h0 + (a1 / s) * h1) + (a2 / s) * h2;
 }
 
-// Right and left reconstructions of one cell from shared indicators.
-// b0, b1, b2 are the right-side indicators of the cell.
+// Right and left reconstructions of one cell from shared indicators
+// (weno_pair_from_beta below; b0, b1, b2 are the right-side indicators).
+
+// Both sides' weights and values from the right-side indicators and the six
+// candidate values (right hr*, left hl*). FP64: the product form with one
+// reciprocal (see the top of this file), guarded by the class of the shared
+// denominator (a positive normal number).
 template <typename T>
-__device__ __forceinline__ void weno_pair_from_beta(T um2, T um1, T u0, T up1,
-                                                    T up2, T b0, T b1, T b2,
-                                                    T eps, T &ul, T &ur);
+__device__ __forceinline__ void weno_combine_pair(T hr0, T hr1, T hr2, T hl0,
+                                                  T hl1, T hl2, T b0, T b1,
+                                                  T b2, T eps, T &ul, T &ur);
 
 template <>
-__device__ __forceinline__ void weno_pair_from_beta<double>(
-    double um2, double um1, double u0, double up1, double up2, double b0,
-    double b1, double b2, double eps, double &ul, double &ur) {
+__device__ __forceinline__ void weno_combine_pair<double>(
+    double hr0, double hr1, double hr2, double hl0, double hl1, double hl2,
+    double b0, double b1, double b2, double eps, double &ul, double &ur) {
   using C = Weno5Coeffs<double>;
-  double hr0, hr1, hr2, hl0, hl1, hl2;
-  weno_uhat(um2, um1, u0, up1, up2, hr0, hr1, hr2);
-  weno_uhat(up2, up1, u0, um1, um2, hl0, hl1, hl2);
-
   const double e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
   const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
   const double p01 = q0 * q1, p02 = q0 * q2, p12 = q1 * q2;
@@ -130,7 +131,7 @@ __device__ __forceinline__ void weno_pair_from_beta<double>(
   const double nl = (wl0 * hl0 + wr1 * hl1) + wl2 * hl2;
 
   const double den = sr * sl;
-  if (den >= Limits<double>::min_normal && den <= Limits<double>::max_finite) {
+  if (__builtin_amdgcn_class(den, 0x100)) {  // +normal: within [DBL_MIN, DBL_MAX]
     // hardware reciprocal refined by two Newton steps (each squares the
     // relative error): within an ulp or two of 1 / den
     double inv = __builtin_amdgcn_rcp(den);
@@ -152,14 +153,10 @@ __device__ __forceinline__ void weno_pair_from_beta<double>(
 }
 
 template <>
-__device__ __forceinline__ void weno_pair_from_beta<float>(
-    float um2, float um1, float u0, float up1, float up2, float b0, float b1,
-    float b2, float eps, float &ul, float &ur) {
+__device__ __forceinline__ void weno_combine_pair<float>(
+    float hr0, float hr1, float hr2, float hl0, float hl1, float hl2, float b0,
+    float b1, float b2, float eps, float &ul, float &ur) {
   using C = Weno5Coeffs<float>;
-  float hr0, hr1, hr2, hl0, hl1, hl2;
-  weno_uhat(um2, um1, u0, up1, up2, hr0, hr1, hr2);
-  weno_uhat(up2, up1, u0, um1, um2, hl0, hl1, hl2);
-
   const float e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
   const float r0 = __builtin_amdgcn_rcpf(e0 * e0);
   const float r1 = __builtin_amdgcn_rcpf(e1 * e1);
@@ -170,6 +167,33 @@ __device__ __forceinline__ void weno_pair_from_beta<float>(
   const float sl = (al0 + a1) + al2;
   ur = ((ar0 * hr0 + a1 * hr1) + ar2 * hr2) * __builtin_amdgcn_rcpf(sr);
   ul = ((al0 * hl0 + a1 * hl1) + al2 * hl2) * __builtin_amdgcn_rcpf(sl);
+}
+
+template <typename T>
+__device__ __forceinline__ void weno_pair_from_beta(T um2, T um1, T u0, T up1,
+                                                    T up2, T b0, T b1, T b2,
+                                                    T eps, T &ul, T &ur) {
+  T hr0, hr1, hr2, hl0, hl1, hl2;
+  weno_uhat(um2, um1, u0, up1, up2, hr0, hr1, hr2);
+  weno_uhat(up2, up1, u0, um1, um2, hl0, hl1, hl2);
+  weno_combine_pair(hr0, hr1, hr2, hl0, hl1, hl2, b0, b1, b2, eps, ul, ur);
+}
+
+// The six candidate values of a cell (weno.py:143-157, right and mirrored
+// left) from u0 and the first differences dm1 = u[i-1] - u[i-2],
+// d0 = u[i] - u[i-1], dp1 = u[i+1] - u[i], dp2 = u[i+2] - u[i+1]: the same
+// quadratic interpolants written as u0 plus differences, two multiply-adds
+// each (hr0 = u0 + 5/6 d0 - 1/3 dm1 = 11/6 u0 - 7/6 u[i-1] + 1/3 u[i-2], ...).
+template <typename T>
+__device__ __forceinline__ void weno_uhat_diff(T u0, T dm1, T d0, T dp1, T dp2,
+                                               T &hr0, T &hr1, T &hr2, T &hl0,
+                                               T &hl1, T &hl2) {
+  hr0 = fma(T(-1.0 / 3.0), dm1, fma(T(5.0 / 6.0), d0, u0));
+  hr1 = fma(T(1.0 / 6.0), d0, fma(T(1.0 / 3.0), dp1, u0));
+  hr2 = fma(T(-1.0 / 6.0), dp2, fma(T(2.0 / 3.0), dp1, u0));
+  hl0 = fma(T(1.0 / 3.0), dp2, fma(T(-5.0 / 6.0), dp1, u0));
+  hl1 = fma(T(-1.0 / 6.0), dp1, fma(T(-1.0 / 3.0), d0, u0));
+  hl2 = fma(T(1.0 / 6.0), dm1, fma(T(-2.0 / 3.0), d0, u0));
 }
 
 template <typename T>
@@ -211,6 +235,14 @@ template <typename T>
 __device__ __forceinline__ T flux_rusanov_nu1(T urL, T ulR, T uL, T uR) {
   const T a = fmax(fabs(uR), fabs(uL));
   return T(0.25) * fma(ulR, ulR, urL * urL) - (T(0.5) * a) * (ulR - urL);
+}
+
+// Twice flux_rusanov_nu1, 0.5 (ul^2 + ur^2) - a (ul - ur): the stage kernels
+// fold the factor 1/2 into the update.
+template <typename T>
+__device__ __forceinline__ T flux_rusanov_nu1_x2(T urL, T ulR, T uL, T uR) {
+  const T a = fmax(fabs(uR), fabs(uL));
+  return T(0.5) * fma(ulR, ulR, urL * urL) - a * (ulR - urL);
 }
 
 // scalar.py:125-130: aavg = (ar + al) / 2 with a = u reconstructed the same
