@@ -291,6 +291,13 @@ typedef struct psa_batch_desc {
   long long nslots;
   void *team_state;
   double team_wait_s;
+  /* PSA_SCHEDULE_STEP (or DEFAULT): one pass per step over an instance's
+   * tiles with the three stages of a tile in LDS (step_tile of the
+   * whole-step kernel; one barrier per step, the state ping-pongs between
+   * u and k1); PSA_SCHEDULE_TILE: one pass and one barrier per stage. Both
+   * compute the same values. */
+  int schedule;
+  int pad_;
 } psa_batch_desc;
 
 #define PSA_TEAM_STATE_BYTES 64

@@ -115,6 +115,8 @@ class PsaBatchDesc(ctypes.Structure):
         ("nslots", ctypes.c_longlong),
         ("team_state", ctypes.c_void_p),
         ("team_wait_s", ctypes.c_double),
+        ("schedule", ctypes.c_int),
+        ("pad_", ctypes.c_int),
     ]
 
 

@@ -46,12 +46,14 @@ class BatchedSolver:
                  tiles: tuple[int, int] | None = None, grid: int = 0,
                  mlp: Any = None, time_f64: bool = True,
                  team_cells: int | None = None, max_team: int = 32,
-                 team_wait_s: float = 60.0) -> None:
+                 team_wait_s: float = 60.0, schedule: str = "step") -> None:
         if scheme not in ("rusanov", "godunov"):
             raise NotImplementedError(
                 f"the batched solver takes Burgers 'rusanov' or 'godunov', not {scheme!r}")
         if nghosts < 3:
             raise ValueError("WENO-JS5 needs nghosts >= 3")
+        if schedule not in ("step", "tile"):
+            raise ValueError(f"schedule must be 'step' or 'tile', not {schedule!r}")
         ncells = np.asarray(ncells, dtype=np.int64)
         if ncells.ndim != 1 or ncells.size == 0 or np.any(ncells < nghosts):
             raise ValueError("every instance needs at least nghosts interior cells")
@@ -103,6 +105,10 @@ class BatchedSolver:
         # FP64 time keeps FP32 instances on schedule over ~10^5 steps; False
         # accumulates t in the state dtype as the reference's step loop does
         self.desc.time_f64 = 1 if time_f64 else 0
+        # one sweep per step with the three stages of a tile in LDS, or one
+        # sweep per stage (the same values)
+        self.desc.schedule = _lib.PSA_SCHEDULE_STEP if schedule == "step" \
+            else _lib.PSA_SCHEDULE_TILE
         self.mlp = mlp
         if mlp is not None:
             # learned weights (pyshocks_amd.learned.MLPWeights)

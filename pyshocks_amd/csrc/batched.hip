@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "../../include/pyshocks_amd.h"
 #include "psa_internal.hpp"
@@ -40,8 +41,11 @@ struct TeamState {
   unsigned int arrive;  // barrier arrivals, monotone
   int abort;
   unsigned int pad0[2];
-  unsigned long long umax[2];  // max|u| bits, alternating between steps
-  unsigned long long pad1[4];
+  // max|u| bits: stage sweeps alternate between the first two (cleared
+  // after the first barrier of a step); whole-step sweeps rotate over all
+  // three (one barrier per step)
+  unsigned long long umax[3];
+  unsigned long long pad1[3];
 };
 static_assert(sizeof(TeamState) == PSA_TEAM_STATE_BYTES, "team state size");
 
@@ -124,10 +128,15 @@ __device__ __forceinline__ bool team_barrier(TeamState *ts, int size,
   return *s_flag != 0;
 }
 
-template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED>
+// WHOLE: one sweep per step with the three stages of each tile in LDS
+// (step_tile), the state alternating between u and k1; otherwise one sweep
+// per stage (stage_tile).
+template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED, bool WHOLE>
 __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
-  constexpr int TILE = BLOCK * CPT;
-  __shared__ StageSmem<T, 0, BLOCK, CPT> sm;
+  using Smem = typename std::conditional<WHOLE, StepSmem<T, BLOCK, CPT>,
+                                         StageSmem<T, 0, BLOCK, CPT>>::type;
+  constexpr int TILE = WHOLE ? StepSmem<T, BLOCK, CPT>::TO : BLOCK * CPT;
+  __shared__ Smem sm;
   __shared__ unsigned long long s_bcast;
   __shared__ long long s_next;
   __shared__ int s_flag;
@@ -212,7 +221,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
         vb = b > vb ? b : vb;
       }
     }
-    unsigned long long umax = team_max(vb, 1);
+    unsigned long long umax = team_max(vb, WHOLE ? 2 : 1);
 
     // time control: in the state dtype (timestepping.py:136-152), or with
     // time_f64 in FP64 (t and the unclipped dt; the stages use dt rounded to
@@ -260,43 +269,68 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
         }
       }
 
-      unsigned long long vdummy = 0;
-      p.un = u;
-      p.uk = u;
-      p.out = k1;
-      for (int64_t tile = tlo; tile < thi; ++tile) {
-        __syncthreads();
-        stage_tile<T, FLUX, 0, 1, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
-                                                          vdummy, q, wlp);
+      if constexpr (WHOLE) {
+        // every member has read the max|u| of two steps ago (before the
+        // barrier of the previous step): clear it for the next step
+        if (ts != nullptr && rank == 0 && tid == 0)
+          __hip_atomic_store(&ts->umax[(steps + 1) % 3], 0ull, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        const T *src = (steps & 1) ? k1 : u;
+        T *dst = (steps & 1) ? u : k1;
+        unsigned long long vmax = 0;
+        for (int64_t tile = tlo; tile < thi; ++tile) {
+          __syncthreads();
+          step_tile<T, FLUX, 0, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, src, dst, sm,
+                                                     vmax, q, wlp);
+        }
+        umax = team_max(vmax, static_cast<int>(steps % 3));  // u^(n+1) complete
+      } else {
+        unsigned long long vdummy = 0;
+        p.un = u;
+        p.uk = u;
+        p.out = k1;
+        for (int64_t tile = tlo; tile < thi; ++tile) {
+          __syncthreads();
+          stage_tile<T, FLUX, 0, 1, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
+                                                            vdummy, q, wlp);
+        }
+        barrier();  // k1 complete
+        // every member has read the max|u| reduced two steps ago: clear that
+        // buffer for the next step
+        if (ts != nullptr && rank == 0 && tid == 0)
+          __hip_atomic_store(&ts->umax[(steps + 1) & 1], 0ull, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        p.uk = k1;
+        p.out = k2;
+        for (int64_t tile = tlo; tile < thi; ++tile) {
+          __syncthreads();
+          stage_tile<T, FLUX, 0, 2, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
+                                                            vdummy, q, wlp);
+        }
+        barrier();  // k2 complete
+        p.uk = k2;
+        p.out = u;
+        unsigned long long vmax = 0;
+        for (int64_t tile = tlo; tile < thi; ++tile) {
+          __syncthreads();
+          stage_tile<T, FLUX, 0, 3, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
+                                                         vmax, q, wlp);
+        }
+        umax = team_max(vmax, static_cast<int>(steps & 1));  // u complete
       }
-      barrier();  // k1 complete
-      // every member has read the max|u| reduced two steps ago: clear that
-      // buffer for the next step
-      if (ts != nullptr && rank == 0 && tid == 0)
-        __hip_atomic_store(&ts->umax[(steps + 1) & 1], 0ull, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      p.uk = k1;
-      p.out = k2;
-      for (int64_t tile = tlo; tile < thi; ++tile) {
-        __syncthreads();
-        stage_tile<T, FLUX, 0, 2, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
-                                                          vdummy, q, wlp);
-      }
-      barrier();  // k2 complete
-      p.uk = k2;
-      p.out = u;
-      unsigned long long vmax = 0;
-      for (int64_t tile = tlo; tile < thi; ++tile) {
-        __syncthreads();
-        stage_tile<T, FLUX, 0, 3, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
-                                                       vmax, q, wlp);
-      }
-      umax = team_max(vmax, static_cast<int>(steps & 1));  // u complete
       t = t + dt;
       t64 = t64 + static_cast<double>(dt);
       steps += 1;
     }
     if (!alive) status = PSA_ERR_HIP;
+    if constexpr (WHOLE) {
+      if (steps & 1) {
+        // the last state is in k1: this member's cells back to u
+        int64_t lo = tlo * TILE, hi = thi * TILE;
+        hi = hi < p.m ? hi : p.m;
+        for (int64_t j = lo + tid; j < hi; j += BLOCK) u[j] = k1[j];
+      }
+    }
     if (tid == 0 && rank == 0) {
       a.t_out[i] = a.time_f64 ? t64 : static_cast<double>(t);
       a.steps_out[i] = steps;
@@ -316,6 +350,7 @@ __global__ void k_batch_reset(unsigned int *counter, TeamState *teams,
     teams[j].abort = 0;
     teams[j].umax[0] = 0ull;
     teams[j].umax[1] = 0ull;
+    teams[j].umax[2] = 0ull;
   }
 }
 
@@ -336,10 +371,10 @@ unsigned int resident(K kernel, int block) {
   return static_cast<unsigned int>(per_cu) * static_cast<unsigned int>(cus);
 }
 
-template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED = false>
+template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED, bool WHOLE>
 int launch_batch_cfg(const BatchArgs<T> &a, int grid, int max_team, long long nteams,
                      hipStream_t s) {
-  const unsigned int cap = resident(k_batch<T, FLUX, BLOCK, CPT, LEARNED>, BLOCK);
+  const unsigned int cap = resident(k_batch<T, FLUX, BLOCK, CPT, LEARNED, WHOLE>, BLOCK);
   if (cap == 0)
     return set_error(PSA_ERR_HIP, "cannot query the resident workgroup count");
   unsigned int blocks = grid > 0 ? static_cast<unsigned int>(grid) : cap;
@@ -362,40 +397,48 @@ int launch_batch_cfg(const BatchArgs<T> &a, int grid, int max_team, long long nt
   hipLaunchKernelGGL(k_batch_reset, dim3(rb), dim3(256), 0, s, a.counter, a.teams, nt);
   int st = check_launch("batch reset");
   if (st != PSA_OK) return st;
-  hipLaunchKernelGGL((k_batch<T, FLUX, BLOCK, CPT, LEARNED>), dim3(blocks),
+  hipLaunchKernelGGL((k_batch<T, FLUX, BLOCK, CPT, LEARNED, WHOLE>), dim3(blocks),
                      dim3(BLOCK), 0, s, a);
   return check_launch("batch solve");
 }
 
-template <typename T, int FLUX>
+template <typename T, int FLUX, bool WHOLE>
 int launch_batch_tiles(const BatchArgs<T> &a, int block, int cpt, int grid,
                        int max_team, long long nteams, hipStream_t s) {
   if (a.mlp != nullptr) {
     if (block == 256 && cpt == 4)
-      return launch_batch_cfg<T, FLUX, 256, 4, true>(a, grid, max_team, nteams, s);
+      return launch_batch_cfg<T, FLUX, 256, 4, true, WHOLE>(a, grid, max_team, nteams, s);
     if (block == 512 && cpt == 2)
-      return launch_batch_cfg<T, FLUX, 512, 2, true>(a, grid, max_team, nteams, s);
+      return launch_batch_cfg<T, FLUX, 512, 2, true, WHOLE>(a, grid, max_team, nteams, s);
     if (block == 1024 && cpt == 2)
-      return launch_batch_cfg<T, FLUX, 1024, 2, true>(a, grid, max_team, nteams, s);
+      return launch_batch_cfg<T, FLUX, 1024, 2, true, WHOLE>(a, grid, max_team, nteams, s);
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "unsupported batch tile configuration for learned "
                      "weights block=" + std::to_string(block) +
                          " cells_per_thread=" + std::to_string(cpt));
   }
   if (block == 256 && cpt == 4)
-    return launch_batch_cfg<T, FLUX, 256, 4>(a, grid, max_team, nteams, s);
+    return launch_batch_cfg<T, FLUX, 256, 4, false, WHOLE>(a, grid, max_team, nteams, s);
   if (block == 256 && cpt == 2)
-    return launch_batch_cfg<T, FLUX, 256, 2>(a, grid, max_team, nteams, s);
+    return launch_batch_cfg<T, FLUX, 256, 2, false, WHOLE>(a, grid, max_team, nteams, s);
   if (block == 512 && cpt == 2)
-    return launch_batch_cfg<T, FLUX, 512, 2>(a, grid, max_team, nteams, s);
+    return launch_batch_cfg<T, FLUX, 512, 2, false, WHOLE>(a, grid, max_team, nteams, s);
   if (block == 512 && cpt == 4)
-    return launch_batch_cfg<T, FLUX, 512, 4>(a, grid, max_team, nteams, s);
+    return launch_batch_cfg<T, FLUX, 512, 4, false, WHOLE>(a, grid, max_team, nteams, s);
   if (block == 1024 && cpt == 2)
-    return launch_batch_cfg<T, FLUX, 1024, 2>(a, grid, max_team, nteams, s);
+    return launch_batch_cfg<T, FLUX, 1024, 2, false, WHOLE>(a, grid, max_team, nteams, s);
   return set_error(PSA_ERR_INVALID_ARGUMENT,
                    "unsupported batch tile configuration block=" +
                        std::to_string(block) + " cells_per_thread=" +
                        std::to_string(cpt));
+}
+
+template <typename T, int FLUX>
+int launch_batch_flux(const BatchArgs<T> &a, bool whole, int block, int cpt,
+                      int grid, int max_team, long long nteams, hipStream_t s) {
+  if (whole)
+    return launch_batch_tiles<T, FLUX, true>(a, block, cpt, grid, max_team, nteams, s);
+  return launch_batch_tiles<T, FLUX, false>(a, block, cpt, grid, max_team, nteams, s);
 }
 
 template <typename T>
@@ -442,11 +485,12 @@ int launch_batch(const psa_batch_desc *d, void *u, void *k1, void *k2,
   const int block = d->block > 0 ? d->block : 256;
   const int cpt = d->cells_per_thread > 0 ? d->cells_per_thread : 4;
   const int max_team = teams ? d->max_team : 1;
+  const bool whole = d->schedule != PSA_SCHEDULE_TILE;
   if (d->flux == PSA_FLUX_RUSANOV)
-    return launch_batch_tiles<T, PSA_FLUX_RUSANOV>(a, block, cpt, d->grid, max_team,
-                                                   d->ninst, s);
-  return launch_batch_tiles<T, PSA_FLUX_BURGERS_UPWIND>(a, block, cpt, d->grid,
-                                                        max_team, d->ninst, s);
+    return launch_batch_flux<T, PSA_FLUX_RUSANOV>(a, whole, block, cpt, d->grid,
+                                                  max_team, d->ninst, s);
+  return launch_batch_flux<T, PSA_FLUX_BURGERS_UPWIND>(a, whole, block, cpt, d->grid,
+                                                       max_team, d->ninst, s);
 }
 
 }  // namespace
@@ -468,6 +512,10 @@ extern "C" int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1,
     return set_error(PSA_ERR_INVALID_ARGUMENT, "needs at least 3 ghost cells");
   if (desc->ninst < 0 || desc->max_steps < 0 || !(desc->eps >= 0.0))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "bad batch parameters");
+  if (desc->schedule != PSA_SCHEDULE_DEFAULT && desc->schedule != PSA_SCHEDULE_TILE &&
+      desc->schedule != PSA_SCHEDULE_STEP)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "the batched solver takes the step or tile schedule");
   if (desc->mlp != nullptr && (!(desc->mlp_tau >= 0.0) || !(desc->mlp_sigma >= 0.0)))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "mlp_tau and mlp_sigma must be >= 0");
   if (u == nullptr || k1 == nullptr || k2 == nullptr || counter == nullptr ||

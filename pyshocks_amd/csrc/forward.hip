@@ -413,17 +413,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage_pipe(StageArgs<T> p) {
 }
 
 // ---------------------------------------------------------------------------
-// whole-step kernel: the three SSP-RK3 stages of a tile in LDS
-//
-// A workgroup owns the output cells [A, A + TO) of the array, TO = C - 16
-// with C = BLOCK * CPT, and holds the cells base .. base + C + 8 of u^n,
-// base = A - 12. Thread t computes the cells j0 = 4 + CPT t .. j0 + CPT - 1
-// (relative to base) in every stage, so each stage computes C cells: stage 1
-// is exact on [5, C + 3) (the two outermost faces have no neighbouring
-// thread), stage 2 on [8, C), stage 3 on [11, C - 3), which contains the
-// output cells [12, C - 4). The halo cells are recomputed by the neighbouring
-// tiles (16 of every C cells). Every cell is computed by the arithmetic of
-// the stage kernels, so the results are the same bit for bit.
+// whole-step kernel (step_tile): the three SSP-RK3 stages of a tile in LDS
 
 // The decision for the next step from max|u| of the state it starts from
 // (stage 1 of timestepping.py:89-152, control_decide).
@@ -482,53 +472,6 @@ __global__ void k_step_finish(psa_control *ctrl) {
   }
 }
 
-template <typename T, int BLOCK, int CPT>
-struct StepSmem {
-  static constexpr int C = BLOCK * CPT;  // cells computed per stage
-  static constexpr int L = C + 8;        // cells held per buffer
-  __align__(16) T x0[L];                 // u^n, then k2
-  __align__(16) T x1[L];                 // k1
-  T urL[BLOCK + 1];
-  T ulF[BLOCK + 1];
-  unsigned long long red[BLOCK / 64];
-};
-
-// One stage of the tile from the LDS buffer X into val (the thread's CPT
-// cells): the reconstruction, the face fluxes and the stage combination of
-// stage_tile (Rusanov with nu = 1, Godunov, or advection with a constant
-// velocity). un holds the thread's cells of u^n.
-template <typename T, int FLUX, int VEL, int BLOCK, int CPT, int STAGE,
-          bool LEARNED>
-__device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
-                                           StepSmem<T, BLOCK, CPT> &sm, T dt,
-                                           const T *un, T *val,
-                                           const MlpSource &src,
-                                           unsigned int *q) {
-  const int tid = threadIdx.x;
-  const int j0 = 4 + CPT * tid;
-  T w[CPT + 4];  // cells j0 - 2 .. j0 + CPT + 1
-#pragma unroll
-  for (int k = 0; k < CPT + 4; ++k) w[k] = X[j0 - 2 + k];
-  T ul[CPT], ur[CPT];
-  reconstruct_cells<T, CPT, LEARNED>(p, w, ul, ur, src, q);
-
-  sm.urL[tid + 1] = ur[CPT - 1];
-  sm.ulF[tid] = ul[0];
-  __syncthreads();
-
-  T F[CPT + 1];
-#pragma unroll
-  for (int k = 0; k <= CPT; ++k) {
-    const T urL = k == 0 ? sm.urL[tid] : ur[k - 1];
-    const T ulR = k == CPT ? sm.ulF[tid + 1] : ul[k];
-    F[k] = stage_flux<T, FLUX>(urL, ulR, w[k + 1], w[k + 2], p.vconst);
-  }
-  const T lam = stage_lambda<T, FLUX>(p, dt);
-#pragma unroll
-  for (int r = 0; r < CPT; ++r)
-    val[r] = stage_value<T, STAGE>(w[r + 2], un[r], lam, F[r + 1] - F[r]);
-}
-
 // One SSP-RK3 step (timestepping.py:312-320) of u^n = p.uk into
 // u^(n+1) = p.out (distinct arrays) with the step control of the control
 // block: the decision made before the launch says whether the step is taken
@@ -538,112 +481,24 @@ template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED>
 __global__ __launch_bounds__(BLOCK) void k_step(StageArgs<T> p) {
   static_assert(VEL != 2, "no velocity array in the whole-step kernel");
   using SM = StepSmem<T, BLOCK, CPT>;
-  constexpr int C = SM::C;
-  constexpr int L = SM::L;
-  constexpr int TO = C - 16;
-  constexpr int VN = Vec16<T>::n;
-  static_assert(C % 16 == 0, "tile geometry");
   __shared__ SM sm;
   __shared__ unsigned int q[LEARNED ? (BLOCK / 64) * learned_queue_words<CPT>() : 1];
   const int tid = threadIdx.x;
   psa_control *ctrl = p.ctrl;
-  const int64_t m = p.m;
-  const int64_t n = p.n;
-  const int g = p.g;
-  const int64_t A = static_cast<int64_t>(blockIdx.x) * TO;
-  const int64_t base = A - 12;
-  const T *__restrict__ uin = p.uk;
-  T *__restrict__ uout = p.out;
+  const int64_t A = static_cast<int64_t>(blockIdx.x) * SM::TO;
 
   if (ctrl->active_next == 0) {
     // the step is not taken: the state moves to the output unchanged
-    for (int j = tid; j < TO; j += BLOCK) {
+    for (int j = tid; j < SM::TO; j += BLOCK) {
       const int64_t a = A + j;
-      if (a < m) uout[a] = uin[a];
+      if (a < p.m) p.out[a] = p.uk[a];
     }
     return;
   }
   const T dt = static_cast<T>(ctrl->dt_next);
-
-  // ---- u^n with a 12-cell margin: interior tiles by 16-byte loads, the
-  // others by periodic index (interior cell (a - g) mod n)
-  const bool fast = p.aligned != 0 && base >= g && base + L <= m - g;
-  if (fast) {
-    Chunks<T, BLOCK, L / VN> cu;
-    cu.load(uin + base, tid);
-    cu.store(sm.x0, tid);
-  } else {
-    for (int j = tid; j < L; j += BLOCK) {
-      int64_t i = (base + j - g) % n;
-      if (i < 0) i += n;
-      sm.x0[j] = uin[g + i];
-    }
-  }
-  // the k1 entries stage 2 reads outside the cells stage 1 writes, and the
-  // exchange entries without a neighbouring thread (their results are
-  // outside the exact ranges; finite values keep them on the fast paths)
-  if (tid < 8) sm.x1[tid < 4 ? tid : C + tid] = T(0);
-  if (tid == 0) {
-    sm.urL[0] = T(0);
-    sm.ulF[BLOCK] = T(0);
-  }
-  __syncthreads();
-
-  const int j0 = 4 + CPT * tid;
-  T un[CPT], val[CPT];
-#pragma unroll
-  for (int r = 0; r < CPT; ++r) un[r] = sm.x0[j0 + r];
-  const MlpSource src{p.mlp, nullptr};
-
-  step_stage<T, FLUX, VEL, BLOCK, CPT, 1, LEARNED>(p, sm.x0, sm, dt, un, val, src, q);
-#pragma unroll
-  for (int r = 0; r < CPT; ++r) sm.x1[j0 + r] = val[r];
-  __syncthreads();
-  // (every read of x0 ended before the exchange barrier of stage 1)
-  step_stage<T, FLUX, VEL, BLOCK, CPT, 2, LEARNED>(p, sm.x1, sm, dt, un, val, src, q);
-#pragma unroll
-  for (int r = 0; r < CPT; ++r) sm.x0[j0 + r] = val[r];
-  __syncthreads();
-  step_stage<T, FLUX, VEL, BLOCK, CPT, 3, LEARNED>(p, sm.x0, sm, dt, un, val, src, q);
-
-  // ---- u^(n+1): the output cells [12, C - 4), with their max|u| if interior
   unsigned long long vmax = 0;
-  const int64_t a0 = base + j0;
-  if (fast && j0 >= 12 && j0 + CPT <= C - 4) {
-    if constexpr (CPT % VN == 0) {
-#pragma unroll
-      for (int r = 0; r < CPT; r += VN) {
-        typename Vec16<T>::type v;
-        if constexpr (VN == 2) {
-          v = {val[r], val[r + 1]};
-        } else {
-          v = {val[r], val[r + 1], val[r + 2], val[r + 3]};
-        }
-        store16(uout + a0 + r, v);
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < CPT; ++r) uout[a0 + r] = val[r];
-    }
-#pragma unroll
-    for (int r = 0; r < CPT; ++r) {
-      const unsigned long long b = abs_bits(val[r]);
-      vmax = b > vmax ? b : vmax;
-    }
-  } else {
-#pragma unroll
-    for (int r = 0; r < CPT; ++r) {
-      const int j = j0 + r;
-      const int64_t a = a0 + r;
-      if (j >= 12 && j < C - 4 && a >= 0 && a < m) {
-        uout[a] = val[r];
-        if (a >= g && a < m - g) {
-          const unsigned long long b = abs_bits(val[r]);
-          vmax = b > vmax ? b : vmax;
-        }
-      }
-    }
-  }
+  step_tile<T, FLUX, VEL, BLOCK, CPT, LEARNED>(p, A, dt, p.uk, p.out, sm, vmax, q,
+                                               nullptr);
 
   // ---- partial max|u| of the new state
   if (ctrl->speed_from_state != 0) {
@@ -814,7 +669,7 @@ int launch_stage_tiles(const StageArgs<T> &a, TileConfig cfg,
 // with a constant velocity.
 template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED>
 int launch_step_cfg(const StageArgs<T> &args, hipStream_t stream) {
-  constexpr int TO = BLOCK * CPT - 16;
+  constexpr int TO = StepSmem<T, BLOCK, CPT>::TO;
   StageArgs<T> a = args;
   a.aligned = aligned16(a.uk) && aligned16(a.out);
   const unsigned int blocks = blocks_for(a.m, TO);

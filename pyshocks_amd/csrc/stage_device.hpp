@@ -685,4 +685,169 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   }
 }
 
+// ---------------------------------------------------------------------------
+// whole-step tile
+
+template <typename T, int BLOCK, int CPT>
+struct StepSmem {
+  static constexpr int C = BLOCK * CPT;  // cells computed per stage
+  static constexpr int L = C + 8;        // cells held per buffer
+  static constexpr int TO = C - 16;      // output cells per tile
+  __align__(16) T x0[L];                 // u^n, then k2
+  __align__(16) T x1[L];                 // k1
+  T urL[BLOCK + 1];
+  T ulF[BLOCK + 1];
+  unsigned long long red[BLOCK / 64];
+};
+
+// One stage of the tile from the LDS buffer X into val (the thread's CPT
+// cells): the reconstruction, the face fluxes and the stage combination of
+// stage_tile (Rusanov with nu = 1, Godunov, or advection with a constant
+// velocity). un holds the thread's cells of u^n.
+template <typename T, int FLUX, int VEL, int BLOCK, int CPT, int STAGE,
+          bool LEARNED>
+__device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
+                                           StepSmem<T, BLOCK, CPT> &sm, T dt,
+                                           const T *un, T *val,
+                                           const MlpSource &src,
+                                           unsigned int *q) {
+  const int tid = threadIdx.x;
+  const int j0 = 4 + CPT * tid;
+  T w[CPT + 4];  // cells j0 - 2 .. j0 + CPT + 1
+#pragma unroll
+  for (int k = 0; k < CPT + 4; ++k) w[k] = X[j0 - 2 + k];
+  T ul[CPT], ur[CPT];
+  reconstruct_cells<T, CPT, LEARNED>(p, w, ul, ur, src, q);
+
+  sm.urL[tid + 1] = ur[CPT - 1];
+  sm.ulF[tid] = ul[0];
+  __syncthreads();
+
+  T F[CPT + 1];
+#pragma unroll
+  for (int k = 0; k <= CPT; ++k) {
+    const T urL = k == 0 ? sm.urL[tid] : ur[k - 1];
+    const T ulR = k == CPT ? sm.ulF[tid + 1] : ul[k];
+    F[k] = stage_flux<T, FLUX>(urL, ulR, w[k + 1], w[k + 2], p.vconst);
+  }
+  const T lam = stage_lambda<T, FLUX>(p, dt);
+#pragma unroll
+  for (int r = 0; r < CPT; ++r)
+    val[r] = stage_value<T, STAGE>(w[r + 2], un[r], lam, F[r + 1] - F[r]);
+}
+
+
+// One whole SSP-RK3 step (timestepping.py:312-320) of the output tile
+// [A, A + TO) from uin = u^n into uout = u^(n+1) (distinct arrays of p.m
+// cells): the workgroup holds the cells base .. base + C + 8 of u^n,
+// base = A - 12 (16-byte loads for interior tiles, the periodic image
+// (a - g) mod n otherwise). Thread t computes the cells j0 = 4 + CPT t ..
+// j0 + CPT - 1 (relative to base) in every stage, so each stage computes C
+// cells: stage 1 is exact on [5, C + 3) (the two outermost faces have no
+// neighbouring thread), stage 2 on [8, C), stage 3 on [11, C - 3), which
+// contains the output cells [12, C - 4). The halo cells are recomputed by the
+// neighbouring tiles (16 of every C cells), with the arithmetic of the stage
+// kernels, so the results are the same bit for bit. Output cells in the ghost
+// layers are written as the periodic images they are. vmax accumulates the
+// bit pattern of max|u| of the interior cells written. Ends with the LDS in
+// use (a barrier before the next tile).
+template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED>
+__device__ __forceinline__ void step_tile(const StageArgs<T> &p, int64_t A, T dt,
+                                          const T *__restrict__ uin,
+                                          T *__restrict__ uout,
+                                          StepSmem<T, BLOCK, CPT> &sm,
+                                          unsigned long long &vmax,
+                                          unsigned int *q,
+                                          const MlpWaveLds *wl) {
+  static_assert(VEL != 2, "no velocity array in the whole-step tile");
+  using SM = StepSmem<T, BLOCK, CPT>;
+  constexpr int C = SM::C;
+  constexpr int L = SM::L;
+  constexpr int VN = Vec16<T>::n;
+  static_assert(C % 16 == 0, "tile geometry");
+  const int tid = threadIdx.x;
+  const int64_t m = p.m;
+  const int64_t n = p.n;
+  const int g = p.g;
+  const int64_t base = A - 12;
+
+  // ---- u^n with a 12-cell margin
+  const bool fast = p.aligned != 0 && base >= g && base + L <= m - g;
+  if (fast) {
+    Chunks<T, BLOCK, L / VN> cu;
+    cu.load(uin + base, tid);
+    cu.store(sm.x0, tid);
+  } else {
+    for (int j = tid; j < L; j += BLOCK) {
+      int64_t i = (base + j - g) % n;
+      if (i < 0) i += n;
+      sm.x0[j] = uin[g + i];
+    }
+  }
+  // the k1 entries stage 2 reads outside the cells stage 1 writes, and the
+  // exchange entries without a neighbouring thread (their results are
+  // outside the exact ranges; finite values keep them on the fast paths)
+  if (tid < 8) sm.x1[tid < 4 ? tid : C + tid] = T(0);
+  if (tid == 0) {
+    sm.urL[0] = T(0);
+    sm.ulF[BLOCK] = T(0);
+  }
+  __syncthreads();
+
+  const int j0 = 4 + CPT * tid;
+  T un[CPT], val[CPT];
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) un[r] = sm.x0[j0 + r];
+  const MlpSource src{p.mlp, wl};
+
+  step_stage<T, FLUX, VEL, BLOCK, CPT, 1, LEARNED>(p, sm.x0, sm, dt, un, val, src, q);
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) sm.x1[j0 + r] = val[r];
+  __syncthreads();
+  // (every read of x0 ended before the exchange barrier of stage 1)
+  step_stage<T, FLUX, VEL, BLOCK, CPT, 2, LEARNED>(p, sm.x1, sm, dt, un, val, src, q);
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) sm.x0[j0 + r] = val[r];
+  __syncthreads();
+  step_stage<T, FLUX, VEL, BLOCK, CPT, 3, LEARNED>(p, sm.x0, sm, dt, un, val, src, q);
+
+  // ---- u^(n+1): the output cells [12, C - 4), with their max|u| if interior
+  const int64_t a0 = base + j0;
+  if (fast && j0 >= 12 && j0 + CPT <= C - 4) {
+    if constexpr (CPT % VN == 0) {
+#pragma unroll
+      for (int r = 0; r < CPT; r += VN) {
+        typename Vec16<T>::type v;
+        if constexpr (VN == 2) {
+          v = {val[r], val[r + 1]};
+        } else {
+          v = {val[r], val[r + 1], val[r + 2], val[r + 3]};
+        }
+        store16(uout + a0 + r, v);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < CPT; ++r) uout[a0 + r] = val[r];
+    }
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+      const unsigned long long b = abs_bits(val[r]);
+      vmax = b > vmax ? b : vmax;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+      const int j = j0 + r;
+      const int64_t a = a0 + r;
+      if (j >= 12 && j < C - 4 && a >= 0 && a < m) {
+        uout[a] = val[r];
+        if (a >= g && a < m - g) {
+          const unsigned long long b = abs_bits(val[r]);
+          vmax = b > vmax ? b : vmax;
+        }
+      }
+    }
+  }
+}
+
 }  // namespace psa

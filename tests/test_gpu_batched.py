@@ -202,3 +202,33 @@ def test_time_f64_reaches_tfinal(torch_mod) -> None:
         assert np.all(r[tf64].status == 0)
     assert np.max(np.abs(r[True].t - 0.3)) < 1e-9
     assert np.max(np.abs(r[False].t - 0.3)) > np.max(np.abs(r[True].t - 0.3))
+
+
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+@pytest.mark.parametrize("scheme", ["rusanov", "godunov"])
+@pytest.mark.parametrize("team_cells", [None, 2048])
+def test_step_schedule_matches_stage_sweeps(torch_mod, dtype_name: str, scheme: str,
+                                            team_cells) -> None:
+    """One sweep per step (the three stages of a tile in LDS, the state
+    alternating between u and k1) computes what one sweep per stage
+    computes, bit for bit: team and single-workgroup instances, odd and even
+    step counts, instances smaller than a tile."""
+    torch = torch_mod
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    dtype = getattr(torch, dtype_name)
+    _, ics = random_instances(6, seed=21)
+    ns = np.array([3, 37, 1000, 9000, 20000, 700])
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    out = []
+    for sched in ("tile", "step"):
+        solver = BatchedSolver(ns, dtype=dtype, scheme=scheme, tiles=(256, 4),
+                               team_cells=team_cells, schedule=sched)
+        solver.set_states(states)
+        res = solver.solve(0.02)
+        assert np.all(res.status == 0)
+        out.append((solver.u.clone(), res))
+    assert len(set(int(k) % 2 for k in out[0][1].steps)) == 2  # odd and even counts
+    np.testing.assert_array_equal(out[0][1].steps, out[1][1].steps)
+    np.testing.assert_array_equal(out[0][1].t, out[1][1].t)
+    assert torch.equal(out[0][0], out[1][0])

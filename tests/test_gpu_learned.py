@@ -213,6 +213,33 @@ def test_batched_learned_equals_single_instance(torch_mod, dtype_name):
         assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+def test_batched_learned_step_schedule(torch_mod, dtype_name):
+    """The batched solver with learned weights: one sweep per step against
+    one sweep per stage."""
+    torch = torch_mod
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    dtype = getattr(torch, dtype_name)
+    w = _weights(_params(), 20.0)
+    _, ics = random_instances(4, seed=8)
+    ns = np.array([5, 300, 1500, 4100])
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    out = []
+    for sched in ("tile", "step"):
+        solver = BatchedSolver(ns, dtype=dtype, tiles=(256, 4), mlp=w, schedule=sched)
+        solver.set_states(states)
+        res = solver.solve(0.04)
+        assert np.all(res.status == 0)
+        out.append((solver.u.clone(), res))
+    np.testing.assert_array_equal(out[0][1].steps, out[1][1].steps)
+    if dtype_name == "float64":
+        assert torch.equal(out[0][0], out[1][0])
+    else:
+        a, b = out[1][0].double(), out[0][0].double()
+        assert float((a - b).norm()) <= 5e-4 * float(b.norm())
+
+
 def test_ideal_network_is_fifth_order_on_smooth_data(torch_mod):
     """All-zero parameters and no fallback give the linear 5th-order
     scheme: on smooth advection-free Burgers data before the shock its error
