@@ -389,12 +389,23 @@ template <typename T>
 __device__ __forceinline__ T learned_right16(const MlpSource &src, T um2, T um1,
                                              T u0, T up1, T up2, T eps, T tau, T sigma,
                                              bool &js) {
-  T b0, b1, b2;
-  weno_beta(um2, um1, u0, up1, up2, b0, b1, b2);
+  // the indicators and candidates of the fused kernels (reconstruct_cells),
+  // expression for expression, so that a cell gets the same regime and
+  // values here (at a tile edge, or from its mirrored stencil)
+  using C = Weno5Coeffs<T>;
+  const T D0 = um1 - um2, D1 = u0 - um1, D2 = up1 - u0, D3 = up2 - up1;
+  const T s0 = D1 - D0, s1 = D2 - D1, s2 = D3 - D2;
+  const T E0 = C::a0 * (s0 * s0), E1 = C::a0 * (s1 * s1), E2 = C::a0 * (s2 * s2);
+  const T t01 = fma(T(2), D1, s0);
+  const T t11 = D2 + D1;
+  const T t21 = fma(T(-2), D2, s2);
+  const T b0 = E0 + C::a1 * (t01 * t01);
+  const T b1 = E1 + C::a1 * (t11 * t11);
+  const T b2 = E2 + C::a1 * (t21 * t21);
   const int regime = mlp_regime(b0, b1, b2, eps, tau, sigma);
   js = regime == 0;
-  T h0, h1, h2;
-  weno_uhat(um2, um1, u0, up1, up2, h0, h1, h2);
+  T h0, h1, h2, g0, g1, g2;
+  weno_uhat_diff(u0, D0, D1, D2, D3, h0, h1, h2, g0, g1, g2);
   T value = mlp_combine_ideal(h0, h1, h2);
   const int lane = threadIdx.x & 63;
   if (__ballot(lane < 2 && regime == 2) != 0) {  // wave-uniform

@@ -232,8 +232,8 @@ constexpr int learned_queue_words() {
 // mirrored stencil) into the wave's LDS queue q; the wave evaluates the
 // queue 64 samples at a time (and only the chains that hold samples; the
 // network is fetched from src only if the queue is not empty), the weights
-// come back through the same slots. Shock cells keep the WENO-JS
-// values already in ul, ur; smooth cells take the ideal weights.
+// come back through the same slots. Shock and smooth cells keep the values
+// already in ul, ur (WENO-JS and ideal weights).
 template <typename T, int CPT>
 __device__ __forceinline__ void learned_cells(const MlpSource &src, const T *w,
                                               const int *regime, T *ul, T *ur,
@@ -299,27 +299,23 @@ __device__ __forceinline__ void learned_cells(const MlpSource &src, const T *w,
     }
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
   }
+  if (ntot == 0) return;
   int slot = first;
 #pragma unroll
   for (int r = 0; r < CPT; ++r) {
-    if (regime[r] == 0) continue;
+    if (regime[r] != 2) continue;
     T hr0, hr1, hr2, hl0, hl1, hl2;
-    weno_uhat(w[r], w[r + 1], w[r + 2], w[r + 3], w[r + 4], hr0, hr1, hr2);
-    weno_uhat(w[r + 4], w[r + 3], w[r + 2], w[r + 1], w[r], hl0, hl1, hl2);
-    if (regime[r] == 1) {
-      ur[r] = mlp_combine_ideal(hr0, hr1, hr2);
-      ul[r] = mlp_combine_ideal(hl0, hl1, hl2);
-    } else {
-      const float wr[MLP_OUT] = {__uint_as_float(qw[3 * slot + 0]),
-                                 __uint_as_float(qw[3 * slot + 1]),
-                                 __uint_as_float(qw[3 * slot + 2])};
-      const float wl[MLP_OUT] = {__uint_as_float(qw[3 * slot + 3]),
-                                 __uint_as_float(qw[3 * slot + 4]),
-                                 __uint_as_float(qw[3 * slot + 5])};
-      ur[r] = mlp_combine(wr, hr0, hr1, hr2);
-      ul[r] = mlp_combine(wl, hl0, hl1, hl2);
-      slot += 2;
-    }
+    weno_uhat_diff(w[r + 2], w[r + 1] - w[r], w[r + 2] - w[r + 1], w[r + 3] - w[r + 2],
+                   w[r + 4] - w[r + 3], hr0, hr1, hr2, hl0, hl1, hl2);
+    const float wr[MLP_OUT] = {__uint_as_float(qw[3 * slot + 0]),
+                               __uint_as_float(qw[3 * slot + 1]),
+                               __uint_as_float(qw[3 * slot + 2])};
+    const float wl[MLP_OUT] = {__uint_as_float(qw[3 * slot + 3]),
+                               __uint_as_float(qw[3 * slot + 4]),
+                               __uint_as_float(qw[3 * slot + 5])};
+    ur[r] = mlp_combine(wr, hr0, hr1, hr2);
+    ul[r] = mlp_combine(wl, hl0, hl1, hl2);
+    slot += 2;
   }
 }
 
@@ -360,14 +356,25 @@ __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
     const T b0 = E[r] + C::a1 * (t01 * t01);
     const T b1 = E[r + 1] + C::a1 * (t11 * t11);
     const T b2 = E[r + 2] + C::a1 * (t21 * t21);
-    // (unconditionally, also in the learned kernels: computed in the same
-    // context, the shock regime reproduces the WENO-JS kernels bit for bit)
     T hr0, hr1, hr2, hl0, hl1, hl2;
     weno_uhat_diff(w[r + 2], D[r], D[r + 1], D[r + 2], D[r + 3], hr0, hr1, hr2,
                    hl0, hl1, hl2);
-    weno_combine_pair(hr0, hr1, hr2, hl0, hl1, hl2, b0, b1, b2, p.eps, ul[r],
-                      ur[r]);
-    if constexpr (LEARNED) regime[r] = mlp_regime(b0, b1, b2, p.eps, p.tau, p.sigma);
+    if constexpr (LEARNED) {
+      // WENO-JS weights only for shock cells (the same arithmetic as the
+      // WENO-JS kernels, so that regime reproduces them bit for bit), the
+      // ideal weights for smooth ones; learned cells get theirs below
+      regime[r] = mlp_regime(b0, b1, b2, p.eps, p.tau, p.sigma);
+      if (regime[r] == 0) {
+        weno_combine_pair(hr0, hr1, hr2, hl0, hl1, hl2, b0, b1, b2, p.eps, ul[r],
+                          ur[r]);
+      } else {
+        ur[r] = mlp_combine_ideal(hr0, hr1, hr2);
+        ul[r] = mlp_combine_ideal(hl0, hl1, hl2);
+      }
+    } else {
+      weno_combine_pair(hr0, hr1, hr2, hl0, hl1, hl2, b0, b1, b2, p.eps, ul[r],
+                        ur[r]);
+    }
   }
   if constexpr (LEARNED) learned_cells<T, CPT>(src, w, regime, ul, ur, q);
 }
