@@ -4,15 +4,18 @@ Workload (the large single-instance configuration): periodic inviscid
 Burgers on [0, 1], N = 2^24 cells, FP64, WENO-JS5 (eps = 1e-6, p = 2) with
 the Rusanov flux, SSP-RK3 at Courant 0.5 (theta = 1), u0 = exact cell
 averages of a seeded sum of 8 sinusoids (wavenumbers 1..32) plus a step. A
-bench step is one SSP-RK3 time step over all N cells (three fused stage
-kernels with the device-side CFL time step); the workload's 100 steps are
+bench step is one SSP-RK3 time step over all N cells (with the device-side
+CFL time step); the workload's 100 steps are
 the default ``--steps``. ``value`` is cell-updates/s = N * steps / wall
 time with the state resident in HBM.
 
-Also reported: the roofline of the dominant stage kernel (algorithmic bytes
-per launch over its event-timed average duration, against the 8 TB/s HBM
-peak) and the CPU restatement of the same step loop on the host cores, on a
-bounded sample.
+A step runs as one whole-step kernel (the three stages of each tile in LDS,
+2 words of HBM traffic per cell-update) where it applies. Also reported:
+the roofline of the dominant kernel (the whole-step kernel is VALU-bound:
+executed flops per cell-update from the committed rocprofv3 counters over
+its event-timed duration, against the vector peak, with its HBM figures
+beside them; the stage kernels' HBM roofline), and the CPU restatement of
+the same step loop on the host cores, on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
@@ -32,6 +35,11 @@ if str(ROOT) not in sys.path:
 
 METRIC = "WENO5-RK3 cell-updates/s + adjoint grads/s, learned vs plain at equal L1 err"
 HBM_PEAK_GBS = 8000.0
+# MI355X published vector peaks (TFLOP/s): FP64 78.6, FP32 157.3
+VALU_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+# instruction issue: 256 CUs x 4 SIMDs, one wave64 VALU instruction per 4
+# cycles each, at the 2.4 GHz peak engine clock
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 
 # words moved per cell by each stage kernel: stage 1 reads u^n and writes
 # u^(1); stages 2 and 3 read u^n and u^(k) and write the next state
@@ -216,6 +224,69 @@ def gpu_leg(args: argparse.Namespace) -> dict:
         "stage_achieved_gbs": stage_gbs,
         "t_final": float(c.t),
         "theta": theta,
+    }
+
+
+def _profile_entry(name: str, key: str):
+    """An entry of a counter summary under profiles/ (rocprofv3 measurements
+    of the same workload), or None."""
+    prof = ROOT / "profiles" / name
+    try:
+        return json.loads(prof.read_text()).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def roofline_record(args: argparse.Namespace, gpu: dict, traffic, valu) -> dict:
+    """Roofline of the dominant kernel. The whole-step kernel moves 2 words
+    per cell-update and is bound by the vector ALU: its executed flops (and
+    VALU instructions) per cell-update come from the rocprofv3 counters in
+    profiles/valu.json, timed live with the event-timed kernel duration; the
+    HBM figures are reported beside them. The stage kernels are HBM-bound."""
+    hbm_gbs = gpu["achieved_gbs"]
+    hbm = {
+        "achieved": hbm_gbs,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": hbm_gbs / HBM_PEAK_GBS,
+        "algorithmic_bytes_per_launch": gpu["algo_bytes"],
+        "traffic": traffic,
+    }
+    stage = {
+        "stage_ms": gpu["stage_ms"],
+        "dominant": gpu["stage_dominant"],
+        "achieved": gpu["stage_achieved_gbs"],
+        "frac": gpu["stage_achieved_gbs"] / HBM_PEAK_GBS,
+        "unit": "GB/s",
+    }
+    if not gpu["whole_step"] or valu is None:
+        return {"bound": "hbm", **{k: hbm[k] for k in ("achieved", "peak", "unit", "frac",
+                                                        "traffic")},
+                "kernel": gpu["kernel"], "kernel_ms": gpu["kernel_ms"],
+                "algorithmic_bytes_per_launch": gpu["algo_bytes"], "stage_kernels": stage}
+    kernel_s = gpu["kernel_ms"] * 1.0e-3
+    flops = valu["flops_per_cell_update"] * args.n
+    tflops = flops / kernel_s / 1.0e12
+    peak = VALU_PEAK_TFLOPS[args.dtype]
+    issue = valu["valu_instr_per_cell_update"] * args.n / 64.0 / kernel_s
+    return {
+        "bound": "valu",
+        "achieved": tflops,
+        "peak": peak,
+        "unit": "TFLOP/s",
+        "frac": tflops / peak,
+        "traffic": traffic,
+        "kernel": gpu["kernel"],
+        "kernel_ms": gpu["kernel_ms"],
+        "flops_per_cell_update": valu["flops_per_cell_update"],
+        "valu_instr_per_cell_update": valu["valu_instr_per_cell_update"],
+        "valu_issue_frac": issue / VALU_PEAK_WAVE_INSTR_PER_S,
+        "valu_busy_profiled": valu["valu_busy"],
+        "hbm": hbm,
+        "stage_kernels": stage,
+        "note": "whole-step kernel: 2 words of HBM traffic per cell-update (8 for the "
+                "stage kernels); executed flops and VALU instructions per cell-update "
+                "from the counters in profiles/valu.json",
     }
 
 
@@ -487,16 +558,11 @@ def main(argv: list[str] | None = None) -> int:
         args, plain_single_value=gpu["value"] if args.dtype == "f64" else None)
     cpu = cpu_leg(args, gpu)
 
-    traffic = None
-    prof = ROOT / "profiles" / "traffic.json"
-    if prof.exists():
-        try:
-            data = json.loads(prof.read_text())
-            kname = "step" if gpu["whole_step"] else f"stage{gpu['stage_dominant']}"
-            key = f"{args.scheme}_{args.dtype}_{args.n}_{kname}"
-            traffic = data.get(key)
-        except (OSError, ValueError):
-            traffic = None
+    kname = "step" if gpu["whole_step"] else f"stage{gpu['stage_dominant']}"
+    key = f"{args.scheme}_{args.dtype}_{args.n}_{kname}"
+    traffic = _profile_entry("traffic.json", key)
+    valu = _profile_entry("valu.json", key)
+    roofline = roofline_record(args, gpu, traffic, valu)
 
     dtype_name = "f64" if args.dtype == "f64" else "f32"
     line = {
@@ -522,23 +588,7 @@ def main(argv: list[str] | None = None) -> int:
                      "each tile in LDS, device CFL dt)" if gpu["whole_step"] else
                      "one SSP-RK3 time step (3 fused stage kernels, device CFL dt)"),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": gpu["achieved_gbs"],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": gpu["achieved_gbs"] / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "kernel": gpu["kernel"],
-            "kernel_ms": gpu["kernel_ms"],
-            "algorithmic_bytes_per_launch": gpu["algo_bytes"],
-            "stage_kernels": {
-                "stage_ms": gpu["stage_ms"],
-                "dominant": gpu["stage_dominant"],
-                "achieved": gpu["stage_achieved_gbs"],
-                "frac": gpu["stage_achieved_gbs"] / HBM_PEAK_GBS,
-            },
-        },
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "learned": learned,
