@@ -74,8 +74,12 @@ class CheckpointedSolve:
     """A fixed-dt forward solve that keeps checkpoints for its adjoint."""
 
     def __init__(self, source: Any, u0: torch.Tensor, nsteps: int, dt: Any,
-                 every: int = 50) -> None:
+                 every: int = 50, schedule: int | None = None) -> None:
         self.desc = _plan_of(source)
+        if schedule is not None:
+            # PSA_SCHEDULE_STEP (default): whole-step kernels for the forward and
+            # recompute passes where they apply; PSA_SCHEDULE_TILE: stage kernels
+            self.desc.schedule = int(schedule)
         _lib.require_device(u0)
         self.nsteps = int(nsteps)
         self.every = int(every)

@@ -430,7 +430,22 @@ int psa_ssprk33_forward_checkpointed(const psa_solver_desc *desc,
   if (st != PSA_OK) return st;
   st = copy_async(cur, u0, bytes, s);
   if (st != PSA_OK) return st;
+  // whole steps, the state alternating between cur and the window's first
+  // array, where the whole-step kernel applies
+  char *alt = win;
   for (long long step = 0; step < nsteps; ++step) {
+    st = step_fixed_dt(desc, cur, alt, nullptr, nullptr, dt, s);
+    if (st == PSA_OK) {
+      char *tmp = cur;
+      cur = alt;
+      alt = tmp;
+      if ((step + 1) % every == 0) {
+        st = copy_async(cp((step + 1) / every), cur, bytes, s);
+        if (st != PSA_OK) return st;
+      }
+      continue;
+    }
+    if (st != PSA_ERR_NOT_IMPLEMENTED) return st;
     st = psa_ssprk33_stage(desc, 1, cur, cur, k1, nullptr, dt, stream);
     if (st != PSA_OK) return st;
     st = psa_ssprk33_stage(desc, 2, cur, k1, k2, nullptr, dt, stream);
@@ -483,6 +498,11 @@ int psa_ssprk33_adjoint(const psa_solver_desc *desc, long long nsteps,
     if (st != PSA_OK) return st;
     for (long long step = s0; step < s1; ++step) {
       const long long i = step - s0;
+      // one whole step writing k1, k2 and the next state (X(every) is the
+      // running-state array, free in this pass)
+      st = step_fixed_dt(desc, X(i), X(i + 1), K1(i), K2(i), dt, s);
+      if (st == PSA_OK) continue;
+      if (st != PSA_ERR_NOT_IMPLEMENTED) return st;
       st = psa_ssprk33_stage(desc, 1, X(i), X(i), K1(i), nullptr, dt, stream);
       if (st != PSA_OK) return st;
       st = psa_ssprk33_stage(desc, 2, X(i), K1(i), K2(i), nullptr, dt, stream);

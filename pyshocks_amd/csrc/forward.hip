@@ -477,6 +477,8 @@ __global__ void k_step_finish(psa_control *ctrl) {
 // block: the decision made before the launch says whether the step is taken
 // (if not, the state is copied to p.out) and with which dt; the workgroups
 // reduce max|u_int| of the new state into the step slots for k_step_finish.
+// Without a control block the step is taken with the device scalar p.dt_ptr
+// (and p.k1out, p.k2out may receive the stage states).
 template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED>
 __global__ __launch_bounds__(BLOCK) void k_step(StageArgs<T> p) {
   static_assert(VEL != 2, "no velocity array in the whole-step kernel");
@@ -487,6 +489,13 @@ __global__ __launch_bounds__(BLOCK) void k_step(StageArgs<T> p) {
   psa_control *ctrl = p.ctrl;
   const int64_t A = static_cast<int64_t>(blockIdx.x) * SM::TO;
 
+  if (ctrl == nullptr) {
+    // a fixed step of the device scalar dt (the adjoint's forward passes)
+    unsigned long long vmax = 0;
+    step_tile<T, FLUX, VEL, BLOCK, CPT, LEARNED>(p, A, *p.dt_ptr, p.uk, p.out, sm, vmax,
+                                                 q, nullptr);
+    return;
+  }
   if (ctrl->active_next == 0) {
     // the step is not taken: the state moves to the output unchanged
     for (int j = tid; j < SM::TO; j += BLOCK) {
@@ -692,9 +701,8 @@ int launch_step_tiles(const StageArgs<T> &a, TileConfig cfg,
 }
 
 // Whether the whole-step kernel runs the steps of this descriptor.
-inline bool step_applies(const psa_solver_desc *d, TileConfig cfg, int vel,
-                         const void *control) {
-  if (!cfg.step || control == nullptr || vel == 2 || d->nu != nullptr) return false;
+inline bool step_applies(const psa_solver_desc *d, TileConfig cfg, int vel) {
+  if (!cfg.step || vel == 2 || d->nu != nullptr) return false;
   if (d->flux == PSA_FLUX_ADVECTION_UPWIND && d->mlp != nullptr) return false;
   return cfg.block == 256 && (cfg.cpt == 4 || cfg.cpt == 2);
 }
@@ -840,7 +848,7 @@ int run_steps(const psa_solver_desc *d, const void *u_in, void *u_out,
   T *b1 = static_cast<T *>(k1);
   T *b2 = static_cast<T *>(k2);
   int st = PSA_OK;
-  if (step_applies(d, cfg, vel, a.ctrl)) {
+  if (a.ctrl != nullptr && step_applies(d, cfg, vel)) {
     // one launch per step; u^n and u^(n+1) are distinct arrays: the state
     // goes u_in -> k1 -> k2 -> k1 ... -> u_out (u_in -> k1 -> u_out, by a
     // copy, for one in-place step)
@@ -892,6 +900,35 @@ int run_steps(const psa_solver_desc *d, const void *u_in, void *u_out,
     }
   }
   return PSA_OK;
+}
+
+// One fixed-dt whole step u_in -> u_out (distinct arrays) with the device
+// scalar dt, writing the stage states into k1, k2 if they are not null, for
+// the adjoint's forward and recompute passes. Returns
+// PSA_ERR_NOT_IMPLEMENTED (and launches nothing) where the whole-step kernel
+// does not apply; the caller then runs the stage kernels.
+int step_fixed_dt(const psa_solver_desc *d, const void *u_in, void *u_out, void *k1,
+                  void *k2, const void *dt, hipStream_t stream) {
+  const TileConfig cfg = tiles_of(d);
+  const int vel = vel_mode(d);
+  if (d->mlp != nullptr || !step_applies(d, cfg, vel))
+    return PSA_ERR_NOT_IMPLEMENTED;
+  if (d->dtype == PSA_FLOAT64) {
+    StageArgs<double> a = make_args<double>(d);
+    a.uk = a.un = static_cast<const double *>(u_in);
+    a.out = static_cast<double *>(u_out);
+    a.k1out = static_cast<double *>(k1);
+    a.k2out = static_cast<double *>(k2);
+    a.dt_ptr = static_cast<const double *>(dt);
+    return launch_step<double>(d, vel, a, cfg, stream);
+  }
+  StageArgs<float> a = make_args<float>(d);
+  a.uk = a.un = static_cast<const float *>(u_in);
+  a.out = static_cast<float *>(u_out);
+  a.k1out = static_cast<float *>(k1);
+  a.k2out = static_cast<float *>(k2);
+  a.dt_ptr = static_cast<const float *>(dt);
+  return launch_step<float>(d, vel, a, cfg, stream);
 }
 
 }  // namespace psa

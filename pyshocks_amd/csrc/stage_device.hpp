@@ -98,6 +98,8 @@ struct StageArgs {
   const float *mlp;  // learned weights (LEARNED kernels)
   T tau;             // their shock threshold
   T sigma;           // their smooth threshold
+  T *k1out;          // whole-step kernel: the stage states k1, k2 as well,
+  T *k2out;          //   or null
 };
 
 template <typename T, int VEL>
@@ -744,6 +746,59 @@ __device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
 }
 
 
+// Stores a thread's CPT values of a whole-step tile (cells j0 = 4 + CPT t ..,
+// relative to base) that fall in the output cells [12, C - 4), and with VMAX
+// accumulates the bit pattern of max|u| of the interior ones.
+template <typename T, int BLOCK, int CPT, bool VMAX>
+__device__ __forceinline__ void step_store(const StageArgs<T> &p, int64_t base,
+                                           bool fast, const T *val,
+                                           T *__restrict__ dst,
+                                           unsigned long long &vmax) {
+  constexpr int C = BLOCK * CPT;
+  constexpr int VN = Vec16<T>::n;
+  const int j0 = 4 + CPT * static_cast<int>(threadIdx.x);
+  const int64_t a0 = base + j0;
+  if (fast && j0 >= 12 && j0 + CPT <= C - 4) {
+    if constexpr (CPT % VN == 0) {
+#pragma unroll
+      for (int r = 0; r < CPT; r += VN) {
+        typename Vec16<T>::type v;
+        if constexpr (VN == 2) {
+          v = {val[r], val[r + 1]};
+        } else {
+          v = {val[r], val[r + 1], val[r + 2], val[r + 3]};
+        }
+        store16(dst + a0 + r, v);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < CPT; ++r) dst[a0 + r] = val[r];
+    }
+    if constexpr (VMAX) {
+#pragma unroll
+      for (int r = 0; r < CPT; ++r) {
+        const unsigned long long b = abs_bits(val[r]);
+        vmax = b > vmax ? b : vmax;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+      const int j = j0 + r;
+      const int64_t a = a0 + r;
+      if (j >= 12 && j < C - 4 && a >= 0 && a < p.m) {
+        dst[a] = val[r];
+        if constexpr (VMAX) {
+          if (a >= p.g && a < p.m - p.g) {
+            const unsigned long long b = abs_bits(val[r]);
+            vmax = b > vmax ? b : vmax;
+          }
+        }
+      }
+    }
+  }
+}
+
 // One whole SSP-RK3 step (timestepping.py:312-320) of the output tile
 // [A, A + TO) from uin = u^n into uout = u^(n+1) (distinct arrays of p.m
 // cells): the workgroup holds the cells base .. base + C + 8 of u^n,
@@ -756,8 +811,10 @@ __device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
 // neighbouring tiles (16 of every C cells), with the arithmetic of the stage
 // kernels, so the results are the same bit for bit. Output cells in the ghost
 // layers are written as the periodic images they are. vmax accumulates the
-// bit pattern of max|u| of the interior cells written. Ends with the LDS in
-// use (a barrier before the next tile).
+// bit pattern of max|u| of the interior cells written. With p.k1out, p.k2out
+// the stage states k1 and k2 of the output cells are written as well (they
+// are exact there). Ends with the LDS in use (a barrier before the next
+// tile).
 template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED>
 __device__ __forceinline__ void step_tile(const StageArgs<T> &p, int64_t A, T dt,
                                           const T *__restrict__ uin,
@@ -807,54 +864,21 @@ __device__ __forceinline__ void step_tile(const StageArgs<T> &p, int64_t A, T dt
   for (int r = 0; r < CPT; ++r) un[r] = sm.x0[j0 + r];
   const MlpSource src{p.mlp, wl};
 
+  unsigned long long unused = 0;
   step_stage<T, FLUX, VEL, BLOCK, CPT, 1, LEARNED>(p, sm.x0, sm, dt, un, val, src, q);
 #pragma unroll
   for (int r = 0; r < CPT; ++r) sm.x1[j0 + r] = val[r];
+  if (p.k1out != nullptr) step_store<T, BLOCK, CPT, false>(p, base, fast, val, p.k1out, unused);
   __syncthreads();
   // (every read of x0 ended before the exchange barrier of stage 1)
   step_stage<T, FLUX, VEL, BLOCK, CPT, 2, LEARNED>(p, sm.x1, sm, dt, un, val, src, q);
 #pragma unroll
   for (int r = 0; r < CPT; ++r) sm.x0[j0 + r] = val[r];
+  if (p.k2out != nullptr) step_store<T, BLOCK, CPT, false>(p, base, fast, val, p.k2out, unused);
   __syncthreads();
   step_stage<T, FLUX, VEL, BLOCK, CPT, 3, LEARNED>(p, sm.x0, sm, dt, un, val, src, q);
 
-  // ---- u^(n+1): the output cells [12, C - 4), with their max|u| if interior
-  const int64_t a0 = base + j0;
-  if (fast && j0 >= 12 && j0 + CPT <= C - 4) {
-    if constexpr (CPT % VN == 0) {
-#pragma unroll
-      for (int r = 0; r < CPT; r += VN) {
-        typename Vec16<T>::type v;
-        if constexpr (VN == 2) {
-          v = {val[r], val[r + 1]};
-        } else {
-          v = {val[r], val[r + 1], val[r + 2], val[r + 3]};
-        }
-        store16(uout + a0 + r, v);
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < CPT; ++r) uout[a0 + r] = val[r];
-    }
-#pragma unroll
-    for (int r = 0; r < CPT; ++r) {
-      const unsigned long long b = abs_bits(val[r]);
-      vmax = b > vmax ? b : vmax;
-    }
-  } else {
-#pragma unroll
-    for (int r = 0; r < CPT; ++r) {
-      const int j = j0 + r;
-      const int64_t a = a0 + r;
-      if (j >= 12 && j < C - 4 && a >= 0 && a < m) {
-        uout[a] = val[r];
-        if (a >= g && a < m - g) {
-          const unsigned long long b = abs_bits(val[r]);
-          vmax = b > vmax ? b : vmax;
-        }
-      }
-    }
-  }
+  step_store<T, BLOCK, CPT, true>(p, base, fast, val, uout, vmax);
 }
 
 }  // namespace psa

@@ -232,3 +232,25 @@ def test_autograd_function(torch_mod) -> None:
     ref = checkpointed_solve(src, _dev(torch, u0), nsteps, dt, every=7).gradient(
         _dev(torch, lam))
     assert _rel(_host(u.grad)[3:-3], _host(ref)[3:-3]) < RTOL
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_checkpointed_adjoint_schedules_agree(torch_mod, kind: str) -> None:
+    """The forward and recompute passes as whole-step kernels (the stage
+    states written by the step) give the stage kernels' final state and
+    gradient bit for bit (several tiles, a partial last window)."""
+    torch = torch_mod
+    from pyshocks_amd import _lib
+    from pyshocks_amd.adjoint import CheckpointedSolve
+
+    n, nsteps, every = 5000, 23, 5
+    u0, lam = _data(n, 7)
+    dt = 0.3 / n
+    src, _ = _source(torch, kind, n)
+    out = []
+    for sched in (_lib.PSA_SCHEDULE_TILE, _lib.PSA_SCHEDULE_STEP):
+        solve = CheckpointedSolve(src, _dev(torch, u0), nsteps, dt, every=every,
+                                  schedule=sched)
+        out.append((solve.u_final.clone(), solve.gradient(_dev(torch, lam))))
+    assert torch.equal(out[0][0], out[1][0])
+    assert torch.equal(out[0][1], out[1][1])

@@ -503,3 +503,37 @@ def test_step_schedule_bit_for_bit(torch_mod, kind: str, dtype_name: str, tiles,
     assert a[2] == b[2] and a[3] == b[3]
     assert torch.equal(a[0], b[0])
     assert torch.equal(a[1], b[1])
+
+
+def test_schedules_mix_on_one_control_block(torch_mod) -> None:
+    """Calls of the whole-step and stage schedules alternate on one control
+    block (each leaves t, m, dt and the max|u| slots as the other expects),
+    through the end of the run."""
+    torch = torch_mod
+    from oracle import exact as E
+    from pyshocks_amd import _lib, make_uniform_cell_grid
+    from pyshocks_amd.timestepping import fused_plan
+
+    n = 100_003
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    st = _stepper(torch, "rusanov", grid)
+    u0 = torch.tensor(E.exact_cell_averages(E.large_initial_condition(seed=2), n),
+                      device="cuda")
+    plan = fused_plan(st, tiles=(256, 4))
+    seqs = {"tile": [_lib.PSA_SCHEDULE_TILE] * 4,
+            "mixed": [_lib.PSA_SCHEDULE_STEP, _lib.PSA_SCHEDULE_TILE,
+                      _lib.PSA_SCHEDULE_STEP, _lib.PSA_SCHEDULE_TILE]}
+    out = {}
+    for name, seq in seqs.items():
+        u = u0.clone()
+        k1, k2 = torch.empty_like(u), torch.empty_like(u)
+        # about 10 steps of dt ~ 0.5 h / max|u|, then steps that are not taken
+        ctrl = plan.new_control(u, 0.0, 10.5 * 0.5 / n / float(u0[3:-3].abs().max()), None)
+        for sched in seq:
+            plan.desc.schedule = sched
+            plan.steps(u, u, k1, k2, ctrl, 4)
+        c = plan.read_control(ctrl)
+        assert c.status == 0 and c.active == 0
+        out[name] = (u, c.t, c.m, c.dt)
+    assert out["tile"][1:] == out["mixed"][1:]
+    assert torch.equal(out["tile"][0], out["mixed"][0])
