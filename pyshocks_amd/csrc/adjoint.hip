@@ -111,47 +111,40 @@ struct VjpArgs {
   T vconst;
 };
 
-// RSTAGE 0: out = JL(state)^T mu; 3, 2, 1: the reverse stages above, with
-// mu = l, lk2, lk1 respectively.
-template <typename T, int FLUX, int RSTAGE>
-__global__ __launch_bounds__(kVjpBlock) void k_stage_vjp(VjpArgs<T> p) {
+// Scratch of one reverse stage over a block of B computing cells.
+template <typename T>
+struct VjpScratch {
+  T ur[kVjpBlock], ul[kVjpBlock];
+  T cR[kVjpBlock], cL[kVjpBlock], sL[kVjpBlock], sR[kVjpBlock];
+  T P[5][kVjpBlock];
+};
+
+// JL(state)^T mu for the output cells Z .. Z + B - 7 of one block: s_u holds
+// the state at cells Z - 5 .. Z + B - 2, s_mu the cotangent at Z - 3 ..
+// Z + B - 3 (both in LDS, visible to the block). Thread t computes the
+// reconstructions of cell Z - 3 + t and returns the gathered sum of output
+// cell Z + t (valid for t < B - 6). Ends with the scratch in use.
+template <typename T, int FLUX>
+__device__ __forceinline__ T vjp_block(const VjpArgs<T> &p, const T *s_u,
+                                       const T *s_mu, VjpScratch<T> &sc) {
   constexpr int B = kVjpBlock;
-  __shared__ T s_u[B + 4];   // cells S - 5 ..
-  __shared__ T s_mu[B + 1];  // cells S - 3 ..
-  __shared__ T s_ur[B], s_ul[B];
-  __shared__ T s_cR[B], s_cL[B], s_sL[B], s_sR[B];
-  __shared__ T s_P[5][B];
-
   const int tid = threadIdx.x;
-  const int64_t n = p.n;
-  const int g = p.g;
-  const int64_t S = g + static_cast<int64_t>(blockIdx.x) * kVjpOut;
 
-  if (blockIdx.x == 0 && tid < g) {
-    // ghost entries of a cotangent are zero
-    p.out[tid] = T(0);
-    p.out[p.m - g + tid] = T(0);
-  }
-
-  for (int j = tid; j < B + 4; j += B) s_u[j] = load_periodic(p.state, S - 5 + j, n, g);
-  for (int j = tid; j < B + 1; j += B) s_mu[j] = load_periodic(p.mu, S - 3 + j, n, g);
-  __syncthreads();
-
-  // ---- reconstructions of computing cell c = S - 3 + tid and gradients
+  // ---- reconstructions of computing cell c = Z - 3 + tid and gradients
   T GR[5], GLm[5];
   const T v0 = s_u[tid], v1 = s_u[tid + 1], v2 = s_u[tid + 2],
           v3 = s_u[tid + 3], v4 = s_u[tid + 4];
-  s_ur[tid] = weno_side_grad(v0, v1, v2, v3, v4, p.eps, GR);
-  s_ul[tid] = weno_side_grad(v4, v3, v2, v1, v0, p.eps, GLm);
+  sc.ur[tid] = weno_side_grad(v0, v1, v2, v3, v4, p.eps, GR);
+  sc.ul[tid] = weno_side_grad(v4, v3, v2, v1, v0, p.eps, GLm);
   if (tid == 0) {
-    s_cL[0] = T(0);
-    s_sR[0] = T(0);
+    sc.cL[0] = T(0);
+    sc.sR[0] = T(0);
   }
   __syncthreads();
 
   // ---- face c + 1/2: cotangent of the flux and its partial derivatives
   if (tid < B - 1) {
-    const T urL = s_ur[tid], ulR = s_ul[tid + 1];
+    const T urL = sc.ur[tid], ulR = sc.ul[tid + 1];
     const T uL = s_u[tid + 2], uR = s_u[tid + 3];
     T gF = (s_mu[tid + 1] - s_mu[tid]) * p.inv_dx;
     T dur = T(0), dul = T(0), duL = T(0), duR = T(0);
@@ -181,46 +174,146 @@ __global__ __launch_bounds__(kVjpBlock) void k_stage_vjp(VjpArgs<T> p) {
         dul = T(1);
       }
     }
-    s_cR[tid] = gF * dur;
-    s_cL[tid + 1] = gF * dul;
-    s_sL[tid] = gF * duL;
-    s_sR[tid + 1] = gF * duR;
+    sc.cR[tid] = gF * dur;
+    sc.cL[tid + 1] = gF * dul;
+    sc.sL[tid] = gF * duL;
+    sc.sR[tid + 1] = gF * duR;
   }
   __syncthreads();
 
   // ---- scatter: contribution of cell c's reconstructions to c - 2 .. c + 2
   if (tid >= 1 && tid <= B - 2) {
-    const T cR = s_cR[tid], cL = s_cL[tid];
+    const T cR = sc.cR[tid], cL = sc.cL[tid];
 #pragma unroll
-    for (int r = 0; r < 5; ++r) s_P[r][tid] = cR * GR[r] + cL * GLm[4 - r];
+    for (int r = 0; r < 5; ++r) sc.P[r][tid] = cR * GR[r] + cL * GLm[4 - r];
   }
   __syncthreads();
 
-  // ---- gather for output cell j = S + tid (local computing index q)
+  // ---- gather for output cell Z + tid (local computing index q)
+  T gsum = T(0);
+  if (tid < B - 6) {
+    const int q = tid + 3;
+    gsum = (sc.sL[q] + sc.sR[q]);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) gsum += sc.P[r][q + 2 - r];
+  }
+  return gsum;
+}
+
+// The reverse stage combinations from mu (the cotangent the stage received)
+// and the gathered JL^T mu.
+template <typename T, int RSTAGE>
+__device__ __forceinline__ T vjp_value(T mu, T gsum, T dt, T lam, T lamk2) {
+  if constexpr (RSTAGE == 0) {
+    return gsum;
+  } else if constexpr (RSTAGE == 3) {
+    return T(2.0 / 3.0) * (mu + dt * gsum);
+  } else if constexpr (RSTAGE == 2) {
+    return T(1.0 / 4.0) * (mu + dt * gsum);
+  } else {
+    return (T(1.0 / 3.0) * lam + T(3.0 / 4.0) * lamk2) + (mu + dt * gsum);
+  }
+}
+
+// RSTAGE 0: out = JL(state)^T mu; 3, 2, 1: the reverse stages above, with
+// mu = l, lk2, lk1 respectively.
+template <typename T, int FLUX, int RSTAGE>
+__global__ __launch_bounds__(kVjpBlock) void k_stage_vjp(VjpArgs<T> p) {
+  constexpr int B = kVjpBlock;
+  __shared__ T s_u[B + 4];   // cells S - 5 ..
+  __shared__ T s_mu[B + 1];  // cells S - 3 ..
+  __shared__ VjpScratch<T> sc;
+
+  const int tid = threadIdx.x;
+  const int64_t n = p.n;
+  const int g = p.g;
+  const int64_t S = g + static_cast<int64_t>(blockIdx.x) * kVjpOut;
+
+  if (blockIdx.x == 0 && tid < g) {
+    // ghost entries of a cotangent are zero
+    p.out[tid] = T(0);
+    p.out[p.m - g + tid] = T(0);
+  }
+
+  for (int j = tid; j < B + 4; j += B) s_u[j] = load_periodic(p.state, S - 5 + j, n, g);
+  for (int j = tid; j < B + 1; j += B) s_mu[j] = load_periodic(p.mu, S - 3 + j, n, g);
+  __syncthreads();
+
+  const T gsum = vjp_block<T, FLUX>(p, s_u, s_mu, sc);
   if (tid < kVjpOut) {
     const int64_t j = S + tid;
     if (j < p.m - g) {
-      const int q = tid + 3;
-      T gsum = (s_sL[q] + s_sR[q]);
-#pragma unroll
-      for (int r = 0; r < 5; ++r) gsum += s_P[r][q + 2 - r];
-      const T mu = s_mu[q];
-      T val;
-      if constexpr (RSTAGE == 0) {
-        val = gsum;
-      } else {
-        const T dt = *p.dt_ptr;
-        if constexpr (RSTAGE == 3) {
-          val = T(2.0 / 3.0) * (mu + dt * gsum);
-        } else if constexpr (RSTAGE == 2) {
-          val = T(1.0 / 4.0) * (mu + dt * gsum);
-        } else {
-          val = (T(1.0 / 3.0) * p.lam[j] + T(3.0 / 4.0) * p.lamk2[j]) +
-                (mu + dt * gsum);
-        }
-      }
-      p.out[j] = val;
+      const T dt = RSTAGE == 0 ? T(0) : *p.dt_ptr;
+      const T lam = RSTAGE == 1 ? p.lam[j] : T(0);
+      const T lamk2 = RSTAGE == 1 ? p.lamk2[j] : T(0);
+      p.out[j] = vjp_value<T, RSTAGE>(s_mu[tid + 3], gsum, dt, lam, lamk2);
     }
+  }
+}
+
+// Arguments of the fused reverse step.
+template <typename T>
+struct StepVjpArgs {
+  VjpArgs<T> v;     // eps, inv_dx, vconst, dt_ptr, m, n, g
+  const T *x;       // the step's input state
+  const T *k1;      // its stage states
+  const T *k2;
+  const T *lam;     // cotangent of the step output
+  T *out;           // cotangent of the step input
+};
+
+constexpr int kStepVjpOut = kVjpBlock - 18;  // output cells per workgroup
+
+// The three reverse stages of one step in one pass: a workgroup loads k2,
+// k1, x and the incoming cotangent with halos of 11, 8, 5 and 9 cells and runs
+// reverse stages 3, 2 and 1 in LDS (vjp_block each): lk2 is exact on
+// [S - 6, S + B - 12), lk1 on [S - 3, S + B - 15), the step's cotangent on the
+// output cells [S, S + B - 18). The same arithmetic as k_stage_vjp.
+template <typename T, int FLUX>
+__global__ __launch_bounds__(kVjpBlock) void k_step_vjp(StepVjpArgs<T> a) {
+  constexpr int B = kVjpBlock;
+  __shared__ T sK2[B + 4], sK1[B + 4], sX[B + 4];
+  __shared__ T sLam[B + 4];  // cells S - 9 ..
+  __shared__ T sLk2[B + 1];  // cells S - 6 ..
+  __shared__ T sLk1[B + 1];  // cells S - 3 ..
+  __shared__ VjpScratch<T> sc;
+  const VjpArgs<T> &p = a.v;
+  const int tid = threadIdx.x;
+  const int64_t n = p.n;
+  const int g = p.g;
+  const int64_t S = g + static_cast<int64_t>(blockIdx.x) * kStepVjpOut;
+  const T dt = *p.dt_ptr;
+
+  if (blockIdx.x == 0 && tid < g) {
+    a.out[tid] = T(0);
+    a.out[p.m - g + tid] = T(0);
+  }
+  for (int j = tid; j < B + 4; j += B) {
+    sK2[j] = load_periodic(a.k2, S - 11 + j, n, g);
+    sK1[j] = load_periodic(a.k1, S - 8 + j, n, g);
+    sX[j] = load_periodic(a.x, S - 5 + j, n, g);
+    sLam[j] = load_periodic(a.lam, S - 9 + j, n, g);
+  }
+  // entries past the exact cotangents (read only for cells outside the
+  // exact ranges; finite values)
+  for (int j = B - 6 + tid; j < B + 1; j += B) sLk2[j] = T(0);
+  for (int j = B - 6 + tid; j < B + 1; j += B) sLk1[j] = T(0);
+  __syncthreads();
+
+  // reverse stage 3 on the output cells S - 6 + t: lk2
+  T gsum = vjp_block<T, FLUX>(p, sK2, sLam, sc);
+  if (tid < B - 6) sLk2[tid] = vjp_value<T, 3>(sLam[tid + 3], gsum, dt, T(0), T(0));
+  __syncthreads();
+  // reverse stage 2 on S - 3 + t: lk1
+  gsum = vjp_block<T, FLUX>(p, sK1, sLk2, sc);
+  if (tid < B - 6) sLk1[tid] = vjp_value<T, 2>(sLk2[tid + 3], gsum, dt, T(0), T(0));
+  __syncthreads();
+  // reverse stage 1 on S + t: the cotangent of the step input
+  gsum = vjp_block<T, FLUX>(p, sX, sLk1, sc);
+  if (tid < kStepVjpOut) {
+    const int64_t j = S + tid;
+    if (j < p.m - g)
+      a.out[j] = vjp_value<T, 1>(sLk1[tid + 3], gsum, dt, sLam[tid + 9], sLk2[tid + 6]);
   }
 }
 
@@ -280,6 +373,39 @@ int launch_vjp(const psa_solver_desc *d, int rstage, const void *state,
   if (d->flux == PSA_FLUX_BURGERS_UPWIND)
     return launch_vjp_flux<T, PSA_FLUX_BURGERS_UPWIND>(rstage, a, s);
   return launch_vjp_flux<T, PSA_FLUX_ADVECTION_UPWIND>(rstage, a, s);
+}
+
+// The three reverse stages of one step as one launch (k_step_vjp).
+template <typename T>
+int launch_step_vjp(const psa_solver_desc *d, const void *x, const void *k1,
+                    const void *k2, const void *lam, void *out, const void *dt,
+                    hipStream_t s) {
+  StepVjpArgs<T> a{};
+  a.v.dt_ptr = static_cast<const T *>(dt);
+  a.v.n = d->n;
+  a.v.g = d->g;
+  a.v.m = d->n + 2 * static_cast<int64_t>(d->g);
+  a.v.inv_dx = T(1) / static_cast<T>(d->dx);
+  a.v.eps = static_cast<T>(d->eps);
+  a.v.vconst = static_cast<T>(d->velocity_const);
+  a.x = static_cast<const T *>(x);
+  a.k1 = static_cast<const T *>(k1);
+  a.k2 = static_cast<const T *>(k2);
+  a.lam = static_cast<const T *>(lam);
+  a.out = static_cast<T *>(out);
+  const unsigned int blocks =
+      static_cast<unsigned int>((d->n + kStepVjpOut - 1) / kStepVjpOut);
+  if (d->flux == PSA_FLUX_RUSANOV) {
+    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_RUSANOV>), dim3(blocks),
+                       dim3(kVjpBlock), 0, s, a);
+  } else if (d->flux == PSA_FLUX_BURGERS_UPWIND) {
+    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_BURGERS_UPWIND>), dim3(blocks),
+                       dim3(kVjpBlock), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ADVECTION_UPWIND>), dim3(blocks),
+                       dim3(kVjpBlock), 0, s, a);
+  }
+  return check_launch("step vjp");
 }
 
 int check_vjp_desc(const psa_solver_desc *d) {
@@ -516,6 +642,20 @@ int psa_ssprk33_adjoint(const psa_solver_desc *desc, long long nsteps,
     // reverse sweep through the window
     for (long long step = s1 - 1; step >= s0; --step) {
       const long long i = step - s0;
+      if (desc->schedule != PSA_SCHEDULE_TILE &&
+          desc->schedule != PSA_SCHEDULE_PIPELINE) {
+        // the three reverse stages in one launch
+        st = desc->dtype == PSA_FLOAT64
+                 ? launch_step_vjp<double>(desc, X(i), K1(i), K2(i), lam, lam_next,
+                                           dt, s)
+                 : launch_step_vjp<float>(desc, X(i), K1(i), K2(i), lam, lam_next,
+                                          dt, s);
+        if (st != PSA_OK) return st;
+        char *tmp = lam;
+        lam = lam_next;
+        lam_next = tmp;
+        continue;
+      }
       st = psa_ssprk33_stage_vjp(desc, 3, K2(i), lam, nullptr, nullptr, lk2, dt,
                                  stream);
       if (st != PSA_OK) return st;

@@ -798,8 +798,9 @@ int check_desc(const psa_solver_desc *d) {
 
 TileConfig tiles_of(const psa_solver_desc *d) {
   TileConfig c = default_tiles(d->dtype);
-  // the learned kernels keep fewer cells per thread to hold occupancy
-  if (d->mlp != nullptr) c.cpt = 2;
+  // below ~2^20 cells a grid of 1024-cell tiles leaves most of the device
+  // idle: half-size tiles double the workgroups in flight
+  if (d->n < (int64_t(1) << 20)) c.cpt = 2;
   if (d->block > 0) c.block = d->block;
   if (d->cells_per_thread > 0) c.cpt = d->cells_per_thread;
   if (d->schedule == PSA_SCHEDULE_TILE) c.step = false;

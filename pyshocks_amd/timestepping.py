@@ -202,14 +202,15 @@ class FusedPlan:
     def uses_step_kernel(self) -> bool:
         """Whether :meth:`steps` runs one whole-step kernel per step (the
         library's rule: schedule default or step, a constant or no velocity,
-        no Rusanov ``nu``, tiles (256, 4) or (256, 2))."""
+        no Rusanov ``nu``, tiles (256, 4) or (256, 2); by default 2 cells per
+        thread below 2^20 cells, 4 above)."""
         d = self.desc
         if d.schedule not in (_lib.PSA_SCHEDULE_DEFAULT, _lib.PSA_SCHEDULE_STEP):
             return False
         if d.nu or (d.flux == _lib.PSA_FLUX_ADVECTION_UPWIND and not d.velocity_is_const):
             return False
         block = d.block if d.block > 0 else 256
-        cpt = d.cells_per_thread if d.cells_per_thread > 0 else (2 if d.mlp else 4)
+        cpt = d.cells_per_thread if d.cells_per_thread > 0 else (2 if d.n < (1 << 20) else 4)
         return block == 256 and cpt in (2, 4)
 
     @staticmethod
