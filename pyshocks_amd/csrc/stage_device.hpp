@@ -245,6 +245,8 @@ __device__ __forceinline__ void learned_cells(const MlpSource &src, const T *w,
   int nmine = 0;
 #pragma unroll
   for (int r = 0; r < CPT; ++r) nmine += regime[r] == 2 ? 2 : 0;
+  // most waves of a smooth solution hold no learned cell (wave-uniform exit)
+  if (__ballot(nmine != 0) == 0) return;
   int incl = nmine;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
