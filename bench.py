@@ -452,9 +452,10 @@ def batched_cpu_leg(args: argparse.Namespace, ns, states) -> dict | None:
 def learned_leg(args: argparse.Namespace, *, ninst: int = 128, nmax: int = 8192,
                 full: bool = False, plain_single_value: float | None = None) -> dict:
     """Learned-weight batched solver (shipped network) against the plain
-    WENO-JS5 FP64 path at equal L1 error, on seeded sine-plus-step Burgers
-    instances with N log-uniform in [1024, nmax], t = 0.5, learned state in
-    FP64 and in FP32.
+    WENO-JS5 FP64 batched solver at equal L1 error, on seeded sine-plus-step
+    Burgers instances with N log-uniform in [1024, nmax], t = 0.5, learned
+    state in FP64 and in FP32; both solvers with tiles tuned on 16 of the
+    instances.
 
     L1 errors are against exact Hopf-Lax cell averages. The plain path's
     error-vs-N order p comes from the same instances at 2 N (the subset
@@ -472,14 +473,21 @@ def learned_leg(args: argparse.Namespace, *, ninst: int = 128, nmax: int = 8192,
     rng = np.random.default_rng(5000 + args.seed)
     ns = np.round(np.exp(rng.uniform(np.log(1024), np.log(nmax), ninst))).astype(np.int64)
     ics = training_ics(ninst, seed=5000 + args.seed)
+    u0s = exact.burgers_cell_averages(ics[:16], ns[:16], 0.0)
+    candidates = ((256, 4), (512, 2), (1024, 2))
 
-    e_p, info_p = l1_errors(ics, ns, 0.5, dtype=torch.float64)
+    # the plain baseline with its own tuned tiles
+    plain_tiles, _ = autotune.tune_batch(ns[:16], [u.cpu().numpy() for u in u0s],
+                                         dtype=torch.float64, tfinal=0.01,
+                                         candidates=candidates)
+    teams = args.team_cells or None
+    e_p, info_p = l1_errors(ics, ns, 0.5, dtype=torch.float64, tiles=plain_tiles,
+                            team_cells=teams)
     sub = ns <= 8192
     e_p2, _ = l1_errors([ic for ic, k in zip(ics, sub) if k], 2 * ns[sub], 0.5,
-                        dtype=torch.float64)
+                        dtype=torch.float64, tiles=plain_tiles, team_cells=teams)
     p = float(np.log2(e_p[sub].mean() / e_p2.mean()))
     plain_batched = info_p["cell_updates"] / info_p["seconds"]
-    u0s = exact.burgers_cell_averages(ics[:16], ns[:16], 0.0)
     out = {
         "config": f"{ninst} seeded sine+step Burgers instances, N log-uniform "
                   f"[1024, {nmax}], t = 0.5, learned batched (tau {w.tau}, sigma {w.sigma}) vs "
@@ -487,13 +495,16 @@ def learned_leg(args: argparse.Namespace, *, ninst: int = 128, nmax: int = 8192,
         "l1_plain_f64": float(e_p.mean()),
         "plain_order": p,
         "plain_f64_batched_cell_updates_per_s": plain_batched,
+        "plain_f64_seconds": info_p["seconds"],
+        "plain_tiles": list(plain_tiles),
+        "team_cells": teams,
     }
     best = None
     for name, dtype in (("f64", torch.float64), ("f32", torch.float32)):
         tiles, _ = autotune.tune_batch(ns[:16], [u.cpu().numpy() for u in u0s], dtype=dtype,
-                                       tfinal=0.01, mlp=w,
-                                       candidates=((256, 4), (512, 2), (1024, 2)))
-        e_l, info_l = l1_errors(ics, ns, 0.5, mlp=w, dtype=dtype, tiles=tiles)
+                                       tfinal=0.01, mlp=w, candidates=candidates)
+        e_l, info_l = l1_errors(ics, ns, 0.5, mlp=w, dtype=dtype, tiles=tiles,
+                                team_cells=teams)
         r = float(e_p.mean() / e_l.mean())
         scale = r ** (2.0 / p) if p > 0 else float("nan")
         cu_plain_equal = info_p["cell_updates"] * scale

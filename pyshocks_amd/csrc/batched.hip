@@ -79,6 +79,19 @@ struct BatchArgs {
   T sigma;
 };
 
+// A value every lane of the wave holds (read from LDS after a broadcast)
+// moved to scalar registers, so that what is derived from it (instance
+// sizes, pointers, time control) stays out of the vector registers.
+__device__ __forceinline__ int wave_uniform(int v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ unsigned long long wave_uniform(unsigned long long v) {
+  const unsigned int lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned int>(v));
+  const unsigned int hi =
+      __builtin_amdgcn_readfirstlane(static_cast<unsigned int>(v >> 32));
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
 template <int BLOCK>
 __device__ __forceinline__ unsigned long long block_max_bcast(
     unsigned long long v, unsigned long long *s_red,
@@ -125,7 +138,7 @@ __device__ __forceinline__ bool team_barrier(TeamState *ts, int size,
     *s_flag = ok;
   }
   __syncthreads();
-  return *s_flag != 0;
+  return wave_uniform(*s_flag) != 0;
 }
 
 // WHOLE: one sweep per step with the three stages of each tile in LDS
@@ -159,7 +172,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
     __syncthreads();  // previous slot done with the LDS and s_next
     if (tid == 0) s_next = static_cast<long long>(atomicAdd(a.counter, 1u));
     __syncthreads();
-    const long long k = s_next;
+    const long long k = wave_uniform(static_cast<int>(s_next));
     if (k >= a.nslots) break;
     const int i = a.order[k];
     const int rank = a.slot_rank != nullptr ? a.slot_rank[k] : 0;
@@ -190,7 +203,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
     // team reduction of max|u| bits into buffer b (a single workgroup keeps
     // the block maximum)
     auto team_max = [&](unsigned long long v, int b) -> unsigned long long {
-      v = block_max_bcast<BLOCK>(v, sm.red, &s_bcast);
+      v = wave_uniform(block_max_bcast<BLOCK>(v, sm.red, &s_bcast));
       if (ts == nullptr) return v;
       if (tid == 0 && v != 0)
         __hip_atomic_fetch_max(&ts->umax[b], v, __ATOMIC_RELAXED,
@@ -200,7 +213,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
         s_bcast = __hip_atomic_load(&ts->umax[b], __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
       __syncthreads();
-      return s_bcast;
+      return wave_uniform(s_bcast);
     };
     auto barrier = [&]() {
       if (ts == nullptr) {

@@ -318,17 +318,20 @@ def make_training_samples(ics: list, *, n: int = 4096, times: tuple[float, ...] 
 def l1_errors(ics: list, ns: np.ndarray, t: float, *, mlp: MLPWeights | None = None,
               dtype: torch.dtype = torch.float64, eps: float = 1.0e-6,
               device: str = "cuda", tiles: tuple[int, int] | None = None,
+              team_cells: int | None = None,
               ) -> tuple[np.ndarray, dict[str, Any]]:
     """Interior L1 errors ``sum_i |u_i - u_exact_i| h`` at time ``t`` of the
     batched solver (WENO-JS, or the learned weights ``mlp``) on instances
-    ``ics`` with ``ns`` cells, against exact Hopf-Lax cell averages."""
+    ``ics`` with ``ns`` cells, against exact Hopf-Lax cell averages.
+    ``team_cells`` splits larger instances among teams of workgroups."""
     from pyshocks_amd import exact
     from pyshocks_amd.batched import BatchedSolver
 
     ns = np.asarray(ns, dtype=np.int64)
     u0 = exact.burgers_cell_averages(ics, ns, 0.0, device=device)
     ue = exact.burgers_cell_averages(ics, ns, t, device=device)
-    solver = BatchedSolver(ns, dtype=dtype, eps=eps, device=device, mlp=mlp, tiles=tiles)
+    solver = BatchedSolver(ns, dtype=dtype, eps=eps, device=device, mlp=mlp, tiles=tiles,
+                           team_cells=team_cells)
     solver.set_states(u0)
     torch.cuda.synchronize()
     import time
