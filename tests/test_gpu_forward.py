@@ -455,14 +455,16 @@ def test_pipelined_schedule_bit_for_bit(torch_mod, kind: str, dtype_name: str, t
 @pytest.mark.parametrize("kind", ["rusanov", "godunov", "advection"])
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
 @pytest.mark.parametrize("tiles", [(256, 4), (256, 2)])
-@pytest.mark.parametrize("n", [37, 1013, 100_003])
+@pytest.mark.parametrize("n", [3, 16, 37, 1013, 100_003])
 def test_step_schedule_bit_for_bit(torch_mod, kind: str, dtype_name: str, tiles,
                                    n: int) -> None:
     """The whole-step kernel (the three stages of a tile in LDS, the halo
     cells recomputed by the neighbouring tiles) computes exactly what the
     stage kernels compute: the state with its ghost images, t, m and dt, over
     in-place and out-of-place calls, a ragged last tile, a grid smaller than
-    one tile, and steps enqueued past tfinal (which leave the state as is)."""
+    one tile (n = 3 and 16: smaller than the 9-cell halo, which then wraps
+    around the periodic grid several times), and steps enqueued past tfinal
+    (which leave the state as is)."""
     torch = torch_mod
     from oracle import exact as E
     from pyshocks_amd import _lib, make_uniform_cell_grid
