@@ -17,6 +17,11 @@ its event-timed duration, against the vector peak, with its HBM figures
 beside them; the stage kernels' HBM roofline), and the CPU restatement of
 the same step loop on the host cores, on a bounded sample.
 
+The other Burgers fluxes run the same workload with ``--scheme godunov``,
+``--scheme eo`` (Engquist-Osher, whole-step kernel) or ``--scheme lf``
+(Lax-Friedrichs: the stage kernels, each reducing the global max|u| that the
+next stage's flux uses).
+
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
 
@@ -56,7 +61,7 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--n", type=int, default=1 << 24)
     p.add_argument("--dtype", choices=["f64", "f32"], default="f64")
-    p.add_argument("--scheme", choices=["rusanov", "godunov", "advection"],
+    p.add_argument("--scheme", choices=["rusanov", "godunov", "lf", "eo", "advection"],
                    default="rusanov")
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--block", type=int, default=0)
@@ -109,7 +114,7 @@ def make_problem(args: argparse.Namespace):
 
     from pyshocks_amd import funcs, make_uniform_cell_grid
     from pyshocks_amd.advection import Godunov as AdvGodunov
-    from pyshocks_amd.burgers import Godunov, Rusanov
+    from pyshocks_amd.burgers import EngquistOsher, Godunov, LaxFriedrichs, Rusanov
     from pyshocks_amd.reconstruction import WENOJS53
     from pyshocks_amd.scalar import PeriodicBoundary
     from pyshocks_amd.timestepping import fused_plan, make_ssprk33
@@ -126,6 +131,10 @@ def make_problem(args: argparse.Namespace):
         scheme, theta = Rusanov(rec=rec), 1.0
     elif args.scheme == "godunov":
         scheme, theta = Godunov(rec=rec), 1.0
+    elif args.scheme == "lf":
+        scheme, theta = LaxFriedrichs(rec=rec), 1.0
+    elif args.scheme == "eo":
+        scheme, theta = EngquistOsher(rec=rec), 1.0
     else:
         vel = torch.ones(grid.n, dtype=dtype, device="cuda")
         scheme, theta = AdvGodunov(rec=rec, velocity=vel), 0.5

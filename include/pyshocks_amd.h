@@ -44,10 +44,22 @@ extern "C" {
  * PSA_FLUX_BURGERS_UPWIND   burgers.Godunov  -> scalar_flux_upwind
  *                           (burgers/schemes.py:75-89, scalar.py:91-132)
  * PSA_FLUX_ADVECTION_UPWIND advection.Godunov -> upwind_flux
- *                           (advection/schemes.py:100-129)               */
+ *                           (advection/schemes.py:100-129)
+ * PSA_FLUX_ENGQUIST_OSHER   burgers.EngquistOsher -> scalar_flux_engquist_osher
+ *                           with omega = 0 (burgers/schemes.py:162-196,
+ *                           scalar.py:287-322)
+ * PSA_FLUX_LAX_FRIEDRICHS   burgers.LaxFriedrichs -> scalar_flux_lax_friedrichs
+ *                           (burgers/schemes.py:136-153, scalar.py:258-278):
+ *                           the Rusanov flux with the global speed max|a|
+ *                           in place of the local one. Without a control
+ *                           block that speed is a device scalar the caller
+ *                           passes (see psa_numerical_flux); with one, the
+ *                           stage kernels reduce it themselves.          */
 #define PSA_FLUX_RUSANOV 0
 #define PSA_FLUX_BURGERS_UPWIND 1
 #define PSA_FLUX_ADVECTION_UPWIND 2
+#define PSA_FLUX_ENGQUIST_OSHER 3
+#define PSA_FLUX_LAX_FRIEDRICHS 4
 
 /* time step modes of the device-side step control */
 #define PSA_DT_CFL 0   /* dt = theta * cfl_num / max|u_int| (or max|a_int|) */
@@ -84,8 +96,11 @@ int psa_apply_boundary_periodic(int dtype, const void *u, void *out,
 
 /* schemes.py:322-336 numerical_flux on u as given (no boundary applied):
  * fnum (m + 1) with fnum[0] = fnum[m] = 0. velocity (m) is required for
- * PSA_FLUX_ADVECTION_UPWIND and ignored otherwise. nu (m - 1) is the Rusanov
- * artificial-viscosity factor grid.df ** (alpha - 1), or NULL for alpha == 1. */
+ * PSA_FLUX_ADVECTION_UPWIND; for PSA_FLUX_LAX_FRIEDRICHS it points to ONE
+ * device value of the dtype, the global speed max|a| (scalar.py:276, e.g.
+ * from psa_max_abs_interior with g = 0); it is ignored otherwise. nu (m - 1)
+ * is the artificial-viscosity factor grid.df ** (alpha - 1) of the Rusanov
+ * and Lax-Friedrichs fluxes, or NULL for alpha == 1. */
 int psa_numerical_flux(int dtype, int flux, const void *u,
                        const void *velocity, const void *nu, void *fnum,
                        int64_t m, double eps, void *stream);
@@ -93,7 +108,9 @@ int psa_numerical_flux(int dtype, int flux, const void *u,
 /* schemes.py:339-346 (conservation law) and advection/schemes.py:62-73:
  * out (m) = -(F[1:] - F[:-1]) / dx (times -velocity for advection), with F
  * the numerical flux of the periodic ghost-filled u; every entry, ghosts
- * included, as the reference computes it. */
+ * included, as the reference computes it. For PSA_FLUX_LAX_FRIEDRICHS,
+ * velocity is the device scalar max|u| of the ghost-filled state (the
+ * interior maximum, psa_max_abs_interior). */
 int psa_apply_operator(int dtype, int flux, const void *u,
                        const void *velocity, const void *nu, void *out,
                        int64_t m, int g, double dx, double eps, void *stream);
@@ -147,7 +164,8 @@ typedef struct psa_control {
   int active;                   /* 1 while the step in flight is taken     */
   int status;                   /* PSA_OK or PSA_ERR_NONFINITE_DT          */
   int dt_mode;                  /* PSA_DT_CFL or PSA_DT_FIXED              */
-  int speed_from_state;         /* 1: max|u| of each state (Burgers)       */
+  int speed_from_state;         /* 1: max|u| of each state (Burgers with
+                                   PSA_DT_CFL, and Lax-Friedrichs)         */
   /* whole-step schedule (PSA_SCHEDULE_STEP): the decision for the next
    * step, made by the one-wave kernel after the previous step launch (or
    * before the first one of psa_ssprk33_steps) */
@@ -159,7 +177,10 @@ typedef struct psa_control {
    * (a workgroup uses slot blockIdx % PSA_UMAX_SLOTS); stage 1 of the next
    * step combines them and stage 2 clears them. The whole-step kernel
    * reduces into word PSA_UMAX_STEP_WORD; the kernel after it leaves the
-   * combined value in word 0 of slot 0 (the other word-0 entries zero). */
+   * combined value in word 0 of slot 0 (the other word-0 entries zero).
+   * Lax-Friedrichs: stage s (1, 2) reduces max|u| of its output into word
+   * s, which stage s + 1 combines as its flux speed; stage 1 clears word 2
+   * and stage 3 clears word 1. */
   unsigned long long umax_slot[PSA_UMAX_SLOTS][PSA_UMAX_SLOT_STRIDE];
 } psa_control;
 #define PSA_UMAX_STEP_WORD 8
@@ -174,8 +195,11 @@ typedef struct psa_solver_desc {
   double dx;                    /* cell size h                             */
   double eps;                   /* WENO-JS epsilon                         */
   double velocity_const;        /* advection constant velocity             */
-  const void *velocity;         /* advection velocity (m) if not const     */
-  const void *nu;               /* Rusanov df^(alpha-1) (m - 1) or NULL    */
+  const void *velocity;         /* advection velocity (m) if not const;
+                                   Lax-Friedrichs without a control block:
+                                   the device scalar max|a| (one value)    */
+  const void *nu;               /* Rusanov / Lax-Friedrichs df^(alpha-1)
+                                   (m - 1) or NULL                         */
   int block;                    /* threads per workgroup (0 = tuned)       */
   int cells_per_thread;         /* cells per thread (0 = tuned)            */
   const float *mlp;             /* learned weights (PSA_MLP_NPARAM FP32) or
@@ -196,7 +220,8 @@ typedef struct psa_solver_desc {
  *   workgroup loads its tile of u^n with a 9-cell halo on each side, runs the
  *   three stages in LDS (recomputing the halo cells of the first two) and
  *   writes u^(n+1) (2 words per cell and step; WENO-JS or learned weights, no
- *   velocity array, no Rusanov nu, a control block, tiles 256 x 4 or
+ *   velocity array, no Rusanov nu, not Lax-Friedrichs (its stages need the
+ *   global max|u| of the previous stage), a control block, tiles 256 x 4 or
  *   256 x 2). Steps ping-pong between u_out and the stage buffers.
  * DEFAULT is STEP where it applies and TILE otherwise. All schedules compute
  * the same values. */
@@ -229,20 +254,24 @@ int psa_ssprk33_steps(const psa_solver_desc *desc, const void *u_in,
  * control is NULL, the device scalar dt. out may equal un but not uk.
  * Stage 0 is the operator itself, out = L(uk) on all m cells
  * (schemes.py:466-473, as psa_apply_operator), for any descriptor,
- * including one with learned weights; un, control and dt are unused. */
+ * including one with learned weights; un, control and dt are unused.
+ * Lax-Friedrichs without a control block (stage 0, or a device dt) takes
+ * the flux speed max|uk| from desc->velocity. */
 int psa_ssprk33_stage(const psa_solver_desc *desc, int stage, const void *un,
                       const void *uk, void *out, void *control,
                       const void *dt, void *stream);
 
 /* timestepping.py:312-320 advance(SSPRK33, dt, t, u) for the bound operator
- * with the dt of a device scalar of the dtype: out = advance(u). */
+ * with the dt of a device scalar of the dtype: out = advance(u). Not for
+ * Lax-Friedrichs (PSA_ERR_NOT_IMPLEMENTED: its stages need a control block
+ * to carry the stage maxima). */
 int psa_ssprk33_advance(const psa_solver_desc *desc, const void *u,
                         void *out, void *k1, void *k2, const void *dt,
                         void *stream);
 
 /* Batched solver ------------------------------------------------------------
- * ninst independent periodic Burgers instances (Rusanov with alpha = 1, or
- * Godunov) of uneven size, each advanced from t = 0 to its own tfinal with
+ * ninst independent periodic Burgers instances (Rusanov with alpha = 1,
+ * Godunov or Engquist-Osher) of uneven size, each advanced from t = 0 to its own tfinal with
  * its own CFL time step (timestepping.py:89-152) in one persistent launch:
  * every workgroup takes the next instance from `order` (an atomic counter)
  * and runs its whole time loop; the launch is graph-capturable (the counter
@@ -254,7 +283,8 @@ int psa_ssprk33_advance(const psa_solver_desc *desc, const void *u,
  * advances t; PSA_ERR_INVALID_ARGUMENT when max_steps is reached). */
 typedef struct psa_batch_desc {
   int dtype;                    /* PSA_FLOAT64 / PSA_FLOAT32               */
-  int flux;                     /* PSA_FLUX_RUSANOV / _BURGERS_UPWIND      */
+  int flux;                     /* PSA_FLUX_RUSANOV / _BURGERS_UPWIND /
+                                   _ENGQUIST_OSHER                         */
   int g;                        /* ghost cells per side (>= 3)             */
   int grid;                     /* workgroups (0 = resident capacity)      */
   long long ninst;              /* instances                               */

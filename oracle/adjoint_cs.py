@@ -46,6 +46,11 @@ def apply_operator_cs(kind: str, grid: W.Grid, u: np.ndarray, *, eps: float,
         fr = ur**2 / 2
         aavg = (ur[:-1] + ul[1:]) / 2
         fnum = np.where(aavg.real > 0, fr[:-1], fl[1:])
+    elif kind == "eo":
+        # f(max(ur, 0)) + f(min(ul, 0)), the switches on the real parts
+        p = np.where(ur[:-1].real > 0, ur[:-1], 0)
+        q = np.where(ul[1:].real < 0, ul[1:], 0)
+        fnum = p**2 / 2 + q**2 / 2
     elif kind == "advection":
         fnum = ur[:-1] if velocity > 0 else ul[1:]
     else:

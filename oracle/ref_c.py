@@ -12,7 +12,7 @@ import numpy as np
 
 LIB = pathlib.Path(__file__).resolve().parent / "libweno_ref.so"
 
-FLUX = {"rusanov": 0, "godunov": 1, "advection": 2}
+FLUX = {"rusanov": 0, "godunov": 1, "advection": 2, "eo": 3, "lf": 4}
 
 _lib: ctypes.CDLL | None = None
 

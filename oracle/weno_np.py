@@ -1,5 +1,5 @@
 """CPU oracle: a NumPy FP64 restatement of pyshocks' WENO-JS5 / Rusanov /
-Godunov / SSP-RK3 hot path.
+Godunov / Lax-Friedrichs / Engquist-Osher / SSP-RK3 hot path.
 
 TEST INFRASTRUCTURE ONLY. This module is the checker for the HIP library in
 ``pyshocks_amd``; it is imported by ``tests/``, ``__graft_entry__.smoke()``
@@ -22,7 +22,14 @@ Pins (each checked in ``tests/test_oracle_*.py``):
 * SSP-RK3 order on ``u' = exp(-t)`` (``tests/test_timestepping.py:26-99``);
 * end-to-end: periodic advection EOC >= 4.5 against the exact shift
   (``tests/test_convergence.py:395-460``) and Burgers against the exact
-  Hopf-Lax cell averages (``oracle/exact.py``) through a shock.
+  Hopf-Lax cell averages (``oracle/exact.py``) through a shock;
+* Engquist-Osher: the face flux against the closed form of its integral
+  definition ``(f(ul) + f(ur)) / 2 - (1/2) int_ul^ur |f'(u)| du`` and against
+  midpoint quadrature of that integral; consistency ``F(u, u) = f(u)``;
+* Lax-Friedrichs: equal to the Rusanov flux wherever ``|u|`` is one constant
+  (local and global speeds agree), the global speed taken over the whole
+  array (a ghost entry that holds the maximum changes every face), and
+  consistency on constant states.
 """
 
 from __future__ import annotations
@@ -272,9 +279,40 @@ def scalar_flux_rusanov(grid: Grid, u: np.ndarray, a: np.ndarray, *, eps: float,
     fr = flux(ur)
 
     a = np.abs(a)
-    a = np.maximum(a[1:], a[:-1])
+    if a.size == u.size:
+        # a scalar speed (Lax-Friedrichs) is used as given (scalar.py:243-246)
+        a = np.maximum(a[1:], a[:-1])
 
     fnum = 0.5 * (fl[1:] + fr[:-1]) - 0.5 * a * nu * (ul[1:] - ur[:-1])
+    return np.pad(fnum, 1)
+
+
+def scalar_flux_lax_friedrichs(grid: Grid, u: np.ndarray, a: np.ndarray, *,
+                               eps: float, alpha: float = 1.0) -> np.ndarray:
+    """Rusanov with the global speed ``max|a|`` over the whole array
+    (``scalar.py:258-278``)."""
+    amax = np.max(np.abs(a))
+    return scalar_flux_rusanov(grid, u, amax, eps=eps, alpha=alpha)
+
+
+def engquist_osher_faces(ur_left: np.ndarray, ul_right: np.ndarray, *,
+                         omega: float = 0.0,
+                         flux: Callable[[np.ndarray], np.ndarray] = burgers_flux,
+                         ) -> np.ndarray:
+    """Face fluxes ``f(max(ur, omega)) + f(min(ul, omega)) - f(omega)`` from
+    the right state of the left cell and the left state of the right cell
+    (``scalar.py:309-319``)."""
+    fr = flux(np.maximum(ur_left, omega))
+    fl = flux(np.minimum(ul_right, omega))
+    fomega = flux(np.full_like(ur_left, omega))
+    return fr + fl - fomega
+
+
+def scalar_flux_engquist_osher(grid: Grid, u: np.ndarray, *, eps: float,
+                               omega: float = 0.0) -> np.ndarray:
+    """``scalar.py:287-322`` with a WENO-JS reconstruction."""
+    ul, ur = reconstruct_wenojs(u, eps)
+    fnum = engquist_osher_faces(ur[:-1], ul[1:], omega=omega)
     return np.pad(fnum, 1)
 
 
@@ -309,8 +347,9 @@ def advection_upwind_flux(grid: Grid, u: np.ndarray, velocity: np.ndarray, *,
 
 @dataclass(frozen=True)
 class Scheme:
-    """A bound scheme of the hot path: ``kind`` is ``"rusanov"`` or
-    ``"godunov"`` (Burgers) or ``"advection"`` (linear advection, Godunov)."""
+    """A bound scheme of the hot path: ``kind`` is ``"rusanov"``,
+    ``"godunov"``, ``"lf"`` (Lax-Friedrichs) or ``"eo"`` (Engquist-Osher)
+    for Burgers, or ``"advection"`` (linear advection, Godunov)."""
 
     kind: str
     eps: float
@@ -319,9 +358,15 @@ class Scheme:
 
 
 def numerical_flux(scheme: Scheme, grid: Grid, u: np.ndarray) -> np.ndarray:
-    """``burgers/schemes.py:83-118`` and ``advection/schemes.py:125-129``."""
+    """``burgers/schemes.py:83-118,145-153,188-196`` and
+    ``advection/schemes.py:125-129``."""
     if scheme.kind == "rusanov":
         return scalar_flux_rusanov(grid, u, u, eps=scheme.eps, alpha=scheme.alpha)
+    if scheme.kind == "lf":
+        return scalar_flux_lax_friedrichs(grid, u, u, eps=scheme.eps,
+                                          alpha=scheme.alpha)
+    if scheme.kind == "eo":
+        return scalar_flux_engquist_osher(grid, u, eps=scheme.eps)
     if scheme.kind == "godunov":
         return scalar_flux_upwind(grid, u, u, eps=scheme.eps)
     if scheme.kind == "advection":
@@ -344,12 +389,12 @@ def apply_operator(scheme: Scheme, grid: Grid, t: float, u: np.ndarray) -> np.nd
 
 
 def predict_timestep(scheme: Scheme, grid: Grid, t: float, u: np.ndarray) -> np.floating[Any]:
-    """``burgers/schemes.py:42-49`` (Godunov), ``:121-127`` (Rusanov) and
-    ``advection/schemes.py:51-59``."""
-    if scheme.kind == "rusanov":
+    """``burgers/schemes.py:42-49`` (Godunov, Engquist-Osher), ``:121-127``
+    (Rusanov, Lax-Friedrichs) and ``advection/schemes.py:51-59``."""
+    if scheme.kind in ("rusanov", "lf"):
         smax = np.max(np.abs(u[grid.i_]))
         return 0.5 * grid.dx_min ** (2 - scheme.alpha) / smax
-    if scheme.kind == "godunov":
+    if scheme.kind in ("godunov", "eo"):
         smax = np.max(np.abs(u[grid.i_]))
         return 0.5 * grid.dx_min / smax
     if scheme.kind == "advection":

@@ -1,6 +1,7 @@
 /*
  * CPU oracle, C restatement of pyshocks' WENO-JS5 / Rusanov / Godunov /
- * SSP-RK3 hot path (FP64), parallelised over cells with OpenMP.
+ * Engquist-Osher / Lax-Friedrichs / SSP-RK3 hot path (FP64), parallelised
+ * over cells with OpenMP.
  *
  * TEST INFRASTRUCTURE ONLY: it is loaded by tests/ (checked against the
  * NumPy restatement oracle/weno_np.py) and timed as the CPU baseline by
@@ -12,6 +13,9 @@
  *                                     reconstruction.py:351-377
  *   numerical flux                    scalar.py:91-132 (upwind),
  *                                     scalar.py:192-249 (Rusanov),
+ *                                     scalar.py:258-278 (Lax-Friedrichs),
+ *                                     scalar.py:287-322 (Engquist-Osher,
+ *                                     omega = 0),
  *                                     advection/schemes.py:100-114
  *   operator -(F[1:]-F[:-1])/dx       schemes.py:339-346,
  *                                     advection/schemes.py:62-73
@@ -35,6 +39,8 @@
 #define ORACLE_RUSANOV 0
 #define ORACLE_BURGERS_UPWIND 1
 #define ORACLE_ADVECTION_UPWIND 2
+#define ORACLE_ENGQUIST_OSHER 3
+#define ORACLE_LAX_FRIEDRICHS 4
 
 static inline double at(const double *u, int64_t i, int64_t m) {
   return (i >= 0 && i < m) ? u[i] : 0.0;
@@ -76,16 +82,22 @@ static inline double recon_left(const double *u, int64_t i, int64_t m,
               at(u, i - 2, m), eps);
 }
 
-/* flux at face j (between cells j - 1 and j), 1 <= j <= m - 1 */
+/* flux at face j (between cells j - 1 and j), 1 <= j <= m - 1; amax is
+ * the global speed of Lax-Friedrichs */
 static inline double face_flux(int flux, const double *u, const double *vel,
-                               int64_t j, int64_t m, double eps) {
+                               int64_t j, int64_t m, double eps, double amax) {
   const double urL = recon_right(u, j - 1, m, eps);
   const double ulR = recon_left(u, j, m, eps);
-  if (flux == ORACLE_RUSANOV) {
+  if (flux == ORACLE_RUSANOV || flux == ORACLE_LAX_FRIEDRICHS) {
     const double fl = (ulR * ulR) / 2.0;
     const double fr = (urL * urL) / 2.0;
-    const double a = fmax(fabs(u[j]), fabs(u[j - 1]));
+    const double a = flux == ORACLE_RUSANOV ? fmax(fabs(u[j]), fabs(u[j - 1])) : amax;
     return 0.5 * (fl + fr) - (0.5 * a) * (ulR - urL);
+  }
+  if (flux == ORACLE_ENGQUIST_OSHER) {
+    /* f(max(ur, 0)) + f(min(ul, 0)) - f(0), f(0) = 0 */
+    const double p = fmax(urL, 0.0), q = fmin(ulR, 0.0);
+    return (p * p) / 2.0 + (q * q) / 2.0;
   }
   if (flux == ORACLE_BURGERS_UPWIND) {
     const double aavg = (urL + ulR) / 2.0;
@@ -115,8 +127,17 @@ void oracle_apply_operator(int flux, const double *u, const double *vel,
   fill_periodic(uf, m, g);
   F[0] = 0.0;
   F[m] = 0.0;
+  double amax = 0.0;
+  if (flux == ORACLE_LAX_FRIEDRICHS) {
+    /* max|a| over the whole ghost-filled array, a = u (scalar.py:276) */
+#pragma omp parallel for reduction(max : amax) schedule(static)
+    for (int64_t i = 0; i < m; ++i) {
+      const double v = fabs(uf[i]);
+      if (v > amax) amax = v;
+    }
+  }
 #pragma omp parallel for schedule(static)
-  for (int64_t j = 1; j < m; ++j) F[j] = face_flux(flux, uf, vel, j, m, eps);
+  for (int64_t j = 1; j < m; ++j) F[j] = face_flux(flux, uf, vel, j, m, eps, amax);
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < m; ++i) {
     const double d = F[i + 1] - F[i];

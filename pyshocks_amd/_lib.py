@@ -32,6 +32,12 @@ PSA_FLOAT32 = 1
 PSA_FLUX_RUSANOV = 0
 PSA_FLUX_BURGERS_UPWIND = 1
 PSA_FLUX_ADVECTION_UPWIND = 2
+PSA_FLUX_ENGQUIST_OSHER = 3
+PSA_FLUX_LAX_FRIEDRICHS = 4
+
+#: flux codes of the batched solver's Burgers scheme names
+BATCH_FLUX = {"rusanov": PSA_FLUX_RUSANOV, "godunov": PSA_FLUX_BURGERS_UPWIND,
+              "eo": PSA_FLUX_ENGQUIST_OSHER}
 
 PSA_DT_CFL = 0
 PSA_DT_FIXED = 1

@@ -104,6 +104,6 @@ def tune_batch(ncells: Sequence[int], states: Sequence[np.ndarray], *,
     if not times:
         raise RuntimeError("no tile candidate could be launched")
     best = min(times, key=lambda c: times[c])
-    flux = _lib.PSA_FLUX_RUSANOV if scheme == "rusanov" else _lib.PSA_FLUX_BURGERS_UPWIND
+    flux = _lib.BATCH_FLUX[scheme]
     _BATCH_CACHE[(_lib.dtype_code(dtype), flux, mlp is not None)] = best
     return best, times

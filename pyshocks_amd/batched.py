@@ -47,9 +47,10 @@ class BatchedSolver:
                  mlp: Any = None, time_f64: bool = True,
                  team_cells: int | None = None, max_team: int = 32,
                  team_wait_s: float = 60.0, schedule: str = "step") -> None:
-        if scheme not in ("rusanov", "godunov"):
+        if scheme not in _lib.BATCH_FLUX:
             raise NotImplementedError(
-                f"the batched solver takes Burgers 'rusanov' or 'godunov', not {scheme!r}")
+                f"the batched solver takes Burgers 'rusanov', 'godunov' or 'eo', "
+                f"not {scheme!r}")
         if nghosts < 3:
             raise ValueError("WENO-JS5 needs nghosts >= 3")
         if schedule not in ("step", "tile"):
@@ -89,8 +90,7 @@ class BatchedSolver:
 
         self.desc = _lib.PsaBatchDesc()
         self.desc.dtype = _lib.dtype_code(dtype)
-        self.desc.flux = _lib.PSA_FLUX_RUSANOV if scheme == "rusanov" \
-            else _lib.PSA_FLUX_BURGERS_UPWIND
+        self.desc.flux = _lib.BATCH_FLUX[scheme]
         self.desc.g = nghosts
         self.desc.grid = int(grid)
         self.desc.ninst = int(ncells.size)

@@ -158,6 +158,10 @@ __device__ __forceinline__ T vjp_block(const VjpArgs<T> &p, const T *s_u,
       const T wR = aR > aL ? T(1) : (aR < aL ? T(0) : T(0.5));
       duR = dFda * wR * sgn(uR);
       duL = dFda * (T(1) - wR) * sgn(uL);
+    } else if constexpr (FLUX == PSA_FLUX_ENGQUIST_OSHER) {
+      // F = max(ur, 0)^2 / 2 + min(ul, 0)^2 / 2
+      dur = urL > T(0) ? urL : T(0);
+      dul = ulR < T(0) ? ulR : T(0);
     } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
       const T aavg = (urL + ulR) / T(2);
       if (aavg > T(0)) {
@@ -372,6 +376,8 @@ int launch_vjp(const psa_solver_desc *d, int rstage, const void *state,
     return launch_vjp_flux<T, PSA_FLUX_RUSANOV>(rstage, a, s);
   if (d->flux == PSA_FLUX_BURGERS_UPWIND)
     return launch_vjp_flux<T, PSA_FLUX_BURGERS_UPWIND>(rstage, a, s);
+  if (d->flux == PSA_FLUX_ENGQUIST_OSHER)
+    return launch_vjp_flux<T, PSA_FLUX_ENGQUIST_OSHER>(rstage, a, s);
   return launch_vjp_flux<T, PSA_FLUX_ADVECTION_UPWIND>(rstage, a, s);
 }
 
@@ -401,6 +407,9 @@ int launch_step_vjp(const psa_solver_desc *d, const void *x, const void *k1,
   } else if (d->flux == PSA_FLUX_BURGERS_UPWIND) {
     hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_BURGERS_UPWIND>), dim3(blocks),
                        dim3(kVjpBlock), 0, s, a);
+  } else if (d->flux == PSA_FLUX_ENGQUIST_OSHER) {
+    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ENGQUIST_OSHER>), dim3(blocks),
+                       dim3(kVjpBlock), 0, s, a);
   } else {
     hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ADVECTION_UPWIND>), dim3(blocks),
                        dim3(kVjpBlock), 0, s, a);
@@ -414,7 +423,10 @@ int check_vjp_desc(const psa_solver_desc *d) {
     return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported dtype");
   if (d->g < 3 || d->n < d->g)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "bad grid size");
-  if (d->flux < 0 || d->flux > 2)
+  if (d->flux == PSA_FLUX_LAX_FRIEDRICHS)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the adjoint does not support the Lax-Friedrichs flux");
+  if (d->flux < 0 || d->flux > 3)
     return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported flux");
   if (d->flux == PSA_FLUX_RUSANOV && d->nu != nullptr)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,

@@ -419,16 +419,22 @@ template <typename T, int FLUX, bool WHOLE>
 int launch_batch_tiles(const BatchArgs<T> &a, int block, int cpt, int grid,
                        int max_team, long long nteams, hipStream_t s) {
   if (a.mlp != nullptr) {
-    if (block == 256 && cpt == 4)
-      return launch_batch_cfg<T, FLUX, 256, 4, true, WHOLE>(a, grid, max_team, nteams, s);
-    if (block == 512 && cpt == 2)
-      return launch_batch_cfg<T, FLUX, 512, 2, true, WHOLE>(a, grid, max_team, nteams, s);
-    if (block == 1024 && cpt == 2)
-      return launch_batch_cfg<T, FLUX, 1024, 2, true, WHOLE>(a, grid, max_team, nteams, s);
-    return set_error(PSA_ERR_INVALID_ARGUMENT,
-                     "unsupported batch tile configuration for learned "
-                     "weights block=" + std::to_string(block) +
-                         " cells_per_thread=" + std::to_string(cpt));
+    if constexpr (FLUX == PSA_FLUX_ENGQUIST_OSHER) {
+      return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                       "learned weights are implemented for the Burgers "
+                       "Rusanov and Godunov fluxes only");
+    } else {
+      if (block == 256 && cpt == 4)
+        return launch_batch_cfg<T, FLUX, 256, 4, true, WHOLE>(a, grid, max_team, nteams, s);
+      if (block == 512 && cpt == 2)
+        return launch_batch_cfg<T, FLUX, 512, 2, true, WHOLE>(a, grid, max_team, nteams, s);
+      if (block == 1024 && cpt == 2)
+        return launch_batch_cfg<T, FLUX, 1024, 2, true, WHOLE>(a, grid, max_team, nteams, s);
+      return set_error(PSA_ERR_INVALID_ARGUMENT,
+                       "unsupported batch tile configuration for learned "
+                       "weights block=" + std::to_string(block) +
+                           " cells_per_thread=" + std::to_string(cpt));
+    }
   }
   if (block == 256 && cpt == 4)
     return launch_batch_cfg<T, FLUX, 256, 4, false, WHOLE>(a, grid, max_team, nteams, s);
@@ -502,6 +508,10 @@ int launch_batch(const psa_batch_desc *d, void *u, void *k1, void *k2,
   if (d->flux == PSA_FLUX_RUSANOV)
     return launch_batch_flux<T, PSA_FLUX_RUSANOV>(a, whole, block, cpt, d->grid,
                                                   max_team, d->ninst, s);
+  if (d->flux == PSA_FLUX_ENGQUIST_OSHER)
+    return launch_batch_flux<T, PSA_FLUX_ENGQUIST_OSHER>(a, whole, block, cpt,
+                                                         d->grid, max_team,
+                                                         d->ninst, s);
   return launch_batch_flux<T, PSA_FLUX_BURGERS_UPWIND>(a, whole, block, cpt, d->grid,
                                                        max_team, d->ninst, s);
 }
@@ -518,9 +528,11 @@ extern "C" int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1,
   if (desc == nullptr) return set_error(PSA_ERR_INVALID_ARGUMENT, "null desc");
   if (desc->dtype != PSA_FLOAT64 && desc->dtype != PSA_FLOAT32)
     return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported dtype");
-  if (desc->flux != PSA_FLUX_RUSANOV && desc->flux != PSA_FLUX_BURGERS_UPWIND)
+  if (desc->flux != PSA_FLUX_RUSANOV && desc->flux != PSA_FLUX_BURGERS_UPWIND &&
+      desc->flux != PSA_FLUX_ENGQUIST_OSHER)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
-                     "the batched solver takes Burgers Rusanov or Godunov");
+                     "the batched solver takes Burgers Rusanov, Godunov or "
+                     "Engquist-Osher");
   if (desc->g < 3)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "needs at least 3 ghost cells");
   if (desc->ninst < 0 || desc->max_steps < 0 || !(desc->eps >= 0.0))

@@ -130,6 +130,12 @@ __global__ void k_numerical_flux(const T *__restrict__ u,
   if constexpr (FLUX == PSA_FLUX_RUSANOV) {
     const T nuj = nu != nullptr ? nu[j - 1] : T(1);
     F = flux_rusanov(urL, ulR, v[2], v[3], nuj);
+  } else if constexpr (FLUX == PSA_FLUX_LAX_FRIEDRICHS) {
+    // vel is the device scalar max|a|
+    const T nuj = nu != nullptr ? nu[j - 1] : T(1);
+    F = flux_lax_friedrichs(urL, ulR, *vel, nuj);
+  } else if constexpr (FLUX == PSA_FLUX_ENGQUIST_OSHER) {
+    F = flux_engquist_osher(urL, ulR);
   } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
     F = flux_burgers_upwind(urL, ulR);
   } else {
@@ -201,6 +207,8 @@ __global__ void k_control_init(psa_control *ctrl, double t, double tfinal,
   ctrl->umax_bits = abs_bits(static_cast<T>(speed_const));
   for (int k = 0; k < PSA_UMAX_SLOTS; ++k) {
     ctrl->umax_slot[k][0] = 0ull;
+    ctrl->umax_slot[k][1] = 0ull;
+    ctrl->umax_slot[k][2] = 0ull;
     ctrl->umax_slot[k][PSA_UMAX_STEP_WORD] = 0ull;
   }
   ctrl->umax_slot[0][0] = ctrl->umax_bits;
@@ -213,13 +221,14 @@ __global__ void k_control_init(psa_control *ctrl, double t, double tfinal,
 }
 
 // Decision of stage 1, evaluated identically by every workgroup.
-// max of the PSA_UMAX_SLOTS partials: lane k of every wave reads slot k and
-// the wave reduces them, so each wave holds the result without a barrier.
+// max of the PSA_UMAX_SLOTS partials in word `word`: lane k of every wave
+// reads slot k and the wave reduces them, so each wave holds the result
+// without a barrier.
 __device__ __forceinline__ unsigned long long gather_umax(
-    const psa_control *ctrl) {
+    const psa_control *ctrl, int word = 0) {
   static_assert(PSA_UMAX_SLOTS == 64, "one slot per lane");
   const int lane = threadIdx.x & 63;
-  unsigned long long v = ctrl->umax_slot[lane][0];
+  unsigned long long v = ctrl->umax_slot[lane][word];
   for (int off = 32; off > 0; off >>= 1) {
     const unsigned long long o = __shfl_xor(v, off, 64);
     v = o > v ? o : v;
@@ -270,11 +279,18 @@ __global__ void k_copy_if_inactive(const T *__restrict__ src,
 // Step control at the start of a stage launch (timestepping.py:89-152 on
 // the device): stage 1 decides whether the step is taken and publishes dt,
 // stage 2 advances t and m and clears the max|u| slots, stage 3 reads dt.
+// Lax-Friedrichs (scalar.py:276-277): speed is max|u| of the stage input,
+// from word 0 (u^n), 1 (k1) or 2 (k2) of the slots, or from p.speed without
+// a control block; stage 1 clears word 2 for stage 2 and stage 3 clears
+// word 1 for the next step's stage 1 (each after its last reader is done).
 // Returns false when the step is not taken (the workgroup then exits).
-template <typename T, int STAGE>
-__device__ __forceinline__ bool stage_control(const StageArgs<T> &p, T &dt) {
+template <typename T, int STAGE, int FLUX>
+__device__ __forceinline__ bool stage_control(const StageArgs<T> &p, T &dt,
+                                              T &speed) {
+  constexpr bool LF = FLUX == PSA_FLUX_LAX_FRIEDRICHS;
   const int tid = threadIdx.x;
   dt = T(0);
+  speed = T(0);
   if constexpr (STAGE > 0) {
     if (p.ctrl != nullptr) {
       if constexpr (STAGE == 1) {
@@ -288,9 +304,15 @@ __device__ __forceinline__ bool stage_control(const StageArgs<T> &p, T &dt) {
           if (active) p.ctrl->dt = static_cast<double>(dt);
         }
         if (!active) return false;
+        if constexpr (LF) {
+          speed = from_bits<T>(umax);
+          if (blockIdx.x == 0 && tid < PSA_UMAX_SLOTS)
+            p.ctrl->umax_slot[tid][2] = 0ull;
+        }
       } else {
         if (p.ctrl->active == 0) return false;
         dt = static_cast<T>(p.ctrl->dt);
+        if constexpr (LF) speed = from_bits<T>(gather_umax(p.ctrl, STAGE - 1));
         if constexpr (STAGE == 2) {
           if (blockIdx.x == 0 && tid == 0) {
             p.ctrl->t = static_cast<double>(static_cast<T>(p.ctrl->t) + dt);
@@ -301,14 +323,46 @@ __device__ __forceinline__ bool stage_control(const StageArgs<T> &p, T &dt) {
               tid < PSA_UMAX_SLOTS) {
             p.ctrl->umax_slot[tid][0] = 0ull;
           }
+        } else if constexpr (LF) {
+          // stage 2 has read word 1
+          if (blockIdx.x == 0 && tid < PSA_UMAX_SLOTS)
+            p.ctrl->umax_slot[tid][1] = 0ull;
         }
       }
     } else {
       dt = *p.dt_ptr;
+      if constexpr (LF) speed = *p.speed;
     }
+  } else if constexpr (LF) {
+    speed = *p.speed;
   }
 
   return true;
+}
+
+// Where a stage launch reduces max|u| of its output (-1: nowhere): word 0
+// for the next step's dt (stage 3), words 1 and 2 for the Lax-Friedrichs
+// speed of the next stage.
+template <int STAGE, int FLUX>
+constexpr int stage_reduce_word() {
+  if constexpr (STAGE == 3) return 0;
+  if constexpr (FLUX == PSA_FLUX_LAX_FRIEDRICHS && STAGE > 0) return STAGE;
+  return -1;
+}
+
+// The workgroup's part of that reduction (after its last tile).
+template <typename T, int STAGE, int FLUX, int BLOCK>
+__device__ __forceinline__ void stage_reduce(const StageArgs<T> &p,
+                                             unsigned long long vmax,
+                                             unsigned long long *red) {
+  constexpr int word = stage_reduce_word<STAGE, FLUX>();
+  if constexpr (word >= 0) {
+    if (p.ctrl != nullptr && p.ctrl->speed_from_state) {
+      vmax = block_max_bits<BLOCK>(vmax, red);
+      if (threadIdx.x == 0)
+        atomicMax(&p.ctrl->umax_slot[blockIdx.x % PSA_UMAX_SLOTS][word], vmax);
+    }
+  }
 }
 
 template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT,
@@ -320,23 +374,19 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   const int tid = threadIdx.x;
   const int64_t ntiles = (p.m + TILE - 1) / TILE;
 
-  T dt;
-  if (!stage_control<T, STAGE>(p, dt)) return;
+  T dt, speed;
+  if (!stage_control<T, STAGE, FLUX>(p, dt, speed)) return;
+  StageArgs<T> pl = p;
+  if constexpr (FLUX == PSA_FLUX_LAX_FRIEDRICHS) pl.vconst = speed;
 
   unsigned long long vmax = 0;
   const int64_t tile = blockIdx.x;  // the grid has one workgroup per tile
   if (tile < ntiles) {
-    stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT, LEARNED>(p, tile * TILE, dt,
+    stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT, LEARNED>(pl, tile * TILE, dt,
                                                          sm, vmax, q);
   }
-
-  if constexpr (STAGE == 3) {
-    if (p.ctrl != nullptr && p.ctrl->speed_from_state) {
-      vmax = block_max_bits<BLOCK>(vmax, sm.red);
-      if (tid == 0)
-        atomicMax(&p.ctrl->umax_slot[blockIdx.x % PSA_UMAX_SLOTS][0], vmax);
-    }
-  }
+  (void)tid;
+  stage_reduce<T, STAGE, FLUX, BLOCK>(p, vmax, sm.red);
 }
 
 // Pipelined stage kernel: a persistent grid (the workgroups the device holds
@@ -357,8 +407,10 @@ __global__ __launch_bounds__(BLOCK) void k_stage_pipe(StageArgs<T> p) {
   const int tid = threadIdx.x;
   const int64_t ntiles = (p.m + TILE - 1) / TILE;
 
-  T dt;
-  if (!stage_control<T, STAGE>(p, dt)) return;
+  T dt, speed;
+  if (!stage_control<T, STAGE, FLUX>(p, dt, speed)) return;
+  StageArgs<T> pl = p;
+  if constexpr (FLUX == PSA_FLUX_LAX_FRIEDRICHS) pl.vconst = speed;
 
   auto fast_tile = [&](int64_t t) {
     const int64_t S = t * TILE;
@@ -379,7 +431,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage_pipe(StageArgs<T> p) {
                   StageSmem<T, VEL, BLOCK, CPT> &nxt) {
     tile_sync<true>();
     issue(t + gridDim.x, nxt);
-    stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT, false, true, true>(p, t * TILE, dt,
+    stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT, false, true, true>(pl, t * TILE, dt,
                                                                  cur, vmax);
     // the copies into nxt are older than this tile's stores: on the fast path
     // (a known number of stores per wave) only the stores may stay in flight
@@ -402,14 +454,8 @@ __global__ __launch_bounds__(BLOCK) void k_stage_pipe(StageArgs<T> p) {
     step(t, smB, smA);
     t += gridDim.x;
   }
-
-  if constexpr (STAGE == 3) {
-    if (p.ctrl != nullptr && p.ctrl->speed_from_state) {
-      vmax = block_max_bits<BLOCK>(vmax, smA.red);
-      if (tid == 0)
-        atomicMax(&p.ctrl->umax_slot[blockIdx.x % PSA_UMAX_SLOTS][0], vmax);
-    }
-  }
+  (void)tid;
+  stage_reduce<T, STAGE, FLUX, BLOCK>(p, vmax, smA.red);
 }
 
 // ---------------------------------------------------------------------------
@@ -703,6 +749,8 @@ int launch_step_tiles(const StageArgs<T> &a, TileConfig cfg,
 // Whether the whole-step kernel runs the steps of this descriptor.
 inline bool step_applies(const psa_solver_desc *d, TileConfig cfg, int vel) {
   if (!cfg.step || vel == 2 || d->nu != nullptr) return false;
+  // the stages of Lax-Friedrichs need the global max|u| of the stage before
+  if (d->flux == PSA_FLUX_LAX_FRIEDRICHS) return false;
   if (d->flux == PSA_FLUX_ADVECTION_UPWIND && d->mlp != nullptr) return false;
   return cfg.block == 256 && (cfg.cpt == 4 || cfg.cpt == 2);
 }
@@ -719,6 +767,8 @@ int launch_step(const psa_solver_desc *d, int vel, const StageArgs<T> &a,
     return launch_step_tiles<T, PSA_FLUX_RUSANOV, 0, false>(a, cfg, stream);
   if (d->flux == PSA_FLUX_BURGERS_UPWIND)
     return launch_step_tiles<T, PSA_FLUX_BURGERS_UPWIND, 0, false>(a, cfg, stream);
+  if (d->flux == PSA_FLUX_ENGQUIST_OSHER)
+    return launch_step_tiles<T, PSA_FLUX_ENGQUIST_OSHER, 0, false>(a, cfg, stream);
   (void)vel;
   return launch_step_tiles<T, PSA_FLUX_ADVECTION_UPWIND, 1, false>(a, cfg, stream);
 }
@@ -754,6 +804,12 @@ int launch_stage(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
   if (flux == PSA_FLUX_BURGERS_UPWIND)
     return launch_stage_tiles<T, PSA_FLUX_BURGERS_UPWIND, 0, STAGE>(a, cfg,
                                                                     stream);
+  if (flux == PSA_FLUX_ENGQUIST_OSHER)
+    return launch_stage_tiles<T, PSA_FLUX_ENGQUIST_OSHER, 0, STAGE>(a, cfg,
+                                                                    stream);
+  if (flux == PSA_FLUX_LAX_FRIEDRICHS)
+    return launch_stage_tiles<T, PSA_FLUX_LAX_FRIEDRICHS, 0, STAGE>(a, cfg,
+                                                                    stream);
   if (flux == PSA_FLUX_ADVECTION_UPWIND) {
     if (vel == 1)
       return launch_stage_tiles<T, PSA_FLUX_ADVECTION_UPWIND, 1, STAGE>(
@@ -785,12 +841,14 @@ int check_desc(const psa_solver_desc *d) {
       d->velocity == nullptr)
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "advection needs a velocity array or a constant");
-  if (d->flux < 0 || d->flux > 2)
+  if (d->flux < 0 || d->flux > 4)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "unsupported flux " + std::to_string(d->flux));
-  if (d->mlp != nullptr && d->flux == PSA_FLUX_ADVECTION_UPWIND)
+  if (d->mlp != nullptr && d->flux != PSA_FLUX_RUSANOV &&
+      d->flux != PSA_FLUX_BURGERS_UPWIND)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
-                     "learned weights are implemented for Burgers only");
+                     "learned weights are implemented for the Burgers Rusanov "
+                     "and Godunov fluxes only");
   if (d->mlp != nullptr && (!(d->mlp_tau >= 0.0) || !(d->mlp_sigma >= 0.0)))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "mlp_tau and mlp_sigma must be >= 0");
   return PSA_OK;
@@ -822,6 +880,10 @@ StageArgs<T> make_args(const psa_solver_desc *d) {
   a.eps = static_cast<T>(d->eps);
   a.vconst = static_cast<T>(d->velocity_const);
   a.vel = static_cast<const T *>(d->velocity);
+  if (d->flux == PSA_FLUX_LAX_FRIEDRICHS) {
+    a.vel = nullptr;
+    a.speed = static_cast<const T *>(d->velocity);
+  }
   a.nu = static_cast<const T *>(d->nu);
   a.aligned = 1;
   a.mlp = d->mlp;
@@ -975,6 +1037,14 @@ int numerical_flux_t(int flux, const void *u, const void *velocity,
     hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_BURGERS_UPWIND>),
                        dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e,
                        ff);
+  } else if (flux == PSA_FLUX_ENGQUIST_OSHER) {
+    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_ENGQUIST_OSHER>),
+                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e,
+                       ff);
+  } else if (flux == PSA_FLUX_LAX_FRIEDRICHS) {
+    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_LAX_FRIEDRICHS>),
+                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e,
+                       ff);
   } else {
     hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_ADVECTION_UPWIND>),
                        dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e,
@@ -1082,7 +1152,7 @@ int psa_numerical_flux(int dtype, int flux, const void *u,
                        int64_t m, double eps, void *stream) {
   int st = check_dtype(dtype);
   if (st != PSA_OK) return st;
-  if (flux < 0 || flux > 2)
+  if (flux < 0 || flux > 4)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "unsupported flux " + std::to_string(flux));
   if (u == nullptr || fnum == nullptr || m < 2)
@@ -1090,6 +1160,9 @@ int psa_numerical_flux(int dtype, int flux, const void *u,
   if (flux == PSA_FLUX_ADVECTION_UPWIND && velocity == nullptr)
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "advection upwind flux needs a velocity array");
+  if (flux == PSA_FLUX_LAX_FRIEDRICHS && velocity == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "the Lax-Friedrichs flux needs the device scalar max|a|");
   auto s = static_cast<hipStream_t>(stream);
   if (dtype == PSA_FLOAT64)
     return numerical_flux_t<double>(flux, u, velocity, nu, fnum, m, eps, s);
@@ -1116,6 +1189,9 @@ int psa_apply_operator(int dtype, int flux, const void *u,
   if (u == out)
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "apply_operator cannot write into its input");
+  if (flux == PSA_FLUX_LAX_FRIEDRICHS && velocity == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "the Lax-Friedrichs operator needs the device scalar max|u|");
   auto s = static_cast<hipStream_t>(stream);
   const TileConfig cfg = default_tiles(dtype);
   const int vel = vel_mode(&d);
@@ -1166,7 +1242,11 @@ int psa_control_init(const psa_solver_desc *desc, const void *u,
     return set_error(PSA_ERR_INVALID_ARGUMENT, "bad dt mode");
   auto s = static_cast<hipStream_t>(stream);
   const bool advection = desc->flux == PSA_FLUX_ADVECTION_UPWIND;
-  const int speed_from_state = (dt_mode == PSA_DT_CFL && !advection) ? 1 : 0;
+  // Lax-Friedrichs reads max|u^n| as the flux speed of stage 1 also with a
+  // fixed dt
+  const bool lf = desc->flux == PSA_FLUX_LAX_FRIEDRICHS;
+  const int speed_from_state =
+      ((dt_mode == PSA_DT_CFL && !advection) || lf) ? 1 : 0;
   auto *ctrl = static_cast<psa_control *>(control);
   // a constant advection velocity without an array: max|a| = |a|
   const bool const_speed = advection && desc->velocity == nullptr;
@@ -1182,7 +1262,7 @@ int psa_control_init(const psa_solver_desc *desc, const void *u,
   }
   st = check_launch("control_init");
   if (st != PSA_OK) return st;
-  if (dt_mode != PSA_DT_CFL || const_speed) return PSA_OK;
+  if ((dt_mode != PSA_DT_CFL && !lf) || const_speed) return PSA_OK;
 
   const int64_t m = desc->n + 2 * static_cast<int64_t>(desc->g);
   const void *src = advection ? desc->velocity : u;
@@ -1229,6 +1309,11 @@ int psa_ssprk33_stage(const psa_solver_desc *desc, int stage, const void *un,
   if (out == uk)
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "a stage cannot write into its stencil input");
+  if (desc->flux == PSA_FLUX_LAX_FRIEDRICHS &&
+      (stage == 0 || control == nullptr) && desc->velocity == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "Lax-Friedrichs without a control block needs the device "
+                     "scalar max|uk| in desc->velocity");
   auto s = static_cast<hipStream_t>(stream);
   const TileConfig cfg = tiles_of(desc);
   const int vel = vel_mode(desc);
@@ -1273,6 +1358,10 @@ int psa_ssprk33_advance(const psa_solver_desc *desc, const void *u,
   if (k1 == k2 || k1 == u || k2 == u || k1 == out || k2 == out)
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "stage buffers must be distinct from the state");
+  if (desc->flux == PSA_FLUX_LAX_FRIEDRICHS)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "Lax-Friedrichs steps need a control block for the stage "
+                     "maxima (psa_ssprk33_steps)");
   auto s = static_cast<hipStream_t>(stream);
   if (desc->dtype == PSA_FLOAT64)
     return run_steps<double>(desc, u, out, k1, k2, nullptr, dt, 1, s);

@@ -94,7 +94,10 @@ struct StageArgs {
   T dx;
   T inv_dx;
   T eps;
-  T vconst;
+  T vconst;         // advection: the constant velocity; Lax-Friedrichs: the
+                    //   global flux speed of the stage (set by the kernel)
+  const T *speed;   // Lax-Friedrichs without a control block: the device
+                    //   scalar max|a| of the stage input
   const float *mlp;  // learned weights (LEARNED kernels)
   T tau;             // their shock threshold
   T sigma;           // their smooth threshold
@@ -383,12 +386,17 @@ __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
   if constexpr (LEARNED) learned_cells<T, CPT>(src, w, regime, ul, ur, q);
 }
 
-// Face flux of the stage kernels for Rusanov with nu = 1 (doubled, see
+// Face flux of the stage kernels for Rusanov, Lax-Friedrichs (global speed
+// vconst) with nu = 1 and Engquist-Osher (all three doubled, see
 // stage_lambda), Godunov, or advection with a constant velocity.
 template <typename T, int FLUX>
 __device__ __forceinline__ T stage_flux(T urL, T ulR, T uL, T uR, T vconst) {
   if constexpr (FLUX == PSA_FLUX_RUSANOV) {
     return flux_rusanov_nu1_x2(urL, ulR, uL, uR);
+  } else if constexpr (FLUX == PSA_FLUX_LAX_FRIEDRICHS) {
+    return flux_lax_friedrichs_nu1_x2(urL, ulR, vconst);
+  } else if constexpr (FLUX == PSA_FLUX_ENGQUIST_OSHER) {
+    return flux_engquist_osher_x2(urL, ulR);
   } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
     return flux_burgers_upwind(urL, ulR);
   } else {
@@ -400,11 +408,12 @@ __device__ __forceinline__ T stage_flux(T urL, T ulR, T uL, T uR, T vconst) {
 
 // u + dt L(u) = u - lam (F(i+1/2) - F(i-1/2)) with lam = dt / dx, times the
 // constant velocity for advection (L = -a dF / dx), times 1/2 for the
-// doubled Rusanov flux of stage_flux.
+// doubled fluxes of stage_flux.
 template <typename T, int FLUX>
 __device__ __forceinline__ T stage_lambda(const StageArgs<T> &p, T dt) {
   const T lam = dt * p.inv_dx;
-  if constexpr (FLUX == PSA_FLUX_RUSANOV) {
+  if constexpr (FLUX == PSA_FLUX_RUSANOV || FLUX == PSA_FLUX_LAX_FRIEDRICHS ||
+                FLUX == PSA_FLUX_ENGQUIST_OSHER) {
     return T(0.5) * lam;
   } else if constexpr (FLUX == PSA_FLUX_ADVECTION_UPWIND) {
     return lam * p.vconst;
@@ -463,9 +472,10 @@ struct StageSmem {
 // indicators), neighbouring threads exchange one right and one left state
 // through LDS, each thread evaluates its CPT + 1 faces, applies the
 // conservative difference and the stage combination, and the results leave
-// through LDS as coalesced stores. For STAGE 3, vmax accumulates the bit
-// pattern of max |u| of the cells written. Ends with the LDS free for reuse
-// only after the caller's next barrier.
+// through LDS as coalesced stores. For STAGE 3 (and stages 1 and 2 of
+// Lax-Friedrichs), vmax accumulates the bit pattern of max |u| of the cells
+// written. Ends with the LDS free for reuse only after the caller's next
+// barrier.
 template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT,
           bool LEARNED = false, bool PRELOADED = false, bool ASYNC = false>
 __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
@@ -478,6 +488,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   constexpr int TILE = BLOCK * CPT;
   constexpr int HALO = 4;
   constexpr int VN = Vec16<T>::n;
+  // max|u| of the output: the next step's dt (stage 3) and the next stage's
+  // Lax-Friedrichs speed (stages 1 and 2)
+  constexpr bool kReducesOutput =
+      STAGE == 3 || (STAGE > 0 && FLUX == PSA_FLUX_LAX_FRIEDRICHS);
   const int tid = threadIdx.x;
   const int64_t m = p.m;
   const int64_t n = p.n;
@@ -590,7 +604,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
 
   // ---- faces l0 - 1/2 .. l0 + CPT - 1/2
   T F[CPT + 1];
-  const bool has_nu = FLUX == PSA_FLUX_RUSANOV && p.nu != nullptr;
+  const bool has_nu = (FLUX == PSA_FLUX_RUSANOV ||
+                       FLUX == PSA_FLUX_LAX_FRIEDRICHS) && p.nu != nullptr;
   // the stage kernels' flux forms (stage_flux) where they apply
   const bool plain = STAGE > 0 && VEL != 2 && !has_nu;
 #pragma unroll
@@ -612,6 +627,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
       } else {
         Fk = flux_rusanov_nu1(urL, ulR, w[k + 1], w[k + 2]);
       }
+    } else if constexpr (FLUX == PSA_FLUX_LAX_FRIEDRICHS) {
+      T nuk = T(1);
+      if (has_nu) {
+        const int64_t gm = S + l0 + k;
+        nuk = (gm >= 1 && gm <= m - 1) ? p.nu[gm - 1] : T(1);
+      }
+      Fk = flux_lax_friedrichs(urL, ulR, p.vconst, nuk);
+    } else if constexpr (FLUX == PSA_FLUX_ENGQUIST_OSHER) {
+      Fk = flux_engquist_osher(urL, ulR);
     } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
       Fk = flux_burgers_upwind(urL, ulR);
     } else {
@@ -667,7 +691,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
         co.r[k] = load16(sm.io + c * VN);
     }
     co.store(p.out + S, tid);
-    if constexpr (STAGE == 3) {
+    if constexpr (kReducesOutput) {
 #pragma unroll
       for (int k = 0; k < co.CH; ++k) {
         const int c = tid + k * BLOCK;
@@ -686,7 +710,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
           p.out[gm] = v;
           if (gm < 2 * g) p.out[gm + n] = v;
           if (gm >= m - 2 * g) p.out[gm - n] = v;
-          if constexpr (STAGE == 3) {
+          if constexpr (kReducesOutput) {
             const unsigned long long b = abs_bits(v);
             vmax = b > vmax ? b : vmax;
           }

@@ -245,6 +245,42 @@ __device__ __forceinline__ T flux_rusanov_nu1_x2(T urL, T ulR, T uL, T uR) {
   return T(0.5) * fma(ulR, ulR, urL * urL) - a * (ulR - urL);
 }
 
+// scalar.py:276-277 with scalar.py:238-248: the Rusanov flux with the global
+// speed a = max|a_i| for every face (Lax-Friedrichs).
+template <typename T>
+__device__ __forceinline__ T flux_lax_friedrichs(T urL, T ulR, T a, T nu) {
+  const T fl = (ulR * ulR) / T(2);
+  const T fr = (urL * urL) / T(2);
+  return T(0.5) * (fl + fr) - ((T(0.5) * a) * nu) * (ulR - urL);
+}
+
+// Twice flux_lax_friedrichs with nu = 1, 0.5 (ul^2 + ur^2) - a (ul - ur)
+// (the form of flux_rusanov_nu1_x2).
+template <typename T>
+__device__ __forceinline__ T flux_lax_friedrichs_nu1_x2(T urL, T ulR, T a) {
+  return T(0.5) * fma(ulR, ulR, urL * urL) - a * (ulR - urL);
+}
+
+// scalar.py:309-319 with the Burgers flux and omega = 0
+// (burgers/schemes.py:184): F = f(max(ur_l, 0)) + f(min(ul_r, 0)) - f(0),
+// f(0) = 0. The comparisons pass a NaN state through, as jnp.maximum and
+// jnp.minimum do.
+template <typename T>
+__device__ __forceinline__ T flux_engquist_osher(T urL, T ulR) {
+  const T p = urL < T(0) ? T(0) : urL;
+  const T q = ulR > T(0) ? T(0) : ulR;
+  return (p * p) / T(2) + (q * q) / T(2);
+}
+
+// Twice flux_engquist_osher, p^2 + q^2: the stage kernels fold the factor
+// 1/2 into the update.
+template <typename T>
+__device__ __forceinline__ T flux_engquist_osher_x2(T urL, T ulR) {
+  const T p = urL < T(0) ? T(0) : urL;
+  const T q = ulR > T(0) ? T(0) : ulR;
+  return fma(p, p, q * q);
+}
+
 // scalar.py:125-130: aavg = (ar + al) / 2 with a = u reconstructed the same
 // way as u, F = fr if aavg > 0 else fl.
 template <typename T>

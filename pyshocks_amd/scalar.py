@@ -1,7 +1,9 @@
 """Scalar numerical fluxes and the periodic boundary of the hot path.
 
 Mirrors ``scalar.py`` of pyshocks: ``scalar_flux_upwind`` (``:91-132``),
-``scalar_flux_rusanov`` (``:192-249``) and ``PeriodicBoundary`` /
+``scalar_flux_rusanov`` (``:192-249``), ``scalar_flux_lax_friedrichs``
+(``:258-278``), ``scalar_flux_engquist_osher`` (``:287-322``) and
+``PeriodicBoundary`` /
 ``apply_boundary`` / ``evaluate_boundary`` (``:521-547``). The fluxes run in
 the library's face kernel, which reconstructs both states of every face with
 WENO-JS5 and evaluates the Burgers flux ``u^2 / 2`` inline, so they accept
@@ -106,6 +108,44 @@ def scalar_flux_rusanov(scheme: ConservationLawScheme, grid: Grid, bc: BoundaryT
     if rec is not None:
         return _learned_flux(rec, _lib.PSA_FLUX_RUSANOV, u, rusanov_nu(grid, alpha))
     return _numerical_flux(_lib.PSA_FLUX_RUSANOV, u, None, rusanov_nu(grid, alpha),
+                           _eps_of(scheme))
+
+
+def scalar_flux_lax_friedrichs(scheme: ConservationLawScheme, grid: Grid,
+                               bc: BoundaryType, t: Any, a: torch.Tensor,
+                               u: torch.Tensor, alpha: float = 1.0) -> torch.Tensor:
+    """Rusanov flux with the global speed ``max|a|`` over the whole array
+    (``scalar.py:276-277``); ``a`` may be any array of ``u``'s shape."""
+    from pyshocks_amd.reductions import max_abs_interior
+
+    assert u.shape[0] == grid.x.numel()
+    _check_burgers(scheme)
+    if _learned(scheme) is not None:
+        raise NotImplementedError(
+            "learned weights are implemented for the Rusanov and Godunov fluxes")
+    _lib.require_device(a)
+    if a.shape != u.shape or a.dtype != u.dtype:
+        raise ValueError("the speed array must have the shape and dtype of u")
+    amax = max_abs_interior(a.contiguous(), 0)
+    return _numerical_flux(_lib.PSA_FLUX_LAX_FRIEDRICHS, u, amax,
+                           rusanov_nu(grid, alpha), _eps_of(scheme))
+
+
+def scalar_flux_engquist_osher(scheme: ConservationLawScheme, grid: Grid,
+                               bc: BoundaryType, t: Any, a: torch.Tensor,
+                               u: torch.Tensor, omega: float = 0.0) -> torch.Tensor:
+    """``f(max(u_l, omega)) + f(min(u_r, omega)) - f(omega)`` on the WENO-JS
+    face states (``scalar.py:309-319``); the Burgers flux has its minimum at
+    ``omega = 0``, the only value the kernel takes. ``a`` is unused, as in the
+    reference's WENO reconstruction."""
+    assert u.shape[0] == grid.x.numel()
+    _check_burgers(scheme)
+    if omega != 0.0:
+        raise NotImplementedError("the GPU Engquist-Osher flux takes omega = 0")
+    if _learned(scheme) is not None:
+        raise NotImplementedError(
+            "learned weights are implemented for the Rusanov and Godunov fluxes")
+    return _numerical_flux(_lib.PSA_FLUX_ENGQUIST_OSHER, u, None, None,
                            _eps_of(scheme))
 
 

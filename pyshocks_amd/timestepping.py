@@ -202,12 +202,14 @@ class FusedPlan:
     def uses_step_kernel(self) -> bool:
         """Whether :meth:`steps` runs one whole-step kernel per step (the
         library's rule: schedule default or step, a constant or no velocity,
-        no Rusanov ``nu``, tiles (256, 4) or (256, 2); by default 2 cells per
-        thread below 2^20 cells, 4 above)."""
+        no Rusanov ``nu``, not Lax-Friedrichs, tiles (256, 4) or (256, 2); by
+        default 2 cells per thread below 2^20 cells, 4 above)."""
         d = self.desc
         if d.schedule not in (_lib.PSA_SCHEDULE_DEFAULT, _lib.PSA_SCHEDULE_STEP):
             return False
         if d.nu or (d.flux == _lib.PSA_FLUX_ADVECTION_UPWIND and not d.velocity_is_const):
+            return False
+        if d.flux == _lib.PSA_FLUX_LAX_FRIEDRICHS:
             return False
         block = d.block if d.block > 0 else 256
         cpt = d.cells_per_thread if d.cells_per_thread > 0 else (2 if d.n < (1 << 20) else 4)
@@ -240,20 +242,24 @@ def _scheme_plan(source: Any) -> tuple[_lib.PsaSolverDesc, tuple[Any, ...]] | No
     from pyshocks_amd.reconstruction import LearnedWENOJS53
 
     if isinstance(scheme.rec, LearnedWENOJS53):
-        if not isinstance(scheme, (burgers.Rusanov, burgers.Godunov)):
+        if not isinstance(scheme, (burgers.Rusanov, burgers.Godunov)) or \
+                isinstance(scheme, burgers.LaxFriedrichs):
             return None
         desc.mlp = scheme.rec.weights.ptr
         desc.mlp_tau = float(scheme.rec.weights.tau)
         desc.mlp_sigma = float(scheme.rec.weights.sigma)
         keep = (scheme.rec.weights,)
     if isinstance(scheme, burgers.Rusanov):
-        desc.flux = _lib.PSA_FLUX_RUSANOV
+        # and its subclass LaxFriedrichs
+        desc.flux = burgers.flux_kind(scheme)
         nu = rusanov_nu(grid, scheme.alpha)
         if nu is not None:
             desc.nu = nu.data_ptr()
             keep = keep + (nu,)
     elif isinstance(scheme, burgers.Godunov):
         desc.flux = _lib.PSA_FLUX_BURGERS_UPWIND
+    elif isinstance(scheme, burgers.EngquistOsher):
+        desc.flux = _lib.PSA_FLUX_ENGQUIST_OSHER
     elif isinstance(scheme, advection.Godunov):
         vel = scheme.velocity
         if vel is None:
@@ -302,8 +308,10 @@ def fused_plan(stepper: Stepper, *,
         dx_min = npt(grid.h)
         scheme = source.scheme
         if isinstance(scheme, burgers.Rusanov):
+            # burgers/schemes.py:121-127, inherited by LaxFriedrichs
             cfl = npt(0.5 * dx_min ** npt(2 - scheme.alpha))
-        elif isinstance(scheme, burgers.Godunov):
+        elif isinstance(scheme, (burgers.Godunov, burgers.EngquistOsher)):
+            # burgers/schemes.py:42-49
             cfl = npt(0.5 * dx_min)
         elif isinstance(scheme, advection.Godunov):
             cfl = dx_min
@@ -489,6 +497,10 @@ def _advance_forward_euler(stepper: ForwardEuler, dt: Any, t: Any,
 @advance.register(SSPRK33)
 def _advance_ssprk33(stepper: SSPRK33, dt: Any, t: Any, u: torch.Tensor) -> torch.Tensor:
     sp = _scheme_plan(stepper.source)
+    # Lax-Friedrichs stages need a control block for their speeds: the
+    # operator path below computes them per stage
+    if sp is not None and sp[0].flux == _lib.PSA_FLUX_LAX_FRIEDRICHS:
+        sp = None
     if sp is not None and isinstance(u, torch.Tensor) and u.is_cuda:
         desc, keep = sp
         plan = FusedPlan(desc=desc, dtype=u.dtype, device=u.device, m=u.numel(),

@@ -11,7 +11,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 RTOL = 1.0e-12
-KINDS = ["rusanov", "godunov", "advection"]
+KINDS = ["rusanov", "godunov", "eo", "advection"]
 
 
 @pytest.fixture(scope="module")
@@ -48,6 +48,10 @@ def _source(torch, kind: str, n: int, velocity: float = 1.0):
         sch = burgers.Rusanov(rec=rec)
     elif kind == "godunov":
         sch = burgers.Godunov(rec=rec)
+    elif kind == "eo":
+        sch = burgers.EngquistOsher(rec=rec)
+    elif kind == "lf":
+        sch = burgers.LaxFriedrichs(rec=rec)
     else:
         vel = torch.full((n + 6,), velocity, dtype=torch.float64, device="cuda")
         sch = advection.Godunov(rec=rec, velocity=vel)

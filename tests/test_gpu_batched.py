@@ -25,20 +25,21 @@ def torch_mod():
 
 def _single(torch, n: int, u0: np.ndarray, dtype, tfinal: float, scheme: str = "rusanov"):
     from pyshocks_amd import make_uniform_cell_grid
-    from pyshocks_amd.burgers import Godunov, Rusanov
+    from pyshocks_amd.burgers import EngquistOsher, Godunov, Rusanov
     from pyshocks_amd.reconstruction import WENOJS53
     from pyshocks_amd.scalar import PeriodicBoundary
     from pyshocks_amd.timestepping import evolve, make_ssprk33
 
     grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, dtype=dtype)
-    sch = (Rusanov if scheme == "rusanov" else Godunov)(rec=WENOJS53(eps=1e-6))
+    cls = {"rusanov": Rusanov, "godunov": Godunov, "eo": EngquistOsher}[scheme]
+    sch = cls(rec=WENOJS53(eps=1e-6))
     st = make_ssprk33(sch, grid, PeriodicBoundary(), theta=1.0)
     u = torch.tensor(u0, device="cuda").to(dtype)
     return evolve(st, u, tfinal=tfinal, tiles=(256, 4))
 
 
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
-@pytest.mark.parametrize("scheme", ["rusanov", "godunov"])
+@pytest.mark.parametrize("scheme", ["rusanov", "godunov", "eo"])
 def test_batched_equals_single_instance(torch_mod, dtype_name: str, scheme: str) -> None:
     torch = torch_mod
     from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
@@ -63,7 +64,8 @@ def test_batched_equals_single_instance(torch_mod, dtype_name: str, scheme: str)
         assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref)
 
 
-def test_batched_vs_oracle(torch_mod) -> None:
+@pytest.mark.parametrize("scheme", ["rusanov", "eo"])
+def test_batched_vs_oracle(torch_mod, scheme: str) -> None:
     torch = torch_mod
     from oracle import weno_np as W
     from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
@@ -71,12 +73,12 @@ def test_batched_vs_oracle(torch_mod) -> None:
     _, ics = random_instances(3, seed=4)
     ns = np.array([64, 200, 513])
     states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
-    solver = BatchedSolver(ns, dtype=torch.float64)
+    solver = BatchedSolver(ns, dtype=torch.float64, scheme=scheme)
     solver.set_states(states)
     res = solver.solve(0.3)
     for i, n in enumerate(ns):
         grid = W.make_uniform_cell_grid(0.0, 1.0, int(n), nghosts=3)
-        u, dts, t = W.solve(W.Scheme("rusanov", eps=1e-6), grid, states[i], theta=1.0,
+        u, dts, t = W.solve(W.Scheme(scheme, eps=1e-6), grid, states[i], theta=1.0,
                             tfinal=0.3)
         assert res.steps[i] == len(dts)
         got = solver.state(i).cpu().numpy()[3:-3]
@@ -205,7 +207,7 @@ def test_time_f64_reaches_tfinal(torch_mod) -> None:
 
 
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
-@pytest.mark.parametrize("scheme", ["rusanov", "godunov"])
+@pytest.mark.parametrize("scheme", ["rusanov", "godunov", "eo"])
 @pytest.mark.parametrize("team_cells", [None, 2048])
 def test_step_schedule_matches_stage_sweeps(torch_mod, dtype_name: str, scheme: str,
                                             team_cells) -> None:

@@ -1,0 +1,265 @@
+"""Lax-Friedrichs and Engquist-Osher fluxes on the GPU against the oracle.
+
+``scalar.py:258-322`` and ``burgers/schemes.py:136-196``: the face fluxes as
+given (no boundary), the whole-array speed of Lax-Friedrichs (ghost entries
+included, any speed array ``a``), ``alpha != 1``, solves through a shock
+with every step schedule, a fixed dt (Lax-Friedrichs still reduces
+``max|u|`` per stage), the generic host loop and the stage entry point with
+a caller-provided speed. The batched solver and the adjoint take
+Engquist-Osher in their own test files; Lax-Friedrichs has neither.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1.0e-12
+RTOL32 = 1.0e-5
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pyshocks_amd import _lib
+
+    _lib.load()
+    return torch
+
+
+def _dev(torch, a: np.ndarray, dtype=None):
+    return torch.tensor(np.asarray(a), device="cuda", dtype=dtype or torch.float64)
+
+
+def _host(t) -> np.ndarray:
+    return t.detach().double().cpu().numpy()
+
+
+def _rel(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _relmax(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _scheme(kind: str, alpha: float = 1.0):
+    from pyshocks_amd import burgers
+    from pyshocks_amd.reconstruction import WENOJS53
+
+    rec = WENOJS53(eps=1e-6)
+    if kind == "lf":
+        return burgers.LaxFriedrichs(rec=rec, alpha=alpha)
+    return burgers.EngquistOsher(rec=rec)
+
+
+def _stepper(kind: str, grid, alpha: float = 1.0, dt=None):
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import make_ssprk33
+
+    return make_ssprk33(_scheme(kind, alpha), grid, PeriodicBoundary(), theta=1.0,
+                        dt=dt)
+
+
+@pytest.mark.parametrize("kind", ["lf", "eo"])
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+@pytest.mark.parametrize("n", [5, 32, 1000])
+def test_numerical_flux_vs_oracle(torch_mod, kind: str, dtype_name: str, n: int) -> None:
+    """Face fluxes of u as given (no ghost fill); the largest |u| of the
+    Lax-Friedrichs case sits in a ghost entry."""
+    torch = torch_mod
+    from oracle import weno_np as W
+    from pyshocks_amd import make_uniform_cell_grid, numerical_flux
+    from pyshocks_amd.scalar import PeriodicBoundary
+
+    dtype = getattr(torch, dtype_name)
+    rng = np.random.default_rng(n)
+    u = rng.uniform(-1, 1, size=n + 6)
+    u[1] = -3.0
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, dtype=dtype)
+    ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    F = _host(numerical_flux(_scheme(kind), grid, PeriodicBoundary(), 0.0,
+                             _dev(torch, u, dtype)))
+    Fref = W.numerical_flux(W.Scheme(kind, eps=1e-6), ogrid, u)
+    assert F.shape == Fref.shape and F[0] == 0.0 and F[-1] == 0.0
+    if dtype == torch.float64:
+        assert _relmax(F, Fref) < RTOL64
+    else:
+        assert _rel(F, Fref) < RTOL32
+
+
+@pytest.mark.parametrize("alpha", [1.0, 0.9])
+def test_lax_friedrichs_speed_array_and_alpha(torch_mod, alpha: float) -> None:
+    """``scalar_flux_lax_friedrichs(scheme, grid, bc, t, a, u, alpha)`` with a
+    speed array other than u (only its max|a| enters) and the artificial
+    viscosity df^(alpha - 1)."""
+    torch = torch_mod
+    from oracle import weno_np as W
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.scalar import scalar_flux_lax_friedrichs
+
+    n = 300
+    rng = np.random.default_rng(4)
+    u = rng.uniform(-1, 1, size=n + 6)
+    a = rng.uniform(-2, 2, size=n + 6)
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    F = _host(scalar_flux_lax_friedrichs(_scheme("lf", alpha), grid, None, 0.0,
+                                         _dev(torch, a), _dev(torch, u), alpha=alpha))
+    Fref = W.scalar_flux_lax_friedrichs(ogrid, u, a, eps=1e-6, alpha=alpha)
+    assert _relmax(F, Fref) < RTOL64
+
+
+@pytest.mark.parametrize("kind", ["lf", "eo"])
+@pytest.mark.parametrize("alpha", [1.0, 0.9])
+def test_fused_c1_through_shock(torch_mod, kind: str, alpha: float) -> None:
+    """N = 256, u0 = 0.5 + sin(2 pi x), Courant 0.5, to t = 0.5 (past the
+    shock): every dt and the final state against the oracle's solve, for each
+    step schedule (Lax-Friedrichs has no whole-step kernel)."""
+    if kind == "eo" and alpha != 1.0:
+        pytest.skip("Engquist-Osher has no alpha")
+    torch = torch_mod
+    from oracle import weno_np as W
+    from pyshocks_amd import _lib, make_uniform_cell_grid
+    from pyshocks_amd.timestepping import evolve, fused_plan
+
+    n = 256
+    x = (np.arange(n + 6) - 2.5) / n
+    u0 = 0.5 + np.sin(2 * np.pi * x)
+    ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    u0 = W.apply_boundary_periodic(ogrid, u0)
+    uref, dts, tref = W.solve(W.Scheme(kind, eps=1e-6, alpha=alpha), ogrid, u0,
+                              theta=1.0, tfinal=0.5)
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    st = _stepper(kind, grid, alpha)
+    plan = fused_plan(st)
+    assert plan is not None
+    assert plan.uses_step_kernel() == (kind == "eo")
+    for sched in (_lib.PSA_SCHEDULE_DEFAULT, _lib.PSA_SCHEDULE_TILE,
+                  _lib.PSA_SCHEDULE_PIPELINE):
+        plan = fused_plan(st)
+        plan.desc.schedule = sched
+        u = _dev(torch, u0)
+        k1, k2 = torch.empty_like(u), torch.empty_like(u)
+        ctrl = plan.new_control(u, 0.0, 0.5, None)
+        plan.steps(u, u, k1, k2, ctrl, len(dts) + 3)
+        c = plan.read_control(ctrl)
+        assert c.status == 0 and c.m == len(dts) and c.active == 0
+        assert abs(c.t - tref) < RTOL64 * 0.5
+        assert _rel(_host(u)[3:-3], uref[3:-3]) < RTOL64
+    ev = evolve(st, _dev(torch, u0), tfinal=0.5)
+    assert ev.iteration == len(dts)
+    np.testing.assert_allclose(float(ev.dt), dts[-1], rtol=0, atol=RTOL64 * 0.5)
+    assert _rel(_host(ev.u)[3:-3], uref[3:-3]) < RTOL64
+
+
+@pytest.mark.parametrize("kind", ["lf", "eo"])
+def test_fixed_dt_and_generic_loop(torch_mod, kind: str) -> None:
+    """A fixed dt (Lax-Friedrichs still takes max|u| of every stage input),
+    and the reference's host loop over the operator (``advance`` of a
+    Lax-Friedrichs stepper takes that loop) against the fused steps."""
+    torch = torch_mod
+    from oracle import exact as E
+    from oracle import weno_np as W
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.timestepping import advance, evolve
+
+    n = 5000
+    u0 = E.exact_cell_averages(E.large_initial_condition(seed=1), n)
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    dt = 0.2 / n
+    ev = evolve(_stepper(kind, grid, dt=dt), _dev(torch, u0), maxit=30)
+    uref, dts, _ = W.solve(W.Scheme(kind, eps=1e-6), ogrid, u0, theta=1.0, maxit=30,
+                           fixed_dt=dt)
+    assert ev.iteration == 30
+    assert _rel(_host(ev.u)[3:-3], uref[3:-3]) < RTOL64
+
+    u1 = _host(advance(_stepper(kind, grid, dt=dt), dt, 0.0, _dev(torch, u0)))
+    u1ref, _, _ = W.solve(W.Scheme(kind, eps=1e-6), ogrid, u0, theta=1.0, maxit=1,
+                          fixed_dt=dt)
+    assert _rel(u1[3:-3], u1ref[3:-3]) < RTOL64
+
+
+def test_lax_friedrichs_stage_entry_without_control(torch_mod) -> None:
+    """``psa_ssprk33_stage`` with a device dt takes the stage speed from
+    ``desc.velocity``: stage by stage with the caller's max|uk| it matches
+    the control-block steps bit for bit; ``psa_ssprk33_advance`` refuses."""
+    torch = torch_mod
+    from oracle import exact as E
+    from pyshocks_amd import _lib, make_uniform_cell_grid
+    from pyshocks_amd.reductions import max_abs_interior
+    from pyshocks_amd.timestepping import fused_plan
+
+    n = 20_011
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    u0 = _dev(torch, E.exact_cell_averages(E.large_initial_condition(seed=4), n))
+    dt = 0.3 / n
+    plan = fused_plan(_stepper("lf", grid, dt=dt))
+    u = u0.clone()
+    k1, k2 = torch.empty_like(u), torch.empty_like(u)
+    ctrl = plan.new_control(u, 0.0, float(np.finfo(np.float64).max), None)
+    plan.steps(u, u, k1, k2, ctrl, 2)
+    # the step's dt: fixed_dt clipped to tfinal - t plus 1e-15
+    # (timestepping.py:140-142)
+    c = plan.read_control(ctrl)
+    assert c.m == 2 and c.dt == dt + 1e-15
+
+    v = u0.clone()
+    dt_dev = torch.tensor(c.dt, dtype=torch.float64, device="cuda")
+    a1, a2 = torch.empty_like(v), torch.empty_like(v)
+    for _ in range(2):
+        for stage, (un, uk, out) in enumerate(((v, v, a1), (v, a1, a2), (v, a2, v)), 1):
+            speed = max_abs_interior(uk, 3)
+            plan.desc.velocity = speed.data_ptr()
+            plan.stage(stage, un, uk, out, None, dt_dev)
+            torch.cuda.synchronize()
+    assert torch.equal(u, v)
+
+    plan.desc.velocity = None
+    with pytest.raises(NotImplementedError):
+        plan.advance(u0, torch.empty_like(u0), k1, k2, dt_dev)
+
+
+def test_lax_friedrichs_operator_needs_speed(torch_mod) -> None:
+    """The C ABI refuses a Lax-Friedrichs flux or operator without the
+    device scalar max|a| (nothing is launched)."""
+    from pyshocks_amd import _lib
+
+    lib = _lib.load()
+    torch = torch_mod
+    u = torch.zeros(22, dtype=torch.float64, device="cuda")
+    f = torch.empty(23, dtype=torch.float64, device="cuda")
+    st = lib.psa_numerical_flux(_lib.PSA_FLOAT64, _lib.PSA_FLUX_LAX_FRIEDRICHS,
+                                ctypes.c_void_p(u.data_ptr()), None, None,
+                                ctypes.c_void_p(f.data_ptr()), 22, 1e-6, None)
+    assert st == _lib.PSA_ERR_INVALID_ARGUMENT
+    st = lib.psa_apply_operator(_lib.PSA_FLOAT64, _lib.PSA_FLUX_LAX_FRIEDRICHS,
+                                ctypes.c_void_p(u.data_ptr()), None, None,
+                                ctypes.c_void_p(f.data_ptr()), 22, 3, 1.0 / 16, 1e-6,
+                                None)
+    assert st == _lib.PSA_ERR_INVALID_ARGUMENT
+
+
+def test_lax_friedrichs_has_no_adjoint(torch_mod) -> None:
+    """The adjoint refuses Lax-Friedrichs (its global max speed); the batched
+    solver refuses it too (its stages need the global max of the stage
+    before)."""
+    torch = torch_mod
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.adjoint import ssprk33_step_vjp
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import BoundOperator
+
+    n = 40
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    u = torch.ones(n + 6, dtype=torch.float64, device="cuda")
+    lf = BoundOperator(_scheme("lf"), grid, PeriodicBoundary())
+    with pytest.raises(NotImplementedError):
+        ssprk33_step_vjp(lf, u, u, 0.01)

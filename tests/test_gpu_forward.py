@@ -101,6 +101,10 @@ def _scheme(kind: str, torch, vel, eps=1e-6):
         return burgers.Rusanov(rec=rec)
     if kind == "godunov":
         return burgers.Godunov(rec=rec)
+    if kind == "lf":
+        return burgers.LaxFriedrichs(rec=rec)
+    if kind == "eo":
+        return burgers.EngquistOsher(rec=rec)
     return advection.Godunov(rec=rec, velocity=vel)
 
 
@@ -128,7 +132,7 @@ def test_numerical_flux_and_operator(torch_mod, kind: str) -> None:
     np.testing.assert_array_equal(ub, g["boundary"])
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov", "advection"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "lf", "eo", "advection"])
 @pytest.mark.parametrize("n", [7, 1000, 70001])
 def test_operator_random_sizes(torch_mod, kind: str, n: int) -> None:
     """Ragged sizes (tile remainders, a tile smaller than the halo)."""
@@ -148,7 +152,7 @@ def test_operator_random_sizes(torch_mod, kind: str, n: int) -> None:
     assert _relmax(L, Lref) < RTOL64
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov", "advection"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "lf", "eo", "advection"])
 def test_predict_timestep(torch_mod, kind: str) -> None:
     torch = torch_mod
     from oracle import weno_np as W
@@ -275,7 +279,8 @@ def test_fused_advection_velocity_array(torch_mod, n: int) -> None:
 
 @pytest.mark.parametrize(("kind", "tiles"), [
     ("rusanov", (256, 1)), ("rusanov", (256, 2)), ("rusanov", (256, 4)),
-    ("godunov", (256, 4)),
+    ("godunov", (256, 4)), ("lf", (256, 4)), ("lf", (256, 2)), ("eo", (256, 4)),
+    ("eo", (256, 2)),
 ])
 def test_fused_large_sine_step(torch_mod, kind: str, tiles) -> None:
     """N = 2^16, sum of 8 sinusoids plus a step, 100 steps at Courant 0.5,
@@ -297,7 +302,8 @@ def test_fused_large_sine_step(torch_mod, kind: str, tiles) -> None:
     assert _rel(_host(ev.u)[3:-3], u[3:-3]) < RTOL64
 
 
-def test_fused_fp32_vs_fp64_oracle(torch_mod) -> None:
+@pytest.mark.parametrize("kind", ["rusanov", "lf", "eo"])
+def test_fused_fp32_vs_fp64_oracle(torch_mod, kind: str) -> None:
     torch = torch_mod
     from oracle import exact as E
     from oracle import weno_np as W
@@ -308,9 +314,9 @@ def test_fused_fp32_vs_fp64_oracle(torch_mod) -> None:
     u0 = E.exact_cell_averages(E.c1_initial_condition(), n)
     grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, dtype=torch.float32)
     ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
-    ev = evolve(_stepper(torch, "rusanov", grid), _dev(torch, u0, torch.float32),
+    ev = evolve(_stepper(torch, kind, grid), _dev(torch, u0, torch.float32),
                 maxit=50)
-    u, _, _ = W.solve(W.Scheme("rusanov", eps=1e-6), ogrid, u0, theta=1.0, maxit=50)
+    u, _, _ = W.solve(W.Scheme(kind, eps=1e-6), ogrid, u0, theta=1.0, maxit=50)
     assert ev.u.dtype == torch.float32
     assert _rel(_host(ev.u)[3:-3], u[3:-3]) < RTOL32
 
@@ -418,7 +424,7 @@ def test_norms(torch_mod, p) -> None:
 # }}}
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov", "advection"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "lf", "eo", "advection"])
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
 @pytest.mark.parametrize("tiles", [(256, 4), (256, 2)])
 def test_pipelined_schedule_bit_for_bit(torch_mod, kind: str, dtype_name: str, tiles) -> None:
@@ -452,7 +458,7 @@ def test_pipelined_schedule_bit_for_bit(torch_mod, kind: str, dtype_name: str, t
     assert torch.equal(out[0][0], out[1][0])
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov", "advection"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "eo", "advection"])
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
 @pytest.mark.parametrize("tiles", [(256, 4), (256, 2)])
 @pytest.mark.parametrize("n", [3, 16, 37, 1013, 100_003])

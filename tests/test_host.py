@@ -86,7 +86,7 @@ def test_product_refuses_cpu_tensors() -> None:
 
 
 def test_registries_and_errors() -> None:
-    from pyshocks_amd import advection, burgers
+    from pyshocks_amd import _lib, advection, burgers
     from pyshocks_amd.reconstruction import WENOJS53, make_reconstruction_from_name
 
     rec = make_reconstruction_from_name("wenojs53", eps=1e-6, unused=3)
@@ -103,6 +103,19 @@ def test_registries_and_errors() -> None:
     assert s.name == "rusanov_wenojs53" and s.order == 3 and s.stencil_width == 3
     with pytest.raises(ValueError):
         burgers.make_scheme_from_name("nope")
+    # burgers/__init__.py:62-72: "default" and "lf" are Lax-Friedrichs, a
+    # Rusanov subclass with its alpha; "eo" has a fixed omega = 0
+    lf = burgers.make_scheme_from_name("default", rec=rec, alpha=0.8)
+    assert type(lf) is burgers.LaxFriedrichs and isinstance(lf, burgers.Rusanov)
+    assert lf.alpha == 0.8 and lf.name == "laxfriedrichs_wenojs53"
+    assert type(burgers.make_scheme_from_name("lf", rec=rec)) is burgers.LaxFriedrichs
+    eo = burgers.make_scheme_from_name("eo", rec=rec)
+    assert type(eo) is burgers.EngquistOsher and eo.omega == 0
+    assert burgers.flux_kind(lf) == _lib.PSA_FLUX_LAX_FRIEDRICHS
+    assert burgers.flux_kind(eo) == _lib.PSA_FLUX_ENGQUIST_OSHER
+    assert burgers.scheme_ids() == ("default", "godunov", "rusanov", "lf", "eo")
+    with pytest.raises(NotImplementedError):
+        burgers.make_scheme_from_name("esweno32", rec=rec)
     a = advection.make_scheme_from_name("godunov", rec=rec)
     assert a.velocity is None
 
@@ -210,7 +223,7 @@ def test_c_oracle_matches_numpy_oracle() -> None:
 
     if not ref_c.LIB.exists():
         oracle_build.build(verbose=False)
-    for kind in ("rusanov", "godunov", "advection"):
+    for kind in ("rusanov", "godunov", "eo", "lf", "advection"):
         n = 200
         grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
         u0 = E.exact_cell_averages(E.large_initial_condition(seed=2), n)

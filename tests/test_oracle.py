@@ -179,7 +179,7 @@ def test_reconstruction_order() -> None:
 # {{{ operator invariants
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov", "advection"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "lf", "eo", "advection"])
 def test_operator_conservation_and_constants(kind: str) -> None:
     n = 64
     grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
@@ -197,6 +197,19 @@ def test_operator_conservation_and_constants(kind: str) -> None:
     np.testing.assert_allclose(L[grid.i_], 0.0, atol=1e-12)
 
 
+def _js_side(v: np.ndarray, eps: float) -> float:
+    # v = (u_{i-2}, ..., u_{i+2}), Jiang-Shu (2.17), (2.11)-(2.13), (2.15)
+    b0 = 13 / 12 * (v[0] - 2 * v[1] + v[2]) ** 2 + 1 / 4 * (v[0] - 4 * v[1] + 3 * v[2]) ** 2
+    b1 = 13 / 12 * (v[1] - 2 * v[2] + v[3]) ** 2 + 1 / 4 * (v[1] - v[3]) ** 2
+    b2 = 13 / 12 * (v[2] - 2 * v[3] + v[4]) ** 2 + 1 / 4 * (3 * v[2] - 4 * v[3] + v[4]) ** 2
+    q0 = (2 * v[0] - 7 * v[1] + 11 * v[2]) / 6
+    q1 = (-v[1] + 5 * v[2] + 2 * v[3]) / 6
+    q2 = (2 * v[2] + 5 * v[3] - v[4]) / 6
+    a = np.array([0.1 / (eps + b0) ** 2, 0.6 / (eps + b1) ** 2, 0.3 / (eps + b2) ** 2])
+    w = a / a.sum()
+    return float(w @ np.array([q0, q1, q2]))
+
+
 def test_rusanov_closed_form_small() -> None:
     """Brute-force flux on a tiny grid with hand-evaluated formulas."""
     n = 8
@@ -205,25 +218,85 @@ def test_rusanov_closed_form_small() -> None:
     u = W.apply_boundary_periodic(grid, rng.uniform(-1, 1, size=grid.n))
     eps = 1e-6
 
-    def side(v: np.ndarray) -> float:
-        # v = (u_{i-2}, ..., u_{i+2}), Jiang-Shu (2.17), (2.11)-(2.13), (2.15)
-        b0 = 13 / 12 * (v[0] - 2 * v[1] + v[2]) ** 2 + 1 / 4 * (v[0] - 4 * v[1] + 3 * v[2]) ** 2
-        b1 = 13 / 12 * (v[1] - 2 * v[2] + v[3]) ** 2 + 1 / 4 * (v[1] - v[3]) ** 2
-        b2 = 13 / 12 * (v[2] - 2 * v[3] + v[4]) ** 2 + 1 / 4 * (3 * v[2] - 4 * v[3] + v[4]) ** 2
-        q0 = (2 * v[0] - 7 * v[1] + 11 * v[2]) / 6
-        q1 = (-v[1] + 5 * v[2] + 2 * v[3]) / 6
-        q2 = (2 * v[2] + 5 * v[3] - v[4]) / 6
-        a = np.array([0.1 / (eps + b0) ** 2, 0.6 / (eps + b1) ** 2, 0.3 / (eps + b2) ** 2])
-        w = a / a.sum()
-        return float(w @ np.array([q0, q1, q2]))
-
     F = W.scalar_flux_rusanov(grid, u, u, eps=eps)
     for j in range(3, grid.n - 2):  # faces with full stencils
-        ur = side(u[j - 3: j + 2])
-        ul = side(u[j - 2: j + 3][::-1])
+        ur = _js_side(u[j - 3: j + 2], eps)
+        ul = _js_side(u[j - 2: j + 3][::-1], eps)
         s = max(abs(u[j - 1]), abs(u[j]))
         ref = 0.5 * (ul**2 / 2 + ur**2 / 2) - 0.5 * s * (ul - ur)
         assert abs(F[j] - ref) < 1e-13
+
+
+def test_engquist_osher_integral_definition() -> None:
+    """The Engquist-Osher face flux is (f(a) + f(b)) / 2 - 1/2 int_a^b |f'| du
+    (LeVeque (12.49)); for f = u^2 / 2, int_a^b |u| du = (b|b| - a|a|) / 2.
+    Checked against that closed form and against midpoint quadrature."""
+    rng = np.random.default_rng(11)
+    a = rng.uniform(-2, 2, size=200)
+    b = rng.uniform(-2, 2, size=200)
+    a[:4] = [0.0, 1.0, -1.0, 0.5]
+    b[:4] = [0.0, -1.0, 1.0, 0.5]
+    F = W.engquist_osher_faces(a, b)
+    closed = (a**2 / 2 + b**2 / 2) / 2 - 0.5 * (b * np.abs(b) - a * np.abs(a)) / 2
+    np.testing.assert_allclose(F, closed, rtol=0, atol=1e-14)
+
+    k = 20000
+    for ai, bi, fi in zip(a[:20], b[:20], F[:20]):
+        s = (np.arange(k) + 0.5) / k
+        quad = np.sum(np.abs(ai + (bi - ai) * s)) * (bi - ai) / k
+        ref = (ai**2 / 2 + bi**2 / 2) / 2 - 0.5 * quad
+        assert abs(fi - ref) < 1e-7
+    # consistency
+    np.testing.assert_allclose(W.engquist_osher_faces(a, a), a**2 / 2, rtol=0,
+                               atol=1e-15)
+
+
+def test_engquist_osher_uses_reconstructed_sides() -> None:
+    """The left-of-face state is the right reconstruction of the cell to the
+    left, the right-of-face state the left reconstruction of the cell to the
+    right (brute force with hand-evaluated WENO-JS formulas)."""
+    n = 8
+    eps = 1e-6
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    u = W.apply_boundary_periodic(
+        grid, np.random.default_rng(5).uniform(-1, 1, size=grid.n))
+    F = W.scalar_flux_engquist_osher(grid, u, eps=eps)
+    for j in range(3, grid.n - 2):
+        ur = _js_side(u[j - 3: j + 2], eps)
+        ul = _js_side(u[j - 2: j + 3][::-1], eps)
+        ref = max(ur, 0.0) ** 2 / 2 + min(ul, 0.0) ** 2 / 2
+        assert abs(F[j] - ref) < 1e-13
+
+
+def test_lax_friedrichs_global_speed() -> None:
+    """Lax-Friedrichs is the Rusanov flux with one speed max|a| over the whole
+    array as given, ghost entries included."""
+    n = 16
+    eps = 1e-6
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    rng = np.random.default_rng(3)
+    # |u| constant: the local and the global speeds agree
+    u = 0.8 * rng.choice([-1.0, 1.0], size=grid.n)
+    np.testing.assert_allclose(W.scalar_flux_lax_friedrichs(grid, u, u, eps=eps),
+                               W.scalar_flux_rusanov(grid, u, u, eps=eps),
+                               rtol=0, atol=1e-15)
+    # the largest |u| sits in a ghost entry: every face sees it
+    u = rng.uniform(-1, 1, size=grid.n)
+    u[0] = -5.0
+    F = W.scalar_flux_lax_friedrichs(grid, u, u, eps=eps)
+    for j in range(3, grid.n - 2):
+        ur = _js_side(u[j - 3: j + 2], eps)
+        ul = _js_side(u[j - 2: j + 3][::-1], eps)
+        ref = 0.5 * (ul**2 / 2 + ur**2 / 2) - 0.5 * 5.0 * (ul - ur)
+        assert abs(F[j] - ref) < 1e-13
+    assert F[0] == 0.0 and F[-1] == 0.0
+    # alpha != 1: nu = df^(alpha - 1) multiplies the dissipation
+    Fa = W.scalar_flux_lax_friedrichs(grid, u, u, eps=eps, alpha=0.9)
+    nu = grid.df ** (0.9 - 1)
+    d = (F - Fa)[1:-1]
+    ul, ur = W.reconstruct_wenojs(u, eps)
+    np.testing.assert_allclose(d, -0.5 * 5.0 * (1 - nu) * (ul[1:] - ur[:-1]),
+                               rtol=1e-12, atol=1e-14)
 
 
 # }}}
@@ -328,7 +401,7 @@ def test_burgers_smooth_order_hopf_lax() -> None:
     assert W.estimate_order_of_convergence(np.array(hs), np.array(errs)) >= 2.8
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "lf", "eo"])
 def test_burgers_shock_hopf_lax(kind: str) -> None:
     """Past the shock (t = 0.5), against exact Hopf-Lax cell averages."""
     ic = E.c1_initial_condition()
