@@ -177,11 +177,13 @@ typedef struct psa_control {
    * (a workgroup uses slot blockIdx % PSA_UMAX_SLOTS); stage 1 of the next
    * step combines them and stage 2 clears them. The whole-step kernel
    * reduces into word PSA_UMAX_STEP_WORD; the kernel after it leaves the
-   * combined value in word 0 of slot 0 (the other word-0 entries zero).
-   * Lax-Friedrichs: stage s (1, 2) reduces max|u| of its output into word
-   * s, which stage s + 1 combines as its flux speed; stage 1 clears word 2
-   * and stage 3 clears word 1. */
+   * combined value in word 0 of slot 0 (the other word-0 entries zero). */
   unsigned long long umax_slot[PSA_UMAX_SLOTS][PSA_UMAX_SLOT_STRIDE];
+  /* Lax-Friedrichs: stage s (1, 2) reduces max|u| of its output into word 0
+   * of speed_slot[s - 1], which stage s + 1 combines as its flux speed;
+   * stage 1 clears speed_slot[1] and stage 3 clears speed_slot[0]. Lines of
+   * their own: no launch reads the lines it updates atomically. */
+  unsigned long long speed_slot[2][PSA_UMAX_SLOTS][PSA_UMAX_SLOT_STRIDE];
 } psa_control;
 #define PSA_UMAX_STEP_WORD 8
 

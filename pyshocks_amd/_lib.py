@@ -68,6 +68,7 @@ class PsaControl(ctypes.Structure):
         ("pad1_", ctypes.c_int),
         ("pad_", ctypes.c_ulonglong * 3),
         ("umax_slot", (ctypes.c_ulonglong * 16) * 64),
+        ("speed_slot", ((ctypes.c_ulonglong * 16) * 64) * 2),
     ]
 
 

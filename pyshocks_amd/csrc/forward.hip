@@ -207,9 +207,9 @@ __global__ void k_control_init(psa_control *ctrl, double t, double tfinal,
   ctrl->umax_bits = abs_bits(static_cast<T>(speed_const));
   for (int k = 0; k < PSA_UMAX_SLOTS; ++k) {
     ctrl->umax_slot[k][0] = 0ull;
-    ctrl->umax_slot[k][1] = 0ull;
-    ctrl->umax_slot[k][2] = 0ull;
     ctrl->umax_slot[k][PSA_UMAX_STEP_WORD] = 0ull;
+    ctrl->speed_slot[0][k][0] = 0ull;
+    ctrl->speed_slot[1][k][0] = 0ull;
   }
   ctrl->umax_slot[0][0] = ctrl->umax_bits;
   ctrl->dt_next = 0.0;
@@ -221,19 +221,9 @@ __global__ void k_control_init(psa_control *ctrl, double t, double tfinal,
 }
 
 // Decision of stage 1, evaluated identically by every workgroup.
-// max of the PSA_UMAX_SLOTS partials in word `word`: lane k of every wave
-// reads slot k and the wave reduces them, so each wave holds the result
-// without a barrier.
 __device__ __forceinline__ unsigned long long gather_umax(
-    const psa_control *ctrl, int word = 0) {
-  static_assert(PSA_UMAX_SLOTS == 64, "one slot per lane");
-  const int lane = threadIdx.x & 63;
-  unsigned long long v = ctrl->umax_slot[lane][word];
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(v, off, 64);
-    v = o > v ? o : v;
-  }
-  return v;
+    const psa_control *ctrl) {
+  return gather_slots(ctrl->umax_slot);
 }
 
 template <typename T>
@@ -280,11 +270,14 @@ __global__ void k_copy_if_inactive(const T *__restrict__ src,
 // the device): stage 1 decides whether the step is taken and publishes dt,
 // stage 2 advances t and m and clears the max|u| slots, stage 3 reads dt.
 // Lax-Friedrichs (scalar.py:276-277): speed is max|u| of the stage input,
-// from word 0 (u^n), 1 (k1) or 2 (k2) of the slots, or from p.speed without
-// a control block; stage 1 clears word 2 for stage 2 and stage 3 clears
-// word 1 for the next step's stage 1 (each after its last reader is done).
+// from the max|u| slots (u^n) or speed_slot[0] (k1) or speed_slot[1] (k2),
+// or from p.speed without a control block; stage 1 clears speed_slot[1] for
+// stage 2 and stage 3 clears speed_slot[0] for the next step's stage 1
+// (each after its last reader is done). DEFER: stages 2 and 3 leave the
+// speed slots to stage_tile (p.speed_slots), which gathers them once its
+// tile loads are in flight.
 // Returns false when the step is not taken (the workgroup then exits).
-template <typename T, int STAGE, int FLUX>
+template <typename T, int STAGE, int FLUX, bool DEFER = false>
 __device__ __forceinline__ bool stage_control(const StageArgs<T> &p, T &dt,
                                               T &speed) {
   constexpr bool LF = FLUX == PSA_FLUX_LAX_FRIEDRICHS;
@@ -307,12 +300,13 @@ __device__ __forceinline__ bool stage_control(const StageArgs<T> &p, T &dt,
         if constexpr (LF) {
           speed = from_bits<T>(umax);
           if (blockIdx.x == 0 && tid < PSA_UMAX_SLOTS)
-            p.ctrl->umax_slot[tid][2] = 0ull;
+            p.ctrl->speed_slot[1][tid][0] = 0ull;
         }
       } else {
         if (p.ctrl->active == 0) return false;
         dt = static_cast<T>(p.ctrl->dt);
-        if constexpr (LF) speed = from_bits<T>(gather_umax(p.ctrl, STAGE - 1));
+        if constexpr (LF && !DEFER)
+          speed = from_bits<T>(gather_slots(p.ctrl->speed_slot[STAGE - 2]));
         if constexpr (STAGE == 2) {
           if (blockIdx.x == 0 && tid == 0) {
             p.ctrl->t = static_cast<double>(static_cast<T>(p.ctrl->t) + dt);
@@ -324,9 +318,9 @@ __device__ __forceinline__ bool stage_control(const StageArgs<T> &p, T &dt,
             p.ctrl->umax_slot[tid][0] = 0ull;
           }
         } else if constexpr (LF) {
-          // stage 2 has read word 1
+          // stage 2 has read speed_slot[0]
           if (blockIdx.x == 0 && tid < PSA_UMAX_SLOTS)
-            p.ctrl->umax_slot[tid][1] = 0ull;
+            p.ctrl->speed_slot[0][tid][0] = 0ull;
         }
       }
     } else {
@@ -340,27 +334,26 @@ __device__ __forceinline__ bool stage_control(const StageArgs<T> &p, T &dt,
   return true;
 }
 
-// Where a stage launch reduces max|u| of its output (-1: nowhere): word 0
-// for the next step's dt (stage 3), words 1 and 2 for the Lax-Friedrichs
-// speed of the next stage.
-template <int STAGE, int FLUX>
-constexpr int stage_reduce_word() {
-  if constexpr (STAGE == 3) return 0;
-  if constexpr (FLUX == PSA_FLUX_LAX_FRIEDRICHS && STAGE > 0) return STAGE;
-  return -1;
-}
-
-// The workgroup's part of that reduction (after its last tile).
+// The workgroup's part of the max|u| reduction of a stage launch's output
+// (after its last tile): into the max|u| slots for the next step's dt
+// (stage 3), into speed_slot[STAGE - 1] for the Lax-Friedrichs speed of the
+// next stage (stages 1 and 2).
 template <typename T, int STAGE, int FLUX, int BLOCK>
 __device__ __forceinline__ void stage_reduce(const StageArgs<T> &p,
                                              unsigned long long vmax,
                                              unsigned long long *red) {
-  constexpr int word = stage_reduce_word<STAGE, FLUX>();
-  if constexpr (word >= 0) {
+  constexpr bool lf = FLUX == PSA_FLUX_LAX_FRIEDRICHS && (STAGE == 1 || STAGE == 2);
+  if constexpr (STAGE == 3 || lf) {
     if (p.ctrl != nullptr && p.ctrl->speed_from_state) {
       vmax = block_max_bits<BLOCK>(vmax, red);
-      if (threadIdx.x == 0)
-        atomicMax(&p.ctrl->umax_slot[blockIdx.x % PSA_UMAX_SLOTS][word], vmax);
+      if (threadIdx.x == 0) {
+        const int k = blockIdx.x % PSA_UMAX_SLOTS;
+        if constexpr (lf) {
+          atomicMax(&p.ctrl->speed_slot[STAGE - 1][k][0], vmax);
+        } else {
+          atomicMax(&p.ctrl->umax_slot[k][0], vmax);
+        }
+      }
     }
   }
 }
@@ -375,9 +368,13 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   const int64_t ntiles = (p.m + TILE - 1) / TILE;
 
   T dt, speed;
-  if (!stage_control<T, STAGE, FLUX>(p, dt, speed)) return;
+  if (!stage_control<T, STAGE, FLUX, true>(p, dt, speed)) return;
   StageArgs<T> pl = p;
-  if constexpr (FLUX == PSA_FLUX_LAX_FRIEDRICHS) pl.vconst = speed;
+  if constexpr (FLUX == PSA_FLUX_LAX_FRIEDRICHS) {
+    pl.vconst = speed;
+    if (STAGE >= 2 && p.ctrl != nullptr)
+      pl.speed_slots = p.ctrl->speed_slot[STAGE >= 2 ? STAGE - 2 : 0];
+  }
 
   unsigned long long vmax = 0;
   const int64_t tile = blockIdx.x;  // the grid has one workgroup per tile

@@ -75,6 +75,21 @@ __device__ __forceinline__ unsigned long long block_max_bits(
   return v;
 }
 
+// max of the PSA_UMAX_SLOTS partials in word 0 of `slots`: lane k of every
+// wave reads slot k and the wave reduces them, so each wave holds the result
+// without a barrier.
+__device__ __forceinline__ unsigned long long gather_slots(
+    const unsigned long long (*slots)[PSA_UMAX_SLOT_STRIDE]) {
+  static_assert(PSA_UMAX_SLOTS == 64, "one slot per lane");
+  const int lane = threadIdx.x & 63;
+  unsigned long long v = slots[lane][0];
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // fused tiled stage kernel
 
@@ -98,6 +113,9 @@ struct StageArgs {
                     //   global flux speed of the stage (set by the kernel)
   const T *speed;   // Lax-Friedrichs without a control block: the device
                     //   scalar max|a| of the stage input
+  // Lax-Friedrichs stage kernels: the slots whose maximum is the flux speed,
+  // gathered once the tile's loads are in flight (null: vconst holds it)
+  const unsigned long long (*speed_slots)[PSA_UMAX_SLOT_STRIDE];
   const float *mlp;  // learned weights (LEARNED kernels)
   T tau;             // their shock threshold
   T sigma;           // their smooth threshold
@@ -502,9 +520,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   const bool fast = p.aligned != 0 && STAGE > 0 && S - HALO >= g &&
                     S + TILE + HALO <= m - g;
 
+  // the constant advection velocity, or the Lax-Friedrichs flux speed
+  T fspeed = p.vconst;
+  auto gather_speed = [&]() {
+    if constexpr (FLUX == PSA_FLUX_LAX_FRIEDRICHS) {
+      if (p.speed_slots != nullptr) fspeed = from_bits<T>(gather_slots(p.speed_slots));
+    }
+  };
+
   // ---- stage the tile and its halo
   if (fast && PRELOADED) {
     // staged by the caller
+    gather_speed();
   } else if (fast) {
     // all loads of the tile, the velocity and u^n first, then the LDS writes
     Chunks<T, BLOCK, (TILE + 2 * HALO) / VN> cu;
@@ -513,10 +540,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     cu.load(p.uk + S - HALO, tid);
     if constexpr (VEL == 2) cv.load(p.vel + S - HALO, tid);
     if constexpr (STAGE >= 2) cn.load(p.un + S, tid);
+    gather_speed();
     cu.store(sm.u, tid);
     if constexpr (VEL == 2) cv.store(sm.v, tid);
     if constexpr (STAGE >= 2) cn.store(sm.io, tid);
   } else {
+    gather_speed();
     for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {
       sm.u[j] = load_filled(p.uk, S - HALO + j, m, n, g);
     }
@@ -618,7 +647,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     }
     T Fk;
     if (plain) {
-      Fk = stage_flux<T, FLUX>(urL, ulR, w[k + 1], w[k + 2], p.vconst);
+      Fk = stage_flux<T, FLUX>(urL, ulR, w[k + 1], w[k + 2], fspeed);
     } else if constexpr (FLUX == PSA_FLUX_RUSANOV) {
       if (has_nu) {
         const int64_t gm = S + l0 + k;  // face between cells gm - 1 and gm
@@ -633,7 +662,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
         const int64_t gm = S + l0 + k;
         nuk = (gm >= 1 && gm <= m - 1) ? p.nu[gm - 1] : T(1);
       }
-      Fk = flux_lax_friedrichs(urL, ulR, p.vconst, nuk);
+      Fk = flux_lax_friedrichs(urL, ulR, fspeed, nuk);
     } else if constexpr (FLUX == PSA_FLUX_ENGQUIST_OSHER) {
       Fk = flux_engquist_osher(urL, ulR);
     } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
