@@ -61,6 +61,18 @@ extern "C" {
 #define PSA_FLUX_ENGQUIST_OSHER 3
 #define PSA_FLUX_LAX_FRIEDRICHS 4
 
+/* reconstructions ----------------------------------------------------------
+ * PSA_REC_WENOJS53  reconstruction.WENOJS53 (reconstruction.py:336-348,
+ *                   weno.py:206-256), nghosts >= 3
+ * PSA_REC_WENOJS32  reconstruction.WENOJS32 (reconstruction.py:320-333,
+ *                   weno.py:166-203), nghosts >= 2
+ * PSA_REC_ESWENO32  reconstruction.ESWENO32: the WENO-JS3 stencils with the
+ *                   ES-WENO weights (reconstruction.py:387-434,
+ *                   weno.py:285-296), nghosts >= 2                       */
+#define PSA_REC_WENOJS53 0
+#define PSA_REC_WENOJS32 1
+#define PSA_REC_ESWENO32 2
+
 /* time step modes of the device-side step control */
 #define PSA_DT_CFL 0   /* dt = theta * cfl_num / max|u_int| (or max|a_int|) */
 #define PSA_DT_FIXED 1 /* dt = fixed_dt every step                           */
@@ -89,6 +101,12 @@ int psa_weno_js_weights(int dtype, const void *u, void *omega, int64_t m,
 int psa_weno_reconstruct(int dtype, const void *f, void *ul, void *ur,
                          int64_t m, double eps, void *stream);
 
+/* The same for any PSA_REC_* (reconstruction.py:358-377,420-434): the
+ * 'same' convolutions pad with zeros; ES-WENO's tau is zero for the first
+ * and last cell (weno.py:293). */
+int psa_reconstruct(int dtype, int rec, const void *f, void *ul, void *ur,
+                    int64_t m, double eps, void *stream);
+
 /* scalar.py:529-540 apply_boundary(PeriodicBoundary): out = u with the
  * g ghost cells per side filled with their periodic images (out may be u). */
 int psa_apply_boundary_periodic(int dtype, const void *u, void *out,
@@ -105,6 +123,12 @@ int psa_numerical_flux(int dtype, int flux, const void *u,
                        const void *velocity, const void *nu, void *fnum,
                        int64_t m, double eps, void *stream);
 
+/* psa_numerical_flux with the reconstruction rec (PSA_REC_*); an advection
+ * velocity array is reconstructed the same way (advection/schemes.py:108). */
+int psa_numerical_flux_rec(int dtype, int flux, int rec, const void *u,
+                           const void *velocity, const void *nu, void *fnum,
+                           int64_t m, double eps, void *stream);
+
 /* schemes.py:339-346 (conservation law) and advection/schemes.py:62-73:
  * out (m) = -(F[1:] - F[:-1]) / dx (times -velocity for advection), with F
  * the numerical flux of the periodic ghost-filled u; every entry, ghosts
@@ -114,6 +138,13 @@ int psa_numerical_flux(int dtype, int flux, const void *u,
 int psa_apply_operator(int dtype, int flux, const void *u,
                        const void *velocity, const void *nu, void *out,
                        int64_t m, int g, double dx, double eps, void *stream);
+
+/* psa_apply_operator with the reconstruction rec (PSA_REC_*); an advection
+ * velocity array is reconstructed the same way. */
+int psa_apply_operator_rec(int dtype, int flux, int rec, const void *u,
+                           const void *velocity, const void *nu, void *out,
+                           int64_t m, int g, double dx, double eps,
+                           void *stream);
 
 /* max |u[g:m-g]| (burgers/schemes.py:42-49,121-127 and
  * advection/schemes.py:51-59 smax / amax) into the device scalar out. */
@@ -209,7 +240,7 @@ typedef struct psa_solver_desc {
   double mlp_tau;               /* shock threshold of the learned weights  */
   double mlp_sigma;             /* smooth threshold of the learned weights */
   int schedule;                 /* stages 1-3: PSA_SCHEDULE_*              */
-  int pad_;
+  int rec;                      /* PSA_REC_* (0: WENO-JS5)                 */
 } psa_solver_desc;
 
 /* Step schedules of psa_ssprk33_steps:

@@ -1,5 +1,6 @@
-"""CPU oracle: a NumPy FP64 restatement of pyshocks' WENO-JS5 / Rusanov /
-Godunov / Lax-Friedrichs / Engquist-Osher / SSP-RK3 hot path.
+"""CPU oracle: a NumPy FP64 restatement of pyshocks' WENO-JS5 (and WENO-JS3,
+ES-WENO3) / Rusanov / Godunov / Lax-Friedrichs / Engquist-Osher / SSP-RK3
+hot path.
 
 TEST INFRASTRUCTURE ONLY. This module is the checker for the HIP library in
 ``pyshocks_amd``; it is imported by ``tests/``, ``__graft_entry__.smoke()``
@@ -82,6 +83,21 @@ def weno_js_53_coefficients() -> Stencil:
 
 WENOJS53 = weno_js_53_coefficients()
 
+
+def weno_js_32_coefficients() -> Stencil:
+    """Third-order WENO-JS coefficients (``weno.py:166-203``), taps ordered
+    ``(i + 1, i, i - 1)`` for a 'same' convolution."""
+    a = np.array([1.0], dtype=np.float64)
+    b = np.array([[[0.0, 1.0, -1.0]], [[1.0, -1.0, 0.0]]], dtype=np.float64)
+    c = np.array([[0.0, 3.0 / 2.0, -1.0 / 2.0], [1.0 / 2.0, 1.0 / 2.0, 0.0]],
+                 dtype=np.float64)
+    d = np.array([[1.0 / 3.0, 2.0 / 3.0]], dtype=np.float64).T
+
+    return Stencil(a=a, b=b, c=c, d=d)
+
+
+WENOJS32 = weno_js_32_coefficients()
+
 # }}}
 
 
@@ -114,6 +130,36 @@ def weno_js_weights(s: Stencil, u: np.ndarray, *, eps: float) -> np.ndarray:
     alpha = s.d / (eps + beta) ** 2
 
     return alpha / np.sum(alpha, axis=0, keepdims=True)
+
+
+def es_weno_weights(s: Stencil, u: np.ndarray, *, eps: float) -> np.ndarray:
+    """ES-WENO weights ``d (1 + tau / (eps + beta))`` (``weno.py:285-296``)."""
+    beta = weno_smoothness(s, u)
+    tau = np.pad((u[2:] - 2 * u[1:-1] + u[:-2]) ** 2, 1)
+    alpha = s.d * (1 + tau / (eps + beta))
+
+    return alpha / np.sum(alpha, axis=0, keepdims=True)
+
+
+def reconstruct_es_side(f: np.ndarray, eps: float) -> np.ndarray:
+    """``reconstruction.py:413-417``."""
+    omega = es_weno_weights(WENOJS32, f, eps=eps)
+    uhat = weno_interp(WENOJS32, f)
+
+    return np.sum(omega * uhat, axis=0)
+
+
+def reconstruct(f: np.ndarray, eps: float, rec: str = "wenojs53",
+                ) -> tuple[np.ndarray, np.ndarray]:
+    """``(ul, ur)`` of ``rec`` in ``{"wenojs53", "wenojs32", "esweno32"}``
+    (``reconstruction.py:358-377,420-434``)."""
+    if rec == "wenojs53":
+        return reconstruct_wenojs(f, eps, WENOJS53)
+    if rec == "wenojs32":
+        return reconstruct_wenojs(f, eps, WENOJS32)
+    if rec == "esweno32":
+        return reconstruct_es_side(f[::-1], eps)[::-1], reconstruct_es_side(f, eps)
+    raise NotImplementedError(rec)
 
 
 def reconstruct_side(f: np.ndarray, eps: float, s: Stencil = WENOJS53) -> np.ndarray:
@@ -265,7 +311,7 @@ def burgers_flux(u: np.ndarray) -> np.ndarray:
 
 
 def scalar_flux_rusanov(grid: Grid, u: np.ndarray, a: np.ndarray, *, eps: float,
-                        alpha: float = 1.0,
+                        alpha: float = 1.0, rec: str = "wenojs53",
                         flux: Callable[[np.ndarray], np.ndarray] = burgers_flux,
                         ) -> np.ndarray:
     """Rusanov flux with the speed from the cell values (``scalar.py:192-249``)."""
@@ -274,7 +320,7 @@ def scalar_flux_rusanov(grid: Grid, u: np.ndarray, a: np.ndarray, *, eps: float,
     else:
         nu = np.array(1.0, dtype=u.dtype)
 
-    ul, ur = reconstruct_wenojs(u, eps)
+    ul, ur = reconstruct(u, eps, rec)
     fl = flux(ul)
     fr = flux(ur)
 
@@ -288,11 +334,12 @@ def scalar_flux_rusanov(grid: Grid, u: np.ndarray, a: np.ndarray, *, eps: float,
 
 
 def scalar_flux_lax_friedrichs(grid: Grid, u: np.ndarray, a: np.ndarray, *,
-                               eps: float, alpha: float = 1.0) -> np.ndarray:
+                               eps: float, alpha: float = 1.0,
+                               rec: str = "wenojs53") -> np.ndarray:
     """Rusanov with the global speed ``max|a|`` over the whole array
     (``scalar.py:258-278``)."""
     amax = np.max(np.abs(a))
-    return scalar_flux_rusanov(grid, u, amax, eps=eps, alpha=alpha)
+    return scalar_flux_rusanov(grid, u, amax, eps=eps, alpha=alpha, rec=rec)
 
 
 def engquist_osher_faces(ur_left: np.ndarray, ul_right: np.ndarray, *,
@@ -309,20 +356,21 @@ def engquist_osher_faces(ur_left: np.ndarray, ul_right: np.ndarray, *,
 
 
 def scalar_flux_engquist_osher(grid: Grid, u: np.ndarray, *, eps: float,
-                               omega: float = 0.0) -> np.ndarray:
+                               omega: float = 0.0, rec: str = "wenojs53") -> np.ndarray:
     """``scalar.py:287-322`` with a WENO-JS reconstruction."""
-    ul, ur = reconstruct_wenojs(u, eps)
+    ul, ur = reconstruct(u, eps, rec)
     fnum = engquist_osher_faces(ur[:-1], ul[1:], omega=omega)
     return np.pad(fnum, 1)
 
 
 def scalar_flux_upwind(grid: Grid, u: np.ndarray, a: np.ndarray, *, eps: float,
+                       rec: str = "wenojs53",
                        flux: Callable[[np.ndarray], np.ndarray] = burgers_flux,
                        ) -> np.ndarray:
     """Upwind flux switched on the averaged reconstructed speed
     (``scalar.py:91-132``)."""
-    ul, ur = reconstruct_wenojs(u, eps)
-    al, ar = reconstruct_wenojs(a, eps)
+    ul, ur = reconstruct(u, eps, rec)
+    al, ar = reconstruct(a, eps, rec)
 
     fl = flux(ul)
     fr = flux(ur)
@@ -334,10 +382,10 @@ def scalar_flux_upwind(grid: Grid, u: np.ndarray, a: np.ndarray, *, eps: float,
 
 
 def advection_upwind_flux(grid: Grid, u: np.ndarray, velocity: np.ndarray, *,
-                          eps: float) -> np.ndarray:
+                          eps: float, rec: str = "wenojs53") -> np.ndarray:
     """``advection/schemes.py:100-114``."""
-    ul, ur = reconstruct_wenojs(u, eps)
-    al, ar = reconstruct_wenojs(velocity, eps)
+    ul, ur = reconstruct(u, eps, rec)
+    al, ar = reconstruct(velocity, eps, rec)
 
     aavg = (ar[:-1] + al[1:]) / 2
     fnum = np.where(aavg > 0, ur[:-1], ul[1:])
@@ -349,29 +397,34 @@ def advection_upwind_flux(grid: Grid, u: np.ndarray, velocity: np.ndarray, *,
 class Scheme:
     """A bound scheme of the hot path: ``kind`` is ``"rusanov"``,
     ``"godunov"``, ``"lf"`` (Lax-Friedrichs) or ``"eo"`` (Engquist-Osher)
-    for Burgers, or ``"advection"`` (linear advection, Godunov)."""
+    for Burgers, or ``"advection"`` (linear advection, Godunov); ``rec`` is
+    the reconstruction (``"wenojs53"``, ``"wenojs32"`` or ``"esweno32"``)."""
 
     kind: str
     eps: float
     alpha: float = 1.0
     velocity: np.ndarray | None = None
+    rec: str = "wenojs53"
 
 
 def numerical_flux(scheme: Scheme, grid: Grid, u: np.ndarray) -> np.ndarray:
     """``burgers/schemes.py:83-118,145-153,188-196`` and
     ``advection/schemes.py:125-129``."""
+    rec = scheme.rec
     if scheme.kind == "rusanov":
-        return scalar_flux_rusanov(grid, u, u, eps=scheme.eps, alpha=scheme.alpha)
+        return scalar_flux_rusanov(grid, u, u, eps=scheme.eps, alpha=scheme.alpha,
+                                   rec=rec)
     if scheme.kind == "lf":
         return scalar_flux_lax_friedrichs(grid, u, u, eps=scheme.eps,
-                                          alpha=scheme.alpha)
+                                          alpha=scheme.alpha, rec=rec)
     if scheme.kind == "eo":
-        return scalar_flux_engquist_osher(grid, u, eps=scheme.eps)
+        return scalar_flux_engquist_osher(grid, u, eps=scheme.eps, rec=rec)
     if scheme.kind == "godunov":
-        return scalar_flux_upwind(grid, u, u, eps=scheme.eps)
+        return scalar_flux_upwind(grid, u, u, eps=scheme.eps, rec=rec)
     if scheme.kind == "advection":
         assert scheme.velocity is not None
-        return advection_upwind_flux(grid, u, scheme.velocity, eps=scheme.eps)
+        return advection_upwind_flux(grid, u, scheme.velocity, eps=scheme.eps,
+                                     rec=rec)
 
     raise NotImplementedError(scheme.kind)
 

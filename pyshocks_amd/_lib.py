@@ -35,6 +35,10 @@ PSA_FLUX_ADVECTION_UPWIND = 2
 PSA_FLUX_ENGQUIST_OSHER = 3
 PSA_FLUX_LAX_FRIEDRICHS = 4
 
+PSA_REC_WENOJS53 = 0
+PSA_REC_WENOJS32 = 1
+PSA_REC_ESWENO32 = 2
+
 #: flux codes of the batched solver's Burgers scheme names
 BATCH_FLUX = {"rusanov": PSA_FLUX_RUSANOV, "godunov": PSA_FLUX_BURGERS_UPWIND,
               "eo": PSA_FLUX_ENGQUIST_OSHER}
@@ -90,7 +94,7 @@ class PsaSolverDesc(ctypes.Structure):
         ("mlp_tau", ctypes.c_double),
         ("mlp_sigma", ctypes.c_double),
         ("schedule", ctypes.c_int),
-        ("pad_", ctypes.c_int),
+        ("rec", ctypes.c_int),
     ]
 
 
@@ -143,9 +147,13 @@ SIGNATURES: dict[str, list[Any]] = {
     "psa_weno_interp": [_I, _VP, _VP, _I64, _VP],
     "psa_weno_js_weights": [_I, _VP, _VP, _I64, _D, _VP],
     "psa_weno_reconstruct": [_I, _VP, _VP, _VP, _I64, _D, _VP],
+    "psa_reconstruct": [_I, _I, _VP, _VP, _VP, _I64, _D, _VP],
     "psa_apply_boundary_periodic": [_I, _VP, _VP, _I64, _I, _VP],
     "psa_numerical_flux": [_I, _I, _VP, _VP, _VP, _VP, _I64, _D, _VP],
     "psa_apply_operator": [_I, _I, _VP, _VP, _VP, _VP, _I64, _I, _D, _D, _VP],
+    "psa_numerical_flux_rec": [_I, _I, _I, _VP, _VP, _VP, _VP, _I64, _D, _VP],
+    "psa_apply_operator_rec": [_I, _I, _I, _VP, _VP, _VP, _VP, _I64, _I, _D, _D,
+                               _VP],
     "psa_max_abs_interior": [_I, _VP, _VP, _I64, _I, _VP],
     "psa_norm_workspace_bytes": [],
     "psa_interior_norm": [_I, _I, _VP, _I64, _I64, _D, _VP, _VP, ctypes.c_size_t,

@@ -50,18 +50,21 @@ class FiniteVolumeScheme(AdvectionScheme, FiniteVolumeSchemeBase):
 
 
 def _eps_and_check(scheme: AdvectionScheme) -> float:
-    from pyshocks_amd.reconstruction import WENOJS53
-
-    if not isinstance(scheme.rec, WENOJS53):
-        raise NotImplementedError(
-            f"reconstruction {type(scheme.rec).__name__} is not on the GPU path")
+    _rec_and_check(scheme)
     return float(scheme.rec.eps)
+
+
+def _rec_and_check(scheme: AdvectionScheme) -> int:
+    """``PSA_REC_*`` of the reconstruction (learned weights are for Burgers)."""
+    from pyshocks_amd.reconstruction import rec_code
+
+    return rec_code(scheme.rec)
 
 
 def upwind_flux(scheme: AdvectionScheme, grid: Grid, bc: Boundary,
                 u: torch.Tensor) -> torch.Tensor:
     """``F = ur if (ar + al) / 2 > 0 else ul`` with ``u`` and the velocity
-    both reconstructed by WENO-JS5."""
+    both reconstructed by the scheme's reconstruction."""
     assert scheme.velocity is not None
     assert scheme.rec is not None
     assert u.shape[0] == grid.x.numel()
@@ -69,7 +72,7 @@ def upwind_flux(scheme: AdvectionScheme, grid: Grid, bc: Boundary,
     from pyshocks_amd.scalar import _numerical_flux
 
     return _numerical_flux(_lib.PSA_FLUX_ADVECTION_UPWIND, u, scheme.velocity,
-                           None, _eps_and_check(scheme))
+                           None, _eps_and_check(scheme), _rec_and_check(scheme))
 
 
 @dataclass(frozen=True)
@@ -96,10 +99,10 @@ def _apply_operator_advection(scheme: Godunov, grid: Grid, bc: Boundary, t: Any,
             f"boundary {type(bc).__name__} is not on the GPU path")
     _lib.require_device(u, scheme.velocity)
     out = torch.empty_like(u)
-    _lib.call("psa_apply_operator", _lib.dtype_code(u.dtype),
-              _lib.PSA_FLUX_ADVECTION_UPWIND, _lib.ptr(u), _lib.ptr(scheme.velocity),
-              None, _lib.ptr(out), u.numel(), grid.nghosts, grid.h,
-              _eps_and_check(scheme), _lib.stream_of(u))
+    _lib.call("psa_apply_operator_rec", _lib.dtype_code(u.dtype),
+              _lib.PSA_FLUX_ADVECTION_UPWIND, _rec_and_check(scheme), _lib.ptr(u),
+              _lib.ptr(scheme.velocity), None, _lib.ptr(out), u.numel(), grid.nghosts,
+              grid.h, _eps_and_check(scheme), _lib.stream_of(u))
     return out
 
 

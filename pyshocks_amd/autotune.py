@@ -1,7 +1,7 @@
 """Tile-size autotuner for the fused stage kernels and the batched solver.
 
 Candidates are timed on the device with events on the launch stream; the
-winner is cached per (dtype, flux, velocity mode, size class) for the stage
+winner is cached per (dtype, flux, velocity mode, reconstruction, size class) for the stage
 kernels and per (dtype, flux) for the batched solver, and used by
 :func:`~pyshocks_amd.timestepping.fused_plan` and
 :class:`~pyshocks_amd.batched.BatchedSolver` when no tiles are given. A
@@ -25,7 +25,7 @@ _BATCH_CACHE: dict[tuple[Any, ...], tuple[int, int]] = {}
 
 def _stage_key(desc: Any) -> tuple[Any, ...]:
     vel = 0 if desc.flux != 2 else (1 if desc.velocity_is_const else 2)
-    return (int(desc.dtype), int(desc.flux), vel, bool(desc.mlp),
+    return (int(desc.dtype), int(desc.flux), vel, bool(desc.mlp), int(desc.rec),
             max(0, math.ceil(math.log2(max(desc.n, 1)))))
 
 

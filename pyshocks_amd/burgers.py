@@ -146,7 +146,7 @@ def _apply_operator_burgers(scheme: FiniteVolumeScheme, grid: Grid, bc: Boundary
                             t: Any, u: torch.Tensor) -> torch.Tensor:
     """``schemes.py:339-346`` fused: periodic ghost fill, WENO-JS5 faces and
     the conservative difference in one kernel."""
-    from pyshocks_amd.scalar import PeriodicBoundary, _eps_of, rusanov_nu
+    from pyshocks_amd.scalar import PeriodicBoundary, _eps_of, _rec_of, rusanov_nu
 
     if not isinstance(bc, PeriodicBoundary):
         raise NotImplementedError(
@@ -174,9 +174,10 @@ def _apply_operator_burgers(scheme: FiniteVolumeScheme, grid: Grid, bc: Boundary
         _lib.call("psa_ssprk33_stage", ctypes.byref(desc), 0, None, _lib.ptr(u),
                   _lib.ptr(out), None, None, _lib.stream_of(u))
         return out
-    _lib.call("psa_apply_operator", _lib.dtype_code(u.dtype), flux_kind(scheme),
-              _lib.ptr(u), _lib.ptr(speed), _lib.ptr(nu), _lib.ptr(out), u.numel(),
-              grid.nghosts, grid.h, _eps_of(scheme), _lib.stream_of(u))
+    _lib.call("psa_apply_operator_rec", _lib.dtype_code(u.dtype), flux_kind(scheme),
+              _rec_of(scheme), _lib.ptr(u), _lib.ptr(speed), _lib.ptr(nu),
+              _lib.ptr(out), u.numel(), grid.nghosts, grid.h, _eps_of(scheme),
+              _lib.stream_of(u))
     return out
 
 

@@ -421,6 +421,9 @@ int check_vjp_desc(const psa_solver_desc *d) {
   if (d == nullptr) return set_error(PSA_ERR_INVALID_ARGUMENT, "null desc");
   if (d->dtype != PSA_FLOAT64 && d->dtype != PSA_FLOAT32)
     return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported dtype");
+  if (d->rec != PSA_REC_WENOJS53)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the adjoint supports the WENO-JS5 reconstruction only");
   if (d->g < 3 || d->n < d->g)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "bad grid size");
   if (d->flux == PSA_FLUX_LAX_FRIEDRICHS)

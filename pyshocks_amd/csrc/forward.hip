@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "../../include/pyshocks_amd.h"
 #include "psa_internal.hpp"
@@ -80,16 +81,26 @@ __global__ void k_weno_fields(const T *__restrict__ u, int64_t m, T eps,
   }
 }
 
-template <typename T>
+// Both states of the cells of f as given ('same' convolutions, zero padding):
+// REC = PSA_REC_WENOJS53, _WENOJS32 or _ESWENO32.
+template <typename T, int REC>
 __global__ void k_weno_reconstruct(const T *__restrict__ f, int64_t m, T eps,
                                    T *__restrict__ ul, T *__restrict__ ur) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  const T um2 = load_raw(f, i - 2, m), um1 = load_raw(f, i - 1, m);
+  const T um1 = load_raw(f, i - 1, m);
   const T u0 = f[i];
-  const T up1 = load_raw(f, i + 1, m), up2 = load_raw(f, i + 2, m);
-  ur[i] = weno_side_ref(um2, um1, u0, up1, up2, eps);
-  ul[i] = weno_side_ref(up2, up1, u0, um1, um2, eps);
+  const T up1 = load_raw(f, i + 1, m);
+  if constexpr (REC == PSA_REC_WENOJS53) {
+    const T um2 = load_raw(f, i - 2, m), up2 = load_raw(f, i + 2, m);
+    ur[i] = weno_side_ref(um2, um1, u0, up1, up2, eps);
+    ul[i] = weno_side_ref(up2, up1, u0, um1, um2, eps);
+  } else {
+    constexpr bool es = REC == PSA_REC_ESWENO32;
+    const T tau = (es && i > 0 && i < m - 1) ? weno3_tau_ref(um1, u0, up1) : T(0);
+    ur[i] = weno3_side_ref(um1, u0, up1, eps, es, tau);
+    ul[i] = weno3_side_ref(up1, u0, um1, eps, es, tau);
+  }
 }
 
 template <typename T>
@@ -108,7 +119,26 @@ __global__ void k_apply_boundary(const T *__restrict__ u, T *__restrict__ out,
 
 // numerical_flux on u as given: one thread per face j in [0, m], face j lies
 // between cells j - 1 and j; fnum[0] = fnum[m] = 0 (jnp.pad(fnum, 1)).
-template <typename T, int FLUX>
+// The states either side of face j (between cells j - 1 and j) from the
+// cells v = j - 3 .. j + 2 of an array of m cells as given: ur of cell j - 1
+// and ul of cell j, by the reconstruction REC.
+template <typename T, int REC>
+__device__ __forceinline__ void face_states(const T *v, int64_t j, int64_t m,
+                                            T eps, T &urL, T &ulR) {
+  if constexpr (REC == PSA_REC_WENOJS53) {
+    urL = weno_side_ref(v[0], v[1], v[2], v[3], v[4], eps);
+    ulR = weno_side_ref(v[5], v[4], v[3], v[2], v[1], eps);
+  } else {
+    // cells j - 2 .. j + 1 are v[1] .. v[4]; tau is zero at the array ends
+    constexpr bool es = REC == PSA_REC_ESWENO32;
+    const T tL = (es && j - 1 > 0) ? weno3_tau_ref(v[1], v[2], v[3]) : T(0);
+    const T tR = (es && j < m - 1) ? weno3_tau_ref(v[2], v[3], v[4]) : T(0);
+    urL = weno3_side_ref(v[1], v[2], v[3], eps, es, tL);
+    ulR = weno3_side_ref(v[4], v[3], v[2], eps, es, tR);
+  }
+}
+
+template <typename T, int FLUX, int REC = PSA_REC_WENOJS53>
 __global__ void k_numerical_flux(const T *__restrict__ u,
                                  const T *__restrict__ vel,
                                  const T *__restrict__ nu, int64_t m, T eps,
@@ -124,8 +154,8 @@ __global__ void k_numerical_flux(const T *__restrict__ u,
 #pragma unroll
   for (int k = 0; k < 6; ++k) v[k] = load_raw(u, j - 3 + k, m);
   // ur of cell j - 1 and ul of cell j
-  const T urL = weno_side_ref(v[0], v[1], v[2], v[3], v[4], eps);
-  const T ulR = weno_side_ref(v[5], v[4], v[3], v[2], v[1], eps);
+  T urL, ulR;
+  face_states<T, REC>(v, j, m, eps, urL, ulR);
   T F;
   if constexpr (FLUX == PSA_FLUX_RUSANOV) {
     const T nuj = nu != nullptr ? nu[j - 1] : T(1);
@@ -142,8 +172,8 @@ __global__ void k_numerical_flux(const T *__restrict__ u,
     T a[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) a[k] = load_raw(vel, j - 3 + k, m);
-    const T arL = weno_side_ref(a[0], a[1], a[2], a[3], a[4], eps);
-    const T alR = weno_side_ref(a[5], a[4], a[3], a[2], a[1], eps);
+    T arL, alR;
+    face_states<T, REC>(a, j, m, eps, arL, alR);
     F = flux_advection_upwind(urL, ulR, arL, alR);
   }
   fnum[j] = F;
@@ -359,7 +389,7 @@ __device__ __forceinline__ void stage_reduce(const StageArgs<T> &p,
 }
 
 template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT,
-          bool LEARNED>
+          bool LEARNED, int REC = 0>
 __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   constexpr int TILE = BLOCK * CPT;
   __shared__ StageSmem<T, VEL, BLOCK, CPT> sm;
@@ -379,8 +409,8 @@ __global__ __launch_bounds__(BLOCK) void k_stage(StageArgs<T> p) {
   unsigned long long vmax = 0;
   const int64_t tile = blockIdx.x;  // the grid has one workgroup per tile
   if (tile < ntiles) {
-    stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT, LEARNED>(pl, tile * TILE, dt,
-                                                         sm, vmax, q);
+    stage_tile<T, FLUX, VEL, STAGE, BLOCK, CPT, LEARNED, false, false, REC>(
+        pl, tile * TILE, dt, sm, vmax, q);
   }
   (void)tid;
   stage_reduce<T, STAGE, FLUX, BLOCK>(p, vmax, sm.red);
@@ -522,7 +552,8 @@ __global__ void k_step_finish(psa_control *ctrl) {
 // reduce max|u_int| of the new state into the step slots for k_step_finish.
 // Without a control block the step is taken with the device scalar p.dt_ptr
 // (and p.k1out, p.k2out may receive the stage states).
-template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED>
+template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED,
+          int REC = 0>
 __global__ __launch_bounds__(BLOCK) void k_step(StageArgs<T> p) {
   static_assert(VEL != 2, "no velocity array in the whole-step kernel");
   using SM = StepSmem<T, BLOCK, CPT>;
@@ -535,8 +566,8 @@ __global__ __launch_bounds__(BLOCK) void k_step(StageArgs<T> p) {
   if (ctrl == nullptr) {
     // a fixed step of the device scalar dt (the adjoint's forward passes)
     unsigned long long vmax = 0;
-    step_tile<T, FLUX, VEL, BLOCK, CPT, LEARNED>(p, A, *p.dt_ptr, p.uk, p.out, sm, vmax,
-                                                 q, nullptr);
+    step_tile<T, FLUX, VEL, BLOCK, CPT, LEARNED, REC>(p, A, *p.dt_ptr, p.uk, p.out, sm,
+                                                      vmax, q, nullptr);
     return;
   }
   if (ctrl->active_next == 0) {
@@ -549,8 +580,8 @@ __global__ __launch_bounds__(BLOCK) void k_step(StageArgs<T> p) {
   }
   const T dt = static_cast<T>(ctrl->dt_next);
   unsigned long long vmax = 0;
-  step_tile<T, FLUX, VEL, BLOCK, CPT, LEARNED>(p, A, dt, p.uk, p.out, sm, vmax, q,
-                                               nullptr);
+  step_tile<T, FLUX, VEL, BLOCK, CPT, LEARNED, REC>(p, A, dt, p.uk, p.out, sm, vmax, q,
+                                                    nullptr);
 
   // ---- partial max|u| of the new state
   if (ctrl->speed_from_state != 0) {
@@ -591,11 +622,11 @@ int check_dtype(int dtype) {
   return PSA_OK;
 }
 
-int check_grid(int64_t m, int g) {
-  if (g < 3) {
+int check_grid(int64_t m, int g, int gmin = 3) {
+  if (g < gmin) {
     return set_error(PSA_ERR_INVALID_ARGUMENT,
-                     "WENO-JS5 needs at least 3 ghost cells (nghosts >= "
-                     "stencil_width), got " +
+                     "the reconstruction needs at least " + std::to_string(gmin) +
+                         " ghost cells (nghosts >= stencil_width), got " +
                          std::to_string(g));
   }
   if (m - 2 * static_cast<int64_t>(g) < g) {
@@ -662,7 +693,7 @@ unsigned int resident_blocks(K kernel, int block) {
 }
 
 template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT,
-          bool LEARNED = false>
+          bool LEARNED = false, int REC = 0>
 int launch_stage_cfg(const StageArgs<T> &args, hipStream_t stream) {
   constexpr int TILE = BLOCK * CPT;
   StageArgs<T> a = args;
@@ -670,7 +701,7 @@ int launch_stage_cfg(const StageArgs<T> &args, hipStream_t stream) {
               (a.un == nullptr || aligned16(a.un)) &&
               (a.vel == nullptr || aligned16(a.vel));
   const unsigned int blocks = blocks_for(a.m, TILE);
-  hipLaunchKernelGGL((k_stage<T, FLUX, VEL, STAGE, BLOCK, CPT, LEARNED>),
+  hipLaunchKernelGGL((k_stage<T, FLUX, VEL, STAGE, BLOCK, CPT, LEARNED, REC>),
                      dim3(blocks), dim3(BLOCK), 0, stream, a);
   return check_launch("ssprk33 stage");
 }
@@ -719,24 +750,25 @@ int launch_stage_tiles(const StageArgs<T> &a, TileConfig cfg,
 
 // Whole-step kernel: Burgers (WENO-JS or learned weights) and advection
 // with a constant velocity.
-template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED>
+template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED,
+          int REC = 0>
 int launch_step_cfg(const StageArgs<T> &args, hipStream_t stream) {
   constexpr int TO = StepSmem<T, BLOCK, CPT>::TO;
   StageArgs<T> a = args;
   a.aligned = aligned16(a.uk) && aligned16(a.out);
   const unsigned int blocks = blocks_for(a.m, TO);
-  hipLaunchKernelGGL((k_step<T, FLUX, VEL, BLOCK, CPT, LEARNED>), dim3(blocks),
+  hipLaunchKernelGGL((k_step<T, FLUX, VEL, BLOCK, CPT, LEARNED, REC>), dim3(blocks),
                      dim3(BLOCK), 0, stream, a);
   return check_launch("ssprk33 step");
 }
 
-template <typename T, int FLUX, int VEL, bool LEARNED>
+template <typename T, int FLUX, int VEL, bool LEARNED, int REC = 0>
 int launch_step_tiles(const StageArgs<T> &a, TileConfig cfg,
                       hipStream_t stream) {
   if (cfg.block == 256 && cfg.cpt == 4)
-    return launch_step_cfg<T, FLUX, VEL, 256, 4, LEARNED>(a, stream);
+    return launch_step_cfg<T, FLUX, VEL, 256, 4, LEARNED, REC>(a, stream);
   if (cfg.block == 256 && cfg.cpt == 2)
-    return launch_step_cfg<T, FLUX, VEL, 256, 2, LEARNED>(a, stream);
+    return launch_step_cfg<T, FLUX, VEL, 256, 2, LEARNED, REC>(a, stream);
   return set_error(PSA_ERR_INVALID_ARGUMENT,
                    "unsupported tile configuration for the whole-step kernel block=" +
                        std::to_string(cfg.block) +
@@ -755,6 +787,15 @@ inline bool step_applies(const psa_solver_desc *d, TileConfig cfg, int vel) {
 template <typename T>
 int launch_step(const psa_solver_desc *d, int vel, const StageArgs<T> &a,
                 TileConfig cfg, hipStream_t stream) {
+  if (a.rec3) {
+    if (d->flux == PSA_FLUX_RUSANOV)
+      return launch_step_tiles<T, PSA_FLUX_RUSANOV, 0, false, 1>(a, cfg, stream);
+    if (d->flux == PSA_FLUX_BURGERS_UPWIND)
+      return launch_step_tiles<T, PSA_FLUX_BURGERS_UPWIND, 0, false, 1>(a, cfg, stream);
+    if (d->flux == PSA_FLUX_ENGQUIST_OSHER)
+      return launch_step_tiles<T, PSA_FLUX_ENGQUIST_OSHER, 0, false, 1>(a, cfg, stream);
+    return launch_step_tiles<T, PSA_FLUX_ADVECTION_UPWIND, 1, false, 1>(a, cfg, stream);
+  }
   if (d->mlp != nullptr) {
     if (d->flux == PSA_FLUX_RUSANOV)
       return launch_step_tiles<T, PSA_FLUX_RUSANOV, 0, true>(a, cfg, stream);
@@ -784,9 +825,41 @@ int launch_stage_learned(const StageArgs<T> &a, TileConfig cfg,
                        " cells_per_thread=" + std::to_string(cfg.cpt));
 }
 
+// Third-order reconstructions: the stage kernels with tiles 256 x 4 or
+// 256 x 2 (the pipelined schedule takes these too).
+template <typename T, int FLUX, int VEL, int STAGE>
+int launch_stage_tiles3(const StageArgs<T> &a, TileConfig cfg,
+                        hipStream_t stream) {
+  if (cfg.block == 256 && cfg.cpt == 4)
+    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 4, false, 1>(a, stream);
+  if (cfg.block == 256 && cfg.cpt == 2)
+    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 2, false, 1>(a, stream);
+  return set_error(PSA_ERR_INVALID_ARGUMENT,
+                   "unsupported tile configuration for the third-order "
+                   "reconstructions block=" + std::to_string(cfg.block) +
+                       " cells_per_thread=" + std::to_string(cfg.cpt));
+}
+
+template <typename T, int STAGE>
+int launch_stage3(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
+                  hipStream_t stream) {
+  if (flux == PSA_FLUX_RUSANOV)
+    return launch_stage_tiles3<T, PSA_FLUX_RUSANOV, 0, STAGE>(a, cfg, stream);
+  if (flux == PSA_FLUX_BURGERS_UPWIND)
+    return launch_stage_tiles3<T, PSA_FLUX_BURGERS_UPWIND, 0, STAGE>(a, cfg, stream);
+  if (flux == PSA_FLUX_ENGQUIST_OSHER)
+    return launch_stage_tiles3<T, PSA_FLUX_ENGQUIST_OSHER, 0, STAGE>(a, cfg, stream);
+  if (flux == PSA_FLUX_LAX_FRIEDRICHS)
+    return launch_stage_tiles3<T, PSA_FLUX_LAX_FRIEDRICHS, 0, STAGE>(a, cfg, stream);
+  if (flux == PSA_FLUX_ADVECTION_UPWIND && vel == 1)
+    return launch_stage_tiles3<T, PSA_FLUX_ADVECTION_UPWIND, 1, STAGE>(a, cfg, stream);
+  return launch_stage_tiles3<T, PSA_FLUX_ADVECTION_UPWIND, 2, STAGE>(a, cfg, stream);
+}
+
 template <typename T, int STAGE>
 int launch_stage(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
                  hipStream_t stream) {
+  if (a.rec3) return launch_stage3<T, STAGE>(flux, vel, a, cfg, stream);
   if (a.mlp != nullptr) {
     if (flux == PSA_FLUX_RUSANOV)
       return launch_stage_learned<T, PSA_FLUX_RUSANOV, STAGE>(a, cfg, stream);
@@ -828,8 +901,16 @@ int check_desc(const psa_solver_desc *d) {
   int st = check_dtype(d->dtype);
   if (st != PSA_OK) return st;
   if (d->n <= 0) return set_error(PSA_ERR_INVALID_ARGUMENT, "n must be > 0");
-  st = check_grid(d->n + 2 * static_cast<int64_t>(d->g), d->g);
+  if (d->rec < PSA_REC_WENOJS53 || d->rec > PSA_REC_ESWENO32)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "unsupported reconstruction " + std::to_string(d->rec));
+  st = check_grid(d->n + 2 * static_cast<int64_t>(d->g), d->g,
+                  d->rec == PSA_REC_WENOJS53 ? 3 : 2);
   if (st != PSA_OK) return st;
+  if (d->rec != PSA_REC_WENOJS53 && d->mlp != nullptr)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "learned weights replace the WENO-JS5 weights only");
+
   if (!(d->dx > 0.0) || !std::isfinite(d->dx))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "dx must be positive");
   if (!(d->eps >= 0.0))
@@ -883,6 +964,8 @@ StageArgs<T> make_args(const psa_solver_desc *d) {
   }
   a.nu = static_cast<const T *>(d->nu);
   a.aligned = 1;
+  a.es = d->rec == PSA_REC_ESWENO32 ? 1 : 0;
+  a.rec3 = d->rec != PSA_REC_WENOJS53 ? 1 : 0;
   a.mlp = d->mlp;
   a.tau = static_cast<T>(d->mlp_tau);
   a.sigma = static_cast<T>(d->mlp_sigma);
@@ -1017,35 +1100,43 @@ int weno_fields(int dtype, const void *u, void *beta, void *uhat,
   return check_launch("weno_fields");
 }
 
-template <typename T>
-int numerical_flux_t(int flux, const void *u, const void *velocity,
-                            const void *nu, void *fnum, int64_t m, double eps,
-                            hipStream_t s) {
+template <typename T, int REC>
+void launch_numerical_flux(int flux, const T *uu, const T *vv, const T *nn, T *ff,
+                           int64_t m, T e, hipStream_t s) {
   const unsigned int blocks = blocks_for(m + 1, kThreads);
+  if (flux == PSA_FLUX_RUSANOV) {
+    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_RUSANOV, REC>), dim3(blocks),
+                       dim3(kThreads), 0, s, uu, vv, nn, m, e, ff);
+  } else if (flux == PSA_FLUX_BURGERS_UPWIND) {
+    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_BURGERS_UPWIND, REC>),
+                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e, ff);
+  } else if (flux == PSA_FLUX_ENGQUIST_OSHER) {
+    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_ENGQUIST_OSHER, REC>),
+                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e, ff);
+  } else if (flux == PSA_FLUX_LAX_FRIEDRICHS) {
+    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_LAX_FRIEDRICHS, REC>),
+                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e, ff);
+  } else {
+    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_ADVECTION_UPWIND, REC>),
+                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e, ff);
+  }
+}
+
+template <typename T>
+int numerical_flux_t(int flux, int rec, const void *u, const void *velocity,
+                     const void *nu, void *fnum, int64_t m, double eps,
+                     hipStream_t s) {
   const T *uu = static_cast<const T *>(u);
   const T *vv = static_cast<const T *>(velocity);
   const T *nn = static_cast<const T *>(nu);
   T *ff = static_cast<T *>(fnum);
   const T e = static_cast<T>(eps);
-  if (flux == PSA_FLUX_RUSANOV) {
-    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_RUSANOV>), dim3(blocks),
-                       dim3(kThreads), 0, s, uu, vv, nn, m, e, ff);
-  } else if (flux == PSA_FLUX_BURGERS_UPWIND) {
-    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_BURGERS_UPWIND>),
-                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e,
-                       ff);
-  } else if (flux == PSA_FLUX_ENGQUIST_OSHER) {
-    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_ENGQUIST_OSHER>),
-                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e,
-                       ff);
-  } else if (flux == PSA_FLUX_LAX_FRIEDRICHS) {
-    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_LAX_FRIEDRICHS>),
-                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e,
-                       ff);
+  if (rec == PSA_REC_WENOJS32) {
+    launch_numerical_flux<T, PSA_REC_WENOJS32>(flux, uu, vv, nn, ff, m, e, s);
+  } else if (rec == PSA_REC_ESWENO32) {
+    launch_numerical_flux<T, PSA_REC_ESWENO32>(flux, uu, vv, nn, ff, m, e, s);
   } else {
-    hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_ADVECTION_UPWIND>),
-                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e,
-                       ff);
+    launch_numerical_flux<T, PSA_REC_WENOJS53>(flux, uu, vv, nn, ff, m, e, s);
   }
   return check_launch("numerical_flux");
 }
@@ -1099,26 +1190,42 @@ int psa_weno_js_weights(int dtype, const void *u, void *omega, int64_t m,
   return weno_fields(dtype, u, nullptr, nullptr, omega, m, eps, stream);
 }
 
-int psa_weno_reconstruct(int dtype, const void *f, void *ul, void *ur,
-                         int64_t m, double eps, void *stream) {
+int psa_reconstruct(int dtype, int rec, const void *f, void *ul, void *ur,
+                    int64_t m, double eps, void *stream) {
   int st = check_dtype(dtype);
   if (st != PSA_OK) return st;
+  if (rec < PSA_REC_WENOJS53 || rec > PSA_REC_ESWENO32)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "unsupported reconstruction " + std::to_string(rec));
   if (f == nullptr || ul == nullptr || ur == nullptr || m <= 0)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer or m <= 0");
   auto s = static_cast<hipStream_t>(stream);
   const unsigned int blocks = blocks_for(m, kThreads);
+  auto launch = [&](auto tag, auto rtag) {
+    using T = decltype(tag);
+    constexpr int R = decltype(rtag)::value;
+    hipLaunchKernelGGL((k_weno_reconstruct<T, R>), dim3(blocks), dim3(kThreads), 0,
+                       s, static_cast<const T *>(f), m, static_cast<T>(eps),
+                       static_cast<T *>(ul), static_cast<T *>(ur));
+  };
+  using R0 = std::integral_constant<int, PSA_REC_WENOJS53>;
+  using R1 = std::integral_constant<int, PSA_REC_WENOJS32>;
+  using R2 = std::integral_constant<int, PSA_REC_ESWENO32>;
   if (dtype == PSA_FLOAT64) {
-    hipLaunchKernelGGL(k_weno_reconstruct<double>, dim3(blocks),
-                       dim3(kThreads), 0, s, static_cast<const double *>(f), m,
-                       eps, static_cast<double *>(ul),
-                       static_cast<double *>(ur));
+    if (rec == PSA_REC_WENOJS53) launch(0.0, R0{});
+    if (rec == PSA_REC_WENOJS32) launch(0.0, R1{});
+    if (rec == PSA_REC_ESWENO32) launch(0.0, R2{});
   } else {
-    hipLaunchKernelGGL(k_weno_reconstruct<float>, dim3(blocks), dim3(kThreads),
-                       0, s, static_cast<const float *>(f), m,
-                       static_cast<float>(eps), static_cast<float *>(ul),
-                       static_cast<float *>(ur));
+    if (rec == PSA_REC_WENOJS53) launch(0.0f, R0{});
+    if (rec == PSA_REC_WENOJS32) launch(0.0f, R1{});
+    if (rec == PSA_REC_ESWENO32) launch(0.0f, R2{});
   }
-  return check_launch("weno_reconstruct");
+  return check_launch("reconstruct");
+}
+
+int psa_weno_reconstruct(int dtype, const void *f, void *ul, void *ur,
+                         int64_t m, double eps, void *stream) {
+  return psa_reconstruct(dtype, PSA_REC_WENOJS53, f, ul, ur, m, eps, stream);
 }
 
 int psa_apply_boundary_periodic(int dtype, const void *u, void *out,
@@ -1147,8 +1254,18 @@ int psa_apply_boundary_periodic(int dtype, const void *u, void *out,
 int psa_numerical_flux(int dtype, int flux, const void *u,
                        const void *velocity, const void *nu, void *fnum,
                        int64_t m, double eps, void *stream) {
+  return psa_numerical_flux_rec(dtype, flux, PSA_REC_WENOJS53, u, velocity, nu,
+                                fnum, m, eps, stream);
+}
+
+int psa_numerical_flux_rec(int dtype, int flux, int rec, const void *u,
+                           const void *velocity, const void *nu, void *fnum,
+                           int64_t m, double eps, void *stream) {
   int st = check_dtype(dtype);
   if (st != PSA_OK) return st;
+  if (rec < PSA_REC_WENOJS53 || rec > PSA_REC_ESWENO32)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "unsupported reconstruction " + std::to_string(rec));
   if (flux < 0 || flux > 4)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "unsupported flux " + std::to_string(flux));
@@ -1162,16 +1279,25 @@ int psa_numerical_flux(int dtype, int flux, const void *u,
                      "the Lax-Friedrichs flux needs the device scalar max|a|");
   auto s = static_cast<hipStream_t>(stream);
   if (dtype == PSA_FLOAT64)
-    return numerical_flux_t<double>(flux, u, velocity, nu, fnum, m, eps, s);
-  return numerical_flux_t<float>(flux, u, velocity, nu, fnum, m, eps, s);
+    return numerical_flux_t<double>(flux, rec, u, velocity, nu, fnum, m, eps, s);
+  return numerical_flux_t<float>(flux, rec, u, velocity, nu, fnum, m, eps, s);
 }
 
 int psa_apply_operator(int dtype, int flux, const void *u,
                        const void *velocity, const void *nu, void *out,
                        int64_t m, int g, double dx, double eps, void *stream) {
+  return psa_apply_operator_rec(dtype, flux, PSA_REC_WENOJS53, u, velocity, nu,
+                                out, m, g, dx, eps, stream);
+}
+
+int psa_apply_operator_rec(int dtype, int flux, int rec, const void *u,
+                           const void *velocity, const void *nu, void *out,
+                           int64_t m, int g, double dx, double eps,
+                           void *stream) {
   psa_solver_desc d{};
   d.dtype = dtype;
   d.flux = flux;
+  d.rec = rec;
   d.g = g;
   d.n = m - 2 * static_cast<int64_t>(g);
   d.dx = dx;

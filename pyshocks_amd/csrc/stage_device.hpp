@@ -116,6 +116,10 @@ struct StageArgs {
   // Lax-Friedrichs stage kernels: the slots whose maximum is the flux speed,
   // gathered once the tile's loads are in flight (null: vconst holds it)
   const unsigned long long (*speed_slots)[PSA_UMAX_SLOT_STRIDE];
+  int es;            // third-order reconstruction: ES-WENO weights (1) or
+                     //   WENO-JS (0)
+  int rec3;          // host side: the kernels take the third-order
+                     //   reconstruction (REC = 1)
   const float *mlp;  // learned weights (LEARNED kernels)
   T tau;             // their shock threshold
   T sigma;           // their smooth threshold
@@ -404,6 +408,46 @@ __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
   if constexpr (LEARNED) learned_cells<T, CPT>(src, w, regime, ul, ur, q);
 }
 
+// Third-order reconstructions (WENO-JS3 or ES-WENO3 weights, weno3_pair) of
+// CPT cells from the layout of reconstruct_cells (w holds cells -2 ..
+// CPT + 1; cells -1 .. CPT are used). EDGE (the operator over the whole
+// array): ES-WENO's tau is zero for the first and last cell of the array,
+// gfirst being the global index of cell 0.
+template <typename T, int CPT, bool EDGE>
+__device__ __forceinline__ void reconstruct_cells3(const StageArgs<T> &p,
+                                                   const T *w, T *ul, T *ur,
+                                                   int64_t gfirst) {
+  // D[k] = u(k) - u(k - 1)
+  T D[CPT + 1];
+#pragma unroll
+  for (int k = 0; k <= CPT; ++k) D[k] = w[k + 2] - w[k + 1];
+  const bool es = p.es != 0;
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) {
+    bool tz = false;
+    if constexpr (EDGE) {
+      const int64_t gm = gfirst + r;
+      tz = gm == 0 || gm == p.m - 1;
+    }
+    weno3_pair(w[r + 2], D[r], D[r + 1], p.eps, es, tz, ul[r], ur[r]);
+  }
+}
+
+// The reconstruction of a fused kernel: REC 0 WENO-JS5 (or learned weights),
+// REC 1 third order (reconstruct_cells3).
+template <typename T, int CPT, bool LEARNED, int REC, bool EDGE = false>
+__device__ __forceinline__ void reconstruct_any(const StageArgs<T> &p,
+                                                const T *w, T *ul, T *ur,
+                                                const MlpSource &src,
+                                                unsigned int *q,
+                                                int64_t gfirst) {
+  if constexpr (REC == 1) {
+    reconstruct_cells3<T, CPT, EDGE>(p, w, ul, ur, gfirst);
+  } else {
+    reconstruct_cells<T, CPT, LEARNED>(p, w, ul, ur, src, q);
+  }
+}
+
 // Face flux of the stage kernels for Rusanov, Lax-Friedrichs (global speed
 // vconst) with nu = 1 and Engquist-Osher (all three doubled, see
 // stage_lambda), Godunov, or advection with a constant velocity.
@@ -495,7 +539,8 @@ struct StageSmem {
 // written. Ends with the LDS free for reuse only after the caller's next
 // barrier.
 template <typename T, int FLUX, int VEL, int STAGE, int BLOCK, int CPT,
-          bool LEARNED = false, bool PRELOADED = false, bool ASYNC = false>
+          bool LEARNED = false, bool PRELOADED = false, bool ASYNC = false,
+          int REC = 0>
 __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
                                            T dt,
                                            StageSmem<T, VEL, BLOCK, CPT> &sm,
@@ -503,6 +548,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
                                            unsigned int *q = nullptr,
                                            const MlpWaveLds *wl = nullptr) {
   static_assert(!LEARNED || VEL == 0, "learned weights are for Burgers");
+  static_assert(REC == 0 || !LEARNED,
+                "learned weights replace the WENO-JS5 weights only");
+  constexpr bool EDGE = REC == 1 && STAGE == 0;
   constexpr int TILE = BLOCK * CPT;
   constexpr int HALO = 4;
   constexpr int VN = Vec16<T>::n;
@@ -570,10 +618,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   for (int k = 0; k < CPT + 4; ++k) w[k] = sm.u[l0 + 2 + k];
   T ul[CPT], ur[CPT];
   const MlpSource src{p.mlp, wl};
-  reconstruct_cells<T, CPT, LEARNED>(p, w, ul, ur, src, q);
+  reconstruct_any<T, CPT, LEARNED, REC, EDGE>(p, w, ul, ur, src, q, S + l0);
 
   T al[VEL == 2 ? CPT : 1], ar[VEL == 2 ? CPT : 1];
-  if constexpr (VEL == 2) {
+  if constexpr (VEL == 2 && REC == 1) {
+    // the velocity is reconstructed the same way (advection/schemes.py:108)
+    reconstruct_cells3<T, CPT, EDGE>(p, &sm.v[l0 + 2], al, ar, S + l0);
+  } else if constexpr (VEL == 2) {
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
       const int b = l0 + HALO + r;
@@ -612,9 +663,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   if (tid == 0) {
     // right state of local cell -1 (the arithmetic of every other cell)
     T el, er;
-    reconstruct_cells<T, 1, false>(p, &sm.u[1], &el, &er, src, nullptr);
+    reconstruct_any<T, 1, false, REC, EDGE>(p, &sm.u[1], &el, &er, src, nullptr,
+                                            S - 1);
     sm.urL[0] = er;
-    if constexpr (VEL == 2) {
+    if constexpr (VEL == 2 && REC == 1) {
+      T al1, ar1;
+      reconstruct_cells3<T, 1, EDGE>(p, &sm.v[1], &al1, &ar1, S - 1);
+      sm.arL[0] = ar1;
+    } else if constexpr (VEL == 2) {
       sm.arL[0] = weno_right(sm.v[1], sm.v[2], sm.v[3], sm.v[4], sm.v[5], p.eps);
     }
   }
@@ -622,9 +678,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     // left state of local cell TILE
     constexpr int c = TILE + HALO;
     T el, er;
-    reconstruct_cells<T, 1, false>(p, &sm.u[c - 2], &el, &er, src, nullptr);
+    reconstruct_any<T, 1, false, REC, EDGE>(p, &sm.u[c - 2], &el, &er, src,
+                                            nullptr, S + TILE);
     sm.ulF[BLOCK] = el;
-    if constexpr (VEL == 2) {
+    if constexpr (VEL == 2 && REC == 1) {
+      T al1, ar1;
+      reconstruct_cells3<T, 1, EDGE>(p, &sm.v[c - 2], &al1, &ar1, S + TILE);
+      sm.alF[BLOCK] = al1;
+    } else if constexpr (VEL == 2) {
       sm.alF[BLOCK] = weno_right(sm.v[c + 2], sm.v[c + 1], sm.v[c],
                                  sm.v[c - 1], sm.v[c - 2], p.eps);
     }
@@ -769,7 +830,7 @@ struct StepSmem {
 // stage_tile (Rusanov with nu = 1, Godunov, or advection with a constant
 // velocity). un holds the thread's cells of u^n.
 template <typename T, int FLUX, int VEL, int BLOCK, int CPT, int STAGE,
-          bool LEARNED>
+          bool LEARNED, int REC = 0>
 __device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
                                            StepSmem<T, BLOCK, CPT> &sm, T dt,
                                            const T *un, T *val,
@@ -781,7 +842,7 @@ __device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
 #pragma unroll
   for (int k = 0; k < CPT + 4; ++k) w[k] = X[j0 - 2 + k];
   T ul[CPT], ur[CPT];
-  reconstruct_cells<T, CPT, LEARNED>(p, w, ul, ur, src, q);
+  reconstruct_any<T, CPT, LEARNED, REC>(p, w, ul, ur, src, q, 0);
 
   sm.urL[tid + 1] = ur[CPT - 1];
   sm.ulF[tid] = ul[0];
@@ -870,7 +931,8 @@ __device__ __forceinline__ void step_store(const StageArgs<T> &p, int64_t base,
 // the stage states k1 and k2 of the output cells are written as well (they
 // are exact there). Ends with the LDS in use (a barrier before the next
 // tile).
-template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED>
+template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED,
+          int REC = 0>
 __device__ __forceinline__ void step_tile(const StageArgs<T> &p, int64_t A, T dt,
                                           const T *__restrict__ uin,
                                           T *__restrict__ uout,
@@ -920,18 +982,18 @@ __device__ __forceinline__ void step_tile(const StageArgs<T> &p, int64_t A, T dt
   const MlpSource src{p.mlp, wl};
 
   unsigned long long unused = 0;
-  step_stage<T, FLUX, VEL, BLOCK, CPT, 1, LEARNED>(p, sm.x0, sm, dt, un, val, src, q);
+  step_stage<T, FLUX, VEL, BLOCK, CPT, 1, LEARNED, REC>(p, sm.x0, sm, dt, un, val, src, q);
 #pragma unroll
   for (int r = 0; r < CPT; ++r) sm.x1[j0 + r] = val[r];
   if (p.k1out != nullptr) step_store<T, BLOCK, CPT, false>(p, base, fast, val, p.k1out, unused);
   __syncthreads();
   // (every read of x0 ended before the exchange barrier of stage 1)
-  step_stage<T, FLUX, VEL, BLOCK, CPT, 2, LEARNED>(p, sm.x1, sm, dt, un, val, src, q);
+  step_stage<T, FLUX, VEL, BLOCK, CPT, 2, LEARNED, REC>(p, sm.x1, sm, dt, un, val, src, q);
 #pragma unroll
   for (int r = 0; r < CPT; ++r) sm.x0[j0 + r] = val[r];
   if (p.k2out != nullptr) step_store<T, BLOCK, CPT, false>(p, base, fast, val, p.k2out, unused);
   __syncthreads();
-  step_stage<T, FLUX, VEL, BLOCK, CPT, 3, LEARNED>(p, sm.x0, sm, dt, un, val, src, q);
+  step_stage<T, FLUX, VEL, BLOCK, CPT, 3, LEARNED, REC>(p, sm.x0, sm, dt, un, val, src, q);
 
   step_store<T, BLOCK, CPT, true>(p, base, fast, val, uout, vmax);
 }

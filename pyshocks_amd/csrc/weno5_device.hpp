@@ -297,4 +297,91 @@ __device__ __forceinline__ T flux_advection_upwind(T urL, T ulR, T arL,
   return aavg > T(0) ? urL : ulR;
 }
 
+// ---------------------------------------------------------------------------
+// Third-order reconstructions: WENO-JS3 (weno.py:166-203 coefficients,
+// weno.py:247-256 weights, reconstruction.py:320-333) and the ES-WENO weights
+// of the same stencils (weno.py:285-296, reconstruction.py:387-434):
+//   beta0 = (u_i - u_{i-1})^2, beta1 = (u_{i+1} - u_i)^2,
+//   uhat0 = 3/2 u_i - 1/2 u_{i-1}, uhat1 = 1/2 u_{i+1} + 1/2 u_i, d = (1/3, 2/3),
+//   JS: alpha_k = d_k / (eps + beta_k)^2,
+//   ES: alpha_k = d_k (1 + tau / (eps + beta_k)), tau = (u_{i+1} - 2 u_i + u_{i-1})^2
+//       (zero for the first and last cell of the array, jnp.pad(.., 1)).
+// The left state reverses the array, which swaps u_{i-1} and u_{i+1}.
+
+template <typename T>
+struct Weno3Coeffs {
+  static constexpr T d0 = T(1.0 / 3.0);
+  static constexpr T d1 = T(2.0 / 3.0);
+  static constexpr T c00 = T(3.0 / 2.0);
+  static constexpr T c01 = T(-1.0 / 2.0);
+  static constexpr T c1 = T(1.0 / 2.0);
+};
+
+// One side in the reference's expression order (standalone kernels): the
+// right state of cell i from (u_{i-1}, u_i, u_{i+1}), or, with the arguments
+// reversed, its left state. es selects the ES-WENO weights with this tau.
+template <typename T>
+__device__ __forceinline__ T weno3_side_ref(T um1, T u0, T up1, T eps, bool es,
+                                            T tau) {
+  using C = Weno3Coeffs<T>;
+  const T t0 = u0 - um1;
+  const T t1 = up1 - u0;
+  const T b0 = t0 * t0;
+  const T b1 = t1 * t1;
+  const T h0 = C::c00 * u0 + C::c01 * um1;
+  const T h1 = C::c1 * up1 + C::c1 * u0;
+  T a0, a1;
+  if (es) {
+    a0 = C::d0 * (T(1) + tau / (eps + b0));
+    a1 = C::d1 * (T(1) + tau / (eps + b1));
+  } else {
+    const T e0 = eps + b0, e1 = eps + b1;
+    a0 = C::d0 / (e0 * e0);
+    a1 = C::d1 / (e1 * e1);
+  }
+  const T s = a0 + a1;
+  return (a0 / s) * h0 + (a1 / s) * h1;
+}
+
+// tau of ES-WENO for cell i (weno.py:293), (u_{i+1} - 2 u_i) + u_{i-1} squared.
+template <typename T>
+__device__ __forceinline__ T weno3_tau_ref(T um1, T u0, T up1) {
+  const T t = (up1 - T(2) * u0) + um1;
+  return t * t;
+}
+
+// Both states of one cell for the fused kernels from the first differences
+// Dm = u_i - u_{i-1}, Dp = u_{i+1} - u_i: with q_k the JS denominators
+// (eps + beta_k)^2, omega_0 = q1 / (q1 + 2 q0) on the right and
+// q0 / (q0 + 2 q1) on the left (beta0 and beta1 swap); with ES-WENO, r_k =
+// (eps + beta_k) and p_k = r_k + tau, omega_0 = p0 r1 / (p0 r1 + 2 p1 r0) and
+// mirrored. The candidates are u_i plus half a difference.
+template <typename T>
+__device__ __forceinline__ void weno3_pair(T u0, T Dm, T Dp, T eps, bool es,
+                                           bool tau_zero, T &ul, T &ur) {
+  const T r0 = fma(Dm, Dm, eps);  // eps + beta0
+  const T r1 = fma(Dp, Dp, eps);  // eps + beta1
+  // right: uhat0 = u + Dm / 2, uhat1 = u + Dp / 2; left: u - Dp / 2, u - Dm / 2
+  const T hr0 = fma(T(0.5), Dm, u0), hr1 = fma(T(0.5), Dp, u0);
+  const T hl0 = fma(T(-0.5), Dp, u0), hl1 = fma(T(-0.5), Dm, u0);
+  T w0r, w1r, w0l, w1l;  // unnormalised weights of the right and left states
+  if (es) {
+    const T dd = Dp - Dm;
+    const T tau = tau_zero ? T(0) : dd * dd;
+    const T p0 = r0 + tau, p1 = r1 + tau;
+    w0r = p0 * r1;
+    w1r = T(2) * (p1 * r0);
+    w0l = p1 * r0;
+    w1l = T(2) * (p0 * r1);
+  } else {
+    const T q0 = r0 * r0, q1 = r1 * r1;
+    w0r = q1;
+    w1r = T(2) * q0;
+    w0l = q0;
+    w1l = T(2) * q1;
+  }
+  ur = fma(w0r, hr0, w1r * hr1) / (w0r + w1r);
+  ul = fma(w0l, hl0, w1l * hl1) / (w0l + w1l);
+}
+
 }  // namespace psa

@@ -229,7 +229,10 @@ def _scheme_plan(source: Any) -> tuple[_lib.PsaSolverDesc, tuple[Any, ...]] | No
     if not isinstance(source, BoundOperator):
         return None
     scheme, grid, bc = source.scheme, source.grid, source.bc
-    if not isinstance(bc, PeriodicBoundary) or not isinstance(scheme.rec, WENOJS53):
+    from pyshocks_amd.reconstruction import ESWENO32, WENOJS32
+
+    if not isinstance(bc, PeriodicBoundary) or \
+            not isinstance(scheme.rec, (WENOJS53, WENOJS32, ESWENO32)):
         return None
 
     desc = _lib.PsaSolverDesc()
@@ -239,7 +242,10 @@ def _scheme_plan(source: Any) -> tuple[_lib.PsaSolverDesc, tuple[Any, ...]] | No
     desc.dx = grid.h
     desc.eps = float(scheme.rec.eps)
     keep: tuple[Any, ...] = ()
-    from pyshocks_amd.reconstruction import LearnedWENOJS53
+    from pyshocks_amd.reconstruction import LearnedWENOJS53, rec_code
+
+    if not isinstance(scheme.rec, LearnedWENOJS53):
+        desc.rec = rec_code(scheme.rec)
 
     if isinstance(scheme.rec, LearnedWENOJS53):
         if not isinstance(scheme, (burgers.Rusanov, burgers.Godunov)) or \

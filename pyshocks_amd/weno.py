@@ -2,8 +2,11 @@
 
 Mirrors ``weno.py`` of pyshocks: ``Stencil`` (``:55-98``),
 ``weno_smoothness`` (``:114-140``), ``weno_interp`` (``:143-157``),
-``weno_js_53_coefficients`` (``:206-244``), ``weno_js_weights``
-(``:247-256``). The coefficient tables are kept as host tensors (they are
+``weno_js_32_coefficients`` (``:166-203``), ``weno_js_53_coefficients``
+(``:206-244``), ``weno_js_weights`` (``:247-256``) and
+``es_weno_parameters`` (``:265-282``). The field-level functions take the
+fifth-order stencil; the third-order reconstructions run whole
+(``reconstruction.reconstruct``). The coefficient tables are kept as host tensors (they are
 the same numbers as the reference's, taps ordered ``(i+2, ..., i-2)``); the
 three array functions evaluate them with the library's kernels, which
 implement the fifth-order stencil with zero padding outside the array exactly
@@ -56,6 +59,30 @@ def weno_js_53_coefficients(dtype: Any = None) -> Stencil:
     d = torch.tensor([[1.0 / 10.0, 6.0 / 10.0, 3.0 / 10.0]], dtype=dtype).T
 
     return Stencil(a=a, b=b, c=c, d=d)
+
+
+def weno_js_32_coefficients(dtype: Any = None) -> Stencil:
+    """Coefficients of the third-order WENO-JS scheme (``weno.py:166-203``),
+    taps ordered ``(i+1, i, i-1)``."""
+    if dtype is None:
+        dtype = torch.float64
+
+    a = torch.tensor([1.0], dtype=dtype)
+    b = torch.tensor([[[0.0, 1.0, -1.0]], [[1.0, -1.0, 0.0]]], dtype=dtype)
+    c = torch.tensor([[0.0, 3.0 / 2.0, -1.0 / 2.0], [1.0 / 2.0, 1.0 / 2.0, 0.0]],
+                     dtype=dtype)
+    d = torch.tensor([[1.0 / 3.0, 2.0 / 3.0]], dtype=dtype).T
+
+    return Stencil(a=a, b=b, c=c, d=d)
+
+
+def es_weno_parameters(grid: Any, u0: torch.Tensor) -> tuple[torch.Tensor, float]:
+    """``(eps, delta)`` of ES-WENO (``weno.py:265-282``): ``eps = sum_int
+    dx |u0| * dx_min^2``, ``delta = dx_min^2``."""
+    from pyshocks_amd.grid import norm
+
+    eps = norm(grid, u0, p=1, weighted=True) * grid.dx_min**2
+    return eps, grid.dx_min**2
 
 
 _JS53 = weno_js_53_coefficients()

@@ -345,6 +345,125 @@ def test_step_dt_rule() -> None:
 # {{{ end-to-end against exact solutions
 
 
+def test_weno3_coefficients_and_loops() -> None:
+    """WENO-JS3 (``weno.py:166-203``): candidates reproduce constants, the
+    ideal combination is the third-order interpolant ``-1/6, 5/6, 1/3`` of
+    ``(u_{i-1}, u_i, u_{i+1})``, beta is ``s^2`` on ``u_j = s j``, and the
+    convolutions equal explicit loops over the taps."""
+    S3 = W.WENOJS32
+    np.testing.assert_allclose(S3.c.sum(axis=1), 1.0, rtol=0, atol=1e-15)
+    combined = (S3.d * S3.c).sum(axis=0)  # taps (i+1, i, i-1)
+    np.testing.assert_allclose(combined, [1 / 3, 5 / 6, -1 / 6], rtol=0, atol=1e-15)
+    j = np.arange(12, dtype=np.float64)
+    beta = W.weno_smoothness(S3, 0.75 * j)
+    np.testing.assert_allclose(beta[:, 1:-1], 0.75**2, rtol=1e-14)
+
+    n = 40
+    u = np.sin(np.linspace(0.0, 2.0 * np.pi, n))
+    beta = W.weno_smoothness(S3, u)
+    uhat = W.weno_interp(S3, u)
+    for i in range(1, n - 1):
+        assert abs(beta[0, i] - (u[i] - u[i - 1]) ** 2) < 1e-15
+        assert abs(beta[1, i] - (u[i + 1] - u[i]) ** 2) < 1e-15
+        assert abs(uhat[0, i] - (1.5 * u[i] - 0.5 * u[i - 1])) < 2e-15
+        assert abs(uhat[1, i] - (0.5 * u[i + 1] + 0.5 * u[i])) < 2e-15
+
+
+@pytest.mark.parametrize("rec", ["wenojs32", "esweno32"])
+def test_weno3_exact_on_linear_data(rec: str) -> None:
+    """Each candidate is exact on the cell averages of a linear function, so
+    any convex weights (JS or ES) reconstruct it exactly at every face."""
+    n = 24
+    f = np.linspace(-1.0, 1.0, n + 1)
+    u = _poly_cell_averages(np.array([0.3, -0.7]), f)
+    ul, ur = W.reconstruct(u, 1.0e-6, rec)
+    np.testing.assert_allclose(ur[1:-1], 0.3 - 0.7 * f[2:-1], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(ul[1:-1], 0.3 - 0.7 * f[1:-2], rtol=0, atol=1e-13)
+
+
+def test_esweno_tau_padding() -> None:
+    """``weno.py:293``: tau is zero for the first and last entry, whatever the
+    data; with tau = 0 the ES weights are the ideal ones."""
+    rng = np.random.default_rng(2)
+    u = rng.uniform(-1, 1, size=9)
+    om = W.es_weno_weights(W.WENOJS32, u, eps=1e-6)
+    np.testing.assert_allclose(om[:, 0], [1 / 3, 2 / 3], rtol=1e-15)
+    np.testing.assert_allclose(om[:, -1], [1 / 3, 2 / 3], rtol=1e-15)
+    tau = (u[2] - 2 * u[1] + u[0]) ** 2
+    b0, b1 = (u[1] - u[0]) ** 2, (u[2] - u[1]) ** 2
+    a0, a1 = (1 + tau / (1e-6 + b0)) / 3, 2 * (1 + tau / (1e-6 + b1)) / 3
+    np.testing.assert_allclose(om[:, 1], [a0 / (a0 + a1), a1 / (a0 + a1)], rtol=1e-14)
+
+
+@pytest.mark.parametrize("rec", ["wenojs32", "esweno32"])
+def test_weno3_reconstruction_order(rec: str) -> None:
+    """``tests/test_weno.py:288-338`` for the third-order reconstructions:
+    order >= 2.5 on a smooth sine (N = 192 .. 384)."""
+    hs, el, er = [], [], []
+    for n in range(192, 384 + 1, 32):
+        grid = W.make_uniform_cell_grid(-1.0, 1.0, n, nghosts=2)
+        u0 = W.leggauss_cell_average(grid, lambda x: np.sin(2 * np.pi * x), order=4)
+        ul, ur = W.reconstruct(u0, 1.0e-6, rec)
+        fx = np.sin(2 * np.pi * grid.f)
+        el.append(W.rnorm(grid, ul, fx[:-1], p=np.inf))
+        er.append(W.rnorm(grid, ur, fx[1:], p=np.inf))
+        hs.append(grid.dx_max)
+
+    assert W.estimate_order_of_convergence(np.array(hs), np.array(el)) >= 2.5
+    assert W.estimate_order_of_convergence(np.array(hs), np.array(er)) >= 2.5
+
+
+def test_weno3_brute_force_faces() -> None:
+    """Both face states of every cell on a tiny grid from the Jiang-Shu
+    third-order formulas, for both weight families."""
+    eps = 1e-6
+    rng = np.random.default_rng(8)
+    u = rng.uniform(-1, 1, size=10)
+    for rec in ("wenojs32", "esweno32"):
+        ul, ur = W.reconstruct(u, eps, rec)
+        for i in range(1, u.size - 1):
+            um, u0, up = u[i - 1], u[i], u[i + 1]
+            tau = (up - 2 * u0 + um) ** 2
+
+            def side(a: float, b: float, c: float) -> float:
+                # a = u_{i-1}, b = u_i, c = u_{i+1} (mirrored for the left side)
+                b0, b1 = (b - a) ** 2, (c - b) ** 2
+                if rec == "wenojs32":
+                    w0, w1 = (1 / 3) / (eps + b0) ** 2, (2 / 3) / (eps + b1) ** 2
+                else:
+                    w0, w1 = (1 + tau / (eps + b0)) / 3, 2 * (1 + tau / (eps + b1)) / 3
+                return (w0 * (1.5 * b - 0.5 * a) + w1 * (0.5 * b + 0.5 * c)) / (w0 + w1)
+
+            assert abs(ur[i] - side(um, u0, up)) < 1e-14
+            assert abs(ul[i] - side(up, u0, um)) < 1e-14
+
+
+@pytest.mark.parametrize("rec", ["wenojs32", "esweno32"])
+def test_advection_order_weno3(rec: str) -> None:
+    """``tests/test_convergence.py:400-402`` (godunov + wenojs32 / esweno32,
+    order 3 - 0.5), periodic on [-1, 1] to t = 1 with dt = 8 (2/n)^(5/3)."""
+    a, b, tfinal = -1.0, 1.0, 1.0
+
+    def ic(grid: W.Grid, x: np.ndarray) -> np.ndarray:
+        xm = (grid.b + grid.a) / 2
+        xx = (x - xm) / (grid.b - grid.a)
+        return np.sin(2 * np.pi * xx + np.sin(2 * np.pi * xx) / np.pi)
+
+    hs, errs = [], []
+    for n in range(192, 384 + 1, 64):
+        grid = W.make_uniform_cell_grid(a, b, n, nghosts=3)
+        vel = np.ones(grid.n)
+        sch = W.Scheme("advection", eps=1.0e-6, velocity=vel, rec=rec)
+        u0 = W.leggauss_cell_average(grid, lambda x: ic(grid, x), order=4)
+        uhat = W.leggauss_cell_average(grid, lambda x: ic(grid, x - tfinal), order=4)
+        maxit, dt = W.predict_maxit_from_timestep(tfinal, 8.0 * ((b - a) / n) ** (5.0 / 3.0))
+        u, _, _ = W.solve(sch, grid, u0, theta=1.0, maxit=maxit, fixed_dt=dt)
+        errs.append(W.norm(grid, u - uhat, p=1, weighted=True)
+                    / W.norm(grid, uhat, p=1, weighted=True))
+        hs.append(float(np.max(np.diff(grid.f))))
+    assert W.estimate_order_of_convergence(np.array(hs), np.array(errs)) >= 2.5
+
+
 def test_advection_order_exact_shift() -> None:
     """``tests/test_convergence.py:395-460`` (godunov + wenojs53), periodic on
     [-1, 1] to t = 1 with dt = 8 (2/n)^(5/3)."""
@@ -426,7 +545,7 @@ def test_burgers_flux_sign_error_is_caught() -> None:
     grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
     u0 = E.exact_cell_averages(ic, n)
 
-    def bad_flux(grid, u, a, *, eps, alpha=1.0, flux=W.burgers_flux):
+    def bad_flux(grid, u, a, *, eps, alpha=1.0, rec="wenojs53", flux=W.burgers_flux):
         ul, ur = W.reconstruct_wenojs(u, eps)
         s = np.maximum(np.abs(a[1:]), np.abs(a[:-1]))
         f = 0.5 * (flux(ul)[1:] + flux(ur)[:-1]) + 0.5 * s * (ul[1:] - ur[:-1])
