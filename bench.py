@@ -20,7 +20,8 @@ the same step loop on the host cores, on a bounded sample.
 The other Burgers fluxes run the same workload with ``--scheme godunov``,
 ``--scheme eo`` (Engquist-Osher, whole-step kernel) or ``--scheme lf``
 (Lax-Friedrichs: the stage kernels, each reducing the global max|u| that the
-next stage's flux uses).
+next stage's flux uses), and ``--rec wenojs32|esweno32`` the third-order
+reconstructions.
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
@@ -49,6 +50,7 @@ VALU_PEAK_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 # words moved per cell by each stage kernel: stage 1 reads u^n and writes
 # u^(1); stages 2 and 3 read u^n and u^(k) and write the next state
 STAGE_WORDS = {1: 2, 2: 3, 3: 3}
+REC_NAMES = {"wenojs53": "WENO-JS5", "wenojs32": "WENO-JS3", "esweno32": "ES-WENO3"}
 # the whole-step kernel reads u^n and writes u^(n+1) (its halo re-reads, 16
 # of every 1024 cells, are not algorithmic traffic)
 WHOLE_STEP_WORDS = 2
@@ -63,6 +65,8 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
     p.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     p.add_argument("--scheme", choices=["rusanov", "godunov", "lf", "eo", "advection"],
                    default="rusanov")
+    p.add_argument("--rec", choices=["wenojs53", "wenojs32", "esweno32"],
+                   default="wenojs53", help="the reconstruction (WENO-JS5 for the metric)")
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--block", type=int, default=0)
     p.add_argument("--cells-per-thread", type=int, default=0)
@@ -115,14 +119,15 @@ def make_problem(args: argparse.Namespace):
     from pyshocks_amd import funcs, make_uniform_cell_grid
     from pyshocks_amd.advection import Godunov as AdvGodunov
     from pyshocks_amd.burgers import EngquistOsher, Godunov, LaxFriedrichs, Rusanov
-    from pyshocks_amd.reconstruction import WENOJS53
     from pyshocks_amd.scalar import PeriodicBoundary
     from pyshocks_amd.timestepping import fused_plan, make_ssprk33
 
     dtype = torch.float64 if args.dtype == "f64" else torch.float32
     grid = make_uniform_cell_grid(0.0, 1.0, args.n, nghosts=3, dtype=dtype,
                                   device="cuda")
-    rec = WENOJS53(eps=1.0e-6)
+    from pyshocks_amd.reconstruction import make_reconstruction_from_name
+
+    rec = make_reconstruction_from_name(args.rec, eps=1.0e-6)
     if args.learned:
         from pyshocks_amd.reconstruction import LearnedWENOJS53
 
@@ -323,12 +328,12 @@ def cpu_leg(args: argparse.Namespace, gpu: dict) -> dict | None:
     # one step to size the sample, then enough steps for ~cpu_seconds
     t0 = time.perf_counter()
     u, _, _, _ = ref_c.solve(kind, u0, n=n, g=3, dx=h, eps=1e-6, theta=gpu["theta"],
-                             cfl_num=cfl, maxit=1, velocity=vel)
+                             cfl_num=cfl, maxit=1, velocity=vel, rec=args.rec)
     one = time.perf_counter() - t0
     nsteps = int(max(1, min(200, args.cpu_seconds / max(one, 1e-9))))
     t0 = time.perf_counter()
     ref_c.solve(kind, u, n=n, g=3, dx=h, eps=1e-6, theta=gpu["theta"], cfl_num=cfl,
-                maxit=nsteps, velocity=vel)
+                maxit=nsteps, velocity=vel, rec=args.rec)
     elapsed = time.perf_counter() - t0
     return {
         "value": n * nsteps / elapsed,
@@ -579,7 +584,8 @@ def main(argv: list[str] | None = None) -> int:
     cpu = cpu_leg(args, gpu)
 
     kname = "step" if gpu["whole_step"] else f"stage{gpu['stage_dominant']}"
-    key = f"{args.scheme}_{args.dtype}_{args.n}_{kname}"
+    recs = "" if args.rec == "wenojs53" else f"_{args.rec}"
+    key = f"{args.scheme}{recs}_{args.dtype}_{args.n}_{kname}"
     traffic = _profile_entry("traffic.json", key)
     valu = _profile_entry("valu.json", key)
     roofline = roofline_record(args, gpu, traffic, valu)
@@ -600,10 +606,11 @@ def main(argv: list[str] | None = None) -> int:
         "data": "synthetic",
         "config": {
             "workload": f"periodic {'burgers' if args.scheme != 'advection' else 'advection'} "
-                        f"{args.scheme} WENO-JS5 SSP-RK3, N={args.n}, {dtype_name}, "
+                        f"{args.scheme} {REC_NAMES[args.rec]} SSP-RK3, N={args.n}, {dtype_name}, "
                         f"Courant 0.5, sine-sum+step u0 seed {args.seed}",
             "n_cells": args.n,
             "scheme": args.scheme,
+            "reconstruction": args.rec,
             "step": ("one SSP-RK3 time step (one whole-step kernel: the three stages of "
                      "each tile in LDS, device CFL dt)" if gpu["whole_step"] else
                      "one SSP-RK3 time step (3 fused stage kernels, device CFL dt)"),

@@ -13,6 +13,8 @@ import numpy as np
 LIB = pathlib.Path(__file__).resolve().parent / "libweno_ref.so"
 
 FLUX = {"rusanov": 0, "godunov": 1, "advection": 2, "eo": 3, "lf": 4}
+#: reconstruction codes, passed in bits 8 and up of the flux argument
+REC = {"wenojs53": 0, "wenojs32": 1, "esweno32": 2}
 
 _lib: ctypes.CDLL | None = None
 
@@ -55,7 +57,8 @@ def num_threads() -> int:
 
 
 def apply_operator(kind: str, u: np.ndarray, *, g: int, dx: float, eps: float,
-                   velocity: np.ndarray | None = None) -> np.ndarray:
+                   velocity: np.ndarray | None = None,
+                   rec: str = "wenojs53") -> np.ndarray:
     lib = load()
     u = np.ascontiguousarray(u, dtype=np.float64)
     m = u.size
@@ -63,15 +66,16 @@ def apply_operator(kind: str, u: np.ndarray, *, g: int, dx: float, eps: float,
     uf = np.empty(m)
     F = np.empty(m + 1)
     vel = None if velocity is None else np.ascontiguousarray(velocity, dtype=np.float64)
-    lib.oracle_apply_operator(FLUX[kind], _p(u), _p(vel), _p(out), _p(uf), _p(F),
-                              m, g, dx, eps)
+    lib.oracle_apply_operator(FLUX[kind] | (REC[rec] << 8), _p(u), _p(vel), _p(out),
+                              _p(uf), _p(F), m, g, dx, eps)
     return out
 
 
 def solve(kind: str, u0: np.ndarray, *, n: int, g: int, dx: float, eps: float,
           theta: float = 1.0, cfl_num: float = 0.0, fixed_dt: float | None = None,
           maxit: int | None = None, tstart: float = 0.0, tfinal: float | None = None,
-          velocity: np.ndarray | None = None) -> tuple[np.ndarray, float, int, np.ndarray]:
+          velocity: np.ndarray | None = None,
+          rec: str = "wenojs53") -> tuple[np.ndarray, float, int, np.ndarray]:
     """Run the reference step loop; returns ``(u, t, nsteps, dts)``."""
     lib = load()
     u = np.array(u0, dtype=np.float64, copy=True)
@@ -82,7 +86,8 @@ def solve(kind: str, u0: np.ndarray, *, n: int, g: int, dx: float, eps: float,
     t = ctypes.c_double(0.0)
     it = ctypes.c_longlong(0)
     status = lib.oracle_ssprk33_run(
-        FLUX[kind], _p(u), _p(vel), n, g, dx, eps, 0 if fixed_dt is None else 1,
+        FLUX[kind] | (REC[rec] << 8), _p(u), _p(vel), n, g, dx, eps,
+        0 if fixed_dt is None else 1,
         theta, cfl_num, 0.0 if fixed_dt is None else fixed_dt,
         -1 if maxit is None else maxit, tstart, tf, ctypes.byref(t),
         ctypes.byref(it), _p(dts), dts.size)

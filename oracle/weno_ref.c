@@ -1,7 +1,7 @@
 /*
- * CPU oracle, C restatement of pyshocks' WENO-JS5 / Rusanov / Godunov /
- * Engquist-Osher / Lax-Friedrichs / SSP-RK3 hot path (FP64), parallelised
- * over cells with OpenMP.
+ * CPU oracle, C restatement of pyshocks' WENO-JS5 (and WENO-JS3, ES-WENO3) /
+ * Rusanov / Godunov / Engquist-Osher / Lax-Friedrichs / SSP-RK3 hot path
+ * (FP64), parallelised over cells with OpenMP.
  *
  * TEST INFRASTRUCTURE ONLY: it is loaded by tests/ (checked against the
  * NumPy restatement oracle/weno_np.py) and timed as the CPU baseline by
@@ -82,12 +82,48 @@ static inline double recon_left(const double *u, int64_t i, int64_t m,
               at(u, i - 2, m), eps);
 }
 
+/* third-order side (weno.py:166-203 with the weights of weno.py:247-256, or
+ * the ES-WENO weights of weno.py:285-296 with this tau); (a, b, c) =
+ * (u[i-1], u[i], u[i+1]) for the right state, mirrored for the left */
+static inline double side3(double a, double b, double c, double eps, int es,
+                           double tau) {
+  const double b0 = (b - a) * (b - a), b1 = (c - b) * (c - b);
+  const double h0 = (3.0 / 2.0) * b + (-1.0 / 2.0) * a;
+  const double h1 = (1.0 / 2.0) * c + (1.0 / 2.0) * b;
+  double a0, a1;
+  if (es) {
+    a0 = (1.0 / 3.0) * (1.0 + tau / (eps + b0));
+    a1 = (2.0 / 3.0) * (1.0 + tau / (eps + b1));
+  } else {
+    a0 = (1.0 / 3.0) / ((eps + b0) * (eps + b0));
+    a1 = (2.0 / 3.0) / ((eps + b1) * (eps + b1));
+  }
+  const double s = a0 + a1;
+  return (a0 / s) * h0 + (a1 / s) * h1;
+}
+
+/* rec 1 (WENO-JS3) or 2 (ES-WENO3); tau is zero at the array ends */
+static inline double recon3(const double *u, int64_t i, int64_t m, double eps,
+                            int rec, int right) {
+  const double a = at(u, i - 1, m), b = u[i], c = at(u, i + 1, m);
+  const int es = rec == 2;
+  double tau = 0.0;
+  if (es && i > 0 && i < m - 1) {
+    const double t = (c - 2.0 * b) + a;
+    tau = t * t;
+  }
+  return right ? side3(a, b, c, eps, es, tau) : side3(c, b, a, eps, es, tau);
+}
+
 /* flux at face j (between cells j - 1 and j), 1 <= j <= m - 1; amax is
- * the global speed of Lax-Friedrichs */
-static inline double face_flux(int flux, const double *u, const double *vel,
-                               int64_t j, int64_t m, double eps, double amax) {
-  const double urL = recon_right(u, j - 1, m, eps);
-  const double ulR = recon_left(u, j, m, eps);
+ * the global speed of Lax-Friedrichs; rec 0 WENO-JS5, 1 WENO-JS3,
+ * 2 ES-WENO3 */
+static inline double face_flux(int flux, int rec, const double *u,
+                               const double *vel, int64_t j, int64_t m,
+                               double eps, double amax) {
+  const double urL = rec ? recon3(u, j - 1, m, eps, rec, 1)
+                         : recon_right(u, j - 1, m, eps);
+  const double ulR = rec ? recon3(u, j, m, eps, rec, 0) : recon_left(u, j, m, eps);
   if (flux == ORACLE_RUSANOV || flux == ORACLE_LAX_FRIEDRICHS) {
     const double fl = (ulR * ulR) / 2.0;
     const double fr = (urL * urL) / 2.0;
@@ -104,8 +140,10 @@ static inline double face_flux(int flux, const double *u, const double *vel,
     return aavg > 0.0 ? (urL * urL) / 2.0 : (ulR * ulR) / 2.0;
   }
   {
-    const double arL = recon_right(vel, j - 1, m, eps);
-    const double alR = recon_left(vel, j, m, eps);
+    const double arL = rec ? recon3(vel, j - 1, m, eps, rec, 1)
+                           : recon_right(vel, j - 1, m, eps);
+    const double alR = rec ? recon3(vel, j, m, eps, rec, 0)
+                           : recon_left(vel, j, m, eps);
     const double aavg = (arL + alR) / 2.0;
     return aavg > 0.0 ? urL : ulR;
   }
@@ -119,10 +157,13 @@ static void fill_periodic(double *u, int64_t m, int g) {
   }
 }
 
-/* L = apply_operator(u) into out (m entries); uf is scratch (m). */
+/* L = apply_operator(u) into out (m entries); uf is scratch (m). flux
+ * carries the reconstruction in its bits 8 and up (0: WENO-JS5). */
 void oracle_apply_operator(int flux, const double *u, const double *vel,
                            double *out, double *uf, double *F, int64_t m,
                            int g, double dx, double eps) {
+  const int rec = flux >> 8;
+  flux &= 0xff;
   memcpy(uf, u, (size_t)m * sizeof(double));
   fill_periodic(uf, m, g);
   F[0] = 0.0;
@@ -137,7 +178,8 @@ void oracle_apply_operator(int flux, const double *u, const double *vel,
     }
   }
 #pragma omp parallel for schedule(static)
-  for (int64_t j = 1; j < m; ++j) F[j] = face_flux(flux, uf, vel, j, m, eps, amax);
+  for (int64_t j = 1; j < m; ++j)
+    F[j] = face_flux(flux, rec, uf, vel, j, m, eps, amax);
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < m; ++i) {
     const double d = F[i + 1] - F[i];
@@ -174,7 +216,8 @@ int oracle_ssprk33_run(int flux, double *u, const double *vel, int64_t n,
   double *uf = (double *)malloc((size_t)m * sizeof(double));
   double *F = (double *)malloc((size_t)(m + 1) * sizeof(double));
   double amax = 0.0;
-  if (flux == ORACLE_ADVECTION_UPWIND) amax = max_abs_interior(vel, m, g);
+  const int kind = flux & 0xff;  /* flux with the reconstruction bits */
+  if (kind == ORACLE_ADVECTION_UPWIND) amax = max_abs_interior(vel, m, g);
 
   double t = tstart;
   long long it = 0;
@@ -188,7 +231,7 @@ int oracle_ssprk33_run(int flux, double *u, const double *vel, int64_t n,
       dt = fixed_dt;
     } else {
       const double smax =
-          flux == ORACLE_ADVECTION_UPWIND ? amax : max_abs_interior(u, m, g);
+          kind == ORACLE_ADVECTION_UPWIND ? amax : max_abs_interior(u, m, g);
       dt = theta * (cfl_num / smax);
     }
     if (!isinf(tfinal)) {

@@ -350,6 +350,41 @@ __device__ __forceinline__ T weno3_tau_ref(T um1, T u0, T up1) {
   return t * t;
 }
 
+// ur = nr / dr and ul = nl / dl. FP64: one reciprocal of dr dl refined by two
+// Newton steps (as in weno_combine_pair), guarded by the class of dr dl (a
+// positive normal number), else two divisions; FP32: the hardware
+// reciprocal of each denominator.
+template <typename T>
+__device__ __forceinline__ void weno3_divide(T nr, T dr, T nl, T dl, T &ur,
+                                             T &ul);
+
+template <>
+__device__ __forceinline__ void weno3_divide<double>(double nr, double dr,
+                                                     double nl, double dl,
+                                                     double &ur, double &ul) {
+  const double den = dr * dl;
+  if (__builtin_amdgcn_class(den, 0x100)) {
+    double inv = __builtin_amdgcn_rcp(den);
+    double e = fma(-den, inv, 1.0);
+    inv = fma(inv, e, inv);
+    e = fma(-den, inv, 1.0);
+    inv = fma(inv, e, inv);
+    ur = (nr * dl) * inv;
+    ul = (nl * dr) * inv;
+  } else {
+    ur = nr / dr;
+    ul = nl / dl;
+  }
+}
+
+template <>
+__device__ __forceinline__ void weno3_divide<float>(float nr, float dr, float nl,
+                                                    float dl, float &ur,
+                                                    float &ul) {
+  ur = nr * __builtin_amdgcn_rcpf(dr);
+  ul = nl * __builtin_amdgcn_rcpf(dl);
+}
+
 // Both states of one cell for the fused kernels from the first differences
 // Dm = u_i - u_{i-1}, Dp = u_{i+1} - u_i: with q_k the JS denominators
 // (eps + beta_k)^2, omega_0 = q1 / (q1 + 2 q0) on the right and
@@ -380,8 +415,8 @@ __device__ __forceinline__ void weno3_pair(T u0, T Dm, T Dp, T eps, bool es,
     w0l = q0;
     w1l = T(2) * q1;
   }
-  ur = fma(w0r, hr0, w1r * hr1) / (w0r + w1r);
-  ul = fma(w0l, hl0, w1l * hl1) / (w0l + w1l);
+  weno3_divide(fma(w0r, hr0, w1r * hr1), w0r + w1r, fma(w0l, hl0, w1l * hl1),
+               w0l + w1l, ur, ul);
 }
 
 }  // namespace psa

@@ -239,3 +239,15 @@ def test_c_oracle_matches_numpy_oracle() -> None:
         np.testing.assert_allclose(dtc, dts, rtol=1e-14, atol=1e-16)
         i = grid.i_
         assert np.linalg.norm(uc[i] - u[i]) / np.linalg.norm(u[i]) < 1e-13
+    for rec in ("wenojs32", "esweno32"):
+        n = 150
+        grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=2)
+        u0 = W.apply_boundary_periodic(grid, np.sin(2 * np.pi * grid.x) + 0.3)
+        u, dts, t = W.solve(W.Scheme("rusanov", eps=1e-6, rec=rec), grid, u0,
+                            theta=1.0, tfinal=0.3)
+        uc, tc, it, dtc = ref_c.solve("rusanov", u0, n=n, g=2, dx=1.0 / n, eps=1e-6,
+                                      theta=1.0, cfl_num=0.5 / n, tfinal=0.3, rec=rec)
+        assert it == len(dts)
+        np.testing.assert_allclose(dtc, dts, rtol=1e-14, atol=1e-16)
+        i = grid.i_
+        assert np.linalg.norm(uc[i] - u[i]) / np.linalg.norm(u[i]) < 1e-13
