@@ -154,6 +154,64 @@ def test_adjoint_step_reference_api(torch_mod) -> None:
     assert _rel(_host(p)[3:-3], lam) < 1e-11
 
 
+_FORWARD_TO_DISK = """
+import sys
+import numpy as np
+import torch
+from pyshocks_amd import make_uniform_cell_grid
+from pyshocks_amd.burgers import Rusanov
+from pyshocks_amd.checkpointing import NumpyCheckpoint
+from pyshocks_amd.reconstruction import WENOJS53
+from pyshocks_amd.scalar import PeriodicBoundary
+from pyshocks_amd.timestepping import make_ssprk33, step
+
+n = int(sys.argv[2])
+u0 = torch.tensor(np.load(sys.argv[3]), device="cuda")
+grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+chk = NumpyCheckpoint(basename="Iteration", dirname=sys.argv[1])
+st = make_ssprk33(Rusanov(rec=WENOJS53(eps=1e-6)), grid, PeriodicBoundary(), theta=1.0,
+                  checkpoint=chk)
+for _ in step(st, u0, maxit=6):
+    pass
+"""
+
+
+def test_adjoint_resumes_from_disk(torch_mod, tmp_path) -> None:
+    """The forward run writes its checkpoints as npz files in another
+    process; ``adjoint_step`` over those files gives the gradient of the
+    in-memory run bit for bit (``checkpointing.py:209-254``)."""
+    import subprocess
+    import sys
+
+    torch = torch_mod
+    from pyshocks_amd.checkpointing import InMemoryCheckpoint, NumpyCheckpoint
+    from pyshocks_amd.timestepping import adjoint_step, make_ssprk33, step
+
+    n = 48
+    u0, _ = _data(n, 8)
+    np.save(tmp_path / "u0.npy", u0)
+    r = subprocess.run([sys.executable, "-c", _FORWARD_TO_DISK, str(tmp_path), str(n),
+                        str(tmp_path / "u0.npy")], timeout=180, capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert (tmp_path / "Iteration_000000006.npz").exists()
+
+    src, grid = _source(torch, "rusanov", n)
+    mem = InMemoryCheckpoint(basename="Iteration")
+    st_mem = make_ssprk33(src.scheme, grid, src.bc, theta=1.0, checkpoint=mem)
+    uT = list(step(st_mem, _dev(torch, u0), maxit=6))[-1].u
+    st_disk = make_ssprk33(src.scheme, grid, src.bc, theta=1.0,
+                           checkpoint=NumpyCheckpoint(basename="Iteration",
+                                                      dirname=tmp_path))
+    out = []
+    for st in (st_mem, st_disk):
+        p = None
+        for ev in adjoint_step(st, uT.clone(), maxit=6):
+            p = ev.p
+        out.append(p)
+    assert torch.equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("kind", KINDS)
 def test_checkpointed_adjoint_matches_step_vjps(torch_mod, kind: str) -> None:
     torch = torch_mod

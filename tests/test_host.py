@@ -135,6 +135,40 @@ def test_checkpoint_semantics() -> None:
         load(chk, 9)
 
 
+@pytest.mark.parametrize("kind", ["npz", "pickle"])
+def test_on_disk_checkpoints(tmp_path, kind: str) -> None:
+    """``checkpointing.py:149-254``: file names ``<basename>_<i:09d>.npz``, the
+    saved dict round-trips (arrays onto the requested device, 0-d values as
+    0-d arrays), ``KeyError`` when saving over an existing file or loading a
+    missing one, ``include``; the npz files are plain ``numpy.savez`` archives
+    in both directions (the reference's ``{m, t, u}`` layout)."""
+    import torch
+
+    from pyshocks_amd.checkpointing import NumpyCheckpoint, PickleCheckpoint, load, save
+
+    cls = NumpyCheckpoint if kind == "npz" else PickleCheckpoint
+    chk = cls(basename="Iteration", dirname=tmp_path, device="cpu")
+    u = torch.linspace(0.0, 1.0, 11, dtype=torch.float64)
+    save(chk, 7, {"m": 7, "t": np.float64(0.25), "u": u})
+    assert chk.index_to_key(7) == tmp_path / "Iteration_000000007.npz"
+    assert 7 in chk and 8 not in chk
+    with pytest.raises(KeyError):
+        save(chk, 7, {"m": 7})
+    with pytest.raises(KeyError):
+        load(chk, 8)
+    got = load(chk, 7)
+    assert int(got["m"]) == 7 and got["m"].ndim == 0 and float(got["t"]) == 0.25
+    assert isinstance(got["u"], torch.Tensor) and torch.equal(got["u"], u)
+    assert set(load(chk, 7, include=("t",))) == {"t"}
+    if kind == "npz":
+        with np.load(chk.index_to_key(7), allow_pickle=False) as f:
+            assert set(f.files) == {"m", "t", "u"}
+            np.testing.assert_array_equal(f["u"], u.numpy())
+        np.savez(tmp_path / "Iteration_000000002.npz", m=2, t=0.5, u=np.arange(3.0))
+        got = load(chk, 2)
+        assert int(got["m"]) == 2 and got["u"].tolist() == [0.0, 1.0, 2.0]
+
+
 def test_fused_plan_selection() -> None:
     from pyshocks_amd import apply_operator, make_uniform_cell_grid
     from pyshocks_amd.burgers import Godunov, Rusanov
