@@ -72,6 +72,21 @@ extern "C" {
 #define PSA_REC_WENOJS53 0
 #define PSA_REC_WENOJS32 1
 #define PSA_REC_ESWENO32 2
+/* PSA_REC_MUSCL     reconstruction.MUSCL, the flux-limited second-order
+ *                   reconstruction (reconstruction.py:172-252) with the
+ *                   limiter PSA_LIMITER_* of limiters.py:132-408,
+ *                   nghosts >= 2                                         */
+#define PSA_REC_MUSCL 3
+
+/* limiters (limiters.py:132-408); MINMOD takes theta in [1, 2] */
+#define PSA_LIMITER_MINMOD 0
+#define PSA_LIMITER_MC 1
+#define PSA_LIMITER_SUPERBEE 2
+#define PSA_LIMITER_VANALBADA1 3
+#define PSA_LIMITER_VANALBADA2 4
+#define PSA_LIMITER_VANLEER 5
+#define PSA_LIMITER_KOREN 6
+#define PSA_LIMITER_NONE 7
 
 /* time step modes of the device-side step control */
 #define PSA_DT_CFL 0   /* dt = theta * cfl_num / max|u_int| (or max|a_int|) */
@@ -101,11 +116,19 @@ int psa_weno_js_weights(int dtype, const void *u, void *omega, int64_t m,
 int psa_weno_reconstruct(int dtype, const void *f, void *ul, void *ur,
                          int64_t m, double eps, void *stream);
 
-/* The same for any PSA_REC_* (reconstruction.py:358-377,420-434): the
+/* The same for any WENO PSA_REC_* (reconstruction.py:358-377,420-434): the
  * 'same' convolutions pad with zeros; ES-WENO's tau is zero for the first
  * and last cell (weno.py:293). */
 int psa_reconstruct(int dtype, int rec, const void *f, void *ul, void *ur,
                     int64_t m, double eps, void *stream);
+
+/* reconstruction.py:205-252 reconstruct(MUSCL(lm, atol), ...) of f as given:
+ * the slope ratio of limiters.py:110-129 (0 where |u[i+1] - u[i]| < atol,
+ * 0 for the first and last cell), ur = f + phi(r) / 2 (f[i+1] - f[i]),
+ * ul = f - phi(1/r) / 2 (f[i] - f[i-1]), ur[m-1] = ul[0] = 0. */
+int psa_muscl_reconstruct(int dtype, int limiter, double theta, double atol,
+                          const void *f, void *ul, void *ur, int64_t m,
+                          void *stream);
 
 /* scalar.py:529-540 apply_boundary(PeriodicBoundary): out = u with the
  * g ghost cells per side filled with their periodic images (out may be u). */
@@ -241,6 +264,10 @@ typedef struct psa_solver_desc {
   double mlp_sigma;             /* smooth threshold of the learned weights */
   int schedule;                 /* stages 1-3: PSA_SCHEDULE_*              */
   int rec;                      /* PSA_REC_* (0: WENO-JS5)                 */
+  int limiter;                  /* PSA_REC_MUSCL: PSA_LIMITER_*            */
+  int pad2_;
+  double limiter_theta;         /* PSA_LIMITER_MINMOD: theta in [1, 2]     */
+  double muscl_atol;            /* PSA_REC_MUSCL: slope cut-off (1e-12)    */
 } psa_solver_desc;
 
 /* Step schedules of psa_ssprk33_steps:
@@ -262,6 +289,14 @@ typedef struct psa_solver_desc {
 #define PSA_SCHEDULE_TILE 1
 #define PSA_SCHEDULE_PIPELINE 2
 #define PSA_SCHEDULE_STEP 3
+
+/* The numerical flux of the descriptor's flux and reconstruction (any
+ * PSA_REC_*, MUSCL's limiter from the descriptor) on u as given, fnum (m + 1)
+ * with fnum[0] = fnum[m] = 0; m = n + 2 g. speed is the Lax-Friedrichs
+ * device scalar max|a| (NULL otherwise); the advection velocity and nu come
+ * from the descriptor. */
+int psa_numerical_flux_desc(const psa_solver_desc *desc, const void *u,
+                            const void *speed, void *fnum, void *stream);
 
 /* Initialise the control block (device memory, sizeof(psa_control)) for a
  * run starting from state u at time tstart: stores t, tfinal, maxit, the dt

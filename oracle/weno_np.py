@@ -149,10 +149,88 @@ def reconstruct_es_side(f: np.ndarray, eps: float) -> np.ndarray:
     return np.sum(omega * uhat, axis=0)
 
 
+# {{{ MUSCL
+
+
+def limiter_phi(name: str, param: float, r: np.ndarray) -> np.ndarray:
+    """``evaluate(lm, r)`` of ``limiters.py:194-399``; ``param`` is minmod's
+    theta or van Albada's variant."""
+    r = np.asarray(r, dtype=np.float64)
+    if name == "minmod":
+        return np.maximum(0.0, np.minimum(np.minimum(param, param * r), (1 + r) / 2))
+    if name == "mc":
+        return np.maximum(0.0, np.minimum(np.minimum(2, 2 * r), (1 + r) / 2))
+    if name == "superbee":
+        return np.maximum(0.0, np.maximum(np.minimum(1, 2 * r), np.minimum(2, r)))
+    if name == "vanalbada":
+        phi = (r**2 + r) / (r**2 + 1) if param == 1 else 2 * r / (r**2 + 1)
+        return np.maximum(0.0, phi)
+    if name == "vanleer":
+        rabs = np.abs(r)
+        return np.maximum(0.0, (r + rabs) / (1 + rabs))
+    if name == "koren":
+        return np.maximum(0.0, np.minimum(np.minimum(2, 2 * r), (1 + 2 * r) / 3))
+    if name == "none":
+        return np.ones_like(r)
+    raise NotImplementedError(name)
+
+
+def limiter_symmetric(name: str, param: float) -> bool:
+    """``Limiter.is_symmetric`` (``limiters.py:62-72`` and the overrides)."""
+    if name == "vanalbada":
+        return param == 1
+    return name in ("minmod", "mc", "superbee", "vanleer")
+
+
+def local_slope_ratio(u: np.ndarray, *, atol: float = 1.0e-12) -> np.ndarray:
+    """``limiters.py:110-129``."""
+    sl = u[1:-1] - u[:-2]
+    sr = u[2:] - u[1:-1]
+    mask = np.abs(sr) < atol
+    sl = np.where(mask, 0.0, sl)
+    sr = np.where(mask, 1.0, sr)
+    return sl / sr
+
+
+def parse_muscl(rec: str) -> tuple[str, float, float]:
+    """``"muscl-<limiter>[-<param>]"`` -> (limiter, param, atol = 1e-12)."""
+    parts = rec.split("-")
+    assert parts[0] == "muscl"
+    name = parts[1] if len(parts) > 1 else "minmod"
+    param = float(parts[2]) if len(parts) > 2 else (1.0 if name in ("minmod", "vanalbada")
+                                                    else 0.0)
+    return name, param, 1.0e-12
+
+
+def reconstruct_muscl(f: np.ndarray, name: str, param: float, atol: float = 1.0e-12,
+                      ) -> tuple[np.ndarray, np.ndarray]:
+    """``reconstruction.py:205-252`` with ``u = f``."""
+    r = np.pad(local_slope_ratio(f, atol=atol), 1)
+    phi_r = limiter_phi(name, param, r)
+    if limiter_symmetric(name, param):
+        small = np.abs(r) < atol
+        phi_inv_r = np.where(small, 0.0, phi_r / np.where(small, 1.0, r))
+    else:
+        small = np.abs(r) < atol
+        inv_r = np.where(small, atol, 1 / np.where(small, 1.0, r))
+        phi_inv_r = limiter_phi(name, param, inv_r)
+
+    ur = np.pad(f[:-1] + 0.5 * phi_r[:-1] * (f[1:] - f[:-1]), (0, 1))
+    ul = np.pad(f[1:] - 0.5 * phi_inv_r[1:] * (f[1:] - f[:-1]), (1, 0))
+
+    return ul, ur
+
+
+# }}}
+
+
 def reconstruct(f: np.ndarray, eps: float, rec: str = "wenojs53",
                 ) -> tuple[np.ndarray, np.ndarray]:
     """``(ul, ur)`` of ``rec`` in ``{"wenojs53", "wenojs32", "esweno32"}``
-    (``reconstruction.py:358-377,420-434``)."""
+    (``reconstruction.py:358-377,420-434``) or ``"muscl-<limiter>[-<param>]"``
+    (``reconstruction.py:205-252``)."""
+    if rec.startswith("muscl"):
+        return reconstruct_muscl(f, *parse_muscl(rec))
     if rec == "wenojs53":
         return reconstruct_wenojs(f, eps, WENOJS53)
     if rec == "wenojs32":

@@ -38,6 +38,11 @@ PSA_FLUX_LAX_FRIEDRICHS = 4
 PSA_REC_WENOJS53 = 0
 PSA_REC_WENOJS32 = 1
 PSA_REC_ESWENO32 = 2
+PSA_REC_MUSCL = 3
+
+(PSA_LIMITER_MINMOD, PSA_LIMITER_MC, PSA_LIMITER_SUPERBEE, PSA_LIMITER_VANALBADA1,
+ PSA_LIMITER_VANALBADA2, PSA_LIMITER_VANLEER, PSA_LIMITER_KOREN,
+ PSA_LIMITER_NONE) = range(8)
 
 #: flux codes of the batched solver's Burgers scheme names
 BATCH_FLUX = {"rusanov": PSA_FLUX_RUSANOV, "godunov": PSA_FLUX_BURGERS_UPWIND,
@@ -95,6 +100,10 @@ class PsaSolverDesc(ctypes.Structure):
         ("mlp_sigma", ctypes.c_double),
         ("schedule", ctypes.c_int),
         ("rec", ctypes.c_int),
+        ("limiter", ctypes.c_int),
+        ("pad2_", ctypes.c_int),
+        ("limiter_theta", ctypes.c_double),
+        ("muscl_atol", ctypes.c_double),
     ]
 
 
@@ -148,6 +157,7 @@ SIGNATURES: dict[str, list[Any]] = {
     "psa_weno_js_weights": [_I, _VP, _VP, _I64, _D, _VP],
     "psa_weno_reconstruct": [_I, _VP, _VP, _VP, _I64, _D, _VP],
     "psa_reconstruct": [_I, _I, _VP, _VP, _VP, _I64, _D, _VP],
+    "psa_muscl_reconstruct": [_I, _I, _D, _D, _VP, _VP, _VP, _I64, _VP],
     "psa_apply_boundary_periodic": [_I, _VP, _VP, _I64, _I, _VP],
     "psa_numerical_flux": [_I, _I, _VP, _VP, _VP, _VP, _I64, _D, _VP],
     "psa_apply_operator": [_I, _I, _VP, _VP, _VP, _VP, _I64, _I, _D, _D, _VP],
@@ -160,6 +170,7 @@ SIGNATURES: dict[str, list[Any]] = {
                           _VP],
     "psa_control_init": [ctypes.POINTER(PsaSolverDesc), _VP, _VP, _D, _D, _LL,
                          _I, _D, _D, _D, _VP],
+    "psa_numerical_flux_desc": [ctypes.POINTER(PsaSolverDesc), _VP, _VP, _VP, _VP],
     "psa_ssprk33_steps": [ctypes.POINTER(PsaSolverDesc), _VP, _VP, _VP, _VP,
                           _VP, _LL, _VP],
     "psa_ssprk33_advance": [ctypes.POINTER(PsaSolverDesc), _VP, _VP, _VP, _VP,

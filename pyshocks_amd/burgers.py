@@ -154,7 +154,7 @@ def _apply_operator_burgers(scheme: FiniteVolumeScheme, grid: Grid, bc: Boundary
     nu = rusanov_nu(grid, scheme.alpha) if isinstance(scheme, Rusanov) else None
     _lib.require_device(u, nu)
     out = torch.empty_like(u)
-    from pyshocks_amd.reconstruction import LearnedWENOJS53
+    from pyshocks_amd.reconstruction import MUSCL, LearnedWENOJS53
 
     # Lax-Friedrichs: the global speed of the ghost-filled state, whose
     # ghosts are images of interior cells
@@ -164,13 +164,16 @@ def _apply_operator_burgers(scheme: FiniteVolumeScheme, grid: Grid, bc: Boundary
             _lib.PSA_FLUX_RUSANOV, _lib.PSA_FLUX_BURGERS_UPWIND):
         raise NotImplementedError(
             "learned weights are implemented for the Rusanov and Godunov fluxes")
-    if isinstance(scheme.rec, LearnedWENOJS53):
-        # the operator stage of the descriptor carries the learned weights
+    if isinstance(scheme.rec, (LearnedWENOJS53, MUSCL)):
+        # the operator stage of the descriptor carries the learned weights or
+        # the limiter
         from pyshocks_amd.timestepping import BoundOperator, _scheme_plan
 
         sp = _scheme_plan(BoundOperator(scheme, grid, bc))
         assert sp is not None
         desc, _keep = sp
+        if speed is not None:
+            desc.velocity = _lib.ptr(speed)
         _lib.call("psa_ssprk33_stage", ctypes.byref(desc), 0, None, _lib.ptr(u),
                   _lib.ptr(out), None, None, _lib.stream_of(u))
         return out

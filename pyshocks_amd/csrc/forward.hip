@@ -103,6 +103,22 @@ __global__ void k_weno_reconstruct(const T *__restrict__ f, int64_t m, T eps,
   }
 }
 
+// reconstruction.py:205-252 MUSCL on f as given (muscl_pair with the ends of
+// the array padded as the reference pads them).
+template <typename T>
+__global__ void k_muscl_reconstruct(const T *__restrict__ f, int64_t m,
+                                    LimParams<T> lim, T *__restrict__ ul,
+                                    T *__restrict__ ur) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const T um1 = load_raw(f, i - 1, m), u0 = f[i], up1 = load_raw(f, i + 1, m);
+  T l, r;
+  muscl_pair(u0, u0 - um1, up1 - u0, lim.lm, lim.theta, lim.atol, i == 0,
+             i == m - 1, l, r);
+  ul[i] = l;
+  ur[i] = r;
+}
+
 template <typename T>
 __global__ void k_apply_boundary(const T *__restrict__ u, T *__restrict__ out,
                                  int64_t m, int64_t n, int g) {
@@ -124,8 +140,17 @@ __global__ void k_apply_boundary(const T *__restrict__ u, T *__restrict__ out,
 // and ul of cell j, by the reconstruction REC.
 template <typename T, int REC>
 __device__ __forceinline__ void face_states(const T *v, int64_t j, int64_t m,
-                                            T eps, T &urL, T &ulR) {
-  if constexpr (REC == PSA_REC_WENOJS53) {
+                                            T eps, T &urL, T &ulR,
+                                            const LimParams<T> &lim) {
+  if constexpr (REC == PSA_REC_MUSCL) {
+    // cells j - 2 .. j + 1 are v[1] .. v[4]; cell j - 1 may be the first
+    // of the array, cell j the last
+    T ul, ur;
+    muscl_pair(v[2], v[2] - v[1], v[3] - v[2], lim.lm, lim.theta, lim.atol,
+               j - 1 == 0, false, ul, urL);
+    muscl_pair(v[3], v[3] - v[2], v[4] - v[3], lim.lm, lim.theta, lim.atol, false,
+               j == m - 1, ulR, ur);
+  } else if constexpr (REC == PSA_REC_WENOJS53) {
     urL = weno_side_ref(v[0], v[1], v[2], v[3], v[4], eps);
     ulR = weno_side_ref(v[5], v[4], v[3], v[2], v[1], eps);
   } else {
@@ -142,7 +167,7 @@ template <typename T, int FLUX, int REC = PSA_REC_WENOJS53>
 __global__ void k_numerical_flux(const T *__restrict__ u,
                                  const T *__restrict__ vel,
                                  const T *__restrict__ nu, int64_t m, T eps,
-                                 T *__restrict__ fnum) {
+                                 T *__restrict__ fnum, LimParams<T> lim) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j > m) return;
   if (j == 0 || j == m) {
@@ -155,7 +180,7 @@ __global__ void k_numerical_flux(const T *__restrict__ u,
   for (int k = 0; k < 6; ++k) v[k] = load_raw(u, j - 3 + k, m);
   // ur of cell j - 1 and ul of cell j
   T urL, ulR;
-  face_states<T, REC>(v, j, m, eps, urL, ulR);
+  face_states<T, REC>(v, j, m, eps, urL, ulR, lim);
   T F;
   if constexpr (FLUX == PSA_FLUX_RUSANOV) {
     const T nuj = nu != nullptr ? nu[j - 1] : T(1);
@@ -173,7 +198,7 @@ __global__ void k_numerical_flux(const T *__restrict__ u,
 #pragma unroll
     for (int k = 0; k < 6; ++k) a[k] = load_raw(vel, j - 3 + k, m);
     T arL, alR;
-    face_states<T, REC>(a, j, m, eps, arL, alR);
+    face_states<T, REC>(a, j, m, eps, arL, alR, lim);
     F = flux_advection_upwind(urL, ulR, arL, alR);
   }
   fnum[j] = F;
@@ -622,6 +647,16 @@ int check_dtype(int dtype) {
   return PSA_OK;
 }
 
+int check_limiter(int lm, double theta, double atol) {
+  if (lm < PSA_LIMITER_MINMOD || lm > PSA_LIMITER_NONE)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported limiter " + std::to_string(lm));
+  if (lm == PSA_LIMITER_MINMOD && !(theta >= 1.0 && theta <= 2.0))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "'theta' must be in [1, 2]");
+  if (!(atol > 0.0))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "the MUSCL slope cut-off atol must be > 0");
+  return PSA_OK;
+}
+
 int check_grid(int64_t m, int g, int gmin = 3) {
   if (g < gmin) {
     return set_error(PSA_ERR_INVALID_ARGUMENT,
@@ -784,18 +819,23 @@ inline bool step_applies(const psa_solver_desc *d, TileConfig cfg, int vel) {
   return cfg.block == 256 && (cfg.cpt == 4 || cfg.cpt == 2);
 }
 
+template <typename T, int REC>
+int launch_step_rec(const psa_solver_desc *d, const StageArgs<T> &a, TileConfig cfg,
+                    hipStream_t stream) {
+  if (d->flux == PSA_FLUX_RUSANOV)
+    return launch_step_tiles<T, PSA_FLUX_RUSANOV, 0, false, REC>(a, cfg, stream);
+  if (d->flux == PSA_FLUX_BURGERS_UPWIND)
+    return launch_step_tiles<T, PSA_FLUX_BURGERS_UPWIND, 0, false, REC>(a, cfg, stream);
+  if (d->flux == PSA_FLUX_ENGQUIST_OSHER)
+    return launch_step_tiles<T, PSA_FLUX_ENGQUIST_OSHER, 0, false, REC>(a, cfg, stream);
+  return launch_step_tiles<T, PSA_FLUX_ADVECTION_UPWIND, 1, false, REC>(a, cfg, stream);
+}
+
 template <typename T>
 int launch_step(const psa_solver_desc *d, int vel, const StageArgs<T> &a,
                 TileConfig cfg, hipStream_t stream) {
-  if (a.rec3) {
-    if (d->flux == PSA_FLUX_RUSANOV)
-      return launch_step_tiles<T, PSA_FLUX_RUSANOV, 0, false, 1>(a, cfg, stream);
-    if (d->flux == PSA_FLUX_BURGERS_UPWIND)
-      return launch_step_tiles<T, PSA_FLUX_BURGERS_UPWIND, 0, false, 1>(a, cfg, stream);
-    if (d->flux == PSA_FLUX_ENGQUIST_OSHER)
-      return launch_step_tiles<T, PSA_FLUX_ENGQUIST_OSHER, 0, false, 1>(a, cfg, stream);
-    return launch_step_tiles<T, PSA_FLUX_ADVECTION_UPWIND, 1, false, 1>(a, cfg, stream);
-  }
+  if (a.rec3) return launch_step_rec<T, 1>(d, a, cfg, stream);
+  if (a.muscl) return launch_step_rec<T, 2>(d, a, cfg, stream);
   if (d->mlp != nullptr) {
     if (d->flux == PSA_FLUX_RUSANOV)
       return launch_step_tiles<T, PSA_FLUX_RUSANOV, 0, true>(a, cfg, stream);
@@ -825,41 +865,48 @@ int launch_stage_learned(const StageArgs<T> &a, TileConfig cfg,
                        " cells_per_thread=" + std::to_string(cfg.cpt));
 }
 
-// Third-order reconstructions: the stage kernels with tiles 256 x 4 or
-// 256 x 2 (the pipelined schedule takes these too).
-template <typename T, int FLUX, int VEL, int STAGE>
-int launch_stage_tiles3(const StageArgs<T> &a, TileConfig cfg,
-                        hipStream_t stream) {
+// The other reconstructions (REC 1 third order, REC 2 MUSCL): the stage
+// kernels with tiles 256 x 4 or 256 x 2 (the pipelined schedule takes these
+// too).
+template <typename T, int FLUX, int VEL, int STAGE, int REC>
+int launch_stage_tiles_rec(const StageArgs<T> &a, TileConfig cfg,
+                           hipStream_t stream) {
   if (cfg.block == 256 && cfg.cpt == 4)
-    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 4, false, 1>(a, stream);
+    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 4, false, REC>(a, stream);
   if (cfg.block == 256 && cfg.cpt == 2)
-    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 2, false, 1>(a, stream);
+    return launch_stage_cfg<T, FLUX, VEL, STAGE, 256, 2, false, REC>(a, stream);
   return set_error(PSA_ERR_INVALID_ARGUMENT,
-                   "unsupported tile configuration for the third-order "
-                   "reconstructions block=" + std::to_string(cfg.block) +
+                   "unsupported tile configuration for the third-order and "
+                   "MUSCL reconstructions block=" + std::to_string(cfg.block) +
                        " cells_per_thread=" + std::to_string(cfg.cpt));
 }
 
-template <typename T, int STAGE>
-int launch_stage3(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
-                  hipStream_t stream) {
+template <typename T, int STAGE, int REC>
+int launch_stage_rec(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
+                     hipStream_t stream) {
   if (flux == PSA_FLUX_RUSANOV)
-    return launch_stage_tiles3<T, PSA_FLUX_RUSANOV, 0, STAGE>(a, cfg, stream);
+    return launch_stage_tiles_rec<T, PSA_FLUX_RUSANOV, 0, STAGE, REC>(a, cfg, stream);
   if (flux == PSA_FLUX_BURGERS_UPWIND)
-    return launch_stage_tiles3<T, PSA_FLUX_BURGERS_UPWIND, 0, STAGE>(a, cfg, stream);
+    return launch_stage_tiles_rec<T, PSA_FLUX_BURGERS_UPWIND, 0, STAGE, REC>(a, cfg,
+                                                                            stream);
   if (flux == PSA_FLUX_ENGQUIST_OSHER)
-    return launch_stage_tiles3<T, PSA_FLUX_ENGQUIST_OSHER, 0, STAGE>(a, cfg, stream);
+    return launch_stage_tiles_rec<T, PSA_FLUX_ENGQUIST_OSHER, 0, STAGE, REC>(a, cfg,
+                                                                            stream);
   if (flux == PSA_FLUX_LAX_FRIEDRICHS)
-    return launch_stage_tiles3<T, PSA_FLUX_LAX_FRIEDRICHS, 0, STAGE>(a, cfg, stream);
+    return launch_stage_tiles_rec<T, PSA_FLUX_LAX_FRIEDRICHS, 0, STAGE, REC>(a, cfg,
+                                                                            stream);
   if (flux == PSA_FLUX_ADVECTION_UPWIND && vel == 1)
-    return launch_stage_tiles3<T, PSA_FLUX_ADVECTION_UPWIND, 1, STAGE>(a, cfg, stream);
-  return launch_stage_tiles3<T, PSA_FLUX_ADVECTION_UPWIND, 2, STAGE>(a, cfg, stream);
+    return launch_stage_tiles_rec<T, PSA_FLUX_ADVECTION_UPWIND, 1, STAGE, REC>(
+        a, cfg, stream);
+  return launch_stage_tiles_rec<T, PSA_FLUX_ADVECTION_UPWIND, 2, STAGE, REC>(a, cfg,
+                                                                            stream);
 }
 
 template <typename T, int STAGE>
 int launch_stage(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
                  hipStream_t stream) {
-  if (a.rec3) return launch_stage3<T, STAGE>(flux, vel, a, cfg, stream);
+  if (a.rec3) return launch_stage_rec<T, STAGE, 1>(flux, vel, a, cfg, stream);
+  if (a.muscl) return launch_stage_rec<T, STAGE, 2>(flux, vel, a, cfg, stream);
   if (a.mlp != nullptr) {
     if (flux == PSA_FLUX_RUSANOV)
       return launch_stage_learned<T, PSA_FLUX_RUSANOV, STAGE>(a, cfg, stream);
@@ -901,9 +948,13 @@ int check_desc(const psa_solver_desc *d) {
   int st = check_dtype(d->dtype);
   if (st != PSA_OK) return st;
   if (d->n <= 0) return set_error(PSA_ERR_INVALID_ARGUMENT, "n must be > 0");
-  if (d->rec < PSA_REC_WENOJS53 || d->rec > PSA_REC_ESWENO32)
+  if (d->rec < PSA_REC_WENOJS53 || d->rec > PSA_REC_MUSCL)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "unsupported reconstruction " + std::to_string(d->rec));
+  if (d->rec == PSA_REC_MUSCL) {
+    st = check_limiter(d->limiter, d->limiter_theta, d->muscl_atol);
+    if (st != PSA_OK) return st;
+  }
   st = check_grid(d->n + 2 * static_cast<int64_t>(d->g), d->g,
                   d->rec == PSA_REC_WENOJS53 ? 3 : 2);
   if (st != PSA_OK) return st;
@@ -965,7 +1016,11 @@ StageArgs<T> make_args(const psa_solver_desc *d) {
   a.nu = static_cast<const T *>(d->nu);
   a.aligned = 1;
   a.es = d->rec == PSA_REC_ESWENO32 ? 1 : 0;
-  a.rec3 = d->rec != PSA_REC_WENOJS53 ? 1 : 0;
+  a.rec3 = (d->rec == PSA_REC_WENOJS32 || d->rec == PSA_REC_ESWENO32) ? 1 : 0;
+  a.muscl = d->rec == PSA_REC_MUSCL ? 1 : 0;
+  a.limiter = d->limiter;
+  a.lim_theta = static_cast<T>(d->limiter_theta);
+  a.lim_atol = static_cast<T>(d->muscl_atol);
   a.mlp = d->mlp;
   a.tau = static_cast<T>(d->mlp_tau);
   a.sigma = static_cast<T>(d->mlp_sigma);
@@ -1102,23 +1157,24 @@ int weno_fields(int dtype, const void *u, void *beta, void *uhat,
 
 template <typename T, int REC>
 void launch_numerical_flux(int flux, const T *uu, const T *vv, const T *nn, T *ff,
-                           int64_t m, T e, hipStream_t s) {
+                           int64_t m, T e, hipStream_t s,
+                           LimParams<T> lim = LimParams<T>{}) {
   const unsigned int blocks = blocks_for(m + 1, kThreads);
   if (flux == PSA_FLUX_RUSANOV) {
     hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_RUSANOV, REC>), dim3(blocks),
-                       dim3(kThreads), 0, s, uu, vv, nn, m, e, ff);
+                       dim3(kThreads), 0, s, uu, vv, nn, m, e, ff, lim);
   } else if (flux == PSA_FLUX_BURGERS_UPWIND) {
     hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_BURGERS_UPWIND, REC>),
-                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e, ff);
+                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e, ff, lim);
   } else if (flux == PSA_FLUX_ENGQUIST_OSHER) {
     hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_ENGQUIST_OSHER, REC>),
-                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e, ff);
+                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e, ff, lim);
   } else if (flux == PSA_FLUX_LAX_FRIEDRICHS) {
     hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_LAX_FRIEDRICHS, REC>),
-                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e, ff);
+                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e, ff, lim);
   } else {
     hipLaunchKernelGGL((k_numerical_flux<T, PSA_FLUX_ADVECTION_UPWIND, REC>),
-                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e, ff);
+                       dim3(blocks), dim3(kThreads), 0, s, uu, vv, nn, m, e, ff, lim);
   }
 }
 
@@ -1228,6 +1284,70 @@ int psa_weno_reconstruct(int dtype, const void *f, void *ul, void *ur,
   return psa_reconstruct(dtype, PSA_REC_WENOJS53, f, ul, ur, m, eps, stream);
 }
 
+int psa_muscl_reconstruct(int dtype, int limiter, double theta, double atol,
+                          const void *f, void *ul, void *ur, int64_t m,
+                          void *stream) {
+  int st = check_dtype(dtype);
+  if (st != PSA_OK) return st;
+  st = check_limiter(limiter, theta, atol);
+  if (st != PSA_OK) return st;
+  if (f == nullptr || ul == nullptr || ur == nullptr || m <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer or m <= 0");
+  auto s = static_cast<hipStream_t>(stream);
+  const unsigned int blocks = blocks_for(m, kThreads);
+  if (dtype == PSA_FLOAT64) {
+    hipLaunchKernelGGL(k_muscl_reconstruct<double>, dim3(blocks), dim3(kThreads), 0, s,
+                       static_cast<const double *>(f), m,
+                       LimParams<double>{limiter, theta, atol},
+                       static_cast<double *>(ul), static_cast<double *>(ur));
+  } else {
+    hipLaunchKernelGGL(k_muscl_reconstruct<float>, dim3(blocks), dim3(kThreads), 0, s,
+                       static_cast<const float *>(f), m,
+                       LimParams<float>{limiter, static_cast<float>(theta),
+                                        static_cast<float>(atol)},
+                       static_cast<float *>(ul), static_cast<float *>(ur));
+  }
+  return check_launch("muscl_reconstruct");
+}
+
+int psa_numerical_flux_desc(const psa_solver_desc *desc, const void *u,
+                            const void *speed, void *fnum, void *stream) {
+  int st = check_desc(desc);
+  if (st != PSA_OK) return st;
+  if (u == nullptr || fnum == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  if (desc->flux == PSA_FLUX_ADVECTION_UPWIND && desc->velocity == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "advection upwind flux needs a velocity array");
+  if (desc->flux == PSA_FLUX_LAX_FRIEDRICHS && speed == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "the Lax-Friedrichs flux needs the device scalar max|a|");
+  if (desc->mlp != nullptr)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "learned weights have psa_learned_reconstruct");
+  auto s = static_cast<hipStream_t>(stream);
+  const int64_t m = desc->n + 2 * static_cast<int64_t>(desc->g);
+  const void *vel = desc->flux == PSA_FLUX_LAX_FRIEDRICHS ? speed : desc->velocity;
+  if (desc->rec != PSA_REC_MUSCL)
+    return psa_numerical_flux_rec(desc->dtype, desc->flux, desc->rec, u, vel,
+                                  desc->nu, fnum, m, desc->eps, stream);
+  if (desc->dtype == PSA_FLOAT64) {
+    launch_numerical_flux<double, PSA_REC_MUSCL>(
+        desc->flux, static_cast<const double *>(u), static_cast<const double *>(vel),
+        static_cast<const double *>(desc->nu), static_cast<double *>(fnum), m,
+        desc->eps, s,
+        LimParams<double>{desc->limiter, desc->limiter_theta, desc->muscl_atol});
+  } else {
+    launch_numerical_flux<float, PSA_REC_MUSCL>(
+        desc->flux, static_cast<const float *>(u), static_cast<const float *>(vel),
+        static_cast<const float *>(desc->nu), static_cast<float *>(fnum), m,
+        static_cast<float>(desc->eps), s,
+        LimParams<float>{desc->limiter, static_cast<float>(desc->limiter_theta),
+                         static_cast<float>(desc->muscl_atol)});
+  }
+  return check_launch("numerical_flux");
+}
+
 int psa_apply_boundary_periodic(int dtype, const void *u, void *out,
                                 int64_t m, int g, void *stream) {
   int st = check_dtype(dtype);
@@ -1294,6 +1414,10 @@ int psa_apply_operator_rec(int dtype, int flux, int rec, const void *u,
                            const void *velocity, const void *nu, void *out,
                            int64_t m, int g, double dx, double eps,
                            void *stream) {
+  if (rec == PSA_REC_MUSCL)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "MUSCL takes its limiter from a descriptor: use "
+                     "psa_ssprk33_stage with stage 0");
   psa_solver_desc d{};
   d.dtype = dtype;
   d.flux = flux;

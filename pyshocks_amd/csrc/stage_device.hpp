@@ -120,6 +120,10 @@ struct StageArgs {
                      //   WENO-JS (0)
   int rec3;          // host side: the kernels take the third-order
                      //   reconstruction (REC = 1)
+  int muscl;         // host side: the kernels take MUSCL (REC = 2)
+  int limiter;       // MUSCL: PSA_LIMITER_*
+  T lim_theta;       // MUSCL: the minmod parameter
+  T lim_atol;        // MUSCL: the slope cut-off
   const float *mlp;  // learned weights (LEARNED kernels)
   T tau;             // their shock threshold
   T sigma;           // their smooth threshold
@@ -433,8 +437,31 @@ __device__ __forceinline__ void reconstruct_cells3(const StageArgs<T> &p,
   }
 }
 
+// MUSCL states (muscl_pair) of CPT cells from the layout of
+// reconstruct_cells; EDGE: the array's first and last cell are padded as the
+// reference pads them.
+template <typename T, int CPT, bool EDGE>
+__device__ __forceinline__ void reconstruct_cells_muscl(const StageArgs<T> &p,
+                                                        const T *w, T *ul,
+                                                        T *ur, int64_t gfirst) {
+  T D[CPT + 1];
+#pragma unroll
+  for (int k = 0; k <= CPT; ++k) D[k] = w[k + 2] - w[k + 1];
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) {
+    bool lo = false, hi = false;
+    if constexpr (EDGE) {
+      const int64_t gm = gfirst + r;
+      lo = gm == 0;
+      hi = gm == p.m - 1;
+    }
+    muscl_pair(w[r + 2], D[r], D[r + 1], p.limiter, p.lim_theta, p.lim_atol, lo,
+               hi, ul[r], ur[r]);
+  }
+}
+
 // The reconstruction of a fused kernel: REC 0 WENO-JS5 (or learned weights),
-// REC 1 third order (reconstruct_cells3).
+// REC 1 third order (reconstruct_cells3), REC 2 MUSCL.
 template <typename T, int CPT, bool LEARNED, int REC, bool EDGE = false>
 __device__ __forceinline__ void reconstruct_any(const StageArgs<T> &p,
                                                 const T *w, T *ul, T *ur,
@@ -443,6 +470,8 @@ __device__ __forceinline__ void reconstruct_any(const StageArgs<T> &p,
                                                 int64_t gfirst) {
   if constexpr (REC == 1) {
     reconstruct_cells3<T, CPT, EDGE>(p, w, ul, ur, gfirst);
+  } else if constexpr (REC == 2) {
+    reconstruct_cells_muscl<T, CPT, EDGE>(p, w, ul, ur, gfirst);
   } else {
     reconstruct_cells<T, CPT, LEARNED>(p, w, ul, ur, src, q);
   }
@@ -550,7 +579,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   static_assert(!LEARNED || VEL == 0, "learned weights are for Burgers");
   static_assert(REC == 0 || !LEARNED,
                 "learned weights replace the WENO-JS5 weights only");
-  constexpr bool EDGE = REC == 1 && STAGE == 0;
+  constexpr bool EDGE = REC != 0 && STAGE == 0;
   constexpr int TILE = BLOCK * CPT;
   constexpr int HALO = 4;
   constexpr int VN = Vec16<T>::n;
@@ -621,9 +650,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   reconstruct_any<T, CPT, LEARNED, REC, EDGE>(p, w, ul, ur, src, q, S + l0);
 
   T al[VEL == 2 ? CPT : 1], ar[VEL == 2 ? CPT : 1];
-  if constexpr (VEL == 2 && REC == 1) {
+  if constexpr (VEL == 2 && REC != 0) {
     // the velocity is reconstructed the same way (advection/schemes.py:108)
-    reconstruct_cells3<T, CPT, EDGE>(p, &sm.v[l0 + 2], al, ar, S + l0);
+    reconstruct_any<T, CPT, false, REC, EDGE>(p, &sm.v[l0 + 2], al, ar, src,
+                                              nullptr, S + l0);
   } else if constexpr (VEL == 2) {
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
@@ -666,9 +696,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     reconstruct_any<T, 1, false, REC, EDGE>(p, &sm.u[1], &el, &er, src, nullptr,
                                             S - 1);
     sm.urL[0] = er;
-    if constexpr (VEL == 2 && REC == 1) {
+    if constexpr (VEL == 2 && REC != 0) {
       T al1, ar1;
-      reconstruct_cells3<T, 1, EDGE>(p, &sm.v[1], &al1, &ar1, S - 1);
+      reconstruct_any<T, 1, false, REC, EDGE>(p, &sm.v[1], &al1, &ar1, src,
+                                              nullptr, S - 1);
       sm.arL[0] = ar1;
     } else if constexpr (VEL == 2) {
       sm.arL[0] = weno_right(sm.v[1], sm.v[2], sm.v[3], sm.v[4], sm.v[5], p.eps);
@@ -681,9 +712,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     reconstruct_any<T, 1, false, REC, EDGE>(p, &sm.u[c - 2], &el, &er, src,
                                             nullptr, S + TILE);
     sm.ulF[BLOCK] = el;
-    if constexpr (VEL == 2 && REC == 1) {
+    if constexpr (VEL == 2 && REC != 0) {
       T al1, ar1;
-      reconstruct_cells3<T, 1, EDGE>(p, &sm.v[c - 2], &al1, &ar1, S + TILE);
+      reconstruct_any<T, 1, false, REC, EDGE>(p, &sm.v[c - 2], &al1, &ar1, src,
+                                              nullptr, S + TILE);
       sm.alF[BLOCK] = al1;
     } else if constexpr (VEL == 2) {
       sm.alF[BLOCK] = weno_right(sm.v[c + 2], sm.v[c + 1], sm.v[c],

@@ -22,6 +22,8 @@
 #include <cfloat>
 #include <cstdint>
 
+#include "../../include/pyshocks_amd.h"
+
 namespace psa {
 
 template <typename T>
@@ -417,6 +419,71 @@ __device__ __forceinline__ void weno3_pair(T u0, T Dm, T Dp, T eps, bool es,
   }
   weno3_divide(fma(w0r, hr0, w1r * hr1), w0r + w1r, fma(w0l, hl0, w1l * hl1),
                w0l + w1l, ur, ul);
+}
+
+// ---------------------------------------------------------------------------
+// MUSCL with a flux limiter (reconstruction.py:172-252, limiters.py:110-408).
+
+// phi(r) of limiters.py:194-399 (evaluate); theta is the minmod parameter.
+template <typename T>
+__device__ __forceinline__ T limiter_phi(int lm, T theta, T r) {
+  switch (lm) {
+    case PSA_LIMITER_MINMOD:
+      return fmax(T(0), fmin(fmin(theta, theta * r), (T(1) + r) / T(2)));
+    case PSA_LIMITER_MC:
+      return fmax(T(0), fmin(fmin(T(2), T(2) * r), (T(1) + r) / T(2)));
+    case PSA_LIMITER_SUPERBEE:
+      return fmax(T(0), fmax(fmin(T(1), T(2) * r), fmin(T(2), r)));
+    case PSA_LIMITER_VANALBADA1:
+      return fmax(T(0), (r * r + r) / (r * r + T(1)));
+    case PSA_LIMITER_VANALBADA2:
+      return fmax(T(0), (T(2) * r) / (r * r + T(1)));
+    case PSA_LIMITER_VANLEER: {
+      const T a = fabs(r);
+      return fmax(T(0), (r + a) / (T(1) + a));
+    }
+    case PSA_LIMITER_KOREN:
+      return fmax(T(0), fmin(fmin(T(2), T(2) * r), (T(1) + T(2) * r) / T(3)));
+    default:  // PSA_LIMITER_NONE
+      return T(1);
+  }
+}
+
+// MUSCL's limiter and its parameters (standalone kernels).
+template <typename T>
+struct LimParams {
+  int lm;
+  T theta;
+  T atol;
+};
+
+// Limiters with phi(1/r) = phi(r) / r (their is_symmetric property).
+__device__ __forceinline__ bool limiter_symmetric(int lm) {
+  return lm == PSA_LIMITER_MINMOD || lm == PSA_LIMITER_MC ||
+         lm == PSA_LIMITER_SUPERBEE || lm == PSA_LIMITER_VANALBADA1 ||
+         lm == PSA_LIMITER_VANLEER;
+}
+
+// Both states of cell i from Dm = u_i - u_{i-1} and Dp = u_{i+1} - u_i:
+// r = Dm / Dp (0 where |Dp| < atol, limiters.py:119-129, and 0 at the ends
+// of the array, jnp.pad), ur = u + phi(r) / 2 Dp, ul = u - phi(1/r) / 2 Dm
+// (phi(r) / r for a symmetric limiter, 0 where |r| < atol; else phi of 1/r,
+// atol where |r| < atol); ur of the last cell and ul of the first are 0.
+template <typename T>
+__device__ __forceinline__ void muscl_pair(T u0, T Dm, T Dp, int lm, T theta,
+                                           T atol, bool lo, bool hi, T &ul,
+                                           T &ur) {
+  T r = fabs(Dp) < atol ? T(0) : Dm / Dp;
+  if (lo || hi) r = T(0);
+  const T phi = limiter_phi(lm, theta, r);
+  T phi_inv;
+  if (limiter_symmetric(lm)) {
+    phi_inv = fabs(r) < atol ? T(0) : phi / r;
+  } else {
+    phi_inv = limiter_phi(lm, theta, fabs(r) < atol ? atol : T(1) / r);
+  }
+  ur = hi ? T(0) : u0 + (T(0.5) * phi) * Dp;
+  ul = lo ? T(0) : u0 - (T(0.5) * phi_inv) * Dm;
 }
 
 }  // namespace psa

@@ -3,8 +3,9 @@
 Mirrors ``reconstruction.py`` of pyshocks: ``Reconstruction`` (``:49-76``),
 ``reconstruct`` (``:80-118``), ``WENOJS`` (``:306-316``), ``WENOJS32``
 (``:320-333``), ``WENOJS53`` (``:336-348``), the WENO-JS registration
-(``:358-377``), ``ESWENO32`` and its registration (``:387-434``) and
-``make_reconstruction_from_name`` (``:528-564``).
+(``:358-377``), ``ESWENO32`` and its registration (``:387-434``), ``MUSCL``
+(``:172-252``, the flux-limited form) and ``make_reconstruction_from_name``
+(``:528-564``).
 """
 
 from __future__ import annotations
@@ -109,8 +110,57 @@ class ESWENO32(Reconstruction):
         return 2
 
 
+@dataclass(frozen=True)
+class MUSCL(Reconstruction):
+    """Second-order MUSCL with a flux limiter:
+    ``ur = f + phi(r) / 2 (f[i+1] - f[i])``, ``ul = f - phi(1/r) / 2
+    (f[i] - f[i-1])``, ``r`` the slope ratio cut off below ``atol``."""
+
+    lm: Any
+    atol: float = 1.0e-12
+
+    @property
+    def name(self) -> str:
+        return f"{type(self).__name__}_{type(self.lm).__name__[:-7]}".lower()
+
+    @property
+    def order(self) -> int:
+        return 2
+
+    @property
+    def stencil_width(self) -> int:
+        return 2
+
+
+@reconstruct.register(MUSCL)
+def _reconstruct_muscl(rec: MUSCL, grid: Grid, bc: BoundaryType, f: torch.Tensor,
+                       u: torch.Tensor, wavespeed: torch.Tensor,
+                       ) -> tuple[torch.Tensor, torch.Tensor]:
+    """The slope ratio is taken from ``f``, which is ``u`` at every call site
+    of the reference's fluxes; a different ``u`` is not supported."""
+    from pyshocks_amd.grid import UniformGrid
+    from pyshocks_amd.limiters import limiter_code
+
+    if not isinstance(grid, UniformGrid):
+        raise NotImplementedError("MUSCL is only implemented for uniform grids.")
+    assert grid.nghosts >= rec.stencil_width
+    if u is not None and u is not f and not torch.equal(u, f):
+        raise NotImplementedError("the GPU MUSCL takes the slope ratio of f itself")
+
+    _lib.require_device(f)
+    ul = torch.empty_like(f)
+    ur = torch.empty_like(f)
+    lm, theta = limiter_code(rec.lm)
+    _lib.call("psa_muscl_reconstruct", _lib.dtype_code(f.dtype), lm, theta,
+              float(rec.atol), _lib.ptr(f), _lib.ptr(ul), _lib.ptr(ur), f.numel(),
+              _lib.stream_of(f))
+    return ul, ur
+
+
 def rec_code(rec: Reconstruction) -> int:
     """The library's ``PSA_REC_*`` code of a reconstruction on the GPU path."""
+    if isinstance(rec, MUSCL):
+        return _lib.PSA_REC_MUSCL
     if isinstance(rec, LearnedWENOJS53):
         raise NotImplementedError("learned weights have their own entry points")
     if isinstance(rec, WENOJS53):
@@ -193,11 +243,12 @@ _RECONSTRUCTION: dict[str, type[Reconstruction]] = {
     "wenojs32": WENOJS32,
     "wenojs53": WENOJS53,
     "esweno32": ESWENO32,
+    "muscl": MUSCL,
     "wenojs53-learned": LearnedWENOJS53,
 }
 
 # names the reference registers that are outside the GPU hot path
-_NOT_ON_GPU = ("constant", "muscl", "muscls", "ssweno242")
+_NOT_ON_GPU = ("constant", "muscls", "ssweno242")
 
 
 def reconstruction_ids() -> tuple[str, ...]:

@@ -12,6 +12,8 @@ Burgers schemes with a WENO-JS5 reconstruction.
 
 from __future__ import annotations
 
+import ctypes
+
 from dataclasses import dataclass
 from typing import TYPE_CHECKING, Any
 
@@ -32,7 +34,7 @@ if TYPE_CHECKING:
 
 def _eps_of(scheme: ConservationLawScheme) -> float:
     _rec_of(scheme)
-    return float(scheme.rec.eps)
+    return float(getattr(scheme.rec, "eps", 0.0))
 
 
 def _rec_of(scheme: ConservationLawScheme) -> int:
@@ -97,7 +99,7 @@ def scalar_flux_upwind(scheme: ConservationLawScheme, grid: Grid, bc: BoundaryTy
     if rec is not None:
         return _learned_flux(rec, _lib.PSA_FLUX_BURGERS_UPWIND, u, None)
     return _numerical_flux(_lib.PSA_FLUX_BURGERS_UPWIND, u, None, None,
-                           _eps_of(scheme), _rec_of(scheme))
+                           _eps_of(scheme), _rec_of(scheme), scheme=scheme, grid=grid)
 
 
 def scalar_flux_rusanov(scheme: ConservationLawScheme, grid: Grid, bc: BoundaryType,
@@ -114,7 +116,7 @@ def scalar_flux_rusanov(scheme: ConservationLawScheme, grid: Grid, bc: BoundaryT
     if rec is not None:
         return _learned_flux(rec, _lib.PSA_FLUX_RUSANOV, u, rusanov_nu(grid, alpha))
     return _numerical_flux(_lib.PSA_FLUX_RUSANOV, u, None, rusanov_nu(grid, alpha),
-                           _eps_of(scheme), _rec_of(scheme))
+                           _eps_of(scheme), _rec_of(scheme), scheme=scheme, grid=grid)
 
 
 def scalar_flux_lax_friedrichs(scheme: ConservationLawScheme, grid: Grid,
@@ -134,7 +136,8 @@ def scalar_flux_lax_friedrichs(scheme: ConservationLawScheme, grid: Grid,
         raise ValueError("the speed array must have the shape and dtype of u")
     amax = max_abs_interior(a.contiguous(), 0)
     return _numerical_flux(_lib.PSA_FLUX_LAX_FRIEDRICHS, u, amax,
-                           rusanov_nu(grid, alpha), _eps_of(scheme), _rec_of(scheme))
+                           rusanov_nu(grid, alpha), _eps_of(scheme), _rec_of(scheme),
+                           scheme=scheme, grid=grid)
 
 
 def scalar_flux_engquist_osher(scheme: ConservationLawScheme, grid: Grid,
@@ -152,14 +155,38 @@ def scalar_flux_engquist_osher(scheme: ConservationLawScheme, grid: Grid,
         raise NotImplementedError(
             "learned weights are implemented for the Rusanov and Godunov fluxes")
     return _numerical_flux(_lib.PSA_FLUX_ENGQUIST_OSHER, u, None, None,
-                           _eps_of(scheme), _rec_of(scheme))
+                           _eps_of(scheme), _rec_of(scheme), scheme=scheme, grid=grid)
 
 
 def _numerical_flux(kind: int, u: torch.Tensor, velocity: torch.Tensor | None,
                     nu: torch.Tensor | None, eps: float,
-                    rec: int = _lib.PSA_REC_WENOJS53) -> torch.Tensor:
+                    rec: int = _lib.PSA_REC_WENOJS53, *, scheme: Any = None,
+                    grid: Any = None) -> torch.Tensor:
+    """Face fluxes of u as given; ``velocity`` is the advection velocity or
+    the Lax-Friedrichs speed. MUSCL (its limiter in the descriptor) needs the
+    scheme and the grid."""
     _lib.require_device(u, velocity, nu)
     fnum = torch.empty(u.numel() + 1, dtype=u.dtype, device=u.device)
+    if rec == _lib.PSA_REC_MUSCL:
+        from pyshocks_amd.limiters import limiter_code
+
+        assert scheme is not None and grid is not None
+        desc = _lib.PsaSolverDesc()
+        desc.dtype = _lib.dtype_code(u.dtype)
+        desc.flux = kind
+        desc.g = grid.nghosts
+        desc.n = u.numel() - 2 * grid.nghosts
+        desc.dx = grid.h
+        desc.rec = rec
+        desc.limiter, desc.limiter_theta = limiter_code(scheme.rec.lm)
+        desc.muscl_atol = float(scheme.rec.atol)
+        desc.nu = _lib.ptr(nu)
+        speed = velocity if kind == _lib.PSA_FLUX_LAX_FRIEDRICHS else None
+        if kind == _lib.PSA_FLUX_ADVECTION_UPWIND:
+            desc.velocity = _lib.ptr(velocity)
+        _lib.call("psa_numerical_flux_desc", ctypes.byref(desc), _lib.ptr(u),
+                  _lib.ptr(speed), _lib.ptr(fnum), _lib.stream_of(u))
+        return fnum
     _lib.call("psa_numerical_flux_rec", _lib.dtype_code(u.dtype), kind, rec,
               _lib.ptr(u), _lib.ptr(velocity), _lib.ptr(nu), _lib.ptr(fnum),
               u.numel(), eps, _lib.stream_of(u))

@@ -96,7 +96,7 @@ def test_registries_and_errors() -> None:
     with pytest.raises(ValueError):
         make_reconstruction_from_name("nope")
     with pytest.raises(NotImplementedError):
-        make_reconstruction_from_name("muscl")
+        make_reconstruction_from_name("muscls")
 
     s = burgers.make_scheme_from_name("rusanov", rec=rec, alpha=0.9, other=1)
     assert isinstance(s, burgers.Rusanov) and s.alpha == 0.9

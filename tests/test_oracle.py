@@ -464,6 +464,92 @@ def test_advection_order_weno3(rec: str) -> None:
     assert W.estimate_order_of_convergence(np.array(hs), np.array(errs)) >= 2.5
 
 
+LIMITERS = [("minmod", 1.0), ("minmod", 1.5), ("minmod", 2.0), ("mc", 0.0),
+            ("superbee", 0.0), ("vanalbada", 1.0), ("vanalbada", 2.0), ("vanleer", 0.0),
+            ("koren", 0.0)]
+
+
+@pytest.mark.parametrize(("name", "param"), LIMITERS)
+def test_limiter_values_and_symmetry(name: str, param: float) -> None:
+    """Closed forms of ``limiters.py``: phi(1) = 1 (second order on linear
+    data), phi(r) = 0 for r <= 0 (for van Albada 1, (r^2 + r) / (r^2 + 1),
+    only on [-1, 0]), Sweby's TVD region 0 <= phi(r) <= min(2, 2 r), and
+    phi(1/r) = phi(r) / r for the limiters flagged symmetric (the shortcut
+    the MUSCL reconstruction takes)."""
+    r = np.linspace(0.01, 5.0, 500)
+    phi = W.limiter_phi(name, param, r)
+    if not (name == "vanalbada" and param == 2):
+        assert abs(float(W.limiter_phi(name, param, np.array(1.0))) - 1.0) < 1e-15
+    neg = -r[r <= 1.0] if (name == "vanalbada" and param == 1) else -r
+    np.testing.assert_array_equal(W.limiter_phi(name, param, neg), 0.0)
+    assert np.all(phi >= 0) and np.all(phi <= np.minimum(2, 2 * r) + 1e-15)
+    sym = np.allclose(W.limiter_phi(name, param, 1 / r), phi / r, rtol=1e-13, atol=0)
+    assert sym == W.limiter_symmetric(name, param)
+
+
+@pytest.mark.parametrize(("name", "param"), LIMITERS)
+def test_muscl_linear_data_and_steps(name: str, param: float) -> None:
+    """MUSCL reproduces the face values of linear data wherever phi(1) = 1,
+    drops to first order (the cell value) on a piecewise-constant step
+    (``tests/test_muscl.py:60-95``), pads ``ur`` of the last and ``ul`` of
+    the first cell with zero, and equals a brute-force loop of the
+    reconstruction formulas on random data."""
+    n = 16
+    f = np.linspace(-1.0, 1.0, n + 1)
+    u = 0.25 + 0.5 * (f[1:] + f[:-1]) / 2
+    ul, ur = W.reconstruct_muscl(u, name, param)
+    if not (name == "vanalbada" and param == 2):
+        np.testing.assert_allclose(ur[1:-1], 0.25 + 0.5 * f[2:-1], rtol=0, atol=1e-14)
+        np.testing.assert_allclose(ul[1:-1], 0.25 + 0.5 * f[1:-2], rtol=0, atol=1e-14)
+    assert ur[-1] == 0.0 and ul[0] == 0.0
+
+    step = np.where(np.arange(n) < n // 2, 1.0, 0.0)
+    ul, ur = W.reconstruct_muscl(step, name, param)
+    # non-symmetric limiters evaluate phi(atol) where 1/r is cut off, so the
+    # first-order value holds to within atol there
+    tol = 0.0 if W.limiter_symmetric(name, param) else 1.0e-11
+    np.testing.assert_allclose(ur[:-1], step[:-1], rtol=0, atol=tol)
+    np.testing.assert_allclose(ul[1:], step[1:], rtol=0, atol=tol)
+
+    u = np.random.default_rng(4).uniform(-1, 1, size=n)
+    ul, ur = W.reconstruct_muscl(u, name, param)
+    atol = 1.0e-12
+    for i in range(1, n - 1):
+        sl, sr = u[i] - u[i - 1], u[i + 1] - u[i]
+        r = 0.0 if abs(sr) < atol else sl / sr
+        phi = float(W.limiter_phi(name, param, np.array(r)))
+        if W.limiter_symmetric(name, param):
+            phi_inv = 0.0 if abs(r) < atol else phi / r
+        else:
+            phi_inv = float(W.limiter_phi(name, param,
+                                          np.array(atol if abs(r) < atol else 1 / r)))
+        assert abs(ur[i] - (u[i] + 0.5 * phi * sr)) < 1e-15
+        assert abs(ul[i] - (u[i] - 0.5 * phi_inv * sl)) < 1e-15
+
+
+@pytest.mark.parametrize(("name", "param"), [("minmod", 1.0), ("vanleer", 0.0),
+                                             ("superbee", 0.0)])
+def test_muscl_step_is_tvd(name: str, param: float) -> None:
+    """Periodic Burgers with MUSCL + Rusanov at Courant 0.5 from a step: the
+    total variation never grows (the TVD property that
+    ``tests/test_muscl.py:175-296`` checks for the slope-limited form)."""
+    n = 128
+    grid = W.make_uniform_cell_grid(-1.0, 1.0, n, nghosts=2)
+    x = grid.x
+    u0 = W.apply_boundary_periodic(grid, np.where(np.abs(x) < 0.5, 1.0, -0.5))
+    sch = W.Scheme("rusanov", eps=0.0, rec=f"muscl-{name}-{param}")
+
+    def tv(u):
+        v = u[grid.i_]
+        return float(np.sum(np.abs(np.diff(np.append(v, v[0])))))
+
+    tvs = []
+    for ev in W.step(lambda t, u: W.predict_timestep(sch, grid, t, u),
+                     lambda t, u: W.apply_operator(sch, grid, t, u), u0, maxit=60):
+        tvs.append(tv(ev.u))
+    assert all(b <= a + 1e-12 for a, b in zip(tvs, tvs[1:]))
+
+
 def test_advection_order_exact_shift() -> None:
     """``tests/test_convergence.py:395-460`` (godunov + wenojs53), periodic on
     [-1, 1] to t = 1 with dt = 8 (2/n)^(5/3)."""
