@@ -20,8 +20,8 @@ the same step loop on the host cores, on a bounded sample.
 The other Burgers fluxes run the same workload with ``--scheme godunov``,
 ``--scheme eo`` (Engquist-Osher, whole-step kernel) or ``--scheme lf``
 (Lax-Friedrichs: the stage kernels, each reducing the global max|u| that the
-next stage's flux uses), and ``--rec wenojs32|esweno32`` the third-order
-reconstructions.
+next stage's flux uses), ``--rec wenojs32|esweno32`` the third-order
+reconstructions, and ``--rec muscl --limiter <name>`` MUSCL.
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
@@ -50,7 +50,8 @@ VALU_PEAK_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 # words moved per cell by each stage kernel: stage 1 reads u^n and writes
 # u^(1); stages 2 and 3 read u^n and u^(k) and write the next state
 STAGE_WORDS = {1: 2, 2: 3, 3: 3}
-REC_NAMES = {"wenojs53": "WENO-JS5", "wenojs32": "WENO-JS3", "esweno32": "ES-WENO3"}
+REC_NAMES = {"wenojs53": "WENO-JS5", "wenojs32": "WENO-JS3", "esweno32": "ES-WENO3",
+             "muscl": "MUSCL"}
 # the whole-step kernel reads u^n and writes u^(n+1) (its halo re-reads, 16
 # of every 1024 cells, are not algorithmic traffic)
 WHOLE_STEP_WORDS = 2
@@ -65,8 +66,10 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
     p.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     p.add_argument("--scheme", choices=["rusanov", "godunov", "lf", "eo", "advection"],
                    default="rusanov")
-    p.add_argument("--rec", choices=["wenojs53", "wenojs32", "esweno32"],
+    p.add_argument("--rec", choices=["wenojs53", "wenojs32", "esweno32", "muscl"],
                    default="wenojs53", help="the reconstruction (WENO-JS5 for the metric)")
+    p.add_argument("--limiter", default="minmod",
+                   help="MUSCL's limiter (minmod with theta 1, mc, superbee, vanleer, ...)")
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--block", type=int, default=0)
     p.add_argument("--cells-per-thread", type=int, default=0)
@@ -127,7 +130,13 @@ def make_problem(args: argparse.Namespace):
                                   device="cuda")
     from pyshocks_amd.reconstruction import make_reconstruction_from_name
 
-    rec = make_reconstruction_from_name(args.rec, eps=1.0e-6)
+    if args.rec == "muscl":
+        from pyshocks_amd.limiters import make_limiter_from_name
+
+        rec = make_reconstruction_from_name(
+            "muscl", lm=make_limiter_from_name(args.limiter, theta=1.0, variant=1))
+    else:
+        rec = make_reconstruction_from_name(args.rec, eps=1.0e-6)
     if args.learned:
         from pyshocks_amd.reconstruction import LearnedWENOJS53
 
@@ -304,11 +313,41 @@ def roofline_record(args: argparse.Namespace, gpu: dict, traffic, valu) -> dict:
     }
 
 
+def cpu_leg_numpy(args: argparse.Namespace, gpu: dict) -> dict:
+    """The NumPy oracle (one host thread) on a bounded sample of the same
+    workload: the MUSCL step, which the C restatement does not have."""
+    from oracle import exact as E
+    from oracle import weno_np as W
+
+    n = args.n
+    u0 = E.exact_cell_averages(E.large_initial_condition(seed=args.seed), n)
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    lm = args.limiter
+    sch = W.Scheme(args.scheme, eps=1e-6, rec=f"muscl-{lm}-1.0")
+    t0 = time.perf_counter()
+    u, _, _ = W.solve(sch, grid, u0, theta=gpu["theta"], maxit=1)
+    one = time.perf_counter() - t0
+    nsteps = int(max(1, min(50, args.cpu_seconds / max(one, 1e-9))))
+    t0 = time.perf_counter()
+    W.solve(sch, grid, u, theta=gpu["theta"], maxit=nsteps)
+    elapsed = time.perf_counter() - t0
+    return {
+        "value": n * nsteps / elapsed,
+        "unit": "cell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"NumPy restatement of the reference step loop, same workload (N={n}, "
+                  f"f64), {nsteps} SSP-RK3 steps after 1 sizing step, {elapsed:.1f} s",
+    }
+
+
 def cpu_leg(args: argparse.Namespace, gpu: dict) -> dict | None:
     """The C restatement of the reference step loop on the host cores, on a
     bounded sample of the same workload."""
     if args.cpu_seconds <= 0:
         return None
+    if args.rec == "muscl":
+        return cpu_leg_numpy(args, gpu)
     import numpy as np
 
     from oracle import build as oracle_build
@@ -585,6 +624,8 @@ def main(argv: list[str] | None = None) -> int:
 
     kname = "step" if gpu["whole_step"] else f"stage{gpu['stage_dominant']}"
     recs = "" if args.rec == "wenojs53" else f"_{args.rec}"
+    if args.rec == "muscl":
+        recs = f"_muscl_{args.limiter}"
     key = f"{args.scheme}{recs}_{args.dtype}_{args.n}_{kname}"
     traffic = _profile_entry("traffic.json", key)
     valu = _profile_entry("valu.json", key)
@@ -610,7 +651,7 @@ def main(argv: list[str] | None = None) -> int:
                         f"Courant 0.5, sine-sum+step u0 seed {args.seed}",
             "n_cells": args.n,
             "scheme": args.scheme,
-            "reconstruction": args.rec,
+            "reconstruction": args.rec if args.rec != "muscl" else f"muscl-{args.limiter}",
             "step": ("one SSP-RK3 time step (one whole-step kernel: the three stages of "
                      "each tile in LDS, device CFL dt)" if gpu["whole_step"] else
                      "one SSP-RK3 time step (3 fused stage kernels, device CFL dt)"),
