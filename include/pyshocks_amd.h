@@ -135,6 +135,21 @@ int psa_muscl_reconstruct(int dtype, int limiter, double theta, double atol,
 int psa_apply_boundary_periodic(int dtype, const void *u, void *out,
                                 int64_t m, int g, void *stream);
 
+/* scalar.py:441-457 (Dirichlet) and scalar.py:477-511 (Neumann) ghost
+ * fills of one or both sides of u (out may be u; the interior is copied):
+ * Dirichlet sets the g ghost cells of a side to the g device values given
+ * for it (g_d(t, x) of the ghost cell centres, left cells 0 .. g-1, right
+ * cells m-g .. m-1; NULL leaves that side); Neumann sets ghost cell g-1-k to
+ * u[g+k] - (2k+1) dx gl and ghost cell m-1-k to u[m-2g+k] + (2k+1-2g) dx gr
+ * for k < g, the reference's u[from] + side (x[from] - x[to]) g_n on a
+ * uniform grid (sides: 1 left, 2 right, 3 both). */
+int psa_apply_boundary_dirichlet(int dtype, const void *u, void *out,
+                                 int64_t m, int g, const void *left,
+                                 const void *right, void *stream);
+int psa_apply_boundary_neumann(int dtype, const void *u, void *out, int64_t m,
+                               int g, double dx, double gl, double gr,
+                               int sides, void *stream);
+
 /* schemes.py:322-336 numerical_flux on u as given (no boundary applied):
  * fnum (m + 1) with fnum[0] = fnum[m] = 0. velocity (m) is required for
  * PSA_FLUX_ADVECTION_UPWIND; for PSA_FLUX_LAX_FRIEDRICHS it points to ONE
@@ -268,7 +283,15 @@ typedef struct psa_solver_desc {
   int pad2_;
   double limiter_theta;         /* PSA_LIMITER_MINMOD: theta in [1, 2]     */
   double muscl_atol;            /* PSA_REC_MUSCL: slope cut-off (1e-12)    */
+  int bc;                       /* PSA_BC_PERIODIC, or PSA_BC_GIVEN: the
+                                   operator (stage 0) takes the ghost cells
+                                   of its input as given (a Dirichlet or
+                                   Neumann fill done beforehand); the
+                                   SSP-RK3 stages are periodic only        */
+  int pad3_;
 } psa_solver_desc;
+#define PSA_BC_PERIODIC 0
+#define PSA_BC_GIVEN 1
 
 /* Step schedules of psa_ssprk33_steps:
  * TILE: one kernel per stage, one workgroup per tile (8 words of HBM traffic

@@ -383,6 +383,33 @@ def apply_boundary_periodic(grid: Grid, u: np.ndarray) -> np.ndarray:
     return u
 
 
+def apply_boundary_dirichlet(grid: Grid, u: np.ndarray, t: float, side: int,
+                             g: Callable[[float, np.ndarray], Any]) -> np.ndarray:
+    """``scalar.py:441-449``: the ghost cells of ``side`` get ``g(t, x)``."""
+    u = u.copy()
+    ng = grid.nghosts
+    ito = np.s_[:ng] if side == -1 else np.s_[u.size - ng:]
+    u[ito] = g(t, grid.x[ito])
+    return u
+
+
+def apply_boundary_neumann(grid: Grid, u: np.ndarray, t: float, side: int,
+                           g: Callable[[float], Any]) -> np.ndarray:
+    """``scalar.py:477-500``: ``u[to] = u[from] + side (x[from] - x[to]) g(t)``
+    with ``from`` the first (last) ``nghosts`` interior cells and ``to`` the
+    ghost cells in reverse order."""
+    u = u.copy()
+    ng = grid.nghosts
+    if side == -1:
+        ifrom = np.arange(ng, 2 * ng)
+        ito = np.arange(ng - 1, -1, -1)
+    else:
+        ifrom = np.arange(u.size - 2 * ng, u.size - ng)
+        ito = np.arange(u.size - 1, u.size - ng - 1, -1)
+    u[ito] = u[ifrom] + side * (grid.x[ifrom] - grid.x[ito]) * g(t)
+    return u
+
+
 def burgers_flux(u: np.ndarray) -> np.ndarray:
     """``burgers/schemes.py:37-39``."""
     return u**2 / 2
@@ -507,10 +534,13 @@ def numerical_flux(scheme: Scheme, grid: Grid, u: np.ndarray) -> np.ndarray:
     raise NotImplementedError(scheme.kind)
 
 
-def apply_operator(scheme: Scheme, grid: Grid, t: float, u: np.ndarray) -> np.ndarray:
+def apply_operator(scheme: Scheme, grid: Grid, t: float, u: np.ndarray,
+                   bc: Callable[[Grid, float, np.ndarray], np.ndarray] | None = None,
+                   ) -> np.ndarray:
     """Conservation law ``schemes.py:339-346``; advection
-    ``advection/schemes.py:62-73``."""
-    u = apply_boundary_periodic(grid, u)
+    ``advection/schemes.py:62-73``; ``bc(grid, t, u)`` fills the ghost cells
+    (periodic when ``None``)."""
+    u = apply_boundary_periodic(grid, u) if bc is None else bc(grid, t, u)
     f = numerical_flux(scheme, grid, u)
 
     if scheme.kind == "advection":
@@ -625,7 +655,9 @@ def step(predict: Callable[[Any, np.ndarray], Any],
 
 def solve(scheme: Scheme, grid: Grid, u0: np.ndarray, *, theta: float,
           tfinal: float | None = None, maxit: int | None = None,
-          fixed_dt: float | None = None) -> tuple[np.ndarray, list[float], float]:
+          fixed_dt: float | None = None,
+          bc: Callable[[Grid, float, np.ndarray], np.ndarray] | None = None,
+          ) -> tuple[np.ndarray, list[float], float]:
     """Forward solve as the reference harness does (``examples/burgers.py:161-179``):
     ``predict = theta * predict_timestep`` (or a fixed ``dt``), ``source =
     apply_operator``. Returns ``(u, dts, t)``."""
@@ -638,7 +670,7 @@ def solve(scheme: Scheme, grid: Grid, u0: np.ndarray, *, theta: float,
             return fixed_dt
 
     def source(t: Any, u: np.ndarray) -> np.ndarray:
-        return apply_operator(scheme, grid, t, u)
+        return apply_operator(scheme, grid, t, u, bc=bc)
 
     dts = []
     event = None

@@ -40,6 +40,9 @@ PSA_REC_WENOJS32 = 1
 PSA_REC_ESWENO32 = 2
 PSA_REC_MUSCL = 3
 
+PSA_BC_PERIODIC = 0
+PSA_BC_GIVEN = 1
+
 (PSA_LIMITER_MINMOD, PSA_LIMITER_MC, PSA_LIMITER_SUPERBEE, PSA_LIMITER_VANALBADA1,
  PSA_LIMITER_VANALBADA2, PSA_LIMITER_VANLEER, PSA_LIMITER_KOREN,
  PSA_LIMITER_NONE) = range(8)
@@ -104,6 +107,8 @@ class PsaSolverDesc(ctypes.Structure):
         ("pad2_", ctypes.c_int),
         ("limiter_theta", ctypes.c_double),
         ("muscl_atol", ctypes.c_double),
+        ("bc", ctypes.c_int),
+        ("pad3_", ctypes.c_int),
     ]
 
 
@@ -159,6 +164,8 @@ SIGNATURES: dict[str, list[Any]] = {
     "psa_reconstruct": [_I, _I, _VP, _VP, _VP, _I64, _D, _VP],
     "psa_muscl_reconstruct": [_I, _I, _D, _D, _VP, _VP, _VP, _I64, _VP],
     "psa_apply_boundary_periodic": [_I, _VP, _VP, _I64, _I, _VP],
+    "psa_apply_boundary_dirichlet": [_I, _VP, _VP, _I64, _I, _VP, _VP, _VP],
+    "psa_apply_boundary_neumann": [_I, _VP, _VP, _I64, _I, _D, _D, _D, _I, _VP],
     "psa_numerical_flux": [_I, _I, _VP, _VP, _VP, _VP, _I64, _D, _VP],
     "psa_apply_operator": [_I, _I, _VP, _VP, _VP, _VP, _I64, _I, _D, _D, _VP],
     "psa_numerical_flux_rec": [_I, _I, _I, _VP, _VP, _VP, _VP, _I64, _D, _VP],

@@ -96,8 +96,10 @@ def _apply_operator_advection(scheme: Godunov, grid: Grid, bc: Boundary, t: Any,
 
     assert scheme.velocity is not None
     if not isinstance(bc, PeriodicBoundary):
-        raise NotImplementedError(
-            f"boundary {type(bc).__name__} is not on the GPU path")
+        from pyshocks_amd.burgers import _apply_operator_ghosts
+
+        _lib.require_device(u, scheme.velocity)
+        return _apply_operator_ghosts(scheme, grid, bc, t, u)
     _lib.require_device(u, scheme.velocity)
     out = torch.empty_like(u)
     if _rec_and_check(scheme) == _lib.PSA_REC_MUSCL:

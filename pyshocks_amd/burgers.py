@@ -149,8 +149,7 @@ def _apply_operator_burgers(scheme: FiniteVolumeScheme, grid: Grid, bc: Boundary
     from pyshocks_amd.scalar import PeriodicBoundary, _eps_of, _rec_of, rusanov_nu
 
     if not isinstance(bc, PeriodicBoundary):
-        raise NotImplementedError(
-            f"boundary {type(bc).__name__} is not on the GPU path")
+        return _apply_operator_ghosts(scheme, grid, bc, t, u)
     nu = rusanov_nu(grid, scheme.alpha) if isinstance(scheme, Rusanov) else None
     _lib.require_device(u, nu)
     out = torch.empty_like(u)
@@ -181,6 +180,33 @@ def _apply_operator_burgers(scheme: FiniteVolumeScheme, grid: Grid, bc: Boundary
               _rec_of(scheme), _lib.ptr(u), _lib.ptr(speed), _lib.ptr(nu),
               _lib.ptr(out), u.numel(), grid.nghosts, grid.h, _eps_of(scheme),
               _lib.stream_of(u))
+    return out
+
+
+def _apply_operator_ghosts(scheme: Any, grid: Grid, bc: Boundary, t: Any,
+                           u: torch.Tensor) -> torch.Tensor:
+    """``schemes.py:339-346`` with a non-periodic boundary: the ghost fill of
+    ``bc`` (Dirichlet, Neumann), then the operator stage on the array as it
+    is (``bc = PSA_BC_GIVEN``); the Lax-Friedrichs speed is the maximum over
+    the whole filled array (``scalar.py:276``)."""
+    from pyshocks_amd.schemes import apply_boundary
+    from pyshocks_amd.timestepping import _scheme_desc
+
+    sd = _scheme_desc(scheme, grid)
+    if sd is None:
+        raise NotImplementedError(f"{type(scheme).__name__} is not on the GPU path")
+    desc, _keep = sd
+    desc.bc = _lib.PSA_BC_GIVEN
+    # the velocity array is reconstructed as the reference does, ends included
+    desc.velocity_is_const = 0
+    ub = apply_boundary(bc, grid, t, u)
+    speed = None
+    if desc.flux == _lib.PSA_FLUX_LAX_FRIEDRICHS:
+        speed = max_abs_interior(ub, 0)
+        desc.velocity = _lib.ptr(speed)
+    out = torch.empty_like(u)
+    _lib.call("psa_ssprk33_stage", ctypes.byref(desc), 0, None, _lib.ptr(ub),
+              _lib.ptr(out), None, None, _lib.stream_of(u))
     return out
 
 

@@ -424,6 +424,8 @@ int check_vjp_desc(const psa_solver_desc *d) {
   if (d->rec != PSA_REC_WENOJS53)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "the adjoint supports the WENO-JS5 reconstruction only");
+  if (d->bc != PSA_BC_PERIODIC)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED, "the adjoint is periodic");
   if (d->g < 3 || d->n < d->g)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "bad grid size");
   if (d->flux == PSA_FLUX_LAX_FRIEDRICHS)

@@ -119,6 +119,41 @@ __global__ void k_muscl_reconstruct(const T *__restrict__ f, int64_t m,
   ur[i] = r;
 }
 
+// scalar.py:441-457: ghost cells of a side from the given values (interior
+// copied when out != u).
+template <typename T>
+__global__ void k_fill_dirichlet(const T *__restrict__ u, T *__restrict__ out,
+                                 int64_t m, int g, const T *__restrict__ left,
+                                 const T *__restrict__ right) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  if (i < g && left != nullptr) {
+    out[i] = left[i];
+  } else if (i >= m - g && right != nullptr) {
+    out[i] = right[i - (m - g)];
+  } else if (out != u) {
+    out[i] = u[i];
+  }
+}
+
+// scalar.py:477-511 on a uniform grid: ghost g-1-k <- u[g+k] - (2k+1) dx gl,
+// ghost m-1-k <- u[m-2g+k] + (2k+1-2g) dx gr (sides: 1 left, 2 right).
+template <typename T>
+__global__ void k_fill_neumann(const T *__restrict__ u, T *__restrict__ out,
+                               int64_t m, int g, T dx, T gl, T gr, int sides) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  if (i < g && (sides & 1)) {
+    const int k = g - 1 - static_cast<int>(i);
+    out[i] = u[g + k] - (T(2 * k + 1) * dx) * gl;
+  } else if (i >= m - g && (sides & 2)) {
+    const int k = static_cast<int>(m - 1 - i);
+    out[i] = u[m - 2 * g + k] + (T(2 * k + 1 - 2 * g) * dx) * gr;
+  } else if (out != u) {
+    out[i] = u[i];
+  }
+}
+
 template <typename T>
 __global__ void k_apply_boundary(const T *__restrict__ u, T *__restrict__ out,
                                  int64_t m, int64_t n, int g) {
@@ -955,6 +990,8 @@ int check_desc(const psa_solver_desc *d) {
     st = check_limiter(d->limiter, d->limiter_theta, d->muscl_atol);
     if (st != PSA_OK) return st;
   }
+  if (d->bc != PSA_BC_PERIODIC && d->bc != PSA_BC_GIVEN)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "bad bc " + std::to_string(d->bc));
   st = check_grid(d->n + 2 * static_cast<int64_t>(d->g), d->g,
                   d->rec == PSA_REC_WENOJS53 ? 3 : 2);
   if (st != PSA_OK) return st;
@@ -1018,6 +1055,7 @@ StageArgs<T> make_args(const psa_solver_desc *d) {
   a.es = d->rec == PSA_REC_ESWENO32 ? 1 : 0;
   a.rec3 = (d->rec == PSA_REC_WENOJS32 || d->rec == PSA_REC_ESWENO32) ? 1 : 0;
   a.muscl = d->rec == PSA_REC_MUSCL ? 1 : 0;
+  a.nowrap = d->bc == PSA_BC_GIVEN ? 1 : 0;
   a.limiter = d->limiter;
   a.lim_theta = static_cast<T>(d->limiter_theta);
   a.lim_atol = static_cast<T>(d->muscl_atol);
@@ -1371,6 +1409,57 @@ int psa_apply_boundary_periodic(int dtype, const void *u, void *out,
   return check_launch("apply_boundary");
 }
 
+int psa_apply_boundary_dirichlet(int dtype, const void *u, void *out,
+                                 int64_t m, int g, const void *left,
+                                 const void *right, void *stream) {
+  int st = check_dtype(dtype);
+  if (st != PSA_OK) return st;
+  if (g < 0 || m - 2 * static_cast<int64_t>(g) < g || m <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "bad grid size");
+  if (u == nullptr || out == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  auto s = static_cast<hipStream_t>(stream);
+  const unsigned int blocks = blocks_for(m, kThreads);
+  if (dtype == PSA_FLOAT64) {
+    hipLaunchKernelGGL(k_fill_dirichlet<double>, dim3(blocks), dim3(kThreads), 0, s,
+                       static_cast<const double *>(u), static_cast<double *>(out), m,
+                       g, static_cast<const double *>(left),
+                       static_cast<const double *>(right));
+  } else {
+    hipLaunchKernelGGL(k_fill_dirichlet<float>, dim3(blocks), dim3(kThreads), 0, s,
+                       static_cast<const float *>(u), static_cast<float *>(out), m, g,
+                       static_cast<const float *>(left),
+                       static_cast<const float *>(right));
+  }
+  return check_launch("apply_boundary_dirichlet");
+}
+
+int psa_apply_boundary_neumann(int dtype, const void *u, void *out, int64_t m,
+                               int g, double dx, double gl, double gr,
+                               int sides, void *stream) {
+  int st = check_dtype(dtype);
+  if (st != PSA_OK) return st;
+  if (g < 0 || m - 2 * static_cast<int64_t>(g) < g || m <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "bad grid size");
+  if (u == nullptr || out == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  if (sides < 0 || sides > 3)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "sides must be 0 .. 3");
+  auto s = static_cast<hipStream_t>(stream);
+  const unsigned int blocks = blocks_for(m, kThreads);
+  if (dtype == PSA_FLOAT64) {
+    hipLaunchKernelGGL(k_fill_neumann<double>, dim3(blocks), dim3(kThreads), 0, s,
+                       static_cast<const double *>(u), static_cast<double *>(out), m,
+                       g, dx, gl, gr, sides);
+  } else {
+    hipLaunchKernelGGL(k_fill_neumann<float>, dim3(blocks), dim3(kThreads), 0, s,
+                       static_cast<const float *>(u), static_cast<float *>(out), m, g,
+                       static_cast<float>(dx), static_cast<float>(gl),
+                       static_cast<float>(gr), sides);
+  }
+  return check_launch("apply_boundary_neumann");
+}
+
 int psa_numerical_flux(int dtype, int flux, const void *u,
                        const void *velocity, const void *nu, void *fnum,
                        int64_t m, double eps, void *stream) {
@@ -1531,6 +1620,9 @@ int psa_ssprk33_steps(const psa_solver_desc *desc, const void *u_in,
   if (k1 == k2 || k1 == u_in || k2 == u_in || k1 == u_out || k2 == u_out)
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "stage buffers must be distinct from the state");
+  if (desc->bc != PSA_BC_PERIODIC)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the fused SSP-RK3 stages are periodic (bc = PSA_BC_PERIODIC)");
   if (nsteps < 0) return set_error(PSA_ERR_INVALID_ARGUMENT, "nsteps < 0");
   if (nsteps == 0) return PSA_OK;
   auto s = static_cast<hipStream_t>(stream);
@@ -1553,6 +1645,9 @@ int psa_ssprk33_stage(const psa_solver_desc *desc, int stage, const void *un,
     return set_error(PSA_ERR_INVALID_ARGUMENT, "null state pointer");
   if (stage > 0 && control == nullptr && dt == nullptr)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "need a control block or dt");
+  if (stage > 0 && desc->bc != PSA_BC_PERIODIC)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the fused SSP-RK3 stages are periodic (bc = PSA_BC_PERIODIC)");
   if (out == uk)
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "a stage cannot write into its stencil input");
@@ -1609,6 +1704,9 @@ int psa_ssprk33_advance(const psa_solver_desc *desc, const void *u,
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "Lax-Friedrichs steps need a control block for the stage "
                      "maxima (psa_ssprk33_steps)");
+  if (desc->bc != PSA_BC_PERIODIC)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the fused SSP-RK3 stages are periodic (bc = PSA_BC_PERIODIC)");
   auto s = static_cast<hipStream_t>(stream);
   if (desc->dtype == PSA_FLOAT64)
     return run_steps<double>(desc, u, out, k1, k2, nullptr, dt, 1, s);

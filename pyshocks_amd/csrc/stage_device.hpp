@@ -121,6 +121,7 @@ struct StageArgs {
   int rec3;          // host side: the kernels take the third-order
                      //   reconstruction (REC = 1)
   int muscl;         // host side: the kernels take MUSCL (REC = 2)
+  int nowrap;        // operator (STAGE 0): the ghost cells of uk as given
   int limiter;       // MUSCL: PSA_LIMITER_*
   T lim_theta;       // MUSCL: the minmod parameter
   T lim_atol;        // MUSCL: the slope cut-off
@@ -623,8 +624,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     if constexpr (STAGE >= 2) cn.store(sm.io, tid);
   } else {
     gather_speed();
-    for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {
-      sm.u[j] = load_filled(p.uk, S - HALO + j, m, n, g);
+    if (STAGE == 0 && p.nowrap) {
+      for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {
+        sm.u[j] = load_raw(p.uk, S - HALO + j, m);
+      }
+    } else {
+      for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {
+        sm.u[j] = load_filled(p.uk, S - HALO + j, m, n, g);
+      }
     }
     if constexpr (VEL == 2) {
       for (int j = tid; j < TILE + 2 * HALO; j += BLOCK) {

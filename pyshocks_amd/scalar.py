@@ -193,6 +193,134 @@ def _numerical_flux(kind: int, u: torch.Tensor, velocity: torch.Tensor | None,
     return fnum
 
 
+# {{{ Dirichlet and Neumann
+
+
+@dataclass(frozen=True)
+class OneSidedBoundary(Boundary):
+    """A boundary condition on one side: ``side`` is ``-1`` (left) or
+    ``+1`` (right) (``scalar.py:333-342``)."""
+
+    side: int
+
+
+@dataclass(frozen=True)
+class TwoSidedBoundary(Boundary):
+    """One boundary condition on each side (``scalar.py:345-382``); each side
+    is applied in turn."""
+
+    left: OneSidedBoundary
+    right: OneSidedBoundary
+
+    def __post_init__(self) -> None:
+        assert isinstance(self.left, OneSidedBoundary)
+        assert isinstance(self.right, OneSidedBoundary)
+        if self.left.side != -1:
+            raise ValueError("Left boundary has incorrect side.")
+        if self.right.side != +1:
+            raise ValueError("Right boundary has incorrect side.")
+
+    @property
+    def boundary_type(self) -> BoundaryType:
+        if self.left.boundary_type != self.right.boundary_type:
+            raise NotImplementedError("Different boundaries on each side.")
+        return self.left.boundary_type
+
+
+@apply_boundary.register(TwoSidedBoundary)
+def _apply_boundary_two_sided(bc: TwoSidedBoundary, grid: Grid, t: Any,
+                              u: torch.Tensor) -> torch.Tensor:
+    u = apply_boundary(bc.left, grid, t, u)
+    return apply_boundary(bc.right, grid, t, u)
+
+
+@evaluate_boundary.register(TwoSidedBoundary)
+def _evaluate_boundary_two_sided(bc: TwoSidedBoundary, grid: Grid, t: Any,
+                                 u: torch.Tensor) -> torch.Tensor:
+    return (evaluate_boundary(bc.left, grid, t, u)
+            + evaluate_boundary(bc.right, grid, t, u))
+
+
+@dataclass(frozen=True)
+class DirichletBoundary(OneSidedBoundary):
+    """``u(t, a) = g(t, x)`` in the ghost cells of ``side``; ``g`` is evaluated
+    on the ghost cell centres (a device tensor) and may return a scalar."""
+
+    g: Any
+
+    @property
+    def boundary_type(self) -> BoundaryType:
+        return BoundaryType.Dirichlet
+
+
+@apply_boundary.register(DirichletBoundary)
+def _apply_boundary_scalar_dirichlet(bc: DirichletBoundary, grid: Grid, t: Any,
+                                     u: torch.Tensor) -> torch.Tensor:
+    """``scalar.py:441-449``: the ghost cells ``grid.g_[side]`` get
+    ``g(t, x[ghosts])``."""
+    assert u.numel() == grid.x.numel()
+    _lib.require_device(u)
+    g = grid.nghosts
+    xg = grid.x[:g] if bc.side == -1 else grid.x[grid.n - g:]
+    vals = torch.as_tensor(bc.g(t, xg), dtype=u.dtype, device=u.device)
+    vals = torch.broadcast_to(vals, (g,)).contiguous()
+    out = torch.empty_like(u)
+    left, right = (vals, None) if bc.side == -1 else (None, vals)
+    _lib.call("psa_apply_boundary_dirichlet", _lib.dtype_code(u.dtype), _lib.ptr(u),
+              _lib.ptr(out), u.numel(), g, _lib.ptr(left), _lib.ptr(right),
+              _lib.stream_of(u))
+    return out
+
+
+def make_dirichlet_boundary(ga: Any, gb: Any = None) -> TwoSidedBoundary:
+    """``scalar.py:452-460``."""
+    if gb is None:
+        gb = ga
+    return TwoSidedBoundary(left=DirichletBoundary(side=-1, g=ga),
+                            right=DirichletBoundary(side=+1, g=gb))
+
+
+@dataclass(frozen=True)
+class NeumannBoundary(OneSidedBoundary):
+    """``side du/dx (t, a) = g(t)`` by a mirror of the interior cells
+    (``scalar.py:466-511``); ``g`` returns a number."""
+
+    g: Any
+
+    @property
+    def boundary_type(self) -> BoundaryType:
+        return BoundaryType.Neumann
+
+
+@apply_boundary.register(NeumannBoundary)
+def _apply_boundary_scalar_neumann(bc: NeumannBoundary, grid: Grid, t: Any,
+                                   u: torch.Tensor) -> torch.Tensor:
+    """``scalar.py:477-500``: ``u[to] = u[from] + side (x[from] - x[to]) g(t)``
+    with the first (last) ``nghosts`` interior cells mirrored into the ghost
+    layer, on a uniform grid ``x[from] - x[to] = (from - to) dx``."""
+    assert u.numel() == grid.x.numel()
+    _lib.require_device(u)
+    gn = float(bc.g(t))
+    out = torch.empty_like(u)
+    left = bc.side == -1
+    _lib.call("psa_apply_boundary_neumann", _lib.dtype_code(u.dtype), _lib.ptr(u),
+              _lib.ptr(out), u.numel(), grid.nghosts, grid.h,
+              gn if left else 0.0, 0.0 if left else gn, 1 if left else 2,
+              _lib.stream_of(u))
+    return out
+
+
+def make_neumann_boundary(ga: Any, gb: Any = None) -> TwoSidedBoundary:
+    """``scalar.py:503-511``."""
+    if gb is None:
+        gb = ga
+    return TwoSidedBoundary(left=NeumannBoundary(side=-1, g=ga),
+                            right=NeumannBoundary(side=+1, g=gb))
+
+
+# }}}
+
+
 # {{{ periodic
 
 

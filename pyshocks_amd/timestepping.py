@@ -222,17 +222,25 @@ class FusedPlan:
 
 
 def _scheme_plan(source: Any) -> tuple[_lib.PsaSolverDesc, tuple[Any, ...]] | None:
-    from pyshocks_amd import advection, burgers
-    from pyshocks_amd.reconstruction import WENOJS53
-    from pyshocks_amd.scalar import PeriodicBoundary, rusanov_nu
+    """The descriptor of a periodic bound operator the fused steps run, or
+    ``None``."""
+    from pyshocks_amd.scalar import PeriodicBoundary
 
     if not isinstance(source, BoundOperator):
         return None
-    scheme, grid, bc = source.scheme, source.grid, source.bc
-    from pyshocks_amd.reconstruction import ESWENO32, MUSCL, WENOJS32
+    if not isinstance(source.bc, PeriodicBoundary):
+        return None
+    return _scheme_desc(source.scheme, source.grid)
 
-    if not isinstance(bc, PeriodicBoundary) or \
-            not isinstance(scheme.rec, (WENOJS53, WENOJS32, ESWENO32, MUSCL)):
+
+def _scheme_desc(scheme: Any, grid: Any) -> tuple[_lib.PsaSolverDesc, tuple[Any, ...]] | None:
+    """The library descriptor of a scheme on a grid (any boundary), with the
+    tensors it points to, or ``None`` if the library does not run it."""
+    from pyshocks_amd import advection, burgers
+    from pyshocks_amd.reconstruction import ESWENO32, MUSCL, WENOJS32, WENOJS53
+    from pyshocks_amd.scalar import rusanov_nu
+
+    if not isinstance(scheme.rec, (WENOJS53, WENOJS32, ESWENO32, MUSCL)):
         return None
 
     desc = _lib.PsaSolverDesc()

@@ -550,6 +550,53 @@ def test_muscl_step_is_tvd(name: str, param: float) -> None:
     assert all(b <= a + 1e-12 for a, b in zip(tvs, tvs[1:]))
 
 
+@pytest.mark.parametrize("side", [-1, +1])
+def test_boundary_fills(side: int) -> None:
+    """``scalar.py:441-500``: Dirichlet ghost cells hold g(t, x) of their
+    centres; a Neumann fill makes the difference quotient of every mirrored
+    pair (ghost, interior) equal to -side g(t) (the reference's sign); the
+    interior is untouched."""
+    n = 12
+    grid = W.make_uniform_cell_grid(-1.0, 1.0, n, nghosts=3)
+    u = np.random.default_rng(6).uniform(-1, 1, size=grid.n)
+    ud = W.apply_boundary_dirichlet(grid, u, 0.3, side, lambda t, x: t + x**2)
+    gs = np.s_[:3] if side == -1 else np.s_[-3:]
+    np.testing.assert_array_equal(ud[gs], 0.3 + grid.x[gs] ** 2)
+    np.testing.assert_array_equal(ud[grid.i_], u[grid.i_])
+
+    un = W.apply_boundary_neumann(grid, u, 0.3, side, lambda t: 2.5 * t)
+    np.testing.assert_array_equal(un[grid.i_], u[grid.i_])
+    for k in range(3):
+        if side == -1:
+            i_g, i_in = 2 - k, 3 + k
+        else:
+            i_g, i_in = grid.n - 1 - k, grid.n - 6 + k
+        q = (un[i_in] - un[i_g]) / (grid.x[i_in] - grid.x[i_g])
+        assert abs(q - (-side) * 0.75) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "eo", "lf"])
+def test_operator_constant_state_with_boundaries(kind: str) -> None:
+    """A constant state with a matching Dirichlet value or a zero Neumann
+    gradient is steady: L vanishes in the interior."""
+    n = 32
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    u = np.full(grid.n, 0.6)
+    sch = W.Scheme(kind, eps=1e-6)
+
+    def dirichlet(g, t, v):
+        v = W.apply_boundary_dirichlet(g, v, t, -1, lambda t, x: 0.6 + 0 * x)
+        return W.apply_boundary_dirichlet(g, v, t, +1, lambda t, x: 0.6 + 0 * x)
+
+    def neumann(g, t, v):
+        v = W.apply_boundary_neumann(g, v, t, -1, lambda t: 0.0)
+        return W.apply_boundary_neumann(g, v, t, +1, lambda t: 0.0)
+
+    for bc in (dirichlet, neumann):
+        L = W.apply_operator(sch, grid, 0.0, u, bc=bc)
+        np.testing.assert_allclose(L[grid.i_], 0.0, atol=1e-12)
+
+
 def test_advection_order_exact_shift() -> None:
     """``tests/test_convergence.py:395-460`` (godunov + wenojs53), periodic on
     [-1, 1] to t = 1 with dt = 8 (2/n)^(5/3)."""
