@@ -464,6 +464,39 @@ __device__ __forceinline__ bool limiter_symmetric(int lm) {
          lm == PSA_LIMITER_VANLEER;
 }
 
+// r = Dm / Dp and its inverse ir = Dp / Dm. FP64: one reciprocal of Dm Dp
+// refined by two Newton steps, r = Dm^2 / (Dm Dp), ir = Dp^2 / (Dm Dp),
+// where Dm Dp is a normal number (either sign), else two divisions; FP32:
+// the hardware reciprocal of each difference. ir is used only where r is
+// not below the cut-off (it is not finite where Dm = 0).
+template <typename T>
+__device__ __forceinline__ void muscl_ratios(T Dm, T Dp, T &r, T &ir);
+
+template <>
+__device__ __forceinline__ void muscl_ratios<double>(double Dm, double Dp,
+                                                     double &r, double &ir) {
+  const double den = Dm * Dp;
+  if (__builtin_amdgcn_class(den, 0x108)) {  // -normal or +normal
+    double inv = __builtin_amdgcn_rcp(den);
+    double e = fma(-den, inv, 1.0);
+    inv = fma(inv, e, inv);
+    e = fma(-den, inv, 1.0);
+    inv = fma(inv, e, inv);
+    r = (Dm * Dm) * inv;
+    ir = (Dp * Dp) * inv;
+  } else {
+    r = Dm / Dp;
+    ir = Dp / Dm;
+  }
+}
+
+template <>
+__device__ __forceinline__ void muscl_ratios<float>(float Dm, float Dp, float &r,
+                                                    float &ir) {
+  r = Dm * __builtin_amdgcn_rcpf(Dp);
+  ir = Dp * __builtin_amdgcn_rcpf(Dm);
+}
+
 // Both states of cell i from Dm = u_i - u_{i-1} and Dp = u_{i+1} - u_i:
 // r = Dm / Dp (0 where |Dp| < atol, limiters.py:119-129, and 0 at the ends
 // of the array, jnp.pad), ur = u + phi(r) / 2 Dp, ul = u - phi(1/r) / 2 Dm
@@ -473,14 +506,18 @@ template <typename T>
 __device__ __forceinline__ void muscl_pair(T u0, T Dm, T Dp, int lm, T theta,
                                            T atol, bool lo, bool hi, T &ul,
                                            T &ur) {
-  T r = fabs(Dp) < atol ? T(0) : Dm / Dp;
+  T q, iq;
+  muscl_ratios(Dm, Dp, q, iq);
+  T r = fabs(Dp) < atol ? T(0) : q;
   if (lo || hi) r = T(0);
   const T phi = limiter_phi(lm, theta, r);
+  const bool small = fabs(r) < atol;
   T phi_inv;
   if (limiter_symmetric(lm)) {
-    phi_inv = fabs(r) < atol ? T(0) : phi / r;
+    // phi(r) / r = phi(r) (Dp / Dm)
+    phi_inv = small ? T(0) : phi * iq;
   } else {
-    phi_inv = limiter_phi(lm, theta, fabs(r) < atol ? atol : T(1) / r);
+    phi_inv = limiter_phi(lm, theta, small ? atol : iq);
   }
   ur = hi ? T(0) : u0 + (T(0.5) * phi) * Dp;
   ul = lo ? T(0) : u0 - (T(0.5) * phi_inv) * Dm;
