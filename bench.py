@@ -21,7 +21,8 @@ The other Burgers fluxes run the same workload with ``--scheme godunov``,
 ``--scheme eo`` (Engquist-Osher, whole-step kernel) or ``--scheme lf``
 (Lax-Friedrichs: the stage kernels, each reducing the global max|u| that the
 next stage's flux uses), ``--rec wenojs32|esweno32`` the third-order
-reconstructions, and ``--rec muscl --limiter <name>`` MUSCL.
+reconstructions, ``--rec muscl --limiter <name>`` MUSCL with a flux limiter
+and ``--rec muscls --limiter <name>`` MUSCL with a slope limiter.
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
@@ -51,7 +52,7 @@ VALU_PEAK_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 # u^(1); stages 2 and 3 read u^n and u^(k) and write the next state
 STAGE_WORDS = {1: 2, 2: 3, 3: 3}
 REC_NAMES = {"wenojs53": "WENO-JS5", "wenojs32": "WENO-JS3", "esweno32": "ES-WENO3",
-             "muscl": "MUSCL"}
+             "muscl": "MUSCL", "muscls": "MUSCL (slope limiter)"}
 # the whole-step kernel reads u^n and writes u^(n+1) (its halo re-reads, 16
 # of every 1024 cells, are not algorithmic traffic)
 WHOLE_STEP_WORDS = 2
@@ -66,7 +67,7 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
     p.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     p.add_argument("--scheme", choices=["rusanov", "godunov", "lf", "eo", "advection"],
                    default="rusanov")
-    p.add_argument("--rec", choices=["wenojs53", "wenojs32", "esweno32", "muscl"],
+    p.add_argument("--rec", choices=["wenojs53", "wenojs32", "esweno32", "muscl", "muscls"],
                    default="wenojs53", help="the reconstruction (WENO-JS5 for the metric)")
     p.add_argument("--limiter", default="minmod",
                    help="MUSCL's limiter (minmod with theta 1, mc, superbee, vanleer, ...)")
@@ -130,11 +131,11 @@ def make_problem(args: argparse.Namespace):
                                   device="cuda")
     from pyshocks_amd.reconstruction import make_reconstruction_from_name
 
-    if args.rec == "muscl":
+    if args.rec in ("muscl", "muscls"):
         from pyshocks_amd.limiters import make_limiter_from_name
 
         rec = make_reconstruction_from_name(
-            "muscl", lm=make_limiter_from_name(args.limiter, theta=1.0, variant=1))
+            args.rec, lm=make_limiter_from_name(args.limiter, theta=1.0, variant=1))
     else:
         rec = make_reconstruction_from_name(args.rec, eps=1.0e-6)
     if args.learned:
@@ -323,7 +324,7 @@ def cpu_leg_numpy(args: argparse.Namespace, gpu: dict) -> dict:
     u0 = E.exact_cell_averages(E.large_initial_condition(seed=args.seed), n)
     grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
     lm = args.limiter
-    sch = W.Scheme(args.scheme, eps=1e-6, rec=f"muscl-{lm}-1.0")
+    sch = W.Scheme(args.scheme, eps=1e-6, rec=f"{args.rec}-{lm}-1.0")
     t0 = time.perf_counter()
     u, _, _ = W.solve(sch, grid, u0, theta=gpu["theta"], maxit=1)
     one = time.perf_counter() - t0
@@ -346,7 +347,7 @@ def cpu_leg(args: argparse.Namespace, gpu: dict) -> dict | None:
     bounded sample of the same workload."""
     if args.cpu_seconds <= 0:
         return None
-    if args.rec == "muscl":
+    if args.rec in ("muscl", "muscls"):
         return cpu_leg_numpy(args, gpu)
     import numpy as np
 
@@ -624,8 +625,8 @@ def main(argv: list[str] | None = None) -> int:
 
     kname = "step" if gpu["whole_step"] else f"stage{gpu['stage_dominant']}"
     recs = "" if args.rec == "wenojs53" else f"_{args.rec}"
-    if args.rec == "muscl":
-        recs = f"_muscl_{args.limiter}"
+    if args.rec in ("muscl", "muscls"):
+        recs = f"_{args.rec}_{args.limiter}"
     key = f"{args.scheme}{recs}_{args.dtype}_{args.n}_{kname}"
     traffic = _profile_entry("traffic.json", key)
     valu = _profile_entry("valu.json", key)
@@ -651,7 +652,8 @@ def main(argv: list[str] | None = None) -> int:
                         f"Courant 0.5, sine-sum+step u0 seed {args.seed}",
             "n_cells": args.n,
             "scheme": args.scheme,
-            "reconstruction": args.rec if args.rec != "muscl" else f"muscl-{args.limiter}",
+            "reconstruction": (f"{args.rec}-{args.limiter}" if args.rec in ("muscl", "muscls")
+                               else args.rec),
             "step": ("one SSP-RK3 time step (one whole-step kernel: the three stages of "
                      "each tile in LDS, device CFL dt)" if gpu["whole_step"] else
                      "one SSP-RK3 time step (3 fused stage kernels, device CFL dt)"),

@@ -77,6 +77,11 @@ extern "C" {
  *                   limiter PSA_LIMITER_* of limiters.py:132-408,
  *                   nghosts >= 2                                         */
 #define PSA_REC_MUSCL 3
+/* PSA_REC_MUSCLS    reconstruction.MUSCLS, the slope-limited form
+ *                   (reconstruction.py:255-296, limiters.py slope_limit):
+ *                   ur, ul = f +- dx / 2 du with the limited slope du of
+ *                   the limiter (not PSA_LIMITER_KOREN), nghosts >= 2     */
+#define PSA_REC_MUSCLS 4
 
 /* limiters (limiters.py:132-408); MINMOD takes theta in [1, 2] */
 #define PSA_LIMITER_MINMOD 0
@@ -129,6 +134,13 @@ int psa_reconstruct(int dtype, int rec, const void *f, void *ul, void *ur,
 int psa_muscl_reconstruct(int dtype, int limiter, double theta, double atol,
                           const void *f, void *ul, void *ur, int64_t m,
                           void *stream);
+
+/* reconstruction.py:279-296 reconstruct(MUSCLS(lm), ...) of f as given on a
+ * uniform grid of cell size dx: du = slope_limit(lm) of the one-sided slopes
+ * (limiters.py:148-371; zero for the first and last cell), ur = f + dx/2 du,
+ * ul = f - dx/2 du. */
+int psa_muscls_reconstruct(int dtype, int limiter, const void *f, void *ul,
+                           void *ur, int64_t m, double dx, void *stream);
 
 /* scalar.py:529-540 apply_boundary(PeriodicBoundary): out = u with the
  * g ghost cells per side filled with their periodic images (out may be u). */

@@ -221,14 +221,60 @@ def reconstruct_muscl(f: np.ndarray, name: str, param: float, atol: float = 1.0e
     return ul, ur
 
 
+def minmod(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """``limiters.py:158-163``."""
+    return np.where(a * b < 0.0, 0.0, np.where(np.abs(a) < np.abs(b), a, b))
+
+
+def slope_limit(name: str, param: float, x: np.ndarray, u: np.ndarray) -> np.ndarray:
+    """``slope_limit(lm, grid, u)`` of ``limiters.py:147-372``: the limited
+    slope in each cell, zero in the first and last."""
+    sl = (u[1:-1] - u[:-2]) / (x[1:-1] - x[:-2])
+    sr = (u[2:] - u[1:-1]) / (x[2:] - x[1:-1])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if name == "none":
+            # the reference's sum, as written (limiters.py:149)
+            s = (u[2:] + u[:-2]) / (x[2:] - x[:-2])
+        elif name == "minmod":
+            s = minmod(sl, sr)
+        elif name == "mc":
+            sc = (u[2:] - u[:-2]) / (x[2:] - x[:-2])
+            s = minmod(minmod(sl, sr), sc)
+        elif name == "superbee":
+            s = np.maximum(minmod(sl, 2 * sr), minmod(2 * sl, sr))
+        elif name == "vanalbada" and param == 1:
+            s = np.where(sl * sr <= 0.0, 0.0, sl * sr * (sl + sr) / (sl**2 + sr**2))
+        elif name == "vanalbada":
+            s = np.where(sl * sr <= 0.0, 0.0, 2.0 * sl * sr / (sl**2 + sr**2))
+        elif name == "vanleer":
+            s = np.where(sl * sr <= 0.0, 0.0, 2.0 * sl * sr / (sl + sr))
+        else:
+            raise NotImplementedError(name)
+    return np.pad(s, 1)
+
+
+def reconstruct_muscls(f: np.ndarray, name: str, param: float, x: np.ndarray,
+                       ) -> tuple[np.ndarray, np.ndarray]:
+    """``reconstruction.py:270-296`` on a uniform grid of spacing
+    ``x[1] - x[0]``."""
+    dx = x[1] - x[0]
+    du = slope_limit(name, param, x, f)
+    return f - 0.5 * dx * du, f + 0.5 * dx * du
+
+
 # }}}
 
 
-def reconstruct(f: np.ndarray, eps: float, rec: str = "wenojs53",
-                ) -> tuple[np.ndarray, np.ndarray]:
+def reconstruct(f: np.ndarray, eps: float, rec: str = "wenojs53", *,
+                x: np.ndarray | None = None) -> tuple[np.ndarray, np.ndarray]:
     """``(ul, ur)`` of ``rec`` in ``{"wenojs53", "wenojs32", "esweno32"}``
-    (``reconstruction.py:358-377,420-434``) or ``"muscl-<limiter>[-<param>]"``
-    (``reconstruction.py:205-252``)."""
+    (``reconstruction.py:358-377,420-434``), ``"muscl-<limiter>[-<param>]"``
+    (``reconstruction.py:205-252``) or ``"muscls-<limiter>[-<param>]"``
+    (``reconstruction.py:270-296``, which needs the cell centres ``x``)."""
+    if rec.startswith("muscls"):
+        assert x is not None and x.shape == f.shape
+        name, param, _ = parse_muscl(rec[:5] + rec[6:])
+        return reconstruct_muscls(f, name, param, x)
     if rec.startswith("muscl"):
         return reconstruct_muscl(f, *parse_muscl(rec))
     if rec == "wenojs53":
@@ -425,7 +471,7 @@ def scalar_flux_rusanov(grid: Grid, u: np.ndarray, a: np.ndarray, *, eps: float,
     else:
         nu = np.array(1.0, dtype=u.dtype)
 
-    ul, ur = reconstruct(u, eps, rec)
+    ul, ur = reconstruct(u, eps, rec, x=grid.x)
     fl = flux(ul)
     fr = flux(ur)
 
@@ -463,7 +509,7 @@ def engquist_osher_faces(ur_left: np.ndarray, ul_right: np.ndarray, *,
 def scalar_flux_engquist_osher(grid: Grid, u: np.ndarray, *, eps: float,
                                omega: float = 0.0, rec: str = "wenojs53") -> np.ndarray:
     """``scalar.py:287-322`` with a WENO-JS reconstruction."""
-    ul, ur = reconstruct(u, eps, rec)
+    ul, ur = reconstruct(u, eps, rec, x=grid.x)
     fnum = engquist_osher_faces(ur[:-1], ul[1:], omega=omega)
     return np.pad(fnum, 1)
 
@@ -474,8 +520,8 @@ def scalar_flux_upwind(grid: Grid, u: np.ndarray, a: np.ndarray, *, eps: float,
                        ) -> np.ndarray:
     """Upwind flux switched on the averaged reconstructed speed
     (``scalar.py:91-132``)."""
-    ul, ur = reconstruct(u, eps, rec)
-    al, ar = reconstruct(a, eps, rec)
+    ul, ur = reconstruct(u, eps, rec, x=grid.x)
+    al, ar = reconstruct(a, eps, rec, x=grid.x)
 
     fl = flux(ul)
     fr = flux(ur)
@@ -489,8 +535,8 @@ def scalar_flux_upwind(grid: Grid, u: np.ndarray, a: np.ndarray, *, eps: float,
 def advection_upwind_flux(grid: Grid, u: np.ndarray, velocity: np.ndarray, *,
                           eps: float, rec: str = "wenojs53") -> np.ndarray:
     """``advection/schemes.py:100-114``."""
-    ul, ur = reconstruct(u, eps, rec)
-    al, ar = reconstruct(velocity, eps, rec)
+    ul, ur = reconstruct(u, eps, rec, x=grid.x)
+    al, ar = reconstruct(velocity, eps, rec, x=grid.x)
 
     aavg = (ar[:-1] + al[1:]) / 2
     fnum = np.where(aavg > 0, ur[:-1], ul[1:])

@@ -39,6 +39,7 @@ PSA_REC_WENOJS53 = 0
 PSA_REC_WENOJS32 = 1
 PSA_REC_ESWENO32 = 2
 PSA_REC_MUSCL = 3
+PSA_REC_MUSCLS = 4
 
 PSA_BC_PERIODIC = 0
 PSA_BC_GIVEN = 1
@@ -163,6 +164,7 @@ SIGNATURES: dict[str, list[Any]] = {
     "psa_weno_reconstruct": [_I, _VP, _VP, _VP, _I64, _D, _VP],
     "psa_reconstruct": [_I, _I, _VP, _VP, _VP, _I64, _D, _VP],
     "psa_muscl_reconstruct": [_I, _I, _D, _D, _VP, _VP, _VP, _I64, _VP],
+    "psa_muscls_reconstruct": [_I, _I, _VP, _VP, _VP, _I64, _D, _VP],
     "psa_apply_boundary_periodic": [_I, _VP, _VP, _I64, _I, _VP],
     "psa_apply_boundary_dirichlet": [_I, _VP, _VP, _I64, _I, _VP, _VP, _VP],
     "psa_apply_boundary_neumann": [_I, _VP, _VP, _I64, _I, _D, _D, _D, _I, _VP],

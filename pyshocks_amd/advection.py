@@ -102,7 +102,7 @@ def _apply_operator_advection(scheme: Godunov, grid: Grid, bc: Boundary, t: Any,
         return _apply_operator_ghosts(scheme, grid, bc, t, u)
     _lib.require_device(u, scheme.velocity)
     out = torch.empty_like(u)
-    if _rec_and_check(scheme) == _lib.PSA_REC_MUSCL:
+    if _rec_and_check(scheme) in (_lib.PSA_REC_MUSCL, _lib.PSA_REC_MUSCLS):
         # the limiter travels in the descriptor: the operator stage
         import ctypes
 

@@ -103,6 +103,21 @@ __global__ void k_weno_reconstruct(const T *__restrict__ f, int64_t m, T eps,
   }
 }
 
+// reconstruction.py:279-296 slope-limited MUSCL on f as given.
+template <typename T>
+__global__ void k_muscls_reconstruct(const T *__restrict__ f, int64_t m,
+                                     LimParams<T> lim, T *__restrict__ ul,
+                                     T *__restrict__ ur) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const T um1 = load_raw(f, i - 1, m), u0 = f[i], up1 = load_raw(f, i + 1, m);
+  T l, r;
+  muscls_pair(u0, u0 - um1, up1 - u0, up1 + um1, lim.lm, lim.dx,
+              i == 0 || i == m - 1, l, r);
+  ul[i] = l;
+  ur[i] = r;
+}
+
 // reconstruction.py:205-252 MUSCL on f as given (muscl_pair with the ends of
 // the array padded as the reference pads them).
 template <typename T>
@@ -185,6 +200,12 @@ __device__ __forceinline__ void face_states(const T *v, int64_t j, int64_t m,
                j - 1 == 0, false, ul, urL);
     muscl_pair(v[3], v[3] - v[2], v[4] - v[3], lim.lm, lim.theta, lim.atol, false,
                j == m - 1, ulR, ur);
+  } else if constexpr (REC == PSA_REC_MUSCLS) {
+    T ul, ur;
+    muscls_pair(v[2], v[2] - v[1], v[3] - v[2], v[3] + v[1], lim.lm, lim.dx,
+                j - 1 == 0, ul, urL);
+    muscls_pair(v[3], v[3] - v[2], v[4] - v[3], v[4] + v[2], lim.lm, lim.dx,
+                j == m - 1, ulR, ur);
   } else if constexpr (REC == PSA_REC_WENOJS53) {
     urL = weno_side_ref(v[0], v[1], v[2], v[3], v[4], eps);
     ulR = weno_side_ref(v[5], v[4], v[3], v[2], v[1], eps);
@@ -692,6 +713,15 @@ int check_limiter(int lm, double theta, double atol) {
   return PSA_OK;
 }
 
+int check_slope_limiter(int lm) {
+  if (lm == PSA_LIMITER_KOREN)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the Koren limiter has no slope form (limiters.py:381-389)");
+  if (lm < PSA_LIMITER_MINMOD || lm > PSA_LIMITER_NONE)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported limiter " + std::to_string(lm));
+  return PSA_OK;
+}
+
 int check_grid(int64_t m, int g, int gmin = 3) {
   if (g < gmin) {
     return set_error(PSA_ERR_INVALID_ARGUMENT,
@@ -871,6 +901,7 @@ int launch_step(const psa_solver_desc *d, int vel, const StageArgs<T> &a,
                 TileConfig cfg, hipStream_t stream) {
   if (a.rec3) return launch_step_rec<T, 1>(d, a, cfg, stream);
   if (a.muscl) return launch_step_rec<T, 2>(d, a, cfg, stream);
+  if (a.muscls) return launch_step_rec<T, 3>(d, a, cfg, stream);
   if (d->mlp != nullptr) {
     if (d->flux == PSA_FLUX_RUSANOV)
       return launch_step_tiles<T, PSA_FLUX_RUSANOV, 0, true>(a, cfg, stream);
@@ -942,6 +973,7 @@ int launch_stage(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
                  hipStream_t stream) {
   if (a.rec3) return launch_stage_rec<T, STAGE, 1>(flux, vel, a, cfg, stream);
   if (a.muscl) return launch_stage_rec<T, STAGE, 2>(flux, vel, a, cfg, stream);
+  if (a.muscls) return launch_stage_rec<T, STAGE, 3>(flux, vel, a, cfg, stream);
   if (a.mlp != nullptr) {
     if (flux == PSA_FLUX_RUSANOV)
       return launch_stage_learned<T, PSA_FLUX_RUSANOV, STAGE>(a, cfg, stream);
@@ -983,11 +1015,15 @@ int check_desc(const psa_solver_desc *d) {
   int st = check_dtype(d->dtype);
   if (st != PSA_OK) return st;
   if (d->n <= 0) return set_error(PSA_ERR_INVALID_ARGUMENT, "n must be > 0");
-  if (d->rec < PSA_REC_WENOJS53 || d->rec > PSA_REC_MUSCL)
+  if (d->rec < PSA_REC_WENOJS53 || d->rec > PSA_REC_MUSCLS)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "unsupported reconstruction " + std::to_string(d->rec));
   if (d->rec == PSA_REC_MUSCL) {
     st = check_limiter(d->limiter, d->limiter_theta, d->muscl_atol);
+    if (st != PSA_OK) return st;
+  }
+  if (d->rec == PSA_REC_MUSCLS) {
+    st = check_slope_limiter(d->limiter);
     if (st != PSA_OK) return st;
   }
   if (d->bc != PSA_BC_PERIODIC && d->bc != PSA_BC_GIVEN)
@@ -1055,6 +1091,7 @@ StageArgs<T> make_args(const psa_solver_desc *d) {
   a.es = d->rec == PSA_REC_ESWENO32 ? 1 : 0;
   a.rec3 = (d->rec == PSA_REC_WENOJS32 || d->rec == PSA_REC_ESWENO32) ? 1 : 0;
   a.muscl = d->rec == PSA_REC_MUSCL ? 1 : 0;
+  a.muscls = d->rec == PSA_REC_MUSCLS ? 1 : 0;
   a.nowrap = d->bc == PSA_BC_GIVEN ? 1 : 0;
   a.limiter = d->limiter;
   a.lim_theta = static_cast<T>(d->limiter_theta);
@@ -1336,16 +1373,41 @@ int psa_muscl_reconstruct(int dtype, int limiter, double theta, double atol,
   if (dtype == PSA_FLOAT64) {
     hipLaunchKernelGGL(k_muscl_reconstruct<double>, dim3(blocks), dim3(kThreads), 0, s,
                        static_cast<const double *>(f), m,
-                       LimParams<double>{limiter, theta, atol},
+                       LimParams<double>{limiter, theta, atol, 0.0},
                        static_cast<double *>(ul), static_cast<double *>(ur));
   } else {
     hipLaunchKernelGGL(k_muscl_reconstruct<float>, dim3(blocks), dim3(kThreads), 0, s,
                        static_cast<const float *>(f), m,
                        LimParams<float>{limiter, static_cast<float>(theta),
-                                        static_cast<float>(atol)},
+                                        static_cast<float>(atol), 0.0f},
                        static_cast<float *>(ul), static_cast<float *>(ur));
   }
   return check_launch("muscl_reconstruct");
+}
+
+int psa_muscls_reconstruct(int dtype, int limiter, const void *f, void *ul,
+                           void *ur, int64_t m, double dx, void *stream) {
+  int st = check_dtype(dtype);
+  if (st != PSA_OK) return st;
+  st = check_slope_limiter(limiter);
+  if (st != PSA_OK) return st;
+  if (f == nullptr || ul == nullptr || ur == nullptr || m <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer or m <= 0");
+  if (!(dx > 0.0)) return set_error(PSA_ERR_INVALID_ARGUMENT, "dx must be positive");
+  auto s = static_cast<hipStream_t>(stream);
+  const unsigned int blocks = blocks_for(m, kThreads);
+  if (dtype == PSA_FLOAT64) {
+    hipLaunchKernelGGL(k_muscls_reconstruct<double>, dim3(blocks), dim3(kThreads), 0,
+                       s, static_cast<const double *>(f), m,
+                       LimParams<double>{limiter, 0.0, 0.0, dx},
+                       static_cast<double *>(ul), static_cast<double *>(ur));
+  } else {
+    hipLaunchKernelGGL(k_muscls_reconstruct<float>, dim3(blocks), dim3(kThreads), 0,
+                       s, static_cast<const float *>(f), m,
+                       LimParams<float>{limiter, 0.0f, 0.0f, static_cast<float>(dx)},
+                       static_cast<float *>(ul), static_cast<float *>(ur));
+  }
+  return check_launch("muscls_reconstruct");
 }
 
 int psa_numerical_flux_desc(const psa_solver_desc *desc, const void *u,
@@ -1366,22 +1428,29 @@ int psa_numerical_flux_desc(const psa_solver_desc *desc, const void *u,
   auto s = static_cast<hipStream_t>(stream);
   const int64_t m = desc->n + 2 * static_cast<int64_t>(desc->g);
   const void *vel = desc->flux == PSA_FLUX_LAX_FRIEDRICHS ? speed : desc->velocity;
-  if (desc->rec != PSA_REC_MUSCL)
+  if (desc->rec != PSA_REC_MUSCL && desc->rec != PSA_REC_MUSCLS)
     return psa_numerical_flux_rec(desc->dtype, desc->flux, desc->rec, u, vel,
                                   desc->nu, fnum, m, desc->eps, stream);
+  auto launch = [&](auto tag) {
+    using T = decltype(tag);
+    const LimParams<T> lim{desc->limiter, static_cast<T>(desc->limiter_theta),
+                           static_cast<T>(desc->muscl_atol), static_cast<T>(desc->dx)};
+    const T *uu = static_cast<const T *>(u);
+    const T *vv = static_cast<const T *>(vel);
+    const T *nn = static_cast<const T *>(desc->nu);
+    T *ff = static_cast<T *>(fnum);
+    if (desc->rec == PSA_REC_MUSCL) {
+      launch_numerical_flux<T, PSA_REC_MUSCL>(desc->flux, uu, vv, nn, ff, m,
+                                              static_cast<T>(desc->eps), s, lim);
+    } else {
+      launch_numerical_flux<T, PSA_REC_MUSCLS>(desc->flux, uu, vv, nn, ff, m,
+                                               static_cast<T>(desc->eps), s, lim);
+    }
+  };
   if (desc->dtype == PSA_FLOAT64) {
-    launch_numerical_flux<double, PSA_REC_MUSCL>(
-        desc->flux, static_cast<const double *>(u), static_cast<const double *>(vel),
-        static_cast<const double *>(desc->nu), static_cast<double *>(fnum), m,
-        desc->eps, s,
-        LimParams<double>{desc->limiter, desc->limiter_theta, desc->muscl_atol});
+    launch(0.0);
   } else {
-    launch_numerical_flux<float, PSA_REC_MUSCL>(
-        desc->flux, static_cast<const float *>(u), static_cast<const float *>(vel),
-        static_cast<const float *>(desc->nu), static_cast<float *>(fnum), m,
-        static_cast<float>(desc->eps), s,
-        LimParams<float>{desc->limiter, static_cast<float>(desc->limiter_theta),
-                         static_cast<float>(desc->muscl_atol)});
+    launch(0.0f);
   }
   return check_launch("numerical_flux");
 }
@@ -1503,7 +1572,7 @@ int psa_apply_operator_rec(int dtype, int flux, int rec, const void *u,
                            const void *velocity, const void *nu, void *out,
                            int64_t m, int g, double dx, double eps,
                            void *stream) {
-  if (rec == PSA_REC_MUSCL)
+  if (rec == PSA_REC_MUSCL || rec == PSA_REC_MUSCLS)
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "MUSCL takes its limiter from a descriptor: use "
                      "psa_ssprk33_stage with stage 0");

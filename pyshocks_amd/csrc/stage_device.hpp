@@ -121,6 +121,7 @@ struct StageArgs {
   int rec3;          // host side: the kernels take the third-order
                      //   reconstruction (REC = 1)
   int muscl;         // host side: the kernels take MUSCL (REC = 2)
+  int muscls;        // host side: the slope-limited MUSCL (REC = 3)
   int nowrap;        // operator (STAGE 0): the ghost cells of uk as given
   int limiter;       // MUSCL: PSA_LIMITER_*
   T lim_theta;       // MUSCL: the minmod parameter
@@ -461,8 +462,27 @@ __device__ __forceinline__ void reconstruct_cells_muscl(const StageArgs<T> &p,
   }
 }
 
+// Slope-limited MUSCL states (muscls_pair) of CPT cells from the layout of
+// reconstruct_cells; EDGE: du = 0 for the array's first and last cell.
+template <typename T, int CPT, bool EDGE>
+__device__ __forceinline__ void reconstruct_cells_muscls(const StageArgs<T> &p,
+                                                         const T *w, T *ul,
+                                                         T *ur, int64_t gfirst) {
+#pragma unroll
+  for (int r = 0; r < CPT; ++r) {
+    bool edge = false;
+    if constexpr (EDGE) {
+      const int64_t gm = gfirst + r;
+      edge = gm == 0 || gm == p.m - 1;
+    }
+    muscls_pair(w[r + 2], w[r + 2] - w[r + 1], w[r + 3] - w[r + 2],
+                w[r + 3] + w[r + 1], p.limiter, p.dx, edge, ul[r], ur[r]);
+  }
+}
+
 // The reconstruction of a fused kernel: REC 0 WENO-JS5 (or learned weights),
-// REC 1 third order (reconstruct_cells3), REC 2 MUSCL.
+// REC 1 third order (reconstruct_cells3), REC 2 MUSCL, REC 3 slope-limited
+// MUSCL.
 template <typename T, int CPT, bool LEARNED, int REC, bool EDGE = false>
 __device__ __forceinline__ void reconstruct_any(const StageArgs<T> &p,
                                                 const T *w, T *ul, T *ur,
@@ -473,6 +493,8 @@ __device__ __forceinline__ void reconstruct_any(const StageArgs<T> &p,
     reconstruct_cells3<T, CPT, EDGE>(p, w, ul, ur, gfirst);
   } else if constexpr (REC == 2) {
     reconstruct_cells_muscl<T, CPT, EDGE>(p, w, ul, ur, gfirst);
+  } else if constexpr (REC == 3) {
+    reconstruct_cells_muscls<T, CPT, EDGE>(p, w, ul, ur, gfirst);
   } else {
     reconstruct_cells<T, CPT, LEARNED>(p, w, ul, ur, src, q);
   }

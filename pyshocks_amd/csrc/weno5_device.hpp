@@ -449,13 +449,59 @@ __device__ __forceinline__ T limiter_phi(int lm, T theta, T r) {
   }
 }
 
-// MUSCL's limiter and its parameters (standalone kernels).
+// MUSCL's limiter and its parameters (standalone kernels); dx is the cell
+// size of the slope-limited form.
 template <typename T>
 struct LimParams {
   int lm;
   T theta;
   T atol;
+  T dx;
 };
+
+// limiters.py:158-163: 0 where a b < 0, else the one of smaller modulus.
+template <typename T>
+__device__ __forceinline__ T minmod2(T a, T b) {
+  return a * b < T(0) ? T(0) : (fabs(a) < fabs(b) ? a : b);
+}
+
+// slope_limit of limiters.py:148-371 from the one-sided slopes sl, sr and
+// the central slope sc of cell i (the unlimited slope is the reference's
+// (u[i+1] + u[i-1]) / (x[i+1] - x[i-1]), given as sc_sum).
+template <typename T>
+__device__ __forceinline__ T slope_limit(int lm, T sl, T sr, T sc, T sc_sum) {
+  switch (lm) {
+    case PSA_LIMITER_MINMOD:
+      return minmod2(sl, sr);
+    case PSA_LIMITER_MC:
+      return minmod2(minmod2(sl, sr), sc);
+    case PSA_LIMITER_SUPERBEE:
+      return fmax(minmod2(sl, T(2) * sr), minmod2(T(2) * sl, sr));
+    case PSA_LIMITER_VANALBADA1:
+      return sl * sr <= T(0) ? T(0) : sl * sr * (sl + sr) / (sl * sl + sr * sr);
+    case PSA_LIMITER_VANALBADA2:
+      return sl * sr <= T(0) ? T(0) : T(2) * sl * sr / (sl * sl + sr * sr);
+    case PSA_LIMITER_VANLEER:
+      return sl * sr <= T(0) ? T(0) : T(2) * sl * sr / (sl + sr);
+    default:  // PSA_LIMITER_NONE
+      return sc_sum;
+  }
+}
+
+// Both states of cell i of the slope-limited MUSCL (reconstruction.py:
+// 279-296) from Dm = u_i - u_{i-1}, Dp = u_{i+1} - u_i, Ds = u_{i+1} + u_{i-1}
+// on a uniform grid: ur, ul = u +- dx/2 du, du = 0 at the ends of the array.
+template <typename T>
+__device__ __forceinline__ void muscls_pair(T u0, T Dm, T Dp, T Ds, int lm, T dx,
+                                            bool edge, T &ul, T &ur) {
+  const T inv_dx = T(1) / dx;
+  const T sl = Dm * inv_dx, sr = Dp * inv_dx;
+  const T sc = (Dm + Dp) * (T(0.5) * inv_dx);
+  const T du = edge ? T(0) : slope_limit(lm, sl, sr, sc, Ds * (T(0.5) * inv_dx));
+  const T h = T(0.5) * dx * du;
+  ur = u0 + h;
+  ul = u0 - h;
+}
 
 // Limiters with phi(1/r) = phi(r) / r (their is_symmetric property).
 __device__ __forceinline__ bool limiter_symmetric(int lm) {

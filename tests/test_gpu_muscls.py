@@ -1,0 +1,242 @@
+"""Slope-limited MUSCL on the GPU against the oracle.
+
+``reconstruction.py:255-296`` and ``limiters.py:147-372``: the
+reconstruction of an array as given (zero slope in the end cells), face
+fluxes and operators of every flux, fused solves through a shock for every
+schedule, and the whole-step kernel against the stage kernels bit for bit.
+FP64 to 1e-12 relative, FP32 to 1e-5 (l2) against FP64. The Koren limiter
+has no slope form and is refused.
+
+The reference divides the slopes by differences of the cell centres
+(``limiters.py:203-204``), the library by the spacing ``dx``: the two agree
+exactly where the centres are exact binary fractions (a power-of-two number
+of cells on [0, 1]) and otherwise differ by the rounding of the centres,
+ulp(x) / dx relative, which the tolerance of such grids adds.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1.0e-12
+RTOL32 = 1.0e-5
+LIMITERS = [("minmod", 1.0), ("mc", 0.0), ("superbee", 0.0), ("vanalbada", 1.0),
+            ("vanalbada", 2.0), ("vanleer", 0.0), ("none", 0.0)]
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pyshocks_amd import _lib
+
+    _lib.load()
+    return torch
+
+
+def _dev(torch, a, dtype=None):
+    return torch.tensor(np.asarray(a), device="cuda", dtype=dtype or torch.float64)
+
+
+def _host(t) -> np.ndarray:
+    return t.detach().double().cpu().numpy()
+
+
+def _rel(a, b) -> float:
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _relmax(a, b) -> float:
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _rec(name: str, param: float):
+    from pyshocks_amd.limiters import make_limiter_from_name
+    from pyshocks_amd.reconstruction import make_reconstruction_from_name
+
+    kw = {"theta": param} if name == "minmod" else {"variant": int(param)}
+    return make_reconstruction_from_name("muscls", lm=make_limiter_from_name(name, **kw))
+
+
+def _oname(name: str, param: float) -> str:
+    return f"muscls-{name}-{param}"
+
+
+def _scheme(kind: str, rec, vel=None):
+    from pyshocks_amd import advection, burgers
+
+    cls = {"rusanov": burgers.Rusanov, "godunov": burgers.Godunov,
+           "lf": burgers.LaxFriedrichs, "eo": burgers.EngquistOsher}
+    if kind == "advection":
+        return advection.Godunov(rec=rec, velocity=vel)
+    return cls[kind](rec=rec)
+
+
+@pytest.mark.parametrize(("name", "param"), LIMITERS)
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+def test_reconstruct_vs_oracle(torch_mod, name: str, param: float, dtype_name: str) -> None:
+    """Random data with a run of equal values and a linear ramp."""
+    torch = torch_mod
+    from oracle import weno_np as W
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.reconstruction import reconstruct
+
+    dtype = getattr(torch, dtype_name)
+    n = 1024
+    rng = np.random.default_rng(len(name) + int(param))
+    f = rng.uniform(-1, 1, size=n + 4)
+    f[100:120] = 0.5
+    f[300:340] = np.linspace(0.0, 1.0, 40)
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=2, dtype=dtype)
+    ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=2)
+    fd = _dev(torch, f, dtype)
+    ul, ur = reconstruct(_rec(name, param), grid, None, fd, fd, None)
+    ulr, urr = W.reconstruct(f, 0.0, _oname(name, param), x=ogrid.x)
+    if dtype == torch.float64:
+        assert _relmax(_host(ul), ulr) < RTOL64 and _relmax(_host(ur), urr) < RTOL64
+        assert _host(ul)[0] == f[0] and _host(ur)[-1] == f[-1]
+    else:
+        assert _rel(_host(ul), ulr) < RTOL32 and _rel(_host(ur), urr) < RTOL32
+
+
+@pytest.mark.parametrize(("name", "param"), [("mc", 0.0), ("vanalbada", 2.0),
+                                             ("none", 0.0)])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "lf", "eo", "advection"])
+@pytest.mark.parametrize("n", [7, 65536, 70001])
+def test_flux_and_operator_vs_oracle(torch_mod, name: str, param: float, kind: str,
+                                     n: int) -> None:
+    torch = torch_mod
+    from oracle import weno_np as W
+    from pyshocks_amd import apply_operator, make_uniform_cell_grid, numerical_flux
+    from pyshocks_amd.scalar import PeriodicBoundary
+
+    # the rounding of the centres (1 ulp of x <= 1 each), relative to dx
+    tol = RTOL64 + (0.0 if n & (n - 1) == 0 else 4 * np.finfo(np.float64).eps * n)
+    g = 2
+    rng = np.random.default_rng(n)
+    u = rng.uniform(-1, 1, size=n + 2 * g)
+    vel = 0.5 + 0.25 * np.sin(2 * np.pi * np.arange(n + 2 * g) / n)
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=g)
+    ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=g)
+    sch = _scheme(kind, _rec(name, param), _dev(torch, vel))
+    osch = W.Scheme(kind, eps=0.0, velocity=vel, rec=_oname(name, param))
+    F = _host(numerical_flux(sch, grid, PeriodicBoundary(), 0.0, _dev(torch, u)))
+    assert _relmax(F, W.numerical_flux(osch, ogrid, u)) < tol
+    L = _host(apply_operator(sch, grid, PeriodicBoundary(), 0.0, _dev(torch, u)))
+    assert _relmax(L, W.apply_operator(osch, ogrid, 0.0, u)) < tol
+
+
+@pytest.mark.parametrize(("name", "param"), [("minmod", 1.0), ("superbee", 0.0),
+                                             ("vanleer", 0.0)])
+@pytest.mark.parametrize("kind", ["rusanov", "lf", "eo"])
+def test_fused_burgers_through_shock(torch_mod, name: str, param: float, kind: str) -> None:
+    torch = torch_mod
+    from oracle import weno_np as W
+    from pyshocks_amd import _lib, make_uniform_cell_grid
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import evolve, fused_plan, make_ssprk33
+
+    n, g = 256, 2
+    ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=g)
+    x = (np.arange(n + 2 * g) - g + 0.5) / n
+    u0 = W.apply_boundary_periodic(ogrid, 0.5 + np.sin(2 * np.pi * x))
+    uref, dts, tref = W.solve(W.Scheme(kind, eps=0.0, rec=_oname(name, param)), ogrid, u0,
+                              theta=1.0, tfinal=0.5)
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=g)
+    st = make_ssprk33(_scheme(kind, _rec(name, param)), grid, PeriodicBoundary(),
+                      theta=1.0)
+    assert fused_plan(st).uses_step_kernel() == (kind != "lf")
+    for sched in (_lib.PSA_SCHEDULE_DEFAULT, _lib.PSA_SCHEDULE_TILE,
+                  _lib.PSA_SCHEDULE_PIPELINE):
+        plan = fused_plan(st)
+        plan.desc.schedule = sched
+        u = _dev(torch, u0)
+        k1, k2 = torch.empty_like(u), torch.empty_like(u)
+        ctrl = plan.new_control(u, 0.0, 0.5, None)
+        plan.steps(u, u, k1, k2, ctrl, len(dts) + 2)
+        c = plan.read_control(ctrl)
+        assert c.status == 0 and c.m == len(dts)
+        assert abs(c.t - tref) < RTOL64 * 0.5
+        assert _rel(_host(u)[g:-g], uref[g:-g]) < RTOL64
+    ev = evolve(st, _dev(torch, u0), tfinal=0.5)
+    assert ev.iteration == len(dts)
+
+
+def test_fused_advection(torch_mod) -> None:
+    torch = torch_mod
+    from oracle import weno_np as W
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import evolve, make_ssprk33
+
+    n, g = 4096, 2
+    rng = np.random.default_rng(5)
+    x = (np.arange(n + 2 * g) - g + 0.5) / n
+    vel = 0.5 + 0.25 * np.sin(2 * np.pi * x)
+    u0 = rng.uniform(-1, 1, size=n + 2 * g)
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=g)
+    ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=g)
+    st = make_ssprk33(_scheme("advection", _rec("mc", 0.0), _dev(torch, vel)), grid,
+                      PeriodicBoundary(), theta=0.5)
+    ev = evolve(st, _dev(torch, u0), maxit=40)
+    u, _, _ = W.solve(W.Scheme("advection", eps=0.0, velocity=vel,
+                               rec=_oname("mc", 0.0)), ogrid, u0, theta=0.5, maxit=40)
+    assert _rel(_host(ev.u)[g:-g], u[g:-g]) < RTOL64
+
+
+@pytest.mark.parametrize(("name", "param"), [("vanalbada", 1.0), ("none", 0.0)])
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+@pytest.mark.parametrize("tiles", [(256, 4), (256, 2)])
+@pytest.mark.parametrize("n", [3, 1013, 100_003])
+def test_step_schedule_bit_for_bit(torch_mod, name: str, param: float, dtype_name: str,
+                                   tiles, n: int) -> None:
+    torch = torch_mod
+    from oracle import exact as E
+    from pyshocks_amd import _lib, make_uniform_cell_grid
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import fused_plan, make_ssprk33
+
+    dtype = getattr(torch, dtype_name)
+    g = 2
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=g, dtype=dtype)
+    st = make_ssprk33(_scheme("rusanov", _rec(name, param)), grid, PeriodicBoundary(),
+                      theta=1.0)
+    avg = E.exact_cell_averages(E.large_initial_condition(seed=9), n)
+    u0 = np.concatenate([avg[3 - g:3], avg[3:-3], avg[-3:-3 + g]])
+    out = []
+    for sched in (_lib.PSA_SCHEDULE_TILE, _lib.PSA_SCHEDULE_STEP):
+        plan = fused_plan(st, tiles=tiles)
+        plan.desc.schedule = sched
+        u = _dev(torch, u0, dtype)
+        v = torch.empty_like(u)
+        k1, k2 = torch.empty_like(u), torch.empty_like(u)
+        ctrl = plan.new_control(u, 0.0, float(np.finfo(plan.np_dtype).max), None)
+        plan.steps(u, v, k1, k2, ctrl, 3)
+        c = plan.read_control(ctrl)
+        assert c.m == 3 and c.status == 0
+        out.append((v, c.t, c.dt))
+    assert out[0][1:] == out[1][1:]
+    assert torch.equal(out[0][0], out[1][0])
+
+
+def test_koren_refused(torch_mod) -> None:
+    """The Koren limiter has no slope form (``limiters.py:381-399``): the
+    reconstruction and the fused solve are refused before any launch."""
+    torch = torch_mod
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.reconstruction import reconstruct
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import evolve, make_ssprk33
+
+    grid = make_uniform_cell_grid(0.0, 1.0, 16, nghosts=2)
+    f = torch.zeros(20, dtype=torch.float64, device="cuda")
+    with pytest.raises(NotImplementedError, match="Koren"):
+        reconstruct(_rec("koren", 0.0), grid, None, f, f, None)
+    st = make_ssprk33(_scheme("rusanov", _rec("koren", 0.0)), grid, PeriodicBoundary(),
+                      theta=1.0)
+    with pytest.raises(NotImplementedError, match="Koren"):
+        evolve(st, f, maxit=1)

@@ -57,6 +57,10 @@ def test_struct_layouts_and_errors() -> None:
     assert lib.psa_weno_reconstruct(7, None, None, None, 10, 1e-6, None) == \
         _lib.PSA_ERR_NOT_IMPLEMENTED
     assert "dtype" in _lib.last_error()
+    # the Koren limiter has no slope form
+    assert lib.psa_muscls_reconstruct(0, _lib.PSA_LIMITER_KOREN, None, None, None, 8,
+                                      0.1, None) == _lib.PSA_ERR_NOT_IMPLEMENTED
+    assert "Koren" in _lib.last_error()
     with pytest.raises(ValueError):
         _lib.check(lib.psa_apply_boundary_periodic(0, None, None, 4, 3, None))
     desc = _lib.PsaSolverDesc()
@@ -96,7 +100,7 @@ def test_registries_and_errors() -> None:
     with pytest.raises(ValueError):
         make_reconstruction_from_name("nope")
     with pytest.raises(NotImplementedError):
-        make_reconstruction_from_name("muscls")
+        make_reconstruction_from_name("ssweno242")
 
     s = burgers.make_scheme_from_name("rusanov", rec=rec, alpha=0.9, other=1)
     assert isinstance(s, burgers.Rusanov) and s.alpha == 0.9

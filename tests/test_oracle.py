@@ -527,6 +527,79 @@ def test_muscl_linear_data_and_steps(name: str, param: float) -> None:
         assert abs(ul[i] - (u[i] - 0.5 * phi_inv * sl)) < 1e-15
 
 
+SLOPE_LIMITERS = [("minmod", 1.0), ("mc", 0.0), ("superbee", 0.0), ("vanalbada", 1.0),
+                  ("vanalbada", 2.0), ("vanleer", 0.0), ("none", 0.0)]
+
+
+@pytest.mark.parametrize(("name", "param", "du"), [
+    ("minmod", 1.0, 1.0), ("mc", 0.0, 1.0), ("superbee", 0.0, 2.0),
+    ("vanalbada", 1.0, 1.2), ("vanalbada", 2.0, 0.6), ("vanleer", 0.0, 1.5),
+    ("none", 0.0, 3.0)])
+def test_slope_limit_known_answers(name: str, param: float, du: float) -> None:
+    """Slopes worked by hand for u = (1, 2, 5) on unit cells: left slope 1,
+    right slope 3, central 2, and the unlimited slope of ``limiters.py:149``
+    is the sum (1 + 5) / 2; the end cells are zero."""
+    x = np.array([0.5, 1.5, 2.5])
+    s = W.slope_limit(name, param, x, np.array([1.0, 2.0, 5.0]))
+    assert s[0] == 0.0 and s[2] == 0.0
+    assert abs(s[1] - du) < 1e-15
+
+
+@pytest.mark.parametrize(("name", "param"), SLOPE_LIMITERS)
+def test_muscls_linear_data_and_extrema(name: str, param: float) -> None:
+    """Every limited slope of linear data is its slope, so the faces are
+    exact (van Albada 2 gives 1, the unlimited form the centred sum);
+    at a local extremum every limited slope is zero (first order)."""
+    n = 16
+    grid = W.make_uniform_cell_grid(-1.0, 1.0, n, nghosts=2)
+    x = grid.x
+    u = 0.25 + 0.5 * x
+    ul, ur = W.reconstruct_muscls(u, name, param, x)
+    slope = {"vanalbada": 0.5 if param == 1 else 1.0, "none": None}.get(name, 0.5)
+    if slope is None:
+        slope = (u[2:] + u[:-2]) / (x[2:] - x[:-2])
+    h = grid.dx[grid.i_][0] / 2
+    np.testing.assert_allclose(ur[1:-1], u[1:-1] + h * slope, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(ul[1:-1], u[1:-1] - h * slope, rtol=0, atol=1e-14)
+    assert ul[0] == ur[0] == u[0] and ul[-1] == ur[-1] == u[-1]
+
+    if name != "none":
+        peak = np.array([0.0, 1.0, 3.0, 2.0, 1.0])
+        ul, ur = W.reconstruct_muscls(peak, name, param, np.arange(5.0))
+        assert ul[2] == ur[2] == 3.0
+
+
+@pytest.mark.parametrize(("name", "param"), [("minmod", 1.0), ("superbee", 0.0),
+                                             ("vanalbada", 1.0), ("vanleer", 0.0)])
+def test_muscls_matches_flux_limiter_form(name: str, param: float) -> None:
+    """Where both slopes are positive the slope form of these limiters is
+    phi(r) times the right slope, so the slope-limited faces equal the
+    flux-limited MUSCL faces (``reconstruction.py:205-252``); for minmod and
+    van Leer that holds in every interior cell. Superbee's slope form takes
+    the larger of two signed minmods and van Albada 1's is zero for r < 0,
+    so they differ from the flux forms elsewhere; MC's slope form also
+    limits by the central slope."""
+    n = 200
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=2)
+    u = np.random.default_rng(11).uniform(-1, 1, size=grid.x.size)
+    ul, ur = W.reconstruct_muscls(u, name, param, grid.x)
+    vl, vr = W.reconstruct_muscl(u, name, param)
+    d = np.diff(u)
+    up = np.zeros(u.size, dtype=bool)
+    up[1:-1] = (d[:-1] > 0) & (d[1:] > 0)
+    every = name in ("minmod", "vanleer")
+    mask = np.ones(u.size, dtype=bool) if every else up
+    mask[0] = mask[-1] = False
+    assert np.count_nonzero(mask & up) > 20
+    np.testing.assert_allclose(ur[mask], vr[mask], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(ul[mask], vl[mask], rtol=0, atol=1e-13)
+    if not every:
+        assert np.max(np.abs(ur - vr)[1:-1]) > 1e-3
+    ml, mr = W.reconstruct_muscls(u, "mc", 0.0, grid.x)
+    wl, wr = W.reconstruct_muscl(u, "mc", 0.0)
+    assert np.max(np.abs(mr - wr)[1:-1]) > 1e-3
+
+
 @pytest.mark.parametrize(("name", "param"), [("minmod", 1.0), ("vanleer", 0.0),
                                              ("superbee", 0.0)])
 def test_muscl_step_is_tvd(name: str, param: float) -> None:
