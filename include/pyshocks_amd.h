@@ -388,7 +388,8 @@ typedef struct psa_batch_desc {
   int dtype;                    /* PSA_FLOAT64 / PSA_FLOAT32               */
   int flux;                     /* PSA_FLUX_RUSANOV / _BURGERS_UPWIND /
                                    _ENGQUIST_OSHER                         */
-  int g;                        /* ghost cells per side (>= 3)             */
+  int g;                        /* ghost cells per side (>= 3; >= 2 for
+                                   the other reconstructions)              */
   int grid;                     /* workgroups (0 = resident capacity)      */
   long long ninst;              /* instances                               */
   const long long *offsets;     /* [ninst] first cell of each instance     */
@@ -430,7 +431,13 @@ typedef struct psa_batch_desc {
    * u and k1); PSA_SCHEDULE_TILE: one pass and one barrier per stage. Both
    * compute the same values. */
   int schedule;
-  int pad_;
+  int rec;                      /* PSA_REC_* (0 = WENO-JS5); the others take
+                                   g >= 2 and tiles 256 x 4 or 256 x 2, and
+                                   no learned weights                      */
+  int limiter;                  /* PSA_LIMITER_* of PSA_REC_MUSCL(S)      */
+  int pad2_;
+  double limiter_theta;         /* minmod's theta (PSA_REC_MUSCL)          */
+  double muscl_atol;            /* slope-ratio cut-off (PSA_REC_MUSCL)     */
 } psa_batch_desc;
 
 #define PSA_TEAM_STATE_BYTES 64

@@ -142,7 +142,11 @@ class PsaBatchDesc(ctypes.Structure):
         ("team_state", ctypes.c_void_p),
         ("team_wait_s", ctypes.c_double),
         ("schedule", ctypes.c_int),
-        ("pad_", ctypes.c_int),
+        ("rec", ctypes.c_int),
+        ("limiter", ctypes.c_int),
+        ("pad2_", ctypes.c_int),
+        ("limiter_theta", ctypes.c_double),
+        ("muscl_atol", ctypes.c_double),
     ]
 
 

@@ -4,7 +4,8 @@ uneven size advanced in one persistent launch.
 Each instance ``i`` is a uniform periodic grid of ``ncells[i]`` cells on
 ``[a, b]`` (3 ghost cells per side) with its own initial state and final time,
 advanced by the fused WENO-JS5 / Rusanov (or Godunov) / SSP-RK3 step (or its
-learned-weight form, ``mlp``) with its own CFL time step -- the loop of ``timestepping.py:89-152`` per instance, as
+learned-weight form, ``mlp``, or another reconstruction, ``rec``: WENO-JS3,
+ES-WENO3 and MUSCL with flux or slope limiters) with its own CFL time step -- the loop of ``timestepping.py:89-152`` per instance, as
 the reference harness would run it for each request separately. The instances
 are packed into three device arrays (state and two stage buffers) at 16-byte
 aligned offsets; the library runs them longest-first from an atomic queue,
@@ -46,13 +47,28 @@ class BatchedSolver:
                  tiles: tuple[int, int] | None = None, grid: int = 0,
                  mlp: Any = None, time_f64: bool = True,
                  team_cells: int | None = None, max_team: int = 32,
-                 team_wait_s: float = 60.0, schedule: str = "step") -> None:
+                 team_wait_s: float = 60.0, schedule: str = "step",
+                 rec: Any = None) -> None:
+        """``rec`` is a reconstruction object of the GPU path
+        (:mod:`pyshocks_amd.reconstruction`; ``None``: WENO-JS5 with ``eps``);
+        its own ``eps`` replaces the argument. The reconstructions other than
+        WENO-JS5 take 2 ghost cells, tiles 256 x 4 or 256 x 2 and no learned
+        weights."""
         if scheme not in _lib.BATCH_FLUX:
             raise NotImplementedError(
                 f"the batched solver takes Burgers 'rusanov', 'godunov' or 'eo', "
                 f"not {scheme!r}")
-        if nghosts < 3:
-            raise ValueError("WENO-JS5 needs nghosts >= 3")
+        rec_kind = _lib.PSA_REC_WENOJS53
+        if rec is not None:
+            from pyshocks_amd.reconstruction import rec_code
+
+            rec_kind = rec_code(rec)
+            eps = float(getattr(rec, "eps", eps))
+        gmin = 3 if rec_kind == _lib.PSA_REC_WENOJS53 else 2
+        if nghosts < gmin:
+            raise ValueError(f"the reconstruction needs nghosts >= {gmin}")
+        if rec_kind != _lib.PSA_REC_WENOJS53 and mlp is not None:
+            raise ValueError("learned weights replace the WENO-JS5 weights only")
         if schedule not in ("step", "tile"):
             raise ValueError(f"schedule must be 'step' or 'tile', not {schedule!r}")
         ncells = np.asarray(ncells, dtype=np.int64)
@@ -109,13 +125,20 @@ class BatchedSolver:
         # sweep per stage (the same values)
         self.desc.schedule = _lib.PSA_SCHEDULE_STEP if schedule == "step" \
             else _lib.PSA_SCHEDULE_TILE
+        self.rec = rec
+        self.desc.rec = rec_kind
+        if rec_kind in (_lib.PSA_REC_MUSCL, _lib.PSA_REC_MUSCLS):
+            from pyshocks_amd.limiters import limiter_code
+
+            self.desc.limiter, self.desc.limiter_theta = limiter_code(rec.lm)
+            self.desc.muscl_atol = float(rec.atol)
         self.mlp = mlp
         if mlp is not None:
             # learned weights (pyshocks_amd.learned.MLPWeights)
             self.desc.mlp = mlp.ptr
             self.desc.mlp_tau = float(mlp.tau)
             self.desc.mlp_sigma = float(mlp.sigma)
-        if tiles is None:
+        if tiles is None and rec_kind == _lib.PSA_REC_WENOJS53:
             from pyshocks_amd.autotune import lookup_batch
 
             tiles = lookup_batch(self.desc.dtype, self.desc.flux, mlp is not None)

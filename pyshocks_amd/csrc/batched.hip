@@ -77,6 +77,10 @@ struct BatchArgs {
   const float *mlp;
   T tau;
   T sigma;
+  int rec;       // PSA_REC_* (ES-WENO3 and WENO-JS3 share the kernels)
+  int limiter;   // PSA_LIMITER_* of the MUSCL reconstructions
+  T lim_theta;
+  T lim_atol;
 };
 
 // A value every lane of the wave holds (read from LDS after a broadcast)
@@ -144,7 +148,9 @@ __device__ __forceinline__ bool team_barrier(TeamState *ts, int size,
 // WHOLE: one sweep per step with the three stages of each tile in LDS
 // (step_tile), the state alternating between u and k1; otherwise one sweep
 // per stage (stage_tile).
-template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED, bool WHOLE>
+// REC: the reconstruction class of the fused kernels (0 WENO-JS5 or learned,
+// 1 third order, 2 MUSCL, 3 slope-limited MUSCL).
+template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED, bool WHOLE, int REC>
 __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
   using Smem = typename std::conditional<WHOLE, StepSmem<T, BLOCK, CPT>,
                                          StageSmem<T, 0, BLOCK, CPT>>::type;
@@ -189,6 +195,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
     p.mlp = a.mlp;
     p.tau = a.tau;
     p.sigma = a.sigma;
+    p.es = a.rec == PSA_REC_ESWENO32 ? 1 : 0;
+    p.limiter = a.limiter;
+    p.lim_theta = a.lim_theta;
+    p.lim_atol = a.lim_atol;
     p.aligned = 1;  // instance offsets are 16-byte aligned
     T *u = a.u + a.offsets[i];
     T *k1 = a.k1 + a.offsets[i];
@@ -293,8 +303,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
         unsigned long long vmax = 0;
         for (int64_t tile = tlo; tile < thi; ++tile) {
           __syncthreads();
-          step_tile<T, FLUX, 0, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, src, dst, sm,
-                                                     vmax, q, wlp);
+          step_tile<T, FLUX, 0, BLOCK, CPT, LEARNED, REC>(p, tile * TILE, dt, src, dst,
+                                                          sm, vmax, q, wlp);
         }
         umax = team_max(vmax, static_cast<int>(steps % 3));  // u^(n+1) complete
       } else {
@@ -304,8 +314,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
         p.out = k1;
         for (int64_t tile = tlo; tile < thi; ++tile) {
           __syncthreads();
-          stage_tile<T, FLUX, 0, 1, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
-                                                            vdummy, q, wlp);
+          stage_tile<T, FLUX, 0, 1, BLOCK, CPT, LEARNED, false, false, REC>(
+              p, tile * TILE, dt, sm, vdummy, q, wlp);
         }
         barrier();  // k1 complete
         // every member has read the max|u| reduced two steps ago: clear that
@@ -317,8 +327,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
         p.out = k2;
         for (int64_t tile = tlo; tile < thi; ++tile) {
           __syncthreads();
-          stage_tile<T, FLUX, 0, 2, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
-                                                            vdummy, q, wlp);
+          stage_tile<T, FLUX, 0, 2, BLOCK, CPT, LEARNED, false, false, REC>(
+              p, tile * TILE, dt, sm, vdummy, q, wlp);
         }
         barrier();  // k2 complete
         p.uk = k2;
@@ -326,8 +336,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
         unsigned long long vmax = 0;
         for (int64_t tile = tlo; tile < thi; ++tile) {
           __syncthreads();
-          stage_tile<T, FLUX, 0, 3, BLOCK, CPT, LEARNED>(p, tile * TILE, dt, sm,
-                                                         vmax, q, wlp);
+          stage_tile<T, FLUX, 0, 3, BLOCK, CPT, LEARNED, false, false, REC>(
+              p, tile * TILE, dt, sm, vmax, q, wlp);
         }
         umax = team_max(vmax, static_cast<int>(steps & 1));  // u complete
       }
@@ -384,10 +394,11 @@ unsigned int resident(K kernel, int block) {
   return static_cast<unsigned int>(per_cu) * static_cast<unsigned int>(cus);
 }
 
-template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED, bool WHOLE>
+template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED, bool WHOLE, int REC = 0>
 int launch_batch_cfg(const BatchArgs<T> &a, int grid, int max_team, long long nteams,
                      hipStream_t s) {
-  const unsigned int cap = resident(k_batch<T, FLUX, BLOCK, CPT, LEARNED, WHOLE>, BLOCK);
+  const unsigned int cap =
+      resident(k_batch<T, FLUX, BLOCK, CPT, LEARNED, WHOLE, REC>, BLOCK);
   if (cap == 0)
     return set_error(PSA_ERR_HIP, "cannot query the resident workgroup count");
   unsigned int blocks = grid > 0 ? static_cast<unsigned int>(grid) : cap;
@@ -410,14 +421,35 @@ int launch_batch_cfg(const BatchArgs<T> &a, int grid, int max_team, long long nt
   hipLaunchKernelGGL(k_batch_reset, dim3(rb), dim3(256), 0, s, a.counter, a.teams, nt);
   int st = check_launch("batch reset");
   if (st != PSA_OK) return st;
-  hipLaunchKernelGGL((k_batch<T, FLUX, BLOCK, CPT, LEARNED, WHOLE>), dim3(blocks),
+  hipLaunchKernelGGL((k_batch<T, FLUX, BLOCK, CPT, LEARNED, WHOLE, REC>), dim3(blocks),
                      dim3(BLOCK), 0, s, a);
   return check_launch("batch solve");
+}
+
+// The other reconstructions take tiles 256 x 4 or 256 x 2, as the fused
+// single-instance kernels.
+template <typename T, int FLUX, bool WHOLE, int REC>
+int launch_batch_rec(const BatchArgs<T> &a, int block, int cpt, int grid,
+                     int max_team, long long nteams, hipStream_t s) {
+  if (block == 256 && cpt == 4)
+    return launch_batch_cfg<T, FLUX, 256, 4, false, WHOLE, REC>(a, grid, max_team, nteams, s);
+  if (block == 256 && cpt == 2)
+    return launch_batch_cfg<T, FLUX, 256, 2, false, WHOLE, REC>(a, grid, max_team, nteams, s);
+  return set_error(PSA_ERR_INVALID_ARGUMENT,
+                   "the third-order and MUSCL reconstructions take tiles 256 x 4 or "
+                   "256 x 2, not block=" + std::to_string(block) +
+                       " cells_per_thread=" + std::to_string(cpt));
 }
 
 template <typename T, int FLUX, bool WHOLE>
 int launch_batch_tiles(const BatchArgs<T> &a, int block, int cpt, int grid,
                        int max_team, long long nteams, hipStream_t s) {
+  if (a.rec == PSA_REC_WENOJS32 || a.rec == PSA_REC_ESWENO32)
+    return launch_batch_rec<T, FLUX, WHOLE, 1>(a, block, cpt, grid, max_team, nteams, s);
+  if (a.rec == PSA_REC_MUSCL)
+    return launch_batch_rec<T, FLUX, WHOLE, 2>(a, block, cpt, grid, max_team, nteams, s);
+  if (a.rec == PSA_REC_MUSCLS)
+    return launch_batch_rec<T, FLUX, WHOLE, 3>(a, block, cpt, grid, max_team, nteams, s);
   if (a.mlp != nullptr) {
     if constexpr (FLUX == PSA_FLUX_ENGQUIST_OSHER) {
       return set_error(PSA_ERR_NOT_IMPLEMENTED,
@@ -491,6 +523,10 @@ int launch_batch(const psa_batch_desc *d, void *u, void *k1, void *k2,
   a.mlp = d->mlp;
   a.tau = static_cast<T>(d->mlp_tau);
   a.sigma = static_cast<T>(d->mlp_sigma);
+  a.rec = d->rec;
+  a.limiter = d->limiter;
+  a.lim_theta = static_cast<T>(d->limiter_theta);
+  a.lim_atol = static_cast<T>(d->muscl_atol);
   int dev = 0, rate_khz = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) !=
@@ -533,8 +569,24 @@ extern "C" int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1,
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "the batched solver takes Burgers Rusanov, Godunov or "
                      "Engquist-Osher");
-  if (desc->g < 3)
-    return set_error(PSA_ERR_INVALID_ARGUMENT, "needs at least 3 ghost cells");
+  if (desc->rec < PSA_REC_WENOJS53 || desc->rec > PSA_REC_MUSCLS)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "unsupported reconstruction " + std::to_string(desc->rec));
+  if (desc->rec != PSA_REC_WENOJS53 && desc->mlp != nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "learned weights replace the WENO-JS5 weights only");
+  if (desc->rec == PSA_REC_MUSCL) {
+    const int st = check_limiter(desc->limiter, desc->limiter_theta, desc->muscl_atol);
+    if (st != PSA_OK) return st;
+  }
+  if (desc->rec == PSA_REC_MUSCLS) {
+    const int st = check_slope_limiter(desc->limiter);
+    if (st != PSA_OK) return st;
+  }
+  const int gmin = desc->rec == PSA_REC_WENOJS53 ? 3 : 2;
+  if (desc->g < gmin)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "needs at least " + std::to_string(gmin) + " ghost cells");
   if (desc->ninst < 0 || desc->max_steps < 0 || !(desc->eps >= 0.0))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "bad batch parameters");
   if (desc->schedule != PSA_SCHEDULE_DEFAULT && desc->schedule != PSA_SCHEDULE_TILE &&

@@ -703,25 +703,6 @@ int check_dtype(int dtype) {
   return PSA_OK;
 }
 
-int check_limiter(int lm, double theta, double atol) {
-  if (lm < PSA_LIMITER_MINMOD || lm > PSA_LIMITER_NONE)
-    return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported limiter " + std::to_string(lm));
-  if (lm == PSA_LIMITER_MINMOD && !(theta >= 1.0 && theta <= 2.0))
-    return set_error(PSA_ERR_INVALID_ARGUMENT, "'theta' must be in [1, 2]");
-  if (!(atol > 0.0))
-    return set_error(PSA_ERR_INVALID_ARGUMENT, "the MUSCL slope cut-off atol must be > 0");
-  return PSA_OK;
-}
-
-int check_slope_limiter(int lm) {
-  if (lm == PSA_LIMITER_KOREN)
-    return set_error(PSA_ERR_NOT_IMPLEMENTED,
-                     "the Koren limiter has no slope form (limiters.py:381-389)");
-  if (lm < PSA_LIMITER_MINMOD || lm > PSA_LIMITER_NONE)
-    return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported limiter " + std::to_string(lm));
-  return PSA_OK;
-}
-
 int check_grid(int64_t m, int g, int gmin = 3) {
   if (g < gmin) {
     return set_error(PSA_ERR_INVALID_ARGUMENT,

@@ -21,4 +21,27 @@ int check_launch(const char *what);
 int step_fixed_dt(const psa_solver_desc *d, const void *u_in, void *u_out, void *k1,
                   void *k2, const void *dt, hipStream_t stream);
 
+// Limiter of the flux-limited MUSCL: a known PSA_LIMITER_*, minmod's theta
+// in [1, 2] (limiters.py:166-198), a positive slope-ratio cut-off.
+inline int check_limiter(int lm, double theta, double atol) {
+  if (lm < PSA_LIMITER_MINMOD || lm > PSA_LIMITER_NONE)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported limiter " + std::to_string(lm));
+  if (lm == PSA_LIMITER_MINMOD && !(theta >= 1.0 && theta <= 2.0))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "'theta' must be in [1, 2]");
+  if (!(atol > 0.0))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "the MUSCL slope cut-off atol must be > 0");
+  return PSA_OK;
+}
+
+// Limiter of the slope-limited MUSCL: every limiter but Koren, which has no
+// slope form (limiters.py:381-399).
+inline int check_slope_limiter(int lm) {
+  if (lm == PSA_LIMITER_KOREN)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the Koren limiter has no slope form (limiters.py:381-389)");
+  if (lm < PSA_LIMITER_MINMOD || lm > PSA_LIMITER_NONE)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported limiter " + std::to_string(lm));
+  return PSA_OK;
+}
+
 }  // namespace psa
