@@ -129,15 +129,7 @@ def make_problem(args: argparse.Namespace):
     dtype = torch.float64 if args.dtype == "f64" else torch.float32
     grid = make_uniform_cell_grid(0.0, 1.0, args.n, nghosts=3, dtype=dtype,
                                   device="cuda")
-    from pyshocks_amd.reconstruction import make_reconstruction_from_name
-
-    if args.rec in ("muscl", "muscls"):
-        from pyshocks_amd.limiters import make_limiter_from_name
-
-        rec = make_reconstruction_from_name(
-            args.rec, lm=make_limiter_from_name(args.limiter, theta=1.0, variant=1))
-    else:
-        rec = make_reconstruction_from_name(args.rec, eps=1.0e-6)
+    rec = make_rec(args)
     if args.learned:
         from pyshocks_amd.reconstruction import LearnedWENOJS53
 
@@ -436,10 +428,25 @@ def adjoint_leg(args: argparse.Namespace, repeats: int = 3) -> dict:
                       f"checkpoints every {every}, J = 0.5||u(T)-u_target||^2 dx"}
 
 
+def make_rec(args: argparse.Namespace):
+    """The reconstruction object of --rec (with --limiter for MUSCL)."""
+    from pyshocks_amd.reconstruction import make_reconstruction_from_name
+
+    if args.rec in ("muscl", "muscls"):
+        from pyshocks_amd.limiters import make_limiter_from_name
+
+        return make_reconstruction_from_name(
+            args.rec, lm=make_limiter_from_name(args.limiter, theta=1.0, variant=1))
+    return make_reconstruction_from_name(args.rec, eps=1.0e-6)
+
+
 def batched_leg(args: argparse.Namespace) -> dict:
     """cell-updates/s of the uneven-size batch: --ninst Burgers instances with
     N log-uniform in [128, 65536] (seeded), FP32, Rusanov, each to t = 0.3,
-    one persistent launch replayed from a HIP graph, tiles autotuned."""
+    one persistent launch replayed from a HIP graph, tiles autotuned; --rec
+    other than WENO-JS5 runs with 2 ghost cells, tiles 256 x 4 or 256 x 2 and
+    (unless --team-cells says otherwise) teams of 16384 cells, which its
+    smaller tiles need to keep the largest instances off the critical path."""
     import numpy as np
     import torch
 
@@ -447,14 +454,20 @@ def batched_leg(args: argparse.Namespace) -> dict:
     from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
 
     _lib.load()
+    rec = None if args.rec == "wenojs53" else make_rec(args)
+    g = 3 if rec is None else 2
     ns, ics = random_instances(args.ninst, seed=args.seed)
-    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    states = [exact_averages_np(ic, int(n), nghosts=g) for ic, n in zip(ics, ns)]
     sample = np.random.default_rng(args.seed + 1).choice(len(ns), size=min(32, len(ns)),
                                                         replace=False)
+    cands = autotune.BATCH_CANDIDATES if rec is None else ((256, 4), (256, 2))
     best, _ = autotune.tune_batch(ns[sample], [states[i] for i in sample],
-                                  dtype=torch.float32, tfinal=0.01)
-    solver = BatchedSolver(ns, dtype=torch.float32, team_cells=args.team_cells or None,
-                           max_team=args.max_team)
+                                  dtype=torch.float32, tfinal=0.01, rec=rec,
+                                  candidates=cands)
+    team_cells = args.team_cells or (None if rec is None else 16384)
+    solver = BatchedSolver(ns, dtype=torch.float32, team_cells=team_cells,
+                           max_team=args.max_team, nghosts=g, rec=rec,
+                           tiles=None if rec is None else best)
     solver.set_states(states)
     u0 = solver.u.clone()
     graph = solver.capture(u0, 0.3)
@@ -475,7 +488,7 @@ def batched_leg(args: argparse.Namespace) -> dict:
 def batched_cpu_leg(args: argparse.Namespace, ns, states) -> dict | None:
     """The C restatement on a seeded subset of the batch's instances (FP64,
     to t = 0.3), within about --cpu-seconds."""
-    if args.cpu_seconds <= 0:
+    if args.cpu_seconds <= 0 or args.rec != "wenojs53":
         return None
     import numpy as np
 
@@ -593,8 +606,12 @@ def main(argv: list[str] | None = None) -> int:
                           "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
                           "data": "synthetic",
                           "config": {"workload": f"{b['ninst']} periodic Burgers Rusanov "
-                                                 "instances, N log-uniform [128, 65536], "
-                                                 "f32, each to t = 0.3, one graph launch"},
+                                                 f"{REC_NAMES[args.rec]} instances, N "
+                                                 "log-uniform [128, 65536], f32, each to "
+                                                 "t = 0.3, one graph launch",
+                                     "reconstruction": (
+                                         f"{args.rec}-{args.limiter}"
+                                         if args.rec in ("muscl", "muscls") else args.rec)},
                           "batched": b, "cpu_baseline": cpu}), flush=True)
         return 0
     if args.workload == "learned":

@@ -76,10 +76,12 @@ def tune_batch(ncells: Sequence[int], states: Sequence[np.ndarray], *,
                dtype: torch.dtype = torch.float32, scheme: str = "rusanov",
                tfinal: float = 0.01, eps: float = 1.0e-6, theta: float = 1.0,
                candidates: Sequence[tuple[int, int]] = BATCH_CANDIDATES,
-               mlp: Any = None,
+               mlp: Any = None, rec: Any = None,
                ) -> tuple[tuple[int, int], dict[tuple[int, int], float]]:
     """Time the batched solve of a sample of instances (to a short final
-    time) for each candidate and cache the fastest."""
+    time) for each candidate and cache the fastest; with a reconstruction
+    other than WENO-JS5 (``rec``, states with 2 ghost cells) the winner is
+    returned but not cached."""
     from pyshocks_amd import _lib
     from pyshocks_amd.batched import BatchedSolver
 
@@ -87,7 +89,8 @@ def tune_batch(ncells: Sequence[int], states: Sequence[np.ndarray], *,
     for cand in candidates:
         try:
             solver = BatchedSolver(ncells, dtype=dtype, scheme=scheme, eps=eps,
-                                   theta=theta, tiles=cand, mlp=mlp)
+                                   theta=theta, tiles=cand, mlp=mlp, rec=rec,
+                                   nghosts=3 if rec is None else 2)
             solver.set_states(states)
             u0 = solver.u.clone()
             solver.solve(tfinal)  # warm up
@@ -105,5 +108,7 @@ def tune_batch(ncells: Sequence[int], states: Sequence[np.ndarray], *,
         raise RuntimeError("no tile candidate could be launched")
     best = min(times, key=lambda c: times[c])
     flux = _lib.BATCH_FLUX[scheme]
+    if rec is not None:
+        return best, times
     _BATCH_CACHE[(_lib.dtype_code(dtype), flux, mlp is not None)] = best
     return best, times
