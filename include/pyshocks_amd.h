@@ -162,6 +162,22 @@ int psa_apply_boundary_neumann(int dtype, const void *u, void *out, int64_t m,
                                int g, double dx, double gl, double gr,
                                int sides, void *stream);
 
+/* The dissipative flux of the ES-WENO schemes (burgers/schemes.py:230-256,
+ * advection/schemes.py:162-186) added to fnum (m + 1) in place: at face j
+ * (cells j - 1, j) for 1 <= j <= m - 1, with w the ES-WENO weight of stencil
+ * `weight` (0 Burgers, 1 advection) of each cell of u (weno.py:284-296: zero
+ * padding at the array ends, tau zero in the end cells) and
+ * dw = w[j] - w[j - 1]: fnum[j] += -(sqrt(dw^2 + delta^2) / 8 + dw / 8)
+ * (u[j] - u[j - 1]). */
+int psa_esweno_dissipation(int dtype, const void *u, void *fnum, int64_t m,
+                           double eps, double delta, int weight, void *stream);
+
+/* schemes.py:466-473 and advection/schemes.py:62-73 from face fluxes:
+ * out[i] = -(fnum[i + 1] - fnum[i]) / dx for every cell, times velocity[i]
+ * when velocity (m) is not null. */
+int psa_flux_divergence(int dtype, const void *fnum, const void *velocity,
+                        void *out, int64_t m, double dx, void *stream);
+
 /* schemes.py:322-336 numerical_flux on u as given (no boundary applied):
  * fnum (m + 1) with fnum[0] = fnum[m] = 0. velocity (m) is required for
  * PSA_FLUX_ADVECTION_UPWIND; for PSA_FLUX_LAX_FRIEDRICHS it points to ONE

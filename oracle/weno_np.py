@@ -547,15 +547,32 @@ def advection_upwind_flux(grid: Grid, u: np.ndarray, velocity: np.ndarray, *,
 @dataclass(frozen=True)
 class Scheme:
     """A bound scheme of the hot path: ``kind`` is ``"rusanov"``,
-    ``"godunov"``, ``"lf"`` (Lax-Friedrichs) or ``"eo"`` (Engquist-Osher)
-    for Burgers, or ``"advection"`` (linear advection, Godunov); ``rec`` is
-    the reconstruction (``"wenojs53"``, ``"wenojs32"`` or ``"esweno32"``)."""
+    ``"godunov"``, ``"lf"`` (Lax-Friedrichs), ``"eo"`` (Engquist-Osher) or
+    ``"esweno32"`` (energy-stable WENO, with ``delta``) for Burgers, or
+    ``"advection"`` (linear advection, Godunov); ``rec`` is the
+    reconstruction (``"wenojs53"``, ``"wenojs32"`` or ``"esweno32"``)."""
 
     kind: str
     eps: float
     alpha: float = 1.0
     velocity: np.ndarray | None = None
     rec: str = "wenojs53"
+    delta: float = 0.0
+
+
+def es_weno_parameters(grid: Grid, u0: np.ndarray) -> tuple[float, float]:
+    """``(eps, delta)`` of ``weno.py:265-281``."""
+    i = grid.i_
+    eps = np.sum(grid.dx[i] * np.abs(u0[i])) * grid.dx_min**2
+    return float(eps), float(grid.dx_min**2)
+
+
+def esweno_dissipation(u: np.ndarray, omega: np.ndarray, delta: float) -> np.ndarray:
+    """The dissipative flux of ES-WENO from the weight ``omega`` of each cell
+    (``burgers/schemes.py:240-252``), padded to the faces."""
+    mu = np.sqrt((omega[1:] - omega[:-1]) ** 2 + delta**2) / 8.0
+    gnum = -(mu + (omega[1:] - omega[:-1]) / 8.0) * (u[1:] - u[:-1])
+    return np.pad(gnum, 1)
 
 
 def numerical_flux(scheme: Scheme, grid: Grid, u: np.ndarray) -> np.ndarray:
@@ -572,6 +589,12 @@ def numerical_flux(scheme: Scheme, grid: Grid, u: np.ndarray) -> np.ndarray:
         return scalar_flux_engquist_osher(grid, u, eps=scheme.eps, rec=rec)
     if scheme.kind == "godunov":
         return scalar_flux_upwind(grid, u, u, eps=scheme.eps, rec=rec)
+    if scheme.kind == "esweno32":
+        # burgers/schemes.py:230-256: the first weight of u
+        assert rec == "esweno32"
+        omega = es_weno_weights(WENOJS32, u, eps=scheme.eps)[0, :]
+        gnum = esweno_dissipation(u, omega, scheme.delta)
+        return scalar_flux_upwind(grid, u, u, eps=scheme.eps, rec=rec) + gnum
     if scheme.kind == "advection":
         assert scheme.velocity is not None
         return advection_upwind_flux(grid, u, scheme.velocity, eps=scheme.eps,
@@ -601,7 +624,7 @@ def predict_timestep(scheme: Scheme, grid: Grid, t: float, u: np.ndarray) -> np.
     if scheme.kind in ("rusanov", "lf"):
         smax = np.max(np.abs(u[grid.i_]))
         return 0.5 * grid.dx_min ** (2 - scheme.alpha) / smax
-    if scheme.kind in ("godunov", "eo"):
+    if scheme.kind in ("godunov", "eo", "esweno32"):
         smax = np.max(np.abs(u[grid.i_]))
         return 0.5 * grid.dx_min / smax
     if scheme.kind == "advection":

@@ -169,6 +169,8 @@ SIGNATURES: dict[str, list[Any]] = {
     "psa_reconstruct": [_I, _I, _VP, _VP, _VP, _I64, _D, _VP],
     "psa_muscl_reconstruct": [_I, _I, _D, _D, _VP, _VP, _VP, _I64, _VP],
     "psa_muscls_reconstruct": [_I, _I, _VP, _VP, _VP, _I64, _D, _VP],
+    "psa_esweno_dissipation": [_I, _VP, _VP, _I64, _D, _D, _I, _VP],
+    "psa_flux_divergence": [_I, _VP, _VP, _VP, _I64, _D, _VP],
     "psa_apply_boundary_periodic": [_I, _VP, _VP, _I64, _I, _VP],
     "psa_apply_boundary_dirichlet": [_I, _VP, _VP, _I64, _I, _VP, _VP, _VP],
     "psa_apply_boundary_neumann": [_I, _VP, _VP, _I64, _I, _D, _D, _D, _I, _VP],

@@ -4,15 +4,15 @@ Lax-Friedrichs and Engquist-Osher schemes.
 Mirrors ``burgers/schemes.py`` of pyshocks (``BurgersScheme`` ``:30-34``,
 ``flux`` ``:37-39``, ``predict_timestep`` ``:42-49``, ``FiniteVolumeScheme``
 ``:53-58``, ``Godunov`` ``:75-89``, ``Rusanov`` ``:99-127``,
-``LaxFriedrichs`` ``:136-153``, ``EngquistOsher`` ``:162-196``) and the
-registry of ``burgers/__init__.py:62-98``.
+``LaxFriedrichs`` ``:136-153``, ``EngquistOsher`` ``:162-196``, ``ESWENO32``
+``:202-256``) and the registry of ``burgers/__init__.py:62-98``.
 """
 
 from __future__ import annotations
 
 import ctypes
 
-from dataclasses import dataclass, field, fields
+from dataclasses import dataclass, field, fields, replace
 from typing import TYPE_CHECKING, Any
 
 import torch
@@ -24,6 +24,7 @@ from pyshocks_amd.schemes import (
     ConservationLawScheme,
     SchemeBase,
     apply_operator,
+    bind,
     flux,
     numerical_flux,
     predict_timestep,
@@ -210,6 +211,72 @@ def _apply_operator_ghosts(scheme: Any, grid: Grid, bc: Boundary, t: Any,
     return out
 
 
+# {{{ ESWENO32
+
+
+@dataclass(frozen=True)
+class ESWENO32(FiniteVolumeScheme):
+    """Energy-stable WENO (``burgers/schemes.py:202-256``): the upwind flux of
+    the ES-WENO3 reconstruction plus the dissipative flux of its first
+    weight; :func:`~pyshocks_amd.schemes.bind` sets the reconstruction's
+    ``eps`` and ``delta`` from the grid. Runs through the operator kernels
+    and the generic step loop, not the fused steps."""
+
+    def __post_init__(self) -> None:
+        from pyshocks_amd.reconstruction import ESWENO32 as ESWENO32Rec
+
+        if not isinstance(self.rec, ESWENO32Rec):
+            raise TypeError("ESWENO32 scheme requires the ESWENO32 reconstruction.")
+
+
+@bind.register(ESWENO32)
+def _bind_burgers_esweno32(scheme: ESWENO32, grid: Grid, bc: Boundary) -> ESWENO32:
+    """``burgers/schemes.py:219-227``: ``es_weno_parameters`` of a unit state."""
+    from pyshocks_amd.weno import es_weno_parameters
+
+    eps, delta = es_weno_parameters(grid, torch.ones_like(grid.x))
+    return replace(scheme, rec=replace(scheme.rec, eps=float(eps), delta=float(delta)))
+
+
+def esweno_dissipation(rec: Any, u: torch.Tensor, fnum: torch.Tensor,
+                       weight: int) -> torch.Tensor:
+    """``fnum`` plus the dissipative flux of the ES-WENO weight ``weight``
+    of ``u`` (``burgers/schemes.py:240-254``), in place."""
+    _lib.require_device(u, fnum)
+    _lib.call("psa_esweno_dissipation", _lib.dtype_code(u.dtype), _lib.ptr(u),
+              _lib.ptr(fnum), u.numel(), float(rec.eps), float(rec.delta), weight,
+              _lib.stream_of(u))
+    return fnum
+
+
+@numerical_flux.register(ESWENO32)
+def _numerical_flux_burgers_esweno32(scheme: ESWENO32, grid: Grid, bc: Boundary,
+                                     t: Any, u: torch.Tensor) -> torch.Tensor:
+    from pyshocks_amd.scalar import _eps_of, _numerical_flux
+
+    _lib.require_device(u)
+    fnum = _numerical_flux(_lib.PSA_FLUX_BURGERS_UPWIND, u, None, None, _eps_of(scheme),
+                           _lib.PSA_REC_ESWENO32)
+    return esweno_dissipation(scheme.rec, u, fnum, 0)
+
+
+@apply_operator.register(ESWENO32)
+def _apply_operator_burgers_esweno32(scheme: ESWENO32, grid: Grid, bc: Boundary,
+                                     t: Any, u: torch.Tensor) -> torch.Tensor:
+    """``schemes.py:466-473``: boundary, face fluxes, their difference."""
+    from pyshocks_amd.schemes import apply_boundary
+
+    ub = apply_boundary(bc, grid, t, u)
+    fnum = numerical_flux(scheme, grid, bc, t, ub)
+    out = torch.empty_like(u)
+    _lib.call("psa_flux_divergence", _lib.dtype_code(u.dtype), _lib.ptr(fnum), None,
+              _lib.ptr(out), u.numel(), grid.h, _lib.stream_of(u))
+    return out
+
+
+# }}}
+
+
 # {{{ registry
 
 _SCHEMES: dict[str, type[BurgersScheme]] = {
@@ -218,9 +285,10 @@ _SCHEMES: dict[str, type[BurgersScheme]] = {
     "rusanov": Rusanov,
     "lf": LaxFriedrichs,
     "eo": EngquistOsher,
+    "esweno32": ESWENO32,
 }
 
-_NOT_ON_GPU = ("esweno32", "ssmuscl", "ssweno242", "splitrusanov")
+_NOT_ON_GPU = ("ssmuscl", "ssweno242", "splitrusanov")
 
 
 def scheme_ids() -> tuple[str, ...]:

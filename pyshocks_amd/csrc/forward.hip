@@ -134,6 +134,43 @@ __global__ void k_muscl_reconstruct(const T *__restrict__ f, int64_t m,
   ur[i] = r;
 }
 
+// The ES-WENO weight of stencil K (0: d = 1/3 on cells i - 1, i; 1: d = 2/3
+// on i, i + 1) of cell i from the cells i - 1 .. i + 1 of weno.py:284-296
+// (zero outside the array; tau zero in the end cells).
+template <typename T, int K>
+__device__ __forceinline__ T es_weight(T um1, T u0, T up1, T eps, bool end) {
+  using C = Weno3Coeffs<T>;
+  const T t0 = u0 - um1;
+  const T t1 = up1 - u0;
+  const T tau = end ? T(0) : weno3_tau_ref(um1, u0, up1);
+  const T a0 = C::d0 * (T(1) + tau / (eps + t0 * t0));
+  const T a1 = C::d1 * (T(1) + tau / (eps + t1 * t1));
+  return (K == 0 ? a0 : a1) / (a0 + a1);
+}
+
+template <typename T, int K>
+__global__ void k_esweno_dissipation(const T *__restrict__ u, T *__restrict__ fnum,
+                                     int64_t m, T eps, T delta) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x + 1;
+  if (j >= m) return;
+  // cells j - 2 .. j + 1
+  const T v0 = load_raw(u, j - 2, m), v1 = u[j - 1], v2 = u[j], v3 = load_raw(u, j + 1, m);
+  const T wl = es_weight<T, K>(v0, v1, v2, eps, j - 1 == 0);
+  const T wr = es_weight<T, K>(v1, v2, v3, eps, j == m - 1);
+  const T dw = wr - wl;
+  const T mu = sqrt(dw * dw + delta * delta) / T(8);
+  fnum[j] = fnum[j] + (-(mu + dw / T(8)) * (v2 - v1));
+}
+
+template <typename T>
+__global__ void k_flux_divergence(const T *__restrict__ fnum, const T *__restrict__ vel,
+                                  T *__restrict__ out, int64_t m, T dx) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const T dF = fnum[i + 1] - fnum[i];
+  out[i] = vel != nullptr ? (-vel[i] * dF) / dx : -dF / dx;
+}
+
 // scalar.py:441-457: ghost cells of a side from the given values (interior
 // copied when out != u).
 template <typename T>
@@ -1457,6 +1494,60 @@ int psa_apply_boundary_periodic(int dtype, const void *u, void *out,
                        static_cast<float *>(out), m, n, g);
   }
   return check_launch("apply_boundary");
+}
+
+int psa_esweno_dissipation(int dtype, const void *u, void *fnum, int64_t m,
+                           double eps, double delta, int weight, void *stream) {
+  int st = check_dtype(dtype);
+  if (st != PSA_OK) return st;
+  if (u == nullptr || fnum == nullptr || m < 2)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer or m < 2");
+  if (weight != 0 && weight != 1)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "weight must be 0 or 1");
+  if (!(eps > 0.0) || !(delta >= 0.0))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "eps must be > 0 and delta >= 0");
+  auto s = static_cast<hipStream_t>(stream);
+  const unsigned int blocks = blocks_for(m - 1, kThreads);
+  auto launch = [&](auto tag) {
+    using T = decltype(tag);
+    const T *uu = static_cast<const T *>(u);
+    T *ff = static_cast<T *>(fnum);
+    if (weight == 0) {
+      hipLaunchKernelGGL((k_esweno_dissipation<T, 0>), dim3(blocks), dim3(kThreads), 0, s,
+                         uu, ff, m, static_cast<T>(eps), static_cast<T>(delta));
+    } else {
+      hipLaunchKernelGGL((k_esweno_dissipation<T, 1>), dim3(blocks), dim3(kThreads), 0, s,
+                         uu, ff, m, static_cast<T>(eps), static_cast<T>(delta));
+    }
+  };
+  if (dtype == PSA_FLOAT64) {
+    launch(0.0);
+  } else {
+    launch(0.0f);
+  }
+  return check_launch("esweno_dissipation");
+}
+
+int psa_flux_divergence(int dtype, const void *fnum, const void *velocity, void *out,
+                        int64_t m, double dx, void *stream) {
+  int st = check_dtype(dtype);
+  if (st != PSA_OK) return st;
+  if (fnum == nullptr || out == nullptr || m <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer or m <= 0");
+  if (!(dx > 0.0)) return set_error(PSA_ERR_INVALID_ARGUMENT, "dx must be positive");
+  auto s = static_cast<hipStream_t>(stream);
+  const unsigned int blocks = blocks_for(m, kThreads);
+  if (dtype == PSA_FLOAT64) {
+    hipLaunchKernelGGL(k_flux_divergence<double>, dim3(blocks), dim3(kThreads), 0, s,
+                       static_cast<const double *>(fnum),
+                       static_cast<const double *>(velocity), static_cast<double *>(out),
+                       m, dx);
+  } else {
+    hipLaunchKernelGGL(k_flux_divergence<float>, dim3(blocks), dim3(kThreads), 0, s,
+                       static_cast<const float *>(fnum), static_cast<const float *>(velocity),
+                       static_cast<float *>(out), m, static_cast<float>(dx));
+  }
+  return check_launch("flux_divergence");
 }
 
 int psa_apply_boundary_dirichlet(int dtype, const void *u, void *out,

@@ -117,8 +117,12 @@ def test_registries_and_errors() -> None:
     assert type(eo) is burgers.EngquistOsher and eo.omega == 0
     assert burgers.flux_kind(lf) == _lib.PSA_FLUX_LAX_FRIEDRICHS
     assert burgers.flux_kind(eo) == _lib.PSA_FLUX_ENGQUIST_OSHER
-    assert burgers.scheme_ids() == ("default", "godunov", "rusanov", "lf", "eo")
+    assert burgers.scheme_ids() == ("default", "godunov", "rusanov", "lf", "eo",
+                                    "esweno32")
     with pytest.raises(NotImplementedError):
+        burgers.make_scheme_from_name("ssweno242", rec=rec)
+    # burgers/schemes.py:214-216: ES-WENO needs its reconstruction
+    with pytest.raises(TypeError):
         burgers.make_scheme_from_name("esweno32", rec=rec)
     a = advection.make_scheme_from_name("godunov", rec=rec)
     assert a.velocity is None
