@@ -60,6 +60,11 @@ extern "C" {
 #define PSA_FLUX_ADVECTION_UPWIND 2
 #define PSA_FLUX_ENGQUIST_OSHER 3
 #define PSA_FLUX_LAX_FRIEDRICHS 4
+/* ES-WENO for Burgers (burgers/schemes.py:230-256): the upwind flux of the
+ * ES-WENO3 reconstruction plus the dissipative flux of its first weight
+ * (psa_esweno_dissipation); fused stages and steps only, with
+ * rec = PSA_REC_ESWENO32 and es_delta set. */
+#define PSA_FLUX_ESWENO 5
 
 /* reconstructions ----------------------------------------------------------
  * PSA_REC_WENOJS53  reconstruction.WENOJS53 (reconstruction.py:336-348,
@@ -317,6 +322,8 @@ typedef struct psa_solver_desc {
                                    Neumann fill done beforehand); the
                                    SSP-RK3 stages are periodic only        */
   int pad3_;
+  double es_delta;              /* PSA_FLUX_ESWENO: delta of the
+                                   dissipative flux                        */
 } psa_solver_desc;
 #define PSA_BC_PERIODIC 0
 #define PSA_BC_GIVEN 1

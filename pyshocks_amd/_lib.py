@@ -34,6 +34,7 @@ PSA_FLUX_BURGERS_UPWIND = 1
 PSA_FLUX_ADVECTION_UPWIND = 2
 PSA_FLUX_ENGQUIST_OSHER = 3
 PSA_FLUX_LAX_FRIEDRICHS = 4
+PSA_FLUX_ESWENO = 5
 
 PSA_REC_WENOJS53 = 0
 PSA_REC_WENOJS32 = 1
@@ -110,6 +111,7 @@ class PsaSolverDesc(ctypes.Structure):
         ("muscl_atol", ctypes.c_double),
         ("bc", ctypes.c_int),
         ("pad3_", ctypes.c_int),
+        ("es_delta", ctypes.c_double),
     ]
 
 

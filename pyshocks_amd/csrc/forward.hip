@@ -134,20 +134,6 @@ __global__ void k_muscl_reconstruct(const T *__restrict__ f, int64_t m,
   ur[i] = r;
 }
 
-// The ES-WENO weight of stencil K (0: d = 1/3 on cells i - 1, i; 1: d = 2/3
-// on i, i + 1) of cell i from the cells i - 1 .. i + 1 of weno.py:284-296
-// (zero outside the array; tau zero in the end cells).
-template <typename T, int K>
-__device__ __forceinline__ T es_weight(T um1, T u0, T up1, T eps, bool end) {
-  using C = Weno3Coeffs<T>;
-  const T t0 = u0 - um1;
-  const T t1 = up1 - u0;
-  const T tau = end ? T(0) : weno3_tau_ref(um1, u0, up1);
-  const T a0 = C::d0 * (T(1) + tau / (eps + t0 * t0));
-  const T a1 = C::d1 * (T(1) + tau / (eps + t1 * t1));
-  return (K == 0 ? a0 : a1) / (a0 + a1);
-}
-
 template <typename T, int K>
 __global__ void k_esweno_dissipation(const T *__restrict__ u, T *__restrict__ fnum,
                                      int64_t m, T eps, T delta) {
@@ -155,11 +141,8 @@ __global__ void k_esweno_dissipation(const T *__restrict__ u, T *__restrict__ fn
   if (j >= m) return;
   // cells j - 2 .. j + 1
   const T v0 = load_raw(u, j - 2, m), v1 = u[j - 1], v2 = u[j], v3 = load_raw(u, j + 1, m);
-  const T wl = es_weight<T, K>(v0, v1, v2, eps, j - 1 == 0);
-  const T wr = es_weight<T, K>(v1, v2, v3, eps, j == m - 1);
-  const T dw = wr - wl;
-  const T mu = sqrt(dw * dw + delta * delta) / T(8);
-  fnum[j] = fnum[j] + (-(mu + dw / T(8)) * (v2 - v1));
+  fnum[j] = fnum[j] + es_dissipation<T, K>(v0, v1, v2, v3, eps, delta, j - 1 == 0,
+                                           j == m - 1);
 }
 
 template <typename T>
@@ -911,6 +894,10 @@ int launch_step_rec(const psa_solver_desc *d, const StageArgs<T> &a, TileConfig 
     return launch_step_tiles<T, PSA_FLUX_BURGERS_UPWIND, 0, false, REC>(a, cfg, stream);
   if (d->flux == PSA_FLUX_ENGQUIST_OSHER)
     return launch_step_tiles<T, PSA_FLUX_ENGQUIST_OSHER, 0, false, REC>(a, cfg, stream);
+  if constexpr (REC == 1) {
+    if (d->flux == PSA_FLUX_ESWENO)
+      return launch_step_tiles<T, PSA_FLUX_ESWENO, 0, false, REC>(a, cfg, stream);
+  }
   return launch_step_tiles<T, PSA_FLUX_ADVECTION_UPWIND, 1, false, REC>(a, cfg, stream);
 }
 
@@ -979,6 +966,10 @@ int launch_stage_rec(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
   if (flux == PSA_FLUX_LAX_FRIEDRICHS)
     return launch_stage_tiles_rec<T, PSA_FLUX_LAX_FRIEDRICHS, 0, STAGE, REC>(a, cfg,
                                                                             stream);
+  if constexpr (REC == 1) {
+    if (flux == PSA_FLUX_ESWENO)
+      return launch_stage_tiles_rec<T, PSA_FLUX_ESWENO, 0, STAGE, REC>(a, cfg, stream);
+  }
   if (flux == PSA_FLUX_ADVECTION_UPWIND && vel == 1)
     return launch_stage_tiles_rec<T, PSA_FLUX_ADVECTION_UPWIND, 1, STAGE, REC>(
         a, cfg, stream);
@@ -1061,9 +1052,14 @@ int check_desc(const psa_solver_desc *d) {
       d->velocity == nullptr)
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "advection needs a velocity array or a constant");
-  if (d->flux < 0 || d->flux > 4)
+  if (d->flux < 0 || d->flux > PSA_FLUX_ESWENO)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "unsupported flux " + std::to_string(d->flux));
+  if (d->flux == PSA_FLUX_ESWENO &&
+      (d->rec != PSA_REC_ESWENO32 || !(d->eps > 0.0) || !(d->es_delta >= 0.0)))
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "the ES-WENO flux needs the ES-WENO3 reconstruction, eps > 0 "
+                     "and es_delta >= 0");
   if (d->mlp != nullptr && d->flux != PSA_FLUX_RUSANOV &&
       d->flux != PSA_FLUX_BURGERS_UPWIND)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
@@ -1110,6 +1106,7 @@ StageArgs<T> make_args(const psa_solver_desc *d) {
   a.rec3 = (d->rec == PSA_REC_WENOJS32 || d->rec == PSA_REC_ESWENO32) ? 1 : 0;
   a.muscl = d->rec == PSA_REC_MUSCL ? 1 : 0;
   a.muscls = d->rec == PSA_REC_MUSCLS ? 1 : 0;
+  a.es_delta = static_cast<T>(d->es_delta);
   a.nowrap = d->bc == PSA_BC_GIVEN ? 1 : 0;
   a.limiter = d->limiter;
   a.lim_theta = static_cast<T>(d->limiter_theta);

@@ -122,6 +122,7 @@ struct StageArgs {
                      //   reconstruction (REC = 1)
   int muscl;         // host side: the kernels take MUSCL (REC = 2)
   int muscls;        // host side: the slope-limited MUSCL (REC = 3)
+  T es_delta;        // PSA_FLUX_ESWENO: delta of the dissipative flux
   int nowrap;        // operator (STAGE 0): the ghost cells of uk as given
   int limiter;       // MUSCL: PSA_LIMITER_*
   T lim_theta;       // MUSCL: the minmod parameter
@@ -758,7 +759,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   const bool has_nu = (FLUX == PSA_FLUX_RUSANOV ||
                        FLUX == PSA_FLUX_LAX_FRIEDRICHS) && p.nu != nullptr;
   // the stage kernels' flux forms (stage_flux) where they apply
-  const bool plain = STAGE > 0 && VEL != 2 && !has_nu;
+  const bool plain = STAGE > 0 && VEL != 2 && !has_nu && FLUX != PSA_FLUX_ESWENO;
 #pragma unroll
   for (int k = 0; k <= CPT; ++k) {
     T urL = k == 0 ? sm.urL[tid] : ur[k - 1];
@@ -789,6 +790,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
       Fk = flux_engquist_osher(urL, ulR);
     } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
       Fk = flux_burgers_upwind(urL, ulR);
+    } else if constexpr (FLUX == PSA_FLUX_ESWENO) {
+      // cells gm - 2 .. gm + 1 are w[k] .. w[k + 3]
+      bool end_l = false, end_r = false;
+      if constexpr (EDGE) {
+        const int64_t gm = S + l0 + k;
+        end_l = gm - 1 == 0;
+        end_r = gm == m - 1;
+      }
+      Fk = flux_burgers_upwind(urL, ulR) +
+           es_dissipation<T, 0>(w[k], w[k + 1], w[k + 2], w[k + 3], p.eps, p.es_delta,
+                                end_l, end_r);
     } else {
       T arL, alR;
       if constexpr (VEL == 2) {
@@ -914,7 +926,13 @@ __device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
   for (int k = 0; k <= CPT; ++k) {
     const T urL = k == 0 ? sm.urL[tid] : ur[k - 1];
     const T ulR = k == CPT ? sm.ulF[tid + 1] : ul[k];
-    F[k] = stage_flux<T, FLUX>(urL, ulR, w[k + 1], w[k + 2], p.vconst);
+    if constexpr (FLUX == PSA_FLUX_ESWENO) {
+      F[k] = flux_burgers_upwind(urL, ulR) +
+             es_dissipation<T, 0>(w[k], w[k + 1], w[k + 2], w[k + 3], p.eps, p.es_delta,
+                                  false, false);
+    } else {
+      F[k] = stage_flux<T, FLUX>(urL, ulR, w[k + 1], w[k + 2], p.vconst);
+    }
   }
   const T lam = stage_lambda<T, FLUX>(p, dt);
 #pragma unroll

@@ -345,6 +345,36 @@ __device__ __forceinline__ T weno3_side_ref(T um1, T u0, T up1, T eps, bool es,
   return (a0 / s) * h0 + (a1 / s) * h1;
 }
 
+template <typename T>
+__device__ __forceinline__ T weno3_tau_ref(T um1, T u0, T up1);  // below
+
+// The ES-WENO weight of stencil K (0: d = 1/3 on cells i - 1, i; 1: d = 2/3
+// on i, i + 1) of cell i from the cells i - 1 .. i + 1 (weno.py:284-296;
+// end: tau zero, the array's first or last cell).
+template <typename T, int K>
+__device__ __forceinline__ T es_weight(T um1, T u0, T up1, T eps, bool end) {
+  using C = Weno3Coeffs<T>;
+  const T t0 = u0 - um1;
+  const T t1 = up1 - u0;
+  const T tau = end ? T(0) : weno3_tau_ref(um1, u0, up1);
+  const T a0 = C::d0 * (T(1) + tau / (eps + t0 * t0));
+  const T a1 = C::d1 * (T(1) + tau / (eps + t1 * t1));
+  return (K == 0 ? a0 : a1) / (a0 + a1);
+}
+
+// The dissipative flux of ES-WENO at the face between cells v1 and v2 from
+// the cells v0 .. v3 (burgers/schemes.py:240-252): with dw the difference of
+// the two cells' weights, -(sqrt(dw^2 + delta^2) / 8 + dw / 8) (v2 - v1).
+template <typename T, int K>
+__device__ __forceinline__ T es_dissipation(T v0, T v1, T v2, T v3, T eps, T delta,
+                                            bool end_l, bool end_r) {
+  const T wl = es_weight<T, K>(v0, v1, v2, eps, end_l);
+  const T wr = es_weight<T, K>(v1, v2, v3, eps, end_r);
+  const T dw = wr - wl;
+  const T mu = sqrt(dw * dw + delta * delta) / T(8);
+  return -(mu + dw / T(8)) * (v2 - v1);
+}
+
 // tau of ES-WENO for cell i (weno.py:293), (u_{i+1} - 2 u_i) + u_{i-1} squared.
 template <typename T>
 __device__ __forceinline__ T weno3_tau_ref(T um1, T u0, T up1) {

@@ -277,6 +277,9 @@ def _scheme_desc(scheme: Any, grid: Any) -> tuple[_lib.PsaSolverDesc, tuple[Any,
             keep = keep + (nu,)
     elif isinstance(scheme, burgers.Godunov):
         desc.flux = _lib.PSA_FLUX_BURGERS_UPWIND
+    elif isinstance(scheme, burgers.ESWENO32):
+        desc.flux = _lib.PSA_FLUX_ESWENO
+        desc.es_delta = float(scheme.rec.delta)
     elif isinstance(scheme, burgers.EngquistOsher):
         desc.flux = _lib.PSA_FLUX_ENGQUIST_OSHER
     elif isinstance(scheme, advection.Godunov):
@@ -329,7 +332,8 @@ def fused_plan(stepper: Stepper, *,
         if isinstance(scheme, burgers.Rusanov):
             # burgers/schemes.py:121-127, inherited by LaxFriedrichs
             cfl = npt(0.5 * dx_min ** npt(2 - scheme.alpha))
-        elif isinstance(scheme, (burgers.Godunov, burgers.EngquistOsher)):
+        elif isinstance(scheme, (burgers.Godunov, burgers.EngquistOsher,
+                                 burgers.ESWENO32)):
             # burgers/schemes.py:42-49
             cfl = npt(0.5 * dx_min)
         elif isinstance(scheme, advection.Godunov):

@@ -1,7 +1,9 @@
 """The ES-WENO scheme for Burgers (``burgers/schemes.py:202-256``) on the
 GPU against the oracle: the bound parameters, the dissipative flux of either
-weight, face fluxes and operators, and a solve through a shock by the
-generic step loop. FP64 to 1e-12 relative, FP32 to 1e-5 (l2) against FP64."""
+weight, face fluxes and operators, solves through a shock by the fused
+stages and steps (every schedule; the whole-step kernel equal to the stage
+kernels bit for bit) and by the generic step loop over the operator. FP64
+to 1e-12 relative, FP32 to 1e-5 (l2) against FP64."""
 
 from __future__ import annotations
 
@@ -106,15 +108,51 @@ def test_solve_through_shock(torch_mod, dtype_name: str, tfinal: float) -> None:
     osch = W.Scheme("esweno32", eps=sch.rec.eps, rec="esweno32", delta=sch.rec.delta)
     uref, dts, tref = W.solve(osch, ogrid, u0, theta=1.0, tfinal=tfinal)
     st = make_ssprk33(sch, grid, bc, theta=1.0)
-    assert fused_plan(st) is None
-    ev = evolve(st, torch.tensor(u0, device="cuda", dtype=dtype), tfinal=tfinal)
-    got = _host(ev.u)[2:-2]
-    rel = np.linalg.norm(got - uref[2:-2]) / np.linalg.norm(uref[2:-2])
-    if dtype == torch.float64:
-        assert ev.iteration == len(dts) and abs(float(ev.t) - tref) < 1e-12
-        assert rel < 1e-12
-    else:
-        assert rel < 1e-5
+    assert fused_plan(st).uses_step_kernel()
+    from pyshocks_amd import apply_operator, predict_timestep
+    from pyshocks_amd.timestepping import SSPRK33
+
+    # the generic loop over the operator kernels (a plain callable source)
+    generic = SSPRK33(
+        predict_timestep=lambda t, u: 1.0 * predict_timestep(sch, grid, bc, t, u),
+        source=lambda t, u: apply_operator(sch, grid, bc, t, u), checkpoint=None)
+    for stepper in (st, generic):
+        ev = evolve(stepper, torch.tensor(u0, device="cuda", dtype=dtype), tfinal=tfinal)
+        got = _host(ev.u)[2:-2]
+        rel = np.linalg.norm(got - uref[2:-2]) / np.linalg.norm(uref[2:-2])
+        if dtype == torch.float64:
+            assert ev.iteration == len(dts) and abs(float(ev.t) - tref) < 1e-12
+            assert rel < 1e-12
+        else:
+            assert rel < 1e-5
+
+
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+@pytest.mark.parametrize("n", [3, 1013, 100_003])
+def test_step_schedule_bit_for_bit(torch_mod, dtype_name: str, n: int) -> None:
+    torch = torch_mod
+    from oracle import exact as E
+    from pyshocks_amd import _lib
+    from pyshocks_amd.timestepping import fused_plan, make_ssprk33
+
+    dtype = getattr(torch, dtype_name)
+    sch, grid, bc = _bound(torch, n, dtype)
+    st = make_ssprk33(sch, grid, bc, theta=1.0)
+    avg = E.exact_cell_averages(E.large_initial_condition(seed=9), n, nghosts=2)
+    out = []
+    for sched in (_lib.PSA_SCHEDULE_TILE, _lib.PSA_SCHEDULE_STEP):
+        plan = fused_plan(st, tiles=(256, 4))
+        plan.desc.schedule = sched
+        u = torch.tensor(avg, device="cuda", dtype=dtype)
+        v = torch.empty_like(u)
+        k1, k2 = torch.empty_like(u), torch.empty_like(u)
+        ctrl = plan.new_control(u, 0.0, float(np.finfo(plan.np_dtype).max), None)
+        plan.steps(u, v, k1, k2, ctrl, 3)
+        c = plan.read_control(ctrl)
+        assert c.m == 3 and c.status == 0
+        out.append((v, c.t, c.dt))
+    assert out[0][1:] == out[1][1:]
+    assert torch.equal(out[0][0], out[1][0])
 
 
 def test_scheme_requires_es_reconstruction(torch_mod) -> None:
