@@ -65,7 +65,8 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--n", type=int, default=1 << 24)
     p.add_argument("--dtype", choices=["f64", "f32"], default="f64")
-    p.add_argument("--scheme", choices=["rusanov", "godunov", "lf", "eo", "advection"],
+    p.add_argument("--scheme", choices=["rusanov", "godunov", "lf", "eo", "esweno32",
+                                        "advection"],
                    default="rusanov")
     p.add_argument("--rec", choices=["wenojs53", "wenojs32", "esweno32", "muscl", "muscls"],
                    default="wenojs53", help="the reconstruction (WENO-JS5 for the metric)")
@@ -100,7 +101,11 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
                    help="only run a plain device copy of N FP64 values (N * 8 "
                         "bytes read and written per launch), the reference "
                         "for scaling the FETCH_SIZE / WRITE_SIZE counters")
-    return p.parse_args(argv)
+    args = p.parse_args(argv)
+    if args.scheme == "esweno32":
+        # burgers/schemes.py:214-216: the scheme takes its own reconstruction
+        args.rec = "esweno32"
+    return args
 
 
 def copy_calibration(args: argparse.Namespace) -> int:
@@ -142,6 +147,12 @@ def make_problem(args: argparse.Namespace):
         scheme, theta = LaxFriedrichs(rec=rec), 1.0
     elif args.scheme == "eo":
         scheme, theta = EngquistOsher(rec=rec), 1.0
+    elif args.scheme == "esweno32":
+        from pyshocks_amd import bind
+        from pyshocks_amd.burgers import ESWENO32
+
+        scheme = bind(ESWENO32(rec=rec), grid, PeriodicBoundary())
+        theta = 1.0
     else:
         vel = torch.ones(grid.n, dtype=dtype, device="cuda")
         scheme, theta = AdvGodunov(rec=rec, velocity=vel), 0.5
@@ -308,15 +319,21 @@ def roofline_record(args: argparse.Namespace, gpu: dict, traffic, valu) -> dict:
 
 def cpu_leg_numpy(args: argparse.Namespace, gpu: dict) -> dict:
     """The NumPy oracle (one host thread) on a bounded sample of the same
-    workload: the MUSCL step, which the C restatement does not have."""
+    workload: the MUSCL and ES-WENO steps, which the C restatement does not
+    have."""
+    import numpy as np
+
     from oracle import exact as E
     from oracle import weno_np as W
 
     n = args.n
     u0 = E.exact_cell_averages(E.large_initial_condition(seed=args.seed), n)
     grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
-    lm = args.limiter
-    sch = W.Scheme(args.scheme, eps=1e-6, rec=f"{args.rec}-{lm}-1.0")
+    if args.scheme == "esweno32":
+        eps, delta = W.es_weno_parameters(grid, np.ones_like(grid.x))
+        sch = W.Scheme("esweno32", eps=eps, rec="esweno32", delta=delta)
+    else:
+        sch = W.Scheme(args.scheme, eps=1e-6, rec=f"{args.rec}-{args.limiter}-1.0")
     t0 = time.perf_counter()
     u, _, _ = W.solve(sch, grid, u0, theta=gpu["theta"], maxit=1)
     one = time.perf_counter() - t0
@@ -339,7 +356,7 @@ def cpu_leg(args: argparse.Namespace, gpu: dict) -> dict | None:
     bounded sample of the same workload."""
     if args.cpu_seconds <= 0:
         return None
-    if args.rec in ("muscl", "muscls"):
+    if args.rec in ("muscl", "muscls") or args.scheme == "esweno32":
         return cpu_leg_numpy(args, gpu)
     import numpy as np
 

@@ -755,6 +755,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   tile_sync<ASYNC>();
 
   // ---- faces l0 - 1/2 .. l0 + CPT - 1/2
+  // ES-WENO: weight 0 of the cells w[1] .. w[CPT + 2] (cell l0 - 1 + r),
+  // each shared by two faces
+  T esw[FLUX == PSA_FLUX_ESWENO ? CPT + 2 : 1];
+  if constexpr (FLUX == PSA_FLUX_ESWENO) {
+#pragma unroll
+    for (int r = 0; r < CPT + 2; ++r) {
+      bool end = false;
+      if constexpr (EDGE) {
+        const int64_t gc = S + l0 - 1 + r;
+        end = gc == 0 || gc == m - 1;
+      }
+      esw[r] = es_weight0_fused(w[r], w[r + 1], w[r + 2], p.eps, end);
+    }
+  }
   T F[CPT + 1];
   const bool has_nu = (FLUX == PSA_FLUX_RUSANOV ||
                        FLUX == PSA_FLUX_LAX_FRIEDRICHS) && p.nu != nullptr;
@@ -791,16 +805,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
     } else if constexpr (FLUX == PSA_FLUX_BURGERS_UPWIND) {
       Fk = flux_burgers_upwind(urL, ulR);
     } else if constexpr (FLUX == PSA_FLUX_ESWENO) {
-      // cells gm - 2 .. gm + 1 are w[k] .. w[k + 3]
-      bool end_l = false, end_r = false;
-      if constexpr (EDGE) {
-        const int64_t gm = S + l0 + k;
-        end_l = gm - 1 == 0;
-        end_r = gm == m - 1;
-      }
+      // the face between cells w[k + 1] and w[k + 2]
       Fk = flux_burgers_upwind(urL, ulR) +
-           es_dissipation<T, 0>(w[k], w[k + 1], w[k + 2], w[k + 3], p.eps, p.es_delta,
-                                end_l, end_r);
+           es_dissipation_w(esw[k], esw[k + 1], w[k + 1], w[k + 2], p.es_delta);
     } else {
       T arL, alR;
       if constexpr (VEL == 2) {
@@ -921,6 +928,12 @@ __device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
   sm.ulF[tid] = ul[0];
   __syncthreads();
 
+  T esw[FLUX == PSA_FLUX_ESWENO ? CPT + 2 : 1];
+  if constexpr (FLUX == PSA_FLUX_ESWENO) {
+#pragma unroll
+    for (int r = 0; r < CPT + 2; ++r)
+      esw[r] = es_weight0_fused(w[r], w[r + 1], w[r + 2], p.eps, false);
+  }
   T F[CPT + 1];
 #pragma unroll
   for (int k = 0; k <= CPT; ++k) {
@@ -928,8 +941,7 @@ __device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
     const T ulR = k == CPT ? sm.ulF[tid + 1] : ul[k];
     if constexpr (FLUX == PSA_FLUX_ESWENO) {
       F[k] = flux_burgers_upwind(urL, ulR) +
-             es_dissipation<T, 0>(w[k], w[k + 1], w[k + 2], w[k + 3], p.eps, p.es_delta,
-                                  false, false);
+             es_dissipation_w(esw[k], esw[k + 1], w[k + 1], w[k + 2], p.es_delta);
     } else {
       F[k] = stage_flux<T, FLUX>(urL, ulR, w[k + 1], w[k + 2], p.vconst);
     }

@@ -362,6 +362,31 @@ __device__ __forceinline__ T es_weight(T um1, T u0, T up1, T eps, bool end) {
   return (K == 0 ? a0 : a1) / (a0 + a1);
 }
 
+// es_weight<T, 0> with one division: a_k = d_k (e_k + tau) / e_k with
+// e_k = eps + beta_k, so w_0 = d_0 (e_0 + tau) e_1 / (d_0 (e_0 + tau) e_1 +
+// d_1 (e_1 + tau) e_0) (the fused kernels; rounding differs from the
+// reference's order by a few ulps of the weight).
+template <typename T>
+__device__ __forceinline__ T es_weight0_fused(T um1, T u0, T up1, T eps, bool end) {
+  using C = Weno3Coeffs<T>;
+  const T t0 = u0 - um1;
+  const T t1 = up1 - u0;
+  const T tau = end ? T(0) : weno3_tau_ref(um1, u0, up1);
+  const T e0 = eps + t0 * t0;
+  const T e1 = eps + t1 * t1;
+  const T n0 = C::d0 * (e0 + tau) * e1;
+  const T n1 = C::d1 * (e1 + tau) * e0;
+  return n0 / (n0 + n1);
+}
+
+// The dissipative flux from the two cells' weights wl, wr and values v1, v2.
+template <typename T>
+__device__ __forceinline__ T es_dissipation_w(T wl, T wr, T v1, T v2, T delta) {
+  const T dw = wr - wl;
+  const T mu = sqrt(dw * dw + delta * delta) / T(8);
+  return -(mu + dw / T(8)) * (v2 - v1);
+}
+
 // The dissipative flux of ES-WENO at the face between cells v1 and v2 from
 // the cells v0 .. v3 (burgers/schemes.py:240-252): with dw the difference of
 // the two cells' weights, -(sqrt(dw^2 + delta^2) / 8 + dw / 8) (v2 - v1).
