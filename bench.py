@@ -647,8 +647,8 @@ def main(argv: list[str] | None = None) -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the workload is a single-GPU instance, see DESIGN.md for how "
-              "it would shard)", file=sys.stderr)
+              "(the workload is one GPU's instance of the periodic grid)",
+              file=sys.stderr)
         return 2
 
     gpu = gpu_leg(args)
