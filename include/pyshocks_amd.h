@@ -177,6 +177,17 @@ int psa_apply_boundary_neumann(int dtype, const void *u, void *out, int64_t m,
 int psa_esweno_dissipation(int dtype, const void *u, void *fnum, int64_t m,
                            double eps, double delta, int weight, void *stream);
 
+/* The ES-WENO scheme for linear advection (advection/schemes.py:135-186) on
+ * u as given: fnum (m + 1), zero at faces 0 and m, at face j
+ * 0.5 (fl[j] + fr[j - 1]) - 0.5 amax (ul[j] - ur[j - 1]) with (ul, ur) the
+ * ES-WENO3 states of u and (fl, fr) those of f = velocity u, amax ONE device
+ * value of the dtype (max|velocity| over the whole array), plus the
+ * dissipative flux of the second ES-WENO weight of u
+ * (psa_esweno_dissipation with weight 1). */
+int psa_esweno_advection_flux(int dtype, const void *u, const void *velocity,
+                              const void *amax, void *fnum, int64_t m, double eps,
+                              double delta, void *stream);
+
 /* schemes.py:466-473 and advection/schemes.py:62-73 from face fluxes:
  * out[i] = -(fnum[i + 1] - fnum[i]) / dx for every cell, times velocity[i]
  * when velocity (m) is not null. */

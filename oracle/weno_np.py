@@ -599,6 +599,16 @@ def numerical_flux(scheme: Scheme, grid: Grid, u: np.ndarray) -> np.ndarray:
         assert scheme.velocity is not None
         return advection_upwind_flux(grid, u, scheme.velocity, eps=scheme.eps,
                                      rec=rec)
+    if scheme.kind == "advection-esweno32":
+        # advection/schemes.py:135-186 with the ES-WENO3 reconstruction
+        assert scheme.velocity is not None and rec == "esweno32"
+        f = scheme.velocity * u
+        ul, ur = reconstruct(u, scheme.eps, rec, x=grid.x)
+        fl, fr = reconstruct(f, scheme.eps, rec, x=grid.x)
+        a = np.max(np.abs(scheme.velocity))
+        fnum = np.pad(0.5 * (fl[1:] + fr[:-1]) - 0.5 * a * (ul[1:] - ur[:-1]), 1)
+        omega1 = es_weno_weights(WENOJS32, u, eps=scheme.eps)[1, :]
+        return fnum + esweno_dissipation(u, omega1, scheme.delta)
 
     raise NotImplementedError(scheme.kind)
 
@@ -612,7 +622,7 @@ def apply_operator(scheme: Scheme, grid: Grid, t: float, u: np.ndarray,
     u = apply_boundary_periodic(grid, u) if bc is None else bc(grid, t, u)
     f = numerical_flux(scheme, grid, u)
 
-    if scheme.kind == "advection":
+    if scheme.kind in ("advection", "advection-esweno32"):
         return -scheme.velocity * (f[1:] - f[:-1]) / grid.dx
 
     return -(f[1:] - f[:-1]) / grid.dx
@@ -627,7 +637,7 @@ def predict_timestep(scheme: Scheme, grid: Grid, t: float, u: np.ndarray) -> np.
     if scheme.kind in ("godunov", "eo", "esweno32"):
         smax = np.max(np.abs(u[grid.i_]))
         return 0.5 * grid.dx_min / smax
-    if scheme.kind == "advection":
+    if scheme.kind in ("advection", "advection-esweno32"):
         amax = np.max(np.abs(scheme.velocity[grid.i_]))
         return grid.dx_min / amax
 

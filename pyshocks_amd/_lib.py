@@ -173,6 +173,7 @@ SIGNATURES: dict[str, list[Any]] = {
     "psa_muscls_reconstruct": [_I, _I, _VP, _VP, _VP, _I64, _D, _VP],
     "psa_esweno_dissipation": [_I, _VP, _VP, _I64, _D, _D, _I, _VP],
     "psa_flux_divergence": [_I, _VP, _VP, _VP, _I64, _D, _VP],
+    "psa_esweno_advection_flux": [_I, _VP, _VP, _VP, _VP, _I64, _D, _D, _VP],
     "psa_apply_boundary_periodic": [_I, _VP, _VP, _I64, _I, _VP],
     "psa_apply_boundary_dirichlet": [_I, _VP, _VP, _I64, _I, _VP, _VP, _VP],
     "psa_apply_boundary_neumann": [_I, _VP, _VP, _I64, _I, _D, _D, _D, _I, _VP],

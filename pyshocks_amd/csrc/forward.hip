@@ -280,6 +280,33 @@ __global__ void k_numerical_flux(const T *__restrict__ u,
   fnum[j] = F;
 }
 
+// advection/schemes.py:135-186 at face j (cells j - 2 .. j + 1).
+template <typename T>
+__global__ void k_esweno_advection_flux(const T *__restrict__ u, const T *__restrict__ vel,
+                                        const T *__restrict__ amax, int64_t m, T eps,
+                                        T delta, T *__restrict__ fnum) {
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j > m) return;
+  if (j == 0 || j == m) {
+    fnum[j] = T(0);
+    return;
+  }
+  T v[6], f[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    v[k] = load_raw(u, j - 3 + k, m);
+    f[k] = load_raw(vel, j - 3 + k, m) * v[k];
+  }
+  const LimParams<T> none{};
+  T urL, ulR, frL, flR;
+  face_states<T, PSA_REC_ESWENO32>(v, j, m, eps, urL, ulR, none);
+  face_states<T, PSA_REC_ESWENO32>(f, j, m, eps, frL, flR, none);
+  const T a = *amax;
+  const T F = T(0.5) * (flR + frL) - T(0.5) * a * (ulR - urL);
+  fnum[j] = F + es_dissipation<T, 1>(v[1], v[2], v[3], v[4], eps, delta, j - 1 == 0,
+                                     j == m - 1);
+}
+
 template <typename T>
 __global__ void k_fill_zero_bits(unsigned long long *bits) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *bits = 0ull;
@@ -1523,6 +1550,31 @@ int psa_esweno_dissipation(int dtype, const void *u, void *fnum, int64_t m,
     launch(0.0f);
   }
   return check_launch("esweno_dissipation");
+}
+
+int psa_esweno_advection_flux(int dtype, const void *u, const void *velocity,
+                              const void *amax, void *fnum, int64_t m, double eps,
+                              double delta, void *stream) {
+  int st = check_dtype(dtype);
+  if (st != PSA_OK) return st;
+  if (u == nullptr || velocity == nullptr || amax == nullptr || fnum == nullptr || m < 2)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer or m < 2");
+  if (!(eps > 0.0) || !(delta >= 0.0))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "eps must be > 0 and delta >= 0");
+  auto s = static_cast<hipStream_t>(stream);
+  const unsigned int blocks = blocks_for(m + 1, kThreads);
+  if (dtype == PSA_FLOAT64) {
+    hipLaunchKernelGGL(k_esweno_advection_flux<double>, dim3(blocks), dim3(kThreads), 0, s,
+                       static_cast<const double *>(u), static_cast<const double *>(velocity),
+                       static_cast<const double *>(amax), m, eps, delta,
+                       static_cast<double *>(fnum));
+  } else {
+    hipLaunchKernelGGL(k_esweno_advection_flux<float>, dim3(blocks), dim3(kThreads), 0, s,
+                       static_cast<const float *>(u), static_cast<const float *>(velocity),
+                       static_cast<const float *>(amax), m, static_cast<float>(eps),
+                       static_cast<float>(delta), static_cast<float *>(fnum));
+  }
+  return check_launch("esweno_advection_flux");
 }
 
 int psa_flux_divergence(int dtype, const void *fnum, const void *velocity, void *out,

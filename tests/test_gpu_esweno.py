@@ -163,3 +163,58 @@ def test_scheme_requires_es_reconstruction(torch_mod) -> None:
     with pytest.raises(TypeError):
         ESWENO32(rec=WENOJS32())
     assert isinstance(make_scheme_from_name("esweno32", rec=ESWENO32Rec()), ESWENO32)
+
+
+def _advection(torch, n: int, vel: np.ndarray):
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.advection import ESWENO32
+    from pyshocks_amd.reconstruction import ESWENO32 as ESWENO32Rec
+    from pyshocks_amd.scalar import PeriodicBoundary
+
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=2)
+    rec = ESWENO32Rec(eps=1.0 / n**2, delta=1.0 / n**2)
+    sch = ESWENO32(rec=rec, velocity=torch.tensor(vel, device="cuda"))
+    return sch, grid, PeriodicBoundary()
+
+
+@pytest.mark.parametrize("n", [7, 4096, 70001])
+def test_advection_flux_and_operator_vs_oracle(torch_mod, n: int) -> None:
+    """advection/schemes.py:135-186 with a variable velocity."""
+    torch = torch_mod
+    from oracle import weno_np as W
+    from pyshocks_amd import apply_operator, numerical_flux
+
+    x = (np.arange(n + 4) - 2 + 0.5) / n
+    vel = 0.5 + 0.25 * np.sin(2 * np.pi * x)
+    sch, grid, bc = _advection(torch, n, vel)
+    ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=2)
+    osch = W.Scheme("advection-esweno32", eps=sch.rec.eps, velocity=vel, rec="esweno32",
+                    delta=sch.rec.delta)
+    u = np.random.default_rng(n).uniform(-1, 1, size=n + 4)
+    ud = torch.tensor(u, device="cuda")
+    F = _host(numerical_flux(sch, grid, bc, 0.0, ud))
+    assert _relmax(F, W.numerical_flux(osch, ogrid, u)) < 1e-12
+    L = _host(apply_operator(sch, grid, bc, 0.0, ud))
+    assert _relmax(L[2:-2], W.apply_operator(osch, ogrid, 0.0, u)[2:-2]) < 1e-12
+
+
+def test_advection_solve_vs_oracle(torch_mod) -> None:
+    torch = torch_mod
+    from oracle import weno_np as W
+    from pyshocks_amd.timestepping import evolve, fused_plan, make_ssprk33
+
+    n = 512
+    x = (np.arange(n + 4) - 2 + 0.5) / n
+    vel = 0.5 + 0.25 * np.sin(2 * np.pi * x)
+    sch, grid, bc = _advection(torch, n, vel)
+    ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=2)
+    u0 = np.random.default_rng(2).uniform(-1, 1, size=n + 4)
+    osch = W.Scheme("advection-esweno32", eps=sch.rec.eps, velocity=vel, rec="esweno32",
+                    delta=sch.rec.delta)
+    uref, dts, _ = W.solve(osch, ogrid, u0, theta=0.5, maxit=40)
+    st = make_ssprk33(sch, grid, bc, theta=0.5)
+    assert fused_plan(st) is None
+    ev = evolve(st, torch.tensor(u0, device="cuda"), maxit=40)
+    got = _host(ev.u)[2:-2]
+    assert ev.iteration == len(dts) == 40
+    assert np.linalg.norm(got - uref[2:-2]) / np.linalg.norm(uref[2:-2]) < 1e-12
