@@ -57,6 +57,24 @@ def test_struct_layouts_and_errors() -> None:
     assert lib.psa_weno_reconstruct(7, None, None, None, 10, 1e-6, None) == \
         _lib.PSA_ERR_NOT_IMPLEMENTED
     assert "dtype" in _lib.last_error()
+    # the ES-WENO entry points validate before any launch
+    assert lib.psa_esweno_dissipation(0, None, None, 8, 1e-6, 1e-6, 0, None) == \
+        _lib.PSA_ERR_INVALID_ARGUMENT
+    assert lib.psa_esweno_dissipation(0, 1, 1, 8, 1e-6, 1e-6, 2, None) == \
+        _lib.PSA_ERR_INVALID_ARGUMENT
+    assert "weight" in _lib.last_error()
+    assert lib.psa_esweno_dissipation(0, 1, 1, 8, 0.0, 1e-6, 0, None) == \
+        _lib.PSA_ERR_INVALID_ARGUMENT
+    assert lib.psa_flux_divergence(0, 1, None, 1, 8, -1.0, None) == \
+        _lib.PSA_ERR_INVALID_ARGUMENT
+    assert lib.psa_esweno_advection_flux(0, 1, None, 1, 1, 8, 1e-6, 1e-6, None) == \
+        _lib.PSA_ERR_INVALID_ARGUMENT
+    desc_es = _lib.PsaSolverDesc()
+    desc_es.dtype, desc_es.flux, desc_es.n, desc_es.g, desc_es.dx = 0, 5, 16, 2, 1 / 16
+    desc_es.rec = _lib.PSA_REC_WENOJS32
+    with pytest.raises(ValueError, match="ES-WENO"):
+        _lib.check(lib.psa_ssprk33_steps(ctypes.byref(desc_es), None, None, None, None,
+                                         None, 1, None))
     # the Koren limiter has no slope form
     assert lib.psa_muscls_reconstruct(0, _lib.PSA_LIMITER_KOREN, None, None, None, 8,
                                       0.1, None) == _lib.PSA_ERR_NOT_IMPLEMENTED
