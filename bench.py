@@ -21,8 +21,9 @@ The other Burgers fluxes run the same workload with ``--scheme godunov``,
 ``--scheme eo`` (Engquist-Osher, whole-step kernel) or ``--scheme lf``
 (Lax-Friedrichs: the stage kernels, each reducing the global max|u| that the
 next stage's flux uses), ``--rec wenojs32|esweno32`` the third-order
-reconstructions, ``--rec muscl --limiter <name>`` MUSCL with a flux limiter
-and ``--rec muscls --limiter <name>`` MUSCL with a slope limiter.
+reconstructions, ``--rec muscl --limiter <name>`` MUSCL with a flux limiter,
+``--rec muscls --limiter <name>`` MUSCL with a slope limiter and ``--scheme
+esweno32`` the ES-WENO scheme (upwind flux plus its dissipation).
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 10]
 """
@@ -95,7 +96,8 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
     p.add_argument("--learned", action="store_true",
                    help="single workload with the shipped learned weights")
     p.add_argument("--team-cells", type=int, default=0,
-                   help="batched workload: cells per team member (0 = no teams)")
+                   help="batched workload: cells per team member (0: the solver's "
+                        "default, no teams for WENO-JS5 and 16384 for the others)")
     p.add_argument("--max-team", type=int, default=32)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain device copy of N FP64 values (N * 8 "
@@ -462,8 +464,9 @@ def batched_leg(args: argparse.Namespace) -> dict:
     N log-uniform in [128, 65536] (seeded), FP32, Rusanov, each to t = 0.3,
     one persistent launch replayed from a HIP graph, tiles autotuned; --rec
     other than WENO-JS5 runs with 2 ghost cells, tiles 256 x 4 or 256 x 2 and
-    (unless --team-cells says otherwise) teams of 16384 cells, which its
-    smaller tiles need to keep the largest instances off the critical path."""
+    (unless --team-cells says otherwise) the solver's teams of 16384 cells,
+    which its smaller tiles need to keep the largest instances off the
+    critical path."""
     import numpy as np
     import torch
 
@@ -481,8 +484,7 @@ def batched_leg(args: argparse.Namespace) -> dict:
     best, _ = autotune.tune_batch(ns[sample], [states[i] for i in sample],
                                   dtype=torch.float32, tfinal=0.01, rec=rec,
                                   candidates=cands)
-    team_cells = args.team_cells or (None if rec is None else 16384)
-    solver = BatchedSolver(ns, dtype=torch.float32, team_cells=team_cells,
+    solver = BatchedSolver(ns, dtype=torch.float32, team_cells=args.team_cells or None,
                            max_team=args.max_team, nghosts=g, rec=rec,
                            tiles=None if rec is None else best)
     solver.set_states(states)

@@ -53,7 +53,8 @@ class BatchedSolver:
         (:mod:`pyshocks_amd.reconstruction`; ``None``: WENO-JS5 with ``eps``);
         its own ``eps`` replaces the argument. The reconstructions other than
         WENO-JS5 take 2 ghost cells, tiles 256 x 4 or 256 x 2 and no learned
-        weights."""
+        weights; their smaller tiles leave a large instance many tiles per
+        step, so ``team_cells`` defaults to 16384 for them (``0``: no teams)."""
         if scheme not in _lib.BATCH_FLUX:
             raise NotImplementedError(
                 f"the batched solver takes Burgers 'rusanov', 'godunov' or 'eo', "
@@ -65,6 +66,8 @@ class BatchedSolver:
             rec_kind = rec_code(rec)
             eps = float(getattr(rec, "eps", eps))
         gmin = 3 if rec_kind == _lib.PSA_REC_WENOJS53 else 2
+        if team_cells is None and rec_kind != _lib.PSA_REC_WENOJS53:
+            team_cells = 16384
         if nghosts < gmin:
             raise ValueError(f"the reconstruction needs nghosts >= {gmin}")
         if rec_kind != _lib.PSA_REC_WENOJS53 and mlp is not None:
