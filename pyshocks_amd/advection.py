@@ -130,8 +130,10 @@ class ESWENO32(FiniteVolumeScheme):
     """Energy-stable WENO for advection (``advection/schemes.py:135-186``):
     a Lax-Friedrichs flux of the reconstructed ``u`` and ``f = velocity u``
     with the global speed ``max|velocity|``, plus the dissipative flux of the
-    second ES-WENO weight of ``u`` when the reconstruction is ES-WENO3. Runs
-    through the operator kernels and the generic step loop."""
+    second ES-WENO weight of ``u``. The GPU path takes the ES-WENO3
+    reconstruction only (the reference also accepts others, without the
+    dissipative term); it runs through the operator kernels and the generic
+    step loop."""
 
 
 @numerical_flux.register(ESWENO32)
