@@ -222,7 +222,9 @@ int psa_apply_operator(int dtype, int flux, const void *u,
                        int64_t m, int g, double dx, double eps, void *stream);
 
 /* psa_apply_operator with the reconstruction rec (PSA_REC_*); an advection
- * velocity array is reconstructed the same way. */
+ * velocity array is reconstructed the same way. The MUSCL reconstructions
+ * and PSA_FLUX_ESWENO take their parameters from a descriptor
+ * (psa_ssprk33_stage, stage 0) and are refused here. */
 int psa_apply_operator_rec(int dtype, int flux, int rec, const void *u,
                            const void *velocity, const void *nu, void *out,
                            int64_t m, int g, double dx, double eps,

@@ -1697,6 +1697,10 @@ int psa_apply_operator_rec(int dtype, int flux, int rec, const void *u,
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "MUSCL takes its limiter from a descriptor: use "
                      "psa_ssprk33_stage with stage 0");
+  if (flux == PSA_FLUX_ESWENO)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "the ES-WENO flux takes es_delta from a descriptor: use "
+                     "psa_ssprk33_stage with stage 0");
   psa_solver_desc d{};
   d.dtype = dtype;
   d.flux = flux;

@@ -69,6 +69,9 @@ def test_struct_layouts_and_errors() -> None:
         _lib.PSA_ERR_INVALID_ARGUMENT
     assert lib.psa_esweno_advection_flux(0, 1, None, 1, 1, 8, 1e-6, 1e-6, None) == \
         _lib.PSA_ERR_INVALID_ARGUMENT
+    assert lib.psa_apply_operator_rec(0, _lib.PSA_FLUX_ESWENO, _lib.PSA_REC_ESWENO32, 1,
+                                      None, None, 1, 16, 2, 0.1, 1e-6, None) == \
+        _lib.PSA_ERR_INVALID_ARGUMENT
     desc_es = _lib.PsaSolverDesc()
     desc_es.dtype, desc_es.flux, desc_es.n, desc_es.g, desc_es.dx = 0, 5, 16, 2, 1 / 16
     desc_es.rec = _lib.PSA_REC_WENOJS32
