@@ -1,12 +1,17 @@
 """Build ``libpyshocks_amd.so`` (the HIP kernels and the C ABI) for gfx950.
 
 Run as ``python -m pyshocks_amd.build``; ``__graft_entry__.build()`` calls
-:func:`build`. The library is written next to this file so that it is found
-in-tree (no install step).
+:func:`build`. Every ``csrc/*.hip`` file is compiled to an object of its own,
+concurrently, and the objects are linked into one shared library written next
+to this file, so that it is found in-tree (no install step).
+
+``python -m pyshocks_amd.build --incremental`` recompiles only the objects
+older than their source or any header of ``csrc/`` and ``include/``.
 """
 
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import pathlib
 import shutil
@@ -16,6 +21,7 @@ import sys
 ROOT = pathlib.Path(__file__).resolve().parent
 CSRC = ROOT / "csrc"
 INCLUDE = ROOT.parent / "include"
+OBJDIR = ROOT.parent / "_build"
 LIBNAME = "libpyshocks_amd.so"
 LIBPATH = ROOT / LIBNAME
 
@@ -26,7 +32,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # no packed FP32 instructions: hipcc would otherwise form them from
     # scalar float code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -58,16 +63,47 @@ def sources() -> list[pathlib.Path]:
     return sorted(CSRC.glob("*.hip"))
 
 
-def build(*, verbose: bool = True, extra_flags: list[str] | None = None) -> pathlib.Path:
-    """Compile every ``csrc/*.hip`` into ``pyshocks_amd/libpyshocks_amd.so``."""
+def _headers() -> list[pathlib.Path]:
+    return sorted([*CSRC.glob("*.hpp"), *INCLUDE.glob("*.h")])
+
+
+def _stale(obj: pathlib.Path, src: pathlib.Path, newest_header: float) -> bool:
+    if not obj.exists():
+        return True
+    t = obj.stat().st_mtime
+    return t < src.stat().st_mtime or t < newest_header
+
+
+def build(*, verbose: bool = True, extra_flags: list[str] | None = None,
+          incremental: bool = False, jobs: int | None = None) -> pathlib.Path:
+    """Compile every ``csrc/*.hip`` and link ``pyshocks_amd/libpyshocks_amd.so``."""
     hipcc = find_hipcc()
     srcs = sources()
     if not srcs:
         raise RuntimeError(f"no HIP sources in {CSRC}")
+    OBJDIR.mkdir(exist_ok=True)
+    flags = [*HIPCC_FLAGS, *(extra_flags or []), f"-I{INCLUDE}", f"-I{CSRC}"]
+    newest_header = max((h.stat().st_mtime for h in _headers()), default=0.0)
+
+    def compile_one(src: pathlib.Path) -> pathlib.Path:
+        obj = OBJDIR / (src.stem + ".o")
+        if incremental and not extra_flags and not _stale(obj, src, newest_header):
+            return obj
+        tmp = obj.with_suffix(".o.tmp")
+        cmd = [hipcc, *flags, "-c", str(src), "-o", str(tmp)]
+        if verbose:
+            print("[pyshocks_amd.build]", " ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, obj)
+        return obj
+
+    workers = jobs or min(len(srcs), max(1, os.cpu_count() or 1))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
+        objs = list(pool.map(compile_one, srcs))
 
     tmp = LIBPATH.with_suffix(".so.tmp")
-    cmd = [hipcc, *HIPCC_FLAGS, *(extra_flags or []), f"-I{INCLUDE}", f"-I{CSRC}",
-           *map(str, srcs), "-o", str(tmp)]
+    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs),
+           "-o", str(tmp)]
     if verbose:
         print("[pyshocks_amd.build]", " ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
@@ -76,4 +112,6 @@ def build(*, verbose: bool = True, extra_flags: list[str] | None = None) -> path
 
 
 if __name__ == "__main__":
-    build(extra_flags=sys.argv[1:])
+    args = sys.argv[1:]
+    inc = "--incremental" in args
+    build(extra_flags=[a for a in args if a != "--incremental"], incremental=inc)
