@@ -87,6 +87,11 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
                         "launch; adjoint: dJ/du0 at N = 16384, 2000 steps")
     p.add_argument("--no-adjoint", action="store_true",
                    help="skip the adjoint gradients/s figure of the default run")
+    p.add_argument("--no-c1", action="store_true",
+                   help="skip the small-workload time-to-solution figures (N = 256, "
+                        "t = 0.5) of the default run")
+    p.add_argument("--no-batched", action="store_true",
+                   help="skip the 4096-instance batched figure of the default run")
     p.add_argument("--no-learned", action="store_true",
                    help="skip the learned-vs-plain equal-error figure of the default run")
     p.add_argument("--ninst", type=int, default=4096)
@@ -319,6 +324,151 @@ def roofline_record(args: argparse.Namespace, gpu: dict, traffic, valu) -> dict:
     }
 
 
+def _cpu_model() -> str:
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def c1_leg(args: argparse.Namespace, repeats: int = 5) -> dict:
+    """The small plain workload: periodic Burgers on [0, 1], N = 256, FP64,
+    u0 = 0.5 + sin(2 pi x) (exact cell averages), WENO-JS5 (eps 1e-6) +
+    Rusanov, SSP-RK3 at Courant 0.5, to t = 0.5 (past the shock).
+
+    GPU: time-to-solution of one persistent workgroup that runs the whole
+    time loop (the batched solver with this one instance), of the
+    device-resident step loop (one psa_ssprk33_steps call of 400 steps from
+    the control-block set-up; the steps after t = 0.5 are no-ops) and of the
+    reference-style host loop (timestepping.step, one host sync per step),
+    best of ``repeats``, each final state checked against the NumPy oracle. CPU: the NumPy
+    restatement (one thread) and the C restatement on one core and on all
+    cores, each the whole solve."""
+    import numpy as np
+    import torch
+
+    from oracle import build as oracle_build
+    from oracle import exact as E
+    from oracle import ref_c
+    from oracle import weno_np as W
+    from pyshocks_amd import _lib, make_uniform_cell_grid
+    from pyshocks_amd.burgers import Rusanov
+    from pyshocks_amd.reconstruction import WENOJS53
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import fused_plan, make_ssprk33, step
+
+    _lib.load()
+    n, tfinal = 256, 0.5
+    u0 = E.exact_cell_averages(E.c1_initial_condition(), n)
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, device="cuda")
+    stepper = make_ssprk33(Rusanov(rec=WENOJS53(eps=1e-6)), grid, PeriodicBoundary(),
+                           theta=1.0)
+    plan = fused_plan(stepper)
+    ud = torch.tensor(u0, device="cuda")
+    out = torch.empty_like(ud)
+    k1, k2 = torch.empty_like(ud), torch.empty_like(ud)
+
+    def device_loop() -> tuple[float, object]:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctrl = plan.new_control(ud, 0.0, tfinal, None)
+        plan.steps(ud, out, k1, k2, ctrl, 400)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, ctrl
+
+    def host_loop() -> tuple[float, object]:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev = None
+        for ev in step(stepper, ud, tfinal=tfinal):
+            pass
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, ev
+
+    # one persistent workgroup running the whole time loop (the batched
+    # solver with one instance, time in the state dtype as the reference)
+    from pyshocks_amd.batched import BatchedSolver
+
+    pers = BatchedSolver([n], dtype=torch.float64, time_f64=False, tiles=(256, 2))
+    pers.set_states([u0])
+    p0 = pers.u.clone()
+
+    def persistent() -> float:
+        pers.u.copy_(p0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pers.solve(tfinal)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    persistent()
+    pers_s = min(persistent() for _ in range(repeats))
+    pers_u = pers.state(0).cpu().numpy()
+    pers_steps = int(pers.result().steps[0])
+
+    device_loop()
+    host_loop()
+    dev_s = min(device_loop()[0] for _ in range(repeats))
+    _, ctrl = device_loop()
+    c = plan.read_control(ctrl)
+    host_s = min(host_loop()[0] for _ in range(repeats))
+    _, ev = host_loop()
+
+    ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    t0 = time.perf_counter()
+    ref, dts, _ = W.solve(W.Scheme("rusanov", eps=1e-6), ogrid, u0, theta=1.0,
+                          tfinal=tfinal)
+    numpy_s = time.perf_counter() - t0
+    i = ogrid.i_
+    got = out.cpu().numpy()
+    rel = float(np.linalg.norm(got[i] - ref[i]) / np.linalg.norm(ref[i]))
+    rel_p = float(np.linalg.norm(pers_u[i] - ref[i]) / np.linalg.norm(ref[i]))
+    if c.m != len(dts) or ev.iteration != len(dts) or rel > 1e-12:
+        raise RuntimeError(f"C1 parity: steps {c.m}/{ev.iteration} vs {len(dts)}, "
+                           f"rel {rel:.3e}")
+    if pers_steps != len(dts) or rel_p > 1e-12:
+        raise RuntimeError(f"C1 parity (persistent): steps {pers_steps} vs {len(dts)}, "
+                           f"rel {rel_p:.3e}")
+
+    if not ref_c.LIB.exists():
+        oracle_build.build(verbose=False)
+    threads = ref_c.num_threads()
+    cpu = {}
+    for cores in (1, threads):
+        # 256 cells are too few for a parallel loop: every operator call then
+        # pays the thread team's synchronisation, and a busy host makes that
+        # slow; repeat only fast runs
+        ref_c.set_num_threads(cores)
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ref_c.solve("rusanov", u0, n=n, g=3, dx=1.0 / n, eps=1e-6, theta=1.0,
+                        cfl_num=0.5 / n, tfinal=tfinal)
+            best = min(best, time.perf_counter() - t0)
+            if best > 1.0:
+                break
+        cpu[cores] = best
+    ref_c.set_num_threads(threads)
+    cu = n * len(dts)
+    return {
+        "config": "periodic burgers rusanov WENO-JS5 SSP-RK3, N=256, f64, Courant 0.5, "
+                  "u0 = 0.5 + sin(2 pi x), t = 0.5",
+        "steps": len(dts),
+        "rel_l2_vs_oracle": rel,
+        "gpu_persistent_ms": pers_s * 1e3,
+        "gpu_device_loop_ms": dev_s * 1e3,
+        "gpu_host_loop_ms": host_s * 1e3,
+        "gpu_cell_updates_per_s": cu / min(pers_s, dev_s),
+        "cpu_numpy_1thread_ms": numpy_s * 1e3,
+        "cpu_c_1core_ms": cpu[1] * 1e3,
+        f"cpu_c_{threads}cores_ms": cpu[threads] * 1e3,
+        "cpu_c_cores": threads,
+    }
+
+
 def cpu_leg_numpy(args: argparse.Namespace, gpu: dict) -> dict:
     """The NumPy oracle (one host thread) on a bounded sample of the same
     workload: the MUSCL and ES-WENO steps, which the C restatement does not
@@ -386,6 +536,17 @@ def cpu_leg(args: argparse.Namespace, gpu: dict) -> dict | None:
     ref_c.solve(kind, u, n=n, g=3, dx=h, eps=1e-6, theta=gpu["theta"], cfl_num=cfl,
                 maxit=nsteps, velocity=vel, rec=args.rec)
     elapsed = time.perf_counter() - t0
+    # the same loop on one core, a few steps (~1/threads of the rate)
+    steps1 = int(max(1, min(5, 0.25 * args.cpu_seconds * nsteps / max(elapsed, 1e-9)
+                            / max(threads, 1))))
+    ref_c.set_num_threads(1)
+    try:
+        t0 = time.perf_counter()
+        ref_c.solve(kind, u, n=n, g=3, dx=h, eps=1e-6, theta=gpu["theta"], cfl_num=cfl,
+                    maxit=steps1, velocity=vel, rec=args.rec)
+        elapsed1 = time.perf_counter() - t0
+    finally:
+        ref_c.set_num_threads(threads)
     return {
         "value": n * nsteps / elapsed,
         "unit": "cell-updates/s",
@@ -394,6 +555,11 @@ def cpu_leg(args: argparse.Namespace, gpu: dict) -> dict | None:
         "sample": f"C/OpenMP restatement of the reference step loop, same workload "
                   f"(N={n}, {args.dtype if args.dtype == 'f64' else 'f64'}), "
                   f"{nsteps} SSP-RK3 steps after 1 sizing step, {elapsed:.1f} s",
+        "one_core": {"value": n * steps1 / elapsed1, "unit": "cell-updates/s",
+                     "cores": 1,
+                     "sample": f"the same loop on one core, {steps1} steps, "
+                               f"{elapsed1:.1f} s"},
+        "cpu_model": _cpu_model(),
     }
 
 
@@ -441,8 +607,38 @@ def adjoint_leg(args: argparse.Namespace, repeats: int = 3) -> dict:
     elapsed = time.perf_counter() - t0
     if not bool(torch.isfinite(g).all()):
         raise RuntimeError("non-finite gradient")
+
+    # a batch of independent gradients (instance b: u0 plus its own seeded
+    # perturbation, J_b against the same target), one launch per step for all
+    from pyshocks_amd.adjoint import BatchedCheckpointedSolve
+
+    batch = 32
+    pb = []
+    for b in range(batch):
+        rb = np.random.default_rng(1000 + b)
+        pb.append(funcs.cell_averages(funcs.SineStep(
+            c=0.0, amplitudes=tuple(0.001 * rb.uniform(-1, 1, 4)), wavenumbers=(1, 2, 3, 4),
+            phases=tuple(rb.uniform(0, 2 * math.pi, 4))), grid))
+    ub = torch.stack([u0 + d for d in pb])
+
+    def gradients() -> torch.Tensor:
+        solve = BatchedCheckpointedSolve(src, ub, nsteps, dt, every)
+        lam = (solve.u_final - target[None, :]) * grid.h
+        return solve.gradient(lam)
+
+    gb = gradients()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gb = gradients()
+    torch.cuda.synchronize()
+    elapsed_b = time.perf_counter() - t0
+    if not bool(torch.isfinite(gb).all()):
+        raise RuntimeError("non-finite batched gradient")
     return {"gradients_per_s": repeats / elapsed,
             "ms_per_gradient": elapsed / repeats * 1e3,
+            "batched": {"batch": batch, "gradients_per_s": batch / elapsed_b,
+                        "ms_per_batch": elapsed_b * 1e3,
+                        "note": "independent gradients (own u0) in one launch per step"},
             "config": f"burgers rusanov N={n} f64, {nsteps} steps fixed dt, "
                       f"checkpoints every {every}, J = 0.5||u(T)-u_target||^2 dx"}
 
@@ -535,20 +731,22 @@ def batched_cpu_leg(args: argparse.Namespace, ns, states) -> dict | None:
                       f"(FP64) each to t = 0.3, {elapsed:.1f} s"}
 
 
-def learned_leg(args: argparse.Namespace, *, ninst: int = 128, nmax: int = 8192,
-                full: bool = False, plain_single_value: float | None = None) -> dict:
+def learned_leg(args: argparse.Namespace, *, ninst: int = 1024, nmin: int = 1024,
+                nmax: int = 32768, scales: tuple[float, float] = (1.0, 1.5)) -> dict:
     """Learned-weight batched solver (shipped network) against the plain
-    WENO-JS5 FP64 batched solver at equal L1 error, on seeded sine-plus-step
-    Burgers instances with N log-uniform in [1024, nmax], t = 0.5, learned
-    state in FP64 and in FP32; both solvers with tiles tuned on 16 of the
-    instances.
+    WENO-JS5 FP64 batched solver at equal L1 error: ``ninst`` seeded
+    sine-plus-step Burgers instances with N log-uniform in [nmin, nmax],
+    t = 0.5, L1 errors against exact Hopf-Lax cell averages; every solver
+    with its own tiles (autotuned, spilling kernels skipped).
 
-    L1 errors are against exact Hopf-Lax cell averages. The plain path's
-    error-vs-N order p comes from the same instances at 2 N (the subset
-    N <= 8192 when ``full``); matching the learned error needs
-    N' = N r^(1/p) plain cells, r = mean E_plain / mean E_learned, i.e.
-    r^(2/p) times the cell-updates (cells and CFL steps both grow with N).
-    Effective throughput = the plain cell-updates needed / learned wall time."""
+    The plain solver's error-vs-N curve is measured on the same instances
+    with every N scaled by each of ``scales``; its order p fixes the scale
+    s* = (E_plain / E_learned)^(1/p) at which the plain solver should match
+    the learned error. The plain solver is then run at s* N (and once more
+    at a secant refinement if its error misses the learned one by more than
+    5 %), and the speedup at equal error is the plain wall time there over
+    the learned wall time; the effective throughput is the plain
+    cell-updates there per learned second."""
     import numpy as np
     import torch
 
@@ -557,60 +755,77 @@ def learned_leg(args: argparse.Namespace, *, ninst: int = 128, nmax: int = 8192,
 
     w = MLPWeights.load()
     rng = np.random.default_rng(5000 + args.seed)
-    ns = np.round(np.exp(rng.uniform(np.log(1024), np.log(nmax), ninst))).astype(np.int64)
+    ns = np.round(np.exp(rng.uniform(np.log(nmin), np.log(nmax), ninst))).astype(np.int64)
     ics = training_ics(ninst, seed=5000 + args.seed)
     u0s = exact.burgers_cell_averages(ics[:16], ns[:16], 0.0)
-    candidates = ((256, 4), (512, 2), (1024, 2))
-
-    # the plain baseline with its own tuned tiles
-    plain_tiles, _ = autotune.tune_batch(ns[:16], [u.cpu().numpy() for u in u0s],
-                                         dtype=torch.float64, tfinal=0.01,
-                                         candidates=candidates)
+    candidates = ((256, 4), (256, 2), (512, 2), (1024, 2))
     teams = args.team_cells or None
-    e_p, info_p = l1_errors(ics, ns, 0.5, dtype=torch.float64, tiles=plain_tiles,
+
+    def tune(dtype, mlp):
+        tiles, _ = autotune.tune_batch(ns[:16], [u.cpu().numpy() for u in u0s], dtype=dtype,
+                                       tfinal=0.01, mlp=mlp, candidates=candidates)
+        return tiles
+
+    plain_tiles = tune(torch.float64, None)
+
+    def plain(scale: float):
+        nsc = np.maximum(np.round(ns * scale).astype(np.int64), 8)
+        e, info = l1_errors(ics, nsc, 0.5, dtype=torch.float64, tiles=plain_tiles,
                             team_cells=teams)
-    sub = ns <= 8192
-    e_p2, _ = l1_errors([ic for ic, k in zip(ics, sub) if k], 2 * ns[sub], 0.5,
-                        dtype=torch.float64, tiles=plain_tiles, team_cells=teams)
-    p = float(np.log2(e_p[sub].mean() / e_p2.mean()))
-    plain_batched = info_p["cell_updates"] / info_p["seconds"]
+        return {"scale": scale, "l1": float(e.mean()), "seconds": info["seconds"],
+                "cell_updates": info["cell_updates"]}
+
+    curve = [plain(sc) for sc in scales]
+    (s1, e1), (s2, e2) = (curve[0]["scale"], curve[0]["l1"]), (curve[1]["scale"], curve[1]["l1"])
+    p = float(np.log(e1 / e2) / np.log(s2 / s1))
     out = {
         "config": f"{ninst} seeded sine+step Burgers instances, N log-uniform "
-                  f"[1024, {nmax}], t = 0.5, learned batched (tau {w.tau}, sigma {w.sigma}) vs "
-                  f"plain WENO-JS5 FP64 batched",
-        "l1_plain_f64": float(e_p.mean()),
-        "plain_order": p,
-        "plain_f64_batched_cell_updates_per_s": plain_batched,
-        "plain_f64_seconds": info_p["seconds"],
+                  f"[{nmin}, {nmax}], t = 0.5, learned batched (tau {w.tau}, sigma {w.sigma}) "
+                  f"vs plain WENO-JS5 FP64 batched, L1 error vs exact Hopf-Lax averages",
         "plain_tiles": list(plain_tiles),
         "team_cells": teams,
+        "plain_curve": curve,
+        "plain_order": p,
     }
     best = None
     for name, dtype in (("f64", torch.float64), ("f32", torch.float32)):
-        tiles, _ = autotune.tune_batch(ns[:16], [u.cpu().numpy() for u in u0s], dtype=dtype,
-                                       tfinal=0.01, mlp=w, candidates=candidates)
+        tiles = tune(dtype, w)
         e_l, info_l = l1_errors(ics, ns, 0.5, mlp=w, dtype=dtype, tiles=tiles,
                                 team_cells=teams)
-        r = float(e_p.mean() / e_l.mean())
-        scale = r ** (2.0 / p) if p > 0 else float("nan")
-        cu_plain_equal = info_p["cell_updates"] * scale
-        eff = cu_plain_equal / info_l["seconds"]
-        rec = {
-            "tiles": list(tiles),
-            "l1": float(e_l.mean()),
-            "error_ratio": r,
-            "cell_updates_per_s": info_l["cell_updates"] / info_l["seconds"],
-            "seconds": info_l["seconds"],
-            "effective_cell_updates_per_s": eff,
-            "speedup_vs_plain_f64_batched": eff / plain_batched,
-        }
-        if plain_single_value:
-            rec["speedup_vs_plain_f64_single"] = eff / plain_single_value
+        el = float(e_l.mean())
+        rec = {"tiles": list(tiles), "l1": el, "error_ratio": e1 / el,
+               "seconds": info_l["seconds"],
+               "cell_updates_per_s": info_l["cell_updates"] / info_l["seconds"]}
         out[f"learned_{name}"] = rec
-        if best is None or eff > best[1]:
-            best = (name, eff)
-    out["best"] = best[0]
-    out["effective_cell_updates_per_s"] = best[1]
+        if best is None or el < best[1]:
+            best = (name, el, info_l["seconds"])
+
+    # the plain solver at the scale that matches the better learned error
+    name, el, tl = best
+    points = sorted([(c["scale"], c["l1"]) for c in curve])
+    s_star = s1 * (e1 / el) ** (1.0 / p) if p > 0 else float("nan")
+    matched = []
+    if np.isfinite(s_star) and 1.0 <= s_star <= 4.0:
+        m = plain(s_star)
+        matched.append(m)
+        if abs(m["l1"] / el - 1.0) > 0.05:
+            # secant in log-log through the measured point nearest in error
+            pts = sorted(points + [(m["scale"], m["l1"])], key=lambda q: abs(np.log(q[1] / el)))
+            (sa, ea), (sb, eb) = pts[0], pts[1]
+            q = float(np.log(ea / eb) / np.log(sb / sa)) if sa != sb else p
+            s_next = sa * (ea / el) ** (1.0 / q) if q > 0 else float("nan")
+            if np.isfinite(s_next) and 1.0 <= s_next <= 4.0:
+                matched.append(plain(s_next))
+    out["plain_matched"] = matched
+    if matched:
+        m = min(matched, key=lambda c: abs(np.log(c["l1"] / el)))
+        speedup = m["seconds"] / tl
+        out.update({"best": name, "speedup_at_equal_error": speedup,
+                    "effective_cell_updates_per_s": m["cell_updates"] / tl,
+                    "matched_error_gap": m["l1"] / el - 1.0})
+    else:
+        out.update({"best": name, "speedup_at_equal_error": None,
+                    "effective_cell_updates_per_s": None})
     return out
 
 
@@ -634,7 +849,7 @@ def main(argv: list[str] | None = None) -> int:
                           "batched": b, "cpu_baseline": cpu}), flush=True)
         return 0
     if args.workload == "learned":
-        lr = learned_leg(args, ninst=1024, nmax=32768, full=True)
+        lr = learned_leg(args)
         print(json.dumps({"metric": METRIC, "value": lr["effective_cell_updates_per_s"],
                           "unit": "cell-updates/s (plain FP64 cell-updates at equal L1 "
                                   "error per learned second)",
@@ -653,10 +868,17 @@ def main(argv: list[str] | None = None) -> int:
               file=sys.stderr)
         return 2
 
-    gpu = gpu_leg(args)
+    # the small workload and the adjoint first: the timed steps of the large
+    # workload then start on a GPU at its running clocks
+    c1 = None if args.no_c1 else c1_leg(args)
     adjoint = None if args.no_adjoint else adjoint_leg(args)
-    learned = None if args.no_learned else learned_leg(
-        args, plain_single_value=gpu["value"] if args.dtype == "f64" else None)
+    gpu = gpu_leg(args)
+    batched = batched_cpu = None
+    if not args.no_batched:
+        b = batched_leg(args)
+        batched_cpu = batched_cpu_leg(args, b.pop("ns"), b.pop("states"))
+        batched = b
+    learned = None if args.no_learned else learned_leg(args)
     cpu = cpu_leg(args, gpu)
 
     kname = "step" if gpu["whole_step"] else f"stage{gpu['stage_dominant']}"
@@ -696,6 +918,9 @@ def main(argv: list[str] | None = None) -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "c1": c1,
+        "batched": batched,
+        "batched_cpu_baseline": batched_cpu,
         "adjoint": adjoint,
         "learned": learned,
     }

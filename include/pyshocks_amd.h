@@ -482,6 +482,15 @@ int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1, void *k2,
                     unsigned int *counter, double *t_out, long long *steps_out,
                     int *status_out, void *stream);
 
+/* Resources of the kernel psa_batch_solve launches for desc (its dtype,
+ * flux, reconstruction, learned weights, schedule, block and
+ * cells_per_thread; the buffers are not read): registers per lane, private
+ * (spill) bytes per lane and static LDS bytes per workgroup. Launches
+ * nothing; the tile autotuner uses it to skip candidates that spill. Part
+ * of the batched front end, which the reference does not have. */
+int psa_batch_kernel_info(const psa_batch_desc *desc, int *vgprs,
+                          long long *private_bytes, long long *lds_bytes);
+
 /* Discrete adjoint --------------------------------------------------------
  * Replaces the dense jax.jacfwd Jacobian of timestepping.py:155-215
  * (adjoint_step) with matrix-free vector-Jacobian products of the fused
@@ -525,6 +534,28 @@ int psa_ssprk33_adjoint(const psa_solver_desc *desc, long long nsteps,
                         const void *dt, int every, const void *lam_final,
                         void *grad_u0, void *workspace, size_t workspace_bytes,
                         void *stream);
+
+/* The same for a batch of `batch` instances of the descriptor's grid, each
+ * with its own initial state, step size and final cotangent: instance b's
+ * arrays start at element b * ld of u0, u_final, lam_final and grad_u0
+ * (ld >= m, ld * element size a multiple of 16 bytes), dt is a device array
+ * of `batch` step sizes. Every step is one launch for the whole batch (the
+ * whole-step kernel forward, the fused reverse step backwards), so many
+ * gradients share each launch's latency; PSA_ERR_NOT_IMPLEMENTED where the
+ * whole-step kernel does not apply. batch = 1 with ld = m is the single
+ * form above (timestepping.py:155-215 applied to many instances). */
+int psa_adjoint_workspace_bytes_batch(const psa_solver_desc *desc, long long batch,
+                                      long long ld, long long nsteps, int every,
+                                      size_t *bytes);
+int psa_ssprk33_forward_checkpointed_batch(const psa_solver_desc *desc, long long batch,
+                                           long long ld, const void *u0, void *u_final,
+                                           long long nsteps, const void *dt, int every,
+                                           void *workspace, size_t workspace_bytes,
+                                           void *stream);
+int psa_ssprk33_adjoint_batch(const psa_solver_desc *desc, long long batch, long long ld,
+                              long long nsteps, const void *dt, int every,
+                              const void *lam_final, void *grad_u0, void *workspace,
+                              size_t workspace_bytes, void *stream);
 
 /* Learned WENO weights ----------------------------------------------------
  * No counterpart in the reference, which has only the fixed WENO-JS weights

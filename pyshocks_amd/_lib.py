@@ -197,6 +197,8 @@ SIGNATURES: dict[str, list[Any]] = {
                           _VP, _VP],
     "psa_batch_solve": [ctypes.POINTER(PsaBatchDesc), _VP, _VP, _VP, _VP, _VP, _VP,
                         _VP, _VP],
+    "psa_batch_kernel_info": [ctypes.POINTER(PsaBatchDesc), ctypes.POINTER(_I),
+                              ctypes.POINTER(_LL), ctypes.POINTER(_LL)],
     "psa_ssprk33_stage_vjp": [ctypes.POINTER(PsaSolverDesc), _I, _VP, _VP, _VP,
                               _VP, _VP, _VP, _VP],
     "psa_ssprk33_step_vjp": [ctypes.POINTER(PsaSolverDesc), _VP, _VP, _VP, _VP,
@@ -207,6 +209,13 @@ SIGNATURES: dict[str, list[Any]] = {
                                          _LL, _VP, _I, _VP, ctypes.c_size_t, _VP],
     "psa_ssprk33_adjoint": [ctypes.POINTER(PsaSolverDesc), _LL, _VP, _I, _VP, _VP,
                             _VP, ctypes.c_size_t, _VP],
+    "psa_adjoint_workspace_bytes_batch": [ctypes.POINTER(PsaSolverDesc), _LL, _LL, _LL,
+                                          _I, ctypes.POINTER(ctypes.c_size_t)],
+    "psa_ssprk33_forward_checkpointed_batch": [ctypes.POINTER(PsaSolverDesc), _LL, _LL,
+                                               _VP, _VP, _LL, _VP, _I, _VP,
+                                               ctypes.c_size_t, _VP],
+    "psa_ssprk33_adjoint_batch": [ctypes.POINTER(PsaSolverDesc), _LL, _LL, _LL, _VP, _I,
+                                  _VP, _VP, _VP, ctypes.c_size_t, _VP],
     "psa_mlp_weights": [_VP, _VP, _LL, _VP, _VP, _VP],
     "psa_mlp_grad_workspace_bytes": [_LL, ctypes.POINTER(ctypes.c_size_t)],
     "psa_mlp_grad": [_VP, _VP, _LL, _D, _D, _VP, _VP, ctypes.c_size_t, _VP],

@@ -9,7 +9,9 @@ library's reverse stage kernels:
   :func:`~pyshocks_amd.timestepping.adjoint_step`);
 * :func:`checkpointed_solve` / :class:`SSPRK33Solve` -- a fixed-dt solve of
   ``nsteps`` steps whose backward pass recomputes windows of ``every`` steps
-  from checkpoints and sweeps them in reverse, exposed to torch autograd.
+  from checkpoints and sweeps them in reverse, exposed to torch autograd;
+* :class:`BatchedCheckpointedSolve` -- the same for many instances of one
+  grid, one launch per step for all of them.
 
 Cotangents use the grid layout (ghost entries are ignored on input and are
 zero on output).
@@ -109,6 +111,62 @@ class CheckpointedSolve:
 def checkpointed_solve(source: Any, u0: torch.Tensor, nsteps: int, dt: Any,
                        every: int = 50) -> CheckpointedSolve:
     return CheckpointedSolve(source, u0, nsteps, dt, every)
+
+
+class BatchedCheckpointedSolve:
+    """:class:`CheckpointedSolve` for ``B`` instances of one grid at once:
+    ``u0`` of shape ``(B, m)``, ``dt`` a scalar or one step size per
+    instance. Each step of the forward, recompute and reverse passes is one
+    launch for the whole batch, so the gradients share its latency."""
+
+    def __init__(self, source: Any, u0: torch.Tensor, nsteps: int, dt: Any,
+                 every: int = 50) -> None:
+        self.desc = _plan_of(source)
+        _lib.require_device(u0)
+        if u0.ndim != 2:
+            raise ValueError("u0 must have shape (batch, m)")
+        self.batch, self.m = int(u0.shape[0]), int(u0.shape[1])
+        # rows padded to a multiple of 32 cells (16-byte aligned in either dtype)
+        self.ld = ((self.m + 31) // 32) * 32
+        self.nsteps = int(nsteps)
+        self.every = int(every)
+        self.dt = torch.as_tensor(dt, dtype=u0.dtype, device=u0.device).expand(
+            self.batch).contiguous()
+        nbytes = ctypes.c_size_t(0)
+        _lib.call("psa_adjoint_workspace_bytes_batch", ctypes.byref(self.desc),
+                  self.batch, self.ld, self.nsteps, self.every, ctypes.byref(nbytes))
+        self.workspace = torch.empty(nbytes.value, dtype=torch.uint8, device=u0.device)
+        u0p = self._padded(u0)
+        self._u_final = torch.empty_like(u0p)
+        _lib.call("psa_ssprk33_forward_checkpointed_batch", ctypes.byref(self.desc),
+                  self.batch, self.ld, _lib.ptr(u0p), _lib.ptr(self._u_final),
+                  self.nsteps, _lib.ptr(self.dt), self.every, _lib.ptr(self.workspace),
+                  self.workspace.numel(), _lib.stream_of(u0))
+
+    def _padded(self, a: torch.Tensor) -> torch.Tensor:
+        if tuple(a.shape) != (self.batch, self.m):
+            raise ValueError(f"expected shape ({self.batch}, {self.m})")
+        out = torch.zeros((self.batch, self.ld), dtype=a.dtype, device=a.device)
+        out[:, : self.m] = a
+        return out
+
+    @property
+    def u_final(self) -> torch.Tensor:
+        """The final states, shape ``(B, m)`` (a view)."""
+        return self._u_final[:, : self.m]
+
+    def gradient(self, lam_final: torch.Tensor) -> torch.Tensor:
+        """``(d u_final_b / d u0_b)^T lam_final_b`` for every instance b,
+        shape ``(B, m)``."""
+        if not lam_final.is_cuda:
+            raise ValueError("lam_final must be on the GPU")
+        lam = self._padded(lam_final)
+        grad = torch.empty_like(lam)
+        _lib.call("psa_ssprk33_adjoint_batch", ctypes.byref(self.desc), self.batch,
+                  self.ld, self.nsteps, _lib.ptr(self.dt), self.every, _lib.ptr(lam),
+                  _lib.ptr(grad), _lib.ptr(self.workspace), self.workspace.numel(),
+                  _lib.stream_of(lam))
+        return grad[:, : self.m]
 
 
 class SSPRK33Solve(torch.autograd.Function):

@@ -5,7 +5,8 @@ winner is cached per (dtype, flux, velocity mode, reconstruction, size class) fo
 kernels and per (dtype, flux) for the batched solver, and used by
 :func:`~pyshocks_amd.timestepping.fused_plan` and
 :class:`~pyshocks_amd.batched.BatchedSolver` when no tiles are given. A
-candidate the device cannot launch is skipped.
+candidate the device cannot launch is skipped, and so is a batched kernel
+that spills registers to private memory.
 """
 
 from __future__ import annotations
@@ -81,7 +82,8 @@ def tune_batch(ncells: Sequence[int], states: Sequence[np.ndarray], *,
     """Time the batched solve of a sample of instances (to a short final
     time) for each candidate and cache the fastest; with a reconstruction
     other than WENO-JS5 (``rec``, states with 2 ghost cells) the winner is
-    returned but not cached."""
+    returned but not cached. A candidate whose kernel spills registers to
+    private memory is skipped without being timed."""
     from pyshocks_amd import _lib
     from pyshocks_amd.batched import BatchedSolver
 
@@ -91,6 +93,8 @@ def tune_batch(ncells: Sequence[int], states: Sequence[np.ndarray], *,
             solver = BatchedSolver(ncells, dtype=dtype, scheme=scheme, eps=eps,
                                    theta=theta, tiles=cand, mlp=mlp, rec=rec,
                                    nghosts=3 if rec is None else 2)
+            if solver.kernel_info()["private_bytes"] > 0:
+                continue
             solver.set_states(states)
             u0 = solver.u.clone()
             solver.solve(tfinal)  # warm up

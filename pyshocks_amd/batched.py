@@ -54,7 +54,9 @@ class BatchedSolver:
         its own ``eps`` replaces the argument. The reconstructions other than
         WENO-JS5 take 2 ghost cells, tiles 256 x 4 or 256 x 2 and no learned
         weights; their smaller tiles leave a large instance many tiles per
-        step, so ``team_cells`` defaults to 16384 for them (``0``: no teams)."""
+        step, so ``team_cells`` defaults to 16384 for them (``0``: no teams).
+        A team's workgroups must all be resident, so with an explicit
+        ``grid`` no team is larger than ``grid`` workgroups."""
         if scheme not in _lib.BATCH_FLUX:
             raise NotImplementedError(
                 f"the batched solver takes Burgers 'rusanov', 'godunov' or 'eo', "
@@ -157,8 +159,10 @@ class BatchedSolver:
         tile = (self.desc.block or 256) * (self.desc.cells_per_thread or 4)
         ntiles = (self.m + tile - 1) // tile
         if team_cells:
-            team = np.clip((self.m + team_cells - 1) // team_cells, 1,
-                           np.minimum(max_team, ntiles)).astype(np.int32)
+            cap = np.minimum(max_team, ntiles)
+            if self.desc.grid > 0:
+                cap = np.minimum(cap, self.desc.grid)
+            team = np.clip((self.m + team_cells - 1) // team_cells, 1, cap).astype(np.int32)
         else:
             team = np.ones(self.ninst, dtype=np.int32)
         self.team = team
@@ -213,6 +217,17 @@ class BatchedSolver:
                   self.k1.data_ptr(), self.k2.data_ptr(), self._counter.data_ptr(),
                   self.t_out.data_ptr(), self.steps_out.data_ptr(),
                   self.status_out.data_ptr(), _lib.stream_of(self.u))
+
+    def kernel_info(self) -> dict[str, int]:
+        """Registers per lane, private (spill) bytes per lane and static LDS
+        bytes per workgroup of the kernel :meth:`launch` runs."""
+        vgprs = ctypes.c_int(0)
+        private = ctypes.c_longlong(0)
+        lds = ctypes.c_longlong(0)
+        _lib.call("psa_batch_kernel_info", ctypes.byref(self.desc), ctypes.byref(vgprs),
+                  ctypes.byref(private), ctypes.byref(lds))
+        return {"vgprs": int(vgprs.value), "private_bytes": int(private.value),
+                "lds_bytes": int(lds.value)}
 
     def solve(self, tfinal: float | Sequence[float]) -> BatchResult:
         """Advance every instance from t = 0 to ``tfinal`` (per instance)."""

@@ -219,8 +219,10 @@ class ESWENO32(FiniteVolumeScheme):
     """Energy-stable WENO (``burgers/schemes.py:202-256``): the upwind flux of
     the ES-WENO3 reconstruction plus the dissipative flux of its first
     weight; :func:`~pyshocks_amd.schemes.bind` sets the reconstruction's
-    ``eps`` and ``delta`` from the grid. Runs through the operator kernels
-    and the generic step loop, not the fused steps."""
+    ``eps`` and ``delta`` from the grid. ``make_ssprk33`` runs it through
+    the fused stage and whole-step kernels (``PSA_FLUX_ESWENO`` with the
+    descriptor's ``es_delta``); ``numerical_flux`` and ``apply_operator``
+    use the standalone kernels (upwind faces, dissipation, difference)."""
 
     def __post_init__(self) -> None:
         from pyshocks_amd.reconstruction import ESWENO32 as ESWENO32Rec

@@ -264,6 +264,8 @@ struct StepVjpArgs {
   const T *k2;
   const T *lam;     // cotangent of the step output
   T *out;           // cotangent of the step input
+  int64_t bstride;  // a batch: instance blockIdx.y at offset blockIdx.y *
+                    //   bstride of every array, its dt at dt_ptr[blockIdx.y]
 };
 
 constexpr int kStepVjpOut = kVjpBlock - 18;  // output cells per workgroup
@@ -286,17 +288,23 @@ __global__ __launch_bounds__(kVjpBlock) void k_step_vjp(StepVjpArgs<T> a) {
   const int64_t n = p.n;
   const int g = p.g;
   const int64_t S = g + static_cast<int64_t>(blockIdx.x) * kStepVjpOut;
-  const T dt = *p.dt_ptr;
+  const int64_t off = static_cast<int64_t>(blockIdx.y) * a.bstride;
+  const T dt = p.dt_ptr[blockIdx.y];
+  const T *xk2 = a.k2 + off;
+  const T *xk1 = a.k1 + off;
+  const T *xx = a.x + off;
+  const T *xlam = a.lam + off;
+  T *xout = a.out + off;
 
   if (blockIdx.x == 0 && tid < g) {
-    a.out[tid] = T(0);
-    a.out[p.m - g + tid] = T(0);
+    xout[tid] = T(0);
+    xout[p.m - g + tid] = T(0);
   }
   for (int j = tid; j < B + 4; j += B) {
-    sK2[j] = load_periodic(a.k2, S - 11 + j, n, g);
-    sK1[j] = load_periodic(a.k1, S - 8 + j, n, g);
-    sX[j] = load_periodic(a.x, S - 5 + j, n, g);
-    sLam[j] = load_periodic(a.lam, S - 9 + j, n, g);
+    sK2[j] = load_periodic(xk2, S - 11 + j, n, g);
+    sK1[j] = load_periodic(xk1, S - 8 + j, n, g);
+    sX[j] = load_periodic(xx, S - 5 + j, n, g);
+    sLam[j] = load_periodic(xlam, S - 9 + j, n, g);
   }
   // entries past the exact cotangents (read only for cells outside the
   // exact ranges; finite values)
@@ -317,16 +325,17 @@ __global__ __launch_bounds__(kVjpBlock) void k_step_vjp(StepVjpArgs<T> a) {
   if (tid < kStepVjpOut) {
     const int64_t j = S + tid;
     if (j < p.m - g)
-      a.out[j] = vjp_value<T, 1>(sLk1[tid + 3], gsum, dt, sLam[tid + 9], sLk2[tid + 6]);
+      xout[j] = vjp_value<T, 1>(sLk1[tid + 3], gsum, dt, sLam[tid + 9], sLk2[tid + 6]);
   }
 }
 
 template <typename T>
-__global__ void k_zero_ghosts(T *u, int64_t m, int g) {
+__global__ void k_zero_ghosts(T *u, int64_t m, int g, int64_t ld) {
   const int t = threadIdx.x;
-  if (blockIdx.x == 0 && t < g) {
-    u[t] = T(0);
-    u[m - g + t] = T(0);
+  T *v = u + static_cast<int64_t>(blockIdx.x) * ld;
+  if (t < g) {
+    v[t] = T(0);
+    v[m - g + t] = T(0);
   }
 }
 
@@ -385,8 +394,9 @@ int launch_vjp(const psa_solver_desc *d, int rstage, const void *state,
 template <typename T>
 int launch_step_vjp(const psa_solver_desc *d, const void *x, const void *k1,
                     const void *k2, const void *lam, void *out, const void *dt,
-                    hipStream_t s) {
+                    hipStream_t s, long long batch = 1, int64_t ld = 0) {
   StepVjpArgs<T> a{};
+  a.bstride = ld;
   a.v.dt_ptr = static_cast<const T *>(dt);
   a.v.n = d->n;
   a.v.g = d->g;
@@ -399,20 +409,19 @@ int launch_step_vjp(const psa_solver_desc *d, const void *x, const void *k1,
   a.k2 = static_cast<const T *>(k2);
   a.lam = static_cast<const T *>(lam);
   a.out = static_cast<T *>(out);
-  const unsigned int blocks =
-      static_cast<unsigned int>((d->n + kStepVjpOut - 1) / kStepVjpOut);
+  const dim3 grid(static_cast<unsigned int>((d->n + kStepVjpOut - 1) / kStepVjpOut),
+                  static_cast<unsigned int>(batch));
   if (d->flux == PSA_FLUX_RUSANOV) {
-    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_RUSANOV>), dim3(blocks),
-                       dim3(kVjpBlock), 0, s, a);
+    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_RUSANOV>), grid, dim3(kVjpBlock), 0, s, a);
   } else if (d->flux == PSA_FLUX_BURGERS_UPWIND) {
-    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_BURGERS_UPWIND>), dim3(blocks),
-                       dim3(kVjpBlock), 0, s, a);
+    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_BURGERS_UPWIND>), grid, dim3(kVjpBlock), 0,
+                       s, a);
   } else if (d->flux == PSA_FLUX_ENGQUIST_OSHER) {
-    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ENGQUIST_OSHER>), dim3(blocks),
-                       dim3(kVjpBlock), 0, s, a);
+    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ENGQUIST_OSHER>), grid, dim3(kVjpBlock), 0,
+                       s, a);
   } else {
-    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ADVECTION_UPWIND>), dim3(blocks),
-                       dim3(kVjpBlock), 0, s, a);
+    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ADVECTION_UPWIND>), grid, dim3(kVjpBlock),
+                       0, s, a);
   }
   return check_launch("step vjp");
 }
@@ -464,11 +473,13 @@ struct AdjointLayout {
   size_t bytes;
 };
 
+// batch instances at a stride of ld elements per array (ld = m for one)
 AdjointLayout adjoint_layout(const psa_solver_desc *d, long long nsteps,
-                             int every) {
+                             int every, long long batch = 1, int64_t ld = 0) {
   AdjointLayout L{};
   const size_t m = static_cast<size_t>(d->n + 2 * static_cast<int64_t>(d->g));
-  L.array_bytes = ((m * elem_size(d) + 255) / 256) * 256;
+  const size_t cells = batch > 1 ? static_cast<size_t>(batch) * static_cast<size_t>(ld) : m;
+  L.array_bytes = ((cells * elem_size(d) + 255) / 256) * 256;
   L.ncp = nsteps / every + 1;
   L.nwin = 3LL * every;
   // checkpoints, window (x, k1, k2 per step), running state, 4 cotangents
@@ -537,32 +548,40 @@ int psa_ssprk33_step_vjp(const psa_solver_desc *desc, const void *u,
   return psa_ssprk33_stage_vjp(desc, 1, u, lk1, lam, lk2, lam_out, dt, stream);
 }
 
-int psa_adjoint_workspace_bytes(const psa_solver_desc *desc, long long nsteps,
-                                int every, size_t *bytes) {
-  int st = check_vjp_desc(desc);
-  if (st != PSA_OK) return st;
-  if (bytes == nullptr || nsteps < 0 || every <= 0)
-    return set_error(PSA_ERR_INVALID_ARGUMENT, "bad arguments");
-  *bytes = adjoint_layout(desc, nsteps, every).bytes;
+}  // extern "C"
+
+namespace {
+
+// The batch arguments: batch >= 1 instances at ld >= m elements apart (ld a
+// multiple of 16 bytes), whole-step kernels only when batch > 1.
+int check_batch(const psa_solver_desc *d, long long batch, long long ld) {
+  const int64_t m = d->n + 2 * static_cast<int64_t>(d->g);
+  if (batch < 1 || batch > 65535)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "batch must be in [1, 65535]");
+  if (batch > 1 && (ld < m || (ld * static_cast<long long>(elem_size(d))) % 16 != 0))
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "ld must be >= m and a multiple of 16 bytes");
   return PSA_OK;
 }
 
-int psa_ssprk33_forward_checkpointed(const psa_solver_desc *desc,
-                                     const void *u0, void *u_final,
-                                     long long nsteps, const void *dt,
-                                     int every, void *workspace,
-                                     size_t workspace_bytes, void *stream) {
+int forward_checkpointed(const psa_solver_desc *desc, long long batch, int64_t ld,
+                         const void *u0, void *u_final, long long nsteps,
+                         const void *dt, int every, void *workspace,
+                         size_t workspace_bytes, hipStream_t s) {
   int st = check_vjp_desc(desc);
+  if (st != PSA_OK) return st;
+  st = check_batch(desc, batch, ld);
   if (st != PSA_OK) return st;
   if (u0 == nullptr || u_final == nullptr || dt == nullptr ||
       workspace == nullptr || nsteps < 0 || every <= 0)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "bad arguments");
-  const AdjointLayout L = adjoint_layout(desc, nsteps, every);
+  const AdjointLayout L = adjoint_layout(desc, nsteps, every, batch, ld);
   if (workspace_bytes < L.bytes)
     return set_error(PSA_ERR_WORKSPACE, "adjoint workspace too small");
-  auto s = static_cast<hipStream_t>(stream);
   const size_t m = static_cast<size_t>(desc->n + 2 * static_cast<int64_t>(desc->g));
-  const size_t bytes = m * elem_size(desc);
+  const size_t bytes =
+      (batch > 1 ? static_cast<size_t>(batch) * static_cast<size_t>(ld) : m) *
+      elem_size(desc);
   char *base = static_cast<char *>(workspace);
   auto cp = [&](long long k) { return base + k * L.array_bytes; };
   char *win = cp(L.ncp);
@@ -577,7 +596,7 @@ int psa_ssprk33_forward_checkpointed(const psa_solver_desc *desc,
   // array, where the whole-step kernel applies
   char *alt = win;
   for (long long step = 0; step < nsteps; ++step) {
-    st = step_fixed_dt(desc, cur, alt, nullptr, nullptr, dt, s);
+    st = step_fixed_dt(desc, cur, alt, nullptr, nullptr, dt, s, batch, ld);
     if (st == PSA_OK) {
       char *tmp = cur;
       cur = alt;
@@ -589,11 +608,15 @@ int psa_ssprk33_forward_checkpointed(const psa_solver_desc *desc,
       continue;
     }
     if (st != PSA_ERR_NOT_IMPLEMENTED) return st;
-    st = psa_ssprk33_stage(desc, 1, cur, cur, k1, nullptr, dt, stream);
+    if (batch > 1)
+      return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                       "a batch runs on the whole-step kernel (default or step "
+                       "schedule, tiles 256 x 4 or 256 x 2)");
+    st = psa_ssprk33_stage(desc, 1, cur, cur, k1, nullptr, dt, s);
     if (st != PSA_OK) return st;
-    st = psa_ssprk33_stage(desc, 2, cur, k1, k2, nullptr, dt, stream);
+    st = psa_ssprk33_stage(desc, 2, cur, k1, k2, nullptr, dt, s);
     if (st != PSA_OK) return st;
-    st = psa_ssprk33_stage(desc, 3, cur, k2, cur, nullptr, dt, stream);
+    st = psa_ssprk33_stage(desc, 3, cur, k2, cur, nullptr, dt, s);
     if (st != PSA_OK) return st;
     if ((step + 1) % every == 0) {
       st = copy_async(cp((step + 1) / every), cur, bytes, s);
@@ -603,21 +626,23 @@ int psa_ssprk33_forward_checkpointed(const psa_solver_desc *desc,
   return copy_async(u_final, cur, bytes, s);
 }
 
-int psa_ssprk33_adjoint(const psa_solver_desc *desc, long long nsteps,
-                        const void *dt, int every, const void *lam_final,
-                        void *grad_u0, void *workspace, size_t workspace_bytes,
-                        void *stream) {
+int adjoint(const psa_solver_desc *desc, long long batch, int64_t ld, long long nsteps,
+            const void *dt, int every, const void *lam_final, void *grad_u0,
+            void *workspace, size_t workspace_bytes, hipStream_t s) {
   int st = check_vjp_desc(desc);
+  if (st != PSA_OK) return st;
+  st = check_batch(desc, batch, ld);
   if (st != PSA_OK) return st;
   if (lam_final == nullptr || grad_u0 == nullptr || dt == nullptr ||
       workspace == nullptr || nsteps < 0 || every <= 0)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "bad arguments");
-  const AdjointLayout L = adjoint_layout(desc, nsteps, every);
+  const AdjointLayout L = adjoint_layout(desc, nsteps, every, batch, ld);
   if (workspace_bytes < L.bytes)
     return set_error(PSA_ERR_WORKSPACE, "adjoint workspace too small");
-  auto s = static_cast<hipStream_t>(stream);
   const size_t m = static_cast<size_t>(desc->n + 2 * static_cast<int64_t>(desc->g));
-  const size_t bytes = m * elem_size(desc);
+  const size_t bytes =
+      (batch > 1 ? static_cast<size_t>(batch) * static_cast<size_t>(ld) : m) *
+      elem_size(desc);
   char *base = static_cast<char *>(workspace);
   auto cp = [&](long long k) { return base + k * L.array_bytes; };
   char *win = cp(L.ncp);
@@ -628,6 +653,8 @@ int psa_ssprk33_adjoint(const psa_solver_desc *desc, long long nsteps,
   char *lam_b = lam_a + L.array_bytes;
   char *lk2 = lam_b + L.array_bytes;
   char *lk1 = lk2 + L.array_bytes;
+  const bool fused_reverse = batch > 1 || (desc->schedule != PSA_SCHEDULE_TILE &&
+                                           desc->schedule != PSA_SCHEDULE_PIPELINE);
 
   st = copy_async(lam_a, lam_final, bytes, s);
   if (st != PSA_OK) return st;
@@ -643,44 +670,43 @@ int psa_ssprk33_adjoint(const psa_solver_desc *desc, long long nsteps,
       const long long i = step - s0;
       // one whole step writing k1, k2 and the next state (X(every) is the
       // running-state array, free in this pass)
-      st = step_fixed_dt(desc, X(i), X(i + 1), K1(i), K2(i), dt, s);
+      st = step_fixed_dt(desc, X(i), X(i + 1), K1(i), K2(i), dt, s, batch, ld);
       if (st == PSA_OK) continue;
       if (st != PSA_ERR_NOT_IMPLEMENTED) return st;
-      st = psa_ssprk33_stage(desc, 1, X(i), X(i), K1(i), nullptr, dt, stream);
+      if (batch > 1)
+        return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                         "a batch runs on the whole-step kernel (default or step "
+                         "schedule, tiles 256 x 4 or 256 x 2)");
+      st = psa_ssprk33_stage(desc, 1, X(i), X(i), K1(i), nullptr, dt, s);
       if (st != PSA_OK) return st;
-      st = psa_ssprk33_stage(desc, 2, X(i), K1(i), K2(i), nullptr, dt, stream);
+      st = psa_ssprk33_stage(desc, 2, X(i), K1(i), K2(i), nullptr, dt, s);
       if (st != PSA_OK) return st;
       if (step + 1 < s1) {
-        st = psa_ssprk33_stage(desc, 3, X(i), K2(i), X(i + 1), nullptr, dt,
-                               stream);
+        st = psa_ssprk33_stage(desc, 3, X(i), K2(i), X(i + 1), nullptr, dt, s);
         if (st != PSA_OK) return st;
       }
     }
     // reverse sweep through the window
     for (long long step = s1 - 1; step >= s0; --step) {
       const long long i = step - s0;
-      if (desc->schedule != PSA_SCHEDULE_TILE &&
-          desc->schedule != PSA_SCHEDULE_PIPELINE) {
+      if (fused_reverse) {
         // the three reverse stages in one launch
         st = desc->dtype == PSA_FLOAT64
                  ? launch_step_vjp<double>(desc, X(i), K1(i), K2(i), lam, lam_next,
-                                           dt, s)
+                                           dt, s, batch, ld)
                  : launch_step_vjp<float>(desc, X(i), K1(i), K2(i), lam, lam_next,
-                                          dt, s);
+                                          dt, s, batch, ld);
         if (st != PSA_OK) return st;
         char *tmp = lam;
         lam = lam_next;
         lam_next = tmp;
         continue;
       }
-      st = psa_ssprk33_stage_vjp(desc, 3, K2(i), lam, nullptr, nullptr, lk2, dt,
-                                 stream);
+      st = psa_ssprk33_stage_vjp(desc, 3, K2(i), lam, nullptr, nullptr, lk2, dt, s);
       if (st != PSA_OK) return st;
-      st = psa_ssprk33_stage_vjp(desc, 2, K1(i), lk2, nullptr, nullptr, lk1, dt,
-                                 stream);
+      st = psa_ssprk33_stage_vjp(desc, 2, K1(i), lk2, nullptr, nullptr, lk1, dt, s);
       if (st != PSA_OK) return st;
-      st = psa_ssprk33_stage_vjp(desc, 1, X(i), lk1, lam, lk2, lam_next, dt,
-                                 stream);
+      st = psa_ssprk33_stage_vjp(desc, 1, X(i), lk1, lam, lk2, lam_next, dt, s);
       if (st != PSA_OK) return st;
       char *tmp = lam;
       lam = lam_next;
@@ -692,16 +718,73 @@ int psa_ssprk33_adjoint(const psa_solver_desc *desc, long long nsteps,
   if (nsteps == 0) {
     // no step: the gradient is the incoming cotangent with zero ghosts
     const int64_t mm = static_cast<int64_t>(m);
+    const dim3 grid(static_cast<unsigned int>(batch));
     if (desc->dtype == PSA_FLOAT64) {
-      hipLaunchKernelGGL(k_zero_ghosts<double>, dim3(1), dim3(64), 0, s,
-                         static_cast<double *>(grad_u0), mm, desc->g);
+      hipLaunchKernelGGL(k_zero_ghosts<double>, grid, dim3(64), 0, s,
+                         static_cast<double *>(grad_u0), mm, desc->g, ld);
     } else {
-      hipLaunchKernelGGL(k_zero_ghosts<float>, dim3(1), dim3(64), 0, s,
-                         static_cast<float *>(grad_u0), mm, desc->g);
+      hipLaunchKernelGGL(k_zero_ghosts<float>, grid, dim3(64), 0, s,
+                         static_cast<float *>(grad_u0), mm, desc->g, ld);
     }
     return check_launch("zero ghosts");
   }
   return PSA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psa_adjoint_workspace_bytes(const psa_solver_desc *desc, long long nsteps,
+                                int every, size_t *bytes) {
+  return psa_adjoint_workspace_bytes_batch(desc, 1, 0, nsteps, every, bytes);
+}
+
+int psa_adjoint_workspace_bytes_batch(const psa_solver_desc *desc, long long batch,
+                                      long long ld, long long nsteps, int every,
+                                      size_t *bytes) {
+  int st = check_vjp_desc(desc);
+  if (st != PSA_OK) return st;
+  st = check_batch(desc, batch, ld);
+  if (st != PSA_OK) return st;
+  if (bytes == nullptr || nsteps < 0 || every <= 0)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "bad arguments");
+  *bytes = adjoint_layout(desc, nsteps, every, batch, ld).bytes;
+  return PSA_OK;
+}
+
+int psa_ssprk33_forward_checkpointed(const psa_solver_desc *desc,
+                                     const void *u0, void *u_final,
+                                     long long nsteps, const void *dt,
+                                     int every, void *workspace,
+                                     size_t workspace_bytes, void *stream) {
+  return forward_checkpointed(desc, 1, 0, u0, u_final, nsteps, dt, every, workspace,
+                              workspace_bytes, static_cast<hipStream_t>(stream));
+}
+
+int psa_ssprk33_forward_checkpointed_batch(const psa_solver_desc *desc, long long batch,
+                                           long long ld, const void *u0, void *u_final,
+                                           long long nsteps, const void *dt, int every,
+                                           void *workspace, size_t workspace_bytes,
+                                           void *stream) {
+  return forward_checkpointed(desc, batch, ld, u0, u_final, nsteps, dt, every, workspace,
+                              workspace_bytes, static_cast<hipStream_t>(stream));
+}
+
+int psa_ssprk33_adjoint(const psa_solver_desc *desc, long long nsteps,
+                        const void *dt, int every, const void *lam_final,
+                        void *grad_u0, void *workspace, size_t workspace_bytes,
+                        void *stream) {
+  return adjoint(desc, 1, 0, nsteps, dt, every, lam_final, grad_u0, workspace,
+                 workspace_bytes, static_cast<hipStream_t>(stream));
+}
+
+int psa_ssprk33_adjoint_batch(const psa_solver_desc *desc, long long batch, long long ld,
+                              long long nsteps, const void *dt, int every,
+                              const void *lam_final, void *grad_u0, void *workspace,
+                              size_t workspace_bytes, void *stream) {
+  return adjoint(desc, batch, ld, nsteps, dt, every, lam_final, grad_u0, workspace,
+                 workspace_bytes, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

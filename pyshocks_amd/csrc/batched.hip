@@ -81,6 +81,9 @@ struct BatchArgs {
   int limiter;   // PSA_LIMITER_* of the MUSCL reconstructions
   T lim_theta;
   T lim_atol;
+  // host only: when set, the dispatch fills in the selected kernel's
+  // attributes instead of launching it
+  hipFuncAttributes *query;
 };
 
 // A value every lane of the wave holds (read from LDS after a broadcast)
@@ -259,6 +262,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
     int status = alive ? PSA_OK : PSA_ERR_HIP;
     while (alive) {
       T dt;
+      // time_f64: the clipped last step advances t by the FP64 step (t
+      // reaches tfinal + 1e-15 as in timestepping.py:140-142; the rounding of
+      // that step to the state dtype must not leave t just short of tfinal)
+      double dt_t64 = 0.0;
       if (a.time_f64) {
         if (t64 >= tf64) break;
       } else {
@@ -270,11 +277,14 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
       }
       if (a.time_f64) {
         double dt64 = a.theta64 * (cfl64 / static_cast<double>(from_bits<T>(umax)));
+        bool clipped = false;
         if (!isinf(tf64)) {
           const double dt_min = tf64 - t64;
+          clipped = !(dt64 < dt_min);
           dt64 = (dt64 < dt_min ? dt64 : dt_min) + 1.0e-15;
         }
         dt = static_cast<T>(dt64);
+        dt_t64 = clipped ? dt64 : static_cast<double>(dt);
         if (!isfinite(dt) || !(dt > T(0)) || !(t64 + static_cast<double>(dt) > t64)) {
           status = PSA_ERR_NONFINITE_DT;
           break;
@@ -342,7 +352,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
         umax = team_max(vmax, static_cast<int>(steps & 1));  // u complete
       }
       t = t + dt;
-      t64 = t64 + static_cast<double>(dt);
+      t64 = t64 + dt_t64;
       steps += 1;
     }
     if (!alive) status = PSA_ERR_HIP;
@@ -397,6 +407,15 @@ unsigned int resident(K kernel, int block) {
 template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED, bool WHOLE, int REC = 0>
 int launch_batch_cfg(const BatchArgs<T> &a, int grid, int max_team, long long nteams,
                      hipStream_t s) {
+  if (a.query != nullptr) {
+    const hipError_t err = hipFuncGetAttributes(
+        a.query,
+        reinterpret_cast<const void *>(k_batch<T, FLUX, BLOCK, CPT, LEARNED, WHOLE, REC>));
+    if (err != hipSuccess)
+      return set_error(PSA_ERR_HIP, std::string("hipFuncGetAttributes: ") +
+                                        hipGetErrorString(err));
+    return PSA_OK;
+  }
   const unsigned int cap =
       resident(k_batch<T, FLUX, BLOCK, CPT, LEARNED, WHOLE, REC>, BLOCK);
   if (cap == 0)
@@ -495,8 +514,9 @@ int launch_batch_flux(const BatchArgs<T> &a, bool whole, int block, int cpt,
 template <typename T>
 int launch_batch(const psa_batch_desc *d, void *u, void *k1, void *k2,
                  unsigned int *counter, double *t_out, long long *steps_out,
-                 int *status_out, hipStream_t s) {
+                 int *status_out, hipStream_t s, hipFuncAttributes *query = nullptr) {
   BatchArgs<T> a{};
+  a.query = query;
   a.u = static_cast<T *>(u);
   a.k1 = static_cast<T *>(k1);
   a.k2 = static_cast<T *>(k2);
@@ -557,10 +577,10 @@ int launch_batch(const psa_batch_desc *d, void *u, void *k1, void *k2,
 
 using namespace psa;
 
-extern "C" int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1,
-                               void *k2, unsigned int *counter, double *t_out,
-                               long long *steps_out, int *status_out,
-                               void *stream) {
+namespace {
+
+// The descriptor checks of psa_batch_solve that do not involve buffers.
+int check_batch_desc(const psa_batch_desc *desc) {
   if (desc == nullptr) return set_error(PSA_ERR_INVALID_ARGUMENT, "null desc");
   if (desc->dtype != PSA_FLOAT64 && desc->dtype != PSA_FLOAT32)
     return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported dtype");
@@ -595,6 +615,36 @@ extern "C" int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1,
                      "the batched solver takes the step or tile schedule");
   if (desc->mlp != nullptr && (!(desc->mlp_tau >= 0.0) || !(desc->mlp_sigma >= 0.0)))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "mlp_tau and mlp_sigma must be >= 0");
+  return PSA_OK;
+}
+
+}  // namespace
+
+extern "C" int psa_batch_kernel_info(const psa_batch_desc *desc, int *vgprs,
+                                     long long *private_bytes, long long *lds_bytes) {
+  const int st = check_batch_desc(desc);
+  if (st != PSA_OK) return st;
+  if (vgprs == nullptr || private_bytes == nullptr || lds_bytes == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  hipFuncAttributes attr{};
+  const int r = desc->dtype == PSA_FLOAT64
+                    ? launch_batch<double>(desc, nullptr, nullptr, nullptr, nullptr,
+                                           nullptr, nullptr, nullptr, nullptr, &attr)
+                    : launch_batch<float>(desc, nullptr, nullptr, nullptr, nullptr,
+                                          nullptr, nullptr, nullptr, nullptr, &attr);
+  if (r != PSA_OK) return r;
+  *vgprs = attr.numRegs;
+  *private_bytes = static_cast<long long>(attr.localSizeBytes);
+  *lds_bytes = static_cast<long long>(attr.sharedSizeBytes);
+  return PSA_OK;
+}
+
+extern "C" int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1,
+                               void *k2, unsigned int *counter, double *t_out,
+                               long long *steps_out, int *status_out,
+                               void *stream) {
+  const int chk = check_batch_desc(desc);
+  if (chk != PSA_OK) return chk;
   if (u == nullptr || k1 == nullptr || k2 == nullptr || counter == nullptr ||
       t_out == nullptr || steps_out == nullptr || status_out == nullptr ||
       desc->offsets == nullptr || desc->ncells == nullptr ||

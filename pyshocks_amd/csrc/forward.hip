@@ -692,10 +692,16 @@ __global__ __launch_bounds__(BLOCK) void k_step(StageArgs<T> p) {
   const int64_t A = static_cast<int64_t>(blockIdx.x) * SM::TO;
 
   if (ctrl == nullptr) {
-    // a fixed step of the device scalar dt (the adjoint's forward passes)
+    // a fixed step of the device scalar dt (the adjoint's forward passes);
+    // in a batch, instance blockIdx.y with its own dt
+    const int64_t off = static_cast<int64_t>(blockIdx.y) * p.bstride;
+    StageArgs<T> pb = p;
+    if (p.k1out != nullptr) pb.k1out = p.k1out + off;
+    if (p.k2out != nullptr) pb.k2out = p.k2out + off;
     unsigned long long vmax = 0;
-    step_tile<T, FLUX, VEL, BLOCK, CPT, LEARNED, REC>(p, A, *p.dt_ptr, p.uk, p.out, sm,
-                                                      vmax, q, nullptr);
+    step_tile<T, FLUX, VEL, BLOCK, CPT, LEARNED, REC>(pb, A, p.dt_ptr[blockIdx.y],
+                                                      p.uk + off, p.out + off, sm, vmax,
+                                                      q, nullptr);
     return;
   }
   if (ctrl->active_next == 0) {
@@ -883,9 +889,11 @@ template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED,
 int launch_step_cfg(const StageArgs<T> &args, hipStream_t stream) {
   constexpr int TO = StepSmem<T, BLOCK, CPT>::TO;
   StageArgs<T> a = args;
-  a.aligned = aligned16(a.uk) && aligned16(a.out);
+  a.aligned = aligned16(a.uk) && aligned16(a.out) &&
+              (a.bstride * static_cast<int64_t>(sizeof(T))) % 16 == 0;
   const unsigned int blocks = blocks_for(a.m, TO);
-  hipLaunchKernelGGL((k_step<T, FLUX, VEL, BLOCK, CPT, LEARNED, REC>), dim3(blocks),
+  const unsigned int batch = a.batch > 1 ? static_cast<unsigned int>(a.batch) : 1u;
+  hipLaunchKernelGGL((k_step<T, FLUX, VEL, BLOCK, CPT, LEARNED, REC>), dim3(blocks, batch),
                      dim3(BLOCK), 0, stream, a);
   return check_launch("ssprk33 step");
 }
@@ -1223,11 +1231,14 @@ int run_steps(const psa_solver_desc *d, const void *u_in, void *u_out,
 // PSA_ERR_NOT_IMPLEMENTED (and launches nothing) where the whole-step kernel
 // does not apply; the caller then runs the stage kernels.
 int step_fixed_dt(const psa_solver_desc *d, const void *u_in, void *u_out, void *k1,
-                  void *k2, const void *dt, hipStream_t stream) {
+                  void *k2, const void *dt, hipStream_t stream, long long batch,
+                  int64_t ld) {
   const TileConfig cfg = tiles_of(d);
   const int vel = vel_mode(d);
   if (d->mlp != nullptr || !step_applies(d, cfg, vel))
     return PSA_ERR_NOT_IMPLEMENTED;
+  if (batch < 1 || batch > 65535)
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "batch must be in [1, 65535]");
   if (d->dtype == PSA_FLOAT64) {
     StageArgs<double> a = make_args<double>(d);
     a.uk = a.un = static_cast<const double *>(u_in);
@@ -1235,6 +1246,8 @@ int step_fixed_dt(const psa_solver_desc *d, const void *u_in, void *u_out, void 
     a.k1out = static_cast<double *>(k1);
     a.k2out = static_cast<double *>(k2);
     a.dt_ptr = static_cast<const double *>(dt);
+    a.bstride = ld;
+    a.batch = static_cast<int>(batch);
     return launch_step<double>(d, vel, a, cfg, stream);
   }
   StageArgs<float> a = make_args<float>(d);
@@ -1243,6 +1256,8 @@ int step_fixed_dt(const psa_solver_desc *d, const void *u_in, void *u_out, void 
   a.k1out = static_cast<float *>(k1);
   a.k2out = static_cast<float *>(k2);
   a.dt_ptr = static_cast<const float *>(dt);
+  a.bstride = ld;
+  a.batch = static_cast<int>(batch);
   return launch_step<float>(d, vel, a, cfg, stream);
 }
 

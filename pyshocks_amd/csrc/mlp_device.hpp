@@ -395,14 +395,14 @@ __device__ __forceinline__ T learned_right16(const MlpSource &src, T um2, T um1,
   using C = Weno5Coeffs<T>;
   const T D0 = um1 - um2, D1 = u0 - um1, D2 = up1 - u0, D3 = up2 - up1;
   const T s0 = D1 - D0, s1 = D2 - D1, s2 = D3 - D2;
-  const T E0 = C::a0 * (s0 * s0), E1 = C::a0 * (s1 * s1), E2 = C::a0 * (s2 * s2);
+  const T E0 = C::a0x4 * (s0 * s0), E1 = C::a0x4 * (s1 * s1), E2 = C::a0x4 * (s2 * s2);
   const T t01 = fma(T(2), D1, s0);
   const T t11 = D2 + D1;
   const T t21 = fma(T(-2), D2, s2);
-  const T b0 = E0 + C::a1 * (t01 * t01);
-  const T b1 = E1 + C::a1 * (t11 * t11);
-  const T b2 = E2 + C::a1 * (t21 * t21);
-  const int regime = mlp_regime(b0, b1, b2, eps, tau, sigma);
+  const T b0 = fma(t01, t01, E0);
+  const T b1 = fma(t11, t11, E1);
+  const T b2 = fma(t21, t21, E2);
+  const int regime = mlp_regime(b0, b1, b2, T(4) * eps, tau, sigma);
   js = regime == 0;
   T h0, h1, h2, g0, g1, g2;
   weno_uhat_diff(u0, D0, D1, D2, D3, h0, h1, h2, g0, g1, g2);

@@ -17,9 +17,11 @@ int check_launch(const char *what);
 
 // One fixed-dt whole step (forward.hip): u_in -> u_out, the stage states
 // into k1, k2 if not null; PSA_ERR_NOT_IMPLEMENTED where the whole-step kernel
-// does not apply (nothing launched).
+// does not apply (nothing launched). A batch of instances takes one launch:
+// instance b at offset b * ld (elements) of every array, dt[b] its step.
 int step_fixed_dt(const psa_solver_desc *d, const void *u_in, void *u_out, void *k1,
-                  void *k2, const void *dt, hipStream_t stream);
+                  void *k2, const void *dt, hipStream_t stream, long long batch = 1,
+                  int64_t ld = 0);
 
 // Limiter of the flux-limited MUSCL: a known PSA_LIMITER_*, minmod's theta
 // in [1, 2] (limiters.py:166-198), a positive slope-ratio cut-off.

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "../../include/pyshocks_amd.h"
 #include "mlp_device.hpp"
@@ -132,6 +133,10 @@ struct StageArgs {
   T sigma;           // their smooth threshold
   T *k1out;          // whole-step kernel: the stage states k1, k2 as well,
   T *k2out;          //   or null
+  int64_t bstride;   // fixed-dt whole steps of a batch: instance blockIdx.y
+                     //   at offset blockIdx.y * bstride of uk, out, k1out,
+                     //   k2out, its dt at dt_ptr[blockIdx.y]
+  int batch;         // host side: instances of the launch (grid.y)
 };
 
 template <typename T, int VEL>
@@ -374,12 +379,51 @@ __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
   T D[CPT + 3];
 #pragma unroll
   for (int k = 0; k < CPT + 3; ++k) D[k] = w[k + 1] - w[k];
-  // second differences at cells -1 .. CPT and a0 times their squares
+  // second differences at cells -1 .. CPT and 13/3 times their squares (the
+  // indicators are carried as 4 beta_k with 4 eps, Weno5Coeffs::a0x4)
   T d2[CPT + 2], E[CPT + 2];
 #pragma unroll
   for (int k = 0; k < CPT + 2; ++k) {
     d2[k] = D[k + 1] - D[k];
-    E[k] = C::a0 * (d2[k] * d2[k]);
+    E[k] = C::a0x4 * (d2[k] * d2[k]);
+  }
+  const T eps4 = T(4) * p.eps;
+
+  if constexpr (!LEARNED && std::is_same<T, double>::value) {
+    // FP64: the shared reciprocal for every cell without a branch; a cell
+    // whose denominator is not a positive normal number is redone in the
+    // reference form from its stencil after one wave-uniform test (its
+    // result depends on its own values only, not on the wave it falls in)
+    unsigned int redo = 0;
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+      const T t01 = fma(T(2), D[r + 1], d2[r]);
+      const T t11 = D[r + 2] + D[r + 1];
+      const T t21 = fma(T(-2), D[r + 2], d2[r + 2]);
+      T hr0, hr1, hr2, hl0, hl1, hl2;
+      weno_uhat_diff(w[r + 2], D[r], D[r + 1], D[r + 2], D[r + 3], hr0, hr1, hr2,
+                     hl0, hl1, hl2);
+      const WenoPairSums ps =
+          weno_pair_sums(hr0, hr1, hr2, hl0, hl1, hl2, fma(t01, t01, E[r]),
+                         fma(t11, t11, E[r + 1]), fma(t21, t21, E[r + 2]), eps4);
+      redo |= weno_pair_fast(ps) ? 0u : (1u << r);
+      weno_pair_divide(ps, ul[r], ur[r]);
+    }
+    if (__any(redo != 0)) {
+#pragma unroll
+      for (int r = 0; r < CPT; ++r) {
+        if (redo & (1u << r)) {
+          T b0, b1, b2, hr0, hr1, hr2, hl0, hl1, hl2;
+          weno_beta(w[r], w[r + 1], w[r + 2], w[r + 3], w[r + 4], b0, b1, b2);
+          weno_uhat(w[r], w[r + 1], w[r + 2], w[r + 3], w[r + 4], hr0, hr1, hr2);
+          weno_uhat(w[r + 4], w[r + 3], w[r + 2], w[r + 1], w[r], hl0, hl1, hl2);
+          const T e0 = p.eps + b0, e1 = p.eps + b1, e2 = p.eps + b2;
+          weno_pair_ref_q(hr0, hr1, hr2, hl0, hl1, hl2, e0 * e0, e1 * e1, e2 * e2,
+                          ul[r], ur[r]);
+        }
+      }
+    }
+    return;
   }
 
   int regime[CPT];
@@ -389,9 +433,9 @@ __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
     const T t01 = fma(T(2), D[r + 1], d2[r]);
     const T t11 = D[r + 2] + D[r + 1];
     const T t21 = fma(T(-2), D[r + 2], d2[r + 2]);
-    const T b0 = E[r] + C::a1 * (t01 * t01);
-    const T b1 = E[r + 1] + C::a1 * (t11 * t11);
-    const T b2 = E[r + 2] + C::a1 * (t21 * t21);
+    const T b0 = fma(t01, t01, E[r]);
+    const T b1 = fma(t11, t11, E[r + 1]);
+    const T b2 = fma(t21, t21, E[r + 2]);
     T hr0, hr1, hr2, hl0, hl1, hl2;
     weno_uhat_diff(w[r + 2], D[r], D[r + 1], D[r + 2], D[r + 3], hr0, hr1, hr2,
                    hl0, hl1, hl2);
@@ -399,16 +443,16 @@ __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
       // WENO-JS weights only for shock cells (the same arithmetic as the
       // WENO-JS kernels, so that regime reproduces them bit for bit), the
       // ideal weights for smooth ones; learned cells get theirs below
-      regime[r] = mlp_regime(b0, b1, b2, p.eps, p.tau, p.sigma);
+      regime[r] = mlp_regime(b0, b1, b2, eps4, p.tau, p.sigma);
       if (regime[r] == 0) {
-        weno_combine_pair(hr0, hr1, hr2, hl0, hl1, hl2, b0, b1, b2, p.eps, ul[r],
+        weno_combine_pair(hr0, hr1, hr2, hl0, hl1, hl2, b0, b1, b2, eps4, ul[r],
                           ur[r]);
       } else {
         ur[r] = mlp_combine_ideal(hr0, hr1, hr2);
         ul[r] = mlp_combine_ideal(hl0, hl1, hl2);
       }
     } else {
-      weno_combine_pair(hr0, hr1, hr2, hl0, hl1, hl2, b0, b1, b2, p.eps, ul[r],
+      weno_combine_pair(hr0, hr1, hr2, hl0, hl1, hl2, b0, b1, b2, eps4, ul[r],
                         ur[r]);
     }
   }

@@ -21,6 +21,7 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <type_traits>
 
 #include "../../include/pyshocks_amd.h"
 
@@ -37,6 +38,10 @@ struct Weno5Coeffs {
   static constexpr T c2 = T(2.0 / 6.0);
   static constexpr T c5 = T(5.0 / 6.0);
   static constexpr T c1 = T(1.0 / 6.0);
+  // the fused kernels carry 4 beta_k = (13/3) t_k0^2 + t_k1^2 with 4 eps: a
+  // power-of-two scale of every eps + beta_k, which the normalised weights
+  // (ratios of products of (eps + beta)^2 of equal degree) do not see
+  static constexpr T a0x4 = T(13.0 / 3.0);
   // ideal weights (weno.py:242)
   static constexpr T d0 = T(1.0 / 10.0);
   static constexpr T d1 = T(6.0 / 10.0);
@@ -108,49 +113,84 @@ __device__ __forceinline__ T weno_side_ref(T um2, T um1, T u0, T up1, T up2,
 
 // Both sides' weights and values from the right-side indicators and the six
 // candidate values (right hr*, left hl*). FP64: the product form with one
-// reciprocal (see the top of this file), guarded by the class of the shared
-// denominator (a positive normal number).
+// reciprocal (see the top of this file) where the shared denominator is a
+// positive normal number, else the reference form.
 template <typename T>
 __device__ __forceinline__ void weno_combine_pair(T hr0, T hr1, T hr2, T hl0,
                                                   T hl1, T hl2, T b0, T b1,
                                                   T b2, T eps, T &ul, T &ur);
 
+// The product form of both sides (FP64): ur = nr / sr, ul = nl / sl.
+struct WenoPairSums {
+  double sr, sl, nr, nl;
+};
+
+__device__ __forceinline__ WenoPairSums weno_pair_sums(double hr0, double hr1,
+                                                       double hr2, double hl0,
+                                                       double hl1, double hl2,
+                                                       double b0, double b1,
+                                                       double b2, double eps) {
+  const double e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+  const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
+  const double p01 = q0 * q1, p02 = q0 * q2, p12 = q1 * q2;
+  // right: alpha_k ~ d_k / q_k; left: d_0 / q_2, d_1 / q_1, d_2 / q_0; the
+  // ideal weights 1/10, 6/10, 3/10 enter in their ratio 1 : 6 : 3, which
+  // the normalisation does not see
+  const double w1 = 6.0 * p02;
+  const double wr2 = 3.0 * p01, wl2 = 3.0 * p12;
+  WenoPairSums r;
+  r.sr = (p12 + w1) + wr2;
+  r.sl = (p01 + w1) + wl2;
+  r.nr = (p12 * hr0 + w1 * hr1) + wr2 * hr2;
+  r.nl = (p01 * hl0 + w1 * hl1) + wl2 * hl2;
+  return r;
+}
+
+// Whether the shared reciprocal applies: sr sl a positive normal number.
+__device__ __forceinline__ bool weno_pair_fast(const WenoPairSums &s) {
+  return __builtin_amdgcn_class(s.sr * s.sl, 0x100);
+}
+
+// ur, ul by the shared reciprocal: the hardware reciprocal of sr sl refined
+// by two Newton steps (each squares the relative error; within an ulp or two
+// of 1 / (sr sl)).
+__device__ __forceinline__ void weno_pair_divide(const WenoPairSums &s, double &ul,
+                                                 double &ur) {
+  const double den = s.sr * s.sl;
+  double inv = __builtin_amdgcn_rcp(den);
+  double e = fma(-den, inv, 1.0);
+  inv = fma(inv, e, inv);
+  e = fma(-den, inv, 1.0);
+  inv = fma(inv, e, inv);
+  ur = (s.nr * s.sl) * inv;
+  ul = (s.nl * s.sr) * inv;
+}
+
+// The reference form from q_k = (eps + beta_k)^2: safe over the whole
+// exponent range.
+__device__ __forceinline__ void weno_pair_ref_q(double hr0, double hr1, double hr2,
+                                                double hl0, double hl1, double hl2,
+                                                double q0, double q1, double q2,
+                                                double &ul, double &ur) {
+  using C = Weno5Coeffs<double>;
+  const double a0 = C::d0 / q0, a1 = C::d1 / q1, a2 = C::d2 / q2;
+  const double s = (a0 + a1) + a2;
+  ur = ((a0 / s) * hr0 + (a1 / s) * hr1) + (a2 / s) * hr2;
+  const double l0 = C::d0 / q2, l2 = C::d2 / q0;
+  const double sll = (l0 + a1) + l2;
+  ul = ((l0 / sll) * hl0 + (a1 / sll) * hl1) + (l2 / sll) * hl2;
+}
+
 template <>
 __device__ __forceinline__ void weno_combine_pair<double>(
     double hr0, double hr1, double hr2, double hl0, double hl1, double hl2,
     double b0, double b1, double b2, double eps, double &ul, double &ur) {
-  using C = Weno5Coeffs<double>;
-  const double e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
-  const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
-  const double p01 = q0 * q1, p02 = q0 * q2, p12 = q1 * q2;
-
-  // right: alpha_k ~ d_k / q_k; left: d_0 / q_2, d_1 / q_1, d_2 / q_0
-  const double wr0 = C::d0 * p12, wr1 = C::d1 * p02, wr2 = C::d2 * p01;
-  const double wl0 = C::d0 * p01, wl2 = C::d2 * p12;
-  const double sr = (wr0 + wr1) + wr2;
-  const double sl = (wl0 + wr1) + wl2;
-  const double nr = (wr0 * hr0 + wr1 * hr1) + wr2 * hr2;
-  const double nl = (wl0 * hl0 + wr1 * hl1) + wl2 * hl2;
-
-  const double den = sr * sl;
-  if (__builtin_amdgcn_class(den, 0x100)) {  // +normal: within [DBL_MIN, DBL_MAX]
-    // hardware reciprocal refined by two Newton steps (each squares the
-    // relative error): within an ulp or two of 1 / den
-    double inv = __builtin_amdgcn_rcp(den);
-    double e = fma(-den, inv, 1.0);
-    inv = fma(inv, e, inv);
-    e = fma(-den, inv, 1.0);
-    inv = fma(inv, e, inv);
-    ur = (nr * sl) * inv;
-    ul = (nl * sr) * inv;
+  const WenoPairSums s = weno_pair_sums(hr0, hr1, hr2, hl0, hl1, hl2, b0, b1, b2, eps);
+  if (weno_pair_fast(s)) {
+    weno_pair_divide(s, ul, ur);
   } else {
-    // reference form: safe over the whole exponent range
-    const double a0 = C::d0 / q0, a1 = C::d1 / q1, a2 = C::d2 / q2;
-    const double s = (a0 + a1) + a2;
-    ur = ((a0 / s) * hr0 + (a1 / s) * hr1) + (a2 / s) * hr2;
-    const double l0 = C::d0 / q2, l2 = C::d2 / q0;
-    const double sll = (l0 + a1) + l2;
-    ul = ((l0 / sll) * hl0 + (a1 / sll) * hl1) + (l2 / sll) * hl2;
+    const double e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+    weno_pair_ref_q(hr0, hr1, hr2, hl0, hl1, hl2, e0 * e0, e1 * e1, e2 * e2, ul, ur);
   }
 }
 
@@ -158,17 +198,17 @@ template <>
 __device__ __forceinline__ void weno_combine_pair<float>(
     float hr0, float hr1, float hr2, float hl0, float hl1, float hl2, float b0,
     float b1, float b2, float eps, float &ul, float &ur) {
-  using C = Weno5Coeffs<float>;
   const float e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
   const float r0 = __builtin_amdgcn_rcpf(e0 * e0);
   const float r1 = __builtin_amdgcn_rcpf(e1 * e1);
   const float r2 = __builtin_amdgcn_rcpf(e2 * e2);
-  const float ar0 = C::d0 * r0, a1 = C::d1 * r1, ar2 = C::d2 * r2;
-  const float al0 = C::d0 * r2, al2 = C::d2 * r0;
-  const float sr = (ar0 + a1) + ar2;
-  const float sl = (al0 + a1) + al2;
-  ur = ((ar0 * hr0 + a1 * hr1) + ar2 * hr2) * __builtin_amdgcn_rcpf(sr);
-  ul = ((al0 * hl0 + a1 * hl1) + al2 * hl2) * __builtin_amdgcn_rcpf(sl);
+  // the ideal weights in their ratio 1 : 6 : 3, as in the FP64 form
+  const float a1 = 6.0f * r1;
+  const float ar2 = 3.0f * r2, al2 = 3.0f * r0;
+  const float sr = (r0 + a1) + ar2;
+  const float sl = (r2 + a1) + al2;
+  ur = ((r0 * hr0 + a1 * hr1) + ar2 * hr2) * __builtin_amdgcn_rcpf(sr);
+  ul = ((r2 * hl0 + a1 * hl1) + al2 * hl2) * __builtin_amdgcn_rcpf(sl);
 }
 
 template <typename T>
@@ -362,21 +402,43 @@ __device__ __forceinline__ T es_weight(T um1, T u0, T up1, T eps, bool end) {
   return (K == 0 ? a0 : a1) / (a0 + a1);
 }
 
-// es_weight<T, 0> with one division: a_k = d_k (e_k + tau) / e_k with
-// e_k = eps + beta_k, so w_0 = d_0 (e_0 + tau) e_1 / (d_0 (e_0 + tau) e_1 +
-// d_1 (e_1 + tau) e_0) (the fused kernels; rounding differs from the
-// reference's order by a few ulps of the weight).
+// es_weight<T, 0> of the fused kernels.
+// FP64, one division: a_k = d_k (e_k + tau) / e_k with e_k = eps + beta_k, so
+// w_0 = d_0 (e_0 + tau) e_1 / (d_0 (e_0 + tau) e_1 + d_1 (e_1 + tau) e_0);
+// the products scale as the fourth power of the state differences, which
+// FP64's exponent range holds for any state FP32 could hold (rounding
+// differs from the reference's order by a few ulps of the weight).
+// FP32: that product would overflow for differences above ~1e9 and lose the
+// eps term below eps ~ 1e-20, so FP32 keeps the reference's scale-free form
+// a_k = d_k (1 + tau / e_k) with hardware reciprocals (1 ulp each).
 template <typename T>
-__device__ __forceinline__ T es_weight0_fused(T um1, T u0, T up1, T eps, bool end) {
-  using C = Weno3Coeffs<T>;
-  const T t0 = u0 - um1;
-  const T t1 = up1 - u0;
-  const T tau = end ? T(0) : weno3_tau_ref(um1, u0, up1);
-  const T e0 = eps + t0 * t0;
-  const T e1 = eps + t1 * t1;
-  const T n0 = C::d0 * (e0 + tau) * e1;
-  const T n1 = C::d1 * (e1 + tau) * e0;
+__device__ __forceinline__ T es_weight0_fused(T um1, T u0, T up1, T eps, bool end);
+
+template <>
+__device__ __forceinline__ double es_weight0_fused<double>(double um1, double u0,
+                                                           double up1, double eps,
+                                                           bool end) {
+  using C = Weno3Coeffs<double>;
+  const double t0 = u0 - um1;
+  const double t1 = up1 - u0;
+  const double tau = end ? 0.0 : weno3_tau_ref(um1, u0, up1);
+  const double e0 = eps + t0 * t0;
+  const double e1 = eps + t1 * t1;
+  const double n0 = C::d0 * (e0 + tau) * e1;
+  const double n1 = C::d1 * (e1 + tau) * e0;
   return n0 / (n0 + n1);
+}
+
+template <>
+__device__ __forceinline__ float es_weight0_fused<float>(float um1, float u0, float up1,
+                                                         float eps, bool end) {
+  using C = Weno3Coeffs<float>;
+  const float t0 = u0 - um1;
+  const float t1 = up1 - u0;
+  const float tau = end ? 0.0f : weno3_tau_ref(um1, u0, up1);
+  const float a0 = C::d0 * fmaf(tau, __builtin_amdgcn_rcpf(eps + t0 * t0), 1.0f);
+  const float a1 = C::d1 * fmaf(tau, __builtin_amdgcn_rcpf(eps + t1 * t1), 1.0f);
+  return a0 * __builtin_amdgcn_rcpf(a0 + a1);
 }
 
 // The dissipative flux from the two cells' weights wl, wr and values v1, v2.
@@ -460,11 +522,24 @@ __device__ __forceinline__ void weno3_pair(T u0, T Dm, T Dp, T eps, bool es,
   if (es) {
     const T dd = Dp - Dm;
     const T tau = tau_zero ? T(0) : dd * dd;
-    const T p0 = r0 + tau, p1 = r1 + tau;
-    w0r = p0 * r1;
-    w1r = T(2) * (p1 * r0);
-    w0l = p1 * r0;
-    w1l = T(2) * (p0 * r1);
+    if constexpr (std::is_same<T, float>::value) {
+      // FP32: the reference's scale-free a_k = 1 + tau / r_k (the products
+      // below scale as the fourth power of the differences and would
+      // overflow above differences of ~1e9); hardware reciprocals
+      const float a0 = fmaf(tau, __builtin_amdgcn_rcpf(r0), 1.0f);
+      const float a1 = fmaf(tau, __builtin_amdgcn_rcpf(r1), 1.0f);
+      w0r = a0;
+      w1r = 2.0f * a1;
+      w0l = a1;
+      w1l = 2.0f * a0;
+    } else {
+      // a_k = (r_k + tau) / r_k, both sides times r0 r1
+      const T p0 = r0 + tau, p1 = r1 + tau;
+      w0r = p0 * r1;
+      w1r = T(2) * (p1 * r0);
+      w0l = p1 * r0;
+      w1l = T(2) * (p0 * r1);
+    }
   } else {
     const T q0 = r0 * r0, q1 = r1 * r1;
     w0r = q1;

@@ -234,3 +234,23 @@ def test_step_schedule_matches_stage_sweeps(torch_mod, dtype_name: str, scheme: 
     np.testing.assert_array_equal(out[0][1].steps, out[1][1].steps)
     np.testing.assert_array_equal(out[0][1].t, out[1][1].t)
     assert torch.equal(out[0][0], out[1][0])
+
+
+def test_kernel_info_and_autotune_skip_spills(torch_mod) -> None:
+    """psa_batch_kernel_info reports the selected kernel's resources, and the
+    autotuner never picks a candidate whose kernel spills."""
+    torch = torch_mod
+    from pyshocks_amd import autotune
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    ns, ics = random_instances(8, nmin=64, nmax=2048, seed=9)
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    for dtype in (torch.float64, torch.float32):
+        for tiles in autotune.BATCH_CANDIDATES:
+            info = BatchedSolver(ns, dtype=dtype, tiles=tiles).kernel_info()
+            assert 0 < info["vgprs"] <= 512
+            assert info["lds_bytes"] > 0 and info["private_bytes"] >= 0
+        best, times = autotune.tune_batch(ns, states, dtype=dtype, tfinal=0.005)
+        assert BatchedSolver(ns, dtype=dtype, tiles=best).kernel_info()["private_bytes"] == 0
+        for cand in times:
+            assert BatchedSolver(ns, dtype=dtype, tiles=cand).kernel_info()["private_bytes"] == 0

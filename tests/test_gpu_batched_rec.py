@@ -104,8 +104,11 @@ def test_batched_vs_oracle(torch_mod, rec_name: str) -> None:
 
 
 @pytest.mark.parametrize("rec_name", ["esweno32", "muscl-minmod", "muscls-vanleer"])
-@pytest.mark.parametrize("team_cells", [None, 2048])
+@pytest.mark.parametrize("team_cells", [0, 2048])
 def test_step_schedule_matches_stage_sweeps(torch_mod, rec_name: str, team_cells) -> None:
+    """team_cells 0: one workgroup per instance, also for the 20000-cell one
+    (the teamless path of these reconstructions on a large instance); 2048:
+    teams of up to 10 workgroups."""
     torch = torch_mod
     from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
 

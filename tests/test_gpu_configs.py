@@ -1,0 +1,297 @@
+"""Each stated workload at its stated size and dtype, against the oracle.
+
+* Large single instance: periodic Burgers (Rusanov, theta = 1) and linear
+  advection a = 1 (theta = 0.5), N = 2^24, u0 = exact averages of the seeded
+  8-sinusoid + step condition, 100 SSP-RK3 steps at Courant 0.5. FP64 against
+  the C restatement of the reference step loop at 1e-12 (interior relative
+  l2) with the same time; FP32 against the same FP64 result at 1e-5.
+* Batched uneven sizes: 4096 Burgers instances, N log-uniform in
+  [128, 65536] (seeded), FP32, one graph launch. Over ~25 steps of the
+  largest instance every instance against the C restatement in FP64 at
+  1e-5 with the same step count; to the stated t = 0.3 (up to ~39000
+  steps) the FP64 batch against the C restatement at 1e-15 per step and the
+  FP32 batch within the drift its flux rounding allows (see the test).
+* Adjoint: Burgers Rusanov N = 16384, FP64, 2000 fixed-dt steps,
+  checkpoints every 50, J = 0.5 ||u(T) - u_target||^2 dx; <grad J, d>
+  against central differences of the GPU forward solve (h = 1e-7) on 32
+  seeded unit directions, the 64 perturbed solves run as one batch.
+* Small plain method: N = 256 to t = 0.5 (the smoke workload) is covered by
+  the golden fixtures of test_gpu_forward.py.
+
+The C restatement runs on the host's cores (OpenMP); the largest cases take
+tens of seconds there.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N_LARGE = 1 << 24
+STEPS_LARGE = 100
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+
+    from pyshocks_amd import _lib
+
+    _lib.load()
+    return torch
+
+
+def _rel(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+_ORACLE_CACHE: dict[str, tuple[np.ndarray, np.ndarray, float, int]] = {}
+
+
+def _large_oracle(kind: str) -> tuple[np.ndarray, np.ndarray, float, int]:
+    """u0 and the C restatement's state after 100 steps (FP64), cached per
+    scheme so that the FP64 and FP32 runs share it."""
+    if kind not in _ORACLE_CACHE:
+        from oracle import build as oracle_build
+        from oracle import exact as E
+        from oracle import ref_c
+
+        if not ref_c.LIB.exists():
+            oracle_build.build(verbose=False)
+        n = N_LARGE
+        u0 = E.exact_cell_averages(E.large_initial_condition(seed=0), n)
+        h = 1.0 / n
+        if kind == "advection":
+            u, t, it, _ = ref_c.solve("advection", u0, n=n, g=3, dx=h, eps=1e-6,
+                                      theta=0.5, cfl_num=h, maxit=STEPS_LARGE,
+                                      velocity=np.ones(n + 6))
+        else:
+            u, t, it, _ = ref_c.solve("rusanov", u0, n=n, g=3, dx=h, eps=1e-6,
+                                      theta=1.0, cfl_num=0.5 * h, maxit=STEPS_LARGE)
+        assert it == STEPS_LARGE
+        _ORACLE_CACHE[kind] = (u0, u, t, it)
+    return _ORACLE_CACHE[kind]
+
+
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+@pytest.mark.parametrize("kind", ["burgers", "advection"])
+def test_large_single_instance_full_size(torch_mod, kind: str, dtype_name: str) -> None:
+    torch = torch_mod
+    from pyshocks_amd import advection, burgers, make_uniform_cell_grid
+    from pyshocks_amd.reconstruction import WENOJS53
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import evolve, fused_plan, make_ssprk33
+
+    dtype = getattr(torch, dtype_name)
+    print(f"[configs] N=2^24 {kind} {dtype_name}", flush=True)
+    u0, ref, t_ref, _ = _large_oracle("advection" if kind == "advection" else "rusanov")
+    n = N_LARGE
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, dtype=dtype, device="cuda")
+    rec = WENOJS53(eps=1e-6)
+    if kind == "advection":
+        vel = torch.ones(n + 6, dtype=dtype, device="cuda")
+        scheme = advection.Godunov(rec=rec, velocity=vel)
+        theta = 0.5
+    else:
+        scheme = burgers.Rusanov(rec=rec)
+        theta = 1.0
+    stepper = make_ssprk33(scheme, grid, PeriodicBoundary(), theta=theta)
+    plan = fused_plan(stepper)
+    assert plan is not None and plan.uses_step_kernel()
+    ev = evolve(stepper, torch.tensor(u0, dtype=dtype, device="cuda"), maxit=STEPS_LARGE)
+    assert ev.iteration == STEPS_LARGE
+    got = ev.u.double().cpu().numpy()
+    assert np.all(np.isfinite(got))
+    err = _rel(got[3:-3], ref[3:-3])
+    print(f"[configs] N=2^24 {kind} {dtype_name}: rel l2 {err:.3e}, "
+          f"t {float(ev.t):.17g} (oracle {t_ref:.17g})", flush=True)
+    if dtype == torch.float64:
+        assert abs(float(ev.t) - t_ref) <= 1e-13 * t_ref
+        assert err < 1e-12
+    else:
+        assert abs(float(ev.t) - t_ref) <= 1e-5 * t_ref
+        assert err < 1e-5
+
+
+def _c3_batch(torch, dtype, tfinal: float):
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    ns, ics = random_instances(4096, seed=0)
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    solver = BatchedSolver(ns, dtype=dtype)
+    solver.set_states(states)
+    graph = solver.capture(solver.u.clone(), tfinal)
+    graph.replay()
+    torch.cuda.synchronize()
+    res = solver.result()
+    assert np.all(res.status == 0)
+    assert np.all(res.t >= tfinal)
+    return ns, states, solver, res
+
+
+def _c_solve(states, ns, i: int, tfinal: float):
+    from oracle import build as oracle_build
+    from oracle import ref_c
+
+    if not ref_c.LIB.exists():
+        oracle_build.build(verbose=False)
+    n = int(ns[i])
+    u, _, steps, _ = ref_c.solve("rusanov", states[i], n=n, g=3, dx=1.0 / n, eps=1e-6,
+                                 theta=1.0, cfl_num=0.5 / n, tfinal=tfinal)
+    return u, steps
+
+
+def test_batched_uneven_full_size_fp32_every_instance(torch_mod) -> None:
+    """The 4096-instance FP32 batch (one graph launch) over a horizon of
+    about 25 steps of its largest instance: every instance against the C
+    restatement in FP64 at 1e-5, with the same step count."""
+    torch = torch_mod
+    tfinal = 2.0e-4
+    ns, states, solver, res = _c3_batch(torch, torch.float32, tfinal)
+    worst = 0.0
+    for i in range(len(ns)):
+        u, steps = _c_solve(states, ns, i, tfinal)
+        got = solver.state(i).double().cpu().numpy()
+        err = _rel(got[3:-3], u[3:-3])
+        worst = max(worst, err)
+        assert res.steps[i] == steps, (i, res.steps[i], steps)
+        assert err < 1e-5, (i, int(ns[i]), err)
+    print(f"[configs] 4096 instances FP32 to t = {tfinal}: worst rel l2 {worst:.3e}, "
+          f"steps up to {int(res.steps.max())}", flush=True)
+
+
+def test_batched_uneven_full_size_t03(torch_mod) -> None:
+    """The stated horizon t = 0.3 (up to ~39000 steps) in FP64 and FP32.
+
+    FP64: the largest, the smallest and 12 seeded instances against the C
+    restatement with the same step count, to 1e-15 per step (reordered
+    rounding accumulates step by step: 2e-11 after 39000 steps).
+    FP32: against the FP64 batch. Every face flux is rounded to ~6e-8 |F| in
+    FP32 and the update divides it by dx, so the FP32 state drifts from the
+    FP64 one by up to ~6e-8 |F| T / dx (|F| = u^2 / 2 <= 2 here), plus the
+    rounding of the state itself each step: the L1 bound asserted per
+    instance is twice their sum; it is the arithmetic's, not a tolerance of
+    the method (1.5e-2 relative l2 at N = 65359, 4e-4 at N = 2718)."""
+    torch = torch_mod
+    ns, states, s64, r64 = _c3_batch(torch, torch.float64, 0.3)
+    _, _, s32, r32 = _c3_batch(torch, torch.float32, 0.3)
+    rng = np.random.default_rng(11)
+    pick = [int(np.argmax(ns)), int(np.argmin(ns)),
+            *rng.choice(len(ns), size=12, replace=False).tolist()]
+    for i in pick:
+        n = int(ns[i])
+        u, steps = _c_solve(states, ns, i, 0.3)
+        got = s64.state(i).cpu().numpy()
+        err = _rel(got[3:-3], u[3:-3])
+        assert r64.steps[i] == steps
+        assert err < 1e-15 * max(steps, 1000), (i, n, steps, err)
+        g32 = s32.state(i).double().cpu().numpy()
+        l1 = float(np.sum(np.abs(g32[3:-3] - got[3:-3]))) / n
+        # flux rounding over dx, plus three roundings of the state (<= 2) per
+        # step, with a factor 2 of margin
+        bound = 2.0 * 6.0e-8 * (2.0 * 0.3 * n + 3.0 * 2.0 * r32.steps[i])
+        print(f"[configs] instance {i} n={n} steps {steps}: FP64 rel l2 {err:.3e}; "
+              f"FP32 steps {r32.steps[i]}, L1 from FP64 {l1:.3e} (bound {bound:.3e}, "
+              f"rel l2 {_rel(g32[3:-3], got[3:-3]):.3e})", flush=True)
+        assert l1 < bound, (i, n, l1, bound)
+
+
+def test_adjoint_full_size_vs_central_fd(torch_mod) -> None:
+    """The gradient of the stated adjoint workload against central
+    differences on 32 seeded unit directions. With the states differenced
+    before the sum, rounding bounds the agreement near 1e-9; what remains
+    is the non-smoothness of the Rusanov speed max(|u_j|, |u_j+1|) (near an
+    extremum of |u| the two values differ by ~dx^2 |u''|, less than a 1e-7
+    perturbation moves them), which differences see and the discrete
+    adjoint's tie convention does not: about 1e-12 absolute in every
+    direction, a few 1e-6 of a typical directional derivative. Judged at
+    1e-4 relative to the derivative, or to 1e-2 |grad J| along directions
+    where it nearly cancels."""
+    torch = torch_mod
+    from pyshocks_amd import funcs, make_uniform_cell_grid
+    from pyshocks_amd.adjoint import BatchedCheckpointedSolve, CheckpointedSolve
+    from pyshocks_amd.burgers import Rusanov
+    from pyshocks_amd.reconstruction import WENOJS53
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import BoundOperator
+
+    n, nsteps, every = 16384, 2000, 50
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, device="cuda")
+    src = BoundOperator(Rusanov(rec=WENOJS53(eps=1e-6)), grid, PeriodicBoundary())
+    u0 = funcs.cell_averages(funcs.sine_offset(0.5), grid)
+    dt = 0.5 * grid.h / float(u0[3:-3].abs().max())
+    rng = np.random.default_rng(0)
+    pert = funcs.SineStep(c=0.0, amplitudes=tuple(0.01 * rng.uniform(-1, 1, 4)),
+                          wavenumbers=(1, 2, 3, 4),
+                          phases=tuple(rng.uniform(0, 2 * math.pi, 4)))
+    target = CheckpointedSolve(src, u0 + funcs.cell_averages(pert, grid), nsteps, dt,
+                               every).u_final
+
+    def objective(u):
+        s = CheckpointedSolve(src, u, nsteps, dt, every)
+        d = (s.u_final - target)[3:-3]
+        return 0.5 * float(torch.sum(d * d)) * grid.h, s
+
+    _, solve = objective(u0)
+    grad = solve.gradient((solve.u_final - target) * grid.h).cpu().numpy()
+    assert np.all(np.isfinite(grad))
+    np.testing.assert_array_equal(grad[:3], 0.0)
+    np.testing.assert_array_equal(grad[-3:], 0.0)
+    # the 64 perturbed solves as one batch (one launch per step for all)
+    h = 1.0e-7
+    drng = np.random.default_rng(32)
+    dirs = drng.standard_normal((32, n + 6))
+    dirs[:, :3] = 0.0
+    dirs[:, -3:] = 0.0
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    dd = torch.tensor(dirs, device="cuda")
+    ub = torch.cat([u0[None, :] + h * dd, u0[None, :] - h * dd])
+    fwd = BatchedCheckpointedSolve(src, ub, nsteps, dt, every=every)
+    # J(u+) - J(u-) = 0.5 sum (u+ - u-) (u+ + u- - 2 u_target) dx: the states
+    # are differenced before the sum, so the rounding of J (~1e-16 J, which
+    # is ~1e-4 of 2 h <grad J, d> here) does not enter
+    up = fwd.u_final[:32, 3:-3]
+    um = fwd.u_final[32:, 3:-3]
+    jd = 0.5 * torch.sum((up - um) * (up + um - 2 * target[None, 3:-3]), dim=1) * grid.h
+    gnorm = float(np.linalg.norm(grad))
+    worst = 0.0
+    for k in range(32):
+        fd = float(jd[k]) / (2 * h)
+        ad = float(grad @ dirs[k])
+        # relative to the directional derivative, or to |grad J| for a
+        # direction along which it nearly cancels
+        rel = abs(fd - ad) / max(abs(ad), 1e-2 * gnorm)
+        worst = max(worst, rel)
+        assert rel <= 1e-4, (k, fd, ad, gnorm)
+    # the batched forward agrees with the single solve on one instance
+    one = CheckpointedSolve(src, ub[0].contiguous(), nsteps, dt, every).u_final
+    assert torch.equal(one, fwd.u_final[0])
+    print(f"[configs] adjoint N={n} {nsteps} steps: worst FD rel diff {worst:.3e}",
+          flush=True)
+
+
+@pytest.mark.parametrize("order", [4, 5])
+def test_leggauss_cell_average_vs_oracle(torch_mod, order: int) -> None:
+    """grid.py:393-435: Gauss-Legendre cell averages over every cell of
+    grid.f (the examples take order = scheme.order + 1 = 4, the convergence
+    tests 5) on the device against the oracle's."""
+    torch = torch_mod
+    from oracle import weno_np as W
+    from pyshocks_amd import make_uniform_cell_grid
+    from pyshocks_amd.grid import cell_average, make_leggauss_quadrature
+
+    for n in (256, 1000, 16384):
+        grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, device="cuda")
+        quad = make_leggauss_quadrature(grid, order)
+        got = cell_average(quad, lambda x: 0.5 + torch.sin(2 * torch.pi * x)
+                           + 0.3 * torch.cos(6 * torch.pi * x) ** 2).cpu().numpy()
+        ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+        ref = W.leggauss_cell_average(
+            ogrid, lambda x: 0.5 + np.sin(2 * np.pi * x) + 0.3 * np.cos(6 * np.pi * x) ** 2,
+            order)
+        assert got.shape == ref.shape == (n + 6,)
+        assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))

@@ -218,3 +218,36 @@ def test_advection_solve_vs_oracle(torch_mod) -> None:
     got = _host(ev.u)[2:-2]
     assert ev.iteration == len(dts) == 40
     assert np.linalg.norm(got - uref[2:-2]) / np.linalg.norm(uref[2:-2]) < 1e-12
+
+
+@pytest.mark.parametrize(("scale", "eps"), [(1.0e10, None), (1.0, 1.0e-20), (1.0, 1.0e-23)])
+def test_fused_fp32_weight_scale_free(torch_mod, scale: float, eps) -> None:
+    """The FP32 ES-WENO weight of the fused kernels stays finite and within
+    1e-5 of the FP64 oracle's operator for states of size 1e10 (whose
+    differences, raised to the fourth power, overflow FP32) and for an eps
+    far below the squared differences (1e-20, 1e-23)."""
+    torch = torch_mod
+    from dataclasses import replace
+
+    from oracle import exact as E
+    from oracle import weno_np as W
+    from pyshocks_amd.timestepping import fused_plan, make_ssprk33
+
+    n = 1000
+    sch, grid, bc = _bound(torch, n, torch.float32)
+    if eps is not None:
+        sch = replace(sch, rec=replace(sch.rec, eps=eps))
+    st = make_ssprk33(sch, grid, bc, theta=1.0)
+    plan = fused_plan(st, tiles=(256, 4))
+    assert plan is not None
+    u = scale * E.exact_cell_averages(E.large_initial_condition(seed=4), n, nghosts=2)
+    u32 = u.astype(np.float32)
+    ud = torch.tensor(u32, device="cuda")
+    L = torch.empty_like(ud)
+    plan.stage(0, None, ud, L)
+    got = _host(L)
+    ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=2)
+    osch = W.Scheme("esweno32", eps=sch.rec.eps, rec="esweno32", delta=sch.rec.delta)
+    ref = W.apply_operator(osch, ogrid, 0.0, u32.astype(np.float64))
+    assert np.all(np.isfinite(got))
+    assert _relmax(got[2:-2], ref[2:-2]) < 1e-5

@@ -854,3 +854,48 @@ def test_esweno_advection_constant_velocity_order() -> None:
         errs.append(W.norm(grid, u - ue, p=1, weighted=True))
         hs.append(1.0 / n)
     assert W.estimate_order_of_convergence(np.array(hs), np.array(errs)) >= 2.8
+
+
+# {{{ Gauss-Legendre cell averages (grid.py:393-435)
+
+
+@pytest.mark.parametrize("order", [2, 4, 5])
+def test_leggauss_cell_average_closed_forms(order: int) -> None:
+    """An order-q rule is exact for polynomials of degree 2 q - 1 and not
+    for degree 2 q; on sin(2 pi x) it converges at order 2 q to the exact
+    average (cos(2 pi x_l) - cos(2 pi x_r)) / (2 pi h). A wrong node
+    mapping, a dropped weight or an off-by-one cell fails these."""
+    grid = W.make_uniform_cell_grid(-0.25, 1.0, 37, nghosts=3)
+    f = grid.f
+    h = f[1:] - f[:-1]
+    rng = np.random.default_rng(order)
+
+    coeffs = rng.standard_normal(2 * order)  # degree 2 q - 1
+    poly = np.polynomial.Polynomial(coeffs)
+    prim = poly.integ()
+    got = W.leggauss_cell_average(grid, poly, order)
+    exact = (prim(f[1:]) - prim(f[:-1])) / h
+    np.testing.assert_allclose(got, exact, rtol=1e-12, atol=1e-12)
+
+    # degree 2 q on unit cells, where the rule's error is well above rounding
+    wide = W.make_uniform_cell_grid(-1.0, 2.0, 3, nghosts=1)
+    fw = wide.f
+    top = np.polynomial.Polynomial([0.0] * (2 * order) + [1.0])
+    prim = top.integ()
+    miss = W.leggauss_cell_average(wide, top, order) - (prim(fw[1:]) - prim(fw[:-1]))
+    assert np.min(np.abs(miss)) > 1e-9
+
+    errs = []
+    for n in (16, 32):
+        g = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+        ff = g.f
+        exact = (np.cos(2 * np.pi * ff[:-1]) - np.cos(2 * np.pi * ff[1:])) \
+            / (2 * np.pi * (ff[1:] - ff[:-1]))
+        got = W.leggauss_cell_average(g, lambda x: np.sin(2 * np.pi * x), order)
+        errs.append(np.max(np.abs(got - exact)))
+    if errs[1] > 1e-14:
+        eoc = np.log2(errs[0] / errs[1])
+        assert eoc > 2 * order - 0.3, (errs, eoc)
+
+
+# }}}
