@@ -11,8 +11,10 @@
  *   PSA_FLOAT32). A grid of n interior cells and g = 3 ghost cells per side
  *   has arrays of m = n + 2 g cells; face arrays have m + 1 entries.
  * - The caller owns every buffer. No function keeps a pointer past its return
- *   or allocates device memory; all work is enqueued asynchronously on the
- *   given hipStream_t (passed as void*; NULL is the default stream).
+ *   or allocates device memory (psa_ssprk33_solve_static uses a static
+ *   device array instead of caller buffers); all work is enqueued
+ *   asynchronously on the given hipStream_t (passed as void*; NULL is the
+ *   default stream).
  * - Reference citations are relative to the pyshocks 2025.10 source tree
  *   (src/pyshocks/...), which this library replaces for the hot path.
  */
@@ -594,6 +596,40 @@ int psa_mlp_adam(float *params, const float *grad, float *m, float *v,
 int psa_learned_reconstruct(int dtype, const float *params, double tau,
                             double sigma, const void *u, void *ul, void *ur,
                             int64_t m, double eps, void *stream);
+
+/* Solve without allocating --------------------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): psa_control_init followed by psa_ssprk33_steps, with the
+ * control block and both stage buffers taken from a static device array of
+ * PSA_STATIC_SCRATCH_BYTES that every GPU the library runs on holds until the
+ * process ends. The caller passes only the input u_in and the output u_out
+ * (m = n + 2 g cells each, distinct or equal) and, optionally, result
+ * (device memory, one psa_static_result: t, the dt of the last step taken,
+ * the steps taken and the status after the last enqueued step; NULL to
+ * skip). Arguments as psa_control_init and psa_ssprk33_steps; the results
+ * are the same bits as theirs.
+ * Limits: m <= PSA_STATIC_MAX_CELLS (either dtype), otherwise
+ * PSA_ERR_WORKSPACE. The first call on a device resolves that device's copy
+ * of the array (hipGetSymbolAddress), so make it outside stream capture.
+ * Every call on one GPU shares that GPU's array and it keeps the last call's
+ * contents: calls that overlap on one GPU (other streams, other host
+ * threads) race. Nothing here calls hipMalloc or hipFree. */
+#define PSA_STATIC_MAX_CELLS (1 << 20)
+#define PSA_STATIC_SCRATCH_BYTES \
+  (24832 + 2 * (size_t)PSA_STATIC_MAX_CELLS * 8) /* control + 2 stages */
+typedef struct psa_static_result {
+  double t;      /* time of the final state                     */
+  double dt;     /* dt of the last step taken                   */
+  long long m;   /* steps taken                                 */
+  int status;    /* PSA_OK or PSA_ERR_NONFINITE_DT              */
+  int pad_;
+} psa_static_result;
+int psa_ssprk33_solve_static(const psa_solver_desc *desc, const void *u_in,
+                             void *u_out, double tstart, double tfinal,
+                             long long maxit, int dt_mode, double theta,
+                             double cfl_num, double fixed_dt, long long nsteps,
+                             void *result, void *stream);
 
 #ifdef __cplusplus
 }

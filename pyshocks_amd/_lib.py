@@ -86,6 +86,20 @@ class PsaControl(ctypes.Structure):
     ]
 
 
+class PsaStaticResult(ctypes.Structure):
+    _fields_ = [
+        ("t", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("m", ctypes.c_longlong),
+        ("status", ctypes.c_int),
+        ("pad_", ctypes.c_int),
+    ]
+
+
+#: m limit of psa_ssprk33_solve_static (cells of either dtype)
+PSA_STATIC_MAX_CELLS = 1 << 20
+
+
 class PsaSolverDesc(ctypes.Structure):
     _fields_ = [
         ("dtype", ctypes.c_int),
@@ -221,6 +235,8 @@ SIGNATURES: dict[str, list[Any]] = {
     "psa_mlp_grad": [_VP, _VP, _LL, _D, _D, _VP, _VP, ctypes.c_size_t, _VP],
     "psa_mlp_adam": [_VP, _VP, _VP, _VP, _LL, _D, _D, _D, _D, _VP],
     "psa_learned_reconstruct": [_I, _VP, _D, _D, _VP, _VP, _VP, _I64, _D, _VP],
+    "psa_ssprk33_solve_static": [ctypes.POINTER(PsaSolverDesc), _VP, _VP, _D, _D,
+                                 _LL, _I, _D, _D, _D, _LL, _VP, _VP],
 }
 
 TEAM_STATE_BYTES = 64

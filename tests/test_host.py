@@ -314,3 +314,27 @@ def test_c_oracle_matches_numpy_oracle() -> None:
         np.testing.assert_allclose(dtc, dts, rtol=1e-14, atol=1e-16)
         i = grid.i_
         assert np.linalg.norm(uc[i] - u[i]) / np.linalg.norm(u[i]) < 1e-13
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def test_static_solve_limits() -> None:
+    """psa_ssprk33_solve_static refuses arrays beyond its static scratch and
+    bad arguments before it touches a device."""
+    from pyshocks_amd import _lib
+
+    lib = _lib.load()
+    assert ctypes.sizeof(_lib.PsaStaticResult) == 32
+    desc = _lib.PsaSolverDesc()
+    desc.dtype, desc.flux, desc.g, desc.dx, desc.eps = 0, 0, 3, 1.0, 1e-6
+    desc.n = _lib.PSA_STATIC_MAX_CELLS - 5  # m = n + 6 > the limit
+    st = lib.psa_ssprk33_solve_static(ctypes.byref(desc), 1, 1, 0.0, 1.0, -1,
+                                      _lib.PSA_DT_CFL, 1.0, 0.5, 0.0, 1, None, None)
+    assert st == _lib.PSA_ERR_WORKSPACE
+    assert "PSA_STATIC_MAX_CELLS" in _lib.last_error()
+    desc.n = 256
+    assert lib.psa_ssprk33_solve_static(ctypes.byref(desc), None, 1, 0.0, 1.0, -1,
+                                        _lib.PSA_DT_CFL, 1.0, 0.5, 0.0, 1, None,
+                                        None) == _lib.PSA_ERR_INVALID_ARGUMENT
+    assert lib.psa_ssprk33_solve_static(ctypes.byref(desc), 1, 1, 0.0, 1.0, -1,
+                                        _lib.PSA_DT_CFL, 1.0, 0.5, 0.0, -1, None,
+                                        None) == _lib.PSA_ERR_INVALID_ARGUMENT
