@@ -500,7 +500,15 @@ int psa_batch_kernel_info(const psa_batch_desc *desc, int *vgprs,
  * input and zero on output. Supported: Rusanov (alpha = 1), Burgers upwind
  * and advection with a constant velocity; dt is a device scalar held
  * constant. Non-smooth points follow jax's conventions (max ties split
- * 0.5 / 0.5, sign(0) = 0, jnp.where differentiates the selected branch). */
+ * 0.5 / 0.5, sign(0) = 0, jnp.where differentiates the selected branch).
+ * Rusanov and Burgers upwind also take the learned reconstruction (desc->mlp,
+ * no counterpart in the reference): the transposed Jacobian is that of the
+ * forward solver's surrogate with the regimes held fixed at the forward
+ * state: shock cells differentiate WENO-JS, smooth cells the ideal weights
+ * (a linear map), learned cells the network with its weights rounded to
+ * BF16 as the forward rounds them and its activations and inputs in the
+ * solver's precision (the forward rounds those to BF16, a step function
+ * whose derivative is zero almost everywhere). */
 
 /* Reverse of one stage. stage 0: out = J_L(state)^T mu. stage 3:
  * out = 2/3 (mu + dt J_L(state)^T mu) with state = k2 and mu the step

@@ -104,3 +104,122 @@ def operator_jacobian(kind: str, grid: W.Grid, u: np.ndarray, *, eps: float,
         out = apply_operator_cs(kind, grid, z, eps=eps, velocity=velocity)
         J[:, j] = out[g:g + n].imag / h
     return J
+
+
+# {{{ the learned reconstruction's surrogate (complex step)
+
+
+def _learned_features_cs(v: np.ndarray) -> np.ndarray:
+    """``x = (v - mean) / max|v - mean|`` on complex stencils (n, 5), the
+    maximum and its sign taken on the real parts (``oracle.mlp_np.features``
+    without the FP32 rounding)."""
+    mean = (((v[:, 0] + v[:, 4]) + (v[:, 1] + v[:, 3])) + v[:, 2]) * 0.2
+    d = v - mean[:, None]
+    km = np.argmax(np.abs(d.real), axis=1)
+    dm = d[np.arange(d.shape[0]), km]
+    s = dm * np.sign(dm.real)
+    ok = s.real > 0
+    return np.where(ok[:, None], d / np.where(ok, s, 1.0)[:, None], 0.0)
+
+
+def _learned_weights_cs(p: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """Softmax weights of the network on complex inputs: W1, W2, W3 rounded
+    to BF16 as the kernels' operands, b3 + ln d summed in FP32 as their
+    accumulator start, inputs and activations unrounded, ReLU on the real
+    parts."""
+    from oracle import mlp_np as M
+
+    P = M.unpack(p)
+    W1, W2, W3 = (M.bf16(P[k]).astype(np.float64) for k in ("W1", "W2", "W3"))
+    c3 = (P["b3"] + M.LN_D32).astype(np.float32).astype(np.float64)
+    a1 = x @ W1.T + P["b1"].astype(np.float64)
+    h1 = np.where(a1.real > 0, a1, 0.0)
+    a2 = h1 @ W2.T + P["b2"].astype(np.float64)
+    h2 = np.where(a2.real > 0, a2, 0.0)
+    z = h2 @ W3.T + c3
+    z = z - np.max(z.real, axis=1, keepdims=True)
+    e = np.exp(z)
+    return e / np.sum(e, axis=1, keepdims=True)
+
+
+def learned_reconstruct_cs(p: np.ndarray, tau: float, sigma: float, u: np.ndarray,
+                           eps: float) -> tuple[np.ndarray, np.ndarray]:
+    """``(ul, ur)`` of the learned reconstruction's surrogate on complex
+    input: WENO-JS where the real stencil is in the shock regime, the ideal
+    weights in the smooth regime, the network (``_learned_weights_cs``)
+    otherwise; regimes from ``oracle.mlp_np.regimes`` of the real parts."""
+    from oracle import mlp_np as M
+
+    v = M.stencils(u)
+    reg = M.regimes(v.real, eps, tau, sigma)
+    vm = v[:, ::-1]
+    wr = _learned_weights_cs(p, _learned_features_cs(v))
+    wl = _learned_weights_cs(p, _learned_features_cs(vm))
+    cr, cl = M.candidates(v), M.candidates(vm)
+    ideal = np.broadcast_to(M.IDEAL, cr.shape)
+    jl, jr = W.reconstruct_wenojs(u, eps)
+    ur = np.where(reg == 0, jr, np.where(reg == 1, M._combine(ideal, cr), M._combine(wr, cr)))
+    ul = np.where(reg == 0, jl, np.where(reg == 1, M._combine(ideal, cl), M._combine(wl, cl)))
+    return ul, ur
+
+
+def learned_operator_cs(p: np.ndarray, tau: float, sigma: float, kind: str,
+                        grid: W.Grid, x: np.ndarray, *, eps: float) -> np.ndarray:
+    """The Burgers operator (rusanov or godunov) of the learned surrogate on
+    complex input (``apply_operator_cs`` with ``learned_reconstruct_cs``)."""
+    x = W.apply_boundary_periodic(grid, x)
+    ul, ur = learned_reconstruct_cs(p, tau, sigma, x, eps)
+    fl, fr = ul**2 / 2, ur**2 / 2
+    if kind == "rusanov":
+        a = _abs_cs(x)
+        a = np.where(a[1:].real >= a[:-1].real, a[1:], a[:-1])
+        fnum = 0.5 * (fl[1:] + fr[:-1]) - 0.5 * a * (ul[1:] - ur[:-1])
+    elif kind == "godunov":
+        aavg = (ur[:-1] + ul[1:]) / 2
+        fnum = np.where(aavg.real > 0, fr[:-1], fl[1:])
+    else:
+        raise NotImplementedError(kind)
+    F = np.pad(fnum, 1)
+    return -(F[1:] - F[:-1]) / grid.dx
+
+
+def learned_operator_jacobian(p: np.ndarray, tau: float, sigma: float, kind: str,
+                              grid: W.Grid, u: np.ndarray, *, eps: float,
+                              h: float = 1.0e-30) -> np.ndarray:
+    """Dense interior Jacobian of the learned surrogate's operator by complex
+    step."""
+    g = grid.nghosts
+    n = grid.n - 2 * g
+    J = np.empty((n, n))
+    base = W.apply_boundary_periodic(grid, np.asarray(u, dtype=np.float64))
+    for j in range(n):
+        z = base.astype(np.complex128)
+        z[g + j] += 1j * h
+        z = W.apply_boundary_periodic(grid, z)
+        J[:, j] = learned_operator_cs(p, tau, sigma, kind, grid, z, eps=eps)[g:g + n].imag / h
+    return J
+
+
+def learned_step_jacobian(p: np.ndarray, tau: float, sigma: float, kind: str,
+                          grid: W.Grid, u: np.ndarray, dt: float, *, eps: float,
+                          h: float = 1.0e-30) -> np.ndarray:
+    """Dense interior Jacobian of one SSP-RK3 step of the learned surrogate
+    (Burgers, ``kind`` rusanov or godunov) by complex step."""
+
+    def operator(t, x):
+        return learned_operator_cs(p, tau, sigma, kind, grid, x, eps=eps)
+
+    g = grid.nghosts
+    n = grid.n - 2 * g
+    J = np.empty((n, n))
+    base = W.apply_boundary_periodic(grid, np.asarray(u, dtype=np.float64))
+    for j in range(n):
+        z = base.astype(np.complex128)
+        z[g + j] += 1j * h
+        z = W.apply_boundary_periodic(grid, z)
+        out = W.ssprk33_advance(operator, dt, 0.0, z)
+        J[:, j] = out[g:g + n].imag / h
+    return J
+
+
+# }}}

@@ -20,6 +20,14 @@
 //
 // The state and the cotangents are periodic: every index is mapped to its
 // interior image, which also covers the 5-cell halo the transpose needs.
+//
+// Learned weights (LEARNED kernels): each cell takes its forward regime (from
+// the same fused indicator arithmetic as the forward kernels); shock cells
+// differentiate WENO-JS, smooth cells the ideal weights, learned cells the
+// network. The network is differentiated as the smooth function it rounds:
+// BF16 parameters (as the forward's operands), inputs and activations
+// unrounded, in the state dtype -- the adjoint of that surrogate along the
+// forward's trajectory.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -28,6 +36,7 @@
 #include <string>
 
 #include "../../include/pyshocks_amd.h"
+#include "mlp_device.hpp"
 #include "psa_internal.hpp"
 #include "weno5_device.hpp"
 
@@ -109,7 +118,196 @@ struct VjpArgs {
   T inv_dx;
   T eps;
   T vconst;
+  const float *mlp;   // LEARNED: the network's parameters (PSA_MLP_NPARAM FP32)
+  T eps4;             // LEARNED: 4 eps (the fused indicators carry 4 beta)
+  T tau;              // LEARNED: shock threshold
+  T sigma;            // LEARNED: smooth threshold
 };
+
+// Regime of a cell (mlp_regime) from its right stencil with the indicator
+// arithmetic of the forward kernels (reconstruct_cells: shared first and
+// second differences, 4 beta, one multiply-add each), so that the reverse
+// kernels see the regime the forward took.
+template <typename T>
+__device__ __forceinline__ int learned_regime5(T v0, T v1, T v2, T v3, T v4, T eps4,
+                                               T tau, T sigma) {
+  using C = Weno5Coeffs<T>;
+  const T D0 = v1 - v0, D1 = v2 - v1, D2 = v3 - v2, D3 = v4 - v3;
+  const T s0 = D1 - D0, s1 = D2 - D1, s2 = D3 - D2;
+  const T E0 = C::a0x4 * (s0 * s0), E1 = C::a0x4 * (s1 * s1), E2 = C::a0x4 * (s2 * s2);
+  const T t01 = fma(T(2), D1, s0);
+  const T t11 = D2 + D1;
+  const T t21 = fma(T(-2), D2, s2);
+  return mlp_regime(fma(t01, t01, E0), fma(t11, t11, E1), fma(t21, t21, E2), eps4, tau,
+                    sigma);
+}
+
+// Gradient of the candidates with respect to the stencil: G += sum_k c_k
+// d uhat_k / dv (weno.py:231-239 taps in the order of weno_side_grad).
+template <typename T>
+__device__ __forceinline__ void uhat_grad(T c0, T c1, T c2, T G[5]) {
+  using C = Weno5Coeffs<T>;
+  G[0] += c0 * C::c2;
+  G[1] += -(c0 * C::c7) - c1 * C::c1;
+  G[2] += (c0 * C::c11 + c1 * C::c5) + c2 * C::c2;
+  G[3] += c1 * C::c2 + c2 * C::c5;
+  G[4] += -(c2 * C::c1);
+}
+
+// Ideal-weight reconstruction (mlp_combine_ideal) and its stencil gradient.
+template <typename T>
+__device__ __forceinline__ T ideal_side_grad(T v0, T v1, T v2, T v3, T v4, T G[5]) {
+  T h0, h1, h2;
+  weno_uhat(v0, v1, v2, v3, v4, h0, h1, h2);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) G[k] = T(0);
+  uhat_grad(T(0.1), T(1) - T(0.1) - T(0.3), T(0.3), G);
+  return mlp_combine_ideal(h0, h1, h2);
+}
+
+// A weight as the forward kernels' BF16 operand (round to nearest even, on
+// the bit pattern: the parameters are uniform, so this stays in scalar
+// registers), as the state dtype.
+template <typename T>
+__device__ __forceinline__ T mlp_weight(const float *__restrict__ P, int i);
+
+template <>
+__device__ __forceinline__ float mlp_weight<float>(const float *__restrict__ P, int i) {
+  unsigned int b = __float_as_uint(P[i]);
+  b = (b + 0x7FFFu + ((b >> 16) & 1u)) & 0xFFFF0000u;
+  return __uint_as_float(b);
+}
+
+// FP64: the BF16 value's exact FP64 bits built from its FP32 bits (sign,
+// exponent rebiased by 1023 - 127, the 7 mantissa bits; BF16 values below
+// the FP32 normal range are taken as zero), so that the weight stays a
+// scalar operand of the multiply-adds.
+template <>
+__device__ __forceinline__ double mlp_weight<double>(const float *__restrict__ P, int i) {
+  unsigned int b = __float_as_uint(P[i]);
+  b = (b + 0x7FFFu + ((b >> 16) & 1u)) & 0xFFFF0000u;
+  const unsigned int e = (b >> 23) & 0xFFu;
+  const unsigned int hi =
+      e == 0u ? (b & 0x80000000u)
+              : ((b & 0x80000000u) | ((e + 896u) << 20) | ((b >> 3) & 0x000FFFFFu));
+  return __hiloint2double(static_cast<int>(hi), 0);
+}
+
+// Network reconstruction of one side, r = h1 + w0 (h0 - h1) + w2 (h2 - h1)
+// with w = softmax(z) of the network on x = (v - mean) / max|v - mean|, and
+// its stencil gradient: the candidates' part with (w0, 1 - w0 - w2, w2), the
+// weights' part back through softmax, the three layers (ReLU masks of this
+// evaluation) and the normalisation. P: the parameters (weights rounded as
+// mlp_weight, b3 + ln d summed in FP32 as the forward's accumulator start).
+// Layer 2 runs row by row, so 32 values of T are live.
+template <typename T>
+__device__ __forceinline__ T learned_side_grad(const float *__restrict__ P, T v0, T v1,
+                                               T v2, T v3, T v4, T G[5]) {
+  const T v[5] = {v0, v1, v2, v3, v4};
+  const T mean = (((v0 + v4) + (v1 + v3)) + v2) * T(0.2);
+  T d[5], x[5];
+  int km = 0;
+  T s = T(0);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    d[k] = v[k] - mean;
+    if (fabs(d[k]) > s) {
+      s = fabs(d[k]);
+      km = k;
+    }
+  }
+  const T inv = s > T(0) ? T(1) / s : T(0);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) x[k] = d[k] * inv;
+
+  // forward: h1 = relu(W1 x + b1), z = W3 relu(W2 h1 + b2) + b3 + ln d
+  T h1[MLP_HID];
+  unsigned int m1 = 0, m2 = 0;
+#pragma unroll
+  for (int j = 0; j < MLP_HID; ++j) {
+    T a = static_cast<T>(P[MLP_B1 + j]);
+#pragma unroll
+    for (int k = 0; k < MLP_IN; ++k) a = fma(mlp_weight<T>(P, MLP_W1 + j * MLP_IN + k), x[k], a);
+    m1 |= a > T(0) ? (1u << j) : 0u;
+    h1[j] = a > T(0) ? a : T(0);
+  }
+  T z[MLP_OUT];
+  const float lnd[3] = {MLP_LN_D0, MLP_LN_D1, MLP_LN_D2};
+#pragma unroll
+  for (int o = 0; o < MLP_OUT; ++o) z[o] = static_cast<T>(P[MLP_B3 + o] + lnd[o]);
+  // rows of layer 2 one at a time (a rolled loop: the row's 32 weights are
+  // loaded when it runs)
+#pragma unroll 1
+  for (int i = 0; i < MLP_HID; ++i) {
+    T a = static_cast<T>(P[MLP_B2 + i]);
+#pragma unroll
+    for (int j = 0; j < MLP_HID; ++j) a = fma(mlp_weight<T>(P, MLP_W2 + i * MLP_HID + j), h1[j], a);
+    if (a > T(0)) {
+      m2 |= 1u << i;
+#pragma unroll
+      for (int o = 0; o < MLP_OUT; ++o)
+        z[o] = fma(mlp_weight<T>(P, MLP_W3 + o * MLP_HID + i), a, z[o]);
+    }
+  }
+  const T zm = fmax(fmax(z[0], z[1]), z[2]);
+  const T e0 = exp(z[0] - zm), e1 = exp(z[1] - zm), e2 = exp(z[2] - zm);
+  const T es = (e0 + e1) + e2;
+  const T w0 = e0 / es, w1 = e1 / es, w2 = e2 / es;
+  T h0c, h1c, h2c;
+  weno_uhat(v0, v1, v2, v3, v4, h0c, h1c, h2c);
+  const T r = (h1c + w0 * (h0c - h1c)) + w2 * (h2c - h1c);
+
+#pragma unroll
+  for (int k = 0; k < 5; ++k) G[k] = T(0);
+  uhat_grad(w0, (T(1) - w0) - w2, w2, G);
+
+  // backward: dr/dw = (h0 - h1, 0, h2 - h1), through softmax
+  const T c0 = h0c - h1c, c2 = h2c - h1c;
+  const T sw = c0 * w0 + c2 * w2;
+  const T zb[MLP_OUT] = {w0 * (c0 - sw), w1 * (T(0) - sw), w2 * (c2 - sw)};
+  // h1b = W2^T (mask2 * W3^T zb), then xb = W1^T (mask1 * h1b)
+  T h1b[MLP_HID];
+#pragma unroll
+  for (int j = 0; j < MLP_HID; ++j) h1b[j] = T(0);
+#pragma unroll 1
+  for (int i = 0; i < MLP_HID; ++i) {
+    if (m2 & (1u << i)) {
+      T h2b = T(0);
+#pragma unroll
+      for (int o = 0; o < MLP_OUT; ++o)
+        h2b = fma(mlp_weight<T>(P, MLP_W3 + o * MLP_HID + i), zb[o], h2b);
+#pragma unroll
+      for (int j = 0; j < MLP_HID; ++j)
+        h1b[j] = fma(mlp_weight<T>(P, MLP_W2 + i * MLP_HID + j), h2b, h1b[j]);
+    }
+  }
+  T xb[MLP_IN] = {T(0), T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int j = 0; j < MLP_HID; ++j) {
+    if (m1 & (1u << j)) {
+#pragma unroll
+      for (int k = 0; k < MLP_IN; ++k)
+        xb[k] = fma(mlp_weight<T>(P, MLP_W1 + j * MLP_IN + k), h1b[j], xb[k]);
+    }
+  }
+  if (s > T(0)) {
+    // x_k = d_k / s: dx_k/dv_j = (delta_kj - 1/5) / s - x_k sgn(d_km)
+    // (delta_km,j - 1/5) / s
+    T sxb = T(0), dot = T(0);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      sxb += xb[k];
+      dot += xb[k] * x[k];
+    }
+    const T sg = d[km] > T(0) ? T(1) : T(-1);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const T ds = sg * ((j == km ? T(1) : T(0)) - T(0.2));
+      G[j] += inv * ((xb[j] - T(0.2) * sxb) - dot * ds);
+    }
+  }
+  return r;
+}
 
 // Scratch of one reverse stage over a block of B computing cells.
 template <typename T>
@@ -124,7 +322,7 @@ struct VjpScratch {
 // Z + B - 3 (both in LDS, visible to the block). Thread t computes the
 // reconstructions of cell Z - 3 + t and returns the gathered sum of output
 // cell Z + t (valid for t < B - 6). Ends with the scratch in use.
-template <typename T, int FLUX>
+template <typename T, int FLUX, bool LEARNED = false>
 __device__ __forceinline__ T vjp_block(const VjpArgs<T> &p, const T *s_u,
                                        const T *s_mu, VjpScratch<T> &sc) {
   constexpr int B = kVjpBlock;
@@ -134,8 +332,38 @@ __device__ __forceinline__ T vjp_block(const VjpArgs<T> &p, const T *s_u,
   T GR[5], GLm[5];
   const T v0 = s_u[tid], v1 = s_u[tid + 1], v2 = s_u[tid + 2],
           v3 = s_u[tid + 3], v4 = s_u[tid + 4];
-  sc.ur[tid] = weno_side_grad(v0, v1, v2, v3, v4, p.eps, GR);
-  sc.ul[tid] = weno_side_grad(v4, v3, v2, v1, v0, p.eps, GLm);
+  if constexpr (LEARNED) {
+    const int regime = learned_regime5(v0, v1, v2, v3, v4, p.eps4, p.tau, p.sigma);
+    if (regime == 0) {
+      sc.ur[tid] = weno_side_grad(v0, v1, v2, v3, v4, p.eps, GR);
+      sc.ul[tid] = weno_side_grad(v4, v3, v2, v1, v0, p.eps, GLm);
+    } else if (regime == 1) {
+      sc.ur[tid] = ideal_side_grad(v0, v1, v2, v3, v4, GR);
+      sc.ul[tid] = ideal_side_grad(v4, v3, v2, v1, v0, GLm);
+    } else {
+      // the right side, then the left one from the mirrored stencil: one
+      // copy of the network code, run twice
+#pragma unroll 1
+      for (int side = 0; side < 2; ++side) {
+        const bool l = side != 0;
+        T G[5];
+        const T r = learned_side_grad(p.mlp, l ? v4 : v0, l ? v3 : v1, v2, l ? v1 : v3,
+                                      l ? v0 : v4, G);
+        if (l) {
+          sc.ul[tid] = r;
+#pragma unroll
+          for (int k = 0; k < 5; ++k) GLm[k] = G[k];
+        } else {
+          sc.ur[tid] = r;
+#pragma unroll
+          for (int k = 0; k < 5; ++k) GR[k] = G[k];
+        }
+      }
+    }
+  } else {
+    sc.ur[tid] = weno_side_grad(v0, v1, v2, v3, v4, p.eps, GR);
+    sc.ul[tid] = weno_side_grad(v4, v3, v2, v1, v0, p.eps, GLm);
+  }
   if (tid == 0) {
     sc.cL[0] = T(0);
     sc.sR[0] = T(0);
@@ -221,7 +449,7 @@ __device__ __forceinline__ T vjp_value(T mu, T gsum, T dt, T lam, T lamk2) {
 
 // RSTAGE 0: out = JL(state)^T mu; 3, 2, 1: the reverse stages above, with
 // mu = l, lk2, lk1 respectively.
-template <typename T, int FLUX, int RSTAGE>
+template <typename T, int FLUX, int RSTAGE, bool LEARNED = false>
 __global__ __launch_bounds__(kVjpBlock) void k_stage_vjp(VjpArgs<T> p) {
   constexpr int B = kVjpBlock;
   __shared__ T s_u[B + 4];   // cells S - 5 ..
@@ -243,7 +471,7 @@ __global__ __launch_bounds__(kVjpBlock) void k_stage_vjp(VjpArgs<T> p) {
   for (int j = tid; j < B + 1; j += B) s_mu[j] = load_periodic(p.mu, S - 3 + j, n, g);
   __syncthreads();
 
-  const T gsum = vjp_block<T, FLUX>(p, s_u, s_mu, sc);
+  const T gsum = vjp_block<T, FLUX, LEARNED>(p, s_u, s_mu, sc);
   if (tid < kVjpOut) {
     const int64_t j = S + tid;
     if (j < p.m - g) {
@@ -275,7 +503,7 @@ constexpr int kStepVjpOut = kVjpBlock - 18;  // output cells per workgroup
 // reverse stages 3, 2 and 1 in LDS (vjp_block each): lk2 is exact on
 // [S - 6, S + B - 12), lk1 on [S - 3, S + B - 15), the step's cotangent on the
 // output cells [S, S + B - 18). The same arithmetic as k_stage_vjp.
-template <typename T, int FLUX>
+template <typename T, int FLUX, bool LEARNED = false>
 __global__ __launch_bounds__(kVjpBlock) void k_step_vjp(StepVjpArgs<T> a) {
   constexpr int B = kVjpBlock;
   __shared__ T sK2[B + 4], sK1[B + 4], sX[B + 4];
@@ -313,15 +541,15 @@ __global__ __launch_bounds__(kVjpBlock) void k_step_vjp(StepVjpArgs<T> a) {
   __syncthreads();
 
   // reverse stage 3 on the output cells S - 6 + t: lk2
-  T gsum = vjp_block<T, FLUX>(p, sK2, sLam, sc);
+  T gsum = vjp_block<T, FLUX, LEARNED>(p, sK2, sLam, sc);
   if (tid < B - 6) sLk2[tid] = vjp_value<T, 3>(sLam[tid + 3], gsum, dt, T(0), T(0));
   __syncthreads();
   // reverse stage 2 on S - 3 + t: lk1
-  gsum = vjp_block<T, FLUX>(p, sK1, sLk2, sc);
+  gsum = vjp_block<T, FLUX, LEARNED>(p, sK1, sLk2, sc);
   if (tid < B - 6) sLk1[tid] = vjp_value<T, 2>(sLk2[tid + 3], gsum, dt, T(0), T(0));
   __syncthreads();
   // reverse stage 1 on S + t: the cotangent of the step input
-  gsum = vjp_block<T, FLUX>(p, sX, sLk1, sc);
+  gsum = vjp_block<T, FLUX, LEARNED>(p, sX, sLk1, sc);
   if (tid < kStepVjpOut) {
     const int64_t j = S + tid;
     if (j < p.m - g)
@@ -339,25 +567,25 @@ __global__ void k_zero_ghosts(T *u, int64_t m, int g, int64_t ld) {
   }
 }
 
-template <typename T, int FLUX>
+template <typename T, int FLUX, bool LEARNED = false>
 int launch_vjp_flux(int rstage, const VjpArgs<T> &a, hipStream_t s) {
   const unsigned int blocks =
       static_cast<unsigned int>((a.n + kVjpOut - 1) / kVjpOut);
   switch (rstage) {
     case 0:
-      hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 0>), dim3(blocks),
+      hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 0, LEARNED>), dim3(blocks),
                          dim3(kVjpBlock), 0, s, a);
       break;
     case 1:
-      hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 1>), dim3(blocks),
+      hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 1, LEARNED>), dim3(blocks),
                          dim3(kVjpBlock), 0, s, a);
       break;
     case 2:
-      hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 2>), dim3(blocks),
+      hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 2, LEARNED>), dim3(blocks),
                          dim3(kVjpBlock), 0, s, a);
       break;
     default:
-      hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 3>), dim3(blocks),
+      hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 3, LEARNED>), dim3(blocks),
                          dim3(kVjpBlock), 0, s, a);
       break;
   }
@@ -381,6 +609,15 @@ int launch_vjp(const psa_solver_desc *d, int rstage, const void *state,
   a.inv_dx = T(1) / static_cast<T>(d->dx);
   a.eps = static_cast<T>(d->eps);
   a.vconst = static_cast<T>(d->velocity_const);
+  a.mlp = d->mlp;
+  a.eps4 = T(4) * a.eps;
+  a.tau = static_cast<T>(d->mlp_tau);
+  a.sigma = static_cast<T>(d->mlp_sigma);
+  if (d->mlp != nullptr) {
+    if (d->flux == PSA_FLUX_RUSANOV)
+      return launch_vjp_flux<T, PSA_FLUX_RUSANOV, true>(rstage, a, s);
+    return launch_vjp_flux<T, PSA_FLUX_BURGERS_UPWIND, true>(rstage, a, s);
+  }
   if (d->flux == PSA_FLUX_RUSANOV)
     return launch_vjp_flux<T, PSA_FLUX_RUSANOV>(rstage, a, s);
   if (d->flux == PSA_FLUX_BURGERS_UPWIND)
@@ -397,6 +634,10 @@ int launch_step_vjp(const psa_solver_desc *d, const void *x, const void *k1,
                     hipStream_t s, long long batch = 1, int64_t ld = 0) {
   StepVjpArgs<T> a{};
   a.bstride = ld;
+  a.v.mlp = d->mlp;
+  a.v.eps4 = T(4) * static_cast<T>(d->eps);
+  a.v.tau = static_cast<T>(d->mlp_tau);
+  a.v.sigma = static_cast<T>(d->mlp_sigma);
   a.v.dt_ptr = static_cast<const T *>(dt);
   a.v.n = d->n;
   a.v.g = d->g;
@@ -411,6 +652,16 @@ int launch_step_vjp(const psa_solver_desc *d, const void *x, const void *k1,
   a.out = static_cast<T *>(out);
   const dim3 grid(static_cast<unsigned int>((d->n + kStepVjpOut - 1) / kStepVjpOut),
                   static_cast<unsigned int>(batch));
+  if (d->mlp != nullptr) {
+    if (d->flux == PSA_FLUX_RUSANOV) {
+      hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_RUSANOV, true>), grid, dim3(kVjpBlock), 0,
+                         s, a);
+    } else {
+      hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_BURGERS_UPWIND, true>), grid,
+                         dim3(kVjpBlock), 0, s, a);
+    }
+    return check_launch("learned step vjp");
+  }
   if (d->flux == PSA_FLUX_RUSANOV) {
     hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_RUSANOV>), grid, dim3(kVjpBlock), 0, s, a);
   } else if (d->flux == PSA_FLUX_BURGERS_UPWIND) {
@@ -448,9 +699,12 @@ int check_vjp_desc(const psa_solver_desc *d) {
   if (d->flux == PSA_FLUX_ADVECTION_UPWIND && !d->velocity_is_const)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "the adjoint supports advection with a constant velocity");
-  if (d->mlp != nullptr)
+  if (d->mlp != nullptr && d->flux != PSA_FLUX_RUSANOV &&
+      d->flux != PSA_FLUX_BURGERS_UPWIND)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
-                     "the adjoint supports the WENO-JS weights only");
+                     "the adjoint of learned weights takes the Rusanov and Godunov fluxes");
+  if (d->mlp != nullptr && (!(d->mlp_tau >= 0.0) || !(d->mlp_sigma >= 0.0)))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "mlp_tau and mlp_sigma must be >= 0");
   if (!(d->dx > 0.0))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "dx must be positive");
   return PSA_OK;
@@ -541,6 +795,7 @@ int psa_ssprk33_step_vjp(const psa_solver_desc *desc, const void *u,
   if (st != PSA_OK) return st;
   st = psa_ssprk33_stage(desc, 2, u, k1, k2, nullptr, dt, stream);
   if (st != PSA_OK) return st;
+
   st = psa_ssprk33_stage_vjp(desc, 3, k2, lam, nullptr, nullptr, lk2, dt, stream);
   if (st != PSA_OK) return st;
   st = psa_ssprk33_stage_vjp(desc, 2, k1, lk2, nullptr, nullptr, lk1, dt, stream);

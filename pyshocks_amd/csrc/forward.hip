@@ -1235,8 +1235,7 @@ int step_fixed_dt(const psa_solver_desc *d, const void *u_in, void *u_out, void 
                   int64_t ld) {
   const TileConfig cfg = tiles_of(d);
   const int vel = vel_mode(d);
-  if (d->mlp != nullptr || !step_applies(d, cfg, vel))
-    return PSA_ERR_NOT_IMPLEMENTED;
+  if (!step_applies(d, cfg, vel)) return PSA_ERR_NOT_IMPLEMENTED;
   if (batch < 1 || batch > 65535)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "batch must be in [1, 65535]");
   if (d->dtype == PSA_FLOAT64) {

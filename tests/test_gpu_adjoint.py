@@ -344,3 +344,130 @@ def test_batched_adjoint_equals_single(torch_mod, kind: str) -> None:
         s = CheckpointedSolve(src, u0[b].contiguous(), nsteps, dts[b], every)
         assert torch.equal(bs.u_final[b], s.u_final)
         assert torch.equal(gb[b], s.gradient(lam[b].contiguous()))
+
+
+# {{{ learned weights
+
+
+def _learned_source(torch, kind: str, n: int, p, tau: float, sigma: float):
+    from pyshocks_amd import burgers, make_uniform_cell_grid
+    from pyshocks_amd.learned import MLPWeights
+    from pyshocks_amd.reconstruction import LearnedWENOJS53
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import BoundOperator
+
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    rec = LearnedWENOJS53(eps=1e-6, weights=MLPWeights.from_numpy(p, tau, sigma))
+    sch = (burgers.Rusanov if kind == "rusanov" else burgers.Godunov)(rec=rec)
+    return BoundOperator(sch, grid, PeriodicBoundary()), grid
+
+
+def _learned_case(n: int, seed: int):
+    """Parameters with BF16-representable weights (so that the oracle's
+    surrogate has the kernels' operands) and a state with smooth parts, a
+    step and noise: shock, smooth and learned cells all occur."""
+    from oracle import mlp_np as M
+
+    rng = np.random.default_rng(seed)
+    p = (0.3 * rng.standard_normal(M.NPARAM)).astype(np.float32)
+    P = M.unpack(p)
+    p = M.pack(M.bf16(P["W1"]), P["b1"], M.bf16(P["W2"]), P["b2"], M.bf16(P["W3"]), P["b3"])
+    x = (np.arange(n + 6) - 2.5) / n
+    u = 0.5 + np.sin(2 * np.pi * x) + 0.3 * (x > 0.5) + 0.02 * rng.standard_normal(n + 6)
+    return p, u, rng.standard_normal(n + 6)
+
+
+@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+def test_learned_operator_vjp_vs_complex_step(torch_mod, kind: str) -> None:
+    """J_L(u)^T mu of the learned operator against the dense complex-step
+    Jacobian of the oracle's surrogate (shock cells WENO-JS, smooth cells
+    the ideal weights, learned cells the network with BF16 weights and
+    unrounded activations), FP64, every regime present."""
+    torch = torch_mod
+    from oracle import adjoint_cs as A
+    from oracle import mlp_np as M
+    from oracle import weno_np as W
+    from pyshocks_amd.adjoint import operator_vjp
+
+    n, tau, sigma = 48, 20.0, 1.5
+    for seed in (1, 2, 3):
+        p, u, mu = _learned_case(n, seed)
+        ogrid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+        uf = W.apply_boundary_periodic(ogrid, u)
+        reg = M.regimes(M.stencils(uf), 1e-6, tau, sigma)[3:-3]
+        assert np.all(np.bincount(reg, minlength=3) > 0)
+        src, _ = _learned_source(torch, kind, n, p, tau, sigma)
+        got = _host(operator_vjp(src, _dev(torch, uf), _dev(torch, mu)))[3:-3]
+        J = A.learned_operator_jacobian(p, tau, sigma, kind, ogrid, uf, eps=1e-6)
+        ref = J.T @ mu[3:-3]
+        assert _rel(got, ref) < 1e-11, (seed, _rel(got, ref))
+
+
+@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+def test_learned_adjoint_tau0_equals_plain(torch_mod, kind: str) -> None:
+    """With tau = 0 every varying cell is in the shock regime: the learned
+    checkpointed adjoint reproduces the WENO-JS one bit for bit."""
+    torch = torch_mod
+    from pyshocks_amd.adjoint import CheckpointedSolve
+
+    n, nsteps, every = 1000, 40, 9
+    p, _, _ = _learned_case(n, 5)
+    # a sine with a step and small noise: every cell varies, and the
+    # reference's Godunov flux (no entropy fix at sonic points) stays finite
+    # on it, which it does not on O(1) cell-to-cell noise
+    rng = np.random.default_rng(6)
+    x = (np.arange(n + 6) - 2.5) / n
+    u0 = 0.5 + np.sin(2 * np.pi * x) + 0.3 * (x > 0.4) + 0.01 * rng.standard_normal(n + 6)
+    lam = rng.standard_normal(n + 6)
+    dt = 0.2 / n
+    src_l, _ = _learned_source(torch, kind, n, p, 0.0, 0.0)
+    src_p, _ = _source(torch, kind, n)
+    sl = CheckpointedSolve(src_l, _dev(torch, u0), nsteps, dt, every)
+    sp = CheckpointedSolve(src_p, _dev(torch, u0), nsteps, dt, every)
+    assert bool(torch.isfinite(sp.u_final[3:-3]).all())
+    assert torch.equal(sl.u_final[3:-3], sp.u_final[3:-3])
+    gl = sl.gradient(_dev(torch, lam))
+    gp = sp.gradient(_dev(torch, lam))
+    assert torch.equal(gl, gp)
+
+
+@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+def test_learned_adjoint_consistency(torch_mod, kind: str) -> None:
+    """The learned checkpointed adjoint (whole-step recompute, fused reverse
+    step) equals the chain of single-step VJPs (stage kernels) bit for bit,
+    and a batch of instances equals the single solves."""
+    torch = torch_mod
+    from pyshocks_amd.adjoint import (BatchedCheckpointedSolve, CheckpointedSolve,
+                                      ssprk33_step_vjp)
+    from pyshocks_amd.timestepping import advance, make_ssprk33
+
+    n, nsteps, every = 700, 12, 5
+    p, _, _ = _learned_case(n, 7)
+    src, grid = _learned_source(torch, kind, n, p, 20.0, 1.5)
+    x = (np.arange(n + 6) - 2.5) / n
+    rng = np.random.default_rng(8)
+    states = [0.5 + np.sin(2 * np.pi * (x + rng.uniform())) + 0.3 * (x > rng.uniform())
+              for _ in range(3)]
+    lams = [rng.standard_normal(n + 6) for _ in range(3)]
+    dt = 0.2 / n
+    stepper = make_ssprk33(src.scheme, grid, src.bc, dt=dt)
+    grads = []
+    for b in range(3):
+        solve = CheckpointedSolve(src, _dev(torch, states[b]), nsteps, dt, every=every)
+        g = solve.gradient(_dev(torch, lams[b]))
+        traj = [_dev(torch, states[b])]
+        for _ in range(nsteps):
+            traj.append(advance(stepper, dt, 0.0, traj[-1]))
+        ref = _dev(torch, lams[b])
+        for s in range(nsteps - 1, -1, -1):
+            ref = ssprk33_step_vjp(src, traj[s], ref, dt)
+        assert bool(torch.isfinite(g).all())
+        assert torch.equal(g[3:-3], ref[3:-3]), float((g - ref).abs().max())
+        grads.append(g)
+    bs = BatchedCheckpointedSolve(src, _dev(torch, np.stack(states)), nsteps, dt, every)
+    gb = bs.gradient(_dev(torch, np.stack(lams)))
+    for b in range(3):
+        assert torch.equal(gb[b], grads[b])
+
+
+# }}}

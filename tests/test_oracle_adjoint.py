@@ -70,3 +70,81 @@ def test_constant_state_jacobian_is_linear_scheme() -> None:
         for k, off in enumerate(range(-3, 3)):
             expected[(i + off) % n] += row[k]
         np.testing.assert_allclose(J[i], expected, rtol=0, atol=1e-9 * n)
+
+
+def _learned_data(n: int, seed: int):
+    from oracle import mlp_np as M
+
+    rng = np.random.default_rng(seed)
+    p = (0.3 * rng.standard_normal(M.NPARAM)).astype(np.float32)
+    # W1, W2, W3 representable in BF16, so that the unrounded oracle uses the
+    # surrogate's weights
+    P = M.unpack(p)
+    p = M.pack(M.bf16(P["W1"]), P["b1"], M.bf16(P["W2"]), P["b2"], M.bf16(P["W3"]), P["b3"])
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    x = (np.arange(n + 6) - 2.5) / n
+    u = 0.5 + np.sin(2 * np.pi * x) + 0.3 * (x > 0.5) + 0.02 * rng.standard_normal(n + 6)
+    return p, grid, W.apply_boundary_periodic(grid, u)
+
+
+def test_learned_surrogate_values() -> None:
+    """The complex surrogate's real part is the oracle's unrounded learned
+    reconstruction (equal weights; b3 + ln d in FP32 against FP64 moves the
+    weights by ~1e-7); with tau = 0 every varying cell is WENO-JS."""
+    from oracle import mlp_np as M
+
+    p, grid, u = _learned_data(48, 3)
+    ul, ur = A.learned_reconstruct_cs(p, 20.0, 1.5, u.astype(complex), 1e-6)
+    rl, rr = M.learned_reconstruct(p, 20.0, u, 1e-6, sigma=1.5, emulate=False)
+    reg = M.regimes(M.stencils(u), 1e-6, 20.0, 1.5)
+    assert np.all(np.bincount(reg, minlength=3) > 0)
+    np.testing.assert_allclose(ur.real, rr, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(ul.real, rl, rtol=0, atol=1e-6)
+    jl, jr = W.reconstruct_wenojs(u, 1e-6)
+    ul0, ur0 = A.learned_reconstruct_cs(p, 0.0, 0.0, u.astype(complex), 1e-6)
+    np.testing.assert_allclose(ur0.real, jr, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(ul0.real, jl, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+def test_learned_step_jacobian_vs_fd(kind: str) -> None:
+    """Complex-step columns of the learned surrogate's step Jacobian against
+    central differences of its real evaluation (the branches fixed by the
+    real parts stay fixed for these random data and h = 1e-6)."""
+    n = 24
+    p, grid, u = _learned_data(n, 4)
+    dt = 0.2 / n
+    J = A.learned_step_jacobian(p, 20.0, 1.5, kind, grid, u, dt, eps=1e-6)
+    h = 1.0e-6
+    g = grid.nghosts
+    for j in (0, 5, 11, 17, 23):
+        e = np.zeros(n + 6)
+        e[g + j] = h
+        # the real evaluation of the surrogate: a complex step of size 0
+        up = _step_real(p, kind, grid, u + e, dt)
+        um = _step_real(p, kind, grid, u - e, dt)
+        fd = (up - um)[g:g + n] / (2 * h)
+        np.testing.assert_allclose(J[:, j], fd, rtol=1e-5, atol=1e-7 * np.abs(J).max())
+
+
+def _step_real(p, kind, grid, u, dt):
+    """One step of the learned surrogate on real input (through the complex
+    code with zero imaginary part)."""
+    ju = W.apply_boundary_periodic(grid, np.asarray(u, dtype=np.float64))
+
+    def operator(t, x):
+        x = W.apply_boundary_periodic(grid, x)
+        ul, ur = A.learned_reconstruct_cs(p, 20.0, 1.5, x, 1e-6)
+        fl, fr = ul**2 / 2, ur**2 / 2
+        if kind == "rusanov":
+            a = np.abs(x)
+            a = np.maximum(a[1:], a[:-1])
+            fnum = 0.5 * (fl[1:] + fr[:-1]) - 0.5 * a * (ul[1:] - ur[:-1])
+        else:
+            aavg = (ur[:-1] + ul[1:]) / 2
+            fnum = np.where(aavg > 0, fr[:-1], fl[1:])
+        F = np.pad(fnum, 1)
+        return -(F[1:] - F[:-1]) / grid.dx
+
+    out = W.ssprk33_advance(operator, dt, 0.0, ju.astype(np.complex128))
+    return out.real
