@@ -266,6 +266,7 @@ int psa_interior_norm(int dtype, int p, const void *u, int64_t lo, int64_t hi,
  * A non-finite dt stops the run with status PSA_ERR_NONFINITE_DT. */
 #define PSA_UMAX_SLOTS 64      /* max|u| partials, one 128-byte line each */
 #define PSA_UMAX_SLOT_STRIDE 16 /* in unsigned long long                    */
+#define PSA_STEP_SLOT_STRIDE 4  /* whole-step partials, in unsigned long long */
 
 typedef struct psa_control {
   double t;                     /* time of the current state               */
@@ -283,26 +284,32 @@ typedef struct psa_control {
   int dt_mode;                  /* PSA_DT_CFL or PSA_DT_FIXED              */
   int speed_from_state;         /* 1: max|u| of each state (Burgers with
                                    PSA_DT_CFL, and Lax-Friedrichs)         */
-  /* whole-step schedule (PSA_SCHEDULE_STEP): the decision for the next
-   * step, made by the one-wave kernel after the previous step launch (or
-   * before the first one of psa_ssprk33_steps) */
-  double dt_next;
-  int active_next;
-  int pad1_;
-  unsigned long long pad_[3];
+  /* whole-step schedule (PSA_SCHEDULE_STEP): t, m and status of the state
+   * that launch s of a psa_ssprk33_steps call starts from, in entry s % 2
+   * (every workgroup of launch s decides step s from them; workgroup 0
+   * records the state after the step in entry (s + 1) % 2) */
+  double seq_t[2];
+  long long seq_m[2];
+  int seq_status[2];
   /* stage 3 reduces max|u_int| of the new state into word 0 of these slots
    * (a workgroup uses slot blockIdx % PSA_UMAX_SLOTS); stage 1 of the next
-   * step combines them and stage 2 clears them. The whole-step kernel
-   * reduces into word PSA_UMAX_STEP_WORD; the kernel after it leaves the
-   * combined value in word 0 of slot 0 (the other word-0 entries zero). */
+   * step combines them and stage 2 clears them. After a psa_ssprk33_steps
+   * call on the whole-step schedule, word 0 of slot 0 holds the combined
+   * value of the final state (the other word-0 entries zero). */
   unsigned long long umax_slot[PSA_UMAX_SLOTS][PSA_UMAX_SLOT_STRIDE];
   /* Lax-Friedrichs: stage s (1, 2) reduces max|u| of its output into word 0
    * of speed_slot[s - 1], which stage s + 1 combines as its flux speed;
    * stage 1 clears speed_slot[1] and stage 3 clears speed_slot[0]. Lines of
    * their own: no launch reads the lines it updates atomically. */
   unsigned long long speed_slot[2][PSA_UMAX_SLOTS][PSA_UMAX_SLOT_STRIDE];
+  /* whole-step schedule: launch s reduces max|u_int| of the new state into
+   * word 0 of step_slot[s % 3] (slot blockIdx % PSA_UMAX_SLOTS), launch
+   * s + 1 combines them in every workgroup for its dt, and workgroup 0 of
+   * launch s clears step_slot[(s + 1) % 3] for launch s + 1 (its last
+   * reader was launch s - 1). Four slots share a 128-byte line: few lines
+   * for every workgroup to read, few atomics per line. */
+  unsigned long long step_slot[3][PSA_UMAX_SLOTS][PSA_STEP_SLOT_STRIDE];
 } psa_control;
-#define PSA_UMAX_STEP_WORD 8
 
 typedef struct psa_solver_desc {
   int dtype;                    /* PSA_FLOAT64 / PSA_FLOAT32               */
@@ -625,7 +632,7 @@ int psa_learned_reconstruct(int dtype, const float *params, double tau,
  * threads) race. Nothing here calls hipMalloc or hipFree. */
 #define PSA_STATIC_MAX_CELLS (1 << 20)
 #define PSA_STATIC_SCRATCH_BYTES \
-  (24832 + 2 * (size_t)PSA_STATIC_MAX_CELLS * 8) /* control + 2 stages */
+  (30976 + 2 * (size_t)PSA_STATIC_MAX_CELLS * 8) /* control + 2 stages */
 typedef struct psa_static_result {
   double t;      /* time of the final state                     */
   double dt;     /* dt of the last step taken                   */

@@ -77,12 +77,12 @@ class PsaControl(ctypes.Structure):
         ("status", ctypes.c_int),
         ("dt_mode", ctypes.c_int),
         ("speed_from_state", ctypes.c_int),
-        ("dt_next", ctypes.c_double),
-        ("active_next", ctypes.c_int),
-        ("pad1_", ctypes.c_int),
-        ("pad_", ctypes.c_ulonglong * 3),
+        ("seq_t", ctypes.c_double * 2),
+        ("seq_m", ctypes.c_longlong * 2),
+        ("seq_status", ctypes.c_int * 2),
         ("umax_slot", (ctypes.c_ulonglong * 16) * 64),
         ("speed_slot", ((ctypes.c_ulonglong * 16) * 64) * 2),
+        ("step_slot", ((ctypes.c_ulonglong * 4) * 64) * 3),
     ]
 
 

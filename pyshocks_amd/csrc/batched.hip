@@ -119,7 +119,7 @@ __device__ __forceinline__ bool team_barrier(TeamState *ts, int size,
                                              int *s_flag) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    __atomic_thread_fence(__ATOMIC_RELEASE);  // device (agent) scope
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_fetch_add(&ts->arrive, 1u, __ATOMIC_RELEASE,
                            __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int target = static_cast<unsigned int>(size) * k;
@@ -141,7 +141,7 @@ __device__ __forceinline__ bool team_barrier(TeamState *ts, int size,
       }
       __builtin_amdgcn_s_sleep(2);
     }
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     *s_flag = ok;
   }
   __syncthreads();

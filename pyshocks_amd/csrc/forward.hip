@@ -365,13 +365,16 @@ __global__ void k_control_init(psa_control *ctrl, double t, double tfinal,
   ctrl->umax_bits = abs_bits(static_cast<T>(speed_const));
   for (int k = 0; k < PSA_UMAX_SLOTS; ++k) {
     ctrl->umax_slot[k][0] = 0ull;
-    ctrl->umax_slot[k][PSA_UMAX_STEP_WORD] = 0ull;
     ctrl->speed_slot[0][k][0] = 0ull;
     ctrl->speed_slot[1][k][0] = 0ull;
+    for (int b = 0; b < 3; ++b) ctrl->step_slot[b][k][0] = 0ull;
   }
   ctrl->umax_slot[0][0] = ctrl->umax_bits;
-  ctrl->dt_next = 0.0;
-  ctrl->active_next = 0;
+  for (int b = 0; b < 2; ++b) {
+    ctrl->seq_t[b] = t;
+    ctrl->seq_m[b] = 0;
+    ctrl->seq_status[b] = PSA_OK;
+  }
   ctrl->active = 0;
   ctrl->status = PSA_OK;
   ctrl->dt_mode = dt_mode;
@@ -384,16 +387,19 @@ __device__ __forceinline__ unsigned long long gather_umax(
   return gather_slots(ctrl->umax_slot);
 }
 
+// Whether the step from a state at time t0 after m0 steps with status
+// status0 is taken, and its dt (timestepping.py:89-152).
 template <typename T>
-__device__ __forceinline__ bool control_decide(const psa_control *ctrl,
-                                               unsigned long long umax_bits,
-                                               T &dt, int &status) {
-  status = ctrl->status;
+__device__ __forceinline__ bool control_decide_at(const psa_control *ctrl, double t0,
+                                                  long long m0, int status0,
+                                                  unsigned long long umax_bits,
+                                                  T &dt, int &status) {
+  status = status0;
   if (status != PSA_OK) return false;
-  const T t = static_cast<T>(ctrl->t);
+  const T t = static_cast<T>(t0);
   const T tfinal = static_cast<T>(ctrl->tfinal);
   if (t >= tfinal) return false;
-  if (ctrl->maxit >= 0 && ctrl->m >= ctrl->maxit) return false;
+  if (ctrl->maxit >= 0 && m0 >= ctrl->maxit) return false;
 
   if (ctrl->dt_mode == PSA_DT_FIXED) {
     dt = static_cast<T>(ctrl->fixed_dt);
@@ -410,6 +416,14 @@ __device__ __forceinline__ bool control_decide(const psa_control *ctrl,
     return false;
   }
   return true;
+}
+
+template <typename T>
+__device__ __forceinline__ bool control_decide(const psa_control *ctrl,
+                                               unsigned long long umax_bits,
+                                               T &dt, int &status) {
+  return control_decide_at<T>(ctrl, ctrl->t, ctrl->m, ctrl->status, umax_bits, dt,
+                              status);
 }
 
 template <typename T>
@@ -616,73 +630,116 @@ __global__ __launch_bounds__(BLOCK) void k_stage_pipe(StageArgs<T> p) {
 // ---------------------------------------------------------------------------
 // whole-step kernel (step_tile): the three SSP-RK3 stages of a tile in LDS
 
-// The decision for the next step from max|u| of the state it starts from
-// (stage 1 of timestepping.py:89-152, control_decide).
-template <typename T>
-__device__ __forceinline__ void step_decide(psa_control *ctrl,
-                                            unsigned long long umax) {
-  T dt;
-  int status;
-  const bool active = control_decide<T>(ctrl, umax, dt, status);
-  ctrl->status = status;
-  ctrl->umax_bits = umax;
-  ctrl->active_next = active ? 1 : 0;
-  ctrl->dt_next = active ? static_cast<double>(dt) : 0.0;
-}
-
-// Before the first whole-step launch of a sequence (one wave): decides the
-// first step from the word-0 slots (as stage 1 does) and clears the slots
-// the step kernel reduces into.
-template <typename T>
+// Before the first whole-step launch of a psa_ssprk33_steps call (one wave):
+// the state's t, m and status into entry 0 of the sequence record, max|u|
+// of the state (or max|a| of an advection velocity array: the word-0 slots,
+// as stage 1 reads them) into umax_bits, which the launches read for a
+// speed that does not change, and into slot 0 of step_slot[2], which launch
+// 0 reads otherwise; step_slot[0], which launch 0 reduces into, cleared.
 __global__ void k_step_prologue(psa_control *ctrl) {
   const int lane = threadIdx.x;
   const unsigned long long umax = gather_umax(ctrl);
-  ctrl->umax_slot[lane][PSA_UMAX_STEP_WORD] = 0ull;
-  if (lane == 0) step_decide<T>(ctrl, umax);
+  ctrl->step_slot[2][lane][0] = lane == 0 ? umax : 0ull;
+  ctrl->step_slot[0][lane][0] = 0ull;
+  if (lane == 0) {
+    ctrl->umax_bits = umax;
+    ctrl->seq_t[0] = ctrl->t;
+    ctrl->seq_m[0] = ctrl->m;
+    ctrl->seq_status[0] = ctrl->status;
+  }
 }
 
-// After each whole-step launch (one wave): if the step was taken, advances
-// t and m (stage 2), combines the max|u| partials of the new state, leaves
-// them in the word-0 slots for the stage kernels and decides the next step;
-// otherwise records that the step was not taken.
-template <typename T>
-__global__ void k_step_finish(psa_control *ctrl) {
+// After the last whole-step launch s of the call (one wave): max|u| of the
+// final state (the partials of step_slot[s % 3]) into word 0 of slot 0 for
+// the stage kernels and the next call (the other word-0 entries zero).
+__global__ void k_step_epilogue(psa_control *ctrl, int last_set) {
   const int lane = threadIdx.x;
-  if (ctrl->active_next == 0) {
-    if (lane == 0) ctrl->active = 0;
-    return;
-  }
-  const T dt = static_cast<T>(ctrl->dt_next);
-  unsigned long long umax = ctrl->umax_bits;
-  if (ctrl->speed_from_state) {
-    unsigned long long v = ctrl->umax_slot[lane][PSA_UMAX_STEP_WORD];
-    for (int off = 32; off > 0; off >>= 1) {
-      const unsigned long long o = __shfl_xor(v, off, 64);
-      v = o > v ? o : v;
-    }
-    umax = v;
-    ctrl->umax_slot[lane][PSA_UMAX_STEP_WORD] = 0ull;
-    ctrl->umax_slot[lane][0] = lane == 0 ? umax : 0ull;
-  }
-  if (lane == 0) {
-    ctrl->t = static_cast<double>(static_cast<T>(ctrl->t) + dt);
-    ctrl->m = ctrl->m + 1;
-    ctrl->dt = static_cast<double>(dt);
-    ctrl->active = 1;
-    step_decide<T>(ctrl, umax);
-  }
+  if (ctrl->speed_from_state == 0) return;
+  const unsigned long long umax = wave_max_bits(ctrl->step_slot[last_set][lane][0]);
+  ctrl->umax_slot[lane][0] = lane == 0 ? umax : 0ull;
 }
+
+// The step control of whole-step launch s = seq (mod 6) of a sequence
+// (timestepping.py:89-152): wave 0 of every workgroup makes the same decision
+// (control_decide_at) from the sequence record seq_*[s % 2] and max|u| of
+// u^n, the maximum of the partials the previous launch left in
+// step_slot[(s + 2) % 3], while the tile's loads are in flight, and leaves
+// it in LDS for the workgroup. Wave 0 of workgroup 0 records the state after
+// the step in seq_*[(s + 1) % 2] and in the fields the host and the stage
+// kernels read (t, m, dt, active, status, umax_bits), and clears
+// step_slot[(s + 1) % 3] for the next launch; a step not taken carries max|u|
+// over into step_slot[s % 3].
+template <typename T>
+struct SeqStep {
+  psa_control *ctrl;
+  int seq;
+  T *s_dt;
+  int *s_go;
+
+  __device__ __forceinline__ void issue() const {
+    const int tid = threadIdx.x;
+    if (tid >= 64) return;  // wave 0
+    const int par = seq & 1;
+    const bool from_state = ctrl->speed_from_state != 0;
+    const unsigned long long umax =
+        from_state ? wave_max_u64(ctrl->step_slot[(seq + 2) % 3][tid][0]) : ctrl->umax_bits;
+    const double t0 = ctrl->seq_t[par];
+    const long long m0 = ctrl->seq_m[par];
+    T dt = T(0);
+    int status;
+    const bool active =
+        control_decide_at<T>(ctrl, t0, m0, ctrl->seq_status[par], umax, dt, status);
+    if (tid == 0) {
+      *s_dt = dt;
+      *s_go = active ? 1 : 0;
+    }
+    if (blockIdx.x != 0) return;
+    ctrl->step_slot[(seq + 1) % 3][tid][0] = 0ull;
+    if (tid == 0) {
+      const double t1 = active ? static_cast<double>(static_cast<T>(t0) + dt) : t0;
+      const long long m1 = active ? m0 + 1 : m0;
+      ctrl->seq_t[par ^ 1] = t1;
+      ctrl->seq_m[par ^ 1] = m1;
+      ctrl->seq_status[par ^ 1] = status;
+      ctrl->t = t1;
+      ctrl->m = m1;
+      ctrl->status = status;
+      ctrl->active = active ? 1 : 0;
+      ctrl->umax_bits = umax;
+      if (active) ctrl->dt = static_cast<double>(dt);
+      // the state does not change: its max|u| carries over to the next launch
+      if (!active && from_state) atomicMax(&ctrl->step_slot[seq % 3][0][0], umax);
+    }
+  }
+  __device__ __forceinline__ bool go() const { return *s_go != 0; }
+  __device__ __forceinline__ T dt() const { return *s_dt; }
+};
 
 // One SSP-RK3 step (timestepping.py:312-320) of u^n = p.uk into
 // u^(n+1) = p.out (distinct arrays) with the step control of the control
-// block: the decision made before the launch says whether the step is taken
-// (if not, the state is copied to p.out) and with which dt; the workgroups
-// reduce max|u_int| of the new state into the step slots for k_step_finish.
+// block, launch s = p.step_seq (mod 6) of a sequence (SeqStep): a step not
+// taken copies the state to p.out; the workgroups reduce max|u_int| of the
+// new state into step_slot[s % 3]. No launch between the steps: each one starts
+// from what the previous one left.
 // Without a control block the step is taken with the device scalar p.dt_ptr
 // (and p.k1out, p.k2out may receive the stage states).
+// Waves per SIMD the compiler keeps the registers of the FP64 WENO-JS5
+// whole-step kernels within (5: at most 96 VGPRs, which the LDS of a
+// 256-thread tile allows as well); the other variants take what they need.
+#ifndef PSA_STEP_F64_WAVES
+#define PSA_STEP_F64_WAVES 5
+#endif
+template <typename T, bool LEARNED, int REC>
+constexpr int step_min_waves() {
+  return std::is_same<T, double>::value && !LEARNED && REC == 0 ? PSA_STEP_F64_WAVES
+                                                                 : 1;
+}
+
 template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED,
           int REC = 0>
-__global__ __launch_bounds__(BLOCK) void k_step(StageArgs<T> p) {
+__global__ __attribute__((amdgpu_flat_work_group_size(1, BLOCK),
+                          amdgpu_waves_per_eu((step_min_waves<T, LEARNED, REC>()))))
+void k_step(StageArgs<T> p) {
   static_assert(VEL != 2, "no velocity array in the whole-step kernel");
   using SM = StepSmem<T, BLOCK, CPT>;
   __shared__ SM sm;
@@ -704,7 +761,13 @@ __global__ __launch_bounds__(BLOCK) void k_step(StageArgs<T> p) {
                                                       q, nullptr);
     return;
   }
-  if (ctrl->active_next == 0) {
+
+  __shared__ T s_dt;
+  __shared__ int s_go;
+  const SeqStep<T> ctl{ctrl, p.step_seq, &s_dt, &s_go};
+  unsigned long long vmax = 0;
+  if (!step_tile_ctl<T, FLUX, VEL, BLOCK, CPT, LEARNED, REC>(p, A, ctl, p.uk, p.out, sm,
+                                                             vmax, q, nullptr)) {
     // the step is not taken: the state moves to the output unchanged
     for (int j = tid; j < SM::TO; j += BLOCK) {
       const int64_t a = A + j;
@@ -712,17 +775,11 @@ __global__ __launch_bounds__(BLOCK) void k_step(StageArgs<T> p) {
     }
     return;
   }
-  const T dt = static_cast<T>(ctrl->dt_next);
-  unsigned long long vmax = 0;
-  step_tile<T, FLUX, VEL, BLOCK, CPT, LEARNED, REC>(p, A, dt, p.uk, p.out, sm, vmax, q,
-                                                    nullptr);
-
   // ---- partial max|u| of the new state
   if (ctrl->speed_from_state != 0) {
     vmax = block_max_bits<BLOCK>(vmax, sm.red);
     if (tid == 0)
-      atomicMax(&ctrl->umax_slot[blockIdx.x % PSA_UMAX_SLOTS][PSA_UMAX_STEP_WORD],
-                vmax);
+      atomicMax(&ctrl->step_slot[p.step_seq % 3][blockIdx.x % PSA_UMAX_SLOTS][0], vmax);
   }
 }
 
@@ -1175,7 +1232,8 @@ int run_steps(const psa_solver_desc *d, const void *u_in, void *u_out,
     // one launch per step; u^n and u^(n+1) are distinct arrays: the state
     // goes u_in -> k1 -> k2 -> k1 ... -> u_out (u_in -> k1 -> u_out, by a
     // copy, for one in-place step)
-    hipLaunchKernelGGL(k_step_prologue<T>, dim3(1), dim3(64), 0, stream, a.ctrl);
+    if (nsteps <= 0) return PSA_OK;
+    hipLaunchKernelGGL(k_step_prologue, dim3(1), dim3(64), 0, stream, a.ctrl);
     st = check_launch("step prologue");
     if (st != PSA_OK) return st;
     const bool copy_back = nsteps == 1 && un == uo;
@@ -1183,12 +1241,14 @@ int run_steps(const psa_solver_desc *d, const void *u_in, void *u_out,
       a.uk = s == 0 ? un : ((s - 1) % 2 == 0 ? b1 : b2);
       a.out = (s == nsteps - 1 && !copy_back) ? uo : (s % 2 == 0 ? b1 : b2);
       a.un = a.uk;
+      a.step_seq = static_cast<int>(s % 6);
       st = launch_step<T>(d, vel, a, cfg, stream);
       if (st != PSA_OK) return st;
-      hipLaunchKernelGGL(k_step_finish<T>, dim3(1), dim3(64), 0, stream, a.ctrl);
-      st = check_launch("step finish");
-      if (st != PSA_OK) return st;
     }
+    hipLaunchKernelGGL(k_step_epilogue, dim3(1), dim3(64), 0, stream, a.ctrl,
+                       static_cast<int>((nsteps - 1) % 3));
+    st = check_launch("step epilogue");
+    if (st != PSA_OK) return st;
     if (copy_back) {
       hipLaunchKernelGGL(k_copy<T>, dim3(blocks_for(a.m, 256)), dim3(256), 0,
                          stream, b1, uo, a.m);

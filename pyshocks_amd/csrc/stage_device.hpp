@@ -48,6 +48,11 @@ __device__ __forceinline__ unsigned long long abs_bits(float v) {
   return static_cast<unsigned long long>(
       static_cast<unsigned int>(__float_as_uint(fabsf(v))));
 }
+// A NaN's pattern (above every other abs_bits value).
+template <typename T>
+__device__ __forceinline__ unsigned long long nan_bits() {
+  return sizeof(T) == 8 ? 0x7FF8000000000000ull : 0x7FC00000ull;
+}
 template <typename T>
 __device__ __forceinline__ T from_bits(unsigned long long b);
 template <>
@@ -59,13 +64,41 @@ __device__ __forceinline__ float from_bits<float>(unsigned long long b) {
   return __uint_as_float(static_cast<unsigned int>(b));
 }
 
+// Maximum of a 32-bit unsigned value over the wave (returned to every lane):
+// DPP row reductions (quad permutations, row rotations by 4 and 8) give each
+// row of 16 lanes its maximum, the row broadcasts of lanes 15 and 31 carry
+// them into the last row, and lane 63 holds the result. Lanes a DPP move
+// does not write read 0, the identity of the maximum.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ unsigned int dpp_max_u32(unsigned int v) {
+  const unsigned int o = static_cast<unsigned int>(
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, ROWS, 0xF, true));
+  return o > v ? o : v;
+}
+
+__device__ __forceinline__ unsigned int wave_max_u32(unsigned int v) {
+  v = dpp_max_u32<0xB1>(v);        // quad_perm [1, 0, 3, 2]
+  v = dpp_max_u32<0x4E>(v);        // quad_perm [2, 3, 0, 1]
+  v = dpp_max_u32<0x124>(v);       // row_ror 4
+  v = dpp_max_u32<0x128>(v);       // row_ror 8
+  v = dpp_max_u32<0x142, 0xA>(v);  // row_bcast 15 into rows 1 and 3
+  v = dpp_max_u32<0x143, 0xC>(v);  // row_bcast 31 into rows 2 and 3
+  return static_cast<unsigned int>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
+// Maximum of a 64-bit unsigned value over the wave (to every lane): the high
+// words first, then the low words of the lanes that hold the high maximum.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+  const unsigned int hi = wave_max_u32(static_cast<unsigned int>(v >> 32));
+  const unsigned int lo = wave_max_u32(
+      static_cast<unsigned int>(v >> 32) == hi ? static_cast<unsigned int>(v) : 0u);
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
 template <int BLOCK>
 __device__ __forceinline__ unsigned long long block_max_bits(
     unsigned long long v, unsigned long long *s_red) {
-  for (int off = 32; off > 0; off >>= 1) {
-    unsigned long long o = __shfl_xor(v, off, 64);
-    v = o > v ? o : v;
-  }
+  v = wave_max_u64(v);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   if (lane == 0) s_red[wave] = v;
@@ -76,19 +109,17 @@ __device__ __forceinline__ unsigned long long block_max_bits(
   return v;
 }
 
-// max of the PSA_UMAX_SLOTS partials in word 0 of `slots`: lane k of every
-// wave reads slot k and the wave reduces them, so each wave holds the result
-// without a barrier.
+// max of PSA_UMAX_SLOTS partials (lane k of every wave reads the k-th and
+// the wave reduces them, so each wave holds the result without a barrier)
+__device__ __forceinline__ unsigned long long wave_max_bits(unsigned long long v) {
+  return wave_max_u64(v);
+}
+
+// max of the PSA_UMAX_SLOTS partials in word 0 of `slots` (wave_max_bits).
 __device__ __forceinline__ unsigned long long gather_slots(
     const unsigned long long (*slots)[PSA_UMAX_SLOT_STRIDE]) {
   static_assert(PSA_UMAX_SLOTS == 64, "one slot per lane");
-  const int lane = threadIdx.x & 63;
-  unsigned long long v = slots[lane][0];
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(v, off, 64);
-    v = o > v ? o : v;
-  }
-  return v;
+  return wave_max_bits(slots[threadIdx.x & 63][0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -137,6 +168,7 @@ struct StageArgs {
                      //   at offset blockIdx.y * bstride of uk, out, k1out,
                      //   k2out, its dt at dt_ptr[blockIdx.y]
   int batch;         // host side: instances of the launch (grid.y)
+  int step_seq;      // whole-step launch s of a controlled sequence: s mod 6
 };
 
 template <typename T, int VEL>
@@ -367,7 +399,8 @@ __device__ __forceinline__ void learned_cells(const MlpSource &src, const T *w,
 // between neighbouring cells: the second differences of the indicators
 // (t00 = D(i) - D(i-1)), t01 = t00 + 2 D(i), t11 = D(i+1) + D(i),
 // t21 = t20 - 2 D(i+1) (weno.py:219-226 rearranged), and the candidates
-// (weno_uhat_diff). LEARNED: the weights of cells outside the shock regime
+// (weno_uhat_diff; in FP64 the middle candidates and the third differences
+// of weno_diff_sums). LEARNED: the weights of cells outside the shock regime
 // come from the network or the ideal weights (wave-collective, learned_cells).
 template <typename T, int CPT, bool LEARNED>
 __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
@@ -389,25 +422,50 @@ __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
   }
   const T eps4 = T(4) * p.eps;
 
-  if constexpr (!LEARNED && std::is_same<T, double>::value) {
-    // FP64: the shared reciprocal for every cell without a branch; a cell
-    // whose denominator is not a positive normal number is redone in the
-    // reference form from its stencil after one wave-uniform test (its
-    // result depends on its own values only, not on the wave it falls in)
+  if constexpr (std::is_same<T, double>::value) {
+    // FP64: the third-difference form (weno_diff_sums) with 13/3 d2^2 + 4 eps
+    // formed once per cell (each enters one indicator of three cells) and the
+    // shared reciprocal for every cell without a branch; a cell whose
+    // denominator is not a positive normal number is redone in the reference
+    // form from its stencil after one wave-uniform test (its result depends
+    // on its own values only, not on the wave it falls in). LEARNED: the same
+    // for the cells in the shock regime, whose regime comes from the
+    // indicators 4 beta_k = t^2 + 13/3 d2^2 of learned_right16.
+    T Ee[CPT + 2], t3[CPT + 1], t2[CPT + 1];
+#pragma unroll
+    for (int k = 0; k < CPT + 2; ++k) Ee[k] = fma(C::a0x4, d2[k] * d2[k], eps4);
+#pragma unroll
+    for (int k = 0; k < CPT + 1; ++k) {
+      // third difference of cells k - 1, k (d2 of cell k minus d2 of cell k - 1)
+      const T t = d2[k + 1] - d2[k];
+      t3[k] = t * (1.0 / 3.0);
+      t2[k] = 0.5 * t;
+    }
+    int regime[CPT];
     unsigned int redo = 0;
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
       const T t01 = fma(T(2), D[r + 1], d2[r]);
       const T t11 = D[r + 2] + D[r + 1];
       const T t21 = fma(T(-2), D[r + 2], d2[r + 2]);
-      T hr0, hr1, hr2, hl0, hl1, hl2;
-      weno_uhat_diff(w[r + 2], D[r], D[r + 1], D[r + 2], D[r + 3], hr0, hr1, hr2,
-                     hl0, hl1, hl2);
-      const WenoPairSums ps =
-          weno_pair_sums(hr0, hr1, hr2, hl0, hl1, hl2, fma(t01, t01, E[r]),
-                         fma(t11, t11, E[r + 1]), fma(t21, t21, E[r + 2]), eps4);
-      redo |= weno_pair_fast(ps) ? 0u : (1u << r);
-      weno_pair_divide(ps, ul[r], ur[r]);
+      regime[r] = 0;
+      if constexpr (LEARNED)
+        regime[r] = mlp_regime(fma(t01, t01, E[r]), fma(t11, t11, E[r + 1]),
+                               fma(t21, t21, E[r + 2]), eps4, p.tau, p.sigma);
+      if (regime[r] == 0) {
+        const WenoDiffSums s =
+            weno_diff_sums(fma(t01, t01, Ee[r]), fma(t11, t11, Ee[r + 1]),
+                           fma(t21, t21, Ee[r + 2]), t3[r], t2[r], t3[r + 1], t2[r + 1]);
+        redo |= weno_diff_fast(s) ? 0u : (1u << r);
+        weno_diff_divide(s, fma(t11 + D[r + 2], 1.0 / 6.0, w[r + 2]),
+                         fma(t11 + D[r + 1], -1.0 / 6.0, w[r + 2]), ul[r], ur[r]);
+      } else {
+        T hr0, hr1, hr2, hl0, hl1, hl2;
+        weno_uhat_diff(w[r + 2], D[r], D[r + 1], D[r + 2], D[r + 3], hr0, hr1, hr2,
+                       hl0, hl1, hl2);
+        ur[r] = mlp_combine_ideal(hr0, hr1, hr2);
+        ul[r] = mlp_combine_ideal(hl0, hl1, hl2);
+      }
     }
     if (__any(redo != 0)) {
 #pragma unroll
@@ -423,6 +481,7 @@ __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
         }
       }
     }
+    if constexpr (LEARNED) learned_cells<T, CPT>(src, w, regime, ul, ur, q);
     return;
   }
 
@@ -1026,11 +1085,16 @@ __device__ __forceinline__ void step_store(const StageArgs<T> &p, int64_t base,
       for (int r = 0; r < CPT; ++r) dst[a0 + r] = val[r];
     }
     if constexpr (VMAX) {
+      // the maximum in the dtype (NaN is tracked apart: fmax skips it)
+      T mx = T(0);
+      bool nan = false;
 #pragma unroll
       for (int r = 0; r < CPT; ++r) {
-        const unsigned long long b = abs_bits(val[r]);
-        vmax = b > vmax ? b : vmax;
+        mx = fmax(mx, fabs(val[r]));
+        nan |= isnan(val[r]);
       }
+      const unsigned long long b = nan ? nan_bits<T>() : abs_bits(mx);
+      vmax = b > vmax ? b : vmax;
     }
   } else {
 #pragma unroll
@@ -1066,15 +1130,20 @@ __device__ __forceinline__ void step_store(const StageArgs<T> &p, int64_t base,
 // the stage states k1 and k2 of the output cells are written as well (they
 // are exact there). Ends with the LDS in use (a barrier before the next
 // tile).
-template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED,
-          int REC = 0>
-__device__ __forceinline__ void step_tile(const StageArgs<T> &p, int64_t A, T dt,
-                                          const T *__restrict__ uin,
-                                          T *__restrict__ uout,
-                                          StepSmem<T, BLOCK, CPT> &sm,
-                                          unsigned long long &vmax,
-                                          unsigned int *q,
-                                          const MlpWaveLds *wl) {
+//
+// Ctl supplies the step: ctl.issue() runs once the tile's loads are in
+// flight, ctl.go() and ctl.dt() are read after the workgroup's first barrier
+// (a step not taken returns false with nothing computed or stored).
+template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED, int REC,
+          class Ctl>
+__device__ __forceinline__ bool step_tile_ctl(const StageArgs<T> &p, int64_t A,
+                                              const Ctl &ctl,
+                                              const T *__restrict__ uin,
+                                              T *__restrict__ uout,
+                                              StepSmem<T, BLOCK, CPT> &sm,
+                                              unsigned long long &vmax,
+                                              unsigned int *q,
+                                              const MlpWaveLds *wl) {
   static_assert(VEL != 2, "no velocity array in the whole-step tile");
   using SM = StepSmem<T, BLOCK, CPT>;
   constexpr int C = SM::C;
@@ -1092,8 +1161,10 @@ __device__ __forceinline__ void step_tile(const StageArgs<T> &p, int64_t A, T dt
   if (fast) {
     Chunks<T, BLOCK, L / VN> cu;
     cu.load(uin + base, tid);
+    ctl.issue();
     cu.store(sm.x0, tid);
   } else {
+    ctl.issue();
     for (int j = tid; j < L; j += BLOCK) {
       int64_t i = (base + j - g) % n;
       if (i < 0) i += n;
@@ -1109,6 +1180,8 @@ __device__ __forceinline__ void step_tile(const StageArgs<T> &p, int64_t A, T dt
     sm.ulF[BLOCK] = T(0);
   }
   __syncthreads();
+  if (!ctl.go()) return false;
+  const T dt = ctl.dt();
 
   const int j0 = 4 + CPT * tid;
   T un[CPT], val[CPT];
@@ -1131,6 +1204,29 @@ __device__ __forceinline__ void step_tile(const StageArgs<T> &p, int64_t A, T dt
   step_stage<T, FLUX, VEL, BLOCK, CPT, 3, LEARNED, REC>(p, sm.x0, sm, dt, un, val, src, q);
 
   step_store<T, BLOCK, CPT, true>(p, base, fast, val, uout, vmax);
+  return true;
+}
+
+// A step of a given dt.
+template <typename T>
+struct FixedStep {
+  T v;
+  __device__ __forceinline__ void issue() const {}
+  __device__ __forceinline__ bool go() const { return true; }
+  __device__ __forceinline__ T dt() const { return v; }
+};
+
+template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED,
+          int REC = 0>
+__device__ __forceinline__ void step_tile(const StageArgs<T> &p, int64_t A, T dt,
+                                          const T *__restrict__ uin,
+                                          T *__restrict__ uout,
+                                          StepSmem<T, BLOCK, CPT> &sm,
+                                          unsigned long long &vmax,
+                                          unsigned int *q,
+                                          const MlpWaveLds *wl) {
+  step_tile_ctl<T, FLUX, VEL, BLOCK, CPT, LEARNED, REC>(p, A, FixedStep<T>{dt}, uin, uout,
+                                                        sm, vmax, q, wl);
 }
 
 }  // namespace psa

@@ -17,7 +17,7 @@
 
 namespace {
 
-constexpr size_t kControlBytes = 24832;  // sizeof(psa_control) rounded to 256
+constexpr size_t kControlBytes = 30976;  // sizeof(psa_control) rounded to 256
 constexpr size_t kStageBytes = static_cast<size_t>(PSA_STATIC_MAX_CELLS) * 8;
 constexpr size_t kScratchBytes = kControlBytes + 2 * kStageBytes;
 static_assert(sizeof(psa_control) <= kControlBytes, "control block layout");

@@ -166,6 +166,53 @@ __device__ __forceinline__ void weno_pair_divide(const WenoPairSums &s, double &
   ul = (s.nl * s.sr) * inv;
 }
 
+// Third-difference form of both sides (FP64, the fused kernels). With the
+// first differences D(c) = u(c) - u(c-1), the second d2(c) = D(c+1) - D(c)
+// and the third T0 = d2(c) - d2(c-1), T1 = d2(c+1) - d2(c), the candidates of
+// weno.py:171-202 differ from the middle ones
+//   hr1 = u(c) + (D(c) + 2 D(c+1)) / 6,   hl1 = u(c) - (2 D(c) + D(c+1)) / 6
+// by hr0 - hr1 = -T0/3, hr2 - hr1 = -T1/6, hl0 - hl1 = T1/3, hl2 - hl1 = T0/6.
+// With the weights of weno_pair_sums (right p12 : 6 p02 : 3 p01, left
+// p01 : 6 p02 : 3 p12 over their sums sr, sl):
+//   ur = hr1 - (p12 T0/3 + p01 T1/2) / sr,  ul = hl1 + (p01 T1/3 + p12 T0/2) / sl,
+// both by the one reciprocal of sr sl. e_k = 4 (eps + beta_k) (the scale
+// cancels).
+struct WenoDiffSums {
+  double sr, sl, nr, nl;
+};
+
+__device__ __forceinline__ WenoDiffSums weno_diff_sums(double e0, double e1, double e2,
+                                                       double t0_3, double t0_2,
+                                                       double t1_3, double t1_2) {
+  const double q0 = e0 * e0, q1 = e1 * e1, q2 = e2 * e2;
+  const double p01 = q0 * q1, p02 = q0 * q2, p12 = q1 * q2;
+  WenoDiffSums r;
+  r.sr = fma(3.0, p01, fma(6.0, p02, p12));
+  r.sl = fma(3.0, p12, fma(6.0, p02, p01));
+  r.nr = fma(p12, t0_3, p01 * t1_2);
+  r.nl = fma(p01, t1_3, p12 * t0_2);
+  return r;
+}
+
+// Whether the shared reciprocal applies: sr sl a positive normal number.
+__device__ __forceinline__ bool weno_diff_fast(const WenoDiffSums &s) {
+  return __builtin_amdgcn_class(s.sr * s.sl, 0x100);
+}
+
+// ur, ul from the middle candidates by the shared reciprocal (the hardware
+// reciprocal of sr sl refined by two Newton steps, as in weno_pair_divide).
+__device__ __forceinline__ void weno_diff_divide(const WenoDiffSums &s, double hr1,
+                                                 double hl1, double &ul, double &ur) {
+  const double den = s.sr * s.sl;
+  double inv = __builtin_amdgcn_rcp(den);
+  double e = fma(-den, inv, 1.0);
+  inv = fma(inv, e, inv);
+  e = fma(-den, inv, 1.0);
+  inv = fma(inv, e, inv);
+  ur = fma(-(s.nr * s.sl), inv, hr1);
+  ul = fma(s.nl * s.sr, inv, hl1);
+}
+
 // The reference form from q_k = (eps + beta_k)^2: safe over the whole
 // exponent range.
 __device__ __forceinline__ void weno_pair_ref_q(double hr0, double hr1, double hr2,
