@@ -422,51 +422,59 @@ __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
   }
   const T eps4 = T(4) * p.eps;
 
-  if constexpr (std::is_same<T, double>::value) {
-    // FP64: the third-difference form (weno_diff_sums) with 13/3 d2^2 + 4 eps
-    // formed once per cell (each enters one indicator of three cells) and the
-    // shared reciprocal for every cell without a branch; a cell whose
-    // denominator is not a positive normal number is redone in the reference
-    // form from its stencil after one wave-uniform test (its result depends
-    // on its own values only, not on the wave it falls in). LEARNED: the same
-    // for the cells in the shock regime, whose regime comes from the
-    // indicators 4 beta_k = t^2 + 13/3 d2^2 of learned_right16.
-    T Ee[CPT + 2], t3[CPT + 1], t2[CPT + 1];
+  // the third-difference form (weno_diff_sums, weno_diff_pair_f32) with
+  // 13/3 d2^2 + 4 eps formed once per cell (each enters one indicator of
+  // three cells). FP64: the shared reciprocal for every cell without a
+  // branch; a cell whose denominator is not a positive normal number is
+  // redone in the reference form from its stencil after one wave-uniform
+  // test (its result depends on its own values only, not on the wave it
+  // falls in). LEARNED: the same for the cells in the shock regime, whose
+  // regime comes from the indicators 4 beta_k = t^2 + 13/3 d2^2 of
+  // learned_right16.
+  T Ee[CPT + 2], t3[CPT + 1], t2[CPT + 1];
 #pragma unroll
-    for (int k = 0; k < CPT + 2; ++k) Ee[k] = fma(C::a0x4, d2[k] * d2[k], eps4);
+  for (int k = 0; k < CPT + 2; ++k) Ee[k] = fma(C::a0x4, d2[k] * d2[k], eps4);
 #pragma unroll
-    for (int k = 0; k < CPT + 1; ++k) {
-      // third difference of cells k - 1, k (d2 of cell k minus d2 of cell k - 1)
-      const T t = d2[k + 1] - d2[k];
-      t3[k] = t * (1.0 / 3.0);
-      t2[k] = 0.5 * t;
-    }
-    int regime[CPT];
-    unsigned int redo = 0;
+  for (int k = 0; k < CPT + 1; ++k) {
+    // third difference of cells k - 1, k (d2 of cell k minus d2 of cell k - 1)
+    const T t = d2[k + 1] - d2[k];
+    t3[k] = t * T(1.0 / 3.0);
+    t2[k] = T(0.5) * t;
+  }
+  int regime[CPT];
+  unsigned int redo = 0;
 #pragma unroll
-    for (int r = 0; r < CPT; ++r) {
-      const T t01 = fma(T(2), D[r + 1], d2[r]);
-      const T t11 = D[r + 2] + D[r + 1];
-      const T t21 = fma(T(-2), D[r + 2], d2[r + 2]);
-      regime[r] = 0;
-      if constexpr (LEARNED)
-        regime[r] = mlp_regime(fma(t01, t01, E[r]), fma(t11, t11, E[r + 1]),
-                               fma(t21, t21, E[r + 2]), eps4, p.tau, p.sigma);
-      if (regime[r] == 0) {
+  for (int r = 0; r < CPT; ++r) {
+    const T t01 = fma(T(2), D[r + 1], d2[r]);
+    const T t11 = D[r + 2] + D[r + 1];
+    const T t21 = fma(T(-2), D[r + 2], d2[r + 2]);
+    regime[r] = 0;
+    if constexpr (LEARNED)
+      regime[r] = mlp_regime(fma(t01, t01, E[r]), fma(t11, t11, E[r + 1]),
+                             fma(t21, t21, E[r + 2]), eps4, p.tau, p.sigma);
+    if (regime[r] == 0) {
+      const T e0 = fma(t01, t01, Ee[r]), e1 = fma(t11, t11, Ee[r + 1]),
+              e2 = fma(t21, t21, Ee[r + 2]);
+      const T hr1 = fma(t11 + D[r + 2], T(1.0 / 6.0), w[r + 2]);
+      const T hl1 = fma(t11 + D[r + 1], T(-1.0 / 6.0), w[r + 2]);
+      if constexpr (std::is_same<T, double>::value) {
         const WenoDiffSums s =
-            weno_diff_sums(fma(t01, t01, Ee[r]), fma(t11, t11, Ee[r + 1]),
-                           fma(t21, t21, Ee[r + 2]), t3[r], t2[r], t3[r + 1], t2[r + 1]);
+            weno_diff_sums(e0, e1, e2, t3[r], t2[r], t3[r + 1], t2[r + 1]);
         redo |= weno_diff_fast(s) ? 0u : (1u << r);
-        weno_diff_divide(s, fma(t11 + D[r + 2], 1.0 / 6.0, w[r + 2]),
-                         fma(t11 + D[r + 1], -1.0 / 6.0, w[r + 2]), ul[r], ur[r]);
+        weno_diff_divide(s, hr1, hl1, ul[r], ur[r]);
       } else {
-        T hr0, hr1, hr2, hl0, hl1, hl2;
-        weno_uhat_diff(w[r + 2], D[r], D[r + 1], D[r + 2], D[r + 3], hr0, hr1, hr2,
-                       hl0, hl1, hl2);
-        ur[r] = mlp_combine_ideal(hr0, hr1, hr2);
-        ul[r] = mlp_combine_ideal(hl0, hl1, hl2);
+        weno_diff_pair_f32(e0, e1, e2, t3[r], t2[r], t3[r + 1], t2[r + 1], hr1, hl1,
+                           ul[r], ur[r]);
       }
+    } else {
+      T hr0, hr1, hr2, hl0, hl1, hl2;
+      weno_uhat_diff(w[r + 2], D[r], D[r + 1], D[r + 2], D[r + 3], hr0, hr1, hr2, hl0,
+                     hl1, hl2);
+      ur[r] = mlp_combine_ideal(hr0, hr1, hr2);
+      ul[r] = mlp_combine_ideal(hl0, hl1, hl2);
     }
+  }
+  if constexpr (std::is_same<T, double>::value) {
     if (__any(redo != 0)) {
 #pragma unroll
       for (int r = 0; r < CPT; ++r) {
@@ -480,39 +488,6 @@ __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
                           ul[r], ur[r]);
         }
       }
-    }
-    if constexpr (LEARNED) learned_cells<T, CPT>(src, w, regime, ul, ur, q);
-    return;
-  }
-
-  int regime[CPT];
-#pragma unroll
-  for (int r = 0; r < CPT; ++r) {
-    // cell r: D(r-1) = D[r], D(r) = D[r+1], D(r+1) = D[r+2], D(r+2) = D[r+3]
-    const T t01 = fma(T(2), D[r + 1], d2[r]);
-    const T t11 = D[r + 2] + D[r + 1];
-    const T t21 = fma(T(-2), D[r + 2], d2[r + 2]);
-    const T b0 = fma(t01, t01, E[r]);
-    const T b1 = fma(t11, t11, E[r + 1]);
-    const T b2 = fma(t21, t21, E[r + 2]);
-    T hr0, hr1, hr2, hl0, hl1, hl2;
-    weno_uhat_diff(w[r + 2], D[r], D[r + 1], D[r + 2], D[r + 3], hr0, hr1, hr2,
-                   hl0, hl1, hl2);
-    if constexpr (LEARNED) {
-      // WENO-JS weights only for shock cells (the same arithmetic as the
-      // WENO-JS kernels, so that regime reproduces them bit for bit), the
-      // ideal weights for smooth ones; learned cells get theirs below
-      regime[r] = mlp_regime(b0, b1, b2, eps4, p.tau, p.sigma);
-      if (regime[r] == 0) {
-        weno_combine_pair(hr0, hr1, hr2, hl0, hl1, hl2, b0, b1, b2, eps4, ul[r],
-                          ur[r]);
-      } else {
-        ur[r] = mlp_combine_ideal(hr0, hr1, hr2);
-        ul[r] = mlp_combine_ideal(hl0, hl1, hl2);
-      }
-    } else {
-      weno_combine_pair(hr0, hr1, hr2, hl0, hl1, hl2, b0, b1, b2, eps4, ul[r],
-                        ur[r]);
     }
   }
   if constexpr (LEARNED) learned_cells<T, CPT>(src, w, regime, ul, ur, q);

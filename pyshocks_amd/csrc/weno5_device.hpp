@@ -213,6 +213,26 @@ __device__ __forceinline__ void weno_diff_divide(const WenoDiffSums &s, double h
   ul = fma(s.nl * s.sr, inv, hl1);
 }
 
+// The third-difference form in FP32 (fused kernels): the reciprocal weights
+// r_k = 1 / e_k^2 (the product form would underflow), right p12 : 6 p02 :
+// 3 p01 becoming r0 : 6 r1 : 3 r2 and left r2 : 6 r1 : 3 r0, so that
+//   ur = hr1 - (r0 T0/3 + r2 T1/2) / sr,  ul = hl1 + (r2 T1/3 + r0 T0/2) / sl.
+__device__ __forceinline__ void weno_diff_pair_f32(float e0, float e1, float e2,
+                                                   float t0_3, float t0_2, float t1_3,
+                                                   float t1_2, float hr1, float hl1,
+                                                   float &ul, float &ur) {
+  const float r0 = __builtin_amdgcn_rcpf(e0 * e0);
+  const float r1 = __builtin_amdgcn_rcpf(e1 * e1);
+  const float r2 = __builtin_amdgcn_rcpf(e2 * e2);
+  const float a1 = 6.0f * r1;
+  const float sr = fmaf(3.0f, r2, r0 + a1);
+  const float sl = fmaf(3.0f, r0, r2 + a1);
+  const float nr = fmaf(r0, t0_3, r2 * t1_2);
+  const float nl = fmaf(r2, t1_3, r0 * t0_2);
+  ur = fmaf(-nr, __builtin_amdgcn_rcpf(sr), hr1);
+  ul = fmaf(nl, __builtin_amdgcn_rcpf(sl), hl1);
+}
+
 // The reference form from q_k = (eps + beta_k)^2: safe over the whole
 // exponent range.
 __device__ __forceinline__ void weno_pair_ref_q(double hr0, double hr1, double hr2,
