@@ -634,11 +634,51 @@ def adjoint_leg(args: argparse.Namespace, repeats: int = 3) -> dict:
     elapsed_b = time.perf_counter() - t0
     if not bool(torch.isfinite(gb).all()):
         raise RuntimeError("non-finite batched gradient")
+
+    # the same gradients through the learned reconstruction (the trained
+    # weights; the adjoint of its surrogate), single and batched
+    from pyshocks_amd.learned import MLPWeights
+    from pyshocks_amd.reconstruction import LearnedWENOJS53
+
+    w = MLPWeights.load()
+    src_l = BoundOperator(Rusanov(rec=LearnedWENOJS53(eps=1e-6, weights=w)), grid,
+                          PeriodicBoundary())
+    target_l = CheckpointedSolve(src_l, u0 + funcs.cell_averages(pert, grid), nsteps, dt,
+                                 every).u_final
+
+    def gradient_l() -> torch.Tensor:
+        solve = CheckpointedSolve(src_l, u0, nsteps, dt, every)
+        return solve.gradient((solve.u_final - target_l) * grid.h)
+
+    def gradients_l() -> torch.Tensor:
+        solve = BatchedCheckpointedSolve(src_l, ub, nsteps, dt, every)
+        return solve.gradient((solve.u_final - target_l[None, :]) * grid.h)
+
+    gradient_l()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        gl = gradient_l()
+    torch.cuda.synchronize()
+    elapsed_l = time.perf_counter() - t0
+    gradients_l()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gbl = gradients_l()
+    torch.cuda.synchronize()
+    elapsed_lb = time.perf_counter() - t0
+    if not (bool(torch.isfinite(gl).all()) and bool(torch.isfinite(gbl).all())):
+        raise RuntimeError("non-finite learned gradient")
     return {"gradients_per_s": repeats / elapsed,
             "ms_per_gradient": elapsed / repeats * 1e3,
             "batched": {"batch": batch, "gradients_per_s": batch / elapsed_b,
                         "ms_per_batch": elapsed_b * 1e3,
                         "note": "independent gradients (own u0) in one launch per step"},
+            "learned": {"gradients_per_s": repeats / elapsed_l,
+                        "batched_gradients_per_s": batch / elapsed_lb,
+                        "tau": w.tau, "sigma": w.sigma,
+                        "note": "the same gradients through LearnedWENOJS53 (trained "
+                                "weights): the adjoint of the learned solver's surrogate"},
             "config": f"burgers rusanov N={n} f64, {nsteps} steps fixed dt, "
                       f"checkpoints every {every}, J = 0.5||u(T)-u_target||^2 dx"}
 
@@ -823,10 +863,77 @@ def learned_leg(args: argparse.Namespace, *, ninst: int = 1024, nmin: int = 1024
         out.update({"best": name, "speedup_at_equal_error": speedup,
                     "effective_cell_updates_per_s": m["cell_updates"] / tl,
                     "matched_error_gap": m["l1"] / el - 1.0})
+        # gradients at the same error ratio: the learned adjoint at N = 16384
+        # against the plain one at the matched N
+        n0 = 16384
+        n_plain = int(round(m["scale"] * n0))
+        g_l = _gradient_rate(n0, w)
+        g_p = _gradient_rate(n_plain, None)
+        out["gradients_at_equal_error"] = {
+            "learned": g_l, "plain": g_p,
+            "speedup": g_l["gradients_per_s"] / g_p["gradients_per_s"],
+            "note": f"batched gradients (batch 32) of J = 0.5||u(T) - u_target||^2 dx over "
+                    f"the horizon of 2000 steps at N = {n0}, Courant 0.5: the learned "
+                    f"FP64 adjoint at N = {n0} against the plain one at N = {n_plain} "
+                    f"(the scale at which the plain forward solve matches the learned "
+                    f"L1 error above)"}
     else:
         out.update({"best": name, "speedup_at_equal_error": None,
                     "effective_cell_updates_per_s": None})
     return out
+
+
+def _gradient_rate(n: int, weights, batch: int = 32, n_ref: int = 16384,
+                   steps_ref: int = 2000, every: int = 50) -> dict:
+    """Batched gradients/s of J = 0.5 ||u(T) - u_target||^2 dx for periodic
+    Burgers Rusanov FP64 (WENO-JS5, or the learned weights) at N = n, over
+    the horizon of steps_ref steps at N = n_ref with Courant number 0.5:
+    instance b starts from 0.5 + sin(2 pi x) plus a seeded perturbation of
+    amplitude 1e-3 and is compared with the solve from the unperturbed
+    state."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from pyshocks_amd import funcs, make_uniform_cell_grid
+    from pyshocks_amd.adjoint import BatchedCheckpointedSolve, CheckpointedSolve
+    from pyshocks_amd.burgers import Rusanov
+    from pyshocks_amd.reconstruction import WENOJS53, LearnedWENOJS53
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import BoundOperator
+
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, device="cuda")
+    rec = WENOJS53(eps=1e-6) if weights is None else LearnedWENOJS53(eps=1e-6, weights=weights)
+    src = BoundOperator(Rusanov(rec=rec), grid, PeriodicBoundary())
+    u0 = funcs.cell_averages(funcs.sine_offset(0.5), grid)
+    umax = float(u0[3:-3].abs().max())
+    t_end = steps_ref * 0.5 / n_ref / umax
+    nsteps = int(math.ceil(t_end / (0.5 * grid.h / umax)))
+    dt = t_end / nsteps
+    target = CheckpointedSolve(src, u0, nsteps, dt, every).u_final
+    ub = []
+    for b in range(batch):
+        rb = np.random.default_rng(2000 + b)
+        ub.append(u0 + funcs.cell_averages(funcs.SineStep(
+            c=0.0, amplitudes=tuple(0.001 * rb.uniform(-1, 1, 4)), wavenumbers=(1, 2, 3, 4),
+            phases=tuple(rb.uniform(0, 2 * math.pi, 4))), grid))
+    ub = torch.stack(ub)
+
+    def gradients() -> torch.Tensor:
+        solve = BatchedCheckpointedSolve(src, ub, nsteps, dt, every)
+        return solve.gradient((solve.u_final - target[None, :]) * grid.h)
+
+    gradients()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = gradients()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if not bool(torch.isfinite(g).all()):
+        raise RuntimeError("non-finite gradient")
+    return {"n": n, "nsteps": nsteps, "batch": batch, "seconds": elapsed,
+            "gradients_per_s": batch / elapsed}
 
 
 def main(argv: list[str] | None = None) -> int:
