@@ -208,8 +208,8 @@ class BatchedSolver:
         self.u.copy_(torch.from_numpy(host))
 
     def set_tfinal(self, tfinal: float | Sequence[float]) -> None:
-        tf = np.broadcast_to(np.asarray(tfinal, dtype=np.float64), (self.ninst,))
-        self._tfinal.copy_(torch.from_numpy(np.ascontiguousarray(tf)))
+        tf = np.array(np.broadcast_to(np.asarray(tfinal, dtype=np.float64), (self.ninst,)))
+        self._tfinal.copy_(torch.from_numpy(tf))
 
     def launch(self) -> None:
         """Enqueue the batched solve on the current stream (no sync)."""
