@@ -338,17 +338,32 @@ typedef struct psa_solver_desc {
   int pad2_;
   double limiter_theta;         /* PSA_LIMITER_MINMOD: theta in [1, 2]     */
   double muscl_atol;            /* PSA_REC_MUSCL: slope cut-off (1e-12)    */
-  int bc;                       /* PSA_BC_PERIODIC, or PSA_BC_GIVEN: the
+  int bc;                       /* PSA_BC_PERIODIC; PSA_BC_GIVEN: the
                                    operator (stage 0) takes the ghost cells
                                    of its input as given (a Dirichlet or
-                                   Neumann fill done beforehand); the
-                                   SSP-RK3 stages are periodic only        */
+                                   Neumann fill done beforehand);
+                                   PSA_BC_NEUMANN / PSA_BC_DIRICHLET: the
+                                   whole-step kernel of psa_ssprk33_steps
+                                   fills the ghost cells of every stage
+                                   input itself (below)                    */
   int pad3_;
   double es_delta;              /* PSA_FLUX_ESWENO: delta of the
                                    dissipative flux                        */
+  /* PSA_BC_NEUMANN (scalar.py:466-511 with a constant g on each side): the
+   * ghost cell at distance e from the boundary (e = 0 innermost) gets the
+   * interior cell at distance e minus (2e + 1) dx bc_left (bc_right on the
+   * right), the reference's u[to] = u[from] + side (x[from] - x[to]) g.
+   * PSA_BC_DIRICHLET (scalar.py:441-460 with a g that does not depend on
+   * t): bc_values points to 2 g device values of the dtype, the left ghost
+   * cells 0 .. g-1 and then the right ones m-g .. m-1. */
+  double bc_left;
+  double bc_right;
+  const void *bc_values;
 } psa_solver_desc;
 #define PSA_BC_PERIODIC 0
 #define PSA_BC_GIVEN 1
+#define PSA_BC_NEUMANN 2
+#define PSA_BC_DIRICHLET 3
 
 /* Step schedules of psa_ssprk33_steps:
  * TILE: one kernel per stage, one workgroup per tile (8 words of HBM traffic

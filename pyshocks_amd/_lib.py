@@ -44,6 +44,8 @@ PSA_REC_MUSCLS = 4
 
 PSA_BC_PERIODIC = 0
 PSA_BC_GIVEN = 1
+PSA_BC_NEUMANN = 2
+PSA_BC_DIRICHLET = 3
 
 (PSA_LIMITER_MINMOD, PSA_LIMITER_MC, PSA_LIMITER_SUPERBEE, PSA_LIMITER_VANALBADA1,
  PSA_LIMITER_VANALBADA2, PSA_LIMITER_VANLEER, PSA_LIMITER_KOREN,
@@ -126,6 +128,9 @@ class PsaSolverDesc(ctypes.Structure):
         ("bc", ctypes.c_int),
         ("pad3_", ctypes.c_int),
         ("es_delta", ctypes.c_double),
+        ("bc_left", ctypes.c_double),
+        ("bc_right", ctypes.c_double),
+        ("bc_values", ctypes.c_void_p),
     ]
 
 

@@ -1127,8 +1127,13 @@ int check_desc(const psa_solver_desc *d) {
     st = check_slope_limiter(d->limiter);
     if (st != PSA_OK) return st;
   }
-  if (d->bc != PSA_BC_PERIODIC && d->bc != PSA_BC_GIVEN)
+  if (d->bc < PSA_BC_PERIODIC || d->bc > PSA_BC_DIRICHLET)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "bad bc " + std::to_string(d->bc));
+  if (d->bc == PSA_BC_NEUMANN && !(std::isfinite(d->bc_left) && std::isfinite(d->bc_right)))
+    return set_error(PSA_ERR_INVALID_ARGUMENT, "the Neumann gradients must be finite");
+  if (d->bc == PSA_BC_DIRICHLET && d->bc_values == nullptr)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "a Dirichlet boundary needs its 2 g ghost values (bc_values)");
   st = check_grid(d->n + 2 * static_cast<int64_t>(d->g), d->g,
                   d->rec == PSA_REC_WENOJS53 ? 3 : 2);
   if (st != PSA_OK) return st;
@@ -1200,6 +1205,10 @@ StageArgs<T> make_args(const psa_solver_desc *d) {
   a.muscls = d->rec == PSA_REC_MUSCLS ? 1 : 0;
   a.es_delta = static_cast<T>(d->es_delta);
   a.nowrap = d->bc == PSA_BC_GIVEN ? 1 : 0;
+  a.bcm = (d->bc == PSA_BC_NEUMANN || d->bc == PSA_BC_DIRICHLET) ? d->bc : PSA_BC_PERIODIC;
+  a.bc_l = static_cast<T>(d->bc_left);
+  a.bc_r = static_cast<T>(d->bc_right);
+  a.bc_vals = static_cast<const T *>(d->bc_values);
   a.limiter = d->limiter;
   a.lim_theta = static_cast<T>(d->limiter_theta);
   a.lim_atol = static_cast<T>(d->muscl_atol);
@@ -1253,6 +1262,21 @@ int run_steps(const psa_solver_desc *d, const void *u_in, void *u_out,
       hipLaunchKernelGGL(k_copy<T>, dim3(blocks_for(a.m, 256)), dim3(256), 0,
                          stream, b1, uo, a.m);
       st = check_launch("copy");
+      if (st != PSA_OK) return st;
+    }
+    if (a.bcm != PSA_BC_PERIODIC) {
+      // the steps fill the ghost cells of every stage input from its interior;
+      // the final state gets them too (the arithmetic of the fill kernels)
+      const unsigned int blocks = blocks_for(a.m, 256);
+      if (a.bcm == PSA_BC_NEUMANN) {
+        hipLaunchKernelGGL(k_fill_neumann<T>, dim3(blocks), dim3(256), 0, stream, uo, uo,
+                           a.m, a.g, a.dx, a.bc_l, a.bc_r, 3);
+      } else {
+        hipLaunchKernelGGL(k_fill_dirichlet<T>, dim3(blocks), dim3(256), 0, stream, uo,
+                           uo, a.m, a.g, a.bc_vals, a.bc_vals + a.g);
+      }
+      st = check_launch("boundary fill");
+      if (st != PSA_OK) return st;
     }
     return st;
   }
@@ -1541,6 +1565,10 @@ int psa_numerical_flux_desc(const psa_solver_desc *desc, const void *u,
   if (desc->mlp != nullptr)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "learned weights have psa_learned_reconstruct");
+  if (desc->bc == PSA_BC_NEUMANN || desc->bc == PSA_BC_DIRICHLET)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the face flux takes the ghost cells as given (PSA_BC_GIVEN after a "
+                     "fill)");
   auto s = static_cast<hipStream_t>(stream);
   const int64_t m = desc->n + 2 * static_cast<int64_t>(desc->g);
   const void *vel = desc->flux == PSA_FLUX_LAX_FRIEDRICHS ? speed : desc->velocity;
@@ -1888,9 +1916,16 @@ int psa_ssprk33_steps(const psa_solver_desc *desc, const void *u_in,
   if (k1 == k2 || k1 == u_in || k2 == u_in || k1 == u_out || k2 == u_out)
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "stage buffers must be distinct from the state");
-  if (desc->bc != PSA_BC_PERIODIC)
+  if (desc->bc == PSA_BC_GIVEN)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
-                     "the fused SSP-RK3 stages are periodic (bc = PSA_BC_PERIODIC)");
+                     "PSA_BC_GIVEN is for the operator (stage 0); the steps fill "
+                     "periodic, Neumann or Dirichlet ghost cells themselves");
+  if ((desc->bc == PSA_BC_NEUMANN || desc->bc == PSA_BC_DIRICHLET) &&
+      !step_applies(desc, tiles_of(desc), vel_mode(desc)))
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "Neumann and Dirichlet fills run in the whole-step kernel only "
+                     "(schedule default or step, no velocity array or Rusanov nu, not "
+                     "Lax-Friedrichs, tiles 256 x 4 or 256 x 2)");
   if (nsteps < 0) return set_error(PSA_ERR_INVALID_ARGUMENT, "nsteps < 0");
   if (nsteps == 0) return PSA_OK;
   auto s = static_cast<hipStream_t>(stream);
@@ -1913,9 +1948,12 @@ int psa_ssprk33_stage(const psa_solver_desc *desc, int stage, const void *un,
     return set_error(PSA_ERR_INVALID_ARGUMENT, "null state pointer");
   if (stage > 0 && control == nullptr && dt == nullptr)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "need a control block or dt");
-  if (stage > 0 && desc->bc != PSA_BC_PERIODIC)
+  if ((stage > 0 && desc->bc != PSA_BC_PERIODIC) || desc->bc == PSA_BC_NEUMANN ||
+      desc->bc == PSA_BC_DIRICHLET)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
-                     "the fused SSP-RK3 stages are periodic (bc = PSA_BC_PERIODIC)");
+                     "the stage kernels are periodic (the operator also takes "
+                     "PSA_BC_GIVEN); Neumann and Dirichlet fills run in "
+                     "psa_ssprk33_steps' whole-step kernel");
   if (out == uk)
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "a stage cannot write into its stencil input");

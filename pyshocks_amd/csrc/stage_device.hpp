@@ -169,7 +169,34 @@ struct StageArgs {
                      //   k2out, its dt at dt_ptr[blockIdx.y]
   int batch;         // host side: instances of the launch (grid.y)
   int step_seq;      // whole-step launch s of a controlled sequence: s mod 6
+  int bcm;           // whole-step kernel: PSA_BC_PERIODIC, PSA_BC_NEUMANN or
+                     //   PSA_BC_DIRICHLET (ghost cells filled by the kernel)
+  T bc_l, bc_r;      // Neumann: the gradients of the left and right sides
+  const T *bc_vals;  // Dirichlet: the 2 g ghost values (left, then right)
 };
+
+// Value of cell gi of a stage input with the non-periodic ghost fill of the
+// whole-step kernel, from the interior cells of u (the state indexed by
+// global cell): zero outside [0, m) (the 'same' convolutions' padding),
+// the fill of scalar.py:441-511 in the ghost cells (the arithmetic of
+// k_fill_neumann / k_fill_dirichlet), u itself inside.
+template <typename T>
+__device__ __forceinline__ T bc_cell(const StageArgs<T> &p, const T *u, int64_t gi) {
+  const int64_t m = p.m;
+  const int g = p.g;
+  if (gi < 0 || gi >= m) return T(0);
+  if (gi < g) {
+    if (p.bcm == PSA_BC_DIRICHLET) return p.bc_vals[gi];
+    const int k = g - 1 - static_cast<int>(gi);
+    return u[g + k] - (T(2 * k + 1) * p.dx) * p.bc_l;
+  }
+  if (gi >= m - g) {
+    if (p.bcm == PSA_BC_DIRICHLET) return p.bc_vals[g + (gi - (m - g))];
+    const int k = static_cast<int>(m - 1 - gi);
+    return u[m - 2 * g + k] + (T(2 * k + 1 - 2 * g) * p.dx) * p.bc_r;
+  }
+  return u[gi];
+}
 
 template <typename T, int VEL>
 __device__ __forceinline__ T velocity_at(const T *s_v, int j, T vconst) {
@@ -1089,6 +1116,21 @@ __device__ __forceinline__ void step_store(const StageArgs<T> &p, int64_t base,
   }
 }
 
+// The ghost cells of a non-periodic boundary in the LDS stage buffer X of a
+// whole-step tile (X[j] is cell base + j) filled from the buffer's interior
+// cells (bc_cell): thread k < g the left cell k, thread g + k the right cell
+// m - g + k, where the tile holds them.
+template <typename T, int BLOCK, int CPT>
+__device__ __forceinline__ void step_refill(const StageArgs<T> &p, int64_t base, T *X) {
+  constexpr int L = StepSmem<T, BLOCK, CPT>::L;
+  const int tid = threadIdx.x;
+  const int g = p.g;
+  if (tid >= 2 * g) return;
+  const int64_t gi = tid < g ? tid : p.m - g + (tid - g);
+  const int64_t j = gi - base;
+  if (j >= 0 && j < L) X[j] = bc_cell(p, X - base, gi);
+}
+
 // One whole SSP-RK3 step (timestepping.py:312-320) of the output tile
 // [A, A + TO) from uin = u^n into uout = u^(n+1) (distinct arrays of p.m
 // cells): the workgroup holds the cells base .. base + C + 8 of u^n,
@@ -1138,13 +1180,16 @@ __device__ __forceinline__ bool step_tile_ctl(const StageArgs<T> &p, int64_t A,
     cu.load(uin + base, tid);
     ctl.issue();
     cu.store(sm.x0, tid);
-  } else {
+  } else if (p.bcm == PSA_BC_PERIODIC) {
     ctl.issue();
     for (int j = tid; j < L; j += BLOCK) {
       int64_t i = (base + j - g) % n;
       if (i < 0) i += n;
       sm.x0[j] = uin[g + i];
     }
+  } else {
+    ctl.issue();
+    for (int j = tid; j < L; j += BLOCK) sm.x0[j] = bc_cell(p, uin, base + j);
   }
   // the k1 entries stage 2 reads outside the cells stage 1 writes, and the
   // exchange entries without a neighbouring thread (their results are
@@ -1164,18 +1209,31 @@ __device__ __forceinline__ bool step_tile_ctl(const StageArgs<T> &p, int64_t A,
   for (int r = 0; r < CPT; ++r) un[r] = sm.x0[j0 + r];
   const MlpSource src{p.mlp, wl};
 
+  // a tile that holds ghost cells of a non-periodic boundary refills them
+  // from each stage's output before the next stage reads it (the interior
+  // cells never read the cells outside [0, m), whose stage values are
+  // meaningless)
+  const bool refill = p.bcm != PSA_BC_PERIODIC && (base < g || base + L > m - g);
   unsigned long long unused = 0;
   step_stage<T, FLUX, VEL, BLOCK, CPT, 1, LEARNED, REC>(p, sm.x0, sm, dt, un, val, src, q);
 #pragma unroll
   for (int r = 0; r < CPT; ++r) sm.x1[j0 + r] = val[r];
   if (p.k1out != nullptr) step_store<T, BLOCK, CPT, false>(p, base, fast, val, p.k1out, unused);
   __syncthreads();
+  if (refill) {
+    step_refill<T, BLOCK, CPT>(p, base, sm.x1);
+    __syncthreads();
+  }
   // (every read of x0 ended before the exchange barrier of stage 1)
   step_stage<T, FLUX, VEL, BLOCK, CPT, 2, LEARNED, REC>(p, sm.x1, sm, dt, un, val, src, q);
 #pragma unroll
   for (int r = 0; r < CPT; ++r) sm.x0[j0 + r] = val[r];
   if (p.k2out != nullptr) step_store<T, BLOCK, CPT, false>(p, base, fast, val, p.k2out, unused);
   __syncthreads();
+  if (refill) {
+    step_refill<T, BLOCK, CPT>(p, base, sm.x0);
+    __syncthreads();
+  }
   step_stage<T, FLUX, VEL, BLOCK, CPT, 3, LEARNED, REC>(p, sm.x0, sm, dt, un, val, src, q);
 
   step_store<T, BLOCK, CPT, true>(p, base, fast, val, uout, vmax);

@@ -244,7 +244,10 @@ def _evaluate_boundary_two_sided(bc: TwoSidedBoundary, grid: Grid, t: Any,
 @dataclass(frozen=True)
 class DirichletBoundary(OneSidedBoundary):
     """``u(t, a) = g(t, x)`` in the ghost cells of ``side``; ``g`` is evaluated
-    on the ghost cell centres (a device tensor) and may return a scalar."""
+    on the ghost cell centres (a device tensor) and may return a scalar. A
+    ``g`` that is not callable (a number, or the ghost values) is the same
+    value at every time: the fused steps then fill the ghost cells on the
+    device."""
 
     g: Any
 
@@ -262,14 +265,27 @@ def _apply_boundary_scalar_dirichlet(bc: DirichletBoundary, grid: Grid, t: Any,
     _lib.require_device(u)
     g = grid.nghosts
     xg = grid.x[:g] if bc.side == -1 else grid.x[grid.n - g:]
-    vals = torch.as_tensor(bc.g(t, xg), dtype=u.dtype, device=u.device)
-    vals = torch.broadcast_to(vals, (g,)).contiguous()
+    vals = dirichlet_values(bc, t, xg, u.dtype)
     out = torch.empty_like(u)
     left, right = (vals, None) if bc.side == -1 else (None, vals)
     _lib.call("psa_apply_boundary_dirichlet", _lib.dtype_code(u.dtype), _lib.ptr(u),
               _lib.ptr(out), u.numel(), g, _lib.ptr(left), _lib.ptr(right),
               _lib.stream_of(u))
     return out
+
+
+def dirichlet_values(bc: DirichletBoundary, t: Any, xg: torch.Tensor,
+                     dtype: torch.dtype) -> torch.Tensor:
+    """The ``nghosts`` ghost values of a Dirichlet side at time ``t`` (``xg``:
+    its ghost cell centres)."""
+    vals = bc.g(t, xg) if callable(bc.g) else bc.g
+    vals = torch.as_tensor(vals, dtype=dtype, device=xg.device)
+    return torch.broadcast_to(vals, xg.shape).contiguous()
+
+
+def neumann_gradient(bc: NeumannBoundary, t: Any) -> float:
+    """``g(t)`` of a Neumann side (the number itself for a constant)."""
+    return float(bc.g(t)) if callable(bc.g) else float(bc.g)
 
 
 def make_dirichlet_boundary(ga: Any, gb: Any = None) -> TwoSidedBoundary:
@@ -283,7 +299,9 @@ def make_dirichlet_boundary(ga: Any, gb: Any = None) -> TwoSidedBoundary:
 @dataclass(frozen=True)
 class NeumannBoundary(OneSidedBoundary):
     """``side du/dx (t, a) = g(t)`` by a mirror of the interior cells
-    (``scalar.py:466-511``); ``g`` returns a number."""
+    (``scalar.py:466-511``); ``g`` returns a number. A ``g`` that is a number
+    is a constant gradient: the fused steps then fill the ghost cells on the
+    device."""
 
     g: Any
 
@@ -300,7 +318,7 @@ def _apply_boundary_scalar_neumann(bc: NeumannBoundary, grid: Grid, t: Any,
     layer, on a uniform grid ``x[from] - x[to] = (from - to) dx``."""
     assert u.numel() == grid.x.numel()
     _lib.require_device(u)
-    gn = float(bc.g(t))
+    gn = neumann_gradient(bc, t)
     out = torch.empty_like(u)
     left = bc.side == -1
     _lib.call("psa_apply_boundary_neumann", _lib.dtype_code(u.dtype), _lib.ptr(u),

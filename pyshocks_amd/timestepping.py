@@ -200,20 +200,8 @@ class FusedPlan:
                   _lib.stream_of(u))
 
     def uses_step_kernel(self) -> bool:
-        """Whether :meth:`steps` runs one whole-step kernel per step (the
-        library's rule: schedule default or step, a constant or no velocity,
-        no Rusanov ``nu``, not Lax-Friedrichs, tiles (256, 4) or (256, 2); by
-        default 2 cells per thread below 2^20 cells, 4 above)."""
-        d = self.desc
-        if d.schedule not in (_lib.PSA_SCHEDULE_DEFAULT, _lib.PSA_SCHEDULE_STEP):
-            return False
-        if d.nu or (d.flux == _lib.PSA_FLUX_ADVECTION_UPWIND and not d.velocity_is_const):
-            return False
-        if d.flux == _lib.PSA_FLUX_LAX_FRIEDRICHS:
-            return False
-        block = d.block if d.block > 0 else 256
-        cpt = d.cells_per_thread if d.cells_per_thread > 0 else (2 if d.n < (1 << 20) else 4)
-        return block == 256 and cpt in (2, 4)
+        """Whether :meth:`steps` runs one whole-step kernel per step."""
+        return _uses_step_kernel(self.desc)
 
     @staticmethod
     def read_control(ctrl: torch.Tensor) -> _lib.PsaControl:
@@ -221,16 +209,74 @@ class FusedPlan:
         return _lib.PsaControl.from_buffer_copy(host)
 
 
-def _scheme_plan(source: Any) -> tuple[_lib.PsaSolverDesc, tuple[Any, ...]] | None:
-    """The descriptor of a periodic bound operator the fused steps run, or
-    ``None``."""
+def _uses_step_kernel(d: _lib.PsaSolverDesc) -> bool:
+    """The library's rule for the whole-step kernel: schedule default or
+    step, a constant or no velocity, no Rusanov ``nu``, not Lax-Friedrichs,
+    tiles (256, 4) or (256, 2); by default 2 cells per thread below 2^20
+    cells, 4 above."""
+    if d.schedule not in (_lib.PSA_SCHEDULE_DEFAULT, _lib.PSA_SCHEDULE_STEP):
+        return False
+    if d.nu or (d.flux == _lib.PSA_FLUX_ADVECTION_UPWIND and not d.velocity_is_const):
+        return False
+    if d.flux == _lib.PSA_FLUX_LAX_FRIEDRICHS:
+        return False
+    block = d.block if d.block > 0 else 256
+    cpt = d.cells_per_thread if d.cells_per_thread > 0 else (2 if d.n < (1 << 20) else 4)
+    return block == 256 and cpt in (2, 4)
+
+
+def _scheme_plan(source: Any, *, fills: bool = False
+                 ) -> tuple[_lib.PsaSolverDesc, tuple[Any, ...]] | None:
+    """The descriptor of a bound operator the fused kernels run, or ``None``:
+    periodic, or with ``fills`` (the whole-step kernel of
+    ``psa_ssprk33_steps``, which fills those ghost cells itself) Neumann or
+    Dirichlet on both sides with values that do not depend on time."""
     from pyshocks_amd.scalar import PeriodicBoundary
 
     if not isinstance(source, BoundOperator):
         return None
-    if not isinstance(source.bc, PeriodicBoundary):
+    if isinstance(source.bc, PeriodicBoundary):
+        return _scheme_desc(source.scheme, source.grid)
+    if not fills:
         return None
-    return _scheme_desc(source.scheme, source.grid)
+    fill = _constant_fill(source.bc, source.grid)
+    if fill is None:
+        return None
+    sd = _scheme_desc(source.scheme, source.grid)
+    if sd is None:
+        return None
+    desc, keep = sd
+    code, left, right, values = fill
+    desc.bc = code
+    desc.bc_left, desc.bc_right = left, right
+    if values is not None:
+        desc.bc_values = values.data_ptr()
+        keep = keep + (values,)
+    return desc, keep
+
+
+def _constant_fill(bc: Any, grid: Any) -> tuple[int, float, float, Any] | None:
+    """``(PSA_BC_*, left gradient, right gradient, ghost values)`` of a
+    two-sided Neumann or Dirichlet boundary whose ``g`` are constants
+    (``scalar.py:452-511``), or ``None``."""
+    from pyshocks_amd.scalar import (DirichletBoundary, NeumannBoundary, TwoSidedBoundary,
+                                     dirichlet_values, neumann_gradient)
+
+    if not isinstance(bc, TwoSidedBoundary):
+        return None
+    sides = (bc.left, bc.right)
+    if any(callable(s.g) for s in sides):
+        return None
+    if all(isinstance(s, NeumannBoundary) for s in sides):
+        return (_lib.PSA_BC_NEUMANN, neumann_gradient(bc.left, 0.0),
+                neumann_gradient(bc.right, 0.0), None)
+    if all(isinstance(s, DirichletBoundary) for s in sides):
+        g = grid.nghosts
+        values = torch.cat([dirichlet_values(bc.left, 0.0, grid.x[:g], grid.dtype),
+                            dirichlet_values(bc.right, 0.0, grid.x[grid.n - g:],
+                                             grid.dtype)]).contiguous()
+        return _lib.PSA_BC_DIRICHLET, 0.0, 0.0, values
+    return None
 
 
 def _scheme_desc(scheme: Any, grid: Any) -> tuple[_lib.PsaSolverDesc, tuple[Any, ...]] | None:
@@ -306,7 +352,7 @@ def fused_plan(stepper: Stepper, *,
     library's tile choice for the stage kernels."""
     if not isinstance(stepper, SSPRK33):
         return None
-    sp = _scheme_plan(stepper.source)
+    sp = _scheme_plan(stepper.source, fills=True)
     if sp is None:
         return None
     desc, keep = sp
@@ -316,6 +362,9 @@ def fused_plan(stepper: Stepper, *,
         tiles = lookup_stage(desc)
     if tiles is not None:
         desc.block, desc.cells_per_thread = int(tiles[0]), int(tiles[1])
+    if desc.bc != _lib.PSA_BC_PERIODIC and not _uses_step_kernel(desc):
+        # the Neumann and Dirichlet fills live in the whole-step kernel only
+        return None
     source: BoundOperator = stepper.source  # type: ignore[assignment]
     grid = source.grid
     npt = _np_dtype(grid.dtype)

@@ -338,3 +338,61 @@ def test_static_solve_limits() -> None:
     assert lib.psa_ssprk33_solve_static(ctypes.byref(desc), 1, 1, 0.0, 1.0, -1,
                                         _lib.PSA_DT_CFL, 1.0, 0.5, 0.0, -1, None,
                                         None) == _lib.PSA_ERR_INVALID_ARGUMENT
+
+
+def test_boundary_fill_plans_and_refusals() -> None:
+    """Constant Neumann / Dirichlet boundaries get a fused plan (the
+    whole-step kernel fills the ghost cells); callables, mixed sides and
+    schemes without the whole-step kernel keep the host loop; every other
+    fused entry point refuses the fills before any device work."""
+    from pyshocks_amd import _lib, make_uniform_cell_grid
+    from pyshocks_amd.burgers import LaxFriedrichs, Rusanov
+    from pyshocks_amd.reconstruction import WENOJS53
+    from pyshocks_amd.scalar import (DirichletBoundary, NeumannBoundary, TwoSidedBoundary,
+                                     make_dirichlet_boundary, make_neumann_boundary)
+    from pyshocks_amd.timestepping import fused_plan, make_ssprk33
+
+    grid = make_uniform_cell_grid(0.0, 1.0, 64, nghosts=3, device="cpu")
+    sch = Rusanov(rec=WENOJS53(eps=1e-6))
+    plan = fused_plan(make_ssprk33(sch, grid, make_neumann_boundary(0.5, -0.25)))
+    assert plan is not None and plan.desc.bc == _lib.PSA_BC_NEUMANN
+    assert (plan.desc.bc_left, plan.desc.bc_right) == (0.5, -0.25)
+    plan = fused_plan(make_ssprk33(sch, grid, make_dirichlet_boundary(1.0, 2.0)))
+    assert plan is not None and plan.desc.bc == _lib.PSA_BC_DIRICHLET
+    (values,) = [k for k in plan.keepalive if getattr(k, "numel", lambda: 0)() == 6]
+    assert values.tolist() == [1.0, 1.0, 1.0, 2.0, 2.0, 2.0]
+    assert fused_plan(make_ssprk33(sch, grid, make_neumann_boundary(lambda t: 0.5))) is None
+    mixed = TwoSidedBoundary(left=NeumannBoundary(side=-1, g=0.5),
+                             right=DirichletBoundary(side=+1, g=1.0))
+    assert fused_plan(make_ssprk33(sch, grid, mixed)) is None
+    lf = LaxFriedrichs(rec=WENOJS53(eps=1e-6))
+    assert fused_plan(make_ssprk33(lf, grid, make_neumann_boundary(0.5))) is None
+
+    lib = _lib.load()
+    desc = _lib.PsaSolverDesc()
+    desc.dtype, desc.flux, desc.n, desc.g, desc.dx = 0, 0, 64, 3, 1 / 64
+    desc.bc = _lib.PSA_BC_NEUMANN
+    p = ctypes.c_void_p
+    assert lib.psa_ssprk33_stage(ctypes.byref(desc), 1, p(8), p(16), p(24), p(32), None,
+                                 None) == _lib.PSA_ERR_NOT_IMPLEMENTED
+    assert lib.psa_ssprk33_stage(ctypes.byref(desc), 0, None, p(16), p(24), None, None,
+                                 None) == _lib.PSA_ERR_NOT_IMPLEMENTED
+    assert lib.psa_ssprk33_advance(ctypes.byref(desc), p(8), p(16), p(24), p(32), p(40),
+                                   None) == _lib.PSA_ERR_NOT_IMPLEMENTED
+    assert lib.psa_numerical_flux_desc(ctypes.byref(desc), p(8), None, p(16), None) == \
+        _lib.PSA_ERR_NOT_IMPLEMENTED
+    desc.bc_left = float("inf")
+    assert lib.psa_ssprk33_steps(ctypes.byref(desc), p(8), p(16), p(24), p(32), p(40), 1,
+                                 None) == _lib.PSA_ERR_INVALID_ARGUMENT
+    desc.bc, desc.bc_left = _lib.PSA_BC_DIRICHLET, 0.0
+    assert lib.psa_ssprk33_steps(ctypes.byref(desc), p(8), p(16), p(24), p(32), p(40), 1,
+                                 None) == _lib.PSA_ERR_INVALID_ARGUMENT
+    assert "bc_values" in _lib.last_error()
+    desc.bc = _lib.PSA_BC_GIVEN
+    assert lib.psa_ssprk33_steps(ctypes.byref(desc), p(8), p(16), p(24), p(32), p(40), 1,
+                                 None) == _lib.PSA_ERR_NOT_IMPLEMENTED
+    # a Neumann step needs the whole-step kernel: a velocity array rules it out
+    desc.bc, desc.flux = _lib.PSA_BC_NEUMANN, _lib.PSA_FLUX_ADVECTION_UPWIND
+    desc.velocity, desc.velocity_is_const = 48, 0
+    assert lib.psa_ssprk33_steps(ctypes.byref(desc), p(8), p(16), p(24), p(32), p(40), 1,
+                                 None) == _lib.PSA_ERR_NOT_IMPLEMENTED
