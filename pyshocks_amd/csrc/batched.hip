@@ -155,7 +155,8 @@ __device__ __forceinline__ bool team_barrier(TeamState *ts, int size,
 // 1 third order, 2 MUSCL, 3 slope-limited MUSCL).
 template <typename T, int FLUX, int BLOCK, int CPT, bool LEARNED, bool WHOLE, int REC>
 __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
-  using Smem = typename std::conditional<WHOLE, StepSmem<T, BLOCK, CPT>,
+  using Smem = typename std::conditional<WHOLE,
+                                         StepSmem<T, BLOCK, CPT, FLUX == PSA_FLUX_ESWENO>,
                                          StageSmem<T, 0, BLOCK, CPT>>::type;
   constexpr int TILE = WHOLE ? StepSmem<T, BLOCK, CPT>::TO : BLOCK * CPT;
   __shared__ Smem sm;

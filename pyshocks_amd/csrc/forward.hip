@@ -741,7 +741,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, BLOCK),
                           amdgpu_waves_per_eu((step_min_waves<T, LEARNED, REC>()))))
 void k_step(StageArgs<T> p) {
   static_assert(VEL != 2, "no velocity array in the whole-step kernel");
-  using SM = StepSmem<T, BLOCK, CPT>;
+  using SM = StepSmem<T, BLOCK, CPT, FLUX == PSA_FLUX_ESWENO>;
   __shared__ SM sm;
   __shared__ unsigned int q[LEARNED ? (BLOCK / 64) * learned_queue_words<CPT>() : 1];
   const int tid = threadIdx.x;

@@ -528,7 +528,7 @@ __device__ __forceinline__ void reconstruct_cells(const StageArgs<T> &p,
 template <typename T, int CPT, bool EDGE>
 __device__ __forceinline__ void reconstruct_cells3(const StageArgs<T> &p,
                                                    const T *w, T *ul, T *ur,
-                                                   int64_t gfirst) {
+                                                   int64_t gfirst, T *om = nullptr) {
   // D[k] = u(k) - u(k - 1)
   T D[CPT + 1];
 #pragma unroll
@@ -541,7 +541,8 @@ __device__ __forceinline__ void reconstruct_cells3(const StageArgs<T> &p,
       const int64_t gm = gfirst + r;
       tz = gm == 0 || gm == p.m - 1;
     }
-    weno3_pair(w[r + 2], D[r], D[r + 1], p.eps, es, tz, ul[r], ur[r]);
+    weno3_pair(w[r + 2], D[r], D[r + 1], p.eps, es, tz, ul[r], ur[r],
+               om != nullptr ? &om[r] : nullptr);
   }
 }
 
@@ -594,9 +595,9 @@ __device__ __forceinline__ void reconstruct_any(const StageArgs<T> &p,
                                                 const T *w, T *ul, T *ur,
                                                 const MlpSource &src,
                                                 unsigned int *q,
-                                                int64_t gfirst) {
+                                                int64_t gfirst, T *om = nullptr) {
   if constexpr (REC == 1) {
-    reconstruct_cells3<T, CPT, EDGE>(p, w, ul, ur, gfirst);
+    reconstruct_cells3<T, CPT, EDGE>(p, w, ul, ur, gfirst, om);
   } else if constexpr (REC == 2) {
     reconstruct_cells_muscl<T, CPT, EDGE>(p, w, ul, ur, gfirst);
   } else if constexpr (REC == 3) {
@@ -782,7 +783,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   for (int k = 0; k < CPT + 4; ++k) w[k] = sm.u[l0 + 2 + k];
   T ul[CPT], ur[CPT];
   const MlpSource src{p.mlp, wl};
-  reconstruct_any<T, CPT, LEARNED, REC, EDGE>(p, w, ul, ur, src, q, S + l0);
+  // ES-WENO: the reconstruction also gives each cell's first weight
+  static_assert(FLUX != PSA_FLUX_ESWENO || REC == 1,
+                "the ES-WENO flux takes its weights from the ES-WENO3 reconstruction");
+  T om[FLUX == PSA_FLUX_ESWENO ? CPT : 1];
+  reconstruct_any<T, CPT, LEARNED, REC, EDGE>(p, w, ul, ur, src, q, S + l0,
+                                              FLUX == PSA_FLUX_ESWENO ? om : nullptr);
 
   T al[VEL == 2 ? CPT : 1], ar[VEL == 2 ? CPT : 1];
   if constexpr (VEL == 2 && REC != 0) {
@@ -864,15 +870,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
   // each shared by two faces
   T esw[FLUX == PSA_FLUX_ESWENO ? CPT + 2 : 1];
   if constexpr (FLUX == PSA_FLUX_ESWENO) {
-#pragma unroll
-    for (int r = 0; r < CPT + 2; ++r) {
-      bool end = false;
-      if constexpr (EDGE) {
-        const int64_t gc = S + l0 - 1 + r;
-        end = gc == 0 || gc == m - 1;
-      }
-      esw[r] = es_weight0_fused(w[r], w[r + 1], w[r + 2], p.eps, end);
+    // the thread's own cells from the reconstruction, the two neighbours'
+    // with the same arithmetic (es_omega0_pair), so a face's two threads
+    // agree bit for bit
+    bool end0 = false, end1 = false;
+    if constexpr (EDGE) {
+      const int64_t g0 = S + l0 - 1, g1 = S + l0 + CPT;
+      end0 = g0 == 0 || g0 == m - 1;
+      end1 = g1 == 0 || g1 == m - 1;
     }
+    esw[0] = es_omega0_pair(w[1] - w[0], w[2] - w[1], p.eps, end0);
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) esw[r + 1] = om[r];
+    esw[CPT + 1] = es_omega0_pair(w[CPT + 2] - w[CPT + 1], w[CPT + 3] - w[CPT + 2], p.eps,
+                                  end1);
   }
   T F[CPT + 1];
   const bool has_nu = (FLUX == PSA_FLUX_RUSANOV ||
@@ -998,7 +1009,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs<T> &p, int64_t S,
 // ---------------------------------------------------------------------------
 // whole-step tile
 
-template <typename T, int BLOCK, int CPT>
+// ES: the ES-WENO flux, whose threads also exchange the first weight of
+// their edge cells (omL, omR).
+template <typename T, int BLOCK, int CPT, bool ES = false>
 struct StepSmem {
   static constexpr int C = BLOCK * CPT;  // cells computed per stage
   static constexpr int L = C + 8;        // cells held per buffer
@@ -1007,6 +1020,8 @@ struct StepSmem {
   __align__(16) T x1[L];                 // k1
   T urL[BLOCK + 1];
   T ulF[BLOCK + 1];
+  T omL[ES ? BLOCK + 1 : 1];
+  T omR[ES ? BLOCK + 1 : 1];
   unsigned long long red[BLOCK / 64];
 };
 
@@ -1017,7 +1032,9 @@ struct StepSmem {
 template <typename T, int FLUX, int VEL, int BLOCK, int CPT, int STAGE,
           bool LEARNED, int REC = 0>
 __device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
-                                           StepSmem<T, BLOCK, CPT> &sm, T dt,
+                                           StepSmem<T, BLOCK, CPT,
+                                                    FLUX == PSA_FLUX_ESWENO> &sm,
+                                           T dt,
                                            const T *un, T *val,
                                            const MlpSource &src,
                                            unsigned int *q) {
@@ -1027,17 +1044,28 @@ __device__ __forceinline__ void step_stage(const StageArgs<T> &p, const T *X,
 #pragma unroll
   for (int k = 0; k < CPT + 4; ++k) w[k] = X[j0 - 2 + k];
   T ul[CPT], ur[CPT];
-  reconstruct_any<T, CPT, LEARNED, REC>(p, w, ul, ur, src, q, 0);
+  static_assert(FLUX != PSA_FLUX_ESWENO || REC == 1,
+                "the ES-WENO flux takes its weights from the ES-WENO3 reconstruction");
+  T om[FLUX == PSA_FLUX_ESWENO ? CPT : 1];
+  reconstruct_any<T, CPT, LEARNED, REC>(p, w, ul, ur, src, q, 0,
+                                        FLUX == PSA_FLUX_ESWENO ? om : nullptr);
 
   sm.urL[tid + 1] = ur[CPT - 1];
   sm.ulF[tid] = ul[0];
+  if constexpr (FLUX == PSA_FLUX_ESWENO) {
+    sm.omL[tid + 1] = om[CPT - 1];
+    sm.omR[tid] = om[0];
+  }
   __syncthreads();
 
+  // ES-WENO: the first weight of cells j0 - 1 .. j0 + CPT, the outer two from
+  // the neighbouring threads
   T esw[FLUX == PSA_FLUX_ESWENO ? CPT + 2 : 1];
   if constexpr (FLUX == PSA_FLUX_ESWENO) {
+    esw[0] = sm.omL[tid];
 #pragma unroll
-    for (int r = 0; r < CPT + 2; ++r)
-      esw[r] = es_weight0_fused(w[r], w[r + 1], w[r + 2], p.eps, false);
+    for (int r = 0; r < CPT; ++r) esw[r + 1] = om[r];
+    esw[CPT + 1] = sm.omR[tid + 1];
   }
   T F[CPT + 1];
 #pragma unroll
@@ -1157,12 +1185,13 @@ __device__ __forceinline__ bool step_tile_ctl(const StageArgs<T> &p, int64_t A,
                                               const Ctl &ctl,
                                               const T *__restrict__ uin,
                                               T *__restrict__ uout,
-                                              StepSmem<T, BLOCK, CPT> &sm,
+                                              StepSmem<T, BLOCK, CPT,
+                                                       FLUX == PSA_FLUX_ESWENO> &sm,
                                               unsigned long long &vmax,
                                               unsigned int *q,
                                               const MlpWaveLds *wl) {
   static_assert(VEL != 2, "no velocity array in the whole-step tile");
-  using SM = StepSmem<T, BLOCK, CPT>;
+  using SM = StepSmem<T, BLOCK, CPT, FLUX == PSA_FLUX_ESWENO>;
   constexpr int C = SM::C;
   constexpr int L = SM::L;
   constexpr int VN = Vec16<T>::n;
@@ -1198,6 +1227,10 @@ __device__ __forceinline__ bool step_tile_ctl(const StageArgs<T> &p, int64_t A,
   if (tid == 0) {
     sm.urL[0] = T(0);
     sm.ulF[BLOCK] = T(0);
+    if constexpr (FLUX == PSA_FLUX_ESWENO) {
+      sm.omL[0] = T(0);
+      sm.omR[BLOCK] = T(0);
+    }
   }
   __syncthreads();
   if (!ctl.go()) return false;
@@ -1254,7 +1287,8 @@ template <typename T, int FLUX, int VEL, int BLOCK, int CPT, bool LEARNED,
 __device__ __forceinline__ void step_tile(const StageArgs<T> &p, int64_t A, T dt,
                                           const T *__restrict__ uin,
                                           T *__restrict__ uout,
-                                          StepSmem<T, BLOCK, CPT> &sm,
+                                          StepSmem<T, BLOCK, CPT,
+                                                   FLUX == PSA_FLUX_ESWENO> &sm,
                                           unsigned long long &vmax,
                                           unsigned int *q,
                                           const MlpWaveLds *wl) {

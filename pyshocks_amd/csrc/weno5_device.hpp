@@ -469,51 +469,30 @@ __device__ __forceinline__ T es_weight(T um1, T u0, T up1, T eps, bool end) {
   return (K == 0 ? a0 : a1) / (a0 + a1);
 }
 
-// es_weight<T, 0> of the fused kernels.
-// FP64, one division: a_k = d_k (e_k + tau) / e_k with e_k = eps + beta_k, so
-// w_0 = d_0 (e_0 + tau) e_1 / (d_0 (e_0 + tau) e_1 + d_1 (e_1 + tau) e_0);
-// the products scale as the fourth power of the state differences, which
-// FP64's exponent range holds for any state FP32 could hold (rounding
-// differs from the reference's order by a few ulps of the weight).
-// FP32: that product would overflow for differences above ~1e9 and lose the
-// eps term below eps ~ 1e-20, so FP32 keeps the reference's scale-free form
-// a_k = d_k (1 + tau / e_k) with hardware reciprocals (1 ulp each).
-template <typename T>
-__device__ __forceinline__ T es_weight0_fused(T um1, T u0, T up1, T eps, bool end);
-
-template <>
-__device__ __forceinline__ double es_weight0_fused<double>(double um1, double u0,
-                                                           double up1, double eps,
-                                                           bool end) {
-  using C = Weno3Coeffs<double>;
-  const double t0 = u0 - um1;
-  const double t1 = up1 - u0;
-  const double tau = end ? 0.0 : weno3_tau_ref(um1, u0, up1);
-  const double e0 = eps + t0 * t0;
-  const double e1 = eps + t1 * t1;
-  const double n0 = C::d0 * (e0 + tau) * e1;
-  const double n1 = C::d1 * (e1 + tau) * e0;
-  return n0 / (n0 + n1);
+// sqrt of a positive normal FP64 value: the hardware reciprocal square root
+// (about 24 bits) refined by one Goldschmidt step (s = x y, h = y / 2,
+// r = 1/2 - s h, s += s r, h += h r: ~48 bits) and two corrections
+// s += (x - s^2) h, each squaring the relative error; anything else goes to
+// the library's sqrt.
+__device__ __forceinline__ double sqrt_pos(double x) {
+  if (!__builtin_amdgcn_class(x, 0x100)) return sqrt(x);
+  const double y = __builtin_amdgcn_rsq(x);
+  double s = x * y;
+  double h = 0.5 * y;
+  const double r = fma(-s, h, 0.5);
+  s = fma(s, r, s);
+  h = fma(h, r, h);
+  s = fma(fma(-s, s, x), h, s);
+  return fma(fma(-s, s, x), h, s);
 }
-
-template <>
-__device__ __forceinline__ float es_weight0_fused<float>(float um1, float u0, float up1,
-                                                         float eps, bool end) {
-  using C = Weno3Coeffs<float>;
-  const float t0 = u0 - um1;
-  const float t1 = up1 - u0;
-  const float tau = end ? 0.0f : weno3_tau_ref(um1, u0, up1);
-  const float a0 = C::d0 * fmaf(tau, __builtin_amdgcn_rcpf(eps + t0 * t0), 1.0f);
-  const float a1 = C::d1 * fmaf(tau, __builtin_amdgcn_rcpf(eps + t1 * t1), 1.0f);
-  return a0 * __builtin_amdgcn_rcpf(a0 + a1);
-}
+__device__ __forceinline__ float sqrt_pos(float x) { return sqrtf(x); }
 
 // The dissipative flux from the two cells' weights wl, wr and values v1, v2.
 template <typename T>
 __device__ __forceinline__ T es_dissipation_w(T wl, T wr, T v1, T v2, T delta) {
   const T dw = wr - wl;
-  const T mu = sqrt(dw * dw + delta * delta) / T(8);
-  return -(mu + dw / T(8)) * (v2 - v1);
+  const T mu = sqrt_pos(fma(dw, dw, delta * delta)) * T(0.125);
+  return -(mu + dw * T(0.125)) * (v2 - v1);
 }
 
 // The dissipative flux of ES-WENO at the face between cells v1 and v2 from
@@ -571,6 +550,31 @@ __device__ __forceinline__ void weno3_divide<float>(float nr, float dr, float nl
   ul = nl * __builtin_amdgcn_rcpf(dl);
 }
 
+// The right state's first weight w0r / dr of weno3_pair's ES-WENO branch,
+// with the same arithmetic (the shared reciprocal of dr dl in FP64), so that
+// a cell's weight has the same bits whichever thread forms it.
+template <typename T>
+__device__ __forceinline__ T weno3_omega0(T w0r, T dr, T dl);
+
+template <>
+__device__ __forceinline__ double weno3_omega0<double>(double w0r, double dr, double dl) {
+  const double den = dr * dl;
+  if (__builtin_amdgcn_class(den, 0x100)) {
+    double inv = __builtin_amdgcn_rcp(den);
+    double e = fma(-den, inv, 1.0);
+    inv = fma(inv, e, inv);
+    e = fma(-den, inv, 1.0);
+    inv = fma(inv, e, inv);
+    return (w0r * dl) * inv;
+  }
+  return w0r / dr;
+}
+
+template <>
+__device__ __forceinline__ float weno3_omega0<float>(float w0r, float dr, float) {
+  return w0r * __builtin_amdgcn_rcpf(dr);
+}
+
 // Both states of one cell for the fused kernels from the first differences
 // Dm = u_i - u_{i-1}, Dp = u_{i+1} - u_i: with q_k the JS denominators
 // (eps + beta_k)^2, omega_0 = q1 / (q1 + 2 q0) on the right and
@@ -579,7 +583,8 @@ __device__ __forceinline__ void weno3_divide<float>(float nr, float dr, float nl
 // mirrored. The candidates are u_i plus half a difference.
 template <typename T>
 __device__ __forceinline__ void weno3_pair(T u0, T Dm, T Dp, T eps, bool es,
-                                           bool tau_zero, T &ul, T &ur) {
+                                           bool tau_zero, T &ul, T &ur,
+                                           T *omega0 = nullptr) {
   const T r0 = fma(Dm, Dm, eps);  // eps + beta0
   const T r1 = fma(Dp, Dp, eps);  // eps + beta1
   // right: uhat0 = u + Dm / 2, uhat1 = u + Dp / 2; left: u - Dp / 2, u - Dm / 2
@@ -614,8 +619,56 @@ __device__ __forceinline__ void weno3_pair(T u0, T Dm, T Dp, T eps, bool es,
     w0l = q0;
     w1l = T(2) * q1;
   }
-  weno3_divide(fma(w0r, hr0, w1r * hr1), w0r + w1r, fma(w0l, hl0, w1l * hl1),
-               w0l + w1l, ur, ul);
+  const T nr = fma(w0r, hr0, w1r * hr1), dr = w0r + w1r;
+  const T nl = fma(w0l, hl0, w1l * hl1), dl = w0l + w1l;
+  if constexpr (std::is_same<T, double>::value) {
+    if (omega0 != nullptr) {
+      // one reciprocal of dr dl for both states and the weight, with
+      // weno3_omega0's arithmetic
+      const double den = dr * dl;
+      if (__builtin_amdgcn_class(den, 0x100)) {
+        double inv = __builtin_amdgcn_rcp(den);
+        double e = fma(-den, inv, 1.0);
+        inv = fma(inv, e, inv);
+        e = fma(-den, inv, 1.0);
+        inv = fma(inv, e, inv);
+        ur = (nr * dl) * inv;
+        ul = (nl * dr) * inv;
+        *omega0 = (w0r * dl) * inv;
+      } else {
+        ur = nr / dr;
+        ul = nl / dl;
+        *omega0 = w0r / dr;
+      }
+      return;
+    }
+  }
+  weno3_divide(nr, dr, nl, dl, ur, ul);
+  if (omega0 != nullptr) *omega0 = weno3_omega0(w0r, dr, dl);
+}
+
+// The first ES-WENO weight of the right state of cell i (the weight of the
+// dissipative flux, burgers/schemes.py:240-252), from Dm = u_i - u_{i-1},
+// Dp = u_{i+1} - u_i, with weno3_pair's arithmetic.
+template <typename T>
+__device__ __forceinline__ T es_omega0_pair(T Dm, T Dp, T eps, bool tau_zero) {
+  const T r0 = fma(Dm, Dm, eps);
+  const T r1 = fma(Dp, Dp, eps);
+  const T dd = Dp - Dm;
+  const T tau = tau_zero ? T(0) : dd * dd;
+  if constexpr (std::is_same<T, float>::value) {
+    // the products in statements of their own, as in weno3_pair (a
+    // multiply-add fuses within an expression only)
+    const float a0 = fmaf(tau, __builtin_amdgcn_rcpf(r0), 1.0f);
+    const float a1 = fmaf(tau, __builtin_amdgcn_rcpf(r1), 1.0f);
+    const float w1r = 2.0f * a1, w1l = 2.0f * a0;
+    return weno3_omega0(a0, a0 + w1r, a1 + w1l);
+  } else {
+    const T p0 = r0 + tau, p1 = r1 + tau;
+    const T w0r = p0 * r1, w1r = T(2) * (p1 * r0);
+    const T w0l = p1 * r0, w1l = T(2) * (p0 * r1);
+    return weno3_omega0(w0r, w0r + w1r, w0l + w1l);
+  }
 }
 
 // ---------------------------------------------------------------------------
