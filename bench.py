@@ -669,6 +669,8 @@ def adjoint_leg(args: argparse.Namespace, repeats: int = 3) -> dict:
     elapsed_lb = time.perf_counter() - t0
     if not (bool(torch.isfinite(gl).all()) and bool(torch.isfinite(gbl).all())):
         raise RuntimeError("non-finite learned gradient")
+    cpu = adjoint_cpu_baseline(args, u0.cpu().numpy(), target.cpu().numpy(), g.cpu().numpy(),
+                               n=n, dx=grid.h, dt=dt, nsteps=nsteps, every=every)
     return {"gradients_per_s": repeats / elapsed,
             "ms_per_gradient": elapsed / repeats * 1e3,
             "batched": {"batch": batch, "gradients_per_s": batch / elapsed_b,
@@ -679,8 +681,51 @@ def adjoint_leg(args: argparse.Namespace, repeats: int = 3) -> dict:
                         "tau": w.tau, "sigma": w.sigma,
                         "note": "the same gradients through LearnedWENOJS53 (trained "
                                 "weights): the adjoint of the learned solver's surrogate"},
+            "cpu_baseline": cpu,
             "config": f"burgers rusanov N={n} f64, {nsteps} steps fixed dt, "
                       f"checkpoints every {every}, J = 0.5||u(T)-u_target||^2 dx"}
+
+
+def adjoint_cpu_baseline(args: argparse.Namespace, u0, target, gpu_grad, *, n: int,
+                         dx: float, dt: float, nsteps: int, every: int) -> dict | None:
+    """The C reverse mode of the oracle on the adjoint workload: the whole
+    gradient on all host cores (checked against the GPU's), and a 200-step
+    sample on one core, scaled to the full horizon (the cost is linear in
+    the steps). The reference itself forms a dense n x n step Jacobian with
+    jax.jacfwd (timestepping.py:155-214), O(n^2) work per step, which the
+    matrix-free restatement does not reproduce."""
+    if args.cpu_seconds <= 0:
+        return None
+    import numpy as np
+
+    from oracle import build as oracle_build
+    from oracle import ref_c
+
+    if not ref_c.LIB.exists():
+        oracle_build.build(verbose=False)
+    threads = ref_c.num_threads()
+    t0 = time.perf_counter()
+    cgrad, _, _ = ref_c.adjoint_rusanov(u0, target, n=n, g=3, dx=dx, eps=1e-6, dt=dt,
+                                        nsteps=nsteps, every=every)
+    full_s = time.perf_counter() - t0
+    rel = float(np.linalg.norm(cgrad[3:-3] - gpu_grad[3:-3])
+                / np.linalg.norm(cgrad[3:-3]))
+    sample = 200
+    ref_c.set_num_threads(1)
+    try:
+        t0 = time.perf_counter()
+        ref_c.adjoint_rusanov(u0, target, n=n, g=3, dx=dx, eps=1e-6, dt=dt, nsteps=sample,
+                              every=every)
+        one_s = (time.perf_counter() - t0) * nsteps / sample
+    finally:
+        ref_c.set_num_threads(threads)
+    return {"value": 1.0 / full_s, "unit": "gradients/s", "cores": threads, "kind": "port",
+            "sample": f"C reverse mode of the oracle (forward with checkpoints every {every}, "
+                      f"window recompute, three stage transposes per step), the whole "
+                      f"{nsteps}-step gradient, {full_s:.1f} s",
+            "gradient_rel_l2_vs_gpu": rel,
+            "one_core": {"value": 1.0 / one_s, "unit": "gradients/s", "cores": 1,
+                         "sample": f"{sample} steps on one core, scaled to {nsteps}"}}
 
 
 def make_rec(args: argparse.Namespace):

@@ -36,6 +36,17 @@ def load() -> ctypes.CDLL:
             ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_int64, ctypes.c_int,
             ctypes.c_double, ctypes.c_double]
         lib.oracle_apply_operator.restype = None
+        lib.oracle_operator_vjp.argtypes = [dp, dp, dp, ctypes.c_int64, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_double]
+        lib.oracle_operator_vjp.restype = None
+        lib.oracle_ssprk33_step_vjp.argtypes = [dp, dp, dp, ctypes.c_int64, ctypes.c_int,
+                                                ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double]
+        lib.oracle_ssprk33_step_vjp.restype = None
+        lib.oracle_adjoint_rusanov.argtypes = [
+            dp, dp, dp, dp, dp, ctypes.c_int64, ctypes.c_int, ctypes.c_double,
+            ctypes.c_double, ctypes.c_double, ctypes.c_longlong, ctypes.c_longlong]
+        lib.oracle_adjoint_rusanov.restype = ctypes.c_int
         lib.oracle_num_threads.restype = ctypes.c_int
         lib.oracle_set_num_threads.argtypes = [ctypes.c_int]
         _lib = lib
@@ -94,3 +105,44 @@ def solve(kind: str, u0: np.ndarray, *, n: int, g: int, dx: float, eps: float,
     if status != 0:
         raise ValueError("Time step is not finite")
     return u, t.value, it.value, dts[: min(it.value, dts.size)]
+
+
+def operator_vjp(u: np.ndarray, mu: np.ndarray, *, g: int, dx: float,
+                 eps: float) -> np.ndarray:
+    """``J_L(u)^T mu`` of the Rusanov WENO-JS5 operator (periodic); ghost
+    entries of the result are zero."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    out = np.empty_like(u)
+    load().oracle_operator_vjp(_p(u), _p(mu), _p(out), u.size, g, dx, eps)
+    return out
+
+
+def step_vjp(u: np.ndarray, lam: np.ndarray, *, g: int, dx: float, eps: float,
+             dt: float) -> np.ndarray:
+    """The transpose of one fixed-dt SSP-RK3 step's Jacobian at ``u`` applied
+    to ``lam`` (Rusanov, WENO-JS5, periodic)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    out = np.empty_like(u)
+    load().oracle_ssprk33_step_vjp(_p(u), _p(lam), _p(out), u.size, g, dx, eps, dt)
+    return out
+
+
+def adjoint_rusanov(u0: np.ndarray, target: np.ndarray, *, n: int, g: int, dx: float,
+                    eps: float, dt: float, nsteps: int,
+                    every: int = 50) -> tuple[np.ndarray, np.ndarray, float]:
+    """Gradient of ``J = 0.5 sum_interior (u(T) - target)^2 dx`` with respect
+    to ``u0`` after ``nsteps`` fixed-dt steps, with checkpoints every
+    ``every`` steps; returns ``(grad, u_final, J)``."""
+    u0 = np.ascontiguousarray(u0, dtype=np.float64)
+    target = np.ascontiguousarray(target, dtype=np.float64)
+    grad = np.empty_like(u0)
+    uf = np.empty_like(u0)
+    J = ctypes.c_double(0.0)
+    status = load().oracle_adjoint_rusanov(_p(u0), _p(target), _p(grad), _p(uf),
+                                           ctypes.byref(J), n, g, dx, eps, dt, nsteps,
+                                           every)
+    if status != 0:
+        raise MemoryError("oracle_adjoint_rusanov: out of memory")
+    return grad, uf, J.value

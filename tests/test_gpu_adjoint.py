@@ -279,6 +279,36 @@ def test_gradient_vs_central_fd(torch_mod) -> None:
         assert abs(fd - ad) <= 1e-4 * max(abs(ad), 1e-12), (k, fd, ad)
 
 
+def test_gradient_vs_c_reverse_mode(torch_mod) -> None:
+    """The checkpointed GPU gradient through the shock of u0 = 0.5 +
+    sin(2 pi x) against the C reverse mode of the oracle (its own forward
+    pass in the reference's expression order, whose states differ from the
+    GPU's by rounding)."""
+    torch = torch_mod
+    from oracle import build as oracle_build
+    from oracle import ref_c
+    from pyshocks_amd.adjoint import checkpointed_solve
+
+    if not ref_c.LIB.exists():
+        oracle_build.build(verbose=False)
+    n, nsteps, every = 1024, 300, 50
+    x = (np.arange(n + 6) - 2.5) / n
+    u0 = 0.5 + np.sin(2 * np.pi * x)
+    target = u0 + 0.2 * np.sin(4 * np.pi * x + 1.0)
+    dt = 0.5 * (1.0 / n) / 1.5
+    src, _ = _source(torch, "rusanov", n)
+    solve = checkpointed_solve(src, _dev(torch, u0), nsteps, dt, every=every)
+    tgt = _dev(torch, target)
+    grad = _host(solve.gradient((solve.u_final - tgt) * (1.0 / n)))
+    cgrad, cu, _ = ref_c.adjoint_rusanov(u0, target, n=n, g=3, dx=1.0 / n, eps=1e-6, dt=dt,
+                                         nsteps=nsteps, every=every)
+    fwd_rel = _rel(_host(solve.u_final)[3:-3], cu[3:-3])
+    rel = _rel(grad[3:-3], cgrad[3:-3])
+    print(f"[adjoint] N={n} {nsteps} steps: u(T) rel {fwd_rel:.3e}, gradient rel {rel:.3e}")
+    assert fwd_rel < RTOL
+    assert rel < RTOL
+
+
 def test_autograd_function(torch_mod) -> None:
     torch = torch_mod
     from pyshocks_amd.adjoint import checkpointed_solve, ssprk33_solve

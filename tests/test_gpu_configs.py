@@ -14,7 +14,9 @@
 * Adjoint: Burgers Rusanov N = 16384, FP64, 2000 fixed-dt steps,
   checkpoints every 50, J = 0.5 ||u(T) - u_target||^2 dx; <grad J, d>
   against central differences of the GPU forward solve (h = 1e-7) on 32
-  seeded unit directions, the 64 perturbed solves run as one batch.
+  seeded unit directions, the 64 perturbed solves run as one batch; the
+  gradient against the C reverse mode of the oracle (its own forward pass
+  in the reference's order) within C4_GRAD_RTOL.
 * Small plain method: N = 256 to t = 0.5 (the smoke workload) is covered by
   the golden fixtures of test_gpu_forward.py.
 
@@ -33,6 +35,11 @@ pytestmark = pytest.mark.gpu
 
 N_LARGE = 1 << 24
 STEPS_LARGE = 100
+#: relative l2 distance of the GPU and C adjoint gradients at the full size:
+#: the two forward passes round differently (u(T) ~3e-14 apart after 2000
+#: steps), which the reverse pass carries into the gradient (measured
+#: 2.5e-12); 40x margin
+C4_GRAD_RTOL = 1.0e-10
 
 
 @pytest.fixture(scope="module")
@@ -241,6 +248,22 @@ def test_adjoint_full_size_vs_central_fd(torch_mod) -> None:
     assert np.all(np.isfinite(grad))
     np.testing.assert_array_equal(grad[:3], 0.0)
     np.testing.assert_array_equal(grad[-3:], 0.0)
+    # the C reverse mode of the oracle on the same u0 and u_target (its own
+    # forward pass in the reference's expression order)
+    from oracle import build as oracle_build
+    from oracle import ref_c
+
+    if not ref_c.LIB.exists():
+        oracle_build.build(verbose=False)
+    cgrad, cu, _ = ref_c.adjoint_rusanov(u0.cpu().numpy(), target.cpu().numpy(), n=n, g=3,
+                                         dx=grid.h, eps=1e-6, dt=dt, nsteps=nsteps,
+                                         every=every)
+    fwd_rel = _rel(solve.u_final.cpu().numpy()[3:-3], cu[3:-3])
+    c_rel = _rel(grad[3:-3], cgrad[3:-3])
+    print(f"[configs] adjoint N={n}: GPU vs C oracle u(T) rel l2 {fwd_rel:.3e}, "
+          f"gradient rel l2 {c_rel:.3e}", flush=True)
+    assert fwd_rel < 1e-12
+    assert c_rel < C4_GRAD_RTOL
     # the 64 perturbed solves as one batch (one launch per step for all)
     h = 1.0e-7
     drng = np.random.default_rng(32)

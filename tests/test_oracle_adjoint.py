@@ -148,3 +148,99 @@ def _step_real(p, kind, grid, u, dt):
 
     out = W.ssprk33_advance(operator, dt, 0.0, ju.astype(np.complex128))
     return out.real
+
+
+# {{{ the C reverse mode (oracle/weno_ref.c), the CPU adjoint
+
+
+@pytest.fixture(scope="module")
+def ref_c():
+    from oracle import build as oracle_build
+    from oracle import ref_c as R
+
+    if not R.LIB.exists():
+        oracle_build.build(verbose=False)
+    return R
+
+
+def test_c_operator_vjp_is_jacobian_transpose(ref_c) -> None:
+    """J_L(u)^T mu of the C reverse mode against the transpose of the
+    complex-step operator Jacobian (Rusanov, WENO-JS5), on noisy data that
+    exercises every weight and both branches of the Rusanov speed."""
+    n = 40
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    u = W.apply_boundary_periodic(grid, _data(n, 3))
+    mu = np.zeros(grid.n)
+    mu[3:-3] = np.random.default_rng(4).standard_normal(n)
+    J = A.operator_jacobian("rusanov", grid, u, eps=1e-6)
+    got = ref_c.operator_vjp(u, mu, g=3, dx=grid.dx_min, eps=1e-6)
+    ref = J.T @ mu[3:-3]
+    np.testing.assert_allclose(got[3:-3], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    assert np.all(got[:3] == 0) and np.all(got[-3:] == 0)
+
+
+def test_c_step_vjp_is_jacobian_transpose(ref_c) -> None:
+    n = 32
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    u = W.apply_boundary_periodic(grid, _data(n, 5))
+    dt = 0.4 / n
+    lam = np.zeros(grid.n)
+    lam[3:-3] = np.random.default_rng(6).standard_normal(n)
+    J = A.step_jacobian("rusanov", grid, u, dt, eps=1e-6)
+    got = ref_c.step_vjp(u, lam, g=3, dx=grid.dx_min, eps=1e-6, dt=dt)
+    ref = J.T @ lam[3:-3]
+    np.testing.assert_allclose(got[3:-3], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def test_c_adjoint_is_chained_jacobians(ref_c) -> None:
+    """The checkpointed C adjoint (windows of 5 steps over 13, the last one
+    partial) against the product of the complex-step Jacobians of every step
+    along the oracle's trajectory; J and u(T) against the oracle's solve."""
+    n, nsteps, every = 24, 13, 5
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    u0 = W.apply_boundary_periodic(grid, _data(n, 7))
+    target = W.apply_boundary_periodic(grid, _data(n, 8))
+    dt = 0.3 / n
+    sch = W.Scheme("rusanov", eps=1e-6)
+    states = [u0]
+    for _ in range(nsteps):
+        x = W.apply_boundary_periodic(grid, states[-1])
+        states.append(W.ssprk33_advance(lambda t, v: W.apply_operator(sch, grid, t, v),
+                                        dt, 0.0, x))
+    uT = states[-1]
+    lam = (uT[3:-3] - target[3:-3]) * grid.dx_min
+    for k in range(nsteps - 1, -1, -1):
+        lam = A.step_jacobian("rusanov", grid, states[k], dt, eps=1e-6).T @ lam
+    grad, uf, Jv = ref_c.adjoint_rusanov(u0, target, n=n, g=3, dx=grid.dx_min, eps=1e-6, dt=dt,
+                                         nsteps=nsteps, every=every)
+    np.testing.assert_allclose(uf[3:-3], uT[3:-3], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(Jv, 0.5 * np.sum((uT[3:-3] - target[3:-3]) ** 2) * grid.dx_min,
+                               rtol=1e-12)
+    np.testing.assert_allclose(grad[3:-3], lam, rtol=0, atol=1e-12 * np.abs(lam).max())
+    assert np.all(grad[:3] == 0) and np.all(grad[-3:] == 0)
+
+
+def test_c_adjoint_dot_product_vs_central_fd(ref_c) -> None:
+    """<dJ/du0, d> against a central difference of J along a smooth direction
+    at a larger size, through the shock of u0 = 0.5 + sin(2 pi x)."""
+    n, nsteps = 256, 120
+    grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    x = grid.x
+    u0 = W.apply_boundary_periodic(grid, 0.5 + np.sin(2 * np.pi * x))
+    target = W.apply_boundary_periodic(grid, 0.5 + np.sin(2 * np.pi * x + 0.1))
+    dt = 0.5 * grid.dx_min / 1.5
+    d = np.zeros(grid.n)
+    d[3:-3] = np.cos(4 * np.pi * x[3:-3]) + 0.5 * np.sin(6 * np.pi * x[3:-3])
+    grad, _, _ = ref_c.adjoint_rusanov(u0, target, n=n, g=3, dx=grid.dx_min, eps=1e-6, dt=dt,
+                                       nsteps=nsteps, every=50)
+
+    def J(v: np.ndarray) -> float:
+        return ref_c.adjoint_rusanov(v, target, n=n, g=3, dx=grid.dx_min, eps=1e-6, dt=dt,
+                                     nsteps=nsteps, every=50)[2]
+
+    h = 1e-6
+    fd = (J(u0 + h * d) - J(u0 - h * d)) / (2 * h)
+    np.testing.assert_allclose(float(grad @ d), fd, rtol=1e-6)
+
+
+# }}}
