@@ -371,19 +371,28 @@ typedef struct psa_solver_desc {
  * PIPELINE: the stage kernels on a persistent grid that copies its next tile
  *   into LDS while computing the current one (WENO-JS weights, no velocity
  *   array);
- * STEP: one kernel per step (and a one-wave step-control kernel after it); a
- *   workgroup loads its tile of u^n with a 9-cell halo on each side, runs the
- *   three stages in LDS (recomputing the halo cells of the first two) and
- *   writes u^(n+1) (2 words per cell and step; WENO-JS or learned weights, no
- *   velocity array, no Rusanov nu, not Lax-Friedrichs (its stages need the
- *   global max|u| of the previous stage), a control block, tiles 256 x 4 or
- *   256 x 2). Steps ping-pong between u_out and the stage buffers.
- * DEFAULT is STEP where it applies and TILE otherwise. All schedules compute
- * the same values. */
+ * STEP: one kernel per step (a one-wave kernel before and after the call's
+ *   steps); every workgroup decides the step from the control block, loads
+ *   its tile of u^n with a 12-cell margin on each side, runs the three
+ *   stages in LDS (recomputing the margin cells of the first two) and writes
+ *   u^(n+1) (2 words per cell and step; any reconstruction, WENO-JS or
+ *   learned weights, no velocity array, no Rusanov nu, not Lax-Friedrichs
+ *   (its stages need the global max|u| of the previous stage), a control
+ *   block, tiles 256 x 4 or 256 x 2). Steps ping-pong between u_out and the
+ *   stage buffers.
+ * RING: one launch for all the steps of the call; one workgroup holds a
+ *   periodic state of n <= PSA_RING_MAX_CELLS cells in LDS (one cell per
+ *   thread) and takes the steps with the arithmetic and the decisions of
+ *   STEP, with no HBM traffic between steps (what STEP takes, periodic, no
+ *   learned weights; the stage buffers are not used).
+ * DEFAULT is RING where it applies, else STEP where it applies, else TILE.
+ * All schedules compute the same values. */
 #define PSA_SCHEDULE_DEFAULT 0
 #define PSA_SCHEDULE_TILE 1
 #define PSA_SCHEDULE_PIPELINE 2
 #define PSA_SCHEDULE_STEP 3
+#define PSA_SCHEDULE_RING 4
+#define PSA_RING_MAX_CELLS 1024
 
 /* The numerical flux of the descriptor's flux and reconstruction (any
  * PSA_REC_*, MUSCL's limiter from the descriptor) on u as given, fnum (m + 1)

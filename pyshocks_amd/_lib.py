@@ -246,6 +246,8 @@ SIGNATURES: dict[str, list[Any]] = {
 
 TEAM_STATE_BYTES = 64
 PSA_SCHEDULE_DEFAULT, PSA_SCHEDULE_TILE, PSA_SCHEDULE_PIPELINE, PSA_SCHEDULE_STEP = 0, 1, 2, 3
+PSA_SCHEDULE_RING = 4
+PSA_RING_MAX_CELLS = 1024
 MLP_NPARAM = 1347
 MLP_RECORD = 12
 

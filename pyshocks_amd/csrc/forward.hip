@@ -790,6 +790,162 @@ __global__ void k_copy(const T *__restrict__ src, T *__restrict__ dst,
   if (i < m) dst[i] = src[i];
 }
 
+// ---------------------------------------------------------------------------
+// ring schedule: a periodic state of n <= PSA_RING_MAX_CELLS cells held in
+// LDS by one workgroup (thread i owns cell i) for all the steps of a call
+
+constexpr int kRingMax = PSA_RING_MAX_CELLS;
+
+// cell i of a stage output into a ring buffer R (R[k] is cell k - 2 mod n)
+template <typename T>
+__device__ __forceinline__ void ring_put(T *R, int n, int i, T v) {
+  R[i + 2] = v;
+  if (i < 2) R[n + 2 + i] = v;
+  if (i >= n - 2) R[i - n + 2] = v;
+}
+
+// One stage of cell i from ring X into ring Y, with the arithmetic of
+// step_stage (one cell per thread): reconstruct_any on the same five-cell
+// stencil, the same face fluxes and stage combination, the neighbours' edge
+// states (and ES-WENO weights) through LDS with the periodic wrap. Ends with
+// Y written and visible to the workgroup.
+template <typename T, int FLUX, int REC, int STAGE>
+__device__ __forceinline__ T ring_stage(const StageArgs<T> &p, const T *X, T *Y, T lam,
+                                        T un, int n, bool live, T *sUR, T *sUL, T *sOM) {
+  constexpr bool ES = FLUX == PSA_FLUX_ESWENO;
+  const int i = threadIdx.x;
+  T w[5];
+  T ul[1], ur[1], om[1];
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) w[k] = X[i + k];  // cells i - 2 .. i + 2
+    const MlpSource src{nullptr, nullptr};
+    reconstruct_any<T, 1, false, REC>(p, w, ul, ur, src, nullptr, 0, ES ? om : nullptr);
+    sUR[i] = ur[0];
+    sUL[i] = ul[0];
+    if constexpr (ES) sOM[i] = om[0];
+  }
+  __syncthreads();
+  T val = T(0);
+  if (live) {
+    const int il = i == 0 ? n - 1 : i - 1;
+    const int ir = i == n - 1 ? 0 : i + 1;
+    T F0, F1;
+    if constexpr (ES) {
+      F0 = flux_burgers_upwind(sUR[il], ul[0]) +
+           es_dissipation_w(sOM[il], om[0], w[1], w[2], p.es_delta);
+      F1 = flux_burgers_upwind(ur[0], sUL[ir]) +
+           es_dissipation_w(om[0], sOM[ir], w[2], w[3], p.es_delta);
+    } else {
+      F0 = stage_flux<T, FLUX>(sUR[il], ul[0], w[1], w[2], p.vconst);
+      F1 = stage_flux<T, FLUX>(ur[0], sUL[ir], w[2], w[3], p.vconst);
+    }
+    val = stage_value<T, STAGE>(w[2], un, lam, F1 - F0);
+    ring_put(Y, n, i, val);
+  }
+  __syncthreads();
+  return val;
+}
+
+// max|u| bit pattern over the live cells (the NaN pattern if any is NaN), in
+// every thread; s_red holds one entry per wave
+template <typename T>
+__device__ __forceinline__ unsigned long long ring_max_bits(T v, bool live,
+                                                            unsigned long long *s_red) {
+  unsigned long long b = 0;
+  if (live) b = isnan(v) ? nan_bits<T>() : abs_bits(v);
+  b = wave_max_u64(b);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = b;
+  __syncthreads();
+  const int nw = static_cast<int>(blockDim.x >> 6);
+  b = s_red[0];
+  for (int k = 1; k < nw; ++k) b = s_red[k] > b ? s_red[k] : b;
+  return b;
+}
+
+// Up to nsteps steps of the control block's sequence (the decisions of the
+// whole-step launches, control_decide_at, from the state's own max|u|), one
+// workgroup of blockDim.x >= n threads, the state in LDS throughout; on exit
+// u_out holds the state (ghosts filled periodically after a step, or u_in
+// as given when no step was taken) and the control block what the
+// whole-step schedule leaves (t, m, status, active, dt, umax_bits and the
+// final state's max|u| in word 0 of umax_slot).
+template <typename T, int FLUX, int REC>
+__global__ __launch_bounds__(kRingMax) void k_ring(StageArgs<T> p, long long nsteps) {
+  constexpr bool ES = FLUX == PSA_FLUX_ESWENO;
+  __shared__ T ringA[kRingMax + 4];
+  __shared__ T ringB[kRingMax + 4];
+  __shared__ T sUR[kRingMax];
+  __shared__ T sUL[kRingMax];
+  __shared__ T sOM[ES ? kRingMax : 1];
+  __shared__ unsigned long long s_red[kRingMax / 64];
+  psa_control *ctrl = p.ctrl;
+  const int i = threadIdx.x;
+  const int n = static_cast<int>(p.n);
+  const int g = p.g;
+  const bool live = i < n;
+
+  T u = live ? p.uk[g + i] : T(0);
+  if (live) ring_put(ringA, n, i, u);
+  T *cur = ringA;
+  T *oth = ringB;
+
+  const bool from_state = ctrl->speed_from_state != 0;
+  unsigned long long umax = gather_umax(ctrl);
+  double t = ctrl->t;
+  long long mstep = ctrl->m;
+  int status = ctrl->status;
+  bool active = false;
+  bool taken = false;
+  T dt_last = T(0);
+  unsigned long long umax_used = umax;  // max|u| of the last decision
+  __syncthreads();
+  for (long long s = 0; s < nsteps; ++s) {
+    T dt = T(0);
+    int st;
+    umax_used = umax;
+    active = control_decide_at<T>(ctrl, t, mstep, status, umax, dt, st);
+    status = st;
+    if (!active) break;
+    const T lam = stage_lambda<T, FLUX>(p, dt);
+    ring_stage<T, FLUX, REC, 1>(p, cur, oth, lam, u, n, live, sUR, sUL, sOM);
+    ring_stage<T, FLUX, REC, 2>(p, oth, cur, lam, u, n, live, sUR, sUL, sOM);
+    u = ring_stage<T, FLUX, REC, 3>(p, cur, oth, lam, u, n, live, sUR, sUL, sOM);
+    T *tmp = cur;
+    cur = oth;
+    oth = tmp;
+    t = static_cast<double>(static_cast<T>(t) + dt);
+    mstep += 1;
+    dt_last = dt;
+    taken = true;
+    if (from_state) umax = ring_max_bits(u, live, s_red);
+  }
+
+  if (taken) {
+    if (live) {
+      p.out[g + i] = u;
+      if (i >= n - g) p.out[i - (n - g)] = u;  // left ghosts: cells n - g ..
+      if (i < g) p.out[n + g + i] = u;          // right ghosts: cells 0 .. g - 1
+    }
+  } else {
+    for (int j = i; j < n + 2 * g; j += static_cast<int>(blockDim.x)) p.out[j] = p.uk[j];
+  }
+  if (i < 64) {
+    if (from_state) ctrl->umax_slot[i][0] = i == 0 ? umax : 0ull;
+    if (i == 0) {
+      ctrl->t = t;
+      ctrl->m = mstep;
+      ctrl->status = status;
+      ctrl->active = active ? 1 : 0;
+      ctrl->umax_bits = umax_used;
+      if (taken) ctrl->dt = static_cast<double>(dt_last);
+      ctrl->seq_t[0] = t;
+      ctrl->seq_m[0] = mstep;
+      ctrl->seq_status[0] = status;
+    }
+  }
+}
+
 }  // namespace psa
 
 // ===========================================================================
@@ -1014,6 +1170,54 @@ int launch_step(const psa_solver_desc *d, int vel, const StageArgs<T> &a,
   return launch_step_tiles<T, PSA_FLUX_ADVECTION_UPWIND, 1, false>(a, cfg, stream);
 }
 
+// Whether the ring schedule runs the steps of this descriptor (with a
+// control block): periodic, at most PSA_RING_MAX_CELLS cells, what the
+// whole-step kernel takes without learned weights.
+inline bool ring_applies(const psa_solver_desc *d, int vel) {
+  if (d->schedule != PSA_SCHEDULE_DEFAULT && d->schedule != PSA_SCHEDULE_RING) return false;
+  if (d->bc != PSA_BC_PERIODIC || d->mlp != nullptr || d->nu != nullptr || vel == 2)
+    return false;
+  if (d->flux == PSA_FLUX_LAX_FRIEDRICHS) return false;
+  return d->n >= 8 && d->n <= PSA_RING_MAX_CELLS && d->g >= 2 && d->g <= d->n;
+}
+
+template <typename T, int REC>
+int launch_ring_rec(const psa_solver_desc *d, const StageArgs<T> &a, long long nsteps,
+                    hipStream_t stream) {
+  const dim3 block(static_cast<unsigned int>((a.n + 63) / 64 * 64));
+  if (d->flux == PSA_FLUX_RUSANOV) {
+    hipLaunchKernelGGL((k_ring<T, PSA_FLUX_RUSANOV, REC>), dim3(1), block, 0, stream, a,
+                       nsteps);
+  } else if (d->flux == PSA_FLUX_BURGERS_UPWIND) {
+    hipLaunchKernelGGL((k_ring<T, PSA_FLUX_BURGERS_UPWIND, REC>), dim3(1), block, 0, stream,
+                       a, nsteps);
+  } else if (d->flux == PSA_FLUX_ENGQUIST_OSHER) {
+    hipLaunchKernelGGL((k_ring<T, PSA_FLUX_ENGQUIST_OSHER, REC>), dim3(1), block, 0, stream,
+                       a, nsteps);
+  } else if (d->flux == PSA_FLUX_ADVECTION_UPWIND) {
+    hipLaunchKernelGGL((k_ring<T, PSA_FLUX_ADVECTION_UPWIND, REC>), dim3(1), block, 0,
+                       stream, a, nsteps);
+  } else if constexpr (REC == 1) {
+    if (d->flux != PSA_FLUX_ESWENO)
+      return set_error(PSA_ERR_NOT_IMPLEMENTED, "flux not on the ring schedule");
+    hipLaunchKernelGGL((k_ring<T, PSA_FLUX_ESWENO, REC>), dim3(1), block, 0, stream, a,
+                       nsteps);
+  } else {
+    return set_error(PSA_ERR_NOT_IMPLEMENTED, "flux not on the ring schedule");
+  }
+  return check_launch("ring steps");
+}
+
+// All nsteps steps of a call on the ring schedule (ring_applies).
+template <typename T>
+int launch_ring(const psa_solver_desc *d, const StageArgs<T> &a, long long nsteps,
+                hipStream_t stream) {
+  if (a.rec3) return launch_ring_rec<T, 1>(d, a, nsteps, stream);
+  if (a.muscl) return launch_ring_rec<T, 2>(d, a, nsteps, stream);
+  if (a.muscls) return launch_ring_rec<T, 3>(d, a, nsteps, stream);
+  return launch_ring_rec<T, 0>(d, a, nsteps, stream);
+}
+
 // Learned weights: Burgers fluxes and two tile shapes.
 template <typename T, int FLUX, int STAGE>
 int launch_stage_learned(const StageArgs<T> &a, TileConfig cfg,
@@ -1152,6 +1356,9 @@ int check_desc(const psa_solver_desc *d) {
   if (d->flux < 0 || d->flux > PSA_FLUX_ESWENO)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "unsupported flux " + std::to_string(d->flux));
+  if (d->schedule < PSA_SCHEDULE_DEFAULT || d->schedule > PSA_SCHEDULE_RING)
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "unknown schedule " + std::to_string(d->schedule));
   if (d->flux == PSA_FLUX_ESWENO &&
       (d->rec != PSA_REC_ESWENO32 || !(d->eps > 0.0) || !(d->es_delta >= 0.0)))
     return set_error(PSA_ERR_INVALID_ARGUMENT,
@@ -1237,6 +1444,12 @@ int run_steps(const psa_solver_desc *d, const void *u_in, void *u_out,
   T *b1 = static_cast<T *>(k1);
   T *b2 = static_cast<T *>(k2);
   int st = PSA_OK;
+  if (a.ctrl != nullptr && ring_applies(d, vel)) {
+    if (nsteps <= 0) return PSA_OK;
+    a.uk = un;
+    a.out = uo;
+    return launch_ring<T>(d, a, nsteps, stream);
+  }
   if (a.ctrl != nullptr && step_applies(d, cfg, vel)) {
     // one launch per step; u^n and u^(n+1) are distinct arrays: the state
     // goes u_in -> k1 -> k2 -> k1 ... -> u_out (u_in -> k1 -> u_out, by a
@@ -1926,6 +2139,11 @@ int psa_ssprk33_steps(const psa_solver_desc *desc, const void *u_in,
                      "Neumann and Dirichlet fills run in the whole-step kernel only "
                      "(schedule default or step, no velocity array or Rusanov nu, not "
                      "Lax-Friedrichs, tiles 256 x 4 or 256 x 2)");
+  if (desc->schedule == PSA_SCHEDULE_RING && !ring_applies(desc, vel_mode(desc)))
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the ring schedule takes periodic states of 8 to "
+                     "PSA_RING_MAX_CELLS cells, no learned weights, velocity array, "
+                     "Rusanov nu or Lax-Friedrichs");
   if (nsteps < 0) return set_error(PSA_ERR_INVALID_ARGUMENT, "nsteps < 0");
   if (nsteps == 0) return PSA_OK;
   auto s = static_cast<hipStream_t>(stream);

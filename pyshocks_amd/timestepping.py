@@ -201,7 +201,12 @@ class FusedPlan:
 
     def uses_step_kernel(self) -> bool:
         """Whether :meth:`steps` runs one whole-step kernel per step."""
-        return _uses_step_kernel(self.desc)
+        return not _uses_ring_kernel(self.desc) and _uses_step_kernel(self.desc)
+
+    def uses_ring_kernel(self) -> bool:
+        """Whether :meth:`steps` runs all its steps in one launch of one
+        workgroup that holds the state in LDS."""
+        return _uses_ring_kernel(self.desc)
 
     @staticmethod
     def read_control(ctrl: torch.Tensor) -> _lib.PsaControl:
@@ -223,6 +228,21 @@ def _uses_step_kernel(d: _lib.PsaSolverDesc) -> bool:
     block = d.block if d.block > 0 else 256
     cpt = d.cells_per_thread if d.cells_per_thread > 0 else (2 if d.n < (1 << 20) else 4)
     return block == 256 and cpt in (2, 4)
+
+
+def _uses_ring_kernel(d: _lib.PsaSolverDesc) -> bool:
+    """The library's rule for the ring schedule: schedule default or ring,
+    periodic, 8 to ``PSA_RING_MAX_CELLS`` cells, no learned weights, a
+    constant or no velocity, no Rusanov ``nu``, not Lax-Friedrichs."""
+    if d.schedule not in (_lib.PSA_SCHEDULE_DEFAULT, _lib.PSA_SCHEDULE_RING):
+        return False
+    if d.bc != _lib.PSA_BC_PERIODIC or d.mlp or d.nu:
+        return False
+    if d.flux == _lib.PSA_FLUX_ADVECTION_UPWIND and not d.velocity_is_const:
+        return False
+    if d.flux == _lib.PSA_FLUX_LAX_FRIEDRICHS:
+        return False
+    return 8 <= d.n <= _lib.PSA_RING_MAX_CELLS and 2 <= d.g <= d.n
 
 
 def _scheme_plan(source: Any, *, fills: bool = False

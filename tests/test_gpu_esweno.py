@@ -108,7 +108,7 @@ def test_solve_through_shock(torch_mod, dtype_name: str, tfinal: float) -> None:
     osch = W.Scheme("esweno32", eps=sch.rec.eps, rec="esweno32", delta=sch.rec.delta)
     uref, dts, tref = W.solve(osch, ogrid, u0, theta=1.0, tfinal=tfinal)
     st = make_ssprk33(sch, grid, bc, theta=1.0)
-    assert fused_plan(st).uses_step_kernel()
+    assert fused_plan(st).uses_ring_kernel()
     from pyshocks_amd import apply_operator, predict_timestep
     from pyshocks_amd.timestepping import SSPRK33
 

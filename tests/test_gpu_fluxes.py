@@ -140,7 +140,7 @@ def test_fused_c1_through_shock(torch_mod, kind: str, alpha: float) -> None:
     st = _stepper(kind, grid, alpha)
     plan = fused_plan(st)
     assert plan is not None
-    assert plan.uses_step_kernel() == (kind == "eo")
+    assert plan.uses_ring_kernel() == (kind == "eo")
     for sched in (_lib.PSA_SCHEDULE_DEFAULT, _lib.PSA_SCHEDULE_TILE,
                   _lib.PSA_SCHEDULE_PIPELINE):
         plan = fused_plan(st)

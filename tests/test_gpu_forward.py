@@ -513,6 +513,136 @@ def test_step_schedule_bit_for_bit(torch_mod, kind: str, dtype_name: str, tiles,
     assert torch.equal(a[1], b[1])
 
 
+def _schedule_run(torch, plan, u0, sched: int):
+    """The call sequence of test_step_schedule_bit_for_bit on one schedule:
+    in place and out of place, one and several steps, then a run whose
+    control block stops inside the enqueued steps; the states, (t, dt, m)
+    of both control blocks and their max|u| fields."""
+    plan.desc.schedule = sched
+    big = float(np.finfo(plan.np_dtype).max)
+    u = u0.clone()
+    v = torch.empty_like(u)
+    k1, k2 = torch.empty_like(u), torch.empty_like(u)
+    ctrl = plan.new_control(u, 0.0, big, None)
+    plan.steps(u, u, k1, k2, ctrl, 3)
+    plan.steps(u, v, k1, k2, ctrl, 1)
+    plan.steps(v, v, k1, k2, ctrl, 1)
+    plan.steps(v, u, k1, k2, ctrl, 2)
+    c = plan.read_control(ctrl)
+    assert c.m == 7 and c.status == 0 and c.active == 1
+    w = torch.empty_like(u)
+    ctrl2 = plan.new_control(u, 0.0, 1.5 * c.dt, None)
+    plan.steps(u, w, k1, k2, ctrl2, 8)
+    c2 = plan.read_control(ctrl2)
+    assert 1 <= c2.m < 8 and c2.active == 0 and c2.status == 0
+    # a call whose first step is not taken leaves the state as given
+    x = torch.full_like(u, 7.0)
+    plan.steps(w, x, k1, k2, ctrl2, 2)
+    c3 = plan.read_control(ctrl2)
+    assert c3.m == c2.m and c3.active == 0
+    assert torch.equal(x, w)
+    fields = [(cc.t, cc.dt, cc.m, cc.umax_bits, cc.umax_slot[0][0]) for cc in (c, c2, c3)]
+    return u, w, fields
+
+
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "eo", "advection"])
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+@pytest.mark.parametrize("n", [8, 37, 256, 1000, 1024])
+def test_ring_schedule_bit_for_bit(torch_mod, kind: str, dtype_name: str, n: int) -> None:
+    """The ring schedule (one workgroup holding the periodic state in LDS for
+    every step of a call) computes exactly what the whole-step kernel
+    computes: states with their ghost images, t, dt, m and the max|u|
+    fields of the control block, over the call sequence of _schedule_run,
+    from 8 cells (a stencil that wraps within a thread's neighbours) to
+    PSA_RING_MAX_CELLS; the default schedule takes the ring."""
+    torch = torch_mod
+    from oracle import exact as E
+    from pyshocks_amd import _lib, make_uniform_cell_grid
+    from pyshocks_amd.timestepping import fused_plan
+
+    dtype = getattr(torch, dtype_name)
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, dtype=dtype)
+    vel = torch.ones(grid.n, dtype=dtype, device="cuda") if kind == "advection" else None
+    st = _stepper(torch, kind, grid, vel=vel, theta=0.5 if kind == "advection" else 1.0)
+    u0 = torch.tensor(E.exact_cell_averages(E.large_initial_condition(seed=6), n),
+                      device="cuda").to(dtype)
+    plan = fused_plan(st)
+    assert plan.uses_ring_kernel()
+    a = _schedule_run(torch, plan, u0, _lib.PSA_SCHEDULE_STEP)
+    b = _schedule_run(torch, plan, u0, _lib.PSA_SCHEDULE_RING)
+    d = _schedule_run(torch, plan, u0, _lib.PSA_SCHEDULE_DEFAULT)
+    assert a[2] == b[2] == d[2]
+    for k in (0, 1):
+        assert torch.equal(a[k], b[k]) and torch.equal(b[k], d[k])
+
+
+@pytest.mark.parametrize("rec", ["wenojs32", "esweno32", "muscl", "muscls", "esweno-flux"])
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+def test_ring_schedule_reconstructions(torch_mod, rec: str, dtype_name: str) -> None:
+    """The ring schedule against the whole-step kernel for the third-order,
+    MUSCL and slope-limited MUSCL reconstructions (Rusanov) and the ES-WENO
+    scheme, bit for bit."""
+    torch = torch_mod
+    from oracle import exact as E
+    from pyshocks_amd import _lib, bind, burgers, make_uniform_cell_grid
+    from pyshocks_amd.limiters import make_limiter_from_name
+    from pyshocks_amd.reconstruction import ESWENO32, make_reconstruction_from_name
+    from pyshocks_amd.scalar import PeriodicBoundary
+    from pyshocks_amd.timestepping import fused_plan, make_ssprk33
+
+    dtype = getattr(torch, dtype_name)
+    n = 300
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=2, dtype=dtype)
+    bc = PeriodicBoundary()
+    if rec == "esweno-flux":
+        sch = bind(burgers.ESWENO32(rec=ESWENO32()), grid, bc)
+    elif rec in ("muscl", "muscls"):
+        lm = make_limiter_from_name("minmod", theta=1.0, variant=1)
+        sch = burgers.Rusanov(rec=make_reconstruction_from_name(rec, lm=lm))
+    else:
+        sch = burgers.Rusanov(rec=make_reconstruction_from_name(rec, eps=1e-6))
+    st = make_ssprk33(sch, grid, bc, theta=1.0)
+    u0 = torch.tensor(E.exact_cell_averages(E.large_initial_condition(seed=8), n, nghosts=2),
+                      device="cuda").to(dtype)
+    plan = fused_plan(st)
+    assert plan.uses_ring_kernel()
+    a = _schedule_run(torch, plan, u0, _lib.PSA_SCHEDULE_STEP)
+    b = _schedule_run(torch, plan, u0, _lib.PSA_SCHEDULE_RING)
+    assert a[2] == b[2]
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+
+
+def test_ring_schedule_mixes_with_the_others(torch_mod) -> None:
+    """Calls on the ring, whole-step and stage schedules alternate on one
+    control block through the end of the run and give what the stage
+    schedule gives alone."""
+    torch = torch_mod
+    from oracle import exact as E
+    from pyshocks_amd import _lib, make_uniform_cell_grid
+    from pyshocks_amd.timestepping import fused_plan
+
+    n = 512
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    st = _stepper(torch, "rusanov", grid)
+    u0 = torch.tensor(E.exact_cell_averages(E.large_initial_condition(seed=4), n),
+                      device="cuda")
+    plan = fused_plan(st)
+    R, S, T = _lib.PSA_SCHEDULE_RING, _lib.PSA_SCHEDULE_STEP, _lib.PSA_SCHEDULE_TILE
+    out = {}
+    for name, seq in {"tile": [T] * 5, "mixed": [R, S, T, R, R]}.items():
+        u = u0.clone()
+        k1, k2 = torch.empty_like(u), torch.empty_like(u)
+        ctrl = plan.new_control(u, 0.0, 17.5 * 0.5 / n / float(u0[3:-3].abs().max()), None)
+        for sched in seq:
+            plan.desc.schedule = sched
+            plan.steps(u, u, k1, k2, ctrl, 4)
+        c = plan.read_control(ctrl)
+        assert c.status == 0 and c.active == 0
+        out[name] = (u, c.t, c.m, c.dt)
+    assert out["tile"][1:] == out["mixed"][1:]
+    assert torch.equal(out["tile"][0], out["mixed"][0])
+
+
 def test_schedules_mix_on_one_control_block(torch_mod) -> None:
     """Calls of the whole-step and stage schedules alternate on one control
     block (each leaves t, m, dt and the max|u| slots as the other expects),

@@ -139,7 +139,7 @@ def test_fused_burgers_through_shock(torch_mod, name: str, param: float, kind: s
     grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=g)
     st = make_ssprk33(_scheme(kind, _rec(name, param)), grid, PeriodicBoundary(),
                       theta=1.0)
-    assert fused_plan(st).uses_step_kernel() == (kind != "lf")
+    assert fused_plan(st).uses_ring_kernel() == (kind != "lf")
     for sched in (_lib.PSA_SCHEDULE_DEFAULT, _lib.PSA_SCHEDULE_TILE,
                   _lib.PSA_SCHEDULE_PIPELINE):
         plan = fused_plan(st)

@@ -129,7 +129,7 @@ def test_fused_burgers_through_shock(torch_mod, rec: str, kind: str) -> None:
                               tfinal=0.5)
     grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=g)
     st = make_ssprk33(_scheme(kind, rec), grid, PeriodicBoundary(), theta=1.0)
-    assert fused_plan(st).uses_step_kernel() == (kind != "lf")
+    assert fused_plan(st).uses_ring_kernel() == (kind != "lf")
     for sched in (_lib.PSA_SCHEDULE_DEFAULT, _lib.PSA_SCHEDULE_TILE,
                   _lib.PSA_SCHEDULE_PIPELINE):
         plan = fused_plan(st)
