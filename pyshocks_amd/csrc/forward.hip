@@ -1016,9 +1016,10 @@ inline bool aligned16(const void *p) {
 }
 
 // Workgroups the device holds at once for a kernel (CUs x occupancy),
-// queried once per kernel instantiation and device.
-template <typename K>
-unsigned int resident_blocks(K kernel, int block) {
+// queried once per kernel and device (the kernel is the template argument,
+// so every kernel has a cache of its own).
+template <auto Kernel>
+unsigned int resident_blocks(int block) {
   constexpr int kMaxDevices = 64;
   static unsigned int cache[kMaxDevices] = {};
   int dev = 0;
@@ -1026,7 +1027,7 @@ unsigned int resident_blocks(K kernel, int block) {
   if (cache[dev] == 0) {
     int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(kernel), block, 0) !=
+            &per_cu, reinterpret_cast<const void *>(Kernel), block, 0) !=
             hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                               dev) != hipSuccess ||
@@ -1059,7 +1060,7 @@ int launch_stage_pipe_cfg(const StageArgs<T> &args, hipStream_t stream) {
   StageArgs<T> a = args;
   a.aligned = aligned16(a.uk) && aligned16(a.out) &&
               (a.un == nullptr || aligned16(a.un));
-  const unsigned int cap = resident_blocks(k_stage_pipe<T, FLUX, VEL, STAGE, BLOCK, CPT>, BLOCK);
+  const unsigned int cap = resident_blocks<k_stage_pipe<T, FLUX, VEL, STAGE, BLOCK, CPT>>(BLOCK);
   if (cap == 0) return launch_stage_cfg<T, FLUX, VEL, STAGE, BLOCK, CPT>(args, stream);
   unsigned int blocks = blocks_for(a.m, TILE);
   if (blocks > cap) blocks = cap;
