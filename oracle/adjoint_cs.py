@@ -165,8 +165,8 @@ def learned_reconstruct_cs(p: np.ndarray, tau: float, sigma: float, u: np.ndarra
 
 def learned_operator_cs(p: np.ndarray, tau: float, sigma: float, kind: str,
                         grid: W.Grid, x: np.ndarray, *, eps: float) -> np.ndarray:
-    """The Burgers operator (rusanov or godunov) of the learned surrogate on
-    complex input (``apply_operator_cs`` with ``learned_reconstruct_cs``)."""
+    """The Burgers operator (rusanov, godunov or eo) of the learned surrogate
+    on complex input (``apply_operator_cs`` with ``learned_reconstruct_cs``)."""
     x = W.apply_boundary_periodic(grid, x)
     ul, ur = learned_reconstruct_cs(p, tau, sigma, x, eps)
     fl, fr = ul**2 / 2, ur**2 / 2
@@ -177,6 +177,11 @@ def learned_operator_cs(p: np.ndarray, tau: float, sigma: float, kind: str,
     elif kind == "godunov":
         aavg = (ur[:-1] + ul[1:]) / 2
         fnum = np.where(aavg.real > 0, fr[:-1], fl[1:])
+    elif kind == "eo":
+        # f(max(ur, 0)) + f(min(ul, 0)), the switches on the real parts
+        pp = np.where(ur[:-1].real > 0, ur[:-1], 0)
+        qq = np.where(ul[1:].real < 0, ul[1:], 0)
+        fnum = pp**2 / 2 + qq**2 / 2
     else:
         raise NotImplementedError(kind)
     F = np.pad(fnum, 1)

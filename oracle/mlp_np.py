@@ -228,17 +228,25 @@ def learned_apply_operator(p: np.ndarray, tau: float, kind: str, grid: weno_np.G
                            u: np.ndarray, eps: float, *, sigma: float = 0.0,
                            emulate: bool = True) -> np.ndarray:
     """The Burgers operator (``schemes.py:466-473``) with the learned
-    reconstruction in place of WENO-JS: Rusanov (alpha = 1) or Godunov."""
+    reconstruction in place of WENO-JS: Rusanov (alpha = 1), Godunov,
+    Lax-Friedrichs (the global speed max|u| of the ghost-filled state,
+    ``scalar.py:258-278``) or Engquist-Osher (omega = 0,
+    ``scalar.py:287-322``)."""
     u = weno_np.apply_boundary_periodic(grid, np.asarray(u, dtype=np.float64))
     ul, ur = learned_reconstruct(p, tau, u, eps, sigma=sigma, emulate=emulate)
     fl, fr = ul**2 / 2, ur**2 / 2
-    if kind == "rusanov":
-        a = np.abs(u)
-        a = np.maximum(a[1:], a[:-1])
+    if kind in ("rusanov", "lf"):
+        if kind == "rusanov":
+            a = np.abs(u)
+            a = np.maximum(a[1:], a[:-1])
+        else:
+            a = np.max(np.abs(u))
         fnum = 0.5 * (fl[1:] + fr[:-1]) - 0.5 * a * (ul[1:] - ur[:-1])
     elif kind == "godunov":
         aavg = (ur[:-1] + ul[1:]) / 2
         fnum = np.where(aavg > 0, fr[:-1], fl[1:])
+    elif kind == "eo":
+        fnum = weno_np.engquist_osher_faces(ur[:-1], ul[1:])
     else:
         raise NotImplementedError(kind)
     f = np.pad(fnum, 1)

@@ -16,8 +16,8 @@ library's reverse stage kernels:
 Cotangents use the grid layout (ghost entries are ignored on input and are
 zero on output).
 
-Rusanov and Godunov schemes with :class:`~pyshocks_amd.reconstruction.LearnedWENOJS53`
-are supported too (the learned weights have no counterpart in the reference).
+Rusanov, Godunov and Engquist-Osher schemes with
+:class:`~pyshocks_amd.reconstruction.LearnedWENOJS53` are supported too (the learned weights have no counterpart in the reference).
 Their transposed Jacobian is that of the forward solver's smooth surrogate,
 with each cell's regime fixed at the forward state:
 WENO-JS in shock cells, the ideal weights in smooth cells, and the network in

@@ -161,9 +161,11 @@ def _apply_operator_burgers(scheme: FiniteVolumeScheme, grid: Grid, bc: Boundary
     speed = max_abs_interior(u, grid.nghosts) if isinstance(scheme, LaxFriedrichs) \
         else None
     if isinstance(scheme.rec, LearnedWENOJS53) and flux_kind(scheme) not in (
-            _lib.PSA_FLUX_RUSANOV, _lib.PSA_FLUX_BURGERS_UPWIND):
+            _lib.PSA_FLUX_RUSANOV, _lib.PSA_FLUX_BURGERS_UPWIND,
+            _lib.PSA_FLUX_LAX_FRIEDRICHS, _lib.PSA_FLUX_ENGQUIST_OSHER):
         raise NotImplementedError(
-            "learned weights are implemented for the Rusanov and Godunov fluxes")
+            "learned weights are implemented for the Rusanov, Godunov, "
+            "Lax-Friedrichs and Engquist-Osher fluxes")
     if isinstance(scheme.rec, (LearnedWENOJS53, MUSCL)):
         # the operator stage of the descriptor carries the learned weights or
         # the limiter

@@ -665,6 +665,8 @@ int launch_vjp(const psa_solver_desc *d, int rstage, const void *state,
   if (d->mlp != nullptr) {
     if (d->flux == PSA_FLUX_RUSANOV)
       return launch_vjp_flux<T, PSA_FLUX_RUSANOV, true>(rstage, a, s);
+    if (d->flux == PSA_FLUX_ENGQUIST_OSHER)
+      return launch_vjp_flux<T, PSA_FLUX_ENGQUIST_OSHER, true>(rstage, a, s);
     return launch_vjp_flux<T, PSA_FLUX_BURGERS_UPWIND, true>(rstage, a, s);
   }
   if (d->flux == PSA_FLUX_RUSANOV)
@@ -705,6 +707,9 @@ int launch_step_vjp(const psa_solver_desc *d, const void *x, const void *k1,
     if (d->flux == PSA_FLUX_RUSANOV) {
       hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_RUSANOV, true>), grid, dim3(kVjpBlock), 0,
                          s, a);
+    } else if (d->flux == PSA_FLUX_ENGQUIST_OSHER) {
+      hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ENGQUIST_OSHER, true>), grid,
+                         dim3(kVjpBlock), 0, s, a);
     } else {
       hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_BURGERS_UPWIND, true>), grid,
                          dim3(kVjpBlock), 0, s, a);
@@ -748,10 +753,9 @@ int check_vjp_desc(const psa_solver_desc *d) {
   if (d->flux == PSA_FLUX_ADVECTION_UPWIND && !d->velocity_is_const)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "the adjoint supports advection with a constant velocity");
-  if (d->mlp != nullptr && d->flux != PSA_FLUX_RUSANOV &&
-      d->flux != PSA_FLUX_BURGERS_UPWIND)
+  if (d->mlp != nullptr && d->flux == PSA_FLUX_ADVECTION_UPWIND)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
-                     "the adjoint of learned weights takes the Rusanov and Godunov fluxes");
+                     "the adjoint of learned weights takes the Burgers fluxes");
   if (d->mlp != nullptr && (!(d->mlp_tau >= 0.0) || !(d->mlp_sigma >= 0.0)))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "mlp_tau and mlp_sigma must be >= 0");
   if (!(d->dx > 0.0))

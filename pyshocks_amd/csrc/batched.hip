@@ -471,22 +471,16 @@ int launch_batch_tiles(const BatchArgs<T> &a, int block, int cpt, int grid,
   if (a.rec == PSA_REC_MUSCLS)
     return launch_batch_rec<T, FLUX, WHOLE, 3>(a, block, cpt, grid, max_team, nteams, s);
   if (a.mlp != nullptr) {
-    if constexpr (FLUX == PSA_FLUX_ENGQUIST_OSHER) {
-      return set_error(PSA_ERR_NOT_IMPLEMENTED,
-                       "learned weights are implemented for the Burgers "
-                       "Rusanov and Godunov fluxes only");
-    } else {
-      if (block == 256 && cpt == 4)
-        return launch_batch_cfg<T, FLUX, 256, 4, true, WHOLE>(a, grid, max_team, nteams, s);
-      if (block == 512 && cpt == 2)
-        return launch_batch_cfg<T, FLUX, 512, 2, true, WHOLE>(a, grid, max_team, nteams, s);
-      if (block == 1024 && cpt == 2)
-        return launch_batch_cfg<T, FLUX, 1024, 2, true, WHOLE>(a, grid, max_team, nteams, s);
-      return set_error(PSA_ERR_INVALID_ARGUMENT,
-                       "unsupported batch tile configuration for learned "
-                       "weights block=" + std::to_string(block) +
-                           " cells_per_thread=" + std::to_string(cpt));
-    }
+    if (block == 256 && cpt == 4)
+      return launch_batch_cfg<T, FLUX, 256, 4, true, WHOLE>(a, grid, max_team, nteams, s);
+    if (block == 512 && cpt == 2)
+      return launch_batch_cfg<T, FLUX, 512, 2, true, WHOLE>(a, grid, max_team, nteams, s);
+    if (block == 1024 && cpt == 2)
+      return launch_batch_cfg<T, FLUX, 1024, 2, true, WHOLE>(a, grid, max_team, nteams, s);
+    return set_error(PSA_ERR_INVALID_ARGUMENT,
+                     "unsupported batch tile configuration for learned "
+                     "weights block=" + std::to_string(block) +
+                         " cells_per_thread=" + std::to_string(cpt));
   }
   if (block == 256 && cpt == 4)
     return launch_batch_cfg<T, FLUX, 256, 4, false, WHOLE>(a, grid, max_team, nteams, s);

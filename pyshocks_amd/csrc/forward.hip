@@ -1159,6 +1159,8 @@ int launch_step(const psa_solver_desc *d, int vel, const StageArgs<T> &a,
   if (d->mlp != nullptr) {
     if (d->flux == PSA_FLUX_RUSANOV)
       return launch_step_tiles<T, PSA_FLUX_RUSANOV, 0, true>(a, cfg, stream);
+    if (d->flux == PSA_FLUX_ENGQUIST_OSHER)
+      return launch_step_tiles<T, PSA_FLUX_ENGQUIST_OSHER, 0, true>(a, cfg, stream);
     return launch_step_tiles<T, PSA_FLUX_BURGERS_UPWIND, 0, true>(a, cfg, stream);
   }
   if (d->flux == PSA_FLUX_RUSANOV)
@@ -1286,6 +1288,10 @@ int launch_stage(int flux, int vel, const StageArgs<T> &a, TileConfig cfg,
     if (flux == PSA_FLUX_BURGERS_UPWIND)
       return launch_stage_learned<T, PSA_FLUX_BURGERS_UPWIND, STAGE>(a, cfg,
                                                                     stream);
+    if (flux == PSA_FLUX_ENGQUIST_OSHER)
+      return launch_stage_learned<T, PSA_FLUX_ENGQUIST_OSHER, STAGE>(a, cfg, stream);
+    if (flux == PSA_FLUX_LAX_FRIEDRICHS)
+      return launch_stage_learned<T, PSA_FLUX_LAX_FRIEDRICHS, STAGE>(a, cfg, stream);
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "learned weights are implemented for Burgers only");
   }
@@ -1365,11 +1371,11 @@ int check_desc(const psa_solver_desc *d) {
     return set_error(PSA_ERR_INVALID_ARGUMENT,
                      "the ES-WENO flux needs the ES-WENO3 reconstruction, eps > 0 "
                      "and es_delta >= 0");
-  if (d->mlp != nullptr && d->flux != PSA_FLUX_RUSANOV &&
-      d->flux != PSA_FLUX_BURGERS_UPWIND)
+  if (d->mlp != nullptr &&
+      (d->flux == PSA_FLUX_ADVECTION_UPWIND || d->flux == PSA_FLUX_ESWENO))
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
-                     "learned weights are implemented for the Burgers Rusanov "
-                     "and Godunov fluxes only");
+                     "learned weights are implemented for the Burgers Rusanov, "
+                     "Godunov, Lax-Friedrichs and Engquist-Osher fluxes");
   if (d->mlp != nullptr && (!(d->mlp_tau >= 0.0) || !(d->mlp_sigma >= 0.0)))
     return set_error(PSA_ERR_INVALID_ARGUMENT, "mlp_tau and mlp_sigma must be >= 0");
   return PSA_OK;

@@ -69,18 +69,27 @@ def _learned(scheme: ConservationLawScheme) -> Any:
     return scheme.rec if isinstance(scheme.rec, LearnedWENOJS53) else None
 
 
-def _learned_flux(rec: Any, kind: int, u: torch.Tensor,
-                  nu: torch.Tensor | None) -> torch.Tensor:
-    """Face fluxes of the learned reconstruction (``scalar.py:125-130`` and
-    ``:238-248`` on the learned states)."""
+def _learned_flux(rec: Any, kind: int, u: torch.Tensor, nu: torch.Tensor | None,
+                  speed: torch.Tensor | None = None) -> torch.Tensor:
+    """Face fluxes of the learned reconstruction (``scalar.py:125-130``,
+    ``:238-248``, ``:276-277`` and ``:309-319`` on the learned states);
+    ``speed`` is the Lax-Friedrichs device scalar max|a|."""
     from pyshocks_amd.learned import learned_reconstruct
 
     ul, ur = learned_reconstruct(rec.weights, u, rec.eps)
     fl, fr = ul * ul / 2, ur * ur / 2
-    if kind == _lib.PSA_FLUX_RUSANOV:
-        a = torch.maximum(u[1:].abs(), u[:-1].abs())
+    if kind in (_lib.PSA_FLUX_RUSANOV, _lib.PSA_FLUX_LAX_FRIEDRICHS):
+        if kind == _lib.PSA_FLUX_RUSANOV:
+            a = torch.maximum(u[1:].abs(), u[:-1].abs())
+        else:
+            assert speed is not None
+            a = speed
         nu_ = 1.0 if nu is None else nu
         f = 0.5 * (fl[1:] + fr[:-1]) - 0.5 * a * nu_ * (ul[1:] - ur[:-1])
+    elif kind == _lib.PSA_FLUX_ENGQUIST_OSHER:
+        p = torch.clamp(ur[:-1], min=0.0)
+        q = torch.clamp(ul[1:], max=0.0)
+        f = p * p / 2 + q * q / 2
     else:
         f = torch.where((ur[:-1] + ul[1:]) / 2 > 0, fr[:-1], fl[1:])
     return torch.nn.functional.pad(f, (1, 1))
@@ -128,13 +137,14 @@ def scalar_flux_lax_friedrichs(scheme: ConservationLawScheme, grid: Grid,
 
     assert u.shape[0] == grid.x.numel()
     _check_burgers(scheme)
-    if _learned(scheme) is not None:
-        raise NotImplementedError(
-            "learned weights are implemented for the Rusanov and Godunov fluxes")
     _lib.require_device(a)
     if a.shape != u.shape or a.dtype != u.dtype:
         raise ValueError("the speed array must have the shape and dtype of u")
     amax = max_abs_interior(a.contiguous(), 0)
+    rec = _learned(scheme)
+    if rec is not None:
+        return _learned_flux(rec, _lib.PSA_FLUX_LAX_FRIEDRICHS, u, rusanov_nu(grid, alpha),
+                             amax)
     return _numerical_flux(_lib.PSA_FLUX_LAX_FRIEDRICHS, u, amax,
                            rusanov_nu(grid, alpha), _eps_of(scheme), _rec_of(scheme),
                            scheme=scheme, grid=grid)
@@ -151,9 +161,9 @@ def scalar_flux_engquist_osher(scheme: ConservationLawScheme, grid: Grid,
     _check_burgers(scheme)
     if omega != 0.0:
         raise NotImplementedError("the GPU Engquist-Osher flux takes omega = 0")
-    if _learned(scheme) is not None:
-        raise NotImplementedError(
-            "learned weights are implemented for the Rusanov and Godunov fluxes")
+    rec = _learned(scheme)
+    if rec is not None:
+        return _learned_flux(rec, _lib.PSA_FLUX_ENGQUIST_OSHER, u, None)
     return _numerical_flux(_lib.PSA_FLUX_ENGQUIST_OSHER, u, None, None,
                            _eps_of(scheme), _rec_of(scheme), scheme=scheme, grid=grid)
 

@@ -327,8 +327,8 @@ def _scheme_desc(scheme: Any, grid: Any) -> tuple[_lib.PsaSolverDesc, tuple[Any,
         desc.muscl_atol = float(scheme.rec.atol)
 
     if isinstance(scheme.rec, LearnedWENOJS53):
-        if not isinstance(scheme, (burgers.Rusanov, burgers.Godunov)) or \
-                isinstance(scheme, burgers.LaxFriedrichs):
+        if not isinstance(scheme, (burgers.Rusanov, burgers.Godunov,
+                                   burgers.EngquistOsher)):
             return None
         desc.mlp = scheme.rec.weights.ptr
         desc.mlp_tau = float(scheme.rec.weights.tau)

@@ -388,8 +388,9 @@ def _learned_source(torch, kind: str, n: int, p, tau: float, sigma: float):
 
     grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
     rec = LearnedWENOJS53(eps=1e-6, weights=MLPWeights.from_numpy(p, tau, sigma))
-    sch = (burgers.Rusanov if kind == "rusanov" else burgers.Godunov)(rec=rec)
-    return BoundOperator(sch, grid, PeriodicBoundary()), grid
+    cls = {"rusanov": burgers.Rusanov, "godunov": burgers.Godunov,
+           "eo": burgers.EngquistOsher}[kind]
+    return BoundOperator(cls(rec=rec), grid, PeriodicBoundary()), grid
 
 
 def _learned_case(n: int, seed: int):
@@ -407,7 +408,7 @@ def _learned_case(n: int, seed: int):
     return p, u, rng.standard_normal(n + 6)
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "eo"])
 def test_learned_operator_vjp_vs_complex_step(torch_mod, kind: str) -> None:
     """J_L(u)^T mu of the learned operator against the dense complex-step
     Jacobian of the oracle's surrogate (shock cells WENO-JS, smooth cells
@@ -433,7 +434,7 @@ def test_learned_operator_vjp_vs_complex_step(torch_mod, kind: str) -> None:
         assert _rel(got, ref) < 1e-11, (seed, _rel(got, ref))
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "eo"])
 def test_learned_adjoint_tau0_equals_plain(torch_mod, kind: str) -> None:
     """With tau = 0 every varying cell is in the shock regime: the learned
     checkpointed adjoint reproduces the WENO-JS one bit for bit."""
@@ -461,7 +462,7 @@ def test_learned_adjoint_tau0_equals_plain(torch_mod, kind: str) -> None:
     assert torch.equal(gl, gp)
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "eo"])
 def test_learned_adjoint_consistency(torch_mod, kind: str) -> None:
     """The learned checkpointed adjoint (whole-step recompute, fused reverse
     step) equals the chain of single-step VJPs (stage kernels) bit for bit,

@@ -45,22 +45,26 @@ def _u0(n, dtype=np.float64):
     return u.astype(dtype)
 
 
+def _scheme(kind, rec):
+    from pyshocks_amd.burgers import EngquistOsher, Godunov, LaxFriedrichs, Rusanov
+
+    return {"rusanov": Rusanov, "godunov": Godunov, "lf": LaxFriedrichs,
+            "eo": EngquistOsher}[kind](rec=rec)
+
+
 def _stepper(torch, kind, grid, rec):
-    from pyshocks_amd.burgers import Godunov, Rusanov
     from pyshocks_amd.scalar import PeriodicBoundary
     from pyshocks_amd.timestepping import make_ssprk33
 
-    sch = (Rusanov if kind == "rusanov" else Godunov)(rec=rec)
-    return make_ssprk33(sch, grid, PeriodicBoundary(), theta=1.0)
+    return make_ssprk33(_scheme(kind, rec), grid, PeriodicBoundary(), theta=1.0)
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "lf", "eo"])
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
 @pytest.mark.parametrize("sigma", [0.0, 1.01])
 def test_learned_operator_vs_oracle(torch_mod, kind, dtype_name, sigma):
     torch = torch_mod
     from pyshocks_amd import apply_operator, make_uniform_cell_grid
-    from pyshocks_amd.burgers import Godunov, Rusanov
     from pyshocks_amd.reconstruction import LearnedWENOJS53
     from pyshocks_amd.scalar import PeriodicBoundary
 
@@ -71,7 +75,7 @@ def test_learned_operator_vs_oracle(torch_mod, kind, dtype_name, sigma):
     from pyshocks_amd.learned import MLPWeights
 
     rec = LearnedWENOJS53(eps=1e-6, weights=MLPWeights.from_numpy(p, 20.0, sigma))
-    sch = (Rusanov if kind == "rusanov" else Godunov)(rec=rec)
+    sch = _scheme(kind, rec)
     u = _u0(n, np.float64 if dtype_name == "float64" else np.float32)
     L = apply_operator(sch, grid, PeriodicBoundary(), 0.0,
                        torch.tensor(u, device="cuda")).cpu().double().numpy()[3:-3]
@@ -90,7 +94,7 @@ def test_learned_operator_vs_oracle(torch_mod, kind, dtype_name, sigma):
     assert abs(np.sum(L)) * grid.h < (1e-12 if dtype_name == "float64" else 1e-4) * scale
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "lf", "eo"])
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
 def test_fallback_is_wenojs_bit_for_bit(torch_mod, kind, dtype_name):
     """tau = 0 flags every cell with any variation: the learned solver must
@@ -144,9 +148,10 @@ def test_learned_fused_vs_generic(torch_mod, tiles):
     assert np.linalg.norm(a - b) <= 1e-10 * np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("kind", ["rusanov", "eo"])
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
 @pytest.mark.parametrize("tiles", [(256, 2), (256, 4)])
-def test_learned_step_schedule_matches_stage_kernels(torch_mod, dtype_name, tiles):
+def test_learned_step_schedule_matches_stage_kernels(torch_mod, dtype_name, tiles, kind):
     """The whole-step kernel with learned weights (three stages of a tile in
     LDS) against the three stage kernels, over several tiles and a ragged
     last one."""
@@ -159,7 +164,7 @@ def test_learned_step_schedule_matches_stage_kernels(torch_mod, dtype_name, tile
     n = 100_003
     grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, dtype=dtype)
     rec = LearnedWENOJS53(eps=1e-6, weights=_weights(_params(), 20.0))
-    st = _stepper(torch, "rusanov", grid, rec)
+    st = _stepper(torch, kind, grid, rec)
     u0 = torch.tensor(_u0(n), device="cuda").to(dtype)
     out = []
     for sched in (_lib.PSA_SCHEDULE_TILE, _lib.PSA_SCHEDULE_STEP):
@@ -180,8 +185,9 @@ def test_learned_step_schedule_matches_stage_kernels(torch_mod, dtype_name, tile
         assert float((a - b).norm()) <= 5e-4 * float(b.norm())
 
 
+@pytest.mark.parametrize("kind", ["rusanov", "eo"])
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
-def test_batched_learned_equals_single_instance(torch_mod, dtype_name):
+def test_batched_learned_equals_single_instance(torch_mod, dtype_name, kind):
     torch = torch_mod
     from pyshocks_amd import make_uniform_cell_grid
     from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
@@ -194,13 +200,13 @@ def test_batched_learned_equals_single_instance(torch_mod, dtype_name):
     ns = np.array([5, 300, 1500, 4100])
     states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
     solver = BatchedSolver(ns, dtype=dtype, tiles=(256, 4), grid=2, mlp=w, time_f64=False,
-                           team_cells=None)
+                           team_cells=None, scheme=kind)
     solver.set_states(states)
     res = solver.solve(0.04)
     assert np.all(res.status == 0)
     for i, n in enumerate(ns):
         grid = make_uniform_cell_grid(0.0, 1.0, int(n), nghosts=3, dtype=dtype)
-        st = _stepper(torch, "rusanov", grid, LearnedWENOJS53(eps=1e-6, weights=w))
+        st = _stepper(torch, kind, grid, LearnedWENOJS53(eps=1e-6, weights=w))
         ev = evolve(st, torch.tensor(states[i], device="cuda").to(dtype), tfinal=0.04,
                     tiles=(256, 4))
         assert res.steps[i] == ev.iteration

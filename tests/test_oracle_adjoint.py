@@ -106,7 +106,7 @@ def test_learned_surrogate_values() -> None:
     np.testing.assert_allclose(ul0.real, jl, rtol=0, atol=1e-15)
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "eo"])
 def test_learned_step_jacobian_vs_fd(kind: str) -> None:
     """Complex-step columns of the learned surrogate's step Jacobian against
     central differences of its real evaluation (the branches fixed by the

@@ -115,7 +115,24 @@ def test_flagged_cells_keep_wenojs():
     assert np.max(np.abs(ur[~flags][5:-5] - jr[~flags][5:-5])) > 1e-8
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "lf", "eo"])
+def test_learned_operator_tau0_is_wenojs_operator(kind):
+    """With tau = 0 every varying cell keeps WENO-JS, so the learned operator
+    of every flux is the oracle's WENO-JS operator exactly (the flux
+    formulas, not just the reconstruction, are the reference's)."""
+    n = 64
+    grid = weno_np.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
+    rng = np.random.default_rng(9)
+    p = (rng.standard_normal(M.NPARAM) * 0.5).astype(np.float32)
+    u = 0.5 + np.sin(2 * np.pi * grid.x) + 0.3 * (grid.x > 0.6) + 0.05 * rng.standard_normal(
+        grid.x.size)
+    L = M.learned_apply_operator(p, 0.0, kind, grid, u, 1e-6)
+    ref = weno_np.apply_operator(weno_np.Scheme(kind, eps=1e-6), grid, 0.0,
+                                 weno_np.apply_boundary_periodic(grid, u))
+    np.testing.assert_array_equal(L[3:-3], ref[3:-3])
+
+
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "lf", "eo"])
 def test_learned_operator_conservative(kind):
     n = 64
     grid = weno_np.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
