@@ -498,7 +498,10 @@ typedef struct psa_batch_desc {
    * tiles with the three stages of a tile in LDS (step_tile of the
    * whole-step kernel; one barrier per step, the state ping-pongs between
    * u and k1); PSA_SCHEDULE_TILE: one pass and one barrier per stage. Both
-   * compute the same values. */
+   * compute the same values. PSA_FLUX_LAX_FRIEDRICHS (WENO-JS5 weights,
+   * no learned weights) always takes the stage passes: each stage's flux
+   * speed is max|u| of its input over the instance, reduced after the
+   * stage before. */
   int schedule;
   int rec;                      /* PSA_REC_* (0 = WENO-JS5); the others take
                                    g >= 2 and tiles 256 x 4 or 256 x 2, and

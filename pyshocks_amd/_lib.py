@@ -53,7 +53,7 @@ PSA_BC_DIRICHLET = 3
 
 #: flux codes of the batched solver's Burgers scheme names
 BATCH_FLUX = {"rusanov": PSA_FLUX_RUSANOV, "godunov": PSA_FLUX_BURGERS_UPWIND,
-              "eo": PSA_FLUX_ENGQUIST_OSHER}
+              "eo": PSA_FLUX_ENGQUIST_OSHER, "lf": PSA_FLUX_LAX_FRIEDRICHS}
 
 PSA_DT_CFL = 0
 PSA_DT_FIXED = 1

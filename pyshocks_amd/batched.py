@@ -59,7 +59,7 @@ class BatchedSolver:
         ``grid`` no team is larger than ``grid`` workgroups."""
         if scheme not in _lib.BATCH_FLUX:
             raise NotImplementedError(
-                f"the batched solver takes Burgers 'rusanov', 'godunov' or 'eo', "
+                f"the batched solver takes Burgers 'rusanov', 'godunov', 'lf' or 'eo', "
                 f"not {scheme!r}")
         rec_kind = _lib.PSA_REC_WENOJS53
         if rec is not None:

@@ -45,7 +45,10 @@ struct TeamState {
   // after the first barrier of a step); whole-step sweeps rotate over all
   // three (one barrier per step)
   unsigned long long umax[3];
-  unsigned long long pad1[3];
+  // Lax-Friedrichs: max|k1| and max|k2| of the step, the flux speeds of
+  // stages 2 and 3 (each cleared after the barrier that follows its reads)
+  unsigned long long speed[2];
+  unsigned long long pad1[1];
 };
 static_assert(sizeof(TeamState) == PSA_TEAM_STATE_BYTES, "team state size");
 
@@ -216,16 +219,16 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
 
     // team reduction of max|u| bits into buffer b (a single workgroup keeps
     // the block maximum)
+    // (b = 0 .. 2: umax[b]; 3, 4: speed[b - 3])
     auto team_max = [&](unsigned long long v, int b) -> unsigned long long {
       v = wave_uniform(block_max_bcast<BLOCK>(v, sm.red, &s_bcast));
       if (ts == nullptr) return v;
+      unsigned long long *slot = b < 3 ? &ts->umax[b] : &ts->speed[b - 3];
       if (tid == 0 && v != 0)
-        __hip_atomic_fetch_max(&ts->umax[b], v, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_max(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       alive = alive && team_barrier(ts, size, ++nbar, a.wait_ticks, &s_flag);
       if (tid == 0)
-        s_bcast = __hip_atomic_load(&ts->umax[b], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
+        s_bcast = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __syncthreads();
       return wave_uniform(s_bcast);
     };
@@ -319,16 +322,26 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
         }
         umax = team_max(vmax, static_cast<int>(steps % 3));  // u^(n+1) complete
       } else {
-        unsigned long long vdummy = 0;
+        // Lax-Friedrichs (scalar.py:276-277): the flux speed of each stage is
+        // max|u| of its input over the instance (its ghosts are images of
+        // interior cells): u^n's from the step control, k1's and k2's from
+        // the maxima the stages before reduce over the team
+        constexpr bool kLF = FLUX == PSA_FLUX_LAX_FRIEDRICHS;
+        unsigned long long v1 = 0, v2 = 0;
         p.un = u;
         p.uk = u;
         p.out = k1;
+        if constexpr (kLF) p.vconst = from_bits<T>(umax);
         for (int64_t tile = tlo; tile < thi; ++tile) {
           __syncthreads();
           stage_tile<T, FLUX, 0, 1, BLOCK, CPT, LEARNED, false, false, REC>(
-              p, tile * TILE, dt, sm, vdummy, q, wlp);
+              p, tile * TILE, dt, sm, v1, q, wlp);
         }
-        barrier();  // k1 complete
+        if constexpr (kLF) {
+          p.vconst = from_bits<T>(team_max(v1, 3));  // k1 complete
+        } else {
+          barrier();  // k1 complete
+        }
         // every member has read the max|u| reduced two steps ago: clear that
         // buffer for the next step
         if (ts != nullptr && rank == 0 && tid == 0)
@@ -339,9 +352,18 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
         for (int64_t tile = tlo; tile < thi; ++tile) {
           __syncthreads();
           stage_tile<T, FLUX, 0, 2, BLOCK, CPT, LEARNED, false, false, REC>(
-              p, tile * TILE, dt, sm, vdummy, q, wlp);
+              p, tile * TILE, dt, sm, v2, q, wlp);
         }
-        barrier();  // k2 complete
+        if constexpr (kLF) {
+          p.vconst = from_bits<T>(team_max(v2, 4));  // k2 complete
+          // every member has read max|k1| (after the barrier before this
+          // one); the next writes come after the step's last barrier
+          if (ts != nullptr && rank == 0 && tid == 0)
+            __hip_atomic_store(&ts->speed[0], 0ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          barrier();  // k2 complete
+        }
         p.uk = k2;
         p.out = u;
         unsigned long long vmax = 0;
@@ -351,6 +373,13 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs<T> a) {
               p, tile * TILE, dt, sm, vmax, q, wlp);
         }
         umax = team_max(vmax, static_cast<int>(steps & 1));  // u complete
+        // every member has read max|k2| (after the barrier before this one);
+        // the next writes come after the next step's first barrier
+        if constexpr (kLF) {
+          if (ts != nullptr && rank == 0 && tid == 0)
+            __hip_atomic_store(&ts->speed[1], 0ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
       t = t + dt;
       t64 = t64 + dt_t64;
@@ -385,6 +414,8 @@ __global__ void k_batch_reset(unsigned int *counter, TeamState *teams,
     teams[j].umax[0] = 0ull;
     teams[j].umax[1] = 0ull;
     teams[j].umax[2] = 0ull;
+    teams[j].speed[0] = 0ull;
+    teams[j].speed[1] = 0ull;
   }
 }
 
@@ -498,6 +529,27 @@ int launch_batch_tiles(const BatchArgs<T> &a, int block, int cpt, int grid,
                        std::to_string(cpt));
 }
 
+// Lax-Friedrichs: stage sweeps only (every stage needs the maximum of its
+// input over the instance), WENO-JS5 weights.
+template <typename T>
+int launch_batch_lf(const BatchArgs<T> &a, int block, int cpt, int grid, int max_team,
+                    long long nteams, hipStream_t s) {
+  constexpr int F = PSA_FLUX_LAX_FRIEDRICHS;
+  if (block == 256 && cpt == 4)
+    return launch_batch_cfg<T, F, 256, 4, false, false>(a, grid, max_team, nteams, s);
+  if (block == 256 && cpt == 2)
+    return launch_batch_cfg<T, F, 256, 2, false, false>(a, grid, max_team, nteams, s);
+  if (block == 512 && cpt == 2)
+    return launch_batch_cfg<T, F, 512, 2, false, false>(a, grid, max_team, nteams, s);
+  if (block == 512 && cpt == 4)
+    return launch_batch_cfg<T, F, 512, 4, false, false>(a, grid, max_team, nteams, s);
+  if (block == 1024 && cpt == 2)
+    return launch_batch_cfg<T, F, 1024, 2, false, false>(a, grid, max_team, nteams, s);
+  return set_error(PSA_ERR_INVALID_ARGUMENT,
+                   "unsupported batch tile configuration block=" + std::to_string(block) +
+                       " cells_per_thread=" + std::to_string(cpt));
+}
+
 template <typename T, int FLUX>
 int launch_batch_flux(const BatchArgs<T> &a, bool whole, int block, int cpt,
                       int grid, int max_team, long long nteams, hipStream_t s) {
@@ -556,6 +608,8 @@ int launch_batch(const psa_batch_desc *d, void *u, void *k1, void *k2,
   const int cpt = d->cells_per_thread > 0 ? d->cells_per_thread : 4;
   const int max_team = teams ? d->max_team : 1;
   const bool whole = d->schedule != PSA_SCHEDULE_TILE;
+  if (d->flux == PSA_FLUX_LAX_FRIEDRICHS)
+    return launch_batch_lf<T>(a, block, cpt, d->grid, max_team, d->ninst, s);
   if (d->flux == PSA_FLUX_RUSANOV)
     return launch_batch_flux<T, PSA_FLUX_RUSANOV>(a, whole, block, cpt, d->grid,
                                                   max_team, d->ninst, s);
@@ -580,10 +634,14 @@ int check_batch_desc(const psa_batch_desc *desc) {
   if (desc->dtype != PSA_FLOAT64 && desc->dtype != PSA_FLOAT32)
     return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported dtype");
   if (desc->flux != PSA_FLUX_RUSANOV && desc->flux != PSA_FLUX_BURGERS_UPWIND &&
-      desc->flux != PSA_FLUX_ENGQUIST_OSHER)
+      desc->flux != PSA_FLUX_ENGQUIST_OSHER && desc->flux != PSA_FLUX_LAX_FRIEDRICHS)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
-                     "the batched solver takes Burgers Rusanov, Godunov or "
-                     "Engquist-Osher");
+                     "the batched solver takes Burgers Rusanov, Godunov, "
+                     "Lax-Friedrichs or Engquist-Osher");
+  if (desc->flux == PSA_FLUX_LAX_FRIEDRICHS &&
+      (desc->rec != PSA_REC_WENOJS53 || desc->mlp != nullptr))
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the batched Lax-Friedrichs solver takes the WENO-JS5 weights");
   if (desc->rec < PSA_REC_WENOJS53 || desc->rec > PSA_REC_MUSCLS)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "unsupported reconstruction " + std::to_string(desc->rec));

@@ -25,13 +25,14 @@ def torch_mod():
 
 def _single(torch, n: int, u0: np.ndarray, dtype, tfinal: float, scheme: str = "rusanov"):
     from pyshocks_amd import make_uniform_cell_grid
-    from pyshocks_amd.burgers import EngquistOsher, Godunov, Rusanov
+    from pyshocks_amd.burgers import EngquistOsher, Godunov, LaxFriedrichs, Rusanov
     from pyshocks_amd.reconstruction import WENOJS53
     from pyshocks_amd.scalar import PeriodicBoundary
     from pyshocks_amd.timestepping import evolve, make_ssprk33
 
     grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, dtype=dtype)
-    cls = {"rusanov": Rusanov, "godunov": Godunov, "eo": EngquistOsher}[scheme]
+    cls = {"rusanov": Rusanov, "godunov": Godunov, "eo": EngquistOsher,
+           "lf": LaxFriedrichs}[scheme]
     sch = cls(rec=WENOJS53(eps=1e-6))
     st = make_ssprk33(sch, grid, PeriodicBoundary(), theta=1.0)
     u = torch.tensor(u0, device="cuda").to(dtype)
@@ -39,7 +40,7 @@ def _single(torch, n: int, u0: np.ndarray, dtype, tfinal: float, scheme: str = "
 
 
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
-@pytest.mark.parametrize("scheme", ["rusanov", "godunov", "eo"])
+@pytest.mark.parametrize("scheme", ["rusanov", "godunov", "eo", "lf"])
 def test_batched_equals_single_instance(torch_mod, dtype_name: str, scheme: str) -> None:
     torch = torch_mod
     from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
@@ -64,7 +65,7 @@ def test_batched_equals_single_instance(torch_mod, dtype_name: str, scheme: str)
         assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref)
 
 
-@pytest.mark.parametrize("scheme", ["rusanov", "eo"])
+@pytest.mark.parametrize("scheme", ["rusanov", "eo", "lf"])
 def test_batched_vs_oracle(torch_mod, scheme: str) -> None:
     torch = torch_mod
     from oracle import weno_np as W
@@ -145,9 +146,12 @@ def test_autotune_picks_a_launchable_config(torch_mod) -> None:
 
 
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
-def test_teams_match_single_workgroup_bit_for_bit(torch_mod, dtype_name: str) -> None:
+@pytest.mark.parametrize("scheme", ["rusanov", "lf"])
+def test_teams_match_single_workgroup_bit_for_bit(torch_mod, dtype_name: str,
+                                                  scheme: str) -> None:
     """Splitting an instance's tiles among a team changes who computes a
-    tile, not what is computed."""
+    tile, not what is computed (Lax-Friedrichs: nor the stage speeds the
+    team reduces)."""
     torch = torch_mod
     from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
 
@@ -157,7 +161,8 @@ def test_teams_match_single_workgroup_bit_for_bit(torch_mod, dtype_name: str) ->
     states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
     out = []
     for team_cells in (None, 2048):
-        solver = BatchedSolver(ns, dtype=dtype, tiles=(256, 4), team_cells=team_cells)
+        solver = BatchedSolver(ns, dtype=dtype, tiles=(256, 4), team_cells=team_cells,
+                               scheme=scheme)
         if team_cells:
             assert solver.team.max() > 1
         solver.set_states(states)
