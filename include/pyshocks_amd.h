@@ -532,8 +532,9 @@ int psa_batch_kernel_info(const psa_batch_desc *desc, int *vgprs,
  * (adjoint_step) with matrix-free vector-Jacobian products of the fused
  * step. Cotangents are arrays of m cells whose ghost entries are ignored on
  * input and zero on output. Supported: Rusanov (alpha = 1), Burgers upwind,
- * Engquist-Osher and advection with a constant velocity; dt is a device
- * scalar held constant. Non-smooth points follow jax's conventions (max ties
+ * Engquist-Osher, Lax-Friedrichs (alpha = 1; its global speed max|u| is
+ * differentiated too; single instances) and advection with a constant
+ * velocity; dt is a device scalar held constant. Non-smooth points follow jax's conventions (max ties
  * split 0.5 / 0.5, sign(0) = 0, jnp.where differentiates the selected
  * branch). The three Burgers fluxes also take the learned reconstruction (desc->mlp,
  * no counterpart in the reference): the transposed Jacobian is that of the

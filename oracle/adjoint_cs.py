@@ -46,6 +46,14 @@ def apply_operator_cs(kind: str, grid: W.Grid, u: np.ndarray, *, eps: float,
         fr = ur**2 / 2
         aavg = (ur[:-1] + ul[1:]) / 2
         fnum = np.where(aavg.real > 0, fr[:-1], fl[1:])
+    elif kind == "lf":
+        # the global speed max|u| over the ghost-filled array: the entry of
+        # the largest real modulus (scalar.py:276-277)
+        fl = ul**2 / 2
+        fr = ur**2 / 2
+        au = _abs_cs(u)
+        a = au[int(np.argmax(au.real))]
+        fnum = 0.5 * (fl[1:] + fr[:-1]) - 0.5 * a * (ul[1:] - ur[:-1])
     elif kind == "eo":
         # f(max(ur, 0)) + f(min(ul, 0)), the switches on the real parts
         p = np.where(ur[:-1].real > 0, ur[:-1], 0)

@@ -171,6 +171,7 @@ struct VjpArgs {
   T eps4;             // LEARNED: 4 eps (the fused indicators carry 4 beta)
   T tau;              // LEARNED: shock threshold
   T sigma;            // LEARNED: smooth threshold
+  T alpha;            // Lax-Friedrichs: the global speed max|state| (in-kernel)
 };
 
 // Regime of a cell (mlp_regime) from its right stencil with the indicator
@@ -435,6 +436,11 @@ __device__ __forceinline__ T vjp_block(const VjpArgs<T> &p, const T *s_u,
       const T wR = aR > aL ? T(1) : (aR < aL ? T(0) : T(0.5));
       duR = dFda * wR * sgn(uR);
       duL = dFda * (T(1) - wR) * sgn(uL);
+    } else if constexpr (FLUX == PSA_FLUX_LAX_FRIEDRICHS) {
+      // F = 0.5 (ul^2/2 + ur^2/2) - 0.5 alpha (ul - ur) with the global speed
+      // alpha, whose own derivative k_lf_speed_vjp adds
+      dur = T(0.5) * urL + T(0.5) * p.alpha;
+      dul = T(0.5) * ulR - T(0.5) * p.alpha;
     } else if constexpr (FLUX == PSA_FLUX_ENGQUIST_OSHER) {
       // F = max(ur, 0)^2 / 2 + min(ul, 0)^2 / 2
       dur = urL > T(0) ? urL : T(0);
@@ -510,7 +516,11 @@ __global__ __launch_bounds__(kVjpBlock) void k_stage_vjp(VjpArgs<T> p) {
   const int g = p.g;
   const int64_t S = g + static_cast<int64_t>(blockIdx.x) * kVjpOut;
 
-  if (blockIdx.x == 0 && tid < g) {
+  if constexpr (FLUX == PSA_FLUX_LAX_FRIEDRICHS) {
+    // the speed k_lf_alpha left in the output's first ghost entry (the
+    // ghosts are cleared by k_lf_speed_vjp after this launch)
+    p.alpha = p.out[0];
+  } else if (blockIdx.x == 0 && tid < g) {
     // ghost entries of a cotangent are zero
     p.out[tid] = T(0);
     p.out[p.m - g + tid] = T(0);
@@ -616,6 +626,108 @@ __global__ void k_zero_ghosts(T *u, int64_t m, int g, int64_t ld) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Lax-Friedrichs: the global speed alpha = max |state| over the ghost-filled
+// array (scalar.py:276-277) couples every face to the cells where |state|
+// is largest. One workgroup each: k_lf_alpha puts alpha into out[0] (a
+// ghost entry) for k_stage_vjp, which differentiates the faces at fixed
+// alpha; k_lf_speed_vjp then adds the derivative through alpha, G =
+// sum over the faces of gF dF/dalpha, dF/dalpha = -(ul - ur) / 2, shared by
+// the entries of the ghost-filled array where |state| = alpha (jax's max:
+// ties share equally; d|x|/dx = sign(x)), mapped to their cells through the
+// periodic fill, and clears the ghost entries.
+
+constexpr int kLfBlock = 256;
+
+template <typename T>
+__device__ __forceinline__ T lf_block_sum(T v, T *s_red) {
+  const int tid = threadIdx.x;
+  s_red[tid] = v;
+  __syncthreads();
+  for (int w = kLfBlock / 2; w > 0; w >>= 1) {
+    if (tid < w) s_red[tid] = s_red[tid] + s_red[tid + w];
+    __syncthreads();
+  }
+  const T r = s_red[0];
+  __syncthreads();
+  return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kLfBlock) void k_lf_alpha(VjpArgs<T> p) {
+  __shared__ T s_red[kLfBlock];
+  const int tid = threadIdx.x;
+  T mx = T(0);
+  bool nan = false;
+  for (int64_t c = p.g + tid; c < p.m - p.g; c += kLfBlock) {
+    const T v = p.state[c];
+    mx = fmax(mx, fabs(v));
+    nan |= isnan(v);
+  }
+  s_red[tid] = nan ? T(NAN) : mx;
+  __syncthreads();
+  for (int w = kLfBlock / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+      const T a = s_red[tid], b = s_red[tid + w];
+      s_red[tid] = (isnan(a) || isnan(b)) ? T(NAN) : fmax(a, b);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) p.out[0] = s_red[0];
+}
+
+template <typename T, int RSTAGE>
+__global__ __launch_bounds__(kLfBlock) void k_lf_speed_vjp(VjpArgs<T> p) {
+  __shared__ T s_red[kLfBlock];
+  const int tid = threadIdx.x;
+  const int64_t n = p.n, m = p.m;
+  const int g = p.g;
+  const T alpha = p.out[0];
+  // G over the n faces of the periodic ring (face j between cells j - 1, j)
+  T acc = T(0);
+  for (int64_t j = g + tid; j < m - g; j += kLfBlock) {
+    T v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = load_periodic(p.state, j - 3 + k, n, g);
+    T G[5];
+    const T urL = weno_side_grad(v[0], v[1], v[2], v[3], v[4], p.eps, G);
+    const T ulR = weno_side_grad(v[5], v[4], v[3], v[2], v[1], p.eps, G);
+    const T gF = (load_periodic(p.mu, j, n, g) - load_periodic(p.mu, j - 1, n, g)) * p.inv_dx;
+    acc += gF * (T(-0.5) * (ulR - urL));
+  }
+  const T G = lf_block_sum(acc, s_red);
+  // entries of the ghost-filled array that hold cell c: c itself, the left
+  // ghost c - n for cells n .. n + g - 1 and the right ghost c + n for cells
+  // g .. 2g - 1
+  auto mult = [&](int64_t c) {
+    return 1 + ((c >= n && c < n + g) ? 1 : 0) + ((c >= g && c < 2 * g) ? 1 : 0);
+  };
+  T cnt = T(0);
+  for (int64_t c = g + tid; c < m - g; c += kLfBlock)
+    if (fabs(p.state[c]) == alpha) cnt += T(mult(c));
+  const T count = lf_block_sum(cnt, s_red);
+  T coef = T(1);
+  if constexpr (RSTAGE == 3) {
+    coef = T(2.0 / 3.0) * *p.dt_ptr;
+  } else if constexpr (RSTAGE == 2) {
+    coef = T(1.0 / 4.0) * *p.dt_ptr;
+  } else if constexpr (RSTAGE == 1) {
+    coef = *p.dt_ptr;
+  }
+  if (count > T(0)) {
+    const T share = coef * G / count;
+    for (int64_t c = g + tid; c < m - g; c += kLfBlock) {
+      const T v = p.state[c];
+      if (fabs(v) == alpha) p.out[c] += share * T(mult(c)) * sgn(v);
+    }
+  }
+  // the ghost entries (out[0] carried alpha; every thread read it above)
+  if (tid < g) {
+    p.out[tid] = T(0);
+    p.out[m - g + tid] = T(0);
+  }
+}
+
 template <typename T, int FLUX, bool LEARNED = false>
 int launch_vjp_flux(int rstage, const VjpArgs<T> &a, hipStream_t s) {
   const unsigned int blocks =
@@ -668,6 +780,28 @@ int launch_vjp(const psa_solver_desc *d, int rstage, const void *state,
     if (d->flux == PSA_FLUX_ENGQUIST_OSHER)
       return launch_vjp_flux<T, PSA_FLUX_ENGQUIST_OSHER, true>(rstage, a, s);
     return launch_vjp_flux<T, PSA_FLUX_BURGERS_UPWIND, true>(rstage, a, s);
+  }
+  if (d->flux == PSA_FLUX_LAX_FRIEDRICHS) {
+    hipLaunchKernelGGL(k_lf_alpha<T>, dim3(1), dim3(kLfBlock), 0, s, a);
+    int st = check_launch("speed of the stage");
+    if (st != PSA_OK) return st;
+    st = launch_vjp_flux<T, PSA_FLUX_LAX_FRIEDRICHS>(rstage, a, s);
+    if (st != PSA_OK) return st;
+    switch (rstage) {
+      case 0:
+        hipLaunchKernelGGL((k_lf_speed_vjp<T, 0>), dim3(1), dim3(kLfBlock), 0, s, a);
+        break;
+      case 1:
+        hipLaunchKernelGGL((k_lf_speed_vjp<T, 1>), dim3(1), dim3(kLfBlock), 0, s, a);
+        break;
+      case 2:
+        hipLaunchKernelGGL((k_lf_speed_vjp<T, 2>), dim3(1), dim3(kLfBlock), 0, s, a);
+        break;
+      default:
+        hipLaunchKernelGGL((k_lf_speed_vjp<T, 3>), dim3(1), dim3(kLfBlock), 0, s, a);
+        break;
+    }
+    return check_launch("speed vjp");
   }
   if (d->flux == PSA_FLUX_RUSANOV)
     return launch_vjp_flux<T, PSA_FLUX_RUSANOV>(rstage, a, s);
@@ -742,14 +876,17 @@ int check_vjp_desc(const psa_solver_desc *d) {
     return set_error(PSA_ERR_NOT_IMPLEMENTED, "the adjoint is periodic");
   if (d->g < 3 || d->n < d->g)
     return set_error(PSA_ERR_INVALID_ARGUMENT, "bad grid size");
-  if (d->flux == PSA_FLUX_LAX_FRIEDRICHS)
-    return set_error(PSA_ERR_NOT_IMPLEMENTED,
-                     "the adjoint does not support the Lax-Friedrichs flux");
-  if (d->flux < 0 || d->flux > 3)
+  if (d->flux < 0 || d->flux > PSA_FLUX_LAX_FRIEDRICHS)
     return set_error(PSA_ERR_NOT_IMPLEMENTED, "unsupported flux");
-  if (d->flux == PSA_FLUX_RUSANOV && d->nu != nullptr)
+  if ((d->flux == PSA_FLUX_RUSANOV || d->flux == PSA_FLUX_LAX_FRIEDRICHS) &&
+      d->nu != nullptr)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
-                     "the adjoint supports the Rusanov flux with alpha = 1");
+                     "the adjoint supports the Rusanov and Lax-Friedrichs fluxes "
+                     "with alpha = 1");
+  if (d->flux == PSA_FLUX_LAX_FRIEDRICHS && d->mlp != nullptr)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the adjoint of learned weights takes Rusanov, Godunov and "
+                     "Engquist-Osher");
   if (d->flux == PSA_FLUX_ADVECTION_UPWIND && !d->velocity_is_const)
     return set_error(PSA_ERR_NOT_IMPLEMENTED,
                      "the adjoint supports advection with a constant velocity");
@@ -789,20 +926,48 @@ AdjointLayout adjoint_layout(const psa_solver_desc *d, long long nsteps,
   L.array_bytes = ((cells * elem_size(d) + 255) / 256) * 256;
   L.ncp = nsteps / every + 1;
   L.nwin = 3LL * every;
-  // checkpoints, window (x, k1, k2 per step), running state, 4 cotangents
-  L.bytes = static_cast<size_t>(L.ncp + L.nwin + 1 + 4) * L.array_bytes;
+  // checkpoints, window (x, k1, k2 per step), running state, 4 cotangents,
+  // and 256 bytes for the Lax-Friedrichs stage speed
+  L.bytes = static_cast<size_t>(L.ncp + L.nwin + 1 + 4) * L.array_bytes + 256;
   return L;
+}
+
+// The device scalar of the Lax-Friedrichs stage speed in the workspace.
+void *lf_speed_slot(void *workspace, const AdjointLayout &L) {
+  return static_cast<char *>(workspace) + (L.bytes - 256);
 }
 
 }  // namespace
 }  // namespace psa
 
-// defined in forward.hip
+// defined in forward.hip and reductions.hip
 extern "C" int psa_ssprk33_stage(const psa_solver_desc *desc, int stage,
                                  const void *un, const void *uk, void *out,
                                  void *control, const void *dt, void *stream);
+extern "C" int psa_max_abs_interior(int dtype, const void *u, void *out, int64_t m,
+                                    int g, void *stream);
 
 using namespace psa;
+
+namespace {
+
+// One fixed-dt stage (psa_ssprk33_stage without a control block); for
+// Lax-Friedrichs the flux speed max|uk| of the stage input first goes to the
+// device scalar speed (scalar.py:276-277: the interior maximum, the ghosts
+// being images).
+int forward_stage(const psa_solver_desc *d, int stage, const void *un, const void *uk,
+                  void *out, const void *dt, void *speed, void *stream) {
+  if (d->flux != PSA_FLUX_LAX_FRIEDRICHS)
+    return psa_ssprk33_stage(d, stage, un, uk, out, nullptr, dt, stream);
+  const int64_t m = d->n + 2 * static_cast<int64_t>(d->g);
+  const int st = psa_max_abs_interior(d->dtype, uk, speed, m, d->g, stream);
+  if (st != PSA_OK) return st;
+  psa_solver_desc dl = *d;
+  dl.velocity = speed;
+  return psa_ssprk33_stage(&dl, stage, un, uk, out, nullptr, dt, stream);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -843,10 +1008,11 @@ int psa_ssprk33_step_vjp(const psa_solver_desc *desc, const void *u,
     return set_error(PSA_ERR_INVALID_ARGUMENT, "lam_out must not alias");
   char *w = static_cast<char *>(work);
   void *k1 = w, *k2 = w + ab, *lk2 = w + 2 * ab, *lk1 = w + 3 * ab;
-  // recompute the stages of the forward step
-  st = psa_ssprk33_stage(desc, 1, u, u, k1, nullptr, dt, stream);
+  // recompute the stages of the forward step (a Lax-Friedrichs stage speed
+  // in lk1's storage, which the reverse stages write later)
+  st = forward_stage(desc, 1, u, u, k1, dt, lk1, stream);
   if (st != PSA_OK) return st;
-  st = psa_ssprk33_stage(desc, 2, u, k1, k2, nullptr, dt, stream);
+  st = forward_stage(desc, 2, u, k1, k2, dt, lk1, stream);
   if (st != PSA_OK) return st;
 
   st = psa_ssprk33_stage_vjp(desc, 3, k2, lam, nullptr, nullptr, lk2, dt, stream);
@@ -919,12 +1085,13 @@ int forward_checkpointed(const psa_solver_desc *desc, long long batch, int64_t l
     if (batch > 1)
       return set_error(PSA_ERR_NOT_IMPLEMENTED,
                        "a batch runs on the whole-step kernel (default or step "
-                       "schedule, tiles 256 x 4 or 256 x 2)");
-    st = psa_ssprk33_stage(desc, 1, cur, cur, k1, nullptr, dt, s);
+                       "schedule, tiles 256 x 4 or 256 x 2, not Lax-Friedrichs)");
+    void *speed = lf_speed_slot(workspace, L);
+    st = forward_stage(desc, 1, cur, cur, k1, dt, speed, s);
     if (st != PSA_OK) return st;
-    st = psa_ssprk33_stage(desc, 2, cur, k1, k2, nullptr, dt, s);
+    st = forward_stage(desc, 2, cur, k1, k2, dt, speed, s);
     if (st != PSA_OK) return st;
-    st = psa_ssprk33_stage(desc, 3, cur, k2, cur, nullptr, dt, s);
+    st = forward_stage(desc, 3, cur, k2, cur, dt, speed, s);
     if (st != PSA_OK) return st;
     if ((step + 1) % every == 0) {
       st = copy_async(cp((step + 1) / every), cur, bytes, s);
@@ -961,8 +1128,15 @@ int adjoint(const psa_solver_desc *desc, long long batch, int64_t ld, long long 
   char *lam_b = lam_a + L.array_bytes;
   char *lk2 = lam_b + L.array_bytes;
   char *lk1 = lk2 + L.array_bytes;
-  const bool fused_reverse = batch > 1 || (desc->schedule != PSA_SCHEDULE_TILE &&
-                                           desc->schedule != PSA_SCHEDULE_PIPELINE);
+  // (Lax-Friedrichs: the stage-wise reverse, whose global speed term needs a
+  // reduction between the stages)
+  if (desc->flux == PSA_FLUX_LAX_FRIEDRICHS && batch > 1)
+    return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                     "the batched adjoint does not take Lax-Friedrichs");
+  const bool fused_reverse =
+      desc->flux != PSA_FLUX_LAX_FRIEDRICHS &&
+      (batch > 1 || (desc->schedule != PSA_SCHEDULE_TILE &&
+                     desc->schedule != PSA_SCHEDULE_PIPELINE));
 
   st = copy_async(lam_a, lam_final, bytes, s);
   if (st != PSA_OK) return st;
@@ -984,13 +1158,14 @@ int adjoint(const psa_solver_desc *desc, long long batch, int64_t ld, long long 
       if (batch > 1)
         return set_error(PSA_ERR_NOT_IMPLEMENTED,
                          "a batch runs on the whole-step kernel (default or step "
-                         "schedule, tiles 256 x 4 or 256 x 2)");
-      st = psa_ssprk33_stage(desc, 1, X(i), X(i), K1(i), nullptr, dt, s);
+                         "schedule, tiles 256 x 4 or 256 x 2, not Lax-Friedrichs)");
+      void *speed = lf_speed_slot(workspace, L);
+      st = forward_stage(desc, 1, X(i), X(i), K1(i), dt, speed, s);
       if (st != PSA_OK) return st;
-      st = psa_ssprk33_stage(desc, 2, X(i), K1(i), K2(i), nullptr, dt, s);
+      st = forward_stage(desc, 2, X(i), K1(i), K2(i), dt, speed, s);
       if (st != PSA_OK) return st;
       if (step + 1 < s1) {
-        st = psa_ssprk33_stage(desc, 3, X(i), K2(i), X(i + 1), nullptr, dt, s);
+        st = forward_stage(desc, 3, X(i), K2(i), X(i + 1), dt, speed, s);
         if (st != PSA_OK) return st;
       }
     }

@@ -12,6 +12,9 @@ pytestmark = pytest.mark.gpu
 
 RTOL = 1.0e-12
 KINDS = ["rusanov", "godunov", "eo", "advection"]
+#: with Lax-Friedrichs (single instances: the stage-wise reverse with the
+#: derivative through its global speed)
+KINDS_LF = KINDS + ["lf"]
 
 
 @pytest.fixture(scope="module")
@@ -66,7 +69,7 @@ def _data(n: int, seed: int):
     return u, lam
 
 
-@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("kind", KINDS_LF)
 def test_operator_vjp_vs_complex_step(torch_mod, kind: str) -> None:
     torch = torch_mod
     from oracle import adjoint_cs as A
@@ -85,7 +88,7 @@ def test_operator_vjp_vs_complex_step(torch_mod, kind: str) -> None:
     np.testing.assert_array_equal(got[-3:], 0.0)
 
 
-@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("kind", KINDS_LF)
 @pytest.mark.parametrize("n", [40, 600])
 def test_step_vjp_vs_complex_step(torch_mod, kind: str, n: int) -> None:
     torch = torch_mod
@@ -212,7 +215,7 @@ def test_adjoint_resumes_from_disk(torch_mod, tmp_path) -> None:
     assert torch.equal(out[0], out[1])
 
 
-@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("kind", KINDS_LF)
 def test_checkpointed_adjoint_matches_step_vjps(torch_mod, kind: str) -> None:
     torch = torch_mod
     from pyshocks_amd.adjoint import checkpointed_solve, ssprk33_step_vjp
@@ -239,7 +242,8 @@ def test_checkpointed_adjoint_matches_step_vjps(torch_mod, kind: str) -> None:
     np.testing.assert_array_equal(_host(got)[:3], 0.0)
 
 
-def test_gradient_vs_central_fd(torch_mod) -> None:
+@pytest.mark.parametrize("kind", ["rusanov", "lf"])
+def test_gradient_vs_central_fd(torch_mod, kind: str) -> None:
     """J = 0.5 ||u(T) - u_target||^2 dx; <grad J, d> against central
     differences of the GPU forward solve with h = 1e-7 on 8 seeded unit
     directions. The rounding noise of the forward solve (~1e-15 per cell)
@@ -255,7 +259,7 @@ def test_gradient_vs_central_fd(torch_mod) -> None:
     target = u0 + 0.2 * np.sin(4 * np.pi * x + 1.0)
     dt = 0.5 * (1.0 / n) / 1.5
     h = 1.0 / n
-    src, _ = _source(torch, "rusanov", n)
+    src, _ = _source(torch, kind, n)
 
     def objective(u):
         s = checkpointed_solve(src, u, nsteps, dt, every=every)

@@ -17,7 +17,7 @@ def _data(n: int, seed: int) -> np.ndarray:
     return 0.3 + np.sin(2 * np.pi * x) + 0.2 * rng.standard_normal(n + 6)
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov", "eo", "advection"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "eo", "advection", "lf"])
 def test_complex_restatement_matches_oracle(kind: str) -> None:
     n = 48
     grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
@@ -28,7 +28,7 @@ def test_complex_restatement_matches_oracle(kind: str) -> None:
     np.testing.assert_allclose(got.real[3:-3], ref[3:-3], rtol=0, atol=1e-13)
 
 
-@pytest.mark.parametrize("kind", ["rusanov", "godunov", "eo", "advection"])
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "eo", "advection", "lf"])
 def test_step_jacobian_vs_central_fd(kind: str) -> None:
     n = 24
     grid = W.make_uniform_cell_grid(0.0, 1.0, n, nghosts=3)
