@@ -100,9 +100,10 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
                         "pipelined grid)")
     p.add_argument("--learned", action="store_true",
                    help="single workload with the shipped learned weights")
-    p.add_argument("--team-cells", type=int, default=0,
-                   help="batched workload: cells per team member (0: the solver's "
-                        "default, no teams for WENO-JS5 and 16384 for the others)")
+    p.add_argument("--team-cells", type=int, default=None,
+                   help="batched and learned workloads: cells per team member (0: no "
+                        "teams; unset: the solver's default, teams sized by work for "
+                        "WENO-JS5 and of 16384 cells for the others)")
     p.add_argument("--max-team", type=int, default=32)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain device copy of N FP64 values (N * 8 "
@@ -747,7 +748,7 @@ def batched_leg(args: argparse.Namespace) -> dict:
     other than WENO-JS5 runs with 2 ghost cells, tiles 256 x 4 or 256 x 2 and
     (unless --team-cells says otherwise) the solver's teams of 16384 cells,
     which its smaller tiles need to keep the largest instances off the
-    critical path."""
+    critical path; WENO-JS5 takes the solver's teams sized by work."""
     import numpy as np
     import torch
 
@@ -759,13 +760,12 @@ def batched_leg(args: argparse.Namespace) -> dict:
     g = 3 if rec is None else 2
     ns, ics = random_instances(args.ninst, seed=args.seed)
     states = [exact_averages_np(ic, int(n), nghosts=g) for ic, n in zip(ics, ns)]
-    sample = np.random.default_rng(args.seed + 1).choice(len(ns), size=min(32, len(ns)),
-                                                        replace=False)
     cands = autotune.BATCH_CANDIDATES if rec is None else ((256, 4), (256, 2))
-    best, _ = autotune.tune_batch(ns[sample], [states[i] for i in sample],
-                                  dtype=torch.float32, tfinal=0.01, rec=rec,
-                                  candidates=cands)
-    solver = BatchedSolver(ns, dtype=torch.float32, team_cells=args.team_cells or None,
+    # the whole batch to t = 0.01 with each candidate's teams: tiles and teams
+    # are chosen together
+    best, _ = autotune.tune_batch(ns, states, dtype=torch.float32, tfinal=0.01, rec=rec,
+                                  candidates=cands, team_cells=args.team_cells)
+    solver = BatchedSolver(ns, dtype=torch.float32, team_cells=args.team_cells,
                            max_team=args.max_team, nghosts=g, rec=rec,
                            tiles=None if rec is None else best)
     solver.set_states(states)
@@ -780,8 +780,10 @@ def batched_leg(args: argparse.Namespace) -> dict:
     if np.any(res.status != 0):
         raise RuntimeError(f"{int(np.sum(res.status != 0))} instances did not finish")
     cu = solver.cell_updates()
+    teams = np.bincount(solver.team)
     return {"value": cu / elapsed, "seconds": elapsed, "cell_updates": cu,
             "tiles": list(best), "ninst": len(ns), "total_cells": int(np.sum(ns)),
+            "teams": {str(k): int(c) for k, c in enumerate(teams) if k > 1 and c},
             "max_steps": int(res.steps.max()), "ns": ns, "states": states}
 
 
@@ -822,7 +824,8 @@ def learned_leg(args: argparse.Namespace, *, ninst: int = 1024, nmin: int = 1024
     WENO-JS5 FP64 batched solver at equal L1 error: ``ninst`` seeded
     sine-plus-step Burgers instances with N log-uniform in [nmin, nmax],
     t = 0.5, L1 errors against exact Hopf-Lax cell averages; every solver
-    with its own tiles (autotuned, spilling kernels skipped).
+    with its own tiles (autotuned on the whole batch with the teams each
+    tile gives it, spilling kernels skipped).
 
     The plain solver's error-vs-N curve is measured on the same instances
     with every N scaled by each of ``scales``; its order p fixes the scale
@@ -842,13 +845,14 @@ def learned_leg(args: argparse.Namespace, *, ninst: int = 1024, nmin: int = 1024
     rng = np.random.default_rng(5000 + args.seed)
     ns = np.round(np.exp(rng.uniform(np.log(nmin), np.log(nmax), ninst))).astype(np.int64)
     ics = training_ics(ninst, seed=5000 + args.seed)
-    u0s = exact.burgers_cell_averages(ics[:16], ns[:16], 0.0)
+    u0s = [u.cpu().numpy() for u in exact.burgers_cell_averages(ics, ns, 0.0)]
     candidates = ((256, 4), (256, 2), (512, 2), (1024, 2))
-    teams = args.team_cells or None
+    teams = args.team_cells
 
     def tune(dtype, mlp):
-        tiles, _ = autotune.tune_batch(ns[:16], [u.cpu().numpy() for u in u0s], dtype=dtype,
-                                       tfinal=0.01, mlp=mlp, candidates=candidates)
+        # the whole batch to t = 0.01 with each candidate's teams
+        tiles, _ = autotune.tune_batch(ns, u0s, dtype=dtype, tfinal=0.01, mlp=mlp,
+                                       candidates=candidates, team_cells=teams)
         return tiles
 
     plain_tiles = tune(torch.float64, None)

@@ -527,6 +527,13 @@ int psa_batch_solve(const psa_batch_desc *desc, void *u, void *k1, void *k2,
 int psa_batch_kernel_info(const psa_batch_desc *desc, int *vgprs,
                           long long *private_bytes, long long *lds_bytes);
 
+/* Workgroups of the kernel psa_batch_solve launches for desc that the
+ * current device holds at once: the persistent grid when desc->grid is 0,
+ * and the most workgroups that a team may span. The host sizes teams from
+ * it. Launches nothing and reads no buffer. Part of the batched front end,
+ * which the reference does not have. */
+int psa_batch_resident_workgroups(const psa_batch_desc *desc, int *workgroups);
+
 /* Discrete adjoint --------------------------------------------------------
  * Replaces the dense jax.jacfwd Jacobian of timestepping.py:155-215
  * (adjoint_step) with matrix-free vector-Jacobian products of the fused

@@ -218,6 +218,7 @@ SIGNATURES: dict[str, list[Any]] = {
                         _VP, _VP],
     "psa_batch_kernel_info": [ctypes.POINTER(PsaBatchDesc), ctypes.POINTER(_I),
                               ctypes.POINTER(_LL), ctypes.POINTER(_LL)],
+    "psa_batch_resident_workgroups": [ctypes.POINTER(PsaBatchDesc), ctypes.POINTER(_I)],
     "psa_ssprk33_stage_vjp": [ctypes.POINTER(PsaSolverDesc), _I, _VP, _VP, _VP,
                               _VP, _VP, _VP, _VP],
     "psa_ssprk33_step_vjp": [ctypes.POINTER(PsaSolverDesc), _VP, _VP, _VP, _VP,

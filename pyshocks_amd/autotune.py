@@ -77,13 +77,18 @@ def tune_batch(ncells: Sequence[int], states: Sequence[np.ndarray], *,
                dtype: torch.dtype = torch.float32, scheme: str = "rusanov",
                tfinal: float = 0.01, eps: float = 1.0e-6, theta: float = 1.0,
                candidates: Sequence[tuple[int, int]] = BATCH_CANDIDATES,
-               mlp: Any = None, rec: Any = None,
+               mlp: Any = None, rec: Any = None, team_cells: int | None = None,
                ) -> tuple[tuple[int, int], dict[tuple[int, int], float]]:
-    """Time the batched solve of a sample of instances (to a short final
-    time) for each candidate and cache the fastest; with a reconstruction
-    other than WENO-JS5 (``rec``, states with 2 ghost cells) the winner is
-    returned but not cached. A candidate whose kernel spills registers to
-    private memory is skipped without being timed."""
+    """Time the batched solve of the instances (to a short final time) for
+    each candidate and cache the fastest; with a reconstruction other than
+    WENO-JS5 (``rec``, states with 2 ghost cells) the winner is returned but
+    not cached. A candidate whose kernel spills registers to private memory
+    is skipped without being timed. Each candidate runs with the teams the
+    solver gives these instances under ``team_cells`` (``None``: the
+    solver's default), which depend on the candidate's resident workgroups:
+    passed the whole batch, the tuner picks tiles and teams together, and a
+    short final time keeps the batch's imbalance, since every instance's
+    step count scales with it."""
     from pyshocks_amd import _lib
     from pyshocks_amd.batched import BatchedSolver
 
@@ -92,7 +97,7 @@ def tune_batch(ncells: Sequence[int], states: Sequence[np.ndarray], *,
         try:
             solver = BatchedSolver(ncells, dtype=dtype, scheme=scheme, eps=eps,
                                    theta=theta, tiles=cand, mlp=mlp, rec=rec,
-                                   nghosts=3 if rec is None else 2)
+                                   nghosts=3 if rec is None else 2, team_cells=team_cells)
             if solver.kernel_info()["private_bytes"] > 0:
                 continue
             solver.set_states(states)

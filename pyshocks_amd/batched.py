@@ -27,6 +27,21 @@ from pyshocks_amd import _lib
 ALIGN = 64  # elements; keeps every instance 16-byte aligned in FP32 and FP64
 
 
+def work_teams(ncells: np.ndarray, grid: int, cap: np.ndarray | int,
+               share: float = 1.0) -> np.ndarray:
+    """Team sizes from work: on one domain an instance's CFL step count grows
+    like its cell count, so its work (cell-updates) like the square. An
+    instance with more than ``share`` times the batch's mean work per
+    workgroup of a ``grid`` of workgroups would still run on one workgroup
+    after the others ran out of instances; it is split among
+    ``ceil(work / (share * mean))`` members (at most ``cap``), which leaves
+    the small instances, whose barriers would cost more than they save, on
+    one workgroup each."""
+    work = np.asarray(ncells, dtype=np.float64) ** 2
+    mean = share * work.sum() / grid
+    return np.clip(np.ceil(work / mean), 1, cap).astype(np.int32)
+
+
 @dataclass
 class BatchResult:
     t: np.ndarray
@@ -56,7 +71,8 @@ class BatchedSolver:
         weights; their smaller tiles leave a large instance many tiles per
         step, so ``team_cells`` defaults to 16384 for them (``0``: no teams).
         A team's workgroups must all be resident, so with an explicit
-        ``grid`` no team is larger than ``grid`` workgroups."""
+        ``grid`` no team is larger than ``grid`` workgroups. For WENO-JS5
+        ``team_cells=None`` sizes teams by work (:meth:`_work_teams`)."""
         if scheme not in _lib.BATCH_FLUX:
             raise NotImplementedError(
                 f"the batched solver takes Burgers 'rusanov', 'godunov', 'lf' or 'eo', "
@@ -151,17 +167,40 @@ class BatchedSolver:
             self.desc.block, self.desc.cells_per_thread = int(tiles[0]), int(tiles[1])
         self._set_teams(order, team_cells, max_team, team_wait_s)
 
+    #: work share (of the mean work per resident workgroup) above which
+    #: :meth:`_work_teams` splits an instance
+    WORK_SHARE = 1.0
+
+    def resident_workgroups(self) -> int:
+        """Workgroups of the selected kernel the device holds at once (the
+        persistent grid when ``grid`` is 0, and the largest team)."""
+        wg = ctypes.c_int(0)
+        _lib.call("psa_batch_resident_workgroups", ctypes.byref(self.desc), ctypes.byref(wg))
+        return int(wg.value)
+
+    def _work_teams(self, cap: np.ndarray) -> np.ndarray:
+        """:func:`work_teams` on the persistent grid (none if an explicit
+        ``grid`` exceeds what the device holds at once)."""
+        resident = self.resident_workgroups()
+        grid = self.desc.grid or resident
+        if grid > resident:
+            return np.ones(self.ninst, dtype=np.int32)
+        return work_teams(self.ncells, grid, cap, self.WORK_SHARE)
+
     def _set_teams(self, order: np.ndarray, team_cells: int | None, max_team: int,
                    wait_s: float) -> None:
-        """Instances above ``team_cells`` cells are split among a team of
-        ``ceil(m / team_cells)`` workgroups (at most ``max_team``, at most one
-        per tile) that meet at a device-wide barrier after every stage."""
+        """Instances above ``team_cells`` cells (``None``: the work rule of
+        :meth:`_work_teams`) are split among a team of ``ceil(m / team_cells)``
+        workgroups (at most ``max_team``, at most one per tile) that meet at a
+        device-wide barrier after every stage."""
         tile = (self.desc.block or 256) * (self.desc.cells_per_thread or 4)
         ntiles = (self.m + tile - 1) // tile
-        if team_cells:
-            cap = np.minimum(max_team, ntiles)
-            if self.desc.grid > 0:
-                cap = np.minimum(cap, self.desc.grid)
+        cap = np.minimum(max_team, ntiles)
+        if self.desc.grid > 0:
+            cap = np.minimum(cap, self.desc.grid)
+        if team_cells is None:
+            team = self._work_teams(cap)
+        elif team_cells:
             team = np.clip((self.m + team_cells - 1) // team_cells, 1, cap).astype(np.int32)
         else:
             team = np.ones(self.ninst, dtype=np.int32)

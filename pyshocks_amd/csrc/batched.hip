@@ -85,8 +85,10 @@ struct BatchArgs {
   T lim_theta;
   T lim_atol;
   // host only: when set, the dispatch fills in the selected kernel's
-  // attributes instead of launching it
+  // attributes (and, if resident is set too, the workgroups the device holds
+  // at once) instead of launching it
   hipFuncAttributes *query;
+  unsigned int *resident;
 };
 
 // A value every lane of the wave holds (read from LDS after a broadcast)
@@ -446,6 +448,11 @@ int launch_batch_cfg(const BatchArgs<T> &a, int grid, int max_team, long long nt
     if (err != hipSuccess)
       return set_error(PSA_ERR_HIP, std::string("hipFuncGetAttributes: ") +
                                         hipGetErrorString(err));
+    if (a.resident != nullptr) {
+      *a.resident = resident(k_batch<T, FLUX, BLOCK, CPT, LEARNED, WHOLE, REC>, BLOCK);
+      if (*a.resident == 0)
+        return set_error(PSA_ERR_HIP, "cannot query the resident workgroup count");
+    }
     return PSA_OK;
   }
   const unsigned int cap =
@@ -561,9 +568,11 @@ int launch_batch_flux(const BatchArgs<T> &a, bool whole, int block, int cpt,
 template <typename T>
 int launch_batch(const psa_batch_desc *d, void *u, void *k1, void *k2,
                  unsigned int *counter, double *t_out, long long *steps_out,
-                 int *status_out, hipStream_t s, hipFuncAttributes *query = nullptr) {
+                 int *status_out, hipStream_t s, hipFuncAttributes *query = nullptr,
+                 unsigned int *resident = nullptr) {
   BatchArgs<T> a{};
   a.query = query;
+  a.resident = resident;
   a.u = static_cast<T *>(u);
   a.k1 = static_cast<T *>(k1);
   a.k2 = static_cast<T *>(k2);
@@ -689,6 +698,23 @@ extern "C" int psa_batch_kernel_info(const psa_batch_desc *desc, int *vgprs,
   *vgprs = attr.numRegs;
   *private_bytes = static_cast<long long>(attr.localSizeBytes);
   *lds_bytes = static_cast<long long>(attr.sharedSizeBytes);
+  return PSA_OK;
+}
+
+extern "C" int psa_batch_resident_workgroups(const psa_batch_desc *desc,
+                                             int *workgroups) {
+  const int st = check_batch_desc(desc);
+  if (st != PSA_OK) return st;
+  if (workgroups == nullptr) return set_error(PSA_ERR_INVALID_ARGUMENT, "null pointer");
+  hipFuncAttributes attr{};
+  unsigned int cap = 0;
+  const int r = desc->dtype == PSA_FLOAT64
+                    ? launch_batch<double>(desc, nullptr, nullptr, nullptr, nullptr,
+                                           nullptr, nullptr, nullptr, nullptr, &attr, &cap)
+                    : launch_batch<float>(desc, nullptr, nullptr, nullptr, nullptr,
+                                          nullptr, nullptr, nullptr, nullptr, &attr, &cap);
+  if (r != PSA_OK) return r;
+  *workgroups = static_cast<int>(cap);
   return PSA_OK;
 }
 

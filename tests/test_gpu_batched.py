@@ -160,18 +160,21 @@ def test_teams_match_single_workgroup_bit_for_bit(torch_mod, dtype_name: str,
     ns = np.array([20000, 50, 3000, 9000, 700])
     states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
     out = []
-    for team_cells in (None, 2048):
+    for team_cells in (0, None, 2048):
         solver = BatchedSolver(ns, dtype=dtype, tiles=(256, 4), team_cells=team_cells,
                                scheme=scheme)
-        if team_cells:
-            assert solver.team.max() > 1
+        # 0: one workgroup each; None: teams sized by work (a batch this
+        # small spreads its larger instances over many workgroups)
+        assert (solver.team.max() > 1) == (team_cells != 0)
+        assert solver.team.max() <= solver.resident_workgroups()
         solver.set_states(states)
         res = solver.solve(0.02)
         assert np.all(res.status == 0)
         out.append((solver.u.clone(), res))
-    assert torch.equal(out[0][0], out[1][0])
-    np.testing.assert_array_equal(out[0][1].steps, out[1][1].steps)
-    np.testing.assert_array_equal(out[0][1].t, out[1][1].t)
+    for o in out[1:]:
+        assert torch.equal(out[0][0], o[0])
+        np.testing.assert_array_equal(out[0][1].steps, o[1].steps)
+        np.testing.assert_array_equal(out[0][1].t, o[1].t)
 
 
 def test_team_wait_limit_terminates(torch_mod) -> None:
@@ -213,7 +216,7 @@ def test_time_f64_reaches_tfinal(torch_mod) -> None:
 
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
 @pytest.mark.parametrize("scheme", ["rusanov", "godunov", "eo"])
-@pytest.mark.parametrize("team_cells", [None, 2048])
+@pytest.mark.parametrize("team_cells", [0, None, 2048])
 def test_step_schedule_matches_stage_sweeps(torch_mod, dtype_name: str, scheme: str,
                                             team_cells) -> None:
     """One sweep per step (the three stages of a tile in LDS, the state

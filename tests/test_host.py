@@ -396,3 +396,29 @@ def test_boundary_fill_plans_and_refusals() -> None:
     desc.velocity, desc.velocity_is_const = 48, 0
     assert lib.psa_ssprk33_steps(ctypes.byref(desc), p(8), p(16), p(24), p(32), p(40), 1,
                                  None) == _lib.PSA_ERR_NOT_IMPLEMENTED
+
+
+def test_work_teams_balance_the_persistent_grid() -> None:
+    """Teams sized by work (cell count squared): the smallest team that
+    brings an instance's work per member to the mean work per workgroup,
+    within the cap; a balanced batch stays on one workgroup per instance."""
+    from pyshocks_amd.batched import work_teams
+
+    assert np.all(work_teams(np.full(1000, 4096), 256, 32) == 1)
+    ns = np.array([65536] + [1024] * 300)
+    team = work_teams(ns, 256, 32)
+    assert team[0] == 32 and np.all(team[1:] == 1)
+    rng = np.random.default_rng(3)
+    for grid in (256, 512, 1024):
+        ns = np.round(np.exp(rng.uniform(np.log(128), np.log(65536), 500))).astype(np.int64)
+        cap = np.minimum(32, (ns + 1023) // 1024)
+        for share in (1.0, 0.5):
+            team = work_teams(ns, grid, cap, share)
+            w = ns.astype(np.float64) ** 2
+            mean = share * w.sum() / grid
+            assert np.all((team >= 1) & (team <= cap))
+            uncapped = team < cap
+            assert np.all(w[uncapped] / team[uncapped] <= mean * (1 + 1e-12))
+            split = team > 1
+            assert np.all(w[split] / (team[split] - 1) > mean)
+        assert np.all(work_teams(ns, grid, cap, 0.5) >= work_teams(ns, grid, cap, 1.0))
