@@ -9,8 +9,9 @@ ES-WENO3 and MUSCL with flux or slope limiters) with its own CFL time step -- th
 the reference harness would run it for each request separately. The instances
 are packed into three device arrays (state and two stage buffers) at 16-byte
 aligned offsets; the library runs them longest-first from an atomic queue,
-one workgroup per instance at a time (``psa_batch_solve``). The launch can be
-captured in a HIP graph and replayed (:meth:`BatchedSolver.capture`).
+one workgroup per instance, or a team of workgroups for an instance with more
+work than the mean per resident workgroup (``psa_batch_solve``). The launch
+can be captured in a HIP graph and replayed (:meth:`BatchedSolver.capture`).
 """
 
 from __future__ import annotations
