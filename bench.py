@@ -102,8 +102,7 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
                    help="single workload with the shipped learned weights")
     p.add_argument("--team-cells", type=int, default=None,
                    help="batched and learned workloads: cells per team member (0: no "
-                        "teams; unset: the solver's default, teams sized by work for "
-                        "WENO-JS5 and of 16384 cells for the others)")
+                        "teams; unset: the solver's default, teams sized by work)")
     p.add_argument("--max-team", type=int, default=32)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain device copy of N FP64 values (N * 8 "
@@ -744,11 +743,10 @@ def make_rec(args: argparse.Namespace):
 def batched_leg(args: argparse.Namespace) -> dict:
     """cell-updates/s of the uneven-size batch: --ninst Burgers instances with
     N log-uniform in [128, 65536] (seeded), FP32, Rusanov, each to t = 0.3,
-    one persistent launch replayed from a HIP graph, tiles autotuned; --rec
-    other than WENO-JS5 runs with 2 ghost cells, tiles 256 x 4 or 256 x 2 and
-    (unless --team-cells says otherwise) the solver's teams of 16384 cells,
-    which its smaller tiles need to keep the largest instances off the
-    critical path; WENO-JS5 takes the solver's teams sized by work."""
+    one persistent launch replayed from a HIP graph, tiles autotuned with
+    (unless --team-cells says otherwise) the solver's teams sized by work;
+    --rec other than WENO-JS5 runs with 2 ghost cells and tiles 256 x 4 or
+    256 x 2."""
     import numpy as np
     import torch
 

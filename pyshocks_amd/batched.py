@@ -62,18 +62,18 @@ class BatchedSolver:
                  device: torch.device | str = "cuda",
                  tiles: tuple[int, int] | None = None, grid: int = 0,
                  mlp: Any = None, time_f64: bool = True,
-                 team_cells: int | None = None, max_team: int = 32,
+                 team_cells: int | str | None = None, max_team: int = 32,
                  team_wait_s: float = 60.0, schedule: str = "step",
                  rec: Any = None) -> None:
         """``rec`` is a reconstruction object of the GPU path
         (:mod:`pyshocks_amd.reconstruction`; ``None``: WENO-JS5 with ``eps``);
         its own ``eps`` replaces the argument. The reconstructions other than
         WENO-JS5 take 2 ghost cells, tiles 256 x 4 or 256 x 2 and no learned
-        weights; their smaller tiles leave a large instance many tiles per
-        step, so ``team_cells`` defaults to 16384 for them (``0``: no teams).
-        A team's workgroups must all be resident, so with an explicit
-        ``grid`` no team is larger than ``grid`` workgroups. For WENO-JS5
-        ``team_cells=None`` sizes teams by work (:meth:`_work_teams`)."""
+        weights. ``team_cells`` ``None`` (or ``"work"``) sizes teams by work
+        (:meth:`_work_teams`), a count splits every instance above that many
+        cells, ``0`` runs one workgroup per instance. A team's workgroups
+        must all be resident, so with an explicit ``grid`` no team is larger
+        than ``grid`` workgroups."""
         if scheme not in _lib.BATCH_FLUX:
             raise NotImplementedError(
                 f"the batched solver takes Burgers 'rusanov', 'godunov', 'lf' or 'eo', "
@@ -85,8 +85,11 @@ class BatchedSolver:
             rec_kind = rec_code(rec)
             eps = float(getattr(rec, "eps", eps))
         gmin = 3 if rec_kind == _lib.PSA_REC_WENOJS53 else 2
-        if team_cells is None and rec_kind != _lib.PSA_REC_WENOJS53:
-            team_cells = 16384
+        if team_cells == "work":
+            team_cells = None
+        elif team_cells is not None and not isinstance(team_cells, (int, np.integer)):
+            raise ValueError(f"team_cells must be a cell count, 'work' or None, "
+                             f"not {team_cells!r}")
         if nghosts < gmin:
             raise ValueError(f"the reconstruction needs nghosts >= {gmin}")
         if rec_kind != _lib.PSA_REC_WENOJS53 and mlp is not None:

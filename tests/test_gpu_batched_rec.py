@@ -132,7 +132,8 @@ def test_step_schedule_matches_stage_sweeps(torch_mod, rec_name: str, team_cells
 def test_refusals(torch_mod) -> None:
     """Learned weights with another reconstruction, tiles other than 256 x 4
     or 256 x 2, and the Koren limiter without a slope form are refused
-    before any launch."""
+    before any launch (when the solver sizes its teams from the descriptor's
+    kernel, or at the latest when it solves)."""
     torch = torch_mod
     from pyshocks_amd.batched import BatchedSolver
 
@@ -140,12 +141,14 @@ def test_refusals(torch_mod) -> None:
     with pytest.raises(ValueError, match="learned"):
         BatchedSolver(ns, dtype=torch.float64, nghosts=2, rec=_rec("wenojs32"),
                       mlp=object())
-    solver = BatchedSolver(ns, dtype=torch.float64, nghosts=2, tiles=(512, 2),
-                           rec=_rec("muscl-minmod"))
-    solver.set_states([np.zeros(68), np.zeros(132)])
-    with pytest.raises(ValueError, match="256"):
-        solver.solve(0.1)
-    solver = BatchedSolver(ns, dtype=torch.float64, nghosts=2, rec=_rec("muscls-koren"))
-    solver.set_states([np.zeros(68), np.zeros(132)])
-    with pytest.raises(NotImplementedError, match="Koren"):
-        solver.solve(0.1)
+    for team_cells in (None, 0):
+        with pytest.raises(ValueError, match="256"):
+            solver = BatchedSolver(ns, dtype=torch.float64, nghosts=2, tiles=(512, 2),
+                                   rec=_rec("muscl-minmod"), team_cells=team_cells)
+            solver.set_states([np.zeros(68), np.zeros(132)])
+            solver.solve(0.1)
+        with pytest.raises(NotImplementedError, match="Koren"):
+            solver = BatchedSolver(ns, dtype=torch.float64, nghosts=2,
+                                   rec=_rec("muscls-koren"), team_cells=team_cells)
+            solver.set_states([np.zeros(68), np.zeros(132)])
+            solver.solve(0.1)
