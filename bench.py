@@ -104,6 +104,9 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
                    help="batched and learned workloads: cells per team member (0: no "
                         "teams; unset: the solver's default, teams sized by work)")
     p.add_argument("--max-team", type=int, default=32)
+    p.add_argument("--adjoint-every", type=int, default=0,
+                   help="adjoint workload: checkpoint interval (0: one window over the "
+                        "horizon, every stage state kept by the forward pass)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain device copy of N FP64 values (N * 8 "
                         "bytes read and written per launch), the reference "
@@ -566,8 +569,12 @@ def cpu_leg(args: argparse.Namespace, gpu: dict) -> dict | None:
 def adjoint_leg(args: argparse.Namespace, repeats: int = 3) -> dict:
     """Gradients/s of J = 0.5 ||u(T) - u_target||^2 dx with respect to u0 for
     periodic Burgers Rusanov, N = 16384, FP64, 2000 SSP-RK3 steps of a fixed
-    dt (Courant 0.5 on the unperturbed u0 = 0.5 + sin(2 pi x)), checkpoints
-    every 50 steps; u_target is the solve from u0 + a seeded perturbation."""
+    dt (Courant 0.5 on the unperturbed u0 = 0.5 + sin(2 pi x)); the forward
+    pass keeps every step's stage states (one window over the horizon: 786 MB
+    per instance, 25 GB for the batch of 32), so the reverse sweep recomputes
+    nothing (--adjoint-every K: checkpoints every K steps and windows
+    recomputed from them); u_target is the solve from u0 + a seeded
+    perturbation."""
     import math
 
     import numpy as np
@@ -580,7 +587,8 @@ def adjoint_leg(args: argparse.Namespace, repeats: int = 3) -> dict:
     from pyshocks_amd.scalar import PeriodicBoundary
     from pyshocks_amd.timestepping import BoundOperator
 
-    n, nsteps, every = 16384, 2000, 50
+    n, nsteps = 16384, 2000
+    every = args.adjoint_every or nsteps
     grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, device="cuda")
     src = BoundOperator(Rusanov(rec=WENOJS53(eps=1e-6)), grid, PeriodicBoundary())
     u0 = funcs.cell_averages(funcs.sine_offset(0.5), grid)
@@ -670,7 +678,7 @@ def adjoint_leg(args: argparse.Namespace, repeats: int = 3) -> dict:
     if not (bool(torch.isfinite(gl).all()) and bool(torch.isfinite(gbl).all())):
         raise RuntimeError("non-finite learned gradient")
     cpu = adjoint_cpu_baseline(args, u0.cpu().numpy(), target.cpu().numpy(), g.cpu().numpy(),
-                               n=n, dx=grid.h, dt=dt, nsteps=nsteps, every=every)
+                               n=n, dx=grid.h, dt=dt, nsteps=nsteps, every=50)
     return {"gradients_per_s": repeats / elapsed,
             "ms_per_gradient": elapsed / repeats * 1e3,
             "batched": {"batch": batch, "gradients_per_s": batch / elapsed_b,
@@ -683,7 +691,9 @@ def adjoint_leg(args: argparse.Namespace, repeats: int = 3) -> dict:
                                 "weights): the adjoint of the learned solver's surrogate"},
             "cpu_baseline": cpu,
             "config": f"burgers rusanov N={n} f64, {nsteps} steps fixed dt, "
-                      f"checkpoints every {every}, J = 0.5||u(T)-u_target||^2 dx"}
+                      + ("every stage state kept by the forward pass" if every >= nsteps
+                         else f"checkpoints every {every}")
+                      + ", J = 0.5||u(T)-u_target||^2 dx"}
 
 
 def adjoint_cpu_baseline(args: argparse.Namespace, u0, target, gpu_grad, *, n: int,
@@ -931,13 +941,14 @@ def learned_leg(args: argparse.Namespace, *, ninst: int = 1024, nmin: int = 1024
 
 
 def _gradient_rate(n: int, weights, batch: int = 32, n_ref: int = 16384,
-                   steps_ref: int = 2000, every: int = 50) -> dict:
+                   steps_ref: int = 2000, every: int = 0) -> dict:
     """Batched gradients/s of J = 0.5 ||u(T) - u_target||^2 dx for periodic
     Burgers Rusanov FP64 (WENO-JS5, or the learned weights) at N = n, over
     the horizon of steps_ref steps at N = n_ref with Courant number 0.5:
     instance b starts from 0.5 + sin(2 pi x) plus a seeded perturbation of
     amplitude 1e-3 and is compared with the solve from the unperturbed
-    state."""
+    state. ``every`` 0 keeps every stage state in the forward pass (one
+    window over the horizon)."""
     import math
 
     import numpy as np
@@ -958,6 +969,7 @@ def _gradient_rate(n: int, weights, batch: int = 32, n_ref: int = 16384,
     t_end = steps_ref * 0.5 / n_ref / umax
     nsteps = int(math.ceil(t_end / (0.5 * grid.h / umax)))
     dt = t_end / nsteps
+    every = every or nsteps
     target = CheckpointedSolve(src, u0, nsteps, dt, every).u_final
     ub = []
     for b in range(batch):

@@ -574,7 +574,10 @@ int psa_ssprk33_step_vjp(const psa_solver_desc *desc, const void *u,
  * gradient grad_u0 = (d u_final / d u0)^T lam_final by recomputing each
  * window of `every` steps from its checkpoint (keeping every stage state)
  * and sweeping it backwards. Both calls use the same workspace of
- * psa_adjoint_workspace_bytes() bytes. */
+ * psa_adjoint_workspace_bytes() bytes. With every >= nsteps (one window over
+ * the horizon, about 3 every arrays) the forward pass keeps every stage state
+ * itself and the reverse sweep recomputes nothing; the adjoint may then be
+ * run any number of times on the workspace the forward pass left. */
 int psa_adjoint_workspace_bytes(const psa_solver_desc *desc, long long nsteps,
                                 int every, size_t *bytes);
 int psa_ssprk33_forward_checkpointed(const psa_solver_desc *desc,

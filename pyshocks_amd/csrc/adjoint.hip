@@ -1038,6 +1038,11 @@ int check_batch(const psa_solver_desc *d, long long batch, long long ld) {
   return PSA_OK;
 }
 
+// With one window over the whole horizon the forward pass keeps the window's
+// states itself (the window's 3 every arrays hold all of them, X(nsteps)
+// being at most the running-state array X(every)).
+bool keeps_window(long long nsteps, int every) { return nsteps > 0 && every >= nsteps; }
+
 int forward_checkpointed(const psa_solver_desc *desc, long long batch, int64_t ld,
                          const void *u0, void *u_final, long long nsteps,
                          const void *dt, int every, void *workspace,
@@ -1064,6 +1069,36 @@ int forward_checkpointed(const psa_solver_desc *desc, long long batch, int64_t l
 
   st = copy_async(cp(0), u0, bytes, s);
   if (st != PSA_OK) return st;
+  if (keeps_window(nsteps, every)) {
+    // one window over the whole horizon: keep every step's states (x, k1,
+    // k2) in it now, so that the reverse sweep recomputes nothing
+    auto X = [&](long long i) { return win + (3 * i) * L.array_bytes; };
+    auto K1 = [&](long long i) { return win + (3 * i + 1) * L.array_bytes; };
+    auto K2 = [&](long long i) { return win + (3 * i + 2) * L.array_bytes; };
+    st = copy_async(X(0), u0, bytes, s);
+    if (st != PSA_OK) return st;
+    for (long long i = 0; i < nsteps; ++i) {
+      st = step_fixed_dt(desc, X(i), X(i + 1), K1(i), K2(i), dt, s, batch, ld);
+      if (st == PSA_OK) continue;
+      if (st != PSA_ERR_NOT_IMPLEMENTED) return st;
+      if (batch > 1)
+        return set_error(PSA_ERR_NOT_IMPLEMENTED,
+                         "a batch runs on the whole-step kernel (default or step "
+                         "schedule, tiles 256 x 4 or 256 x 2, not Lax-Friedrichs)");
+      void *speed = lf_speed_slot(workspace, L);
+      st = forward_stage(desc, 1, X(i), X(i), K1(i), dt, speed, s);
+      if (st != PSA_OK) return st;
+      st = forward_stage(desc, 2, X(i), K1(i), K2(i), dt, speed, s);
+      if (st != PSA_OK) return st;
+      st = forward_stage(desc, 3, X(i), K2(i), X(i + 1), dt, speed, s);
+      if (st != PSA_OK) return st;
+    }
+    if (nsteps % every == 0) {
+      st = copy_async(cp(1), X(nsteps), bytes, s);
+      if (st != PSA_OK) return st;
+    }
+    return copy_async(u_final, X(nsteps), bytes, s);
+  }
   st = copy_async(cur, u0, bytes, s);
   if (st != PSA_OK) return st;
   // whole steps, the state alternating between cur and the window's first
@@ -1146,9 +1181,13 @@ int adjoint(const psa_solver_desc *desc, long long batch, int64_t ld, long long 
     const long long s0 = w * every;
     const long long s1 = (s0 + every < nsteps) ? s0 + every : nsteps;
     // recompute the window from its checkpoint, keeping every stage state
-    st = copy_async(X(0), cp(w), bytes, s);
-    if (st != PSA_OK) return st;
-    for (long long step = s0; step < s1; ++step) {
+    // (kept by the forward pass already when it spans the horizon)
+    const bool kept = keeps_window(nsteps, every);
+    if (!kept) {
+      st = copy_async(X(0), cp(w), bytes, s);
+      if (st != PSA_OK) return st;
+    }
+    for (long long step = s0; step < s1 && !kept; ++step) {
       const long long i = step - s0;
       // one whole step writing k1, k2 and the next state (X(every) is the
       // running-state array, free in this pass)

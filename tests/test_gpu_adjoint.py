@@ -380,6 +380,54 @@ def test_batched_adjoint_equals_single(torch_mod, kind: str) -> None:
         assert torch.equal(gb[b], s.gradient(lam[b].contiguous()))
 
 
+@pytest.mark.parametrize("kind", KINDS_LF)
+def test_one_window_keeps_the_forward_states(torch_mod, kind: str) -> None:
+    """With every >= nsteps the forward pass keeps the stage states and the
+    reverse sweep recomputes nothing: the same final state and gradient, bit
+    for bit, as windows recomputed from checkpoints, and the same gradient
+    again on a second reverse sweep (single solves on both schedules, and a
+    batch)."""
+    torch = torch_mod
+    from pyshocks_amd import _lib
+    from pyshocks_amd.adjoint import BatchedCheckpointedSolve, CheckpointedSolve
+
+    n, nsteps = 3000, 23
+    # smooth data (the upwind flux without an entropy fix blows up on the
+    # rough data of _data, and NaN compares unequal to itself)
+    x = (np.arange(n + 6) - 2.5) / n
+    rng = np.random.default_rng(11)
+    u0 = 0.3 + np.sin(2 * np.pi * x) + 0.01 * np.sin(14 * np.pi * x)
+    lam = rng.standard_normal(n + 6)
+    dt = 0.3 / n
+    src, _ = _source(torch, kind, n)
+    for sched in (_lib.PSA_SCHEDULE_TILE, _lib.PSA_SCHEDULE_STEP):
+        out = []
+        for every in (5, nsteps, nsteps + 4):
+            solve = CheckpointedSolve(src, _dev(torch, u0), nsteps, dt, every=every,
+                                      schedule=sched)
+            g = solve.gradient(_dev(torch, lam))
+            assert bool(torch.isfinite(g).all())
+            assert torch.equal(g, solve.gradient(_dev(torch, lam)))
+            out.append((solve.u_final.clone(), g))
+        for o in out[1:]:
+            assert torch.equal(out[0][0], o[0])
+            assert torch.equal(out[0][1], o[1])
+    if kind == "lf":
+        return
+    x = (np.arange(n + 6) - 2.5) / n
+    rng = np.random.default_rng(101)
+    u0b = torch.tensor(np.stack([0.3 + np.sin(2 * np.pi * (x + rng.uniform()))
+                                 for _ in range(3)]), dtype=torch.float64, device="cuda")
+    lamb = torch.tensor(rng.standard_normal((3, n + 6)), dtype=torch.float64, device="cuda")
+    dts = [0.1 / n * (1.0 + 0.1 * b) for b in range(3)]
+    out = []
+    for every in (8, nsteps):
+        bs = BatchedCheckpointedSolve(src, u0b, nsteps, dts, every)
+        out.append((bs.u_final.clone(), bs.gradient(lamb)))
+    assert torch.equal(out[0][0], out[1][0])
+    assert torch.equal(out[0][1], out[1][1])
+
+
 # {{{ learned weights
 
 
