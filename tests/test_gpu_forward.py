@@ -576,6 +576,36 @@ def test_ring_schedule_bit_for_bit(torch_mod, kind: str, dtype_name: str, n: int
         assert torch.equal(a[k], b[k]) and torch.equal(b[k], d[k])
 
 
+@pytest.mark.parametrize("kind", ["rusanov", "godunov", "eo", "advection"])
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+@pytest.mark.parametrize("n", [1500, 20000])
+def test_one_cell_per_thread_tiles_bit_for_bit(torch_mod, kind: str, dtype_name: str,
+                                               n: int) -> None:
+    """Below 2^17 cells the whole-step kernel runs one cell per thread by
+    default (tiles 256 x 1); it computes what the 256 x 2 and 256 x 4 tiles
+    compute, bit for bit, over the call sequence of _schedule_run (a last
+    tile that is partial, a control block that stops inside the call)."""
+    torch = torch_mod
+    from oracle import exact as E
+    from pyshocks_amd import _lib, make_uniform_cell_grid
+    from pyshocks_amd.timestepping import fused_plan
+
+    dtype = getattr(torch, dtype_name)
+    grid = make_uniform_cell_grid(0.0, 1.0, n, nghosts=3, dtype=dtype)
+    vel = torch.ones(grid.n, dtype=dtype, device="cuda") if kind == "advection" else None
+    st = _stepper(torch, kind, grid, vel=vel, theta=0.5 if kind == "advection" else 1.0)
+    u0 = torch.tensor(E.exact_cell_averages(E.large_initial_condition(seed=8), n),
+                      device="cuda").to(dtype)
+    runs = []
+    for tiles in (None, (256, 1), (256, 2), (256, 4)):
+        plan = fused_plan(st, tiles=tiles)
+        assert plan.uses_step_kernel()
+        runs.append(_schedule_run(torch, plan, u0, _lib.PSA_SCHEDULE_STEP))
+    for r in runs[1:]:
+        assert r[2] == runs[0][2]
+        assert torch.equal(r[0], runs[0][0]) and torch.equal(r[1], runs[0][1])
+
+
 @pytest.mark.parametrize("rec", ["wenojs32", "esweno32", "muscl", "muscls", "esweno-flux"])
 @pytest.mark.parametrize("dtype_name", ["float64", "float32"])
 def test_ring_schedule_reconstructions(torch_mod, rec: str, dtype_name: str) -> None:
