@@ -104,6 +104,9 @@ def parse_args(argv: list[str] | None = None) -> argparse.Namespace:
                    help="batched and learned workloads: cells per team member (0: no "
                         "teams; unset: the solver's default, teams sized by work)")
     p.add_argument("--max-team", type=int, default=32)
+    p.add_argument("--adjoint-batch", type=int, default=64,
+                   help="adjoint workload: instances per batched gradient launch (64, "
+                        "the directions of the finite-difference check)")
     p.add_argument("--adjoint-every", type=int, default=0,
                    help="adjoint workload: checkpoint interval (0: one window over the "
                         "horizon, every stage state kept by the forward pass)")
@@ -571,7 +574,7 @@ def adjoint_leg(args: argparse.Namespace, repeats: int = 3) -> dict:
     periodic Burgers Rusanov, N = 16384, FP64, 2000 SSP-RK3 steps of a fixed
     dt (Courant 0.5 on the unperturbed u0 = 0.5 + sin(2 pi x)); the forward
     pass keeps every step's stage states (one window over the horizon: 786 MB
-    per instance, 25 GB for the batch of 32), so the reverse sweep recomputes
+    per instance, 50 GB for the batch of 64), so the reverse sweep recomputes
     nothing (--adjoint-every K: checkpoints every K steps and windows
     recomputed from them); u_target is the solve from u0 + a seeded
     perturbation."""
@@ -620,7 +623,7 @@ def adjoint_leg(args: argparse.Namespace, repeats: int = 3) -> dict:
     # perturbation, J_b against the same target), one launch per step for all
     from pyshocks_amd.adjoint import BatchedCheckpointedSolve
 
-    batch = 32
+    batch = args.adjoint_batch
     pb = []
     for b in range(batch):
         rb = np.random.default_rng(1000 + b)
