@@ -43,7 +43,8 @@
 namespace psa {
 namespace {
 
-constexpr int kVjpBlock = 256;
+constexpr int kVjpBlock = 256;                // computing cells per workgroup
+constexpr int kVjpThreads = 2 * kVjpBlock;    // a thread per cell and side
 constexpr int kVjpOut = kVjpBlock - 6;  // output cells per workgroup
 
 // cell gm of a periodic array with n interior cells and g ghosts
@@ -360,59 +361,80 @@ __device__ __forceinline__ T learned_side_grad(const float *__restrict__ P, T v0
 }
 
 // Scratch of one reverse stage over a block of B computing cells.
-template <typename T>
+template <typename T, bool SPLIT = true>
 struct VjpScratch {
   T ur[kVjpBlock], ul[kVjpBlock];
   T cR[kVjpBlock], cL[kVjpBlock], sL[kVjpBlock], sR[kVjpBlock];
   T P[5][kVjpBlock];
+  // gradients of the left reconstructions (computed by the other half of a
+  // split block)
+  T GL[SPLIT ? 5 : 1][SPLIT ? kVjpBlock : 1];
 };
 
 // JL(state)^T mu for the output cells Z .. Z + B - 7 of one block: s_u holds
 // the state at cells Z - 5 .. Z + B - 2, s_mu the cotangent at Z - 3 ..
-// Z + B - 3 (both in LDS, visible to the block). Thread t computes the
-// reconstructions of cell Z - 3 + t and returns the gathered sum of output
-// cell Z + t (valid for t < B - 6). Ends with the scratch in use.
-template <typename T, int FLUX, bool LEARNED = false>
+// Z + B - 3 (both in LDS, visible to the block). SPLIT (2 B threads): thread
+// t < B computes the right reconstruction of cell Z - 3 + t and its
+// gradient, thread B + t the left one (the two chains side by side, for
+// grids too small to fill the device); otherwise (B threads) thread t
+// computes both. Thread t returns the gathered sum of output cell Z + t
+// (valid for t < B - 6; threads B .. 2 B - 1 return 0). Both forms compute
+// the same values with the same roundings. Ends with the scratch in use.
+template <typename T, int FLUX, bool LEARNED = false, bool SPLIT = true>
 __device__ __forceinline__ T vjp_block(const VjpArgs<T> &p, const T *s_u,
-                                       const T *s_mu, VjpScratch<T> &sc) {
+                                       const T *s_mu, VjpScratch<T, SPLIT> &sc) {
   constexpr int B = kVjpBlock;
   const int tid = threadIdx.x;
+  const bool left = SPLIT && tid >= B;
+  const int c = left ? tid - B : tid;
 
-  // ---- reconstructions of computing cell c = Z - 3 + tid and gradients
-  T GR[5], GLm[5];
-  const T v0 = s_u[tid], v1 = s_u[tid + 1], v2 = s_u[tid + 2],
-          v3 = s_u[tid + 3], v4 = s_u[tid + 4];
+  // ---- one reconstruction of computing cell c = Z - 3 + c and its gradient:
+  // the right one from v0 .. v4, the left one from the mirrored stencil
+  T G[5];
+  const T v0 = s_u[c], v1 = s_u[c + 1], v2 = s_u[c + 2], v3 = s_u[c + 3], v4 = s_u[c + 4];
+  const T w0 = left ? v4 : v0, w1 = left ? v3 : v1, w3 = left ? v1 : v3,
+          w4 = left ? v0 : v4;
+  T rec;
   if constexpr (LEARNED) {
     const int regime = learned_regime5(v0, v1, v2, v3, v4, p.eps4, p.tau, p.sigma);
     if (regime == 0) {
-      sc.ur[tid] = weno_side_grad(v0, v1, v2, v3, v4, p.eps, GR);
-      sc.ul[tid] = weno_side_grad(v4, v3, v2, v1, v0, p.eps, GLm);
+      rec = weno_side_grad(w0, w1, v2, w3, w4, p.eps, G);
     } else if (regime == 1) {
-      sc.ur[tid] = ideal_side_grad(v0, v1, v2, v3, v4, GR);
-      sc.ul[tid] = ideal_side_grad(v4, v3, v2, v1, v0, GLm);
+      rec = ideal_side_grad(w0, w1, v2, w3, w4, G);
     } else {
-      // the right side, then the left one from the mirrored stencil: one
-      // copy of the network code, run twice
-#pragma unroll 1
-      for (int side = 0; side < 2; ++side) {
-        const bool l = side != 0;
-        T G[5];
-        const T r = learned_side_grad(p.mlp, l ? v4 : v0, l ? v3 : v1, v2, l ? v1 : v3,
-                                      l ? v0 : v4, G);
-        if (l) {
-          sc.ul[tid] = r;
-#pragma unroll
-          for (int k = 0; k < 5; ++k) GLm[k] = G[k];
-        } else {
-          sc.ur[tid] = r;
-#pragma unroll
-          for (int k = 0; k < 5; ++k) GR[k] = G[k];
-        }
-      }
+      rec = learned_side_grad(p.mlp, w0, w1, v2, w3, w4, G);
     }
   } else {
-    sc.ur[tid] = weno_side_grad(v0, v1, v2, v3, v4, p.eps, GR);
-    sc.ul[tid] = weno_side_grad(v4, v3, v2, v1, v0, p.eps, GLm);
+    rec = weno_side_grad(w0, w1, v2, w3, w4, p.eps, G);
+  }
+  if constexpr (SPLIT) {
+    if (left) {
+      sc.ul[c] = rec;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) sc.GL[k][c] = G[k];
+    } else {
+      sc.ur[c] = rec;
+    }
+  } else {
+    sc.ur[c] = rec;
+  }
+  T GLm[5];
+  if constexpr (!SPLIT) {
+    // the left reconstruction in the same thread, kept in registers
+    T recl;
+    if constexpr (LEARNED) {
+      const int regime = learned_regime5(v0, v1, v2, v3, v4, p.eps4, p.tau, p.sigma);
+      if (regime == 0) {
+        recl = weno_side_grad(v4, v3, v2, v1, v0, p.eps, GLm);
+      } else if (regime == 1) {
+        recl = ideal_side_grad(v4, v3, v2, v1, v0, GLm);
+      } else {
+        recl = learned_side_grad(p.mlp, v4, v3, v2, v1, v0, GLm);
+      }
+    } else {
+      recl = weno_side_grad(v4, v3, v2, v1, v0, p.eps, GLm);
+    }
+    sc.ul[c] = recl;
   }
   if (tid == 0) {
     sc.cL[0] = T(0);
@@ -471,8 +493,17 @@ __device__ __forceinline__ T vjp_block(const VjpArgs<T> &p, const T *s_u,
   // ---- scatter: contribution of cell c's reconstructions to c - 2 .. c + 2
   if (tid >= 1 && tid <= B - 2) {
     const T cR = sc.cR[tid], cL = sc.cL[tid];
+    // (one explicit multiply-add, so that both forms round alike)
 #pragma unroll
-    for (int r = 0; r < 5; ++r) sc.P[r][tid] = cR * GR[r] + cL * GLm[4 - r];
+    for (int r = 0; r < 5; ++r) {
+      T gl;
+      if constexpr (SPLIT) {
+        gl = sc.GL[4 - r][tid];
+      } else {
+        gl = GLm[4 - r];
+      }
+      sc.P[r][tid] = fma(cR, G[r], cL * gl);
+    }
   }
   __syncthreads();
 
@@ -505,7 +536,7 @@ __device__ __forceinline__ T vjp_value(T mu, T gsum, T dt, T lam, T lamk2) {
 // RSTAGE 0: out = JL(state)^T mu; 3, 2, 1: the reverse stages above, with
 // mu = l, lk2, lk1 respectively.
 template <typename T, int FLUX, int RSTAGE, bool LEARNED = false>
-__global__ __launch_bounds__(kVjpBlock) void k_stage_vjp(VjpArgs<T> p) {
+__global__ __launch_bounds__(kVjpThreads) void k_stage_vjp(VjpArgs<T> p) {
   constexpr int B = kVjpBlock;
   __shared__ T s_u[B + 4];   // cells S - 5 ..
   __shared__ T s_mu[B + 1];  // cells S - 3 ..
@@ -526,8 +557,10 @@ __global__ __launch_bounds__(kVjpBlock) void k_stage_vjp(VjpArgs<T> p) {
     p.out[p.m - g + tid] = T(0);
   }
 
-  for (int j = tid; j < B + 4; j += B) s_u[j] = load_periodic(p.state, S - 5 + j, n, g);
-  for (int j = tid; j < B + 1; j += B) s_mu[j] = load_periodic(p.mu, S - 3 + j, n, g);
+  for (int j = tid; j < B + 4; j += kVjpThreads)
+    s_u[j] = load_periodic(p.state, S - 5 + j, n, g);
+  for (int j = tid; j < B + 1; j += kVjpThreads)
+    s_mu[j] = load_periodic(p.mu, S - 3 + j, n, g);
   __syncthreads();
 
   const T gsum = vjp_block<T, FLUX, LEARNED>(p, s_u, s_mu, sc);
@@ -562,14 +595,16 @@ constexpr int kStepVjpOut = kVjpBlock - 18;  // output cells per workgroup
 // reverse stages 3, 2 and 1 in LDS (vjp_block each): lk2 is exact on
 // [S - 6, S + B - 12), lk1 on [S - 3, S + B - 15), the step's cotangent on the
 // output cells [S, S + B - 18). The same arithmetic as k_stage_vjp.
-template <typename T, int FLUX, bool LEARNED = false>
-__global__ __launch_bounds__(kVjpBlock) void k_step_vjp(StepVjpArgs<T> a) {
+template <typename T, int FLUX, bool LEARNED = false, bool SPLIT = true>
+__global__ __launch_bounds__(SPLIT ? kVjpThreads : kVjpBlock) void k_step_vjp(
+    StepVjpArgs<T> a) {
   constexpr int B = kVjpBlock;
+  constexpr int NT = SPLIT ? kVjpThreads : kVjpBlock;
   __shared__ T sK2[B + 4], sK1[B + 4], sX[B + 4];
   __shared__ T sLam[B + 4];  // cells S - 9 ..
   __shared__ T sLk2[B + 1];  // cells S - 6 ..
   __shared__ T sLk1[B + 1];  // cells S - 3 ..
-  __shared__ VjpScratch<T> sc;
+  __shared__ VjpScratch<T, SPLIT> sc;
   const VjpArgs<T> &p = a.v;
   const int tid = threadIdx.x;
   const int64_t n = p.n;
@@ -587,7 +622,7 @@ __global__ __launch_bounds__(kVjpBlock) void k_step_vjp(StepVjpArgs<T> a) {
     xout[tid] = T(0);
     xout[p.m - g + tid] = T(0);
   }
-  for (int j = tid; j < B + 4; j += B) {
+  for (int j = tid; j < B + 4; j += NT) {
     sK2[j] = load_periodic(xk2, S - 11 + j, n, g);
     sK1[j] = load_periodic(xk1, S - 8 + j, n, g);
     sX[j] = load_periodic(xx, S - 5 + j, n, g);
@@ -600,15 +635,15 @@ __global__ __launch_bounds__(kVjpBlock) void k_step_vjp(StepVjpArgs<T> a) {
   __syncthreads();
 
   // reverse stage 3 on the output cells S - 6 + t: lk2
-  T gsum = vjp_block<T, FLUX, LEARNED>(p, sK2, sLam, sc);
+  T gsum = vjp_block<T, FLUX, LEARNED, SPLIT>(p, sK2, sLam, sc);
   if (tid < B - 6) sLk2[tid] = vjp_value<T, 3>(sLam[tid + 3], gsum, dt, T(0), T(0));
   __syncthreads();
   // reverse stage 2 on S - 3 + t: lk1
-  gsum = vjp_block<T, FLUX, LEARNED>(p, sK1, sLk2, sc);
+  gsum = vjp_block<T, FLUX, LEARNED, SPLIT>(p, sK1, sLk2, sc);
   if (tid < B - 6) sLk1[tid] = vjp_value<T, 2>(sLk2[tid + 3], gsum, dt, T(0), T(0));
   __syncthreads();
   // reverse stage 1 on S + t: the cotangent of the step input
-  gsum = vjp_block<T, FLUX, LEARNED>(p, sX, sLk1, sc);
+  gsum = vjp_block<T, FLUX, LEARNED, SPLIT>(p, sX, sLk1, sc);
   if (tid < kStepVjpOut) {
     const int64_t j = S + tid;
     if (j < p.m - g)
@@ -735,19 +770,19 @@ int launch_vjp_flux(int rstage, const VjpArgs<T> &a, hipStream_t s) {
   switch (rstage) {
     case 0:
       hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 0, LEARNED>), dim3(blocks),
-                         dim3(kVjpBlock), 0, s, a);
+                         dim3(kVjpThreads), 0, s, a);
       break;
     case 1:
       hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 1, LEARNED>), dim3(blocks),
-                         dim3(kVjpBlock), 0, s, a);
+                         dim3(kVjpThreads), 0, s, a);
       break;
     case 2:
       hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 2, LEARNED>), dim3(blocks),
-                         dim3(kVjpBlock), 0, s, a);
+                         dim3(kVjpThreads), 0, s, a);
       break;
     default:
       hipLaunchKernelGGL((k_stage_vjp<T, FLUX, 3, LEARNED>), dim3(blocks),
-                         dim3(kVjpBlock), 0, s, a);
+                         dim3(kVjpThreads), 0, s, a);
       break;
   }
   return check_launch("stage vjp");
@@ -813,6 +848,29 @@ int launch_vjp(const psa_solver_desc *d, int rstage, const void *state,
 }
 
 // The three reverse stages of one step as one launch (k_step_vjp).
+// Grids of at most this many workgroups run the split reverse step.
+constexpr long long kVjpSplitMaxBlocks = 512;
+
+template <typename T, bool SPLIT>
+int launch_step_vjp_plain(const psa_solver_desc *d, dim3 grid, const StepVjpArgs<T> &a,
+                          hipStream_t s) {
+  const dim3 block(SPLIT ? kVjpThreads : kVjpBlock);
+  if (d->flux == PSA_FLUX_RUSANOV) {
+    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_RUSANOV, false, SPLIT>), grid, block, 0, s,
+                       a);
+  } else if (d->flux == PSA_FLUX_BURGERS_UPWIND) {
+    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_BURGERS_UPWIND, false, SPLIT>), grid, block,
+                       0, s, a);
+  } else if (d->flux == PSA_FLUX_ENGQUIST_OSHER) {
+    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ENGQUIST_OSHER, false, SPLIT>), grid, block,
+                       0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ADVECTION_UPWIND, false, SPLIT>), grid,
+                       block, 0, s, a);
+  }
+  return check_launch("step vjp");
+}
+
 template <typename T>
 int launch_step_vjp(const psa_solver_desc *d, const void *x, const void *k1,
                     const void *k2, const void *lam, void *out, const void *dt,
@@ -839,30 +897,23 @@ int launch_step_vjp(const psa_solver_desc *d, const void *x, const void *k1,
                   static_cast<unsigned int>(batch));
   if (d->mlp != nullptr) {
     if (d->flux == PSA_FLUX_RUSANOV) {
-      hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_RUSANOV, true>), grid, dim3(kVjpBlock), 0,
+      hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_RUSANOV, true>), grid, dim3(kVjpThreads), 0,
                          s, a);
     } else if (d->flux == PSA_FLUX_ENGQUIST_OSHER) {
       hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ENGQUIST_OSHER, true>), grid,
-                         dim3(kVjpBlock), 0, s, a);
+                         dim3(kVjpThreads), 0, s, a);
     } else {
       hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_BURGERS_UPWIND, true>), grid,
-                         dim3(kVjpBlock), 0, s, a);
+                         dim3(kVjpThreads), 0, s, a);
     }
     return check_launch("learned step vjp");
   }
-  if (d->flux == PSA_FLUX_RUSANOV) {
-    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_RUSANOV>), grid, dim3(kVjpBlock), 0, s, a);
-  } else if (d->flux == PSA_FLUX_BURGERS_UPWIND) {
-    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_BURGERS_UPWIND>), grid, dim3(kVjpBlock), 0,
-                       s, a);
-  } else if (d->flux == PSA_FLUX_ENGQUIST_OSHER) {
-    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ENGQUIST_OSHER>), grid, dim3(kVjpBlock), 0,
-                       s, a);
-  } else {
-    hipLaunchKernelGGL((k_step_vjp<T, PSA_FLUX_ADVECTION_UPWIND>), grid, dim3(kVjpBlock),
-                       0, s, a);
-  }
-  return check_launch("step vjp");
+  // a grid that leaves the device short of waves splits each cell's two
+  // reconstructions between two threads (latency); a larger one keeps them
+  // in one thread (throughput)
+  if (static_cast<long long>(grid.x) * grid.y <= kVjpSplitMaxBlocks)
+    return launch_step_vjp_plain<T, true>(d, grid, a, s);
+  return launch_step_vjp_plain<T, false>(d, grid, a, s);
 }
 
 int check_vjp_desc(const psa_solver_desc *d) {

@@ -380,6 +380,31 @@ def test_batched_adjoint_equals_single(torch_mod, kind: str) -> None:
         assert torch.equal(gb[b], s.gradient(lam[b].contiguous()))
 
 
+@pytest.mark.parametrize("kind", KINDS)
+def test_reverse_step_forms_agree(torch_mod, kind: str) -> None:
+    """A batch whose reverse step is a grid of more than 512 workgroups
+    (8 instances of 16384 cells) runs each cell's two reconstructions in one
+    thread, a single instance splits them between two threads: the same
+    gradients bit for bit."""
+    torch = torch_mod
+    from pyshocks_amd.adjoint import BatchedCheckpointedSolve, CheckpointedSolve
+
+    n, nsteps, batch = 16384, 6, 8
+    src, _ = _source(torch, kind, n)
+    x = (np.arange(n + 6) - 2.5) / n
+    rng = np.random.default_rng(102)
+    u0 = torch.tensor(np.stack([0.3 + np.sin(2 * np.pi * (x + rng.uniform()))
+                                for _ in range(batch)]), dtype=torch.float64, device="cuda")
+    lam = torch.tensor(rng.standard_normal((batch, n + 6)), dtype=torch.float64,
+                       device="cuda")
+    dts = [0.3 / n * (1.0 + 0.05 * b) for b in range(batch)]
+    gb = BatchedCheckpointedSolve(src, u0, nsteps, dts, nsteps).gradient(lam)
+    assert bool(torch.isfinite(gb).all())
+    for b in (0, batch - 1):
+        s = CheckpointedSolve(src, u0[b].contiguous(), nsteps, dts[b], nsteps)
+        assert torch.equal(gb[b], s.gradient(lam[b].contiguous()))
+
+
 @pytest.mark.parametrize("kind", KINDS_LF)
 def test_one_window_keeps_the_forward_states(torch_mod, kind: str) -> None:
     """With every >= nsteps the forward pass keeps the stage states and the
