@@ -190,7 +190,9 @@ class Trainer:
     def fit(self, samples: torch.Tensor, *, epochs: int = 4, batch: int = 65536,
             seed: int = 0, lr_final: float | None = None) -> list[float]:
         """Epochs of shuffled mini-batches (cosine learning-rate decay to
-        ``lr_final``); records the mean loss of each epoch."""
+        ``lr_final``); records the mean loss of each epoch. Raises
+        ``RuntimeError`` after an epoch whose loss or parameters are not
+        finite (a diverged run, whose weights would turn solutions into NaN)."""
         n = samples.shape[0]
         gen = torch.Generator(device=samples.device)
         gen.manual_seed(seed)
@@ -207,6 +209,10 @@ class Trainer:
                 acc += self.step(samples[perm[b * batch:(b + 1) * batch]], lr).double()
                 k += 1
             self.history.append(float(acc) / nb)
+            if not (math.isfinite(self.history[-1])
+                    and bool(torch.isfinite(self.weights.params).all())):
+                raise RuntimeError(f"training diverged in epoch {len(self.history)} "
+                                   f"(loss {self.history[-1]}); lower the learning rate")
         return self.history
 
 

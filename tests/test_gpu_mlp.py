@@ -125,6 +125,27 @@ def test_training_reduces_loss(torch_mod):
     assert abs(loss - hist[-1]) < 0.5 * hist[-1] + 1e-3
 
 
+def test_training_stops_when_it_diverges(torch_mod):
+    """A run whose loss or parameters stop being finite ends with an error
+    after that epoch instead of returning weights that turn solutions into
+    NaN (here a NaN planted in the initial parameters)."""
+    torch = torch_mod
+    from pyshocks_amd.learned import MLPWeights, Trainer, init_params
+
+    rng = np.random.default_rng(8)
+    n = 4096
+    rec = np.zeros((n, M.RECORD), dtype=np.float32)
+    rec[:, :5] = M.features(rng.standard_normal((n, 5)))
+    rec[:, 5:8] = rng.standard_normal((n, 3)) * 0.1
+    rec[:, 8:11] = 1.0 / 3.0
+    rec[:, 11] = 1.0
+    p = init_params(0)
+    p[0] = np.nan
+    tr = Trainer(MLPWeights.from_numpy(p), lr=1e-2, lambda_js=0.0, delta=1e-6)
+    with pytest.raises(RuntimeError, match="diverged in epoch 1"):
+        tr.fit(torch.tensor(rec, device="cuda"), epochs=3, batch=1024)
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_learned_reconstruct_vs_oracle(torch_mod, dtype):
     torch = torch_mod
