@@ -87,8 +87,9 @@ class BatchedSolver:
         gmin = 3 if rec_kind == _lib.PSA_REC_WENOJS53 else 2
         if team_cells == "work":
             team_cells = None
-        elif team_cells is not None and not isinstance(team_cells, (int, np.integer)):
-            raise ValueError(f"team_cells must be a cell count, 'work' or None, "
+        elif team_cells is not None and (not isinstance(team_cells, (int, np.integer))
+                                         or isinstance(team_cells, bool) or team_cells < 0):
+            raise ValueError(f"team_cells must be a cell count (>= 0), 'work' or None, "
                              f"not {team_cells!r}")
         if nghosts < gmin:
             raise ValueError(f"the reconstruction needs nghosts >= {gmin}")

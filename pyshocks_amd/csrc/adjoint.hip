@@ -43,8 +43,8 @@
 namespace psa {
 namespace {
 
-constexpr int kVjpBlock = 256;                // computing cells per workgroup
-constexpr int kVjpThreads = 2 * kVjpBlock;    // a thread per cell and side
+constexpr int kVjpBlock = 256;              // computing cells per workgroup
+constexpr int kVjpThreads = 2 * kVjpBlock;  // split form: a thread per cell and side
 constexpr int kVjpOut = kVjpBlock - 6;  // output cells per workgroup
 
 // cell gm of a periodic array with n interior cells and g ghosts
@@ -388,7 +388,7 @@ __device__ __forceinline__ T vjp_block(const VjpArgs<T> &p, const T *s_u,
   const bool left = SPLIT && tid >= B;
   const int c = left ? tid - B : tid;
 
-  // ---- one reconstruction of computing cell c = Z - 3 + c and its gradient:
+  // ---- one reconstruction of computing cell Z - 3 + c and its gradient:
   // the right one from v0 .. v4, the left one from the mirrored stencil
   T G[5];
   const T v0 = s_u[c], v1 = s_u[c + 1], v2 = s_u[c + 2], v3 = s_u[c + 3], v4 = s_u[c + 4];
@@ -847,7 +847,6 @@ int launch_vjp(const psa_solver_desc *d, int rstage, const void *state,
   return launch_vjp_flux<T, PSA_FLUX_ADVECTION_UPWIND>(rstage, a, s);
 }
 
-// The three reverse stages of one step as one launch (k_step_vjp).
 // Grids of at most this many workgroups run the split reverse step.
 constexpr long long kVjpSplitMaxBlocks = 512;
 
@@ -871,6 +870,7 @@ int launch_step_vjp_plain(const psa_solver_desc *d, dim3 grid, const StepVjpArgs
   return check_launch("step vjp");
 }
 
+// The three reverse stages of one step as one launch (k_step_vjp).
 template <typename T>
 int launch_step_vjp(const psa_solver_desc *d, const void *x, const void *k1,
                     const void *k2, const void *lam, void *out, const void *dt,
