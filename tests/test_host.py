@@ -422,3 +422,13 @@ def test_work_teams_balance_the_persistent_grid() -> None:
             split = team > 1
             assert np.all(w[split] / (team[split] - 1) > mean)
         assert np.all(work_teams(ns, grid, cap, 0.5) >= work_teams(ns, grid, cap, 1.0))
+
+
+def test_batched_team_size_argument_checked_before_any_allocation() -> None:
+    """team_cells takes a cell count >= 0, "work" or None; anything else is
+    refused before the solver touches a device."""
+    from pyshocks_amd.batched import BatchedSolver
+
+    for bad in (-1, 2.5, True, "cells"):
+        with pytest.raises(ValueError, match="team_cells"):
+            BatchedSolver([64, 128], team_cells=bad)
