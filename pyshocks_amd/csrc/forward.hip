@@ -1119,11 +1119,11 @@ int launch_step_tiles(const StageArgs<T> &a, TileConfig cfg,
     return launch_step_cfg<T, FLUX, VEL, 256, 4, LEARNED, REC>(a, stream);
   if (cfg.block == 256 && cfg.cpt == 2)
     return launch_step_cfg<T, FLUX, VEL, 256, 2, LEARNED, REC>(a, stream);
-  if constexpr (!LEARNED && REC == 0) {
-    // one cell per thread (WENO-JS5 without learned weights): the shortest
+  if constexpr (REC == 0) {
+    // one cell per thread (WENO-JS5, learned weights or not): the shortest
     // launch for the smaller grids, whose steps are latency
     if (cfg.block == 256 && cfg.cpt == 1)
-      return launch_step_cfg<T, FLUX, VEL, 256, 1, false, 0>(a, stream);
+      return launch_step_cfg<T, FLUX, VEL, 256, 1, LEARNED, 0>(a, stream);
   }
   return set_error(PSA_ERR_INVALID_ARGUMENT,
                    "unsupported tile configuration for the whole-step kernel block=" +
@@ -1138,8 +1138,7 @@ inline bool step_applies(const psa_solver_desc *d, TileConfig cfg, int vel) {
   if (d->flux == PSA_FLUX_LAX_FRIEDRICHS) return false;
   if (d->flux == PSA_FLUX_ADVECTION_UPWIND && d->mlp != nullptr) return false;
   if (cfg.block == 256 && cfg.cpt == 1)
-    return d->mlp == nullptr && d->rec == PSA_REC_WENOJS53 &&
-           d->flux != PSA_FLUX_ESWENO;
+    return d->rec == PSA_REC_WENOJS53 && d->flux != PSA_FLUX_ESWENO;
   return cfg.block == 256 && (cfg.cpt == 4 || cfg.cpt == 2);
 }
 
@@ -1230,7 +1229,7 @@ int launch_ring(const psa_solver_desc *d, const StageArgs<T> &a, long long nstep
   return launch_ring_rec<T, 0>(d, a, nsteps, stream);
 }
 
-// Learned weights: Burgers fluxes and two tile shapes.
+// Learned weights: Burgers fluxes and three tile shapes.
 template <typename T, int FLUX, int STAGE>
 int launch_stage_learned(const StageArgs<T> &a, TileConfig cfg,
                          hipStream_t stream) {
@@ -1238,6 +1237,8 @@ int launch_stage_learned(const StageArgs<T> &a, TileConfig cfg,
     return launch_stage_cfg<T, FLUX, 0, STAGE, 256, 4, true>(a, stream);
   if (cfg.block == 256 && cfg.cpt == 2)
     return launch_stage_cfg<T, FLUX, 0, STAGE, 256, 2, true>(a, stream);
+  if (cfg.block == 256 && cfg.cpt == 1)
+    return launch_stage_cfg<T, FLUX, 0, STAGE, 256, 1, true>(a, stream);
   return set_error(PSA_ERR_INVALID_ARGUMENT,
                    "unsupported tile configuration for learned weights block=" +
                        std::to_string(cfg.block) +
@@ -1403,7 +1404,7 @@ TileConfig tiles_of(const psa_solver_desc *d) {
     c.step = false;
   }
   // below 2^17 cells, one cell per thread where the whole-step kernel takes
-  // it (WENO-JS5 without learned weights): the steps there are latency, and
+  // it (WENO-JS5, learned weights or not): the steps there are latency, and
   // the shorter per-thread chain makes them 8-15% shorter
   if (d->n < (int64_t(1) << 17) && d->block <= 0 && d->cells_per_thread <= 0) {
     TileConfig one = c;
@@ -1556,7 +1557,11 @@ int run_steps(const psa_solver_desc *d, const void *u_in, void *u_out,
 int step_fixed_dt(const psa_solver_desc *d, const void *u_in, void *u_out, void *k1,
                   void *k2, const void *dt, hipStream_t stream, long long batch,
                   int64_t ld) {
-  const TileConfig cfg = tiles_of(d);
+  TileConfig cfg = tiles_of(d);
+  // a batch fills the device with its instances: there the one-cell tiles of
+  // the small single grids give way to two cells per thread (the learned
+  // step 6% faster in batches of 64, the plain one the same)
+  if (batch > 1 && cfg.cpt == 1 && d->cells_per_thread <= 0) cfg.cpt = 2;
   const int vel = vel_mode(d);
   if (!step_applies(d, cfg, vel)) return PSA_ERR_NOT_IMPLEMENTED;
   if (batch < 1 || batch > 65535)

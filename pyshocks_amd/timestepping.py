@@ -217,8 +217,8 @@ class FusedPlan:
 def _uses_step_kernel(d: _lib.PsaSolverDesc) -> bool:
     """The library's rule for the whole-step kernel: schedule default or
     step, a constant or no velocity, no Rusanov ``nu``, not Lax-Friedrichs,
-    tiles (256, 4) or (256, 2), or (256, 1) for WENO-JS5 without learned
-    weights; by default 1 cell per thread below 2^17 cells where this kernel
+    tiles (256, 4) or (256, 2), or (256, 1) for WENO-JS5 (learned weights
+    or not); by default 1 cell per thread below 2^17 cells where this kernel
     takes it (then it also takes 2, the answer here), 2 below 2^20, 4
     above."""
     if d.schedule not in (_lib.PSA_SCHEDULE_DEFAULT, _lib.PSA_SCHEDULE_STEP):
@@ -230,8 +230,7 @@ def _uses_step_kernel(d: _lib.PsaSolverDesc) -> bool:
     block = d.block if d.block > 0 else 256
     cpt = d.cells_per_thread if d.cells_per_thread > 0 else (2 if d.n < (1 << 20) else 4)
     if block == 256 and cpt == 1:
-        return (not d.mlp and d.rec == _lib.PSA_REC_WENOJS53
-                and d.flux != _lib.PSA_FLUX_ESWENO)
+        return d.rec == _lib.PSA_REC_WENOJS53 and d.flux != _lib.PSA_FLUX_ESWENO
     return block == 256 and cpt in (2, 4)
 
 
