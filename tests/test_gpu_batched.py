@@ -177,6 +177,33 @@ def test_teams_match_single_workgroup_bit_for_bit(torch_mod, dtype_name: str,
         np.testing.assert_array_equal(out[0][1].t, o[1].t)
 
 
+def test_work_teams_respect_an_explicit_grid(torch_mod) -> None:
+    """Teams sized by work fit the grid: none when an explicit grid exceeds
+    the resident workgroups (a team's members must all be resident), at
+    most ``grid`` members for a smaller explicit grid; every form gives the
+    same bits."""
+    torch = torch_mod
+    from pyshocks_amd.batched import BatchedSolver, exact_averages_np, random_instances
+
+    _, ics = random_instances(5, seed=12)
+    ns = np.array([20000, 50, 3000, 9000, 700])
+    states = [exact_averages_np(ic, int(n)) for ic, n in zip(ics, ns)]
+    resident = BatchedSolver(ns, dtype=torch.float64, tiles=(256, 4)).resident_workgroups()
+    assert resident > 0
+    out = []
+    for grid, most in ((0, None), (2 * resident, 1), (3, 3)):
+        solver = BatchedSolver(ns, dtype=torch.float64, tiles=(256, 4), grid=grid)
+        if most is None:
+            assert solver.team.max() > 3
+        else:
+            assert solver.team.max() == most
+        solver.set_states(states)
+        res = solver.solve(0.02)
+        assert np.all(res.status == 0)
+        out.append(solver.u.clone())
+    assert torch.equal(out[0], out[1]) and torch.equal(out[0], out[2])
+
+
 def test_team_wait_limit_terminates(torch_mod) -> None:
     """A wait limit of zero makes a member that has to wait at a barrier
     abort its team: the launch still ends, team instances report
