@@ -379,7 +379,9 @@ struct VjpScratch {
 // grids too small to fill the device); otherwise (B threads) thread t
 // computes both. Thread t returns the gathered sum of output cell Z + t
 // (valid for t < B - 6; threads B .. 2 B - 1 return 0). Both forms compute
-// the same values with the same roundings. Ends with the scratch in use.
+// the same values with the same roundings. s_u is read only before the
+// second barrier (a caller may overwrite it once this returns). Ends with
+// the scratch in use.
 template <typename T, int FLUX, bool LEARNED = false, bool SPLIT = true>
 __device__ __forceinline__ T vjp_block(const VjpArgs<T> &p, const T *s_u,
                                        const T *s_mu, VjpScratch<T, SPLIT> &sc) {
@@ -596,15 +598,20 @@ constexpr int kStepVjpOut = kVjpBlock - 18;  // output cells per workgroup
 // [S - 6, S + B - 12), lk1 on [S - 3, S + B - 15), the step's cotangent on the
 // output cells [S, S + B - 18). The same arithmetic as k_stage_vjp.
 template <typename T, int FLUX, bool LEARNED = false, bool SPLIT = true>
-__global__ __launch_bounds__(SPLIT ? kVjpThreads : kVjpBlock) void k_step_vjp(
+__global__ __launch_bounds__(SPLIT ? kVjpThreads : kVjpBlock)
+__attribute__((amdgpu_waves_per_eu(SPLIT || LEARNED ? 1 : 5))) void k_step_vjp(
     StepVjpArgs<T> a) {
   constexpr int B = kVjpBlock;
   constexpr int NT = SPLIT ? kVjpThreads : kVjpBlock;
   __shared__ T sK2[B + 4], sK1[B + 4], sX[B + 4];
   __shared__ T sLam[B + 4];  // cells S - 9 ..
-  __shared__ T sLk2[B + 1];  // cells S - 6 ..
-  __shared__ T sLk1[B + 1];  // cells S - 3 ..
   __shared__ VjpScratch<T, SPLIT> sc;
+  // lk2 (cells S - 6 ..) takes k2's buffer and lk1 (cells S - 3 ..) k1's:
+  // each is written once the reverse stage that reads that state has
+  // returned, and vjp_block reads its state only before its second barrier
+  // (4 KB less LDS: one more workgroup per CU for the one-thread form)
+  T *const sLk2 = sK2;
+  T *const sLk1 = sK1;
   const VjpArgs<T> &p = a.v;
   const int tid = threadIdx.x;
   const int64_t n = p.n;
@@ -628,19 +635,19 @@ __global__ __launch_bounds__(SPLIT ? kVjpThreads : kVjpBlock) void k_step_vjp(
     sX[j] = load_periodic(xx, S - 5 + j, n, g);
     sLam[j] = load_periodic(xlam, S - 9 + j, n, g);
   }
-  // entries past the exact cotangents (read only for cells outside the
-  // exact ranges; finite values)
-  for (int j = B - 6 + tid; j < B + 1; j += B) sLk2[j] = T(0);
-  for (int j = B - 6 + tid; j < B + 1; j += B) sLk1[j] = T(0);
   __syncthreads();
 
-  // reverse stage 3 on the output cells S - 6 + t: lk2
+  // reverse stage 3 on the output cells S - 6 + t: lk2 (with zeros in the
+  // entries past the exact cotangents, read only for cells outside the exact
+  // ranges: finite values)
   T gsum = vjp_block<T, FLUX, LEARNED, SPLIT>(p, sK2, sLam, sc);
   if (tid < B - 6) sLk2[tid] = vjp_value<T, 3>(sLam[tid + 3], gsum, dt, T(0), T(0));
+  for (int j = B - 6 + tid; j < B + 1; j += B) sLk2[j] = T(0);
   __syncthreads();
   // reverse stage 2 on S - 3 + t: lk1
   gsum = vjp_block<T, FLUX, LEARNED, SPLIT>(p, sK1, sLk2, sc);
   if (tid < B - 6) sLk1[tid] = vjp_value<T, 2>(sLk2[tid + 3], gsum, dt, T(0), T(0));
+  for (int j = B - 6 + tid; j < B + 1; j += B) sLk1[j] = T(0);
   __syncthreads();
   // reverse stage 1 on S + t: the cotangent of the step input
   gsum = vjp_block<T, FLUX, LEARNED, SPLIT>(p, sX, sLk1, sc);
